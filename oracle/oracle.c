@@ -1,0 +1,1196 @@
+/*
+ * CPU oracle for the MACIS determinant-CI hot path -- TEST INFRASTRUCTURE.
+ *
+ * A plain-C restatement of the reference algorithms, function by function,
+ * each citing the reference file:line it follows (paths relative to the
+ * dfablan/MACIS source tree). It is the checker for the HIP path and the
+ * timed CPU baseline; product code never links or calls it.
+ *
+ * Floating-point policy: built with -ffp-contract=off so every sum is
+ * evaluated in the reference's source order without fused multiply-adds
+ * (matrix elements involve only additions and multiplications by +-1).
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   - matrix elements / intermediates: H2O cc-pVDZ known answers (E_HF,
+ *     excited diagonals, MP2, Brillouin), brute-force second quantisation
+ *     on small spaces, dense eigensolves;
+ *   - CSR build: h2o.ccpvdz.cisd.rowptr.bin fixture (exact), FCI row counts,
+ *     pattern symmetry;
+ *   - gespmbv: the 6x6 known answer;
+ *   - Davidson: H2O CISD E0 fixture and dense eigensolves;
+ *   - ASCI grow/refine: asci.cxx energies; search: brute-force rescoring.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ===================================================================== */
+/* determinant helpers                                                    */
+/* ===================================================================== */
+typedef struct {
+  uint64_t a; /* alpha string */
+  uint64_t b; /* beta string  */
+} det_t;
+
+static inline det_t load_det(int W, const uint64_t* p) {
+  det_t d;
+  if(W == 1) {
+    d.a = p[0] & 0xffffffffULL;
+    d.b = p[0] >> 32;
+  } else {
+    d.a = p[0];
+    d.b = p[1];
+  }
+  return d;
+}
+
+static inline void store_det(int W, det_t d, uint64_t* p) {
+  if(W == 1)
+    p[0] = d.a | (d.b << 32);
+  else {
+    p[0] = d.a;
+    p[1] = d.b;
+  }
+}
+
+static inline int spin_bits(int W) { return W == 1 ? 32 : 64; }
+static inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+static inline uint32_t ffs0(uint64_t x) { return (uint32_t)__builtin_ctzll(x); }
+/* full_mask<N>(i): bits [0, i)  (bitset_operations.hpp:225-230) */
+static inline uint64_t lowmask(uint32_t i) {
+  return i >= 64 ? ~0ULL : ((1ULL << i) - 1ULL);
+}
+
+/* bits_to_indices (bitset_operations.hpp:294-306) */
+static int bits_to_indices(uint64_t x, uint32_t* idx) {
+  int c = 0;
+  while(x) {
+    idx[c++] = ffs0(x);
+    x &= x - 1;
+  }
+  return c;
+}
+
+/* bitset_to_occ_vir (sd_operations.hpp:82-97): occupied = set bits,
+ * virtual = the first (norb - nocc) clear bits, ascending. */
+static void occ_vir(int64_t norb, uint64_t s, uint32_t* occ, int* nocc,
+                    uint32_t* vir, int* nvir) {
+  *nocc = bits_to_indices(s, occ);
+  int nv = (int)norb - *nocc;
+  uint64_t t = ~s;
+  for(int i = 0; i < nv; ++i) {
+    uint32_t a = ffs0(t);
+    vir[i] = a;
+    t &= t - 1;
+  }
+  *nvir = nv;
+}
+
+/* ===================================================================== */
+/* sign primitives (sd_operations.hpp:497-551)                            */
+/* ===================================================================== */
+double orc_single_excitation_sign(uint64_t state, uint32_t p, uint32_t q) {
+  /* :503-513 -- parity of occupied orbitals strictly between p and q */
+  uint64_t m;
+  if(p > q)
+    m = state & (lowmask(p) ^ lowmask(q + 1));
+  else
+    m = state & (lowmask(q) ^ lowmask(p + 1));
+  return (popc(m) & 1) ? -1. : 1.;
+}
+
+double orc_doubles_sign_indices(uint64_t bra, uint64_t ket, uint64_t ex,
+                                uint32_t* o1, uint32_t* v1, uint32_t* o2,
+                                uint32_t* v2) {
+  /* :528-543 -- two sequential singles on the updated ket */
+  *o1 = ffs0(ket & ex);
+  *v1 = ffs0(bra & ex);
+  double sign = orc_single_excitation_sign(ket, *v1, *o1);
+  ket ^= (1ULL << *o1) | (1ULL << *v1);
+  ex ^= (1ULL << *o1) | (1ULL << *v1);
+  *o2 = ffs0(ket & ex);
+  *v2 = ffs0(bra & ex);
+  sign *= orc_single_excitation_sign(ket, *v2, *o2);
+  return sign;
+}
+
+/* ===================================================================== */
+/* Hamiltonian generator                                                  */
+/* ===================================================================== */
+struct orc_ham {
+  int64_t n;
+  double* T;
+  double* V;
+  double* G;
+  double* G_red;
+  double* V_red;
+  double* G2_red;
+  double* V2_red;
+};
+
+orc_ham* orc_ham_create(int64_t n, const double* T, const double* V) {
+  /* HamiltonianGenerator ctor + generate_integral_intermediates
+   * (impl.hpp:14-70) */
+  if(n <= 0 || !T || !V) return NULL;
+  orc_ham* h = (orc_ham*)calloc(1, sizeof(orc_ham));
+  if(!h) return NULL;
+  const int64_t n2 = n * n, n3 = n2 * n, n4 = n3 * n;
+  h->n = n;
+  h->T = (double*)malloc(sizeof(double) * n2);
+  h->V = (double*)malloc(sizeof(double) * n4);
+  h->G = (double*)malloc(sizeof(double) * n4);
+  h->G_red = (double*)malloc(sizeof(double) * n3);
+  h->V_red = (double*)malloc(sizeof(double) * n3);
+  h->G2_red = (double*)malloc(sizeof(double) * n2);
+  h->V2_red = (double*)malloc(sizeof(double) * n2);
+  if(!h->T || !h->V || !h->G || !h->G_red || !h->V_red || !h->G2_red ||
+     !h->V2_red) {
+    orc_ham_destroy(h);
+    return NULL;
+  }
+  memcpy(h->T, T, sizeof(double) * n2);
+  memcpy(h->V, V, sizeof(double) * n4);
+#define IDX4(i, j, k, l) ((i) + n * (j) + n2 * (k) + n3 * (l))
+  /* G(i,j,k,l) = V(i,j,k,l) - V(i,l,k,j)   (:36-44) */
+  for(int64_t i = 0; i < n; ++i)
+    for(int64_t j = 0; j < n; ++j)
+      for(int64_t k = 0; k < n; ++k)
+        for(int64_t l = 0; l < n; ++l)
+          h->G[IDX4(i, j, k, l)] = V[IDX4(i, j, k, l)] - V[IDX4(i, l, k, j)];
+  /* G_red(k,i,j) = G(k,k,i,j); V_red(k,i,j) = V(k,k,i,j)   (:46-57) */
+  for(int64_t j = 0; j < n; ++j)
+    for(int64_t i = 0; i < n; ++i)
+      for(int64_t k = 0; k < n; ++k) {
+        h->G_red[k + n * i + n2 * j] = h->G[IDX4(k, k, i, j)];
+        h->V_red[k + n * i + n2 * j] = V[IDX4(k, k, i, j)];
+      }
+  /* G2_red(i,j) = 0.5*G(i,i,j,j); V2_red(i,j) = V(i,i,j,j)   (:59-69) */
+  for(int64_t j = 0; j < n; ++j)
+    for(int64_t i = 0; i < n; ++i) {
+      h->G2_red[i + n * j] = 0.5 * h->G[IDX4(i, i, j, j)];
+      h->V2_red[i + n * j] = V[IDX4(i, i, j, j)];
+    }
+#undef IDX4
+  return h;
+}
+
+void orc_ham_destroy(orc_ham* h) {
+  if(!h) return;
+  free(h->T);
+  free(h->V);
+  free(h->G);
+  free(h->G_red);
+  free(h->V_red);
+  free(h->G2_red);
+  free(h->V2_red);
+  free(h);
+}
+
+int64_t orc_ham_norb(const orc_ham* h) { return h->n; }
+
+const double* orc_ham_array(const orc_ham* h, int which) {
+  switch(which) {
+    case 0: return h->T;
+    case 1: return h->V;
+    case 2: return h->G;
+    case 3: return h->G_red;
+    case 4: return h->V_red;
+    case 5: return h->G2_red;
+    case 6: return h->V2_red;
+    default: return NULL;
+  }
+}
+
+/* matrix_element_4 (matrix_elements.hpp:65-71) */
+static double me_4(const orc_ham* h, uint64_t bra, uint64_t ket, uint64_t ex) {
+  uint32_t o1, v1, o2, v2;
+  const int64_t n = h->n, n2 = n * n, n3 = n2 * n;
+  double sign = orc_doubles_sign_indices(bra, ket, ex, &o1, &v1, &o2, &v2);
+  return sign * h->G[v1 + n * o1 + n2 * v2 + n3 * o2];
+}
+
+/* matrix_element_22 (matrix_elements.hpp:73-84) */
+static double me_22(const orc_ham* h, det_t bra, det_t ket, uint64_t exa,
+                    uint64_t exb) {
+  const int64_t n = h->n, n2 = n * n, n3 = n2 * n;
+  uint32_t o1 = ffs0(ket.a & exa), v1 = ffs0(bra.a & exa);
+  double sa = orc_single_excitation_sign(ket.a, v1, o1);
+  uint32_t o2 = ffs0(ket.b & exb), v2 = ffs0(bra.b & exb);
+  double sb = orc_single_excitation_sign(ket.b, v2, o2);
+  double sign = sa * sb;
+  return sign * h->V[v1 + n * o1 + n2 * v2 + n3 * o2];
+}
+
+/* matrix_element_2 (matrix_elements.hpp:86-106) */
+static double me_2(const orc_ham* h, uint64_t bra, uint64_t ket, uint64_t ex,
+                   const uint32_t* occ_same, int nsame,
+                   const uint32_t* occ_othr, int nothr) {
+  const int64_t n = h->n, n2 = n * n;
+  uint32_t o1 = ffs0(ket & ex), v1 = ffs0(bra & ex);
+  double sign = orc_single_excitation_sign(ket, v1, o1);
+  double h_el = h->T[v1 + n * o1];
+  const double* G_ov = h->G_red + n * v1 + n2 * o1;
+  for(int p = 0; p < nsame; ++p) h_el += G_ov[occ_same[p]];
+  const double* V_ov = h->V_red + n * v1 + n2 * o1;
+  for(int p = 0; p < nothr; ++p) h_el += V_ov[occ_othr[p]];
+  return sign * h_el;
+}
+
+/* matrix_element_diag (matrix_elements.hpp:108-135) */
+static double me_diag(const orc_ham* h, const uint32_t* oa, int na,
+                      const uint32_t* ob, int nb) {
+  const int64_t n = h->n;
+  double h_el = 0;
+  for(int i = 0; i < na; ++i) h_el += h->T[oa[i] * (n + 1)];
+  for(int i = 0; i < nb; ++i) h_el += h->T[ob[i] * (n + 1)];
+  for(int q = 0; q < na; ++q)
+    for(int p = 0; p < na; ++p) h_el += h->G2_red[oa[p] + n * oa[q]];
+  for(int q = 0; q < nb; ++q)
+    for(int p = 0; p < nb; ++p) h_el += h->G2_red[ob[p] + n * ob[q]];
+  for(int q = 0; q < nb; ++q)
+    for(int p = 0; p < na; ++p) h_el += h->V2_red[oa[p] + n * ob[q]];
+  return h_el;
+}
+
+/* matrix_element dispatch (matrix_elements.hpp:32-63) */
+static double me_dispatch(const orc_ham* h, det_t bra, det_t ket,
+                          const uint32_t* oa, int na, const uint32_t* ob,
+                          int nb) {
+  const uint64_t exa = bra.a ^ ket.a, exb = bra.b ^ ket.b;
+  const int ca = popc(exa), cb = popc(exb);
+  if(ca + cb > 4) return 0.;
+  if(ca == 4) return me_4(h, bra.a, ket.a, exa);
+  if(cb == 4) return me_4(h, bra.b, ket.b, exb);
+  if(ca == 2 && cb == 2) return me_22(h, bra, ket, exa, exb);
+  if(ca == 2) return me_2(h, bra.a, ket.a, exa, oa, na, ob, nb);
+  if(cb == 2) return me_2(h, bra.b, ket.b, exb, ob, nb, oa, na);
+  return me_diag(h, oa, na, ob, nb);
+}
+
+double orc_matrix_element(const orc_ham* h, int W, const uint64_t* bra_p,
+                          const uint64_t* ket_p) {
+  /* matrix_elements.hpp:14-30 */
+  uint32_t oa[64], ob[64];
+  det_t bra = load_det(W, bra_p), ket = load_det(W, ket_p);
+  int na = bits_to_indices(bra.a, oa), nb = bits_to_indices(bra.b, ob);
+  return me_dispatch(h, bra, ket, oa, na, ob, nb);
+}
+
+/* ---- fast diagonals (fast_diagonals.hpp:14-127) ---------------------- */
+void orc_single_orbital_ens(const orc_ham* h, int64_t norb,
+                            const uint32_t* ss_occ, int64_t nss,
+                            const uint32_t* os_occ, int64_t nos, double* ens) {
+  const int64_t n = h->n;
+  for(int64_t i = 0; i < norb; ++i) {
+    double e = h->T[i * (n + 1)];
+    for(int64_t q = 0; q < nss; ++q)
+      e += h->G2_red[i + n * ss_occ[q]] + h->G2_red[ss_occ[q] + n * i];
+    e -= h->G2_red[i * (n + 1)];
+    for(int64_t q = 0; q < nos; ++q) e += h->V2_red[i + n * os_occ[q]];
+    ens[i] = e;
+  }
+}
+
+#define G2(p, q) (h->G2_red[(p) + h->n * (q)])
+#define V2(p, q) (h->V2_red[(p) + h->n * (q)])
+double orc_fast_diag_single(const orc_ham* h, double hol_en, double par_en,
+                            uint32_t oh, uint32_t op, double Hii) {
+  /* :53-60 */
+  return Hii + par_en - hol_en - G2(op, oh) - G2(oh, op);
+}
+
+double orc_fast_diag_ss_double(const orc_ham* h, double h1, double h2,
+                               double p1, double p2, uint32_t oh1,
+                               uint32_t oh2, uint32_t op1, uint32_t op2,
+                               double Hii) {
+  /* :72-85 */
+  return Hii + p1 + p2 - h1 - h2 + G2(oh1, oh2) + G2(oh2, oh1) +
+         G2(op1, op2) + G2(op2, op1) - G2(op1, oh1) - G2(oh1, op1) -
+         G2(op2, oh1) - G2(oh1, op2) - G2(op1, oh2) - G2(oh2, op1) -
+         G2(op2, oh2) - G2(oh2, op2);
+}
+
+double orc_fast_diag_os_double(const orc_ham* h, double hu, double hd,
+                               double pu, double pd, uint32_t ohu,
+                               uint32_t ohd, uint32_t opu, uint32_t opd,
+                               double Hii) {
+  /* :101-112 */
+  return Hii + pu + pd - hu - hd + V2(ohu, ohd) + V2(opu, opd) -
+         G2(opu, ohu) - G2(ohu, opu) - G2(opd, ohd) - G2(ohd, opd) -
+         V2(opu, ohd) - V2(ohu, opd);
+}
+#undef G2
+#undef V2
+
+/* ===================================================================== */
+/* determinant spaces (sd_operations.hpp:129-476)                         */
+/* ===================================================================== */
+/* std::prev_permutation on a 0/1 array */
+static int prev_perm(unsigned char* a, int64_t n) {
+  if(n < 2) return 0;
+  int64_t i = n - 1;
+  while(i > 0 && a[i - 1] <= a[i]) --i;
+  if(i == 0) {
+    for(int64_t l = 0, r = n - 1; l < r; ++l, --r) {
+      unsigned char t = a[l];
+      a[l] = a[r];
+      a[r] = t;
+    }
+    return 0;
+  }
+  int64_t j = n - 1;
+  while(a[j] >= a[i - 1]) --j;
+  unsigned char t = a[i - 1];
+  a[i - 1] = a[j];
+  a[j] = t;
+  for(int64_t l = i, r = n - 1; l < r; ++l, --r) {
+    t = a[l];
+    a[l] = a[r];
+    a[r] = t;
+  }
+  return 1;
+}
+
+/* generate_combs (sd_operations.hpp:440-458) */
+static int64_t gen_combs(int64_t nbits, int64_t nset, uint64_t* out) {
+  unsigned char v[128];
+  for(int64_t i = 0; i < nbits; ++i) v[i] = (unsigned char)(i < nset);
+  int64_t c = 0;
+  do {
+    uint64_t t = 0;
+    for(int64_t i = 0; i < nbits; ++i)
+      if(v[i]) t |= (1ULL << i);
+    if(out) out[c] = t;
+    ++c;
+  } while(prev_perm(v, nbits));
+  return c;
+}
+
+int64_t orc_generate_hilbert_space(int W, int64_t norb, int64_t na,
+                                   int64_t nb, uint64_t* out) {
+  /* :460-476 -- alpha outer, beta inner */
+  if(norb > spin_bits(W) || na > norb || nb > norb) return -1;
+  int64_t nA = gen_combs(norb, na, NULL), nB = gen_combs(norb, nb, NULL);
+  if(!out) return nA * nB;
+  uint64_t* A = (uint64_t*)malloc(sizeof(uint64_t) * nA);
+  uint64_t* B = (uint64_t*)malloc(sizeof(uint64_t) * nB);
+  gen_combs(norb, na, A);
+  gen_combs(norb, nb, B);
+  int64_t c = 0;
+  for(int64_t i = 0; i < nA; ++i)
+    for(int64_t j = 0; j < nB; ++j) {
+      det_t d = {A[i], B[j]};
+      store_det(W, d, out + W * c);
+      ++c;
+    }
+  free(A);
+  free(B);
+  return c;
+}
+
+/* generate_singles_doubles on one spin string (append_singles :129-143,
+ * append_doubles :145-165). Returns counts. */
+static void spin_singles_doubles(int64_t norb, uint64_t s, uint64_t* sing,
+                                 int64_t* ns, uint64_t* dbl, int64_t* nd) {
+  uint32_t occ[64], vir[64];
+  int no, nv;
+  occ_vir(norb, s, occ, &no, vir, &nv);
+  int64_t c = 0;
+  for(int a = 0; a < nv; ++a)
+    for(int i = 0; i < no; ++i) {
+      if(sing) sing[c] = s ^ ((1ULL << occ[i]) | (1ULL << vir[a]));
+      ++c;
+    }
+  *ns = c;
+  c = 0;
+  for(int a = 0; a < nv; ++a)
+    for(int i = 0; i < no; ++i)
+      for(int b = a + 1; b < nv; ++b)
+        for(int j = i + 1; j < no; ++j) {
+          if(dbl)
+            dbl[c] = s ^ ((1ULL << occ[i]) | (1ULL << vir[a]) |
+                          (1ULL << occ[j]) | (1ULL << vir[b]));
+          ++c;
+        }
+  *nd = c;
+}
+
+int64_t orc_generate_cisd_hilbert_space(int W, int64_t norb,
+                                        const uint64_t* state_p,
+                                        uint64_t* out) {
+  /* generate_cisd_hilbert_space (:421-430) over
+   * generate_singles_doubles_spin (:345-419) */
+  det_t st = load_det(W, state_p);
+  int64_t nsa, nda, nsb, ndb;
+  spin_singles_doubles(norb, st.a, NULL, &nsa, NULL, &nda);
+  spin_singles_doubles(norb, st.b, NULL, &nsb, NULL, &ndb);
+  int64_t total = 1 + nsa + nsb + nda + ndb + nsa * nsb;
+  if(!out) return total;
+  uint64_t* sa = (uint64_t*)malloc(sizeof(uint64_t) * (nsa + 1));
+  uint64_t* da = (uint64_t*)malloc(sizeof(uint64_t) * (nda + 1));
+  uint64_t* sb = (uint64_t*)malloc(sizeof(uint64_t) * (nsb + 1));
+  uint64_t* db = (uint64_t*)malloc(sizeof(uint64_t) * (ndb + 1));
+  spin_singles_doubles(norb, st.a, sa, &nsa, da, &nda);
+  spin_singles_doubles(norb, st.b, sb, &nsb, db, &ndb);
+  int64_t c = 0;
+  store_det(W, st, out + W * c++);
+  for(int64_t i = 0; i < nsa; ++i) {
+    det_t d = {sa[i], st.b};
+    store_det(W, d, out + W * c++);
+  }
+  for(int64_t i = 0; i < nsb; ++i) {
+    det_t d = {st.a, sb[i]};
+    store_det(W, d, out + W * c++);
+  }
+  for(int64_t i = 0; i < nda; ++i) {
+    det_t d = {da[i], st.b};
+    store_det(W, d, out + W * c++);
+  }
+  for(int64_t i = 0; i < ndb; ++i) {
+    det_t d = {st.a, db[i]};
+    store_det(W, d, out + W * c++);
+  }
+  for(int64_t i = 0; i < nsa; ++i)
+    for(int64_t j = 0; j < nsb; ++j) {
+      det_t d = {sa[i], sb[j]};
+      store_det(W, d, out + W * c++);
+    }
+  free(sa);
+  free(da);
+  free(sb);
+  free(db);
+  return c;
+}
+
+/* ===================================================================== */
+/* CSR Hamiltonian (double_loop.hpp:30-98)                                */
+/* ===================================================================== */
+typedef struct {
+  int32_t* col;
+  double* val;
+  int64_t n, cap;
+} entry_buf;
+
+static int buf_push(entry_buf* b, int32_t c, double v) {
+  if(b->n == b->cap) {
+    int64_t nc = b->cap ? 2 * b->cap : 1024;
+    int32_t* c2 = (int32_t*)realloc(b->col, sizeof(int32_t) * nc);
+    if(!c2) return ORC_ERR_NOMEM;
+    b->col = c2;
+    double* v2 = (double*)realloc(b->val, sizeof(double) * nc);
+    if(!v2) return ORC_ERR_NOMEM;
+    b->val = v2;
+    b->cap = nc;
+  }
+  b->col[b->n] = c;
+  b->val[b->n] = v;
+  b->n++;
+  return ORC_OK;
+}
+
+/* One bra row of the double loop (:45-89). Calls sink for each kept
+ * entry in ascending ket order; returns the row count. */
+static int64_t csr_row(const orc_ham* h, int W, det_t bra, const uint64_t* ket,
+                       int64_t nket, double thresh, int32_t* col, double* val,
+                       entry_buf* buf, int* err) {
+  int64_t nrow = 0;
+  if(!(popc(bra.a) + popc(bra.b))) return 0; /* zero bra: empty row (:50) */
+  uint32_t oa[64], ob[64];
+  int na = bits_to_indices(bra.a, oa), nb = bits_to_indices(bra.b, ob);
+  for(int64_t j = 0; j < nket; ++j) {
+    det_t k = load_det(W, ket + W * j);
+    if(!(popc(k.a) + popc(k.b))) continue; /* zero ket (:62) */
+    if(popc(bra.a ^ k.a) + popc(bra.b ^ k.b) > 4) continue; /* :66-67 */
+    double h_el = me_dispatch(h, bra, k, oa, na, ob, nb);
+    if(fabs(h_el) > thresh) { /* :76-80 */
+      if(col) {
+        col[nrow] = (int32_t)j;
+        val[nrow] = h_el;
+      }
+      if(buf && buf_push(buf, (int32_t)j, h_el) != ORC_OK) *err = 1;
+      ++nrow;
+    }
+  }
+  return nrow;
+}
+
+int orc_csr_build(const orc_ham* h, int W, const uint64_t* bra, int64_t nbra,
+                  const uint64_t* ket, int64_t nket, double thresh,
+                  int64_t** rowptr_out, int32_t** colind_out,
+                  double** nzval_out, int64_t* nnz_out) {
+  if(!h || (W != 1 && W != 2) || nbra < 0 || nket < 0) return ORC_ERR_ARG;
+  int64_t* rowptr = (int64_t*)malloc(sizeof(int64_t) * (nbra + 1));
+  if(!rowptr) return ORC_ERR_NOMEM;
+  entry_buf buf = {NULL, NULL, 0, 0};
+  int err = 0;
+  rowptr[0] = 0;
+  for(int64_t i = 0; i < nbra; ++i) {
+    det_t b = load_det(W, bra + W * i);
+    int64_t nrow =
+        nket ? csr_row(h, W, b, ket, nket, thresh, NULL, NULL, &buf, &err) : 0;
+    rowptr[i + 1] = rowptr[i] + nrow;
+    if(err) {
+      free(rowptr);
+      free(buf.col);
+      free(buf.val);
+      return ORC_ERR_NOMEM;
+    }
+  }
+  *rowptr_out = rowptr;
+  *colind_out = buf.col ? buf.col : (int32_t*)malloc(sizeof(int32_t));
+  *nzval_out = buf.val ? buf.val : (double*)malloc(sizeof(double));
+  *nnz_out = buf.n;
+  return ORC_OK;
+}
+
+int orc_csr_rows(const orc_ham* h, int W, const uint64_t* bra,
+                 int64_t row_begin, int64_t row_end, const uint64_t* ket,
+                 int64_t nket, double thresh, int64_t* row_counts,
+                 int32_t* colind, double* nzval) {
+  if(!h || (W != 1 && W != 2) || row_end < row_begin) return ORC_ERR_ARG;
+  int64_t off = 0;
+  int err = 0;
+  for(int64_t i = row_begin; i < row_end; ++i) {
+    det_t b = load_det(W, bra + W * i);
+    int64_t c = csr_row(h, W, b, ket, nket, thresh, colind ? colind + off : NULL,
+                        colind ? nzval + off : NULL, NULL, &err);
+    if(row_counts) row_counts[i - row_begin] = c;
+    off += c;
+  }
+  return ORC_OK;
+}
+
+void orc_free(void* p) { free(p); }
+
+/* ===================================================================== */
+/* sparse BLAS                                                            */
+/* ===================================================================== */
+void orc_gespmbv(int64_t K, double alpha, int64_t M, const int64_t* Arp,
+                 const int32_t* Aci, const double* Anz, int64_t indexing,
+                 const double* V, int64_t LDV, double beta, double* AV,
+                 int64_t LDAV) {
+  /* spmbv.hpp:37-73 (serial dot per row in CSR order) */
+  for(int64_t k = 0; k < K; ++k)
+    for(int64_t i = 0; i < M; ++i) {
+      const int64_t j_st = Arp[i] - indexing, j_en = Arp[i + 1] - indexing;
+      const double* V_k = V + k * LDV - indexing;
+      double av = 0.;
+      for(int64_t j = j_st; j < j_en; ++j) av += Anz[j] * V_k[Aci[j]];
+      AV[i + k * LDAV] = alpha * av + beta * AV[i + k * LDAV];
+    }
+}
+
+void orc_extract_diagonal(int64_t M, const int64_t* Arp, const int32_t* Aci,
+                          const double* Anz, int64_t indexing, double* D) {
+  /* submatrix.hpp:264-295 -- first column equal to i, else 0 */
+  for(int64_t i = 0; i < M; ++i) {
+    const int64_t j_st = Arp[i] - indexing, j_en = Arp[i + 1] - indexing;
+    D[i] = 0.;
+    for(int64_t j = j_st; j < j_en; ++j)
+      if(Aci[j] == i + indexing) {
+        D[i] = Anz[j];
+        break;
+      }
+  }
+}
+
+/* ===================================================================== */
+/* symmetric eigensolver: cyclic Jacobi (Golub & Van Loan, Alg. 8.5.3)     */
+/* ===================================================================== */
+void orc_syev_lower(int64_t n, double* A, int64_t lda, double* W) {
+  double* a = (double*)malloc(sizeof(double) * n * n);
+  double* v = (double*)calloc(n * n, sizeof(double));
+  for(int64_t j = 0; j < n; ++j)
+    for(int64_t i = j; i < n; ++i) {
+      a[i + n * j] = A[i + lda * j];
+      a[j + n * i] = A[i + lda * j];
+    }
+  for(int64_t i = 0; i < n; ++i) v[i + n * i] = 1.;
+  for(int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0., tot = 0.;
+    for(int64_t j = 0; j < n; ++j)
+      for(int64_t i = 0; i < n; ++i) {
+        tot += a[i + n * j] * a[i + n * j];
+        if(i != j) off += a[i + n * j] * a[i + n * j];
+      }
+    if(off <= 1e-32 * tot || off == 0.) break;
+    for(int64_t p = 0; p < n - 1; ++p)
+      for(int64_t q = p + 1; q < n; ++q) {
+        const double apq = a[p + n * q];
+        if(apq == 0.) continue;
+        const double tau = (a[q + n * q] - a[p + n * p]) / (2. * apq);
+        const double t = (tau >= 0. ? 1. : -1.) / (fabs(tau) + sqrt(1. + tau * tau));
+        const double c = 1. / sqrt(1. + t * t), s = t * c;
+        for(int64_t k = 0; k < n; ++k) { /* A <- A J (columns p,q) */
+          const double akp = a[k + n * p], akq = a[k + n * q];
+          a[k + n * p] = c * akp - s * akq;
+          a[k + n * q] = s * akp + c * akq;
+        }
+        for(int64_t k = 0; k < n; ++k) { /* A <- J^T A (rows p,q) */
+          const double apk = a[p + n * k], aqk = a[q + n * k];
+          a[p + n * k] = c * apk - s * aqk;
+          a[q + n * k] = s * apk + c * aqk;
+        }
+        for(int64_t k = 0; k < n; ++k) { /* V <- V J */
+          const double vkp = v[k + n * p], vkq = v[k + n * q];
+          v[k + n * p] = c * vkp - s * vkq;
+          v[k + n * q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  /* sort ascending (selection sort; n <= a few dozen) */
+  int64_t* ord = (int64_t*)malloc(sizeof(int64_t) * n);
+  for(int64_t i = 0; i < n; ++i) ord[i] = i;
+  for(int64_t i = 0; i < n; ++i)
+    for(int64_t j = i + 1; j < n; ++j)
+      if(a[ord[j] * (n + 1)] < a[ord[i] * (n + 1)]) {
+        int64_t t = ord[i];
+        ord[i] = ord[j];
+        ord[j] = t;
+      }
+  for(int64_t j = 0; j < n; ++j) {
+    W[j] = a[ord[j] * (n + 1)];
+    for(int64_t i = 0; i < n; ++i) A[i + lda * j] = v[i + n * ord[j]];
+  }
+  free(ord);
+  free(a);
+  free(v);
+}
+
+/* ===================================================================== */
+/* Davidson (davidson.hpp:61-225)                                         */
+/* ===================================================================== */
+void orc_diagonal_guess(int64_t N, const double* D, double* X) {
+  /* :61-68 -- unit vector at the first minimum of D */
+  int64_t mi = 0;
+  for(int64_t i = 1; i < N; ++i)
+    if(D[i] < D[mi]) mi = i;
+  if(N > 0) X[mi] = 1.;
+}
+
+/* gram_schmidt (:111-126): CGS2 then normalise */
+static void gram_schmidt(int64_t N, int64_t K, const double* V, int64_t LDV,
+                         double* w) {
+  double inner[64];
+  for(int pass = 0; pass < 2; ++pass) {
+    for(int64_t j = 0; j < K; ++j) {
+      double s = 0.;
+      for(int64_t n = 0; n < N; ++n) s += V[n + j * LDV] * w[n];
+      inner[j] = s;
+    }
+    for(int64_t n = 0; n < N; ++n) {
+      double s = 0.;
+      for(int64_t j = 0; j < K; ++j) s += V[n + j * LDV] * inner[j];
+      w[n] = -1. * s + w[n];
+    }
+  }
+  double nrm = 0.;
+  for(int64_t n = 0; n < N; ++n) nrm += w[n] * w[n];
+  nrm = sqrt(nrm);
+  const double sc = 1. / nrm;
+  for(int64_t n = 0; n < N; ++n) w[n] *= sc;
+}
+
+int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
+                           const int64_t* rp, const int32_t* ci,
+                           const double* nz, const double* D, double tol,
+                           double* X, double* E, int64_t* iters,
+                           int64_t* restarts_out) {
+  /* davidson (davidson.hpp:128-225). With max_restarts == 0 this is the
+   * reference algorithm exactly: no restart, error when the subspace of
+   * max_m vectors is exhausted. With max_restarts > 0 (a labelled variant,
+   * not in the reference) an exhausted subspace restarts from
+   * [X, t], X the current Ritz vector and t the preconditioned, already
+   * orthogonalised residual that would have been the next basis vector. */
+  if(!X) return ORC_ERR_NO_GUESS;
+  if(max_m > N) max_m = N;
+  if(max_m > 62) return ORC_ERR_ARG;
+  const int64_t ld = max_m + 1;
+  double* V = (double*)calloc((size_t)N * ld, sizeof(double));
+  double* AV = (double*)calloc((size_t)N * ld, sizeof(double));
+  double C[64 * 64], LAM[64];
+  if(!V || !AV) {
+    free(V);
+    free(AV);
+    return ORC_ERR_NOMEM;
+  }
+  memcpy(V, X, sizeof(double) * N);
+  orc_gespmbv(1, 1., N, rp, ci, nz, 0, V, N, 0., AV, N); /* :153 */
+  memcpy(V + N, AV, sizeof(double) * N);                   /* :156 */
+  gram_schmidt(N, 1, V, N, V + N);                          /* :157 */
+  int converged = 0;
+  int64_t iter = 1, restarts = 0;
+  for(;;) {
+    for(int64_t i = 1; i < max_m; ++i, ++iter) {
+      const int64_t k = i + 1;
+      orc_gespmbv(1, 1., N, rp, ci, nz, 0, V + i * N, N, 0., AV + i * N, N);
+      /* rayleigh_ritz (rayleigh_ritz.hpp:70-77): C = V^T AV, syev(Lower) */
+      for(int64_t b = 0; b < k; ++b)
+        for(int64_t a = 0; a < k; ++a) {
+          double s = 0.;
+          for(int64_t n = 0; n < N; ++n) s += V[n + a * N] * AV[n + b * N];
+          C[a + k * b] = s;
+        }
+      orc_syev_lower(k, C, k, LAM);
+      double* R = V + (i + 1) * N;
+      /* X = V*C(:,0); R = AV*C(:,0) - LAM0*X; ||R||   (:181-193) */
+      double rn = 0.;
+      for(int64_t n = 0; n < N; ++n) {
+        double x = 0., r = 0.;
+        for(int64_t j = 0; j < k; ++j) x += V[n + j * N] * C[j];
+        for(int64_t j = 0; j < k; ++j) r += AV[n + j * N] * C[j];
+        X[n] = x;
+        R[n] = r + (-LAM[0]) * x;
+        rn += R[n] * R[n];
+      }
+      rn = sqrt(rn);
+      if(rn < tol) {
+        converged = 1;
+        break;
+      }
+      for(int64_t j = 0; j < N; ++j) R[j] = -R[j] / (D[j] - LAM[0]); /* :212 */
+      gram_schmidt(N, k, V, N, R);                                    /* :217 */
+    }
+    if(converged || restarts >= max_restarts) break;
+    /* restart: V0 = X / ||X||, V1 = V(:,max_m), AV0 = A*V0 */
+    ++restarts;
+    double xn = 0.;
+    for(int64_t n = 0; n < N; ++n) xn += X[n] * X[n];
+    xn = 1. / sqrt(xn);
+    for(int64_t n = 0; n < N; ++n) V[n] = X[n] * xn;
+    memmove(V + N, V + max_m * N, sizeof(double) * N);
+    orc_gespmbv(1, 1., N, rp, ci, nz, 0, V, N, 0., AV, N);
+  }
+  free(V);
+  free(AV);
+  *E = LAM[0];
+  if(iters) *iters = iter;
+  if(restarts_out) *restarts_out = restarts;
+  return converged ? ORC_OK : ORC_ERR_NOT_CONVERGED;
+}
+
+int orc_davidson(int64_t N, int64_t max_m, const int64_t* rp,
+                 const int32_t* ci, const double* nz, const double* D,
+                 double tol, double* X, double* E, int64_t* iters) {
+  return orc_davidson_restarted(N, max_m, 0, rp, ci, nz, D, tol, X, E, iters,
+                                NULL);
+}
+
+int orc_selected_ci_diag_restarted(const orc_ham* h, int W,
+                                   const uint64_t* dets, int64_t ndets,
+                                   double h_el_tol, int64_t max_m,
+                                   int64_t max_restarts, double res_tol,
+                                   double* C, double* E, int64_t* nnz_out) {
+  /* selected_ci_diag (:124-207) -> serial_selected_ci_diag (:75-122) */
+  int64_t *rp = NULL, nnz = 0;
+  int32_t* ci = NULL;
+  double* nz = NULL;
+  int rc = orc_csr_build(h, W, dets, ndets, dets, ndets, h_el_tol, &rp, &ci,
+                         &nz, &nnz);
+  if(rc) return rc;
+  if(nnz_out) *nnz_out = nnz;
+  double* D = (double*)malloc(sizeof(double) * (ndets + 1));
+  orc_extract_diagonal(ndets, rp, ci, nz, rp[0], D);
+  double max_c = 0.;
+  for(int64_t i = 0; i < ndets; ++i)
+    if(fabs(C[i]) > max_c) max_c = fabs(C[i]);
+  if(!(max_c > 1. / (double)ndets)) orc_diagonal_guess(ndets, D, C); /* :99 */
+  int64_t it = 0;
+  rc = orc_davidson_restarted(ndets, max_m, max_restarts, rp, ci, nz, D,
+                              res_tol, C, E, &it, NULL);
+  free(D);
+  free(rp);
+  free(ci);
+  free(nz);
+  return rc;
+}
+
+int orc_selected_ci_diag(const orc_ham* h, int W, const uint64_t* dets,
+                         int64_t ndets, double h_el_tol, int64_t max_m,
+                         double res_tol, double* C, double* E,
+                         int64_t* nnz_out) {
+  return orc_selected_ci_diag_restarted(h, W, dets, ndets, h_el_tol, max_m, 0,
+                                        res_tol, C, E, nnz_out);
+}
+
+/* ===================================================================== */
+/* ASCI                                                                   */
+/* ===================================================================== */
+void orc_asci_default_settings(orc_asci_settings* s) {
+  /* ASCISettings (determinant_search.hpp:34-51) */
+  s->ntdets_max = 100000;
+  s->ntdets_min = 100;
+  s->ncdets_max = 100;
+  s->h_el_tol = 1e-8;
+  s->rv_prune_tol = 1e-8;
+  s->pair_size_max = 500000000;
+  s->just_singles = 0;
+  s->grow_factor = 8;
+  s->max_refine_iter = 6;
+  s->refine_energy_tol = 1e-6;
+}
+
+typedef struct {
+  det_t s;
+  double rv;
+  int64_t idx; /* generation order: makes every sort deterministic */
+} pair_t;
+
+typedef struct {
+  pair_t* p;
+  int64_t n, cap;
+} pair_buf;
+
+static int pb_push(pair_buf* b, det_t s, double rv) {
+  if(b->n == b->cap) {
+    int64_t nc = b->cap ? 2 * b->cap : 4096;
+    pair_t* q = (pair_t*)realloc(b->p, sizeof(pair_t) * nc);
+    if(!q) return ORC_ERR_NOMEM;
+    b->p = q;
+    b->cap = nc;
+  }
+  b->p[b->n].s = s;
+  b->p[b->n].rv = rv;
+  b->p[b->n].idx = b->n;
+  b->n++;
+  return ORC_OK;
+}
+
+/* bitset_less (bitset_operations.hpp:377-395): numeric order of the whole
+ * word, i.e. beta string most significant. */
+static inline int det_cmp(det_t x, det_t y) {
+  if(x.b != y.b) return x.b < y.b ? -1 : 1;
+  if(x.a != y.a) return x.a < y.a ? -1 : 1;
+  return 0;
+}
+static int cmp_state_idx(const void* x_, const void* y_) {
+  const pair_t *x = (const pair_t*)x_, *y = (const pair_t*)y_;
+  int c = det_cmp(x->s, y->s);
+  if(c) return c;
+  return (x->idx > y->idx) - (x->idx < y->idx);
+}
+static int cmp_absrv_desc(const void* x_, const void* y_) {
+  const pair_t *x = (const pair_t*)x_, *y = (const pair_t*)y_;
+  double ax = fabs(x->rv), ay = fabs(y->rv);
+  if(ax != ay) return ax > ay ? -1 : 1;
+  return det_cmp(x->s, y->s);
+}
+
+/* sort_and_accumulate_asci_pairs (determinant_sort.hpp:39-83): sort by
+ * determinant (generation order among equals), sum rv into the first copy,
+ * drop duplicates. */
+static void sort_and_accumulate(pair_buf* b) {
+  if(!b->n) return;
+  for(int64_t i = 0; i < b->n; ++i) b->p[i].idx = i;
+  qsort(b->p, b->n, sizeof(pair_t), cmp_state_idx);
+  int64_t w = 0;
+  for(int64_t i = 1; i < b->n; ++i) {
+    if(det_cmp(b->p[i].s, b->p[w].s) != 0)
+      b->p[++w] = b->p[i];
+    else
+      b->p[w].rv += b->p[i].rv;
+  }
+  b->n = w + 1;
+}
+
+/* keep_only_largest_copy_asci_pairs (determinant_sort.hpp:85-119) */
+static void keep_largest(pair_buf* b) {
+  if(!b->n) return;
+  for(int64_t i = 0; i < b->n; ++i) b->p[i].idx = i;
+  qsort(b->p, b->n, sizeof(pair_t), cmp_state_idx);
+  int64_t w = 0;
+  for(int64_t i = 1; i < b->n; ++i) {
+    if(det_cmp(b->p[i].s, b->p[w].s) != 0)
+      b->p[++w] = b->p[i];
+    else if(b->p[i].rv > b->p[w].rv)
+      b->p[w].rv = b->p[i].rv;
+  }
+  b->n = w + 1;
+}
+
+/* asci_contributions_standard (determinant_search.hpp:53-141) with the
+ * append_* kernels of determinant_contributions.hpp:25-160. When `prune`
+ * is set, applies the pair_size_max pruning of :120-137. */
+static int asci_contribs(const orc_ham* h, int W, const orc_asci_settings* st,
+                         const uint64_t* cdets, int64_t ncdets, double E0,
+                         const double* C, int64_t norb, pair_buf* out,
+                         int prune) {
+  const int64_t n = h->n, n2 = n * n;
+  const int NB = spin_bits(W);
+  const double tol = st->h_el_tol;
+  uint32_t oa[64], va[64], ob[64], vb[64];
+  int na, nva, nb, nvb;
+  double *eps_a = (double*)malloc(sizeof(double) * norb),
+         *eps_b = (double*)malloc(sizeof(double) * norb);
+  for(int64_t c = 0; c < ncdets; ++c) {
+    det_t s = load_det(W, cdets + W * c);
+    const double coeff = C[c];
+    occ_vir(norb, s.a, oa, &na, va, &nva);
+    occ_vir(norb, s.b, ob, &nb, vb, &nvb);
+    orc_single_orbital_ens(h, norb, oa, na, ob, nb, eps_a);
+    orc_single_orbital_ens(h, norb, ob, nb, oa, na, eps_b);
+    const double h_diag = me_diag(h, oa, na, ob, nb);
+    /* singles AA then BB (append_singles_asci_contributions :25-66) */
+    for(int spin = 0; spin < 2; ++spin) {
+      const uint64_t ss = spin ? s.b : s.a;
+      const uint32_t* occ = spin ? ob : oa;
+      const uint32_t* vir = spin ? vb : va;
+      const uint32_t* oth = spin ? oa : ob;
+      const int no = spin ? nb : na, nv = spin ? nvb : nva,
+                nt = spin ? na : nb;
+      const double* eps = spin ? eps_b : eps_a;
+      for(int ii = 0; ii < no; ++ii)
+        for(int aa = 0; aa < nv; ++aa) {
+          const uint32_t i = occ[ii], a = vir[aa];
+          double h_el = h->T[a + i * n];
+          const double* G_ov = h->G_red + a * n + i * n2;
+          const double* V_ov = h->V_red + a * n + i * n2;
+          for(int p = 0; p < no; ++p) h_el += G_ov[occ[p]];
+          for(int p = 0; p < nt; ++p) h_el += V_ov[oth[p]];
+          if(fabs(h_el) < tol) continue;
+          det_t ex = s;
+          if(spin)
+            ex.b ^= (1ULL << i) | (1ULL << a);
+          else
+            ex.a ^= (1ULL << i) | (1ULL << a);
+          const double sign = orc_single_excitation_sign(ss, a, i);
+          h_el *= sign;
+          const double hd =
+              orc_fast_diag_single(h, eps[i], eps[a], i, a, h_diag);
+          h_el /= (E0 - hd);
+          if(pb_push(out, ex, coeff * h_el)) goto nomem;
+        }
+    }
+    if(!st->just_singles) {
+      /* same-spin doubles AAAA then BBBB (:68-120) */
+      for(int spin = 0; spin < 2; ++spin) {
+        const uint64_t ss = spin ? s.b : s.a;
+        const uint32_t* occ = spin ? ob : oa;
+        const uint32_t* vir = spin ? vb : va;
+        const int no = spin ? nb : na, nv = spin ? nvb : nva;
+        const double* eps = spin ? eps_b : eps_a;
+        for(int ii = 0; ii < no; ++ii)
+          for(int aa = 0; aa < nv; ++aa) {
+            const uint32_t i = occ[ii], a = vir[aa];
+            const double* G_ai = h->G + (a + i * n) * n2;
+            for(int jj = ii + 1; jj < no; ++jj)
+              for(int bb = aa + 1; bb < nv; ++bb) {
+                const uint32_t j = occ[jj], b = vir[bb];
+                const double G_aibj = G_ai[b + j * n];
+                if(fabs(G_aibj) < tol) continue;
+                const uint64_t fex =
+                    (1ULL << i) | (1ULL << j) | (1ULL << a) | (1ULL << b);
+                const uint64_t exs = ss ^ fex;
+                uint32_t o1, v1, o2, v2;
+                const double sign =
+                    orc_doubles_sign_indices(ss, exs, fex, &o1, &v1, &o2, &v2);
+                det_t ex = s;
+                if(spin)
+                  ex.b ^= fex;
+                else
+                  ex.a ^= fex;
+                double h_el = sign * G_aibj;
+                const double hd = orc_fast_diag_ss_double(
+                    h, eps[i], eps[j], eps[a], eps[b], i, j, a, b, h_diag);
+                h_el /= (E0 - hd);
+                if(pb_push(out, ex, coeff * h_el)) goto nomem;
+              }
+          }
+      }
+      /* opposite-spin doubles AABB (:122-160) */
+      for(int ii = 0; ii < na; ++ii)
+        for(int aa = 0; aa < nva; ++aa) {
+          const uint32_t i = oa[ii], a = va[aa];
+          const double* V_ai = h->V + a + i * n;
+          const double sa = orc_single_excitation_sign(s.a, a, i);
+          for(int jj = 0; jj < nb; ++jj)
+            for(int bb = 0; bb < nvb; ++bb) {
+              const uint32_t j = ob[jj], b = vb[bb];
+              const double V_aibj = V_ai[(b + j * n) * n2];
+              if(fabs(V_aibj) < tol) continue;
+              const double sb = orc_single_excitation_sign(s.b, b, j);
+              const double sign = sa * sb;
+              det_t ex = s;
+              ex.a ^= (1ULL << a) | (1ULL << i);
+              ex.b ^= (1ULL << j) | (1ULL << b);
+              double h_el = sign * V_aibj;
+              const double hd = orc_fast_diag_os_double(
+                  h, eps_a[i], eps_b[j], eps_a[a], eps_b[b], i, j, a, b,
+                  h_diag);
+              h_el /= (E0 - hd);
+              if(pb_push(out, ex, coeff * h_el)) goto nomem;
+            }
+        }
+    }
+    (void)NB;
+    if(prune && out->n > st->pair_size_max) { /* :120-137 */
+      int64_t w = 0;
+      for(int64_t i = 0; i < out->n; ++i)
+        if(fabs(out->p[i].rv) > st->rv_prune_tol) out->p[w++] = out->p[i];
+      out->n = w;
+      if(out->n > st->pair_size_max) sort_and_accumulate(out);
+      for(int64_t i = 0; i < out->n; ++i) out->p[i].idx = i;
+    }
+  }
+  free(eps_a);
+  free(eps_b);
+  return ORC_OK;
+nomem:
+  free(eps_a);
+  free(eps_b);
+  return ORC_ERR_NOMEM;
+}
+
+int64_t orc_asci_contributions(const orc_ham* h, int W,
+                               const orc_asci_settings* s,
+                               const uint64_t* cdets, int64_t ncdets,
+                               double E_ASCI, const double* C, int64_t norb,
+                               uint64_t* states, double* rv) {
+  pair_buf b = {NULL, 0, 0};
+  if(asci_contribs(h, W, s, cdets, ncdets, E_ASCI, C, norb, &b, 0)) {
+    free(b.p);
+    return ORC_ERR_NOMEM;
+  }
+  if(states)
+    for(int64_t i = 0; i < b.n; ++i) {
+      store_det(W, b.p[i].s, states + W * i);
+      rv[i] = b.p[i].rv;
+    }
+  int64_t n = b.n;
+  free(b.p);
+  return n;
+}
+
+int orc_asci_search(const orc_ham* h, int W, const orc_asci_settings* s,
+                    int64_t ndets_max, const uint64_t* cdets, int64_t ncdets,
+                    double E_ASCI, const double* C, int64_t norb,
+                    uint64_t* new_dets, int64_t* n_new, double* rv_out) {
+  /* asci_search (determinant_search.hpp:394-663), world_size == 1 */
+  pair_buf b = {NULL, 0, 0};
+  int rc = asci_contribs(h, W, s, cdets, ncdets, E_ASCI, C, norb, &b, 1);
+  if(rc) {
+    free(b.p);
+    return rc;
+  }
+  sort_and_accumulate(&b);                                    /* :498 */
+  for(int64_t i = 0; i < b.n; ++i) b.p[i].rv = -fabs(b.p[i].rv); /* :530 */
+  for(int64_t i = 0; i < ncdets; ++i) /* :533-536 */
+    if(pb_push(&b, load_det(W, cdets + W * i), fabs(C[i]))) {
+      free(b.p);
+      return ORC_ERR_NOMEM;
+    }
+  keep_largest(&b); /* :540 */
+  {                 /* keep rv < 0 (:542-544) */
+    int64_t w = 0;
+    for(int64_t i = 0; i < b.n; ++i)
+      if(b.p[i].rv < 0.0) b.p[w++] = b.p[i];
+    b.n = w;
+  }
+  const int64_t top_k = ndets_max - ncdets; /* :547 */
+  if(b.n > top_k) {                         /* :565-629 */
+    qsort(b.p, b.n, sizeof(pair_t), cmp_absrv_desc);
+    b.n = top_k;
+  }
+  for(int64_t i = 0; i < b.n; ++i) {
+    store_det(W, b.p[i].s, new_dets + W * i);
+    if(rv_out) rv_out[i] = b.p[i].rv;
+  }
+  memcpy(new_dets + W * b.n, cdets, sizeof(uint64_t) * W * ncdets); /* :653 */
+  *n_new = b.n + ncdets;
+  free(b.p);
+  return ORC_OK;
+}
+
+typedef struct {
+  double c;
+  int64_t i;
+} cidx_t;
+static int cmp_absc_desc(const void* x_, const void* y_) {
+  const cidx_t *x = (const cidx_t*)x_, *y = (const cidx_t*)y_;
+  double ax = fabs(x->c), ay = fabs(y->c);
+  if(ax != ay) return ax > ay ? -1 : 1;
+  return (x->i > y->i) - (x->i < y->i);
+}
+
+int orc_asci_iter(const orc_ham* h, int W, const orc_asci_settings* s,
+                  int64_t ci_max_subspace, double ci_res_tol,
+                  double ci_matel_tol, int64_t ndets_max, double E0,
+                  uint64_t* wfn, double* X, int64_t* nwfn, int64_t norb,
+                  double* E_out) {
+  /* asci_iter (iteration.hpp:16-67) */
+  int64_t nw = *nwfn;
+  if(nw > 1) { /* reorder_ci_on_coeff (determinant_sort.hpp:18-37) */
+    cidx_t* o = (cidx_t*)malloc(sizeof(cidx_t) * nw);
+    uint64_t* tw = (uint64_t*)malloc(sizeof(uint64_t) * W * nw);
+    double* tx = (double*)malloc(sizeof(double) * nw);
+    for(int64_t i = 0; i < nw; ++i) {
+      o[i].c = X[i];
+      o[i].i = i;
+    }
+    qsort(o, nw, sizeof(cidx_t), cmp_absc_desc);
+    for(int64_t i = 0; i < nw; ++i) {
+      tx[i] = X[o[i].i];
+      memcpy(tw + W * i, wfn + W * o[i].i, sizeof(uint64_t) * W);
+    }
+    memcpy(X, tx, sizeof(double) * nw);
+    memcpy(wfn, tw, sizeof(uint64_t) * W * nw);
+    free(o);
+    free(tw);
+    free(tx);
+  }
+  const int64_t nkeep = s->ncdets_max < nw ? s->ncdets_max : nw;
+  uint64_t* core = (uint64_t*)malloc(sizeof(uint64_t) * W * (nkeep + 1));
+  double* Cc = (double*)malloc(sizeof(double) * (nkeep + 1));
+  memcpy(core, wfn, sizeof(uint64_t) * W * nkeep);
+  memcpy(Cc, X, sizeof(double) * nkeep);
+  int64_t nn = 0;
+  int rc = orc_asci_search(h, W, s, ndets_max, core, nkeep, E0, Cc, norb, wfn,
+                           &nn, NULL);
+  free(core);
+  free(Cc);
+  if(rc) return rc;
+  *nwfn = nn;
+  memset(X, 0, sizeof(double) * nn); /* fresh guess (iteration.hpp:33) */
+  return orc_selected_ci_diag(h, W, wfn, nn, ci_matel_tol, ci_max_subspace,
+                              ci_res_tol, X, E_out, NULL);
+}
+
+int orc_asci_grow(const orc_ham* h, int W, const orc_asci_settings* s,
+                  int64_t ci_max_subspace, double ci_res_tol,
+                  double ci_matel_tol, double* E0, uint64_t* wfn, double* X,
+                  int64_t* nwfn, int64_t norb) {
+  /* asci_grow (grow.hpp:16-173), grow_with_rot = false */
+  while(*nwfn < s->ntdets_max) {
+    int64_t nd = *nwfn * s->grow_factor;
+    if(nd < s->ntdets_min) nd = s->ntdets_min;
+    if(nd > s->ntdets_max) nd = s->ntdets_max;
+    double E;
+    int rc = orc_asci_iter(h, W, s, ci_max_subspace, ci_res_tol, ci_matel_tol,
+                           nd, *E0, wfn, X, nwfn, norb, &E);
+    if(rc) return rc;
+    if(nd > *nwfn) return ORC_ERR_DID_NOT_GROW;
+    *E0 = E;
+  }
+  return ORC_OK;
+}
+
+int orc_asci_refine(const orc_ham* h, int W, const orc_asci_settings* s,
+                    int64_t ci_max_subspace, double ci_res_tol,
+                    double ci_matel_tol, double* E0, uint64_t* wfn,
+                    double* X, int64_t* nwfn, int64_t norb) {
+  /* asci_refine (refine.hpp:14-68) */
+  const int64_t ndets = *nwfn;
+  for(int64_t it = 0; it < s->max_refine_iter; ++it) {
+    double E;
+    int rc = orc_asci_iter(h, W, s, ci_max_subspace, ci_res_tol, ci_matel_tol,
+                           ndets, *E0, wfn, X, nwfn, norb, &E);
+    if(rc) return rc;
+    if(*nwfn != ndets) return ORC_ERR_REFINE_SIZE;
+    const double dE = E - *E0;
+    *E0 = E;
+    if(fabs(dE) < s->refine_energy_tol) return ORC_OK;
+  }
+  return ORC_ERR_REFINE;
+}
