@@ -1,0 +1,232 @@
+/*
+ * macis_hip.h -- C-ABI of the MI355X (gfx950) determinant-CI hot path.
+ *
+ * Plain pointers and sizes only; no C++ or framework types cross this
+ * boundary. Every entry point returns MHIP_SUCCESS (0) or a negative
+ * MHIP_ERR_* code; mhip_last_error() gives the thread's last message. The
+ * C++ mirror of the reference API (macis_amd/include/macis/...) maps the
+ * codes to the reference's std::runtime_error messages.
+ *
+ * Reference seams replaced (paths inside dfablan/MACIS):
+ *   - HamiltonianGenerator<N> ctor + generate_integral_intermediates
+ *       include/macis/hamiltonian_generator/impl.hpp:14-70
+ *   - HamiltonianGenerator<N>::make_csr_hamiltonian_block_{32,64}bit_
+ *       include/macis/hamiltonian_generator.hpp:66-72, implemented by
+ *       include/macis/hamiltonian_generator/double_loop.hpp:30-114
+ *   - sparsexx::spblas::gespmbv
+ *       src/sparsexx/include/sparsexx/spblas/spmbv.hpp:37-73
+ *   - sparsexx::extract_diagonal_elements
+ *       src/sparsexx/include/sparsexx/util/submatrix.hpp:264-295
+ *   - macis::davidson / diagonal_guess / SparseMatrixOperator
+ *       include/macis/solvers/davidson.hpp:29-225
+ *   - macis::selected_ci_diag / serial_selected_ci_diag
+ *       include/macis/solvers/selected_ci_diag.hpp:75-207
+ *   - macis::asci_search (world_size == 1)
+ *       include/macis/asci/determinant_search.hpp:394-663
+ *
+ * Determinant layout (std::bitset<N> words, bitset_operations.hpp:353-375):
+ *   words_per_det = 1 -> wfn_t<64>:  alpha in bits 0-31, beta in bits 32-63
+ *   words_per_det = 2 -> wfn_t<128>: word 0 alpha, word 1 beta
+ * Integrals are column-major: T(p,q) = T[p + n*q],
+ *   V(p,q,r,s) = V[p + n*q + n^2*r + n^3*s]   (types.hpp:22-33).
+ *
+ * Device CSR layout (one GPU, HBM resident): rowptr int64[m+1] (0-based),
+ * colind int32[nnz], nzval double[nnz]; columns ascending within a row.
+ *
+ * Threading: a context owns one HIP stream and is used by one host thread
+ * at a time (the reference generator is not re-entrant either,
+ * hamiltonian_generator.hpp:40-64).
+ */
+#ifndef MACIS_HIP_H
+#define MACIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHIP_VERSION 10000
+
+/* ---- status codes ------------------------------------------------------ */
+#define MHIP_SUCCESS 0
+#define MHIP_ERR_INVALID_ARG -1
+#define MHIP_ERR_HIP -2
+#define MHIP_ERR_OUT_OF_MEMORY -3
+#define MHIP_ERR_NO_INTEGRALS -4
+#define MHIP_ERR_NOT_CONVERGED -5   /* davidson.hpp:221 */
+#define MHIP_ERR_NO_GUESS -6        /* davidson.hpp:134 */
+#define MHIP_ERR_UNSUPPORTED -7     /* hamiltonian_generator.hpp:153 */
+#define MHIP_ERR_INDEX_OVERFLOW -8  /* nnz does not fit the index type */
+#define MHIP_ERR_NO_DEVICE -9
+
+typedef struct mhip_context mhip_context;
+typedef struct mhip_csr mhip_csr;
+
+/* Phase timings of the last call on a context, milliseconds measured with
+ * HIP events on the context's stream. */
+typedef struct mhip_timings {
+  double ham_bounds_ms;   /* ket tile AND/OR bounds */
+  double ham_count_ms;    /* structural count pass (XOR/popcount scan) */
+  double ham_scan_ms;     /* device exclusive scan of row counts */
+  double ham_fill_ms;     /* structural fill pass (ascending columns) */
+  double ham_elem_ms;     /* Slater-Condon elements + threshold marks */
+  double ham_compact_ms;  /* threshold compaction (0 when nothing dropped) */
+  double ham_total_ms;
+  int64_t ham_nnz_struct; /* pairs with popcount(bra^ket) <= 4 */
+  int64_t ham_nnz;        /* kept after |h| > thresh */
+  double dav_spmv_ms;     /* summed SpMV time inside the last Davidson */
+  int64_t dav_spmv_calls;
+  double dav_total_ms;
+  int64_t dav_iters;
+  int64_t dav_restarts;
+  double dav_residual;
+  double asci_contrib_ms; /* candidate generation */
+  double asci_sort_ms;    /* sort + accumulate + dedupe */
+  double asci_topk_ms;    /* exclusion of core dets + top-k selection */
+  double asci_total_ms;
+  int64_t asci_npairs;    /* raw candidates */
+  int64_t asci_nunique;   /* unique candidates after accumulation */
+} mhip_timings;
+
+/* Settings of asci_search (ASCISettings, determinant_search.hpp:34-51). */
+typedef struct mhip_asci_settings {
+  int64_t ncdets_max;
+  double h_el_tol;
+  double rv_prune_tol;
+  int64_t pair_size_max;
+  int32_t just_singles;
+} mhip_asci_settings;
+
+/* ---- library / context --------------------------------------------------- */
+int mhip_get_version(void);
+const char* mhip_error_string(int code);
+/* Message of the last failing call on this host thread ("" if none). */
+const char* mhip_last_error(void);
+
+/* Creates a context on `device` with its own HIP stream. */
+int mhip_context_create(int device, mhip_context** ctx);
+int mhip_context_destroy(mhip_context* ctx);
+/* hipStream_t of the context (for events / interop). */
+int mhip_context_get_stream(mhip_context* ctx, void** stream);
+int mhip_synchronize(mhip_context* ctx);
+int mhip_get_timings(mhip_context* ctx, mhip_timings* out);
+
+/* Device memory helpers, so callers need no HIP runtime of their own. */
+int mhip_malloc(mhip_context* ctx, size_t bytes, void** dptr);
+int mhip_free(mhip_context* ctx, void* dptr);
+int mhip_memcpy_h2d(mhip_context* ctx, void* dst, const void* src,
+                    size_t bytes);
+int mhip_memcpy_d2h(mhip_context* ctx, void* dst, const void* src,
+                    size_t bytes);
+int mhip_memcpy_d2d(mhip_context* ctx, void* dst, const void* src,
+                    size_t bytes);
+int mhip_memset(mhip_context* ctx, void* dst, int value, size_t bytes);
+
+/* ---- integrals ----------------------------------------------------------- */
+/* HamiltonianGenerator(T, V) + generate_integral_intermediates
+ * (impl.hpp:14-70). T (norb^2) and V (norb^4) are HOST pointers, copied;
+ * G, G_red, V_red, G2_red and V2_red are formed and kept on the device.
+ * Calling again re-uploads (asci_grow rotates integrals in place,
+ * grow.hpp:104-122). norb <= 64. */
+int mhip_set_integrals(mhip_context* ctx, int64_t norb, const double* T,
+                       const double* V);
+
+/* ---- Hamiltonian generation (K1) ------------------------------------------ */
+/* make_csr_hamiltonian_block over bra x ket with |h| > thresh, built on the
+ * device (double_loop.hpp:30-98): structural count, device scan, ascending
+ * fill, Slater-Condon elements, threshold compaction. d_bra / d_ket are
+ * DEVICE arrays of words_per_det uint64 per determinant. Zero determinants
+ * give empty rows and are never columns. nbra == 0 or nket == 0 gives an
+ * empty matrix (csr_hamiltonian.hpp:30-35). The CSR is owned by *out. */
+int mhip_ham_build(mhip_context* ctx, int words_per_det,
+                   const uint64_t* d_bra, int64_t nbra, const uint64_t* d_ket,
+                   int64_t nket, double thresh, mhip_csr** out);
+
+/* Diagonal Slater-Condon elements H_ii of each determinant
+ * (matrix_element_diag, matrix_elements.hpp:108-135) into DEVICE d_diag. */
+int mhip_ham_diagonal(mhip_context* ctx, int words_per_det,
+                      const uint64_t* d_dets, int64_t ndets, double* d_diag);
+
+/* ---- CSR handles ----------------------------------------------------------- */
+int mhip_csr_info(const mhip_csr* csr, int64_t* m, int64_t* n, int64_t* nnz);
+/* Device arrays of the handle (borrowed; valid until mhip_csr_destroy). */
+int mhip_csr_device_arrays(const mhip_csr* csr, const int64_t** d_rowptr,
+                           const int32_t** d_colind, const double** d_nzval);
+/* Copy to HOST arrays in the reference csr_matrix<double,int32_t> /
+ * <double,int64_t> layout (hamiltonian_generator.hpp:27-28). The int32
+ * variant fails with MHIP_ERR_INDEX_OVERFLOW when nnz > INT32_MAX. */
+int mhip_csr_copy_to_host_i32(mhip_context* ctx, const mhip_csr* csr,
+                              int32_t* rowptr, int32_t* colind,
+                              double* nzval);
+int mhip_csr_copy_to_host_i64(mhip_context* ctx, const mhip_csr* csr,
+                              int64_t* rowptr, int64_t* colind,
+                              double* nzval);
+/* Upload a host CSR (rowptr int64[m+1], colind int32[nnz]) with base
+ * `indexing` (0 or 1, csr_matrix.hpp:65-82) into a device handle. */
+int mhip_csr_create_from_host(mhip_context* ctx, int64_t m, int64_t n,
+                              int64_t nnz, int64_t indexing,
+                              const int64_t* rowptr, const int32_t* colind,
+                              const double* nzval, mhip_csr** out);
+int mhip_csr_destroy(mhip_csr* csr);
+
+/* ---- sparse BLAS (K2) ------------------------------------------------------ */
+/* gespmbv: AV[:,k] = alpha*A*V[:,k] + beta*AV[:,k] for k < K, on DEVICE
+ * column-major blocks (spmbv.hpp:37-73). */
+int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
+              const double* d_V, int64_t LDV, double beta, double* d_AV,
+              int64_t LDAV);
+/* extract_diagonal_elements (submatrix.hpp:264-295) into DEVICE d_D. */
+int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
+
+/* ---- Davidson (K3-K5) ------------------------------------------------------ */
+/* davidson(N, max_m, SparseMatrixOperator(A), D, tol, X) with everything
+ * on the device (davidson.hpp:128-225). d_X (DEVICE, length N) is the guess
+ * in and the Ritz vector out; d_D may be NULL (extracted from A). With
+ * max_restarts == 0 the reference algorithm is followed exactly and
+ * MHIP_ERR_NOT_CONVERGED is returned when max_m is exhausted; with
+ * max_restarts > 0 (a labelled variant, not in the reference) an exhausted
+ * subspace restarts from [X, next preconditioned residual]. E receives
+ * LAM(0), iters the iteration count (davidson.hpp:224). */
+int mhip_davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
+                  int64_t max_restarts, double tol, const double* d_D,
+                  double* d_X, double* E, int64_t* iters);
+
+/* Tall-skinny FP64 GEMM on MFMA: C(k x m, ldc) = A^T B with A N x k (lda)
+ * and B N x m (ldb), all DEVICE, k, m <= 32. This is the projection of
+ * lobpcgxx::rayleigh_ritz (rayleigh_ritz.hpp:73-74) and of the CGS2 inner
+ * products (davidson.hpp:114-121) that Davidson runs on the device. */
+int mhip_gemm_tn(mhip_context* ctx, int64_t N, int k, const double* d_A,
+                 int64_t lda, int m, const double* d_B, int64_t ldb,
+                 double* d_C, int ldc);
+
+/* selected_ci_diag (selected_ci_diag.hpp:124-207, serial path): build H over
+ * DEVICE d_dets with threshold h_el_tol, extract the diagonal, guess rule
+ * (reuse d_C if max|C| > 1/ndets, else unit vector at argmin D), Davidson.
+ * d_C (DEVICE, length ndets) is guess in / eigenvector out. If H_out is not
+ * NULL the Hamiltonian handle is returned to the caller. */
+int mhip_selected_ci_diag(mhip_context* ctx, int words_per_det,
+                          const uint64_t* d_dets, int64_t ndets,
+                          double h_el_tol, int64_t max_m, int64_t max_restarts,
+                          double res_tol, double* d_C, double* E,
+                          int64_t* nnz, mhip_csr** H_out);
+
+/* ---- ASCI search (K6-K8) ---------------------------------------------------- */
+/* asci_search (determinant_search.hpp:394-663, world_size == 1): scores
+ * all singles/doubles of the DEVICE core dets d_cdets (coefficients DEVICE
+ * d_C, length ncdets) against E_ASCI, accumulates duplicates, removes core
+ * dets and keeps the ndets_max - ncdets largest |rv|. Writes the new list
+ * (top-k, then the core dets) to DEVICE d_new_dets (capacity ndets_max) and
+ * its length to *n_new. Top-k ties are broken by determinant order. */
+int mhip_asci_search(mhip_context* ctx, int words_per_det,
+                     const mhip_asci_settings* settings, int64_t ndets_max,
+                     const uint64_t* d_cdets, int64_t ncdets, double E_ASCI,
+                     const double* d_C, int64_t norb, uint64_t* d_new_dets,
+                     int64_t* n_new);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MACIS_HIP_H */

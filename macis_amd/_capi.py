@@ -1,0 +1,343 @@
+"""ctypes binding of libmacis_hip.so (the C-ABI declared in include/macis_hip.h).
+
+This is the Python-side caller used by the tests and the benchmark; the
+C++ mirror of the reference API (macis_amd/include/macis/...) is the other
+caller. Importing raises if the HIP library has not been built: there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmacis_hip.so")
+
+MHIP_SUCCESS = 0
+MHIP_ERR_NOT_CONVERGED = -5
+MHIP_ERR_INDEX_OVERFLOW = -8
+
+_p = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+_i64 = C.c_int64
+_i32 = C.c_int
+_d = C.c_double
+_pi64 = C.POINTER(C.c_int64)
+_pd = C.POINTER(C.c_double)
+
+
+class Timings(C.Structure):
+    _fields_ = [
+        ("ham_bounds_ms", C.c_double),
+        ("ham_count_ms", C.c_double),
+        ("ham_scan_ms", C.c_double),
+        ("ham_fill_ms", C.c_double),
+        ("ham_elem_ms", C.c_double),
+        ("ham_compact_ms", C.c_double),
+        ("ham_total_ms", C.c_double),
+        ("ham_nnz_struct", C.c_int64),
+        ("ham_nnz", C.c_int64),
+        ("dav_spmv_ms", C.c_double),
+        ("dav_spmv_calls", C.c_int64),
+        ("dav_total_ms", C.c_double),
+        ("dav_iters", C.c_int64),
+        ("dav_restarts", C.c_int64),
+        ("dav_residual", C.c_double),
+        ("asci_contrib_ms", C.c_double),
+        ("asci_sort_ms", C.c_double),
+        ("asci_topk_ms", C.c_double),
+        ("asci_total_ms", C.c_double),
+        ("asci_npairs", C.c_int64),
+        ("asci_nunique", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ASCISettings(C.Structure):
+    """mhip_asci_settings (ASCISettings, determinant_search.hpp:34-51)."""
+
+    _fields_ = [
+        ("ncdets_max", C.c_int64),
+        ("h_el_tol", C.c_double),
+        ("rv_prune_tol", C.c_double),
+        ("pair_size_max", C.c_int64),
+        ("just_singles", C.c_int32),
+    ]
+
+    @classmethod
+    def defaults(cls, **kw):
+        s = cls(ncdets_max=100, h_el_tol=1e-8, rv_prune_tol=1e-8,
+                pair_size_max=500000000, just_singles=0)
+        for k, v in kw.items():
+            setattr(s, k, v)
+        return s
+
+
+_SIGS = {
+    "mhip_get_version": (_i32, []),
+    "mhip_error_string": (C.c_char_p, [_i32]),
+    "mhip_last_error": (C.c_char_p, []),
+    "mhip_context_create": (_i32, [_i32, _pp]),
+    "mhip_context_destroy": (_i32, [_p]),
+    "mhip_context_get_stream": (_i32, [_p, _pp]),
+    "mhip_synchronize": (_i32, [_p]),
+    "mhip_get_timings": (_i32, [_p, C.POINTER(Timings)]),
+    "mhip_malloc": (_i32, [_p, C.c_size_t, _pp]),
+    "mhip_free": (_i32, [_p, _p]),
+    "mhip_memcpy_h2d": (_i32, [_p, _p, _p, C.c_size_t]),
+    "mhip_memcpy_d2h": (_i32, [_p, _p, _p, C.c_size_t]),
+    "mhip_memcpy_d2d": (_i32, [_p, _p, _p, C.c_size_t]),
+    "mhip_memset": (_i32, [_p, _p, _i32, C.c_size_t]),
+    "mhip_set_integrals": (_i32, [_p, _i64, _p, _p]),
+    "mhip_ham_build": (_i32, [_p, _i32, _p, _i64, _p, _i64, _d, _pp]),
+    "mhip_ham_diagonal": (_i32, [_p, _i32, _p, _i64, _p]),
+    "mhip_csr_info": (_i32, [_p, _pi64, _pi64, _pi64]),
+    "mhip_csr_device_arrays": (_i32, [_p, _pp, _pp, _pp]),
+    "mhip_csr_copy_to_host_i32": (_i32, [_p, _p, _p, _p, _p]),
+    "mhip_csr_copy_to_host_i64": (_i32, [_p, _p, _p, _p, _p]),
+    "mhip_csr_create_from_host": (_i32, [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _pp]),
+    "mhip_csr_destroy": (_i32, [_p]),
+    "mhip_spmv": (_i32, [_p, _p, _i64, _d, _p, _i64, _d, _p, _i64]),
+    "mhip_extract_diagonal": (_i32, [_p, _p, _p]),
+    "mhip_gemm_tn": (_i32, [_p, _i64, _i32, _p, _i64, _i32, _p, _i64, _p, _i32]),
+    "mhip_davidson": (_i32, [_p, _p, _i64, _i64, _d, _p, _p, _pd, _pi64]),
+    "mhip_selected_ci_diag": (_i32, [_p, _i32, _p, _i64, _d, _i64, _i64, _d, _p, _pd, _pi64, _pp]),
+    "mhip_asci_search": (_i32, [_p, _i32, C.POINTER(ASCISettings), _i64, _p, _i64, _d, _p, _i64,
+                                _p, _pi64]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it is missing (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Names of the entry points declared in include/macis_hip.h."""
+    import re
+
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "macis_hip.h")
+    with open(hdr) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(mhip_[a-z0-9_]+)\s*\(", txt)))
+
+
+class MhipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def check(rc, what=""):
+    if rc != MHIP_SUCCESS:
+        L = lib()
+        msg = L.mhip_last_error().decode() or L.mhip_error_string(rc).decode()
+        raise MhipError(rc, f"{what}: {msg} (code {rc})")
+
+
+class DeviceArray:
+    """Owning device buffer with a numpy dtype/shape (freed on __del__)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().mhip_malloc(ctx.h, self.nbytes, C.byref(p)), "mhip_malloc")
+        self.ptr = p.value or 0
+
+    def __del__(self):
+        if getattr(self, "ptr", 0) and self.ctx.h:
+            lib().mhip_free(self.ctx.h, self.ptr)
+            self.ptr = 0
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            check(lib().mhip_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), "d2h")
+        return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        if self.nbytes:
+            check(lib().mhip_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes), "h2d")
+        return self
+
+    def zero(self):
+        if self.nbytes:
+            check(lib().mhip_memset(self.ctx.h, self.ptr, 0, self.nbytes), "memset")
+        return self
+
+
+class CSR:
+    """Device CSR handle (mhip_csr)."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().mhip_csr_info(h, C.byref(m), C.byref(n), C.byref(nnz)), "csr_info")
+        self.m, self.n, self.nnz = m.value, n.value, nnz.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mhip_csr_destroy(self.h)
+            self.h = None
+
+    def to_host(self):
+        """(rowptr int64, colind int32, nzval float64) copied to the host."""
+        rp = np.empty(self.m + 1, dtype=np.int64)
+        ci64 = np.empty(max(self.nnz, 1), dtype=np.int64)
+        nz = np.empty(max(self.nnz, 1), dtype=np.float64)
+        check(lib().mhip_csr_copy_to_host_i64(self.ctx.h, self.h, rp.ctypes.data,
+                                              ci64.ctypes.data, nz.ctypes.data), "copy_to_host")
+        return rp, ci64[: self.nnz].astype(np.int32), nz[: self.nnz]
+
+    def to_host_i32(self):
+        rp = np.empty(self.m + 1, dtype=np.int32)
+        ci = np.empty(max(self.nnz, 1), dtype=np.int32)
+        nz = np.empty(max(self.nnz, 1), dtype=np.float64)
+        check(lib().mhip_csr_copy_to_host_i32(self.ctx.h, self.h, rp.ctypes.data,
+                                              ci.ctypes.data, nz.ctypes.data), "copy_to_host_i32")
+        return rp, ci[: self.nnz], nz[: self.nnz]
+
+    def device_arrays(self):
+        rp, ci, nz = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().mhip_csr_device_arrays(self.h, C.byref(rp), C.byref(ci), C.byref(nz)), "arrays")
+        return rp.value, ci.value, nz.value
+
+
+class Context:
+    """mhip_context: one device, one stream."""
+
+    def __init__(self, device=0):
+        p = C.c_void_p()
+        check(lib().mhip_context_create(device, C.byref(p)), "context_create")
+        self.h = p.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhip_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- memory
+    def empty(self, shape, dtype):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or np.asarray(a).dtype)
+        return DeviceArray(self, a.shape, a.dtype).upload(a)
+
+    def synchronize(self):
+        check(lib().mhip_synchronize(self.h), "synchronize")
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().mhip_context_get_stream(self.h, C.byref(s)), "get_stream")
+        return s.value
+
+    def timings(self):
+        t = Timings()
+        check(lib().mhip_get_timings(self.h, C.byref(t)), "get_timings")
+        return t.as_dict()
+
+    # ---- hot path
+    def set_integrals(self, norb, T, V):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        assert T.size == norb**2 and V.size == norb**4
+        check(lib().mhip_set_integrals(self.h, norb, T.ctypes.data, V.ctypes.data), "set_integrals")
+
+    def ham_build(self, W, d_bra, nbra, d_ket=None, nket=None, thresh=1e-16):
+        if d_ket is None:
+            d_ket, nket = d_bra, nbra
+        out = C.c_void_p()
+        check(lib().mhip_ham_build(self.h, W, _ptr(d_bra), nbra, _ptr(d_ket), nket, thresh,
+                                   C.byref(out)), "ham_build")
+        return CSR(self, out.value)
+
+    def ham_diagonal(self, W, d_dets, n, d_out):
+        check(lib().mhip_ham_diagonal(self.h, W, _ptr(d_dets), n, _ptr(d_out)), "ham_diagonal")
+
+    def csr_from_host(self, m, n, rowptr, colind, nzval, indexing=0):
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        colind = np.ascontiguousarray(colind, dtype=np.int32)
+        nzval = np.ascontiguousarray(nzval, dtype=np.float64)
+        out = C.c_void_p()
+        nnz = len(nzval)
+        check(lib().mhip_csr_create_from_host(self.h, m, n, nnz, indexing, rowptr.ctypes.data,
+                                              colind.ctypes.data, nzval.ctypes.data,
+                                              C.byref(out)), "csr_create_from_host")
+        return CSR(self, out.value)
+
+    def spmv(self, A, K, alpha, d_V, LDV, beta, d_AV, LDAV):
+        check(lib().mhip_spmv(self.h, A.h, K, alpha, _ptr(d_V), LDV, beta, _ptr(d_AV), LDAV),
+              "spmv")
+
+    def extract_diagonal(self, A, d_D):
+        check(lib().mhip_extract_diagonal(self.h, A.h, _ptr(d_D)), "extract_diagonal")
+
+    def gemm_tn(self, N, k, d_A, lda, m, d_B, ldb, d_C, ldc):
+        check(lib().mhip_gemm_tn(self.h, N, k, _ptr(d_A), lda, m, _ptr(d_B), ldb, _ptr(d_C), ldc),
+              "gemm_tn")
+
+    def davidson(self, A, max_m, tol, d_X, d_D=None, max_restarts=0):
+        E = C.c_double()
+        it = C.c_int64()
+        check(lib().mhip_davidson(self.h, A.h, max_m, max_restarts, tol,
+                                  _ptr(d_D) if d_D is not None else None, _ptr(d_X),
+                                  C.byref(E), C.byref(it)), "davidson")
+        return it.value, E.value
+
+    def selected_ci_diag(self, W, d_dets, ndets, d_C, h_el_tol=np.finfo(float).eps, max_m=20,
+                         res_tol=1e-8, max_restarts=0, keep_H=False):
+        E = C.c_double()
+        nnz = C.c_int64()
+        H = C.c_void_p()
+        check(lib().mhip_selected_ci_diag(self.h, W, _ptr(d_dets), ndets, h_el_tol, max_m,
+                                          max_restarts, res_tol, _ptr(d_C), C.byref(E),
+                                          C.byref(nnz), C.byref(H) if keep_H else None),
+              "selected_ci_diag")
+        return E.value, nnz.value, (CSR(self, H.value) if keep_H else None)
+
+    def asci_search(self, W, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out):
+        n = C.c_int64()
+        check(lib().mhip_asci_search(self.h, W, C.byref(settings), ndets_max, _ptr(d_cdets),
+                                     ncdets, E_ASCI, _ptr(d_C), norb, _ptr(d_out), C.byref(n)),
+              "asci_search")
+        return n.value
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    return int(x)

@@ -1,0 +1,273 @@
+// Tests of the C++ mirror of the MACIS API (macis_amd/include), written the
+// way the reference's Catch2 tests use that API (tests/double_loop.cxx,
+// csr_hamiltonian.cxx, davidson.cxx, asci.cxx, sd_operations.cxx,
+// src/sparsexx/tests/test_spmbv.cxx), with tighter tolerances.
+//
+// usage: test_api <golden dir> [--cpu-only]
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include <macis/asci/grow.hpp>
+#include <macis/asci/refine.hpp>
+#include <macis/csr_hamiltonian.hpp>
+#include <macis/hip/hip_hamiltonian_generator.hpp>
+#include <macis/sd_operations.hpp>
+#include <macis/solvers/davidson.hpp>
+#include <macis/solvers/selected_ci_diag.hpp>
+#include <macis/util/fcidump.hpp>
+#include <sparsexx/spblas/spmbv.hpp>
+#include <sparsexx/util/submatrix.hpp>
+
+static int g_fail = 0, g_checks = 0;
+#define REQUIRE(cond)                                                   \
+  do {                                                                  \
+    ++g_checks;                                                         \
+    if(!(cond)) {                                                       \
+      ++g_fail;                                                         \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);   \
+    }                                                                   \
+  } while(0)
+#define REQUIRE_NEAR(a, b, tol)                                                           \
+  do {                                                                                    \
+    ++g_checks;                                                                           \
+    const double _a = (a), _b = (b);                                                      \
+    if(!(std::abs(_a - _b) <= (tol))) {                                                   \
+      ++g_fail;                                                                           \
+      std::printf("  FAILED %s:%d: %s = %.15g vs %.15g\n", __FILE__, __LINE__, #a, _a, _b); \
+    }                                                                                     \
+  } while(0)
+
+struct H2O {
+  size_t norb;
+  double E_core;
+  std::vector<double> T, V;
+};
+
+static H2O load_h2o(const std::string& dir) {
+  const std::string f = dir + "/h2o.ccpvdz.fci.dat";
+  H2O h;
+  h.norb = macis::read_fcidump_norb(f);
+  h.E_core = macis::read_fcidump_core(f);
+  h.T.assign(h.norb * h.norb, 0.);
+  h.V.assign(h.norb * h.norb * h.norb * h.norb, 0.);
+  macis::read_fcidump_1body(f, h.T.data(), h.norb);
+  macis::read_fcidump_2body(f, h.V.data(), h.norb);
+  return h;
+}
+
+// ---------------------------------------------------------------- CPU cases
+static void test_sd_operations() {
+  // sd_operations.cxx:217-237
+  auto combs = macis::generate_combs<64>(4, 2);
+  std::vector<unsigned long long> ref = {0x3, 0x5, 0x9, 0x6, 0xA, 0xC};
+  REQUIRE(combs.size() == ref.size());
+  for(size_t i = 0; i < ref.size(); ++i) REQUIRE(combs[i].to_ullong() == ref[i]);
+  auto dets = macis::generate_hilbert_space<64>(4, 2, 2);
+  REQUIRE(dets.size() == 36);
+  REQUIRE(dets[0].to_ullong() == 0x300000003ull);
+  REQUIRE(dets[1].to_ullong() == 0x500000003ull);
+  REQUIRE(dets[35].to_ullong() == 0xC0000000Cull);
+  auto hf = macis::canonical_hf_determinant<64>(4, 2);
+  REQUIRE(hf.to_ullong() == 0x30000000Full);
+  auto h128 = macis::canonical_hf_determinant<128>(3, 2);
+  REQUIRE(macis::bitset_lo_word(h128).to_ullong() == 0x7ull);
+  REQUIRE(macis::bitset_hi_word(h128).to_ullong() == 0x3ull);
+  std::bitset<64> s(0xAFull);
+  std::vector<uint32_t> occ, vir;
+  macis::bitset_to_occ_vir(12, s, occ, vir);  // sd_operations.cxx:32-40
+  REQUIRE((occ == std::vector<uint32_t>{0, 1, 2, 3, 5, 7}));
+  REQUIRE((vir == std::vector<uint32_t>{4, 6, 8, 9, 10, 11}));
+  REQUIRE(macis::bitset_less(std::bitset<64>(0x100000000ull), std::bitset<64>(0x200000000ull)));
+  REQUIRE(!macis::bitset_less(std::bitset<64>(0x100000001ull), std::bitset<64>(0x000000002ull)));
+  auto cisd = macis::generate_cisd_hilbert_space<64>(8, macis::canonical_hf_determinant<64>(4, 4));
+  REQUIRE(cisd.size() == 1 + 2 * 16 + 2 * 36 + 256);
+}
+
+static void test_host_generator_elements(const H2O& h) {
+  // The base-class matrix_element (double_loop.cxx:44-53), through a
+  // generator that is never asked for a CSR.
+  struct NoCsr : macis::HamiltonianGenerator<64> {
+    using HamiltonianGenerator::HamiltonianGenerator;
+    sparse_matrix_type<int32_t> make_csr_hamiltonian_block_32bit_(full_det_iterator, full_det_iterator,
+                                                                  full_det_iterator, full_det_iterator,
+                                                                  double) override {
+      throw std::runtime_error("unused");
+    }
+    sparse_matrix_type<int64_t> make_csr_hamiltonian_block_64bit_(full_det_iterator, full_det_iterator,
+                                                                  full_det_iterator, full_det_iterator,
+                                                                  double) override {
+      throw std::runtime_error("unused");
+    }
+    void form_rdms(full_det_iterator, full_det_iterator, full_det_iterator, full_det_iterator,
+                   double*, matrix_span_t, rank4_span_t) override {}
+  };
+  auto T = h.T;
+  auto V = h.V;
+  NoCsr g(macis::matrix_span<double>(T.data(), h.norb, h.norb),
+          macis::rank4_span<double>(V.data(), h.norb, h.norb, h.norb, h.norb));
+  auto hf = macis::canonical_hf_determinant<64>(5, 5);
+  REQUIRE_NEAR(g.matrix_element(hf, hf) + h.E_core, -76.0267803489191, 1e-10);
+  auto s = hf;
+  s.flip(0).flip(5);
+  REQUIRE_NEAR(g.matrix_element(s, s), -6.488097259228e+01, 1e-10);
+  bool threw = false;
+  try {
+    std::vector<macis::wfn_t<64>> d{hf};
+    g.make_csr_hamiltonian_block<int16_t>(d.begin(), d.end(), d.begin(), d.end(), 0.);
+  } catch(const std::runtime_error& e) {
+    threw = std::string(e.what()) == "Unsupported index_t";
+  }
+  REQUIRE(threw);
+}
+
+// ---------------------------------------------------------------- GPU cases
+static void test_csr_hamiltonian(const H2O& h, const std::string& dir) {
+  // csr_hamiltonian.cxx:18-71
+  auto T = h.T;
+  auto V = h.V;
+  macis::HipHamiltonianGenerator<64> gen(macis::matrix_span<double>(T.data(), h.norb, h.norb),
+                                         macis::rank4_span<double>(V.data(), h.norb, h.norb,
+                                                                   h.norb, h.norb));
+  auto hf = macis::canonical_hf_determinant<64>(5, 5);
+  auto dets = macis::generate_cisd_hilbert_space(h.norb, hf);
+  REQUIRE(dets.size() == 12636);
+  auto H = macis::make_csr_hamiltonian_block<int32_t>(dets.begin(), dets.end(), dets.begin(),
+                                                      dets.end(), gen, 1e-16);
+  std::vector<int32_t> ref(H.rowptr().size());
+  std::ifstream f(dir + "/h2o.ccpvdz.cisd.rowptr.bin", std::ios::binary);
+  f.read((char*)ref.data(), ref.size() * sizeof(int32_t));
+  REQUIRE(H.rowptr() == ref);
+  REQUIRE(H.nnz() == 3517816);
+  REQUIRE(H.indexing() == 0);
+  auto H64 = macis::make_csr_hamiltonian<int64_t>(dets.begin(), dets.end(), gen, 1e-16);
+  REQUIRE(H64.nnz() == H.nnz());
+  bool same = true;
+  for(int64_t i = 0; i < H.nnz(); ++i)
+    same = same && H64.colind()[i] == H.colind()[i] && H64.nzval()[i] == H.nzval()[i];
+  REQUIRE(same);
+  // empty range -> empty matrix with indexing 0 (csr_hamiltonian.hpp:30-35)
+  auto E = macis::make_csr_hamiltonian_block<int32_t>(dets.begin(), dets.begin(), dets.begin(),
+                                                      dets.end(), gen, 1e-16);
+  REQUIRE(E.m() == 0 && E.n() == 12636 && E.nnz() == 0);
+}
+
+static void test_davidson(const H2O& h) {
+  // davidson.cxx:19-70
+  auto T = h.T;
+  auto V = h.V;
+  macis::HipHamiltonianGenerator<64> gen(macis::matrix_span<double>(T.data(), h.norb, h.norb),
+                                         macis::rank4_span<double>(V.data(), h.norb, h.norb,
+                                                                   h.norb, h.norb));
+  auto dets = macis::generate_cisd_hilbert_space(h.norb, macis::canonical_hf_determinant<64>(5, 5));
+  auto H = macis::make_csr_hamiltonian<int32_t>(dets.begin(), dets.end(), gen, 1e-16);
+  std::vector<double> X(H.n());
+  macis::diagonal_guess(H.n(), H, X.data());
+  auto D = sparsexx::extract_diagonal_elements(H);
+  auto [niter, E0] = macis::davidson(H.n(), 15, macis::SparseMatrixOperator(H), D.data(), 1e-8,
+                                     X.data());
+  REQUIRE(niter > 0);
+  REQUIRE_NEAR(E0 + h.E_core, -76.23197835987, 1e-9);
+  double nrm = 0.;
+  for(auto x : X) nrm += x * x;
+  REQUIRE_NEAR(std::sqrt(nrm), 1.0, 1e-12);
+  std::vector<double> AX(X.size());
+  sparsexx::spblas::gespmbv(1, 1., H, X.data(), H.n(), 0., AX.data(), H.n());
+  double xax = 0.;
+  for(size_t i = 0; i < X.size(); ++i) xax += X[i] * AX[i];
+  REQUIRE_NEAR(xax, E0, 1e-9);
+  // selected_ci_diag on the device path gives the same energy
+  std::vector<double> C;
+  auto E1 = macis::selected_ci_diag<64, int32_t>(dets.begin(), dets.end(), gen, 1e-16, 15, 1e-8, C);
+  REQUIRE_NEAR(E1, E0, 1e-10);
+  // non-convergence is reported with the reference's message
+  std::vector<double> Y(H.n(), 0.);
+  macis::diagonal_guess(H.n(), H, Y.data());
+  std::string msg;
+  try {
+    macis::davidson(H.n(), 3, macis::SparseMatrixOperator(H), D.data(), 1e-8, Y.data());
+  } catch(const std::runtime_error& e) {
+    msg = e.what();
+  }
+  REQUIRE(msg == "Davidson Did Not Converge!");
+}
+
+static void test_spmbv() {
+  // src/sparsexx/tests/test_spmbv.cxx:14-53
+  for(int indexing : {0, 1}) {
+    sparsexx::csr_matrix<double, int32_t> A(6, 6, 16, indexing);
+    const int i = indexing;
+    A.rowptr() = {i, i + 2, i + 5, i + 8, i + 11, i + 14, i + 16};
+    A.colind() = {i + 0, i + 1, i + 0, i + 1, i + 2, i + 1, i + 2, i + 3,
+                  i + 2, i + 3, i + 4, i + 3, i + 4, i + 5, i + 4, i + 5};
+    A.nzval() = {2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2};
+    std::vector<double> V = {1, 2, 3, 4, 5, 6}, AV(6, 1.);
+    sparsexx::spblas::gespmbv(1, 1., A, V.data(), 6, 0., AV.data(), 6);
+    std::vector<double> ref = {4, 8, 12, 16, 20, 17};
+    REQUIRE(AV == ref);
+  }
+}
+
+static void test_asci(const H2O& h) {
+  // asci.cxx:209-276
+  auto T = h.T;
+  auto V = h.V;
+  macis::HipHamiltonianGenerator<64> gen(macis::matrix_span<double>(T.data(), h.norb, h.norb),
+                                         macis::rank4_span<double>(V.data(), h.norb, h.norb,
+                                                                   h.norb, h.norb));
+  macis::ASCISettings asci_settings;
+  macis::MCSCFSettings mcscf_settings;
+  std::vector<macis::wfn_t<64>> dets = {macis::canonical_hf_determinant<64>(5, 5)};
+  std::vector<double> C = {1.0};
+  double E0 = gen.matrix_element(dets[0], dets[0]);
+  asci_settings.ntdets_max = 10000;
+  std::tie(E0, dets, C) = macis::asci_grow(asci_settings, mcscf_settings, E0, std::move(dets),
+                                           std::move(C), gen, h.norb);
+  REQUIRE_NEAR(E0, -8.542926243842e+01, 1e-8);
+  REQUIRE(dets.size() == 10000);
+  REQUIRE(C.size() == 10000);
+  double n2 = 0.;
+  for(auto c : C) n2 += c * c;
+  REQUIRE_NEAR(n2, 1.0, 1e-10);
+  std::tie(E0, dets, C) = macis::asci_refine(asci_settings, mcscf_settings, E0, std::move(dets),
+                                             std::move(C), gen, h.norb);
+  REQUIRE_NEAR(E0, -8.542925964708e+01, 1e-8);
+  REQUIRE(dets.size() == 10000);
+}
+
+int main(int argc, char** argv) {
+  if(argc < 2) {
+    std::printf("usage: %s <golden dir> [--cpu-only]\n", argv[0]);
+    return 2;
+  }
+  const std::string dir = argv[1];
+  const bool cpu_only = argc > 2 && std::string(argv[2]) == "--cpu-only";
+  auto h = load_h2o(dir);
+  std::vector<std::pair<std::string, std::function<void()>>> cases = {
+      {"sd_operations", [] { test_sd_operations(); }},
+      {"host generator elements", [&] { test_host_generator_elements(h); }},
+  };
+  if(!cpu_only) {
+    cases.push_back({"CSR Hamiltonian (device)", [&] { test_csr_hamiltonian(h, dir); }});
+    cases.push_back({"spmbv (device)", [] { test_spmbv(); }});
+    cases.push_back({"Davidson (device)", [&] { test_davidson(h); }});
+    cases.push_back({"ASCI grow/refine (device)", [&] { test_asci(h); }});
+  }
+  for(auto& [name, fn] : cases) {
+    std::printf("[case] %s\n", name.c_str());
+    std::fflush(stdout);
+    const int before = g_fail;
+    try {
+      fn();
+    } catch(const std::exception& e) {
+      ++g_fail;
+      std::printf("  FAILED with exception: %s\n", e.what());
+    }
+    std::printf("  %s\n", g_fail == before ? "ok" : "FAILED");
+  }
+  std::printf("%d checks, %d failures\n", g_checks, g_fail);
+  return g_fail ? 1 : 0;
+}

@@ -1,0 +1,564 @@
+// K6-K8: ASCI search on the device (asci_search, world_size == 1,
+// determinant_search.hpp:394-663; candidate kernels of
+// determinant_contributions.hpp:25-160).
+//
+//   prep    : per core determinant, occupied/virtual lists, the orbital
+//             energies single_orbital_ens (fast_diagonals.hpp:31-51) and the
+//             diagonal element.
+//   count   : every core determinant owns a flat candidate index space
+//             [alpha singles | beta singles | aa doubles | bb doubles |
+//             ab doubles]; one wavefront walks a 4096-candidate chunk, applies
+//             the reference's screens (|h| < h_el_tol for singles,
+//             |G| / |V| < h_el_tol for doubles) and counts survivors with a
+//             ballot.
+//   fill    : the chunk offsets (device scan) give every survivor a fixed
+//             slot; the same walk writes (determinant key, rv) with
+//             rv = c * sign * h / (E0 - H_kk^fast), bit-identical to the
+//             reference's expression.
+//   sort    : radix sort by the determinant key (bitset_less order: beta
+//             word most significant), reduce-by-key sums duplicates
+//             (sort_and_accumulate_asci_pairs, determinant_sort.hpp:39-83).
+//   top-k   : drops core determinants and zero scores (:528-544), then a
+//             stable descending radix sort on |rv| keeps the
+//             ndets_max - ncdets largest (:563-629); ties keep determinant
+//             order. The output is top-k followed by the core dets (:653).
+// The count/fill decomposition is fixed, so the result is run-to-run
+// deterministic. The reference's pair_size_max pruning (:120-137) is not
+// applied: candidates are all kept, which equals the reference whenever
+// its pruning does not trigger.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mhip_device.hpp"
+#include "mhip_internal.hpp"
+
+namespace mhip {
+IntsView device_view(const Integrals& I);
+
+namespace {
+
+constexpr int kChunk = 4096;  // candidates per wavefront chunk
+using u128 = rocprim::uint128_t;
+
+struct CoreArrays {
+  const uint64_t* __restrict__ sa;     // alpha strings [nc]
+  const uint64_t* __restrict__ sb;     // beta strings [nc]
+  const uint8_t* __restrict__ lists;   // [nc][4][64]: occ_a, vir_a, occ_b, vir_b
+  const int32_t* __restrict__ counts;  // [nc][4]: na, nva, nb, nvb
+  const double* __restrict__ eps;      // [nc][2][n]
+  const double* __restrict__ hdiag;    // [nc]
+  const double* __restrict__ coeff;    // [nc]
+  const int64_t* __restrict__ chunk_start;  // [nc+1]
+  int64_t nc;
+  int n;
+  int just_singles;
+};
+
+// ---- prep: thread per core det -------------------------------------------
+template <int W>
+__global__ void prep_kernel(IntsView I, const uint64_t* __restrict__ cdets, int64_t nc,
+                            int norb, uint64_t* __restrict__ sa, uint64_t* __restrict__ sb,
+                            uint8_t* __restrict__ lists, int32_t* __restrict__ counts,
+                            double* __restrict__ eps, double* __restrict__ hdiag) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= nc) return;
+  const Spins<W> s = load_spins<W>(cdets, c);
+  sa[c] = s.a;
+  sb[c] = s.b;
+  const int n = I.n;
+  for(int spin = 0; spin < 2; ++spin) {
+    const uint64_t st = spin ? s.b : s.a;
+    uint8_t* occ = lists + (c * 4 + 2 * spin) * 64;
+    uint8_t* vir = lists + (c * 4 + 2 * spin + 1) * 64;
+    int no = 0;
+    for(uint64_t m = st; m; m &= m - 1) occ[no++] = (uint8_t)ctz64(m);
+    // bitset_to_occ_vir: the first norb - nocc clear bits (sd_operations.hpp:82-97)
+    const int nv = norb - no;
+    uint64_t t = ~st;
+    for(int v = 0; v < nv; ++v) {
+      vir[v] = (uint8_t)ctz64(t);
+      t &= t - 1;
+    }
+    counts[c * 4 + 2 * spin] = no;
+    counts[c * 4 + 2 * spin + 1] = nv;
+    // single_orbital_ens(norb, ss_occ, os_occ)
+    const uint64_t ss = st, os = spin ? s.a : s.b;
+    double* e = eps + (c * 2 + spin) * n;
+    for(int i = 0; i < norb; ++i) {
+      double v = I.T[i * (n + 1)];
+      for(uint64_t m = ss; m; m &= m - 1) {
+        const int q = ctz64(m);
+        v += I.G2_red[i + n * q] + I.G2_red[q + n * i];
+      }
+      v -= I.G2_red[i * (n + 1)];
+      for(uint64_t m = os; m; m &= m - 1) v += I.V2_red[i + n * ctz64(m)];
+      e[i] = v;
+    }
+  }
+  hdiag[c] = me_diag(I, s.a, s.b);
+}
+
+// Pair (x < y) of the lexicographic enumeration of pairs of m items.
+__device__ __forceinline__ void tri_decode(int64_t p, int m, int& x, int& y) {
+  const double b = 2.0 * m - 1.0;
+  int xx = (int)floor((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
+  auto start = [&](int q) { return (int64_t)q * m - (int64_t)q * (q + 1) / 2; };
+  if(xx < 0) xx = 0;
+  while(xx > 0 && start(xx) > p) --xx;
+  while(xx + 1 < m && start(xx + 1) <= p) ++xx;
+  x = xx;
+  y = (int)(p - start(xx)) + xx + 1;
+}
+
+__device__ __forceinline__ int64_t pairs(int m) { return (int64_t)m * (m - 1) / 2; }
+
+struct Cand {
+  bool keep;
+  uint64_t a, b;  // excited alpha / beta strings
+  double rv;
+};
+
+// Evaluates flat candidate f of core det c. With FULL = false only the
+// screen is evaluated (count pass).
+template <bool FULL>
+__device__ __forceinline__ Cand candidate(const IntsView& I, const CoreArrays& A, int64_t c,
+                                          int64_t f, double E0, double tol) {
+  Cand r;
+  r.keep = false;
+  r.a = r.b = 0;
+  r.rv = 0.;
+  const int n = I.n;
+  const int na = A.counts[c * 4 + 0], nva = A.counts[c * 4 + 1];
+  const int nb = A.counts[c * 4 + 2], nvb = A.counts[c * 4 + 3];
+  const uint8_t* L = A.lists + c * 4 * 64;
+  const uint64_t sa = A.sa[c], sb = A.sb[c];
+  const double hd0 = A.hdiag[c];
+  const double coeff = A.coeff[c];
+  const double* eps_a = A.eps + (c * 2 + 0) * n;
+  const double* eps_b = A.eps + (c * 2 + 1) * n;
+  const int64_t Sa = (int64_t)na * nva, Sb = (int64_t)nb * nvb;
+  if(f < Sa + Sb) {
+    // singles (append_singles_asci_contributions, :25-66)
+    const int spin = f < Sa ? 0 : 1;
+    const int64_t g = spin ? f - Sa : f;
+    const int nv = spin ? nvb : nva;
+    const int i = L[(2 * spin) * 64 + g / nv], a = L[(2 * spin + 1) * 64 + g % nv];
+    const uint64_t same = spin ? sb : sa, othr = spin ? sa : sb;
+    double h = I.T[a + i * n];
+    const double* G_ov = I.G_red + a * n + (int64_t)i * n * n;
+    const double* V_ov = I.V_red + a * n + (int64_t)i * n * n;
+    for(uint64_t m = same; m; m &= m - 1) h += G_ov[ctz64(m)];
+    for(uint64_t m = othr; m; m &= m - 1) h += V_ov[ctz64(m)];
+    if(fabs(h) < tol) return r;
+    r.keep = true;
+    if constexpr(FULL) {
+      const uint64_t fl = (1ull << i) | (1ull << a);
+      r.a = spin ? sa : sa ^ fl;
+      r.b = spin ? sb ^ fl : sb;
+      h *= excitation_sign(same, a, i);
+      const double* eps = spin ? eps_b : eps_a;
+      const double hd = hd0 + eps[a] - eps[i] - I.G2_red[a + n * i] - I.G2_red[i + n * a];
+      h /= (E0 - hd);
+      r.rv = coeff * h;
+    }
+    return r;
+  }
+  if(A.just_singles) return r;
+  int64_t g = f - Sa - Sb;
+  const int64_t Da = pairs(na) * pairs(nva), Db = pairs(nb) * pairs(nvb);
+  if(g < Da + Db) {
+    // same-spin doubles (append_ss_doubles_asci_contributions, :68-120)
+    const int spin = g < Da ? 0 : 1;
+    if(spin) g -= Da;
+    const int no = spin ? nb : na, nv = spin ? nvb : nva;
+    const int64_t pv = pairs(nv);
+    int ii, jj, aa, bb;
+    tri_decode(g / pv, no, ii, jj);
+    tri_decode(g % pv, nv, aa, bb);
+    const int i = L[(2 * spin) * 64 + ii], j = L[(2 * spin) * 64 + jj];
+    const int a = L[(2 * spin + 1) * 64 + aa], b = L[(2 * spin + 1) * 64 + bb];
+    const double Gv = I.G[((int64_t)a + (int64_t)i * n) * n * n + b + (int64_t)j * n];
+    if(fabs(Gv) < tol) return r;
+    r.keep = true;
+    if constexpr(FULL) {
+      const uint64_t st = spin ? sb : sa;
+      const uint64_t fex = (1ull << i) | (1ull << j) | (1ull << a) | (1ull << b);
+      uint64_t ket = st ^ fex, ex = fex;
+      // doubles_sign(state, excited, ex) (sd_operations.hpp:528-551)
+      const uint32_t o1 = ctz64(ket & ex), v1 = ctz64(st & ex);
+      double sign = excitation_sign(ket, v1, o1);
+      const uint64_t f1 = (1ull << o1) | (1ull << v1);
+      ket ^= f1;
+      ex ^= f1;
+      const uint32_t o2 = ctz64(ket & ex), v2 = ctz64(st & ex);
+      sign *= excitation_sign(ket, v2, o2);
+      r.a = spin ? sa : sa ^ fex;
+      r.b = spin ? sb ^ fex : sb;
+      double h = sign * Gv;
+      const double* eps = spin ? eps_b : eps_a;
+      const double* G2 = I.G2_red;
+      // fast_diag_ss_double (fast_diagonals.hpp:72-85), left to right
+      const double hd = hd0 + eps[a] + eps[b] - eps[i] - eps[j] + G2[i + n * j] +
+                        G2[j + n * i] + G2[a + n * b] + G2[b + n * a] - G2[a + n * i] -
+                        G2[i + n * a] - G2[b + n * i] - G2[i + n * b] - G2[a + n * j] -
+                        G2[j + n * a] - G2[b + n * j] - G2[j + n * b];
+      h /= (E0 - hd);
+      r.rv = coeff * h;
+    }
+    return r;
+  }
+  // opposite-spin doubles (append_os_doubles_asci_contributions, :122-160)
+  g -= Da + Db;
+  const int64_t sbj = (int64_t)nb * nvb;
+  const int64_t ia = g / sbj, jb = g % sbj;
+  const int i = L[0 * 64 + ia / nva], a = L[1 * 64 + ia % nva];
+  const int j = L[2 * 64 + jb / nvb], b = L[3 * 64 + jb % nvb];
+  const double Vv = I.V[a + (int64_t)i * n + ((int64_t)b + (int64_t)j * n) * n * n];
+  if(fabs(Vv) < tol) return r;
+  r.keep = true;
+  if constexpr(FULL) {
+    const double sgn_a = excitation_sign(sa, a, i);
+    const double sgn_b = excitation_sign(sb, b, j);
+    const double sign = sgn_a * sgn_b;
+    r.a = sa ^ ((1ull << a) | (1ull << i));
+    r.b = sb ^ ((1ull << j) | (1ull << b));
+    double h = sign * Vv;
+    const double* G2 = I.G2_red;
+    const double* V2 = I.V2_red;
+    // fast_diag_os_double (fast_diagonals.hpp:101-112), left to right
+    const double hd = hd0 + eps_a[a] + eps_b[b] - eps_a[i] - eps_b[j] + V2[i + n * j] +
+                      V2[a + n * b] - G2[a + n * i] - G2[i + n * a] - G2[b + n * j] -
+                      G2[j + n * b] - V2[a + n * j] - V2[i + n * b];
+    h /= (E0 - hd);
+    r.rv = coeff * h;
+  }
+  return r;
+}
+
+__device__ __forceinline__ int64_t find_core(const int64_t* __restrict__ start, int64_t nc,
+                                             int64_t chunk) {
+  int64_t lo = 0, hi = nc;  // last c with start[c] <= chunk
+  while(hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if(start[mid] <= chunk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int W, bool FILL>
+__global__ __launch_bounds__(256) void contrib_kernel(
+    IntsView I, CoreArrays A, const int64_t* __restrict__ ncand, int64_t nchunks, double E0,
+    double tol, int64_t* __restrict__ chunk_counts, const int64_t* __restrict__ chunk_off,
+    typename std::conditional<W == 1, uint64_t, u128>::type* __restrict__ keys,
+    double* __restrict__ vals) {
+  const int64_t chunk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if(chunk >= nchunks) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t c = find_core(A.chunk_start, A.nc, chunk);
+  const int64_t f0 = (chunk - A.chunk_start[c]) * kChunk;
+  const int64_t f1 = min(f0 + (int64_t)kChunk, ncand[c]);
+  int64_t out = FILL ? chunk_off[chunk] : 0;
+  int64_t cnt = 0;
+  for(int64_t fb = f0; fb < f1; fb += 64) {
+    const int64_t f = fb + lane;
+    Cand cd;
+    cd.keep = false;
+    if(f < f1) cd = candidate<FILL>(I, A, c, f, E0, tol);
+    const uint64_t m = ballot(cd.keep);
+    if constexpr(FILL) {
+      if(cd.keep) {
+        const int64_t p = out + lanes_below(m);
+        if constexpr(W == 1)
+          keys[p] = cd.a | (cd.b << 32);
+        else
+          keys[p] = ((u128)cd.b << 64) | (u128)cd.a;
+        vals[p] = cd.rv;
+      }
+      out += popc64(m);
+    } else {
+      cnt += popc64(m);
+    }
+  }
+  if constexpr(!FILL)
+    if(lane == 0) chunk_counts[chunk] = cnt;
+}
+
+// keep = not a core det and |rv| > 0 (the -|rv| < 0 partition, :528-544);
+// absval receives |rv| for the top-k key.
+template <typename K>
+__global__ void flag_kernel(const K* __restrict__ ukeys, const double* __restrict__ usum,
+                            const int64_t* __restrict__ nu_dev, const K* __restrict__ core,
+                            int64_t nc, uint8_t* __restrict__ flags,
+                            uint64_t* __restrict__ absbits) {
+  const int64_t nu = *nu_dev;
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
+      i += (int64_t)gridDim.x * blockDim.x) {
+    const K k = ukeys[i];
+    int64_t lo = 0, hi = nc;
+    while(lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if(core[mid] < k)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const bool in_core = lo < nc && core[lo] == k;
+    const double v = fabs(usum[i]);
+    flags[i] = (!in_core && -v < 0.0) ? 1 : 0;
+    absbits[i] = __double_as_longlong(v);
+  }
+}
+
+template <int W, typename K>
+__global__ void keys_to_dets_kernel(const K* __restrict__ keys, int64_t n,
+                                    uint64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  if constexpr(W == 1)
+    out[i] = (uint64_t)keys[i];
+  else {
+    out[2 * i] = (uint64_t)keys[i];
+    out[2 * i + 1] = (uint64_t)(keys[i] >> 64);
+  }
+}
+
+template <int W, typename K>
+__global__ void dets_to_keys_kernel(const uint64_t* __restrict__ d, int64_t n,
+                                    K* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  if constexpr(W == 1)
+    keys[i] = d[i];
+  else
+    keys[i] = ((K)d[2 * i + 1] << 64) | (K)d[2 * i];
+}
+
+inline int64_t host_pairs(int64_t m) { return m * (m - 1) / 2; }
+
+template <int W>
+int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndets_max,
+                    const uint64_t* d_cdets, int64_t nc, double E0, const double* d_C,
+                    int64_t norb, uint64_t* d_out) {
+  using K = typename std::conditional<W == 1, uint64_t, u128>::type;
+  hipStream_t st = ctx->stream;
+  mhip_timings& tm = ctx->timings;
+  Timer t_all(st), t(st);
+  t_all.start();
+  t.start();
+  const IntsView I = device_view(ctx->ints);
+  const int n = (int)ctx->ints.norb;
+
+  // candidate counts per core det (closed form of the loop bounds)
+  std::vector<uint64_t> hc(nc * W);
+  MHIP_HIP_CHECK(hipMemcpyAsync(hc.data(), d_cdets, 8 * W * nc, hipMemcpyDeviceToHost, st));
+  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<int64_t> ncand(nc), cstart(nc + 1, 0);
+  for(int64_t c = 0; c < nc; ++c) {
+    uint64_t a, b;
+    if(W == 1) {
+      a = hc[c] & 0xffffffffull;
+      b = hc[c] >> 32;
+    } else {
+      a = hc[2 * c];
+      b = hc[2 * c + 1];
+    }
+    MHIP_REQUIRE(!((a | b) & (norb >= 64 ? 0ull : ~((1ull << norb) - 1ull))),
+                 MHIP_ERR_INVALID_ARG, "core determinant occupies an orbital >= norb");
+    const int64_t na = __builtin_popcountll(a), nb = __builtin_popcountll(b);
+    const int64_t nva = norb - na, nvb = norb - nb;
+    int64_t k = na * nva + nb * nvb;
+    if(!s->just_singles)
+      k += host_pairs(na) * host_pairs(nva) + host_pairs(nb) * host_pairs(nvb) +
+           na * nva * nb * nvb;
+    ncand[c] = k;
+    cstart[c + 1] = cstart[c] + (k + kChunk - 1) / kChunk;
+  }
+  const int64_t nchunks = cstart[nc];
+
+  DevBuf sa(8 * nc), sb(8 * nc), lists(4 * 64 * nc), counts(16 * nc), eps(16 * n * nc),
+      hdiag(8 * nc), dn(8 * nc), dstart(8 * (nc + 1));
+  MHIP_HIP_CHECK(hipMemcpyAsync(dn.p, ncand.data(), 8 * nc, hipMemcpyHostToDevice, st));
+  MHIP_HIP_CHECK(hipMemcpyAsync(dstart.p, cstart.data(), 8 * (nc + 1), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(prep_kernel<W>, dim3((nc + 63) / 64), dim3(64), 0, st, I, d_cdets, nc,
+                     (int)norb, sa.as<uint64_t>(), sb.as<uint64_t>(), lists.as<uint8_t>(),
+                     counts.as<int32_t>(), eps.as<double>(), hdiag.as<double>());
+  CoreArrays A;
+  A.sa = sa.as<uint64_t>();
+  A.sb = sb.as<uint64_t>();
+  A.lists = lists.as<uint8_t>();
+  A.counts = counts.as<int32_t>();
+  A.eps = eps.as<double>();
+  A.hdiag = hdiag.as<double>();
+  A.coeff = d_C;
+  A.chunk_start = dstart.as<int64_t>();
+  A.nc = nc;
+  A.n = n;
+  A.just_singles = s->just_singles;
+
+  DevBuf ccount(8 * (nchunks + 1)), coff(8 * (nchunks + 1));
+  MHIP_HIP_CHECK(hipMemsetAsync(ccount.p, 0, 8 * (nchunks + 1), st));
+  const int64_t grid = (nchunks + 3) / 4;
+  MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many candidates");
+  hipLaunchKernelGGL((contrib_kernel<W, false>), dim3(grid), dim3(256), 0, st, I, A,
+                     dn.as<int64_t>(), nchunks, E0, s->h_el_tol, ccount.as<int64_t>(), nullptr,
+                     nullptr, nullptr);
+  MHIP_HIP_CHECK(hipGetLastError());
+  size_t tmp = 0;
+  MHIP_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp, ccount.as<int64_t>(), coff.as<int64_t>(),
+                                         (int64_t)0, (size_t)(nchunks + 1),
+                                         rocprim::plus<int64_t>(), st));
+  ctx->scratch_c.reserve(tmp + 16);
+  MHIP_HIP_CHECK(rocprim::exclusive_scan(ctx->scratch_c.p, tmp, ccount.as<int64_t>(),
+                                         coff.as<int64_t>(), (int64_t)0, (size_t)(nchunks + 1),
+                                         rocprim::plus<int64_t>(), st));
+  int64_t P = 0;
+  MHIP_HIP_CHECK(hipMemcpyAsync(&P, coff.as<int64_t>() + nchunks, 8, hipMemcpyDeviceToHost, st));
+  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  DevBuf keys(sizeof(K) * std::max<int64_t>(P, 1)), vals(8 * std::max<int64_t>(P, 1));
+  hipLaunchKernelGGL((contrib_kernel<W, true>), dim3(grid), dim3(256), 0, st, I, A,
+                     dn.as<int64_t>(), nchunks, E0, s->h_el_tol, nullptr, coff.as<int64_t>(),
+                     keys.as<K>(), vals.as<double>());
+  MHIP_HIP_CHECK(hipGetLastError());
+  t.stop();
+  tm.asci_contrib_ms = t.ms();
+  tm.asci_npairs = P;
+
+  // ---- sort + accumulate
+  t.start();
+  DevBuf keys2(sizeof(K) * std::max<int64_t>(P, 1)), vals2(8 * std::max<int64_t>(P, 1));
+  DevBuf ukeys(sizeof(K) * std::max<int64_t>(P, 1)), usum(8 * std::max<int64_t>(P, 1));
+  DevBuf nu_dev(16);
+  MHIP_HIP_CHECK(hipMemsetAsync(nu_dev.p, 0, 16, st));
+  int64_t nu = 0;
+  if(P) {
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys.as<K>(), keys2.as<K>(),
+                                             vals.as<double>(), vals2.as<double>(), (size_t)P,
+                                             0, 32 * W * 2, st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, keys.as<K>(), keys2.as<K>(),
+                                             vals.as<double>(), vals2.as<double>(), (size_t)P,
+                                             0, 32 * W * 2, st));
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::reduce_by_key(nullptr, tmp, keys2.as<K>(), vals2.as<double>(),
+                                          (size_t)P, ukeys.as<K>(), usum.as<double>(),
+                                          nu_dev.as<int64_t>(), rocprim::plus<double>(),
+                                          rocprim::equal_to<K>(), st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::reduce_by_key(ctx->scratch_c.p, tmp, keys2.as<K>(),
+                                          vals2.as<double>(), (size_t)P, ukeys.as<K>(),
+                                          usum.as<double>(), nu_dev.as<int64_t>(),
+                                          rocprim::plus<double>(), rocprim::equal_to<K>(), st));
+    MHIP_HIP_CHECK(hipMemcpyAsync(&nu, nu_dev.p, 8, hipMemcpyDeviceToHost, st));
+  }
+  t.stop();
+  tm.asci_sort_ms = t.ms();
+  tm.asci_nunique = nu;
+  keys.release();
+  vals.release();
+
+  // ---- exclude core dets and zero scores, top-k
+  t.start();
+  DevBuf core_keys(sizeof(K) * std::max<int64_t>(nc, 1)), core_sorted(sizeof(K) * std::max<int64_t>(nc, 1));
+  hipLaunchKernelGGL((dets_to_keys_kernel<W, K>), dim3((nc + 255) / 256), dim3(256), 0, st,
+                     d_cdets, nc, core_keys.as<K>());
+  tmp = 0;
+  MHIP_HIP_CHECK(rocprim::radix_sort_keys(nullptr, tmp, core_keys.as<K>(), core_sorted.as<K>(),
+                                          (size_t)nc, 0, 32 * W * 2, st));
+  ctx->scratch_c.reserve(tmp + 16);
+  MHIP_HIP_CHECK(rocprim::radix_sort_keys(ctx->scratch_c.p, tmp, core_keys.as<K>(),
+                                          core_sorted.as<K>(), (size_t)nc, 0, 32 * W * 2, st));
+  const int64_t top_k = ndets_max - nc;
+  int64_t nkeep = 0;
+  DevBuf flags(std::max<int64_t>(nu, 1)), absbits(8 * std::max<int64_t>(nu, 1));
+  DevBuf kkeys(sizeof(K) * std::max<int64_t>(nu, 1)), kabs(8 * std::max<int64_t>(nu, 1));
+  if(nu) {
+    hipLaunchKernelGGL((flag_kernel<K>), dim3(std::min<int64_t>(4096, (nu + 255) / 256)),
+                       dim3(256), 0, st, ukeys.as<K>(), usum.as<double>(), nu_dev.as<int64_t>(),
+                       core_sorted.as<K>(), nc, flags.as<uint8_t>(), absbits.as<uint64_t>());
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::select(nullptr, tmp, ukeys.as<K>(), flags.as<uint8_t>(), kkeys.as<K>(),
+                                   nu_dev.as<int64_t>() + 1, (size_t)nu, st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, ukeys.as<K>(), flags.as<uint8_t>(),
+                                   kkeys.as<K>(), nu_dev.as<int64_t>() + 1, (size_t)nu, st));
+    MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, absbits.as<uint64_t>(),
+                                   flags.as<uint8_t>(), kabs.as<uint64_t>(),
+                                   nu_dev.as<int64_t>() + 1, (size_t)nu, st));
+    MHIP_HIP_CHECK(hipMemcpyAsync(&nkeep, nu_dev.as<int64_t>() + 1, 8, hipMemcpyDeviceToHost, st));
+    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  int64_t nout = nkeep;
+  const K* outkeys = kkeys.as<K>();
+  DevBuf skeys, sabs;
+  if(nkeep > top_k) {
+    // stable descending sort on |rv| bits: ties keep ascending det order
+    skeys.alloc(sizeof(K) * nkeep);
+    sabs.alloc(8 * nkeep);
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, kabs.as<uint64_t>(),
+                                                  sabs.as<uint64_t>(), kkeys.as<K>(),
+                                                  skeys.as<K>(), (size_t)nkeep, 0, 64, st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, kabs.as<uint64_t>(),
+                                                  sabs.as<uint64_t>(), kkeys.as<K>(),
+                                                  skeys.as<K>(), (size_t)nkeep, 0, 64, st));
+    outkeys = skeys.as<K>();
+    nout = std::max<int64_t>(top_k, 0);
+  }
+  if(nout)
+    hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nout + 255) / 256), dim3(256), 0, st,
+                       outkeys, nout, d_out);
+  if(nc)
+    MHIP_HIP_CHECK(hipMemcpyAsync(d_out + W * nout, d_cdets, 8 * W * nc,
+                                  hipMemcpyDeviceToDevice, st));
+  MHIP_HIP_CHECK(hipGetLastError());
+  t.stop();
+  tm.asci_topk_ms = t.ms();
+  t_all.stop();
+  tm.asci_total_ms = t_all.ms();
+  return nout + nc;
+}
+
+}  // namespace
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" int mhip_asci_search(mhip_context* ctx, int words_per_det,
+                                const mhip_asci_settings* settings,
+                                int64_t ndets_max, const uint64_t* d_cdets,
+                                int64_t ncdets, double E_ASCI, const double* d_C,
+                                int64_t norb, uint64_t* d_new_dets,
+                                int64_t* n_new) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(settings && n_new, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_REQUIRE(words_per_det == 1 || words_per_det == 2, MHIP_ERR_UNSUPPORTED,
+                 "words_per_det must be 1 or 2");
+    MHIP_REQUIRE(ctx->ints.norb > 0, MHIP_ERR_NO_INTEGRALS, "no integrals set");
+    MHIP_REQUIRE(norb > 0 && norb <= ctx->ints.norb && norb <= 32 * words_per_det,
+                 MHIP_ERR_INVALID_ARG, "bad norb");
+    MHIP_REQUIRE(ncdets >= 0 && ndets_max >= ncdets, MHIP_ERR_INVALID_ARG,
+                 "ndets_max must be >= ncdets");
+    MHIP_REQUIRE(ncdets == 0 || (d_cdets && d_C && d_new_dets), MHIP_ERR_INVALID_ARG,
+                 "null pointer");
+    *n_new = 0;
+    if(!ncdets) return;
+    if(words_per_det == 1)
+      *n_new = search_impl<1>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
+                              d_new_dets);
+    else
+      *n_new = search_impl<2>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
+                              d_new_dets);
+  });
+}

@@ -1,0 +1,219 @@
+// Context, device memory helpers, integral upload and error reporting of
+// the C-ABI (include/macis_hip.h).
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mhip_internal.hpp"
+
+namespace mhip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void use_device(mhip_context* ctx) {
+  MHIP_REQUIRE(ctx, MHIP_ERR_INVALID_ARG, "null context");
+  MHIP_HIP_CHECK(hipSetDevice(ctx->device));
+}
+
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" {
+
+int mhip_get_version(void) { return MHIP_VERSION; }
+
+const char* mhip_error_string(int code) {
+  switch(code) {
+    case MHIP_SUCCESS: return "success";
+    case MHIP_ERR_INVALID_ARG: return "invalid argument";
+    case MHIP_ERR_HIP: return "HIP runtime error";
+    case MHIP_ERR_OUT_OF_MEMORY: return "out of memory";
+    case MHIP_ERR_NO_INTEGRALS: return "no integrals set on the context";
+    case MHIP_ERR_NOT_CONVERGED: return "Davidson Did Not Converge!";
+    case MHIP_ERR_NO_GUESS: return "Davidson: No Guess Provided";
+    case MHIP_ERR_UNSUPPORTED: return "Unsupported index_t";
+    case MHIP_ERR_INDEX_OVERFLOW: return "index type overflow";
+    case MHIP_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown error";
+  }
+}
+
+const char* mhip_last_error(void) { return g_last_error.c_str(); }
+
+int mhip_context_create(int device, mhip_context** out) {
+  return guarded([&] {
+    MHIP_REQUIRE(out, MHIP_ERR_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      throw Error(MHIP_ERR_NO_DEVICE, "no HIP device visible");
+    MHIP_REQUIRE(device >= 0 && device < ndev, MHIP_ERR_INVALID_ARG,
+                 "device index out of range");
+    auto* ctx = new mhip_context();
+    ctx->device = device;
+    try {
+      MHIP_HIP_CHECK(hipSetDevice(device));
+      // Default priority stream (non-blocking with respect to stream 0).
+      MHIP_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    } catch(...) {
+      delete ctx;
+      throw;
+    }
+    *out = ctx;
+  });
+}
+
+int mhip_context_destroy(mhip_context* ctx) {
+  return guarded([&] {
+    if(!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if(ctx->stream) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+  });
+}
+
+int mhip_context_get_stream(mhip_context* ctx, void** stream) {
+  return guarded([&] {
+    MHIP_REQUIRE(ctx && stream, MHIP_ERR_INVALID_ARG, "null argument");
+    *stream = (void*)ctx->stream;
+  });
+}
+
+int mhip_synchronize(mhip_context* ctx) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mhip_get_timings(mhip_context* ctx, mhip_timings* out) {
+  return guarded([&] {
+    MHIP_REQUIRE(ctx && out, MHIP_ERR_INVALID_ARG, "null argument");
+    *out = ctx->timings;
+  });
+}
+
+int mhip_malloc(mhip_context* ctx, size_t bytes, void** dptr) {
+  return guarded([&] {
+    MHIP_REQUIRE(dptr, MHIP_ERR_INVALID_ARG, "null output pointer");
+    use_device(ctx);
+    *dptr = nullptr;
+    if(bytes) MHIP_HIP_CHECK(hipMalloc(dptr, bytes));
+  });
+}
+
+int mhip_free(mhip_context* ctx, void* dptr) {
+  return guarded([&] {
+    use_device(ctx);
+    if(dptr) {
+      MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      MHIP_HIP_CHECK(hipFree(dptr));
+    }
+  });
+}
+
+int mhip_memcpy_h2d(mhip_context* ctx, void* dst, const void* src,
+                    size_t bytes) {
+  return guarded([&] {
+    use_device(ctx);
+    if(!bytes) return;
+    MHIP_REQUIRE(dst && src, MHIP_ERR_INVALID_ARG, "null pointer");
+    MHIP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mhip_memcpy_d2h(mhip_context* ctx, void* dst, const void* src,
+                    size_t bytes) {
+  return guarded([&] {
+    use_device(ctx);
+    if(!bytes) return;
+    MHIP_REQUIRE(dst && src, MHIP_ERR_INVALID_ARG, "null pointer");
+    MHIP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mhip_memcpy_d2d(mhip_context* ctx, void* dst, const void* src,
+                    size_t bytes) {
+  return guarded([&] {
+    use_device(ctx);
+    if(!bytes) return;
+    MHIP_REQUIRE(dst && src, MHIP_ERR_INVALID_ARG, "null pointer");
+    MHIP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
+
+int mhip_memset(mhip_context* ctx, void* dst, int value, size_t bytes) {
+  return guarded([&] {
+    use_device(ctx);
+    if(!bytes) return;
+    MHIP_REQUIRE(dst, MHIP_ERR_INVALID_ARG, "null pointer");
+    MHIP_HIP_CHECK(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  });
+}
+
+int mhip_set_integrals(mhip_context* ctx, int64_t norb, const double* T,
+                       const double* V) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(T && V, MHIP_ERR_INVALID_ARG, "null integral pointer");
+    MHIP_REQUIRE(norb > 0 && norb <= 64, MHIP_ERR_INVALID_ARG,
+                 "norb must be in [1, 64]");
+    const size_t n = (size_t)norb, n2 = n * n, n3 = n2 * n, n4 = n3 * n;
+    // generate_integral_intermediates (impl.hpp:25-70). Only subtraction
+    // and exact scaling by 0.5 are involved, so the tables equal the
+    // reference's bit for bit.
+    std::vector<double> G(n4), G_red(n3), V_red(n3), G2(n2), V2(n2);
+    auto idx = [&](size_t i, size_t j, size_t k, size_t l) {
+      return i + n * j + n2 * k + n3 * l;
+    };
+    for(size_t l = 0; l < n; ++l)
+      for(size_t k = 0; k < n; ++k)
+        for(size_t j = 0; j < n; ++j)
+          for(size_t i = 0; i < n; ++i)
+            G[idx(i, j, k, l)] = V[idx(i, j, k, l)] - V[idx(i, l, k, j)];
+    for(size_t j = 0; j < n; ++j)
+      for(size_t i = 0; i < n; ++i)
+        for(size_t k = 0; k < n; ++k) {
+          G_red[k + n * i + n2 * j] = G[idx(k, k, i, j)];
+          V_red[k + n * i + n2 * j] = V[idx(k, k, i, j)];
+        }
+    for(size_t j = 0; j < n; ++j)
+      for(size_t i = 0; i < n; ++i) {
+        G2[i + n * j] = 0.5 * G[idx(i, i, j, j)];
+        V2[i + n * j] = V[idx(i, i, j, j)];
+      }
+    Integrals& I = ctx->ints;
+    I.norb = 0;
+    I.T.alloc(n2 * 8);
+    I.V.alloc(n4 * 8);
+    I.G.alloc(n4 * 8);
+    I.G_red.alloc(n3 * 8);
+    I.V_red.alloc(n3 * 8);
+    I.G2_red.alloc(n2 * 8);
+    I.V2_red.alloc(n2 * 8);
+    auto up = [&](DevBuf& d, const double* h, size_t cnt) {
+      MHIP_HIP_CHECK(hipMemcpyAsync(d.p, h, cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+    };
+    up(I.T, T, n2);
+    up(I.V, V, n4);
+    up(I.G, G.data(), n4);
+    up(I.G_red, G_red.data(), n3);
+    up(I.V_red, V_red.data(), n3);
+    up(I.G2_red, G2.data(), n2);
+    up(I.V2_red, V2.data(), n2);
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    I.norb = norb;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,572 @@
+// K3-K5: device-resident Davidson (davidson.hpp:61-225) and the serial
+// selected_ci_diag driver (selected_ci_diag.hpp:75-207).
+//
+// Per iteration everything N-long stays in HBM:
+//   SpMV AV(:,i) = A V(:,i)                                      (K2)
+//   Rayleigh-Ritz C = V^T AV (k x k) as a tall-skinny FP64 GEMM on MFMA
+//     (v_mfma_f64_16x16x4f64), recomputed in full as rayleigh_ritz.hpp:73
+//     does; the k x k (k <= 21) eigenproblem runs on the host (Jacobi).
+//   Residual X = V c, R = AV c - lam X, ||R||^2 in one fused pass.
+//   Preconditioner R = -R / (D - lam), CGS2 projections (two dot/update
+//     passes) and normalisation (davidson.hpp:111-126, 212-217).
+// Reductions are written as per-block partials and summed in a fixed order,
+// so results are run-to-run deterministic.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "mhip_device.hpp"
+#include "mhip_internal.hpp"
+
+namespace mhip {
+namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxK = 32;  // columns handled by the projection kernels
+constexpr int kRedBlocks = 512;
+
+int red_blocks(int64_t N) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (N + 255) / 256));
+}
+
+// ---- tall-skinny GEMM on MFMA: partial C = A^T B per block --------------
+// A: N x k (lda), B: N x m (ldb), k, m <= 16*TI, 16*TJ. Lane l owns output
+// column l%16 and reduction slot l/16; each lane feeds 8 consecutive rows
+// per step, one MFMA (K = 4) per row offset s: slot g = l/16 carries row
+// n0 + 8g + s in both operands, so the pairing of A and B rows is exact.
+template <int TI, int TJ>
+__global__ __launch_bounds__(256) void gram_mfma_kernel(
+    int64_t N, int k, const double* __restrict__ A, int64_t lda, int m,
+    const double* __restrict__ B, int64_t ldb, double* __restrict__ partials) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + wv, nw = (int64_t)gridDim.x * 4;
+  const int col = lane & 15, grp = lane >> 4;
+  d4_t acc[TI][TJ];
+#pragma unroll
+  for(int i = 0; i < TI; ++i)
+#pragma unroll
+    for(int j = 0; j < TJ; ++j) acc[i][j] = d4_t{0., 0., 0., 0.};
+  for(int64_t n0 = gw * 32; n0 < N; n0 += nw * 32) {
+    const int64_t nb = n0 + grp * 8;
+    double a[TI][8], b[TJ][8];
+#pragma unroll
+    for(int i = 0; i < TI; ++i) {
+      const int c = i * 16 + col;
+      const double* p = A + (int64_t)c * lda;
+#pragma unroll
+      for(int s = 0; s < 8; ++s) a[i][s] = (c < k && nb + s < N) ? p[nb + s] : 0.;
+    }
+#pragma unroll
+    for(int j = 0; j < TJ; ++j) {
+      const int c = j * 16 + col;
+      const double* p = B + (int64_t)c * ldb;
+#pragma unroll
+      for(int s = 0; s < 8; ++s) b[j][s] = (c < m && nb + s < N) ? p[nb + s] : 0.;
+    }
+#pragma unroll
+    for(int s = 0; s < 8; ++s)
+#pragma unroll
+      for(int i = 0; i < TI; ++i)
+#pragma unroll
+        for(int j = 0; j < TJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+  }
+  __shared__ double red[4][TI * TJ * 256];
+#pragma unroll
+  for(int i = 0; i < TI; ++i)
+#pragma unroll
+    for(int j = 0; j < TJ; ++j)
+#pragma unroll
+      for(int r = 0; r < 4; ++r) red[wv][(i * TJ + j) * 256 + r * 64 + lane] = acc[i][j][r];
+  __syncthreads();
+  for(int e = threadIdx.x; e < TI * TJ * 256; e += 256)
+    partials[(int64_t)blockIdx.x * (TI * TJ * 256) + e] =
+        (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+}
+
+// Sum the gram partials over blocks and scatter to column-major C (ldc).
+// Output layout of v_mfma_f64_16x16x4f64: lane l, register r holds
+// D(4*(l/16) + r, l%16).
+template <int TI, int TJ>
+__global__ void gram_finalize_kernel(const double* __restrict__ partials,
+                                     int nblocks, int k, int m,
+                                     double* __restrict__ C, int ldc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= TI * TJ * 256) return;
+  const int tile = e / 256, rem = e % 256, r = rem / 64, lane = rem % 64;
+  const int ti = tile / TJ, tj = tile % TJ;
+  const int i = ti * 16 + 4 * (lane / 16) + r, j = tj * 16 + lane % 16;
+  if(i >= k || j >= m) return;
+  double s = 0.;
+  for(int b = 0; b < nblocks; ++b) s += partials[(int64_t)b * (TI * TJ * 256) + e];
+  C[i + (int64_t)j * ldc] = s;
+}
+
+// inner[j] partials = V(:,j) . w for j < k (vector FMA; a GEMV is HBM bound).
+__global__ __launch_bounds__(256) void dots_kernel(int64_t N, int k,
+                                                    const double* __restrict__ V,
+                                                    int64_t ldv,
+                                                    const double* __restrict__ w,
+                                                    double* __restrict__ partials) {
+  double acc[kMaxK];
+#pragma unroll
+  for(int j = 0; j < kMaxK; ++j) acc[j] = 0.;
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256) {
+    const double wn = w[n];
+#pragma unroll
+    for(int j = 0; j < kMaxK; ++j)
+      if(j < k) acc[j] = fma(V[n + j * ldv], wn, acc[j]);
+  }
+  __shared__ double lds[4][kMaxK];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for(int j = 0; j < kMaxK; ++j) {
+    if(j < k) {
+      const double v = wave_sum(acc[j]);
+      if(lane == 0) lds[wv][j] = v;
+    }
+  }
+  __syncthreads();
+  if(threadIdx.x < k)
+    partials[(int64_t)blockIdx.x * kMaxK + threadIdx.x] =
+        (lds[0][threadIdx.x] + lds[1][threadIdx.x]) + (lds[2][threadIdx.x] + lds[3][threadIdx.x]);
+}
+
+// out[i] = sum_b partials[b*stride + i] in block order.
+__global__ void reduce_kernel(const double* __restrict__ partials, int nblocks,
+                              int count, int stride, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= count) return;
+  double s = 0.;
+  for(int b = 0; b < nblocks; ++b) s += partials[(int64_t)b * stride + i];
+  out[i] = s;
+}
+
+// w = w - V(:, :k) * coef   (gram_schmidt gemm #2, davidson.hpp:116-117)
+__global__ __launch_bounds__(256) void update_kernel(int64_t N, int k,
+                                                      const double* __restrict__ V,
+                                                      int64_t ldv,
+                                                      const double* __restrict__ coef,
+                                                      double* __restrict__ w) {
+  __shared__ double c[kMaxK];
+  if(threadIdx.x < k) c[threadIdx.x] = coef[threadIdx.x];
+  __syncthreads();
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256) {
+    double s = 0.;
+#pragma unroll
+    for(int j = 0; j < kMaxK; ++j)
+      if(j < k) s = fma(V[n + j * ldv], c[j], s);
+    w[n] = w[n] - s;
+  }
+}
+
+struct RitzCoef {
+  double c[kMaxK];
+  double lam;
+};
+
+// X = V c; R = AV c - lam X; partial ||R||^2   (davidson.hpp:181-193)
+__global__ __launch_bounds__(256) void residual_kernel(int64_t N, int k,
+                                                        const double* __restrict__ V,
+                                                        const double* __restrict__ AV,
+                                                        int64_t ld, RitzCoef rc,
+                                                        double* __restrict__ X,
+                                                        double* __restrict__ R,
+                                                        double* __restrict__ partials) {
+  double acc = 0.;
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256) {
+    double x = 0., r = 0.;
+#pragma unroll
+    for(int j = 0; j < kMaxK; ++j)
+      if(j < k) {
+        x = fma(V[n + j * ld], rc.c[j], x);
+        r = fma(AV[n + j * ld], rc.c[j], r);
+      }
+    X[n] = x;
+    const double rr = r + (-rc.lam) * x;
+    R[n] = rr;
+    acc = fma(rr, rr, acc);
+  }
+  __shared__ double lds[4];
+  const double v = wave_sum(acc);
+  if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if(threadIdx.x == 0) partials[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+// R = -R / (D - lam)   (davidson.hpp:212-214)
+__global__ void precond_kernel(int64_t N, double* __restrict__ R,
+                               const double* __restrict__ D, double lam) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256)
+    R[n] = -R[n] / (D[n] - lam);
+}
+
+// w *= 1 / sqrt(nrm2[0])   (nrm2 + scal, davidson.hpp:124-125)
+__global__ void scale_kernel(int64_t N, double* __restrict__ w,
+                             const double* __restrict__ nrm2) {
+  const double s = 1. / sqrt(nrm2[0]);
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256)
+    w[n] *= s;
+}
+
+// Per-block first-index argmin of D and max |C|.
+__global__ __launch_bounds__(256) void argmin_maxabs_kernel(
+    int64_t N, const double* __restrict__ D, const double* __restrict__ C,
+    double* __restrict__ pmin, int64_t* __restrict__ pidx,
+    double* __restrict__ pmax) {
+  double mn = INFINITY, mx = 0.;
+  int64_t mi = INT64_MAX;
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256) {
+    const double d = D[n];
+    if(d < mn || (d == mn && n < mi)) {
+      mn = d;
+      mi = n;
+    }
+    if(C) mx = fmax(mx, fabs(C[n]));
+  }
+  __shared__ double smn[256], smx[256];
+  __shared__ int64_t smi[256];
+  smn[threadIdx.x] = mn;
+  smi[threadIdx.x] = mi;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for(int off = 128; off > 0; off >>= 1) {
+    if(threadIdx.x < off) {
+      const int o = threadIdx.x + off;
+      if(smn[o] < smn[threadIdx.x] || (smn[o] == smn[threadIdx.x] && smi[o] < smi[threadIdx.x])) {
+        smn[threadIdx.x] = smn[o];
+        smi[threadIdx.x] = smi[o];
+      }
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[o]);
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) {
+    pmin[blockIdx.x] = smn[0];
+    pidx[blockIdx.x] = smi[0];
+    pmax[blockIdx.x] = smx[0];
+  }
+}
+
+__global__ void set_unit_kernel(double* X, int64_t i) { X[i] = 1.; }
+
+// ---- host-side helpers -----------------------------------------------------
+struct Ops {
+  mhip_context* ctx;
+  int64_t N;
+  DevBuf part;   // reduction partials
+  DevBuf small;  // reduced results (k inner products, norms, C)
+  Ops(mhip_context* c, int64_t n) : ctx(c), N(n) {
+    part.alloc(sizeof(double) * std::max(kRedBlocks * kMaxK, 64 * 4 * 256));
+    small.alloc(sizeof(double) * (kMaxK * kMaxK + 4 * kMaxK));
+  }
+  hipStream_t st() const { return ctx->stream; }
+  double* inner() { return small.as<double>(); }
+  double* scalar() { return small.as<double>() + kMaxK; }
+  double* Cdev() { return small.as<double>() + 4 * kMaxK; }
+
+  void dots(const double* V, int64_t ldv, int k, const double* w, double* out) {
+    const int nb = red_blocks(N);
+    hipLaunchKernelGGL(dots_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, ldv, w,
+                       part.as<double>());
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, k,
+                       kMaxK, out);
+  }
+  void update(const double* V, int64_t ldv, int k, const double* coef, double* w) {
+    hipLaunchKernelGGL(update_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N, k, V, ldv,
+                       coef, w);
+  }
+  // gram_schmidt (davidson.hpp:111-126): CGS2 against V(:, :k), normalise.
+  void gram_schmidt(const double* V, int k, double* w) {
+    for(int pass = 0; pass < 2; ++pass) {
+      dots(V, N, k, w, inner());
+      update(V, N, k, inner(), w);
+    }
+    normalize(w);
+  }
+  void normalize(double* w) {
+    dots(w, N, 1, w, scalar());
+    hipLaunchKernelGGL(scale_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N, w, scalar());
+  }
+  // C (k x k, ldc = k) = V^T AV on MFMA, copied to host.
+  void gram(const double* V, const double* AV, int k, double* C_host) {
+    const int nb = std::max<int64_t>(1, std::min<int64_t>(64, (N + 127) / 128));
+    if(k <= 16) {
+      hipLaunchKernelGGL((gram_mfma_kernel<1, 1>), dim3(nb), dim3(256), 0, st(), N, k, V, N, k,
+                         AV, N, part.as<double>());
+      hipLaunchKernelGGL((gram_finalize_kernel<1, 1>), dim3(4), dim3(64), 0, st(),
+                         part.as<double>(), nb, k, k, Cdev(), k);
+    } else {
+      hipLaunchKernelGGL((gram_mfma_kernel<2, 2>), dim3(nb), dim3(256), 0, st(), N, k, V, N, k,
+                         AV, N, part.as<double>());
+      hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(64), 0, st(),
+                         part.as<double>(), nb, k, k, Cdev(), k);
+    }
+    MHIP_HIP_CHECK(hipGetLastError());
+    MHIP_HIP_CHECK(hipMemcpyAsync(C_host, Cdev(), sizeof(double) * k * k,
+                                  hipMemcpyDeviceToHost, st()));
+    MHIP_HIP_CHECK(hipStreamSynchronize(st()));
+  }
+  double residual(const double* V, const double* AV, int k, const RitzCoef& rc,
+                  double* X, double* R) {
+    const int nb = red_blocks(N);
+    hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, AV, N, rc, X, R,
+                       part.as<double>());
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, 1, 1,
+                       scalar());
+    double r2 = 0.;
+    MHIP_HIP_CHECK(hipMemcpyAsync(&r2, scalar(), 8, hipMemcpyDeviceToHost, st()));
+    MHIP_HIP_CHECK(hipStreamSynchronize(st()));
+    return std::sqrt(r2);
+  }
+};
+
+}  // namespace
+
+// Cyclic Jacobi on the lower triangle (Golub & Van Loan, Alg. 8.5.3);
+// eigenvalues ascending, eigenvectors in the columns of A (replaces
+// lapack::syev(Vec, Lower), rayleigh_ritz.hpp:75).
+void syev_lower(int n, double* A, int lda, double* W) {
+  std::vector<double> a(n * n), v(n * n, 0.);
+  for(int j = 0; j < n; ++j)
+    for(int i = j; i < n; ++i) a[i + n * j] = a[j + n * i] = A[i + lda * j];
+  for(int i = 0; i < n; ++i) v[i + n * i] = 1.;
+  for(int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0., tot = 0.;
+    for(int j = 0; j < n; ++j)
+      for(int i = 0; i < n; ++i) {
+        tot += a[i + n * j] * a[i + n * j];
+        if(i != j) off += a[i + n * j] * a[i + n * j];
+      }
+    if(off <= 1e-32 * tot || off == 0.) break;
+    for(int p = 0; p < n - 1; ++p)
+      for(int q = p + 1; q < n; ++q) {
+        const double apq = a[p + n * q];
+        if(apq == 0.) continue;
+        const double tau = (a[q + n * q] - a[p + n * p]) / (2. * apq);
+        const double t = (tau >= 0. ? 1. : -1.) / (std::fabs(tau) + std::sqrt(1. + tau * tau));
+        const double c = 1. / std::sqrt(1. + t * t), s = t * c;
+        for(int r = 0; r < n; ++r) {
+          const double akp = a[r + n * p], akq = a[r + n * q];
+          a[r + n * p] = c * akp - s * akq;
+          a[r + n * q] = s * akp + c * akq;
+        }
+        for(int r = 0; r < n; ++r) {
+          const double apk = a[p + n * r], aqk = a[q + n * r];
+          a[p + n * r] = c * apk - s * aqk;
+          a[q + n * r] = s * apk + c * aqk;
+        }
+        for(int r = 0; r < n; ++r) {
+          const double vkp = v[r + n * p], vkq = v[r + n * q];
+          v[r + n * p] = c * vkp - s * vkq;
+          v[r + n * q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for(int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int x, int y) { return a[x * (n + 1)] < a[y * (n + 1)]; });
+  for(int j = 0; j < n; ++j) {
+    W[j] = a[ord[j] * (n + 1)];
+    for(int i = 0; i < n; ++i) A[i + lda * j] = v[i + n * ord[j]];
+  }
+}
+
+bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
+              int64_t max_restarts, double tol, const double* d_D, double* d_X,
+              double* E, int64_t* iters) {
+  MHIP_REQUIRE(d_X, MHIP_ERR_NO_GUESS, "Davidson: No Guess Provided");
+  MHIP_REQUIRE(A->m == A->n, MHIP_ERR_INVALID_ARG, "Davidson needs a square matrix");
+  const int64_t N = A->m;
+  mhip_timings& tm = ctx->timings;
+  tm.dav_spmv_ms = 0.;
+  tm.dav_spmv_calls = 0;
+  tm.dav_iters = 0;
+  tm.dav_restarts = 0;
+  tm.dav_residual = 0.;
+  if(N == 0) {
+    *E = 0.;
+    if(iters) *iters = 0;
+    return false;
+  }
+  max_m = std::min(max_m, N);  // davidson.hpp:140
+  MHIP_REQUIRE(max_m >= 1 && max_m < kMaxK, MHIP_ERR_INVALID_ARG,
+               "max_m must be in [1, 31]");
+  hipStream_t st = ctx->stream;
+  Timer t_all(st), t_op(st);
+  t_all.start();
+  Ops ops(ctx, N);
+  const int64_t ld = N;
+  DevBuf Vb(sizeof(double) * N * (max_m + 1)), AVb(sizeof(double) * N * (max_m + 1));
+  double* V = Vb.as<double>();
+  double* AV = AVb.as<double>();
+  DevBuf Dbuf;
+  if(!d_D) {
+    Dbuf.alloc(sizeof(double) * N);
+    extract_diagonal(ctx, A, Dbuf.as<double>());
+    d_D = Dbuf.as<double>();
+  }
+  auto op = [&](const double* x, double* y) {
+    t_op.start();
+    spmv(ctx, A, 1., x, 0., y);
+    t_op.stop();
+    tm.dav_spmv_ms += t_op.ms();
+    tm.dav_spmv_calls++;
+  };
+  MHIP_HIP_CHECK(hipMemsetAsync(AV, 0, sizeof(double) * N * (max_m + 1), st));
+  MHIP_HIP_CHECK(hipMemcpyAsync(V, d_X, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+  op(V, AV);                                                               // :153
+  MHIP_HIP_CHECK(hipMemcpyAsync(V + ld, AV, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+  ops.gram_schmidt(V, 1, V + ld);                                           // :157
+
+  bool converged = false;
+  int64_t iter = 1, restarts = 0;
+  double C[kMaxK * kMaxK], LAM[kMaxK];
+  double lam0 = 0., rnorm = 0.;
+  for(;;) {
+    for(int64_t i = 1; i < max_m; ++i, ++iter) {
+      const int k = (int)(i + 1);
+      op(V + i * ld, AV + i * ld);                                          // :166
+      ops.gram(V, AV, k, C);                                                // :172
+      syev_lower(k, C, k, LAM);
+      lam0 = LAM[0];
+      RitzCoef rc;
+      for(int j = 0; j < kMaxK; ++j) rc.c[j] = j < k ? C[j] : 0.;
+      rc.lam = lam0;
+      double* R = V + (i + 1) * ld;
+      rnorm = ops.residual(V, AV, k, rc, d_X, R);                           // :181-193
+      if(!std::isfinite(rnorm)) break;
+      if(rnorm < tol) {                                                     // :205
+        converged = true;
+        break;
+      }
+      hipLaunchKernelGGL(precond_kernel, dim3(red_blocks(N)), dim3(256), 0, st, N, R, d_D,
+                         lam0);                                             // :212
+      ops.gram_schmidt(V, k, R);                                            // :217
+    }
+    if(converged || restarts >= max_restarts || !std::isfinite(rnorm)) break;
+    // Restart (labelled variant): V0 = X/||X||, V1 = V(:,max_m), AV0 = A V0.
+    ++restarts;
+    MHIP_HIP_CHECK(hipMemcpyAsync(V, d_X, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    ops.normalize(V);
+    MHIP_HIP_CHECK(hipMemcpyAsync(V + ld, V + max_m * ld, sizeof(double) * N,
+                                  hipMemcpyDeviceToDevice, st));
+    op(V, AV);
+  }
+  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  t_all.stop();
+  tm.dav_total_ms = t_all.ms();
+  tm.dav_iters = iter;
+  tm.dav_restarts = restarts;
+  tm.dav_residual = rnorm;
+  *E = lam0;
+  if(iters) *iters = iter;
+  return converged;
+}
+
+// Guess rule of serial_selected_ci_diag (selected_ci_diag.hpp:94-104).
+void guess_rule(mhip_context* ctx, int64_t N, const double* d_D, double* d_C) {
+  const int nb = red_blocks(N);
+  DevBuf p(sizeof(double) * nb * 3);
+  double* pmin = p.as<double>();
+  int64_t* pidx = reinterpret_cast<int64_t*>(pmin + nb);
+  double* pmax = pmin + 2 * nb;
+  hipLaunchKernelGGL(argmin_maxabs_kernel, dim3(nb), dim3(256), 0, ctx->stream, N, d_D, d_C,
+                     pmin, pidx, pmax);
+  std::vector<double> h(3 * nb);
+  MHIP_HIP_CHECK(hipMemcpyAsync(h.data(), p.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  double mn = INFINITY, mx = 0.;
+  int64_t mi = 0;
+  for(int b = 0; b < nb; ++b) {
+    int64_t idx;
+    std::memcpy(&idx, &h[nb + b], 8);
+    if(h[b] < mn || (h[b] == mn && idx < mi)) {
+      mn = h[b];
+      mi = idx;
+    }
+    mx = std::max(mx, h[2 * nb + b]);
+  }
+  if(!(mx > 1. / (double)N))  // diagonal_guess (davidson.hpp:61-68)
+    hipLaunchKernelGGL(set_unit_kernel, dim3(1), dim3(1), 0, ctx->stream, d_C, mi);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+// Tall-skinny GEMM entry used by tests and the C++ layer: C = A^T B.
+void gemm_tn(mhip_context* ctx, int64_t N, int k, const double* A, int64_t lda, int m,
+             const double* B, int64_t ldb, double* d_C, int ldc) {
+  MHIP_REQUIRE(k >= 1 && k <= 32 && m >= 1 && m <= 32, MHIP_ERR_INVALID_ARG,
+               "k, m must be in [1, 32]");
+  const int nb = std::max<int64_t>(1, std::min<int64_t>(64, (N + 127) / 128));
+  DevBuf part(sizeof(double) * nb * 4 * 256);
+  hipLaunchKernelGGL((gram_mfma_kernel<2, 2>), dim3(nb), dim3(256), 0, ctx->stream, N, k, A,
+                     lda, m, B, ldb, part.as<double>());
+  hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(64), 0, ctx->stream,
+                     part.as<double>(), nb, k, m, d_C, ldc);
+  MHIP_HIP_CHECK(hipGetLastError());
+  MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" {
+
+int mhip_davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
+                  int64_t max_restarts, double tol, const double* d_D,
+                  double* d_X, double* E, int64_t* iters) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(A && E, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_REQUIRE(max_restarts >= 0, MHIP_ERR_INVALID_ARG, "negative max_restarts");
+    if(!davidson(ctx, A, max_m, max_restarts, tol, d_D, d_X, E, iters))
+      throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");
+  });
+}
+
+int mhip_selected_ci_diag(mhip_context* ctx, int words_per_det,
+                          const uint64_t* d_dets, int64_t ndets,
+                          double h_el_tol, int64_t max_m, int64_t max_restarts,
+                          double res_tol, double* d_C, double* E,
+                          int64_t* nnz, mhip_csr** H_out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(E && (ndets == 0 || d_C), MHIP_ERR_INVALID_ARG, "null argument");
+    if(H_out) *H_out = nullptr;
+    std::unique_ptr<mhip_csr> H(ham_build(ctx, words_per_det, d_dets, ndets, d_dets, ndets,
+                                          h_el_tol));
+    if(nnz) *nnz = H->nnz;
+    DevBuf D(sizeof(double) * std::max<int64_t>(ndets, 1));
+    extract_diagonal(ctx, H.get(), D.as<double>());
+    if(ndets) guess_rule(ctx, ndets, D.as<double>(), d_C);
+    bool ok = davidson(ctx, H.get(), max_m, max_restarts, res_tol, D.as<double>(), d_C, E,
+                       nullptr);
+    if(H_out) *H_out = H.release();
+    if(!ok) throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");
+  });
+}
+
+int mhip_gemm_tn(mhip_context* ctx, int64_t N, int k, const double* d_A,
+                 int64_t lda, int m, const double* d_B, int64_t ldb, double* d_C,
+                 int ldc) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(d_A && d_B && d_C && lda >= N && ldb >= N && ldc >= k,
+                 MHIP_ERR_INVALID_ARG, "bad arguments");
+    gemm_tn(ctx, N, k, d_A, lda, m, d_B, ldb, d_C, ldc);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,579 @@
+// K1: Hamiltonian generation on the device.
+//
+// Replaces DoubleLoopHamiltonianGenerator::make_csr_hamiltonian_block_
+// (double_loop.hpp:30-98): for every bra row, every ket j with
+// popcount(bra ^ ket) <= 4 is a candidate; the element is evaluated with the
+// Slater-Condon rules and kept when |h| > thresh, columns ascending.
+//
+// Device pipeline (all HBM resident):
+//   1. bounds  : AND/OR of every 64-ket tile and every 64-tile superblock.
+//                popcount(b & ~OR) + popcount(AND & ~b) is a lower bound of
+//                popcount(b ^ k) over the block, so blocks with bound > 4
+//                are skipped without touching their kets.
+//   2. count   : one wavefront handles R consecutive bra rows (bras held in
+//                scalar registers); lanes screen 64 superblocks / 64 tiles at
+//                a time, then test 64 kets per step with XOR + popcount and
+//                count hits with a wavefront ballot.
+//   3. scan    : device exclusive scan of the row counts -> row pointers.
+//   4. fill    : the same walk writes ascending column indices at
+//                rowptr + running count + ballot prefix (mbcnt).
+//   5. element : one wavefront per row evaluates the Slater-Condon element
+//                of each stored column (full lane utilisation) and counts
+//                |h| > thresh.
+//   6. compact : only when some element fell at or below the threshold,
+//                a second scan + a stable ballot compaction drops them.
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <memory>
+
+#include "mhip_device.hpp"
+#include "mhip_internal.hpp"
+
+namespace mhip {
+namespace {
+
+constexpr int kTile = 64;   // kets per tile (one per lane)
+constexpr int kSuper = 64;  // tiles per superblock
+constexpr int kRows = 8;    // bra rows per wavefront in the scan passes
+constexpr int kWavesPerBlock = 4;
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// AND / OR reduction of groups of 64 items (W words each); one wavefront
+// per output group. Used for ket tiles (in_and == in_or == kets) and for
+// superblocks (over tile bounds).
+template <int W>
+__global__ __launch_bounds__(256) void bounds_kernel(
+    const uint64_t* __restrict__ in_and, const uint64_t* __restrict__ in_or,
+    int64_t nin, int64_t nout, uint64_t* __restrict__ out_and,
+    uint64_t* __restrict__ out_or) {
+  const int64_t g = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(g >= nout) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t i = g * 64 + lane;
+#pragma unroll
+  for(int w = 0; w < W; ++w) {
+    uint64_t a = ~0ull, o = 0ull;
+    if(i < nin) {
+      a = in_and[i * W + w];
+      o = in_or[i * W + w];
+    }
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) {
+      a &= __shfl_xor(a, off, 64);
+      o |= __shfl_xor(o, off, 64);
+    }
+    if(lane == 0) {
+      out_and[g * W + w] = a;
+      out_or[g * W + w] = o;
+    }
+  }
+}
+
+template <int W>
+__device__ __forceinline__ int excitation_bound(const uint64_t (&b)[W],
+                                                const uint64_t (&a)[W],
+                                                const uint64_t (&o)[W]) {
+  int lb = 0;
+#pragma unroll
+  for(int w = 0; w < W; ++w) lb += popc64(b[w] & ~o[w]) + popc64(a[w] & ~b[w]);
+  return lb;
+}
+
+// Structural count (FILL = false) or fill (FILL = true) pass.
+template <int W, int R, bool FILL>
+__global__ __launch_bounds__(256) void scan_pass_kernel(
+    const uint64_t* __restrict__ bra, int64_t nbra,
+    const uint64_t* __restrict__ ket, int64_t nket,
+    const uint64_t* __restrict__ tand, const uint64_t* __restrict__ tor,
+    int64_t ntiles, const uint64_t* __restrict__ sand,
+    const uint64_t* __restrict__ sor, int64_t nsuper, int64_t nwaves,
+    int64_t nblocks, uint32_t* __restrict__ row_counts,
+    const int64_t* __restrict__ rowptr, int32_t* __restrict__ colind) {
+  // XCD-aware mapping: hardware dispatches block b to XCD b % 8; logical
+  // blocks (consecutive rows, which share spin strings and therefore ket
+  // tiles) are laid out so each XCD's L2 sees a contiguous range of rows.
+  const int64_t per = gridDim.x / 8;
+  const int64_t lblk = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if(lblk >= nblocks) return;
+  const int64_t wave = lblk * kWavesPerBlock + (threadIdx.x >> 6);
+  if(wave >= nwaves) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t row0 = wave * R;
+
+  uint64_t b[R][W];
+  uint32_t live = 0;
+  int64_t base[R];
+  uint32_t cnt[R];
+#pragma unroll
+  for(int r = 0; r < R; ++r) {
+    const int64_t row = row0 + r;
+    uint64_t nz = 0;
+#pragma unroll
+    for(int w = 0; w < W; ++w) {
+      b[r][w] = row < nbra ? uniform_u64(bra[row * W + w]) : 0ull;
+      nz |= b[r][w];
+    }
+    if(nz) live |= 1u << r;  // zero bras give empty rows (double_loop.hpp:50)
+    cnt[r] = 0;
+    base[r] = 0;
+    if constexpr(FILL) {
+      if(row < nbra) base[r] = (int64_t)uniform_u64((uint64_t)rowptr[row]);
+    }
+  }
+  if(!live) {
+    if constexpr(!FILL) {
+      if(lane == 0)
+        for(int r = 0; r < R; ++r)
+          if(row0 + r < nbra) row_counts[row0 + r] = 0;
+    }
+    return;
+  }
+
+  for(int64_t g = 0; g < nsuper; g += 64) {
+    const int64_t s = g + lane;
+    const bool sv = s < nsuper;
+    uint64_t sa[W], so[W];
+#pragma unroll
+    for(int w = 0; w < W; ++w) {
+      sa[w] = sv ? sand[s * W + w] : ~0ull;
+      so[w] = sv ? sor[s * W + w] : 0ull;
+    }
+    uint64_t smask[R];
+    uint64_t sany = 0;
+#pragma unroll
+    for(int r = 0; r < R; ++r) {
+      smask[r] = ((live >> r) & 1u) ? ballot(sv && excitation_bound<W>(b[r], sa, so) <= 4) : 0ull;
+      sany |= smask[r];
+    }
+    while(sany) {
+      const uint32_t sb = ctz64(sany);
+      sany &= sany - 1;
+      const int64_t S = g + sb;
+      const int64_t t = S * kSuper + lane;
+      const bool tv = t < ntiles;
+      uint64_t ta[W], to[W];
+#pragma unroll
+      for(int w = 0; w < W; ++w) {
+        ta[w] = tv ? tand[t * W + w] : ~0ull;
+        to[w] = tv ? tor[t * W + w] : 0ull;
+      }
+      uint64_t tm[R];
+      uint64_t tany = 0;
+#pragma unroll
+      for(int r = 0; r < R; ++r) {
+        tm[r] = ((smask[r] >> sb) & 1ull) ? ballot(tv && excitation_bound<W>(b[r], ta, to) <= 4) : 0ull;
+        tany |= tm[r];
+      }
+      while(tany) {
+        const uint32_t tb = ctz64(tany);
+        tany &= tany - 1;
+        const int64_t j = (S * kSuper + tb) * kTile + lane;
+        uint64_t k[W];
+        uint64_t knz = 0;
+        const bool kv = j < nket;
+#pragma unroll
+        for(int w = 0; w < W; ++w) {
+          k[w] = kv ? ket[j * W + w] : 0ull;
+          knz |= k[w];
+        }
+#pragma unroll
+        for(int r = 0; r < R; ++r) {
+          if((tm[r] >> tb) & 1ull) {
+            int pc = 0;
+#pragma unroll
+            for(int w = 0; w < W; ++w) pc += popc64(b[r][w] ^ k[w]);
+            const bool hit = knz && pc <= 4;
+            const uint64_t m = ballot(hit);
+            if constexpr(FILL) {
+              if(hit) colind[base[r] + lanes_below(m)] = (int32_t)j;
+              base[r] += popc64(m);
+            } else {
+              cnt[r] += (uint32_t)popc64(m);
+            }
+          }
+        }
+      }
+    }
+  }
+  if constexpr(!FILL) {
+    if(lane == 0)
+      for(int r = 0; r < R; ++r)
+        if(row0 + r < nbra) row_counts[row0 + r] = cnt[r];
+  }
+}
+
+// OR of all determinant words (range validation of orbital indices).
+template <int W>
+__global__ __launch_bounds__(256) void or_all_kernel(const uint64_t* __restrict__ d,
+                                                      int64_t n,
+                                                      unsigned long long* __restrict__ out) {
+  uint64_t o[W];
+#pragma unroll
+  for(int w = 0; w < W; ++w) o[w] = 0;
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+      i += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+    for(int w = 0; w < W; ++w) o[w] |= d[i * W + w];
+#pragma unroll
+  for(int w = 0; w < W; ++w) {
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) o[w] |= __shfl_xor(o[w], off, 64);
+    if((threadIdx.x & 63) == 0) atomicOr(out + w, (unsigned long long)o[w]);
+  }
+}
+
+// Widen uint32 row counts into the int64 scan input (last entry 0).
+__global__ void widen_counts_kernel(const uint32_t* __restrict__ c, int64_t n,
+                                    int64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) out[i] = c[i];
+  if(i == n) out[n] = 0;
+}
+
+// Diagonal element of every determinant (thread per determinant).
+template <int W>
+__global__ __launch_bounds__(256) void diag_kernel(IntsView I,
+                                                    const uint64_t* __restrict__ dets,
+                                                    int64_t n, double* __restrict__ diag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const Spins<W> s = load_spins<W>(dets, i);
+  diag[i] = me_diag(I, s.a, s.b);
+}
+
+// Element pass: one wavefront per bra row.
+template <int W>
+__global__ __launch_bounds__(256) void element_kernel(
+    IntsView I, const uint64_t* __restrict__ bra, int64_t nbra,
+    const uint64_t* __restrict__ ket, const double* __restrict__ diag,
+    const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    double* __restrict__ nzval, double thresh, int64_t* __restrict__ kept,
+    int* __restrict__ dropped) {
+  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(row >= nbra) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t st = rowptr[row], en = rowptr[row + 1];
+  if(st == en) {
+    if(lane == 0) kept[row] = 0;
+    return;
+  }
+  Spins<W> bs = load_spins<W>(bra, row);
+  bs.a = uniform_u64(bs.a);
+  bs.b = uniform_u64(bs.b);
+  const double d = diag[row];
+  int64_t nk = 0;
+  for(int64_t j = st + lane; j < en; j += 64) {
+    const Spins<W> ks = load_spins<W>(ket, colind[j]);
+    const double h = matrix_element(I, bs.a, bs.b, ks.a, ks.b, d);
+    nzval[j] = h;
+    nk += (fabs(h) > thresh) ? 1 : 0;  // double_loop.hpp:76
+  }
+  nk = wave_sum(nk);
+  if(lane == 0) {
+    kept[row] = nk;
+    if(nk != en - st) atomicOr(dropped, 1);
+  }
+}
+
+// Stable compaction of the kept entries (one wavefront per row).
+__global__ __launch_bounds__(256) void compact_kernel(
+    int64_t nbra, const int64_t* __restrict__ rp_old,
+    const int32_t* __restrict__ col_old, const double* __restrict__ val_old,
+    const int64_t* __restrict__ rp_new, int32_t* __restrict__ col_new,
+    double* __restrict__ val_new, double thresh) {
+  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(row >= nbra) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t st = rp_old[row], en = rp_old[row + 1];
+  int64_t out = rp_new[row];
+  for(int64_t j0 = st; j0 < en; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const double v = j < en ? val_old[j] : 0.;
+    const bool keep = j < en && fabs(v) > thresh;
+    const uint64_t m = ballot(keep);
+    if(keep) {
+      const int64_t p = out + lanes_below(m);
+      col_new[p] = col_old[j];
+      val_new[p] = v;
+    }
+    out += popc64(m);
+  }
+}
+
+IntsView view(const Integrals& I) {
+  IntsView v;
+  v.n = (int)I.norb;
+  v.T = I.T.as<double>();
+  v.V = I.V.as<double>();
+  v.G = I.G.as<double>();
+  v.G_red = I.G_red.as<double>();
+  v.V_red = I.V_red.as<double>();
+  v.G2_red = I.G2_red.as<double>();
+  v.V2_red = I.V2_red.as<double>();
+  return v;
+}
+
+// Exclusive scan of n+1 int64 values (in[n] must be 0) into out.
+void exclusive_scan(mhip_context* ctx, const int64_t* in, int64_t* out,
+                    int64_t n1) {
+  size_t tmp = 0;
+  MHIP_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp, in, out, (int64_t)0,
+                                         (size_t)n1, rocprim::plus<int64_t>(),
+                                         ctx->stream));
+  ctx->scratch_c.reserve(tmp + 16);
+  MHIP_HIP_CHECK(rocprim::exclusive_scan(ctx->scratch_c.p, tmp, in, out,
+                                         (int64_t)0, (size_t)n1,
+                                         rocprim::plus<int64_t>(), ctx->stream));
+}
+
+template <int W>
+void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
+                const uint64_t* d_ket, int64_t nket, double thresh,
+                mhip_csr* csr) {
+  hipStream_t st = ctx->stream;
+  mhip_timings& tm = ctx->timings;
+  Timer t_all(st), t(st);
+  t_all.start();
+  const IntsView I = view(ctx->ints);
+
+  // 1. bounds
+  t.start();
+  const int64_t ntiles = (nket + kTile - 1) / kTile;
+  const int64_t nsuper = (ntiles + kSuper - 1) / kSuper;
+  DevBuf bnd(sizeof(uint64_t) * W * 2 * (ntiles + nsuper));
+  uint64_t* tand = bnd.as<uint64_t>();
+  uint64_t* tor = tand + W * ntiles;
+  uint64_t* sand = tor + W * ntiles;
+  uint64_t* sor = sand + W * nsuper;
+  hipLaunchKernelGGL(bounds_kernel<W>, dim3((ntiles + 3) / 4), dim3(256), 0, st,
+                     d_ket, d_ket, nket, ntiles, tand, tor);
+  hipLaunchKernelGGL(bounds_kernel<W>, dim3((nsuper + 3) / 4), dim3(256), 0, st,
+                     tand, tor, ntiles, nsuper, sand, sor);
+  MHIP_HIP_CHECK(hipGetLastError());
+  t.stop();
+  tm.ham_bounds_ms = t.ms();
+
+  // 2. count
+  const int64_t nwaves = (nbra + kRows - 1) / kRows;
+  const int64_t nblocks = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t grid = (nblocks + 7) / 8 * 8;
+  MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many bra rows");
+  DevBuf counts32(sizeof(uint32_t) * (nbra + 1));
+  DevBuf counts64(sizeof(int64_t) * (nbra + 1));
+  t.start();
+  hipLaunchKernelGGL((scan_pass_kernel<W, kRows, false>), dim3(grid), dim3(256), 0, st,
+                     d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
+                     nwaves, nblocks, counts32.as<uint32_t>(), nullptr, nullptr);
+  MHIP_HIP_CHECK(hipGetLastError());
+  t.stop();
+  tm.ham_count_ms = t.ms();
+
+  // 3. scan
+  t.start();
+  hipLaunchKernelGGL(widen_counts_kernel, dim3((nbra + 1 + 255) / 256), dim3(256), 0, st,
+                     counts32.as<uint32_t>(), nbra, counts64.as<int64_t>());
+  DevBuf rp_struct(sizeof(int64_t) * (nbra + 1));
+  exclusive_scan(ctx, counts64.as<int64_t>(), rp_struct.as<int64_t>(), nbra + 1);
+  int64_t nnz_struct = 0;
+  MHIP_HIP_CHECK(hipMemcpyAsync(&nnz_struct, rp_struct.as<int64_t>() + nbra, 8,
+                                hipMemcpyDeviceToHost, st));
+  t.stop();
+  tm.ham_scan_ms = t.ms();  // synchronises
+  tm.ham_nnz_struct = nnz_struct;
+  counts32.release();
+
+  // 4. fill
+  DevBuf col(sizeof(int32_t) * std::max<int64_t>(nnz_struct, 1));
+  DevBuf val(sizeof(double) * std::max<int64_t>(nnz_struct, 1));
+  t.start();
+  if(nnz_struct)
+    hipLaunchKernelGGL((scan_pass_kernel<W, kRows, true>), dim3(grid), dim3(256), 0, st,
+                       d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
+                       nwaves, nblocks, nullptr, rp_struct.as<int64_t>(),
+                       col.as<int32_t>());
+  MHIP_HIP_CHECK(hipGetLastError());
+  t.stop();
+  tm.ham_fill_ms = t.ms();
+  bnd.release();
+
+  // 5. elements
+  DevBuf diag(sizeof(double) * nbra);
+  DevBuf flag(sizeof(int));
+  MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+  t.start();
+  hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
+                     nbra, diag.as<double>());
+  if(nnz_struct)
+    hipLaunchKernelGGL(element_kernel<W>, dim3((nbra + 3) / 4), dim3(256), 0, st, I, d_bra,
+                       nbra, d_ket, diag.as<double>(), rp_struct.as<int64_t>(),
+                       col.as<int32_t>(), val.as<double>(), thresh,
+                       counts64.as<int64_t>(), flag.as<int>());
+  MHIP_HIP_CHECK(hipGetLastError());
+  int dropped = 0;
+  MHIP_HIP_CHECK(hipMemcpyAsync(&dropped, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  t.stop();
+  tm.ham_elem_ms = t.ms();
+
+  // 6. compaction (only when the threshold removed something)
+  tm.ham_compact_ms = 0.;
+  if(!dropped) {
+    csr->rowptr = std::move(rp_struct);
+    csr->colind = std::move(col);
+    csr->nzval = std::move(val);
+    csr->nnz = nnz_struct;
+  } else {
+    t.start();
+    MHIP_HIP_CHECK(hipMemsetAsync(counts64.as<int64_t>() + nbra, 0, 8, st));
+    DevBuf rp_new(sizeof(int64_t) * (nbra + 1));
+    exclusive_scan(ctx, counts64.as<int64_t>(), rp_new.as<int64_t>(), nbra + 1);
+    int64_t nnz = 0;
+    MHIP_HIP_CHECK(hipMemcpyAsync(&nnz, rp_new.as<int64_t>() + nbra, 8,
+                                  hipMemcpyDeviceToHost, st));
+    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    DevBuf col2(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
+    DevBuf val2(sizeof(double) * std::max<int64_t>(nnz, 1));
+    hipLaunchKernelGGL(compact_kernel, dim3((nbra + 3) / 4), dim3(256), 0, st, nbra,
+                       rp_struct.as<int64_t>(), col.as<int32_t>(), val.as<double>(),
+                       rp_new.as<int64_t>(), col2.as<int32_t>(), val2.as<double>(),
+                       thresh);
+    MHIP_HIP_CHECK(hipGetLastError());
+    t.stop();
+    tm.ham_compact_ms = t.ms();
+    csr->rowptr = std::move(rp_new);
+    csr->colind = std::move(col2);
+    csr->nzval = std::move(val2);
+    csr->nnz = nnz;
+  }
+  t_all.stop();
+  tm.ham_total_ms = t_all.ms();
+  tm.ham_nnz = csr->nnz;
+}
+
+void check_dets(int W, const uint64_t* p, int64_t n, const char* what) {
+  MHIP_REQUIRE(W == 1 || W == 2, MHIP_ERR_UNSUPPORTED,
+               "words_per_det must be 1 (wfn_t<64>) or 2 (wfn_t<128>)");
+  MHIP_REQUIRE(n >= 0, MHIP_ERR_INVALID_ARG, std::string(what) + ": negative count");
+  MHIP_REQUIRE(n == 0 || p, MHIP_ERR_INVALID_ARG, std::string(what) + ": null pointer");
+  MHIP_REQUIRE(((uintptr_t)p % (8 * W)) == 0, MHIP_ERR_INVALID_ARG,
+               std::string(what) + ": determinant array must be 8*W-byte aligned");
+}
+
+}  // namespace
+
+void validate_dets(mhip_context* ctx, int W, const uint64_t* d, int64_t n);
+
+mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
+                    int64_t nbra, const uint64_t* d_ket, int64_t nket,
+                    double thresh) {
+  use_device(ctx);
+  check_dets(W, d_bra, nbra, "bra");
+  check_dets(W, d_ket, nket, "ket");
+  MHIP_REQUIRE(nket < (1ll << 31), MHIP_ERR_INDEX_OVERFLOW,
+               "ket count exceeds the int32 column index range");
+  std::unique_ptr<mhip_csr> csr(new mhip_csr());
+  csr->device = ctx->device;
+  csr->m = nbra;
+  csr->n = nket;
+  csr->indexing = 0;
+  if(nbra == 0 || nket == 0) {  // csr_hamiltonian.hpp:30-35
+    csr->rowptr.alloc(sizeof(int64_t) * (nbra + 1));
+    MHIP_HIP_CHECK(hipMemsetAsync(csr->rowptr.p, 0, sizeof(int64_t) * (nbra + 1), ctx->stream));
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    csr->nnz = 0;
+    ctx->timings.ham_nnz = ctx->timings.ham_nnz_struct = 0;
+    return csr.release();
+  }
+  MHIP_REQUIRE(ctx->ints.norb > 0, MHIP_ERR_NO_INTEGRALS,
+               "mhip_set_integrals must be called before building H");
+  MHIP_REQUIRE(ctx->ints.norb <= 32 * W, MHIP_ERR_INVALID_ARG,
+               "norb exceeds the spin-string width of the determinant type");
+  validate_dets(ctx, W, d_bra, nbra);
+  if(d_ket != d_bra || nket != nbra) validate_dets(ctx, W, d_ket, nket);
+  if(W == 1)
+    build_impl<1>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get());
+  else
+    build_impl<2>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get());
+  return csr.release();
+}
+
+void ham_diagonal(mhip_context* ctx, int W, const uint64_t* d_dets,
+                  int64_t ndets, double* d_diag) {
+  use_device(ctx);
+  check_dets(W, d_dets, ndets, "dets");
+  MHIP_REQUIRE(ctx->ints.norb > 0, MHIP_ERR_NO_INTEGRALS, "no integrals set");
+  if(!ndets) return;
+  validate_dets(ctx, W, d_dets, ndets);
+  const IntsView I = view(ctx->ints);
+  if(W == 1)
+    hipLaunchKernelGGL(diag_kernel<1>, dim3((ndets + 255) / 256), dim3(256), 0, ctx->stream,
+                       I, d_dets, ndets, d_diag);
+  else
+    hipLaunchKernelGGL(diag_kernel<2>, dim3((ndets + 255) / 256), dim3(256), 0, ctx->stream,
+                       I, d_dets, ndets, d_diag);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+IntsView device_view(const Integrals& I) { return view(I); }
+
+// Rejects determinants with occupied orbitals >= norb (they would index
+// the integral tables out of range).
+void validate_dets(mhip_context* ctx, int W, const uint64_t* d, int64_t n) {
+  if(!n) return;
+  DevBuf o(16);
+  MHIP_HIP_CHECK(hipMemsetAsync(o.p, 0, 16, ctx->stream));
+  const int64_t blocks = std::min<int64_t>(1024, (n + 255) / 256);
+  if(W == 1)
+    hipLaunchKernelGGL(or_all_kernel<1>, dim3(blocks), dim3(256), 0, ctx->stream, d, n,
+                       o.as<unsigned long long>());
+  else
+    hipLaunchKernelGGL(or_all_kernel<2>, dim3(blocks), dim3(256), 0, ctx->stream, d, n,
+                       o.as<unsigned long long>());
+  MHIP_HIP_CHECK(hipGetLastError());
+  uint64_t h[2] = {0, 0};
+  MHIP_HIP_CHECK(hipMemcpyAsync(h, o.p, 8 * W, hipMemcpyDeviceToHost, ctx->stream));
+  MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  const int64_t norb = ctx->ints.norb;
+  uint64_t a, b;
+  if(W == 1) {
+    a = h[0] & 0xffffffffull;
+    b = h[0] >> 32;
+  } else {
+    a = h[0];
+    b = h[1];
+  }
+  const uint64_t bad = norb >= 64 ? 0ull : ~((1ull << norb) - 1ull);
+  MHIP_REQUIRE(!((a | b) & bad), MHIP_ERR_INVALID_ARG,
+               "determinant occupies an orbital index >= norb");
+}
+
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" {
+
+int mhip_ham_build(mhip_context* ctx, int words_per_det, const uint64_t* d_bra,
+                   int64_t nbra, const uint64_t* d_ket, int64_t nket,
+                   double thresh, mhip_csr** out) {
+  return guarded([&] {
+    MHIP_REQUIRE(out, MHIP_ERR_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    *out = ham_build(ctx, words_per_det, d_bra, nbra, d_ket, nket, thresh);
+  });
+}
+
+int mhip_ham_diagonal(mhip_context* ctx, int words_per_det,
+                      const uint64_t* d_dets, int64_t ndets, double* d_diag) {
+  return guarded([&] {
+    MHIP_REQUIRE(ndets == 0 || d_diag, MHIP_ERR_INVALID_ARG, "null output");
+    ham_diagonal(ctx, words_per_det, d_dets, ndets, d_diag);
+  });
+}
+
+}  // extern "C"

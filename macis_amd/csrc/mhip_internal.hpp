@@ -1,0 +1,169 @@
+// Internal declarations shared by the HIP translation units of libmacis_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "macis_hip.h"
+
+namespace mhip {
+
+// Error carrying one of the MHIP_ERR_* codes.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define MHIP_HIP_CHECK(expr)                                                 \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if(_e != hipSuccess) {                                                   \
+      throw ::mhip::Error(_e == hipErrorOutOfMemory ? MHIP_ERR_OUT_OF_MEMORY \
+                                                    : MHIP_ERR_HIP,          \
+                          std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    }                                                                        \
+  } while(0)
+
+#define MHIP_REQUIRE(cond, code, msg)       \
+  do {                                      \
+    if(!(cond)) throw ::mhip::Error(code, msg); \
+  } while(0)
+
+// Runs `f` and converts exceptions to status codes (the C-ABI contract).
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    set_last_error("");
+    return MHIP_SUCCESS;
+  } catch(const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch(const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return MHIP_ERR_OUT_OF_MEMORY;
+  } catch(const std::exception& e) {
+    set_last_error(e.what());
+    return MHIP_ERR_HIP;
+  }
+}
+
+// Owning device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if(this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t b) {
+    release();
+    if(b) MHIP_HIP_CHECK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  // Grow-only reuse.
+  void reserve(size_t b) {
+    if(b > bytes) alloc(b);
+  }
+  void release() {
+    if(p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Integral tables on the device (column-major, see macis_hip.h).
+struct Integrals {
+  int64_t norb = 0;
+  DevBuf T, V, G, G_red, V_red, G2_red, V2_red;
+};
+
+// Event pair timer on a stream.
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s = nullptr;
+  explicit Timer(hipStream_t st) : s(st) {
+    MHIP_HIP_CHECK(hipEventCreate(&a));
+    MHIP_HIP_CHECK(hipEventCreate(&b));
+  }
+  ~Timer() {
+    if(a) (void)hipEventDestroy(a);
+    if(b) (void)hipEventDestroy(b);
+  }
+  void start() { MHIP_HIP_CHECK(hipEventRecord(a, s)); }
+  void stop() { MHIP_HIP_CHECK(hipEventRecord(b, s)); }
+  double ms() {
+    MHIP_HIP_CHECK(hipEventSynchronize(b));
+    float t = 0.f;
+    MHIP_HIP_CHECK(hipEventElapsedTime(&t, a, b));
+    return t;
+  }
+};
+
+}  // namespace mhip
+
+struct mhip_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mhip::Integrals ints;
+  mhip_timings timings{};
+  // Reusable scratch (partials, flags, temp storage of device primitives).
+  mhip::DevBuf scratch_a, scratch_b, scratch_c;
+  std::vector<char> host_scratch;
+};
+
+struct mhip_csr {
+  int device = 0;
+  int64_t m = 0, n = 0, nnz = 0;
+  int64_t indexing = 0;
+  mhip::DevBuf rowptr;  // int64[m+1]
+  mhip::DevBuf colind;  // int32[nnz]
+  mhip::DevBuf nzval;   // double[nnz]
+};
+
+namespace mhip {
+// Makes the context's device current.
+void use_device(mhip_context* ctx);
+
+// Hamiltonian construction shared by mhip_ham_build and selected_ci_diag.
+mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
+                    int64_t nbra, const uint64_t* d_ket, int64_t nket,
+                    double thresh);
+void ham_diagonal(mhip_context* ctx, int W, const uint64_t* d_dets,
+                  int64_t ndets, double* d_diag);
+// y = alpha*A*x + beta*y on the context stream (one column).
+void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
+          double beta, double* y);
+void extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
+// Davidson core; returns convergence flag.
+bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
+              int64_t max_restarts, double tol, const double* d_D, double* d_X,
+              double* E, int64_t* iters);
+void validate_dets(mhip_context* ctx, int W, const uint64_t* d, int64_t n);
+// Host symmetric eigensolver on the lower triangle (cyclic Jacobi).
+void syev_lower(int n, double* A, int lda, double* W);
+}  // namespace mhip

@@ -1,0 +1,253 @@
+// K2: FP64 CSR SpMV (gespmbv, spmbv.hpp:37-73), diagonal extraction
+// (extract_diagonal_elements, submatrix.hpp:264-295) and the CSR handle
+// entry points.
+//
+// SpMV layout: one wavefront per row; lanes stride the row, values and
+// column indices are streamed with non-temporal loads (read once, kept out
+// of L2 so the gathered x vector stays cached), x is gathered through
+// L2 / Infinity Cache, and the row sum is reduced across the wavefront.
+// Bound: HBM (2 flops per 12 streamed bytes).
+#include <algorithm>
+#include <limits>
+#include <memory>
+#include <vector>
+
+#include "mhip_device.hpp"
+#include "mhip_internal.hpp"
+
+namespace mhip {
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+
+__global__ __launch_bounds__(256) void spmv_kernel(
+    int64_t m, const int64_t* __restrict__ rowptr,
+    const int32_t* __restrict__ colind, const double* __restrict__ nzval,
+    int64_t indexing, double alpha, const double* __restrict__ x, double beta,
+    double* __restrict__ y) {
+  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(row >= m) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t st = rowptr[row] - indexing, en = rowptr[row + 1] - indexing;
+  const double* xb = x - indexing;
+  double acc0 = 0., acc1 = 0., acc2 = 0., acc3 = 0.;
+  int64_t j = st + lane;
+  for(; j + 192 < en; j += 256) {
+    const int32_t c0 = __builtin_nontemporal_load(colind + j);
+    const int32_t c1 = __builtin_nontemporal_load(colind + j + 64);
+    const int32_t c2 = __builtin_nontemporal_load(colind + j + 128);
+    const int32_t c3 = __builtin_nontemporal_load(colind + j + 192);
+    const double v0 = __builtin_nontemporal_load(nzval + j);
+    const double v1 = __builtin_nontemporal_load(nzval + j + 64);
+    const double v2 = __builtin_nontemporal_load(nzval + j + 128);
+    const double v3 = __builtin_nontemporal_load(nzval + j + 192);
+    acc0 = fma(v0, xb[c0], acc0);
+    acc1 = fma(v1, xb[c1], acc1);
+    acc2 = fma(v2, xb[c2], acc2);
+    acc3 = fma(v3, xb[c3], acc3);
+  }
+  for(; j < en; j += 64) {
+    const int32_t c = __builtin_nontemporal_load(colind + j);
+    const double v = __builtin_nontemporal_load(nzval + j);
+    acc0 = fma(v, xb[c], acc0);
+  }
+  double acc = (acc0 + acc1) + (acc2 + acc3);
+  acc = wave_sum(acc);
+  if(lane == 0) y[row] = alpha * acc + beta * y[row];  // spmbv.hpp:71
+}
+
+__global__ __launch_bounds__(256) void extract_diag_kernel(
+    int64_t m, const int64_t* __restrict__ rowptr,
+    const int32_t* __restrict__ colind, const double* __restrict__ nzval,
+    int64_t indexing, double* __restrict__ D) {
+  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(row >= m) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t st = rowptr[row] - indexing, en = rowptr[row + 1] - indexing;
+  double d = 0.;
+  for(int64_t j0 = st; j0 < en; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const bool hit = j < en && (int64_t)colind[j] == row + indexing;
+    const uint64_t mask = ballot(hit);
+    if(mask) {  // first occurrence in row order (std::find)
+      const int first = (int)ctz64(mask);
+      const double v = __shfl(hit ? nzval[j] : 0., first, 64);
+      d = v;
+      break;
+    }
+  }
+  if(lane == 0) D[row] = d;
+}
+
+}  // namespace
+
+void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
+          double beta, double* y) {
+  if(A->m == 0) return;
+  hipLaunchKernelGGL(spmv_kernel, dim3((A->m + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, ctx->stream, A->m, A->rowptr.as<int64_t>(),
+                     A->colind.as<int32_t>(), A->nzval.as<double>(), A->indexing, alpha,
+                     x, beta, y);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+void extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D) {
+  if(A->m == 0) return;
+  hipLaunchKernelGGL(extract_diag_kernel, dim3((A->m + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, ctx->stream, A->m, A->rowptr.as<int64_t>(),
+                     A->colind.as<int32_t>(), A->nzval.as<double>(), A->indexing, d_D);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" {
+
+int mhip_csr_info(const mhip_csr* csr, int64_t* m, int64_t* n, int64_t* nnz) {
+  return guarded([&] {
+    MHIP_REQUIRE(csr, MHIP_ERR_INVALID_ARG, "null csr");
+    if(m) *m = csr->m;
+    if(n) *n = csr->n;
+    if(nnz) *nnz = csr->nnz;
+  });
+}
+
+int mhip_csr_device_arrays(const mhip_csr* csr, const int64_t** d_rowptr,
+                           const int32_t** d_colind, const double** d_nzval) {
+  return guarded([&] {
+    MHIP_REQUIRE(csr, MHIP_ERR_INVALID_ARG, "null csr");
+    if(d_rowptr) *d_rowptr = csr->rowptr.as<int64_t>();
+    if(d_colind) *d_colind = csr->colind.as<int32_t>();
+    if(d_nzval) *d_nzval = csr->nzval.as<double>();
+  });
+}
+
+int mhip_csr_copy_to_host_i32(mhip_context* ctx, const mhip_csr* csr,
+                              int32_t* rowptr, int32_t* colind,
+                              double* nzval) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(csr, MHIP_ERR_INVALID_ARG, "null csr");
+    MHIP_REQUIRE(csr->nnz + csr->indexing <= std::numeric_limits<int32_t>::max() &&
+                     csr->n <= std::numeric_limits<int32_t>::max(),
+                 MHIP_ERR_INDEX_OVERFLOW, "nnz does not fit int32 row pointers");
+    std::vector<int64_t> rp(csr->m + 1);
+    MHIP_HIP_CHECK(hipMemcpyAsync(rp.data(), csr->rowptr.p, 8 * (csr->m + 1),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    if(csr->nnz) {
+      if(colind)
+        MHIP_HIP_CHECK(hipMemcpyAsync(colind, csr->colind.p, 4 * csr->nnz,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+      if(nzval)
+        MHIP_HIP_CHECK(hipMemcpyAsync(nzval, csr->nzval.p, 8 * csr->nnz,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    }
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if(rowptr)
+      for(int64_t i = 0; i <= csr->m; ++i) rowptr[i] = (int32_t)rp[i];
+  });
+}
+
+int mhip_csr_copy_to_host_i64(mhip_context* ctx, const mhip_csr* csr,
+                              int64_t* rowptr, int64_t* colind,
+                              double* nzval) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(csr, MHIP_ERR_INVALID_ARG, "null csr");
+    if(rowptr)
+      MHIP_HIP_CHECK(hipMemcpyAsync(rowptr, csr->rowptr.p, 8 * (csr->m + 1),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<int32_t> c;
+    if(csr->nnz) {
+      if(colind) {
+        c.resize(csr->nnz);
+        MHIP_HIP_CHECK(hipMemcpyAsync(c.data(), csr->colind.p, 4 * csr->nnz,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+      }
+      if(nzval)
+        MHIP_HIP_CHECK(hipMemcpyAsync(nzval, csr->nzval.p, 8 * csr->nnz,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    }
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if(colind)
+      for(size_t i = 0; i < c.size(); ++i) colind[i] = c[i];
+  });
+}
+
+int mhip_csr_create_from_host(mhip_context* ctx, int64_t m, int64_t n,
+                              int64_t nnz, int64_t indexing,
+                              const int64_t* rowptr, const int32_t* colind,
+                              const double* nzval, mhip_csr** out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(out, MHIP_ERR_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    MHIP_REQUIRE(m >= 0 && n >= 0 && nnz >= 0, MHIP_ERR_INVALID_ARG, "negative size");
+    MHIP_REQUIRE(indexing == 0 || indexing == 1, MHIP_ERR_INVALID_ARG,
+                 "indexing must be 0 or 1");
+    MHIP_REQUIRE(rowptr && (nnz == 0 || (colind && nzval)), MHIP_ERR_INVALID_ARG,
+                 "null array");
+    MHIP_REQUIRE(rowptr[m] - indexing == nnz && rowptr[0] == indexing,
+                 MHIP_ERR_INVALID_ARG, "rowptr inconsistent with nnz / indexing");
+    for(int64_t i = 0; i < m; ++i)
+      MHIP_REQUIRE(rowptr[i + 1] >= rowptr[i], MHIP_ERR_INVALID_ARG,
+                   "rowptr not monotone");
+    for(int64_t j = 0; j < nnz; ++j)
+      MHIP_REQUIRE(colind[j] - indexing >= 0 && colind[j] - indexing < n,
+                   MHIP_ERR_INVALID_ARG, "column index out of range");
+    std::unique_ptr<mhip_csr> csr(new mhip_csr());
+    csr->device = ctx->device;
+    csr->m = m;
+    csr->n = n;
+    csr->nnz = nnz;
+    csr->indexing = indexing;
+    csr->rowptr.alloc(8 * (m + 1));
+    csr->colind.alloc(4 * std::max<int64_t>(nnz, 1));
+    csr->nzval.alloc(8 * std::max<int64_t>(nnz, 1));
+    MHIP_HIP_CHECK(hipMemcpyAsync(csr->rowptr.p, rowptr, 8 * (m + 1), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    if(nnz) {
+      MHIP_HIP_CHECK(hipMemcpyAsync(csr->colind.p, colind, 4 * nnz, hipMemcpyHostToDevice,
+                                    ctx->stream));
+      MHIP_HIP_CHECK(hipMemcpyAsync(csr->nzval.p, nzval, 8 * nnz, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    }
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = csr.release();
+  });
+}
+
+int mhip_csr_destroy(mhip_csr* csr) {
+  return guarded([&] {
+    if(!csr) return;
+    (void)hipSetDevice(csr->device);
+    (void)hipDeviceSynchronize();
+    delete csr;
+  });
+}
+
+int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
+              const double* d_V, int64_t LDV, double beta, double* d_AV,
+              int64_t LDAV) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(A, MHIP_ERR_INVALID_ARG, "null matrix");
+    MHIP_REQUIRE(K >= 0 && LDV >= A->n && LDAV >= A->m, MHIP_ERR_INVALID_ARG,
+                 "bad block dimensions");
+    MHIP_REQUIRE(K == 0 || A->m == 0 || (d_V && d_AV), MHIP_ERR_INVALID_ARG,
+                 "null vector");
+    for(int64_t k = 0; k < K; ++k) spmv(ctx, A, alpha, d_V + k * LDV, beta, d_AV + k * LDAV);
+  });
+}
+
+int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(A && (A->m == 0 || d_D), MHIP_ERR_INVALID_ARG, "null argument");
+    extract_diagonal(ctx, A, d_D);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// HipHamiltonianGenerator<N>: the MI355X drop-in for the reference's
+// HamiltonianGenerator seam (hamiltonian_generator.hpp:66-72), replacing
+// DoubleLoopHamiltonianGenerator::make_csr_hamiltonian_block_
+// (double_loop.hpp:30-114). The CSR is built on the device by
+// mhip_ham_build; the *_bit_ overrides copy it back into the reference's
+// host csr_matrix, device_csr() keeps it in HBM.
+#pragma once
+#include <limits>
+
+#include <macis/hamiltonian_generator.hpp>
+#include <macis/hip/runtime.hpp>
+
+namespace macis {
+
+template <size_t N = 64>
+class HipHamiltonianGenerator : public HamiltonianGenerator<N> {
+ public:
+  using base_type = HamiltonianGenerator<N>;
+  using typename base_type::full_det_iterator;
+  using typename base_type::matrix_span_t;
+  using typename base_type::rank4_span_t;
+  template <typename index_t>
+  using sparse_matrix_type = sparsexx::csr_matrix<double, index_t>;
+  static constexpr int W = hip::words_per_det(N);
+
+  HipHamiltonianGenerator(matrix_span_t T, rank4_span_t V, int device = 0)
+      : base_type(T, V), ctx_(std::make_unique<hip::Context>(device)) {
+    upload();
+  }
+
+  hip::Context& context() { return *ctx_; }
+
+  // Device-resident CSR over [bra) x [ket).
+  hip::DeviceCSR device_csr(full_det_iterator bra_begin, full_det_iterator bra_end,
+                            full_det_iterator ket_begin, full_det_iterator ket_end,
+                            double H_thresh) {
+    auto db = hip::upload_dets<N>(*ctx_, bra_begin, bra_end);
+    auto dk = hip::upload_dets<N>(*ctx_, ket_begin, ket_end);
+    mhip_csr* h = nullptr;
+    hip::check(mhip_ham_build(ctx_->get(), W, db.data(), std::distance(bra_begin, bra_end),
+                              dk.data(), std::distance(ket_begin, ket_end), H_thresh, &h));
+    return hip::DeviceCSR(h);
+  }
+
+  sparse_matrix_type<int32_t> make_csr_hamiltonian_block_32bit_(
+      full_det_iterator bra_begin, full_det_iterator bra_end, full_det_iterator ket_begin,
+      full_det_iterator ket_end, double H_thresh) override {
+    auto H = device_csr(bra_begin, bra_end, ket_begin, ket_end, H_thresh);
+    std::vector<int32_t> rp(H.m() + 1), ci(H.nnz());
+    std::vector<double> nz(H.nnz());
+    hip::check(mhip_csr_copy_to_host_i32(ctx_->get(), H.get(), rp.data(), ci.data(), nz.data()));
+    return sparse_matrix_type<int32_t>(H.m(), H.n(), std::move(rp), std::move(ci), std::move(nz));
+  }
+
+  sparse_matrix_type<int64_t> make_csr_hamiltonian_block_64bit_(
+      full_det_iterator bra_begin, full_det_iterator bra_end, full_det_iterator ket_begin,
+      full_det_iterator ket_end, double H_thresh) override {
+    auto H = device_csr(bra_begin, bra_end, ket_begin, ket_end, H_thresh);
+    std::vector<int64_t> rp(H.m() + 1), ci(H.nnz());
+    std::vector<double> nz(H.nnz());
+    hip::check(mhip_csr_copy_to_host_i64(ctx_->get(), H.get(), rp.data(), ci.data(), nz.data()));
+    return sparse_matrix_type<int64_t>(H.m(), H.n(), std::move(rp), std::move(ci), std::move(nz));
+  }
+
+  // RDMs are outside the accelerated path.
+  void form_rdms(full_det_iterator, full_det_iterator, full_det_iterator, full_det_iterator,
+                 double*, matrix_span_t, rank4_span_t) override {
+    throw std::runtime_error("HipHamiltonianGenerator::form_rdms is not implemented");
+  }
+
+ protected:
+  void integrals_changed() override {
+    if(ctx_) upload();
+  }
+
+ private:
+  void upload() {
+    hip::check(mhip_set_integrals(ctx_->get(), this->norb_, this->T_pq_.data_handle(),
+                                  this->V_pqrs_.data_handle()));
+  }
+  std::unique_ptr<hip::Context> ctx_;
+};
+
+}  // namespace macis

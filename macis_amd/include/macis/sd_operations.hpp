@@ -1,0 +1,111 @@
+// Determinant-space helpers of the MACIS API on the hot path (restating
+// include/macis/sd_operations.hpp:31-97, 421-476).
+#pragma once
+#include <algorithm>
+#include <bitset>
+#include <vector>
+
+#include <macis/bitset_operations.hpp>
+
+namespace macis {
+
+// Lowest nalpha alpha and nbeta beta orbitals occupied (:31-37).
+template <size_t N>
+std::bitset<N> canonical_hf_determinant(uint32_t nalpha, uint32_t nbeta) {
+  static_assert(N % 2 == 0, "N Must Be Even");
+  std::bitset<N> s;
+  for(uint32_t i = 0; i < nalpha; ++i) s.set(i);
+  for(uint32_t i = 0; i < nbeta; ++i) s.set(i + N / 2);
+  return s;
+}
+
+// Occupied / first (norb - nocc) unoccupied orbitals (:82-97).
+template <size_t N>
+void bitset_to_occ_vir(size_t norb, std::bitset<N> state, std::vector<uint32_t>& occ,
+                       std::vector<uint32_t>& vir) {
+  bits_to_indices(state, occ);
+  vir.clear();
+  const size_t nvir = norb - occ.size();
+  for(size_t i = 0; i < N && vir.size() < nvir; ++i)
+    if(!state[i]) vir.push_back(static_cast<uint32_t>(i));
+}
+
+// All nset-subsets of nbits bits in the std::prev_permutation order of the
+// reference (:440-458), i.e. lexicographic order of the occupied indices.
+template <size_t N>
+std::vector<std::bitset<N>> generate_combs(uint64_t nbits, uint64_t nset) {
+  std::vector<std::bitset<N>> out;
+  std::vector<uint32_t> idx(nset);
+  for(uint32_t i = 0; i < nset; ++i) idx[i] = i;
+  if(nset > nbits) return out;
+  for(;;) {
+    std::bitset<N> b;
+    for(auto i : idx) b.set(i);
+    out.push_back(b);
+    int64_t k = static_cast<int64_t>(nset) - 1;
+    while(k >= 0 && idx[k] == nbits - nset + k) --k;
+    if(k < 0) break;
+    ++idx[k];
+    for(uint64_t j = k + 1; j < nset; ++j) idx[j] = idx[j - 1] + 1;
+  }
+  return out;
+}
+
+// FCI space, alpha strings outer, beta inner (:460-476).
+template <size_t N>
+std::vector<std::bitset<N>> generate_hilbert_space(size_t norbs, size_t nalpha, size_t nbeta) {
+  auto a = generate_combs<N>(norbs, nalpha);
+  auto b = generate_combs<N>(norbs, nbeta);
+  std::vector<std::bitset<N>> out;
+  out.reserve(a.size() * b.size());
+  for(auto& x : a)
+    for(auto& y : b) out.push_back(x | (y << (N / 2)));
+  return out;
+}
+
+// CISD space: HF, alpha singles, beta singles, alpha doubles, beta doubles,
+// alpha x beta singles (:345-438).
+template <size_t N>
+std::vector<std::bitset<N>> generate_cisd_hilbert_space(size_t norb, std::bitset<N> state) {
+  auto spin = [&](std::bitset<N / 2> s, std::vector<std::bitset<N / 2>>& sing,
+                  std::vector<std::bitset<N / 2>>& dbl) {
+    std::vector<uint32_t> occ, vir;
+    bitset_to_occ_vir(norb, s, occ, vir);
+    for(size_t a = 0; a < vir.size(); ++a)
+      for(size_t i = 0; i < occ.size(); ++i) {
+        auto t = s;
+        t.flip(occ[i]).flip(vir[a]);
+        sing.push_back(t);
+      }
+    for(size_t a = 0; a < vir.size(); ++a)
+      for(size_t i = 0; i < occ.size(); ++i)
+        for(size_t b = a + 1; b < vir.size(); ++b)
+          for(size_t j = i + 1; j < occ.size(); ++j) {
+            auto t = s;
+            t.flip(occ[i]).flip(vir[a]).flip(occ[j]).flip(vir[b]);
+            dbl.push_back(t);
+          }
+  };
+  auto lo = bitset_lo_word(state), hi = bitset_hi_word(state);
+  std::vector<std::bitset<N / 2>> sa, da, sb, db;
+  spin(lo, sa, da);
+  spin(hi, sb, db);
+  auto full = [](std::bitset<N / 2> a, std::bitset<N / 2> b) {
+    std::bitset<N> r;
+    for(size_t i = 0; i < N / 2; ++i) {
+      if(a[i]) r.set(i);
+      if(b[i]) r.set(i + N / 2);
+    }
+    return r;
+  };
+  std::vector<std::bitset<N>> out{state};
+  for(auto& x : sa) out.push_back(full(x, hi));
+  for(auto& x : sb) out.push_back(full(lo, x));
+  for(auto& x : da) out.push_back(full(x, hi));
+  for(auto& x : db) out.push_back(full(lo, x));
+  for(auto& x : sa)
+    for(auto& y : sb) out.push_back(full(x, y));
+  return out;
+}
+
+}  // namespace macis

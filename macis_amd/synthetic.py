@@ -37,3 +37,89 @@ def seeded_integrals(n: int, seed: int):
         l = L.ravel(order="F")
         M += np.multiply.outer(l, l)
     return np.ascontiguousarray(T.ravel(order="F")), np.ascontiguousarray(M.ravel(order="F"))
+
+
+def _gumbel_choose(rng, logw, k):
+    """Per row, choose k[i] distinct columns with probability proportional to
+    exp(logw) (Gumbel top-k). Returns a boolean (rows, cols) mask."""
+    g = logw[None, :] - np.log(-np.log(rng.random((len(k), len(logw)))))
+    order = np.argsort(-g, axis=1)
+    rank = np.empty_like(order)
+    rows = np.arange(len(k))[:, None]
+    rank[rows, order] = np.arange(len(logw))[None, :]
+    return rank < k[:, None]
+
+
+def selected_ci_dets(n: int, nalpha: int, nbeta: int, ndets: int, seed: int,
+                     T=None, weights=(0.1, 0.5, 0.2, 0.2), batch: int = 400_000):
+    """Seeded selected-CI determinant list for wfn_t<128> (W = 2).
+
+    The HF determinant plus random 1-4-fold excitations of it: the level L
+    is drawn with probabilities `weights`; L_alpha ~ Binomial(L, 1/2) clipped
+    to what each spin allows, L_beta = L - L_alpha; holes and particles are
+    drawn without replacement with probability proportional to
+    exp(-|e_p - e_F|), e_p = h_pp (one-body diagonal, or p when T is None) and
+    e_F halfway between the highest occupied and lowest virtual e_p of each
+    spin. Unique dets are collected until `ndets` (HF included), sorted by
+    bitset_less (beta word most significant, then alpha) and truncated.
+    Returns an (ndets, 2) uint64 array [alpha, beta].
+    """
+    assert n <= 64
+    rng = np.random.Generator(np.random.PCG64(seed))
+    e = np.arange(n, dtype=float) if T is None else np.asarray(T).reshape(n, n, order="F").diagonal().copy()
+    order = np.argsort(e, kind="stable")
+    def spin_setup(ne):
+        occ = order[:ne]
+        vir = order[ne:]
+        eF = 0.5 * (e[occ].max() + e[vir].min()) if 0 < ne < n else 0.0
+        return occ, vir, -np.abs(e[occ] - eF), -np.abs(e[vir] - eF)
+    oa, va, lwoa, lwva = spin_setup(nalpha)
+    ob, vb, lwob, lwvb = spin_setup(nbeta)
+    hf_a = np.uint64(sum(1 << int(i) for i in oa))
+    hf_b = np.uint64(sum(1 << int(i) for i in ob))
+    bits_oa = (np.uint64(1) << oa.astype(np.uint64))
+    bits_va = (np.uint64(1) << va.astype(np.uint64))
+    bits_ob = (np.uint64(1) << ob.astype(np.uint64))
+    bits_vb = (np.uint64(1) << vb.astype(np.uint64))
+    maxa = min(nalpha, n - nalpha)
+    maxb = min(nbeta, n - nbeta)
+    seen = {(int(hf_a), int(hf_b))}
+    a_all = [np.array([hf_a])]
+    b_all = [np.array([hf_b])]
+    total = 1
+    while total < ndets:
+        M = int(min(batch, max(1000, 2 * (ndets - total))))
+        L = rng.choice(4, size=M, p=np.asarray(weights) / np.sum(weights)) + 1
+        la = rng.binomial(L, 0.5)
+        la = np.minimum(la, maxa)
+        lb = np.minimum(L - la, maxb)
+        la = np.minimum(L - lb, maxa)
+        ha = _gumbel_choose(rng, lwoa, la)
+        pa = _gumbel_choose(rng, lwva, la)
+        hb = _gumbel_choose(rng, lwob, lb)
+        pb = _gumbel_choose(rng, lwvb, lb)
+        xa = np.bitwise_or.reduce(np.where(ha, bits_oa, np.uint64(0)), axis=1) | \
+            np.bitwise_or.reduce(np.where(pa, bits_va, np.uint64(0)), axis=1)
+        xb = np.bitwise_or.reduce(np.where(hb, bits_ob, np.uint64(0)), axis=1) | \
+            np.bitwise_or.reduce(np.where(pb, bits_vb, np.uint64(0)), axis=1)
+        A = hf_a ^ xa
+        B = hf_b ^ xb
+        key = np.stack([B, A], axis=1)
+        _, first = np.unique(key, axis=0, return_index=True)
+        first.sort()
+        newa, newb = [], []
+        for i in first:
+            t = (int(A[i]), int(B[i]))
+            if t not in seen:
+                seen.add(t)
+                newa.append(A[i])
+                newb.append(B[i])
+                total += 1
+                if total >= ndets:
+                    break
+        a_all.append(np.array(newa, dtype=np.uint64))
+        b_all.append(np.array(newb, dtype=np.uint64))
+    a = np.concatenate(a_all)[:ndets]
+    b = np.concatenate(b_all)[:ndets]
+    srt = np.lexsort((a, b))
+    return np.ascontiguousarray(np.stack([a[srt], b[srt]], axis=1))

@@ -40,3 +40,19 @@ def h2o_cisd(h2o):
     norb = h2o[0]
     hf = D.canonical_hf_determinant(1, 5, 5)
     return O.cisd_space(1, norb, hf)
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """One HIP context on cuda:0 for the GPU tests (C-ABI, no fallback)."""
+    from macis_amd import capi
+
+    C = capi()
+    c = C.Context(0)
+    yield c
+    c.close()
+
+
+def announce(msg):
+    """Print the case name before GPU work so a crash can be attributed."""
+    print(f"\n[case] {msg}", flush=True)

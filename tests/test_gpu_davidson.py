@@ -1,0 +1,130 @@
+"""GPU parity of SpMV (K2), the MFMA tall-skinny GEMM (K3) and Davidson
+(K3-K5) against the oracle and the reference's known answers
+(test_spmbv.cxx:14-53, davidson.cxx:19-70)."""
+import numpy as np
+import pytest
+
+from macis_amd import dets as D
+from macis_amd.synthetic import seeded_integrals
+from oracle import oracle as O
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(float).eps
+
+
+@pytest.mark.parametrize("indexing", [0, 1])
+def test_spmv_known_answer(ctx, indexing):
+    announce(f"spmv 6x6 known answer indexing={indexing}")
+    i = indexing
+    rowptr = np.array([i, i + 2, i + 5, i + 8, i + 11, i + 14, i + 16])
+    colind = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4, 5, 4, 5]) + i
+    nzval = np.array([2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2], dtype=float)
+    A = ctx.csr_from_host(6, 6, rowptr, colind, nzval, indexing=i)
+    x = ctx.to_device(np.arange(1, 7, dtype=float))
+    y = ctx.to_device(np.ones(6))
+    ctx.spmv(A, 1, 1.0, x, 6, 0.0, y, 6)
+    assert np.array_equal(y.numpy(), [4, 8, 12, 16, 20, 17])
+    Dg = ctx.empty(6, np.float64)
+    ctx.extract_diagonal(A, Dg)
+    assert np.array_equal(Dg.numpy(), np.full(6, 2.0))
+
+
+def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd):
+    announce("spmv H2O CISD K=2 alpha/beta")
+    norb, _, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    N = len(h2o_cisd)
+    H = ctx.ham_build(1, ctx.to_device(h2o_cisd), N, thresh=1e-16)
+    rp, ci, nz = h2o_oracle.csr(1, h2o_cisd, thresh=1e-16)
+    rng = np.random.default_rng(1)
+    LD = N + 3
+    Vh = rng.standard_normal(2 * LD)
+    AVh = rng.standard_normal(2 * LD)
+    dV, dAV = ctx.to_device(Vh), ctx.to_device(AVh)
+    ctx.spmv(H, 2, 0.7, dV, LD, -1.3, dAV, LD)
+    ref = AVh.copy()
+    O.gespmbv(2, 0.7, N, rp, ci, nz, 0, Vh, LD, -1.3, ref, LD)
+    got = dAV.numpy()
+    scale = np.abs(nz).max() * np.abs(Vh).max() * 4000
+    assert np.max(np.abs(got - ref)) <= 1e-13 * scale
+    assert np.array_equal(got[N:LD], ref[N:LD])  # padding untouched
+    Dg = ctx.empty(N, np.float64)
+    ctx.extract_diagonal(H, Dg)
+    assert np.array_equal(Dg.numpy(), O.extract_diagonal(N, rp, ci, nz))
+
+
+@pytest.mark.parametrize("k,m", [(1, 1), (5, 1), (16, 16), (17, 17), (21, 21), (3, 29)])
+def test_gemm_tn_mfma(ctx, k, m):
+    announce(f"gemm_tn k={k} m={m}")
+    rng = np.random.default_rng(k * 100 + m)
+    N = 10007
+    A = rng.standard_normal((k, N))  # column-major N x k == row-major k x N
+    B = rng.standard_normal((m, N))
+    Cd = ctx.empty(k * m, np.float64)
+    ctx.gemm_tn(N, k, ctx.to_device(A), N, m, ctx.to_device(B), N, Cd, k)
+    C = Cd.numpy().reshape(m, k).T
+    ref = A @ B.T
+    assert np.allclose(C, ref, rtol=0, atol=1e-11 * np.sqrt(N))
+
+
+def test_davidson_h2o_cisd(ctx, h2o, h2o_oracle, h2o_cisd):
+    announce("davidson H2O CISD max_m 15 tol 1e-8 (reference semantics)")
+    norb, E_core, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    N = len(h2o_cisd)
+    H = ctx.ham_build(1, ctx.to_device(h2o_cisd), N, thresh=1e-16)
+    Dg = ctx.empty(N, np.float64)
+    ctx.extract_diagonal(H, Dg)
+    Dh = Dg.numpy()
+    X0 = np.zeros(N)
+    X0[np.argmin(Dh)] = 1.0
+    X = ctx.to_device(X0)
+    it, E0 = ctx.davidson(H, 15, 1e-8, X, Dg)
+    # davidson.cxx:62 and the oracle
+    assert E0 + E_core == pytest.approx(-76.23197835987, abs=1e-9)
+    rp, ci, nz = h2o_oracle.csr(1, h2o_cisd, thresh=1e-16)
+    ito, Eo, Xo = O.davidson(N, 15, rp, ci, nz, Dh, 1e-8, X0)
+    assert E0 == pytest.approx(Eo, abs=1e-10)
+    Xh = X.numpy()
+    assert np.linalg.norm(Xh) == pytest.approx(1.0, abs=1e-12)
+    assert abs(abs(Xh @ Xo) - 1.0) < 1e-9
+    AX = np.zeros(N)
+    O.gespmbv(1, 1.0, N, rp, ci, nz, 0, Xh, N, 0.0, AX, N)
+    assert Xh @ AX == pytest.approx(E0, abs=1e-9)
+    t = ctx.timings()
+    assert t["dav_spmv_calls"] == it + 1 and t["dav_restarts"] == 0
+
+
+def test_davidson_not_converged(ctx, h2o, h2o_cisd):
+    announce("davidson non-convergence error path")
+    from macis_amd import _capi
+
+    norb, _, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    N = len(h2o_cisd)
+    H = ctx.ham_build(1, ctx.to_device(h2o_cisd), N, thresh=1e-16)
+    X = ctx.empty(N, np.float64).zero()
+    Dg = ctx.empty(N, np.float64)
+    ctx.extract_diagonal(H, Dg)
+    X.upload(np.eye(1, N, int(np.argmin(Dg.numpy())))[0])
+    with pytest.raises(_capi.MhipError, match="Did Not Converge") as e:
+        ctx.davidson(H, 3, 1e-8, X, Dg)
+    assert e.value.code == _capi.MHIP_ERR_NOT_CONVERGED
+
+
+@pytest.mark.parametrize("n,ne,seed", [(8, 4, 108), (10, 5, 1)])
+def test_selected_ci_diag_restarted_vs_oracle(ctx, n, ne, seed):
+    announce(f"selected_ci_diag restarted CAS({2*ne},{n})")
+    T, V = seeded_integrals(n, seed)
+    ctx.set_integrals(n, T, V)
+    dets = D.generate_hilbert_space(1, n, ne, ne)
+    N = len(dets)
+    X = ctx.empty(N, np.float64).zero()
+    E, nnz, _ = ctx.selected_ci_diag(1, ctx.to_device(dets), N, X, max_restarts=100)
+    Eo, Xo, nnzo = O.Ham(T, V).selected_ci_diag(1, dets, max_restarts=100)
+    assert nnz == nnzo
+    assert E == pytest.approx(Eo, abs=1e-8)
+    assert abs(abs(X.numpy() @ Xo) - 1.0) < 1e-6
+    assert ctx.timings()["dav_residual"] < 1e-8

@@ -1,0 +1,170 @@
+"""GPU parity of the Hamiltonian build (K1) against the CPU oracle.
+
+The oracle restates DoubleLoopHamiltonianGenerator::make_csr_hamiltonian_block_
+(double_loop.hpp:30-98); the reference test csr_hamiltonian.cxx:18-71 pins
+the H2O CISD row pointers. Pattern (rowptr, colind) must be identical and
+values bit-identical (the device evaluates the same sums in the same order
+without FMA contraction; the north-star bound is 1e-12 relative).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from macis_amd import dets as D
+from macis_amd.synthetic import seeded_integrals, selected_ci_dets
+from oracle import oracle as O
+
+from conftest import GOLDEN, announce
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(float).eps
+
+
+def assert_same_csr(gpu, ref):
+    rp, ci, nz = gpu
+    rpo, cio, nzo = ref
+    assert np.array_equal(rp, rpo), "rowptr differs"
+    assert np.array_equal(ci, cio), "colind differs"
+    rel = np.abs(nz - nzo) / np.maximum(np.abs(nzo), 1e-300)
+    assert rel.max(initial=0.0) <= 1e-12
+    assert np.array_equal(nz, nzo), f"{np.count_nonzero(nz != nzo)} values not bit-identical"
+
+
+def test_h2o_cisd_matches_fixture_and_oracle(ctx, h2o, h2o_oracle, h2o_cisd):
+    announce("hamgen H2O CISD 12636 dets, thresh 1e-16")
+    norb, _, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    d = ctx.to_device(h2o_cisd)
+    H = ctx.ham_build(1, d, len(h2o_cisd), thresh=1e-16)
+    ref_rp = np.fromfile(os.path.join(GOLDEN, "h2o.ccpvdz.cisd.rowptr.bin"), dtype=np.int32)
+    rp32, ci32, nz32 = H.to_host_i32()
+    assert np.array_equal(rp32, ref_rp)
+    assert H.nnz == 3517816
+    assert_same_csr(H.to_host(), h2o_oracle.csr(1, h2o_cisd, thresh=1e-16))
+    t = ctx.timings()
+    assert t["ham_nnz"] == 3517816 and t["ham_nnz_struct"] >= t["ham_nnz"]
+    assert t["ham_compact_ms"] > 0  # symmetry zeros are dropped by compaction
+
+
+@pytest.mark.parametrize("W", [1, 2])
+@pytest.mark.parametrize("n,na,nb,seed", [(6, 3, 3, 1), (8, 4, 4, 2), (8, 5, 3, 3), (10, 3, 2, 4)])
+def test_fci_synthetic(ctx, W, n, na, nb, seed):
+    announce(f"hamgen FCI n={n} na={na} nb={nb} W={W}")
+    T, V = seeded_integrals(n, seed)
+    ctx.set_integrals(n, T, V)
+    dets = D.generate_hilbert_space(W, n, na, nb)
+    H = ctx.ham_build(W, ctx.to_device(dets), len(dets), thresh=EPS)
+    assert_same_csr(H.to_host(), O.Ham(T, V).csr(W, dets, thresh=EPS))
+
+
+@pytest.mark.parametrize("thresh", [0.0, 1e-3, 0.05, 1e3])
+def test_threshold_compaction(ctx, thresh):
+    announce(f"hamgen threshold {thresh}")
+    n = 8
+    T, V = seeded_integrals(n, 9)
+    ctx.set_integrals(n, T, V)
+    dets = D.generate_hilbert_space(1, n, 4, 3)
+    H = ctx.ham_build(1, ctx.to_device(dets), len(dets), thresh=thresh)
+    assert_same_csr(H.to_host(), O.Ham(T, V).csr(1, dets, thresh=thresh))
+
+
+def test_rectangular_blocks_zero_dets_and_empty(ctx):
+    announce("hamgen rectangular / zero dets / empty ranges")
+    n = 8
+    T, V = seeded_integrals(n, 5)
+    ctx.set_integrals(n, T, V)
+    orc = O.Ham(T, V)
+    dets = D.generate_hilbert_space(1, n, 4, 4)
+    rng = np.random.default_rng(0)
+    bra = dets[rng.permutation(len(dets))[:777]].copy()
+    ket = dets[::-1].copy()
+    bra[5] = 0   # zero bra -> empty row (double_loop.hpp:50)
+    ket[17] = 0  # zero ket -> never a column (double_loop.hpp:62)
+    H = ctx.ham_build(1, ctx.to_device(bra), len(bra), ctx.to_device(ket), len(ket), thresh=EPS)
+    assert (H.m, H.n) == (len(bra), len(ket))
+    assert_same_csr(H.to_host(), orc.csr(1, bra, ket, thresh=EPS))
+    # empty ranges give an empty matrix (csr_hamiltonian.hpp:30-35)
+    d = ctx.to_device(bra)
+    E = ctx.ham_build(1, d, 0, d, len(bra), thresh=EPS)
+    assert (E.m, E.n, E.nnz) == (0, len(bra), 0)
+    E = ctx.ham_build(1, d, 10, d, 0, thresh=EPS)
+    assert (E.m, E.n, E.nnz) == (10, 0, 0)
+    assert np.array_equal(E.to_host()[0], np.zeros(11, np.int64))
+
+
+def test_selected_ci_128bit(ctx):
+    announce("hamgen 64 orbitals 20a20b, 6000 sorted dets, W=2")
+    n = 64
+    T, V = seeded_integrals(n, 77)
+    ctx.set_integrals(n, T, V)
+    dets = selected_ci_dets(n, 20, 20, 6000, seed=3, T=T)
+    H = ctx.ham_build(2, ctx.to_device(dets), len(dets), thresh=EPS)
+    assert_same_csr(H.to_host(), O.Ham(T, V).csr(2, dets, thresh=EPS))
+
+
+def test_diagonal_entry_point(ctx):
+    announce("ham_diagonal")
+    n = 10
+    T, V = seeded_integrals(n, 8)
+    ctx.set_integrals(n, T, V)
+    orc = O.Ham(T, V)
+    dets = D.generate_hilbert_space(1, n, 5, 4)[:500]
+    out = ctx.empty(len(dets), np.float64)
+    ctx.ham_diagonal(1, ctx.to_device(dets), len(dets), out)
+    ref = np.array([orc.matrix_element(1, d, d) for d in dets])
+    assert np.array_equal(out.numpy(), ref)
+
+
+def test_cas10_sampled_rows(ctx):
+    announce("hamgen CAS(10e,10o) 63504 dets, sampled rows vs oracle")
+    n = 10
+    T, V = seeded_integrals(n, 10)
+    ctx.set_integrals(n, T, V)
+    dets = D.generate_hilbert_space(1, n, 5, 5)
+    H = ctx.ham_build(1, ctx.to_device(dets), len(dets), thresh=EPS)
+    assert H.nnz == 55629504
+    rp, ci, nz = H.to_host()
+    assert np.all(np.diff(rp) == 876)
+    orc = O.Ham(T, V)
+    for r0 in (0, 12345, 31752, 63504 - 40):
+        cnt, col, val = orc.csr_rows(1, dets, r0, r0 + 40, thresh=EPS)
+        assert np.array_equal(np.diff(rp[r0:r0 + 41]), cnt)
+        assert np.array_equal(ci[rp[r0]:rp[r0 + 40]], col)
+        assert np.array_equal(nz[rp[r0]:rp[r0 + 40]], val)
+    # pattern symmetry and ascending columns
+    assert np.all(np.diff(ci.reshape(-1, 876), axis=1) > 0)
+    rows = np.repeat(np.arange(len(dets), dtype=np.int64), 876)
+    key = rows * len(dets) + ci
+    keyT = ci.astype(np.int64) * len(dets) + rows
+    assert np.array_equal(np.sort(key), np.sort(keyT))
+
+
+def test_cas12_full_size_properties(ctx):
+    announce("hamgen CAS(12e,12o) 853776 dets full size")
+    import ctypes
+
+    from macis_amd import _capi
+
+    n = 12
+    T, V = seeded_integrals(n, 12)
+    ctx.set_integrals(n, T, V)
+    dets = D.generate_hilbert_space(1, n, 6, 6)
+    N = len(dets)
+    H = ctx.ham_build(1, ctx.to_device(dets), N, thresh=EPS)
+    assert H.nnz == 1553018544  # 1819 per row: no element at or below eps
+    d_rp, d_ci, d_nz = H.device_arrays()
+    L = _capi.lib()
+    rp = np.empty(N + 1, np.int64)
+    L.mhip_memcpy_d2h(ctx.h, rp.ctypes.data, d_rp, rp.nbytes)
+    assert np.all(np.diff(rp) == 1819)
+    orc = O.Ham(T, V)
+    for r0 in (0, 400000, N - 16):
+        cnt, col, val = orc.csr_rows(1, dets, r0, r0 + 16, thresh=EPS)
+        lo, hi = int(rp[r0]), int(rp[r0 + 16])
+        gc = np.empty(hi - lo, np.int32)
+        gv = np.empty(hi - lo, np.float64)
+        L.mhip_memcpy_d2h(ctx.h, gc.ctypes.data, d_ci + 4 * lo, gc.nbytes)
+        L.mhip_memcpy_d2h(ctx.h, gv.ctypes.data, d_nz + 8 * lo, gv.nbytes)
+        assert np.array_equal(gc, col)
+        assert np.array_equal(gv, val)
