@@ -225,6 +225,64 @@ int mhip_asci_search(mhip_context* ctx, int words_per_det,
                      const double* d_C, int64_t norb, uint64_t* d_new_dets,
                      int64_t* n_new);
 
+/* ---- batched CAS-CI (no counterpart in the reference, which solves one
+ * instance per compute_casci_rdms call, util/cas.hpp:32-64) ------------------
+ * ninst independent problems, each with its own norb (<= 32, wfn_t<64>),
+ * HOST determinants dets[det_offsets[b] .. det_offsets[b+1]) and HOST
+ * integrals T[b] (norb^2), V[b] (norb^4). All instances share one
+ * block-diagonal device CSR built by one launch per stage. */
+typedef struct mhip_batch mhip_batch;
+int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
+                      const int64_t* det_offsets, const uint64_t* dets,
+                      const double* const* T, const double* const* V,
+                      mhip_batch** out);
+int mhip_batch_destroy(mhip_batch* batch);
+int mhip_batch_info(const mhip_batch* batch, int64_t* ndets, int64_t* nnz);
+/* Hamiltonians of all instances with |h| > thresh (double_loop.hpp:30-98
+ * per instance). */
+int mhip_batch_build(mhip_context* ctx, mhip_batch* batch, double thresh);
+/* Autotuner: picks the SpMV lanes per row (8/16/32/64) and the rows per
+ * Hamiltonian work item for every instance-size bucket (largest first);
+ * lanes_out (optional) receives one entry per bucket. */
+int mhip_batch_tune(mhip_context* ctx, mhip_batch* batch, int32_t* lanes_out);
+/* Davidson of every instance in lockstep (davidson.hpp:128-225 per
+ * instance; max_restarts > 0 selects the restarted variant), iterations
+ * captured as hipGraphs when use_graphs != 0. E and residual (optional) are
+ * indexed by instance; ms_out (optional) gets the wall time in ms. Returns
+ * MHIP_ERR_NOT_CONVERGED if any instance did not converge. */
+int mhip_batch_davidson(mhip_context* ctx, mhip_batch* batch, int64_t max_m,
+                        int64_t max_restarts, double tol, int use_graphs,
+                        double* E, double* residual, double* ms_out);
+
+/* ---- learned ASCI pre-screen (no counterpart in the reference) ---------------
+ * An MLP 128 -> 512 -> 512 -> 1 with GELU on the occupation bits of a
+ * determinant (bits 0-63 alpha, 64-127 beta for wfn_t<128>; wfn_t<64> uses
+ * the 64 bits of its word), predicting log|rv| of asci_search scores. BF16
+ * operands on MFMA with FP32 accumulation; FP32 master weights, Adam.
+ * Parameters are one FP32 block: W1 (512x128, row-major) | b1 (512) |
+ * W2 (512x512) | b2 (512) | w3 (512) | b3 (1). */
+typedef struct mhip_mlp mhip_mlp;
+int mhip_mlp_create(mhip_context* ctx, uint64_t seed, mhip_mlp** out);
+int mhip_mlp_destroy(mhip_mlp* mlp);
+int64_t mhip_mlp_num_params(void);
+int mhip_mlp_get_params(mhip_context* ctx, const mhip_mlp* mlp, float* host);
+int mhip_mlp_set_params(mhip_context* ctx, mhip_mlp* mlp, const float* host);
+/* Gradients of the last mhip_mlp_train_step (same layout as the params). */
+int mhip_mlp_get_grads(mhip_context* ctx, const mhip_mlp* mlp, float* host);
+/* d_out[i] = prediction for DEVICE determinant i (FP32, DEVICE). */
+int mhip_mlp_predict(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
+                     const uint64_t* d_dets, int64_t n, float* d_out);
+/* One Adam step on the mean-squared error over a DEVICE mini-batch
+ * (n <= 4194304); lr == 0 forms the gradients only. */
+int mhip_mlp_train_step(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
+                        const uint64_t* d_dets, const float* d_target,
+                        int64_t n, float lr, float* loss);
+/* BF16 GEMM of the MLP kernels, for parity tests: C (FP32, M x N) = A B^T,
+ * A (M x K) and B (N x K) row-major BF16 bit patterns, all DEVICE; M and N
+ * multiples of 128, K of 32. */
+int mhip_gemm_bf16_nt(mhip_context* ctx, int M, int N, int K,
+                      const uint16_t* d_A, const uint16_t* d_B, float* d_C);
+
 #ifdef __cplusplus
 }
 #endif

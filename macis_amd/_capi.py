@@ -108,6 +108,21 @@ _SIGS = {
     "mhip_selected_ci_diag": (_i32, [_p, _i32, _p, _i64, _d, _i64, _i64, _d, _p, _pd, _pi64, _pp]),
     "mhip_asci_search": (_i32, [_p, _i32, C.POINTER(ASCISettings), _i64, _p, _i64, _d, _p, _i64,
                                 _p, _pi64]),
+    "mhip_mlp_create": (_i32, [_p, C.c_uint64, _pp]),
+    "mhip_mlp_destroy": (_i32, [_p]),
+    "mhip_mlp_num_params": (_i64, []),
+    "mhip_mlp_get_params": (_i32, [_p, _p, _p]),
+    "mhip_mlp_set_params": (_i32, [_p, _p, _p]),
+    "mhip_mlp_get_grads": (_i32, [_p, _p, _p]),
+    "mhip_mlp_predict": (_i32, [_p, _p, _i32, _p, _i64, _p]),
+    "mhip_mlp_train_step": (_i32, [_p, _p, _i32, _p, _p, _i64, C.c_float, C.POINTER(C.c_float)]),
+    "mhip_gemm_bf16_nt": (_i32, [_p, _i32, _i32, _i32, _p, _p, _p]),
+    "mhip_batch_create": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _pp]),
+    "mhip_batch_destroy": (_i32, [_p]),
+    "mhip_batch_info": (_i32, [_p, _pi64, _pi64]),
+    "mhip_batch_build": (_i32, [_p, _p, _d]),
+    "mhip_batch_tune": (_i32, [_p, _p, _p]),
+    "mhip_batch_davidson": (_i32, [_p, _p, _i64, _i64, _d, _i32, _p, _p, _pd]),
 }
 
 _lib = None
@@ -333,6 +348,94 @@ class Context:
                                      ncdets, E_ASCI, _ptr(d_C), norb, _ptr(d_out), C.byref(n)),
               "asci_search")
         return n.value
+
+
+class Batch:
+    """Batched CAS-CI instances (mhip_batch)."""
+
+    def __init__(self, ctx, norbs, det_lists, Ts, Vs):
+        self.ctx = ctx
+        n = len(norbs)
+        norb = np.ascontiguousarray(norbs, dtype=np.int32)
+        offs = np.zeros(n + 1, dtype=np.int64)
+        offs[1:] = np.cumsum([len(d) for d in det_lists])
+        dets = np.ascontiguousarray(np.concatenate([np.asarray(d).reshape(-1) for d in det_lists]),
+                                    dtype=np.uint64)
+        self._T = [np.ascontiguousarray(t, dtype=np.float64) for t in Ts]
+        self._V = [np.ascontiguousarray(v, dtype=np.float64) for v in Vs]
+        Tp = (C.c_void_p * n)(*[t.ctypes.data for t in self._T])
+        Vp = (C.c_void_p * n)(*[v.ctypes.data for v in self._V])
+        p = C.c_void_p()
+        check(lib().mhip_batch_create(ctx.h, n, norb.ctypes.data, offs.ctypes.data,
+                                      dets.ctypes.data, Tp, Vp, C.byref(p)), "batch_create")
+        self.h = p.value
+        self.ninst = n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mhip_batch_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().mhip_batch_info(self.h, C.byref(a), C.byref(b)), "batch_info")
+        return a.value, b.value
+
+    def build(self, thresh):
+        check(lib().mhip_batch_build(self.ctx.h, self.h, thresh), "batch_build")
+
+    def tune(self):
+        lanes = np.zeros(self.ninst, np.int32)
+        check(lib().mhip_batch_tune(self.ctx.h, self.h, lanes.ctypes.data), "batch_tune")
+        return lanes
+
+    def davidson(self, max_m=20, max_restarts=0, tol=1e-8, use_graphs=True):
+        E = np.zeros(self.ninst)
+        res = np.zeros(self.ninst)
+        ms = C.c_double()
+        check(lib().mhip_batch_davidson(self.ctx.h, self.h, max_m, max_restarts, tol,
+                                        int(use_graphs), E.ctypes.data, res.ctypes.data,
+                                        C.byref(ms)), "batch_davidson")
+        return E, res, ms.value
+
+
+class MLP:
+    """Learned pre-screen network (mhip_mlp)."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        p = C.c_void_p()
+        check(lib().mhip_mlp_create(ctx.h, seed, C.byref(p)), "mlp_create")
+        self.h = p.value
+        self.nparam = lib().mhip_mlp_num_params()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mhip_mlp_destroy(self.h)
+            self.h = None
+
+    def params(self):
+        out = np.empty(self.nparam, np.float32)
+        check(lib().mhip_mlp_get_params(self.ctx.h, self.h, out.ctypes.data), "get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().mhip_mlp_set_params(self.ctx.h, self.h, p.ctypes.data), "set_params")
+
+    def grads(self):
+        out = np.empty(self.nparam, np.float32)
+        check(lib().mhip_mlp_get_grads(self.ctx.h, self.h, out.ctypes.data), "get_grads")
+        return out
+
+    def predict(self, W, d_dets, n, d_out):
+        check(lib().mhip_mlp_predict(self.ctx.h, self.h, W, _ptr(d_dets), n, _ptr(d_out)), "predict")
+
+    def train_step(self, W, d_dets, d_target, n, lr):
+        loss = C.c_float()
+        check(lib().mhip_mlp_train_step(self.ctx.h, self.h, W, _ptr(d_dets), _ptr(d_target), n,
+                                        lr, C.byref(loss)), "train_step")
+        return loss.value
 
 
 def _ptr(x):

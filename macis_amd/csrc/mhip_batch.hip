@@ -1,0 +1,721 @@
+// Batched CAS-CI: many independent instances of uneven size solved together
+// (no reference counterpart; the reference runs one compute_casci_rdms per
+// instance, util/cas.hpp:32-64).
+//
+// All instances share one concatenated determinant array; their
+// Hamiltonians form one block-diagonal CSR (absolute column indices) built
+// by a single launch per stage of the Hamiltonian kernels, driven by a
+// work-item table {rows of one instance, that instance's ket range} and a
+// per-instance integral table. Davidson runs all instances in lockstep
+// (same subspace index, same restart points), which is exactly the
+// per-instance algorithm: every reduction is segmented per instance and the
+// k x k eigenproblems are solved on the device (one wavefront per instance,
+// cyclic Jacobi), so an iteration has no host round trip and is captured
+// once per subspace size as a hipGraph and replayed. Converged instances
+// are frozen (their rows are skipped by the SpMV and the vector kernels).
+// An autotuner picks, per instance-size bucket, the lanes per row of the
+// SpMV and the rows per work item of the Hamiltonian scan.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "mhip_device.hpp"
+#include "mhip_internal.hpp"
+
+namespace mhip {
+namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+constexpr int kMaxK = 32;
+
+// ---- SpMV over a row range with a lanes-per-row schedule, skipping rows
+// of converged instances.
+template <int LPR>
+__global__ __launch_bounds__(256) void bspmv_kernel(
+    int64_t r0, int64_t r1, const int64_t* __restrict__ rowptr,
+    const int32_t* __restrict__ colind, const double* __restrict__ nzval,
+    const double* __restrict__ x, double* __restrict__ y, const int32_t* __restrict__ row_inst,
+    const int32_t* __restrict__ conv) {
+  constexpr int RPW = 64 / LPR;  // rows per wavefront
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / LPR, sl = lane % LPR;
+  const int64_t row = r0 + ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + sub;
+  const bool active = row < r1 && !(conv && conv[row_inst[row]]);
+  double acc = 0.;
+  if(active) {
+    const int64_t st = rowptr[row], en = rowptr[row + 1];
+    for(int64_t j = st + sl; j < en; j += LPR)
+      acc = fma(__builtin_nontemporal_load(nzval + j), x[__builtin_nontemporal_load(colind + j)], acc);
+  }
+#pragma unroll
+  for(int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if(active && sl == 0) y[row] = acc;
+}
+
+// ---- per-instance reductions: one workgroup per instance -----------------
+struct Seg {
+  const int64_t* __restrict__ off;  // [ninst + 1] row offsets
+  const int32_t* __restrict__ conv;  // [ninst]
+};
+
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double s = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return s;
+}
+
+// C_b = V_b^T AV_b (k x k, ld kMaxK) on MFMA; one workgroup per instance.
+__global__ __launch_bounds__(256) void bgram_kernel(Seg sg, int k, const double* __restrict__ V,
+                                                    const double* __restrict__ AV, int64_t ld,
+                                                    double* __restrict__ C) {
+  const int b = blockIdx.x;
+  if(sg.conv[b]) return;
+  const int64_t lo = sg.off[b], hi = sg.off[b + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  d4_t acc[2][2];
+  for(int i = 0; i < 2; ++i)
+    for(int j = 0; j < 2; ++j) acc[i][j] = d4_t{0., 0., 0., 0.};
+  for(int64_t n0 = lo + wv * 32; n0 < hi; n0 += 4 * 32) {
+    const int64_t nb = n0 + grp * 8;
+    double a[2][8], bb[2][8];
+    for(int i = 0; i < 2; ++i) {
+      const int c = i * 16 + col;
+      for(int s = 0; s < 8; ++s) {
+        const bool ok = c < k && nb + s < hi;
+        a[i][s] = ok ? V[(int64_t)c * ld + nb + s] : 0.;
+        bb[i][s] = ok ? AV[(int64_t)c * ld + nb + s] : 0.;
+      }
+    }
+    for(int s = 0; s < 8; ++s)
+      for(int i = 0; i < 2; ++i)
+        for(int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], bb[j][s], acc[i][j], 0, 0, 0);
+  }
+  __shared__ double red[4][1024];
+  for(int i = 0; i < 2; ++i)
+    for(int j = 0; j < 2; ++j)
+      for(int r = 0; r < 4; ++r) red[wv][(i * 2 + j) * 256 + r * 64 + lane] = acc[i][j][r];
+  __syncthreads();
+  for(int e = threadIdx.x; e < 1024; e += 256) {
+    const int tile = e / 256, rem = e % 256, r = rem / 64, ln = rem % 64;
+    const int i = (tile / 2) * 16 + 4 * (ln / 16) + r, j = (tile % 2) * 16 + ln % 16;
+    if(i < k && j < k)
+      C[(int64_t)b * kMaxK * kMaxK + i + j * kMaxK] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+  }
+}
+
+// Lowest eigenpair of each C_b (lower triangle) by cyclic Jacobi in LDS,
+// one wavefront per instance: lam[b], coef[b][:k].
+__global__ __launch_bounds__(64) void bjacobi_kernel(Seg sg, int k, const double* __restrict__ C,
+                                                     double* __restrict__ lam,
+                                                     double* __restrict__ coef) {
+  const int b = blockIdx.x;
+  if(sg.conv[b]) return;
+  __shared__ double a[kMaxK][kMaxK + 1], v[kMaxK][kMaxK + 1];
+  const int t = threadIdx.x;
+  for(int e = t; e < k * k; e += 64) {
+    const int i = e % k, j = e / k;
+    const double x = i >= j ? C[(int64_t)b * kMaxK * kMaxK + i + j * kMaxK]
+                            : C[(int64_t)b * kMaxK * kMaxK + j + i * kMaxK];
+    a[i][j] = x;
+    v[i][j] = i == j ? 1. : 0.;
+  }
+  __syncthreads();
+  for(int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0., tot = 0.;
+    for(int e = t; e < k * k; e += 64) {
+      const int i = e % k, j = e / k;
+      tot += a[i][j] * a[i][j];
+      if(i != j) off += a[i][j] * a[i][j];
+    }
+    off = wave_sum(off);
+    tot = wave_sum(tot);
+    if(off <= 1e-32 * tot || off == 0.) break;
+    for(int p = 0; p < k - 1; ++p)
+      for(int q = p + 1; q < k; ++q) {
+        const double apq = a[p][q];
+        if(apq == 0.) continue;
+        const double tau = (a[q][q] - a[p][p]) / (2. * apq);
+        const double tt = (tau >= 0. ? 1. : -1.) / (fabs(tau) + sqrt(1. + tau * tau));
+        const double c = 1. / sqrt(1. + tt * tt), s = tt * c;
+        __syncthreads();
+        if(t < k) {
+          const double akp = a[t][p], akq = a[t][q];
+          a[t][p] = c * akp - s * akq;
+          a[t][q] = s * akp + c * akq;
+          const double vkp = v[t][p], vkq = v[t][q];
+          v[t][p] = c * vkp - s * vkq;
+          v[t][q] = s * vkp + c * vkq;
+        }
+        __syncthreads();
+        if(t < k) {
+          const double apk = a[p][t], aqk = a[q][t];
+          a[p][t] = c * apk - s * aqk;
+          a[q][t] = s * apk + c * aqk;
+        }
+        __syncthreads();
+      }
+  }
+  // lowest eigenvalue (first on ties)
+  int best = 0;
+  for(int i = 1; i < k; ++i)
+    if(a[i][i] < a[best][best]) best = i;
+  if(t == 0) lam[b] = a[best][best];
+  if(t < kMaxK) coef[(int64_t)b * kMaxK + t] = t < k ? v[t][best] : 0.;
+}
+
+// X = V c_b; R = AV c_b - lam_b X (rows of unconverged instances).
+__global__ __launch_bounds__(256) void bresidual_kernel(
+    int64_t N, int k, const double* __restrict__ V, const double* __restrict__ AV, int64_t ld,
+    const int32_t* __restrict__ row_inst, const int32_t* __restrict__ conv,
+    const double* __restrict__ coef, const double* __restrict__ lam, double* __restrict__ X,
+    double* __restrict__ R) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    const int b = row_inst[n];
+    if(conv[b]) continue;
+    const double* c = coef + (int64_t)b * kMaxK;
+    double x = 0., r = 0.;
+    for(int j = 0; j < k; ++j) {
+      x = fma(V[n + j * ld], c[j], x);
+      r = fma(AV[n + j * ld], c[j], r);
+    }
+    X[n] = x;
+    R[n] = r + (-lam[b]) * x;
+  }
+}
+
+// out[b*stride + j] = V_j . w over instance b's rows, j < k (k <= 32).
+__global__ __launch_bounds__(256) void bdots_kernel(Seg sg, int k, const double* __restrict__ V,
+                                                    int64_t ld, const double* __restrict__ w,
+                                                    double* __restrict__ out) {
+  const int b = blockIdx.x;
+  if(sg.conv[b]) return;
+  __shared__ double lds[4];
+  const int64_t lo = sg.off[b], hi = sg.off[b + 1];
+  for(int j = 0; j < k; ++j) {
+    double s = 0.;
+    for(int64_t n = lo + threadIdx.x; n < hi; n += 256) s = fma(V[n + j * ld], w[n], s);
+    s = block_sum(s, lds);
+    if(threadIdx.x == 0) out[(int64_t)b * kMaxK + j] = s;
+  }
+}
+
+// w -= V c_b (CGS update) over unconverged rows.
+__global__ __launch_bounds__(256) void bupdate_kernel(int64_t N, int k, const double* __restrict__ V,
+                                                      int64_t ld, const int32_t* __restrict__ row_inst,
+                                                      const int32_t* __restrict__ conv,
+                                                      const double* __restrict__ coef,
+                                                      double* __restrict__ w) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    const int b = row_inst[n];
+    if(conv[b]) continue;
+    const double* c = coef + (int64_t)b * kMaxK;
+    double s = 0.;
+    for(int j = 0; j < k; ++j) s = fma(V[n + j * ld], c[j], s);
+    w[n] = w[n] - s;
+  }
+}
+
+// R = -R / (D - lam_b)
+__global__ void bprecond_kernel(int64_t N, double* __restrict__ R, const double* __restrict__ D,
+                                const int32_t* __restrict__ row_inst,
+                                const int32_t* __restrict__ conv, const double* __restrict__ lam) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    const int b = row_inst[n];
+    if(!conv[b]) R[n] = -R[n] / (D[n] - lam[b]);
+  }
+}
+
+// w *= 1/sqrt(nrm2_b)
+__global__ void bscale_kernel(int64_t N, double* __restrict__ w, const int32_t* __restrict__ row_inst,
+                              const int32_t* __restrict__ conv, const double* __restrict__ nrm2) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    const int b = row_inst[n];
+    if(!conv[b]) w[n] *= 1. / sqrt(nrm2[(int64_t)b * kMaxK]);
+  }
+}
+
+// Convergence check after the residual norms: converged instances record E
+// and are frozen; counts the unconverged ones.
+__global__ void bcheck_kernel(int ninst, int32_t* __restrict__ conv, const double* __restrict__ rn2,
+                              const double* __restrict__ lam, double tol, double* __restrict__ E,
+                              double* __restrict__ res, int32_t* __restrict__ nleft) {
+  __shared__ int cnt;
+  if(threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for(int b = threadIdx.x; b < ninst; b += blockDim.x) {
+    if(conv[b]) continue;
+    const double r = sqrt(rn2[(int64_t)b * kMaxK]);
+    res[b] = r;
+    E[b] = lam[b];
+    if(r < tol)
+      conv[b] = 1;
+    else
+      ++mine;
+  }
+  atomicAdd(&cnt, mine);
+  __syncthreads();
+  if(threadIdx.x == 0) nleft[0] = cnt;
+}
+
+// Unit guess at the first minimum of D per instance; zeroes X elsewhere.
+__global__ __launch_bounds__(256) void bguess_kernel(Seg sg, const double* __restrict__ D,
+                                                     double* __restrict__ X) {
+  const int b = blockIdx.x;
+  const int64_t lo = sg.off[b], hi = sg.off[b + 1];
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  double mn = INFINITY;
+  int64_t mi = INT64_MAX;
+  for(int64_t n = lo + threadIdx.x; n < hi; n += 256) {
+    X[n] = 0.;
+    if(D[n] < mn) {
+      mn = D[n];
+      mi = n;
+    }
+  }
+  sv[threadIdx.x] = mn;
+  si[threadIdx.x] = mi;
+  __syncthreads();
+  for(int o = 128; o > 0; o >>= 1) {
+    if(threadIdx.x < o) {
+      const int q = threadIdx.x + o;
+      if(sv[q] < sv[threadIdx.x] || (sv[q] == sv[threadIdx.x] && si[q] < si[threadIdx.x])) {
+        sv[threadIdx.x] = sv[q];
+        si[threadIdx.x] = si[q];
+      }
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x == 0 && si[0] < hi) X[si[0]] = 1.;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = v;
+}
+
+__global__ void copy_unconv_kernel(int64_t N, const double* __restrict__ src, double* __restrict__ dst,
+                                   const int32_t* __restrict__ row_inst,
+                                   const int32_t* __restrict__ conv) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256)
+    if(!conv[row_inst[n]]) dst[n] = src[n];
+}
+
+}  // namespace
+}  // namespace mhip
+
+struct mhip_batch {
+  int device = 0;
+  int ninst = 0;
+  int64_t N = 0;
+  std::vector<int> order;          // internal position -> caller's instance index
+  std::vector<int64_t> off;        // internal row offsets [ninst + 1]
+  std::vector<int32_t> norb;       // internal order
+  mhip::DevBuf dets, row_inst, d_off, views, ints;
+  std::unique_ptr<mhip_csr> H;
+  // autotuned schedule per size bucket: rows [r0, r1), lanes per row
+  struct Bucket {
+    int64_t r0, r1, ndets;
+    int lpr;
+    int rows_per_item;
+  };
+  std::vector<Bucket> buckets;
+  bool tuned = false;
+  // Davidson workspace
+  int64_t max_m = 0;
+  mhip::DevBuf V, AV, X, D, C, coef, lam, tmp, conv, E, res, nleft;
+  std::map<int, hipGraphExec_t> graphs;
+  ~mhip_batch() {
+    for(auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  }
+};
+
+namespace mhip {
+namespace {
+
+void launch_spmv(mhip_context* ctx, mhip_batch* B, const double* x, double* y, bool masked) {
+  const int32_t* conv = masked ? B->conv.as<int32_t>() : nullptr;
+  for(auto& bk : B->buckets) {
+    const int64_t rows = bk.r1 - bk.r0;
+    if(rows <= 0) continue;
+    const int rpw = 64 / bk.lpr;
+    const int64_t waves = (rows + rpw - 1) / rpw;
+    const dim3 grid((waves + 3) / 4);
+#define MHIP_BSPMV(L)                                                                           \
+  hipLaunchKernelGGL(bspmv_kernel<L>, grid, dim3(256), 0, ctx->stream, bk.r0, bk.r1,            \
+                     B->H->rowptr.as<int64_t>(), B->H->colind.as<int32_t>(),                    \
+                     B->H->nzval.as<double>(), x, y, B->row_inst.as<int32_t>(), conv)
+    if(bk.lpr == 8)
+      MHIP_BSPMV(8);
+    else if(bk.lpr == 16)
+      MHIP_BSPMV(16);
+    else if(bk.lpr == 32)
+      MHIP_BSPMV(32);
+    else
+      MHIP_BSPMV(64);
+#undef MHIP_BSPMV
+  }
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+int red_grid(int64_t N) { return (int)std::max<int64_t>(1, std::min<int64_t>(2048, (N + 255) / 256)); }
+
+// CGS2 of w against V(:, :k) per instance, then per-instance normalisation.
+void bgram_schmidt(mhip_context* ctx, mhip_batch* B, int k, double* w) {
+  hipStream_t st = ctx->stream;
+  Seg sg{B->d_off.as<int64_t>(), B->conv.as<int32_t>()};
+  const int64_t N = B->N, ld = B->N;
+  for(int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, k, B->V.as<double>(),
+                       ld, w, B->tmp.as<double>());
+    hipLaunchKernelGGL(bupdate_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, k,
+                       B->V.as<double>(), ld, B->row_inst.as<int32_t>(), B->conv.as<int32_t>(),
+                       B->tmp.as<double>(), w);
+  }
+  hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, 1, w, ld, w,
+                     B->tmp.as<double>());
+  hipLaunchKernelGGL(bscale_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, w,
+                     B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->tmp.as<double>());
+}
+
+// One Davidson iteration with subspace index i (k = i + 1), all on device.
+void biteration(mhip_context* ctx, mhip_batch* B, int i, double tol) {
+  hipStream_t st = ctx->stream;
+  const int k = i + 1;
+  const int64_t N = B->N, ld = B->N;
+  Seg sg{B->d_off.as<int64_t>(), B->conv.as<int32_t>()};
+  double* V = B->V.as<double>();
+  double* AV = B->AV.as<double>();
+  launch_spmv(ctx, B, V + i * ld, AV + i * ld, true);
+  hipLaunchKernelGGL(bgram_kernel, dim3(B->ninst), dim3(256), 0, st, sg, k, V, AV, ld,
+                     B->C.as<double>());
+  hipLaunchKernelGGL(bjacobi_kernel, dim3(B->ninst), dim3(64), 0, st, sg, k, B->C.as<double>(),
+                     B->lam.as<double>(), B->coef.as<double>());
+  double* R = V + (i + 1) * ld;
+  hipLaunchKernelGGL(bresidual_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, k, V, AV, ld,
+                     B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->coef.as<double>(),
+                     B->lam.as<double>(), B->X.as<double>(), R);
+  hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, 1, R, ld, R,
+                     B->tmp.as<double>());
+  hipLaunchKernelGGL(bcheck_kernel, dim3(1), dim3(256), 0, st, B->ninst, B->conv.as<int32_t>(),
+                     B->tmp.as<double>(), B->lam.as<double>(), tol, B->E.as<double>(),
+                     B->res.as<double>(), B->nleft.as<int32_t>());
+  hipLaunchKernelGGL(bprecond_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, R, B->D.as<double>(),
+                     B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->lam.as<double>());
+  bgram_schmidt(ctx, B, k, R);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" {
+
+int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
+                      const int64_t* det_offsets, const uint64_t* dets, const double* const* T,
+                      const double* const* V, mhip_batch** out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(out && ninst > 0 && norb && det_offsets && dets && T && V,
+                 MHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<mhip_batch> B(new mhip_batch());
+    B->device = ctx->device;
+    B->ninst = ninst;
+    // internal order: by size, largest first (contiguous size buckets)
+    B->order.resize(ninst);
+    std::iota(B->order.begin(), B->order.end(), 0);
+    std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) {
+      return det_offsets[a + 1] - det_offsets[a] > det_offsets[b + 1] - det_offsets[b];
+    });
+    B->off.assign(ninst + 1, 0);
+    B->norb.resize(ninst);
+    for(int p = 0; p < ninst; ++p) {
+      const int b = B->order[p];
+      const int64_t n = det_offsets[b + 1] - det_offsets[b];
+      MHIP_REQUIRE(n > 0, MHIP_ERR_INVALID_ARG, "empty instance");
+      MHIP_REQUIRE(norb[b] >= 1 && norb[b] <= 32, MHIP_ERR_INVALID_ARG,
+                   "batched instances use wfn_t<64> (norb <= 32)");
+      B->off[p + 1] = B->off[p] + n;
+      B->norb[p] = norb[b];
+    }
+    const int64_t N = B->N = B->off[ninst];
+    MHIP_REQUIRE(N < (1ll << 31), MHIP_ERR_INDEX_OVERFLOW, "too many determinants");
+    std::vector<uint64_t> hd(N);
+    std::vector<int32_t> ri(N);
+    for(int p = 0; p < ninst; ++p) {
+      const int b = B->order[p];
+      const uint64_t bad = ~((1ull << B->norb[p]) - 1ull);
+      for(int64_t i = 0; i < B->off[p + 1] - B->off[p]; ++i) {
+        const uint64_t d = dets[det_offsets[b] + i];
+        MHIP_REQUIRE(!(((d & 0xffffffffull) | (d >> 32)) & bad), MHIP_ERR_INVALID_ARG,
+                     "determinant occupies an orbital >= norb");
+        hd[B->off[p] + i] = d;
+        ri[B->off[p] + i] = p;
+      }
+    }
+    B->dets.alloc(8 * N);
+    B->row_inst.alloc(4 * N);
+    B->d_off.alloc(8 * (ninst + 1));
+    MHIP_HIP_CHECK(hipMemcpyAsync(B->dets.p, hd.data(), 8 * N, hipMemcpyHostToDevice, ctx->stream));
+    MHIP_HIP_CHECK(hipMemcpyAsync(B->row_inst.p, ri.data(), 4 * N, hipMemcpyHostToDevice, ctx->stream));
+    MHIP_HIP_CHECK(hipMemcpyAsync(B->d_off.p, B->off.data(), 8 * (ninst + 1), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    // integrals + intermediates per instance, packed (impl.hpp:25-70)
+    size_t total = 0;
+    std::vector<size_t> ioff(ninst);
+    for(int p = 0; p < ninst; ++p) {
+      const size_t n = B->norb[p];
+      ioff[p] = total;
+      total += 2 * n * n + 2 * n * n * n * n + 2 * n * n * n + 2 * n * n;
+    }
+    std::vector<double> packed(total);
+    std::vector<IntsView> hv(ninst);
+    B->ints.alloc(8 * total);
+    double* dbase = B->ints.as<double>();
+    for(int p = 0; p < ninst; ++p) {
+      const int b = B->order[p];
+      const size_t n = B->norb[p], n2 = n * n, n3 = n2 * n, n4 = n3 * n;
+      double* h = packed.data() + ioff[p];
+      double *hT = h, *hV = hT + n2, *hG = hV + n4, *hGr = hG + n4, *hVr = hGr + n3,
+             *hG2 = hVr + n3, *hV2 = hG2 + n2;
+      std::memcpy(hT, T[b], 8 * n2);
+      std::memcpy(hV, V[b], 8 * n4);
+      auto idx = [&](size_t i, size_t j, size_t kk, size_t l) { return i + n * j + n2 * kk + n3 * l; };
+      for(size_t l = 0; l < n; ++l)
+        for(size_t kk = 0; kk < n; ++kk)
+          for(size_t j = 0; j < n; ++j)
+            for(size_t i = 0; i < n; ++i) hG[idx(i, j, kk, l)] = hV[idx(i, j, kk, l)] - hV[idx(i, l, kk, j)];
+      for(size_t j = 0; j < n; ++j)
+        for(size_t i = 0; i < n; ++i)
+          for(size_t kk = 0; kk < n; ++kk) {
+            hGr[kk + n * i + n2 * j] = hG[idx(kk, kk, i, j)];
+            hVr[kk + n * i + n2 * j] = hV[idx(kk, kk, i, j)];
+          }
+      for(size_t j = 0; j < n; ++j)
+        for(size_t i = 0; i < n; ++i) {
+          hG2[i + n * j] = 0.5 * hG[idx(i, i, j, j)];
+          hV2[i + n * j] = hV[idx(i, i, j, j)];
+        }
+      double* d = dbase + ioff[p];
+      hv[p].n = (int)n;
+      hv[p].T = d;
+      hv[p].V = d + n2;
+      hv[p].G = d + n2 + n4;
+      hv[p].G_red = d + n2 + 2 * n4;
+      hv[p].V_red = d + n2 + 2 * n4 + n3;
+      hv[p].G2_red = d + n2 + 2 * n4 + 2 * n3;
+      hv[p].V2_red = d + 2 * n2 + 2 * n4 + 2 * n3;
+    }
+    MHIP_HIP_CHECK(hipMemcpyAsync(B->ints.p, packed.data(), 8 * total, hipMemcpyHostToDevice, ctx->stream));
+    B->views.alloc(sizeof(IntsView) * ninst);
+    MHIP_HIP_CHECK(hipMemcpyAsync(B->views.p, hv.data(), sizeof(IntsView) * ninst,
+                                  hipMemcpyHostToDevice, ctx->stream));
+    // size buckets (instances with equal determinant counts are contiguous)
+    for(int p = 0; p < ninst;) {
+      int q = p;
+      while(q < ninst && B->off[q + 1] - B->off[q] == B->off[p + 1] - B->off[p]) ++q;
+      B->buckets.push_back({B->off[p], B->off[q], B->off[p + 1] - B->off[p], 64, 8});
+      p = q;
+    }
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = B.release();
+  });
+}
+
+int mhip_batch_destroy(mhip_batch* B) {
+  return guarded([&] {
+    if(!B) return;
+    (void)hipSetDevice(B->device);
+    (void)hipDeviceSynchronize();
+    delete B;
+  });
+}
+
+int mhip_batch_info(const mhip_batch* B, int64_t* ndets, int64_t* nnz) {
+  return guarded([&] {
+    MHIP_REQUIRE(B, MHIP_ERR_INVALID_ARG, "null batch");
+    if(ndets) *ndets = B->N;
+    if(nnz) *nnz = B->H ? B->H->nnz : 0;
+  });
+}
+
+// Builds the block-diagonal Hamiltonian of all instances.
+int mhip_batch_build(mhip_context* ctx, mhip_batch* B, double thresh) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B, MHIP_ERR_INVALID_ARG, "null batch");
+    // work items: rows of one instance, in groups of the bucket's size
+    std::vector<int64_t> items;
+    for(auto& bk : B->buckets)
+      for(int64_t r = bk.r0; r < bk.r1; r += bk.ndets)
+        for(int64_t q = r; q < r + bk.ndets; q += bk.rows_per_item) {
+          items.push_back(q);
+          items.push_back(std::min<int64_t>(bk.rows_per_item, r + bk.ndets - q));
+          items.push_back(r);
+          items.push_back(r + bk.ndets);
+        }
+    DevBuf ditems(8 * items.size());
+    MHIP_HIP_CHECK(hipMemcpyAsync(ditems.p, items.data(), 8 * items.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    BuildExtra ex;
+    ex.items = ditems.as<int64_t>();
+    ex.nitems = (int64_t)items.size() / 4;
+    ex.views = B->views.as<IntsView>();
+    ex.row_inst = B->row_inst.as<int32_t>();
+    B->H.reset(new mhip_csr());
+    B->H->device = ctx->device;
+    B->H->m = B->H->n = B->N;
+    build_impl<1>(ctx, B->dets.as<uint64_t>(), B->N, B->dets.as<uint64_t>(), B->N, thresh,
+                  B->H.get(), ex);
+  });
+}
+
+// Autotuner: per size bucket, time the SpMV with 8/16/32/64 lanes per row
+// on the built matrix and keep the fastest. Rows per scan work item are set
+// from the same choice (short rows -> more rows per item).
+int mhip_batch_tune(mhip_context* ctx, mhip_batch* B, int32_t* lanes_out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B && B->H, MHIP_ERR_INVALID_ARG, "build the batch first");
+    DevBuf x(8 * B->N), y(8 * B->N);
+    MHIP_HIP_CHECK(hipMemsetAsync(x.p, 0, 8 * B->N, ctx->stream));
+    Timer t(ctx->stream);
+    int bi = 0;
+    for(auto& bk : B->buckets) {
+      double best = 1e30;
+      int bestl = 64;
+      for(int l : {8, 16, 32, 64}) {
+        bk.lpr = l;
+        std::vector<mhip_batch::Bucket> saved = B->buckets;
+        B->buckets = {bk};
+        launch_spmv(ctx, B, x.as<double>(), y.as<double>(), false);  // warm
+        t.start();
+        for(int r = 0; r < 3; ++r) launch_spmv(ctx, B, x.as<double>(), y.as<double>(), false);
+        t.stop();
+        const double ms = t.ms();
+        B->buckets = saved;
+        if(ms < best) {
+          best = ms;
+          bestl = l;
+        }
+      }
+      bk.lpr = bestl;
+      bk.rows_per_item = bestl >= 32 ? 8 : 4;
+      if(lanes_out) lanes_out[bi] = bestl;
+      ++bi;
+    }
+    B->tuned = true;
+  });
+}
+
+// Lockstep Davidson over all instances (per-instance algorithm of
+// davidson.hpp:128-225 with the restart variant when max_restarts > 0).
+// E / iters / residual are per CALLER instance index. Returns
+// MHIP_ERR_NOT_CONVERGED if some instance did not converge.
+int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t max_restarts,
+                        double tol, int use_graphs, double* E, double* residual,
+                        double* ms_out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B && B->H && E, MHIP_ERR_INVALID_ARG, "build the batch first");
+    int64_t min_n = B->N;
+    for(int p = 0; p < B->ninst; ++p) min_n = std::min(min_n, B->off[p + 1] - B->off[p]);
+    MHIP_REQUIRE(max_m >= 2 && max_m < kMaxK && max_m <= min_n, MHIP_ERR_INVALID_ARG,
+                 "max_m must be in [2, 31] and <= the smallest instance");
+    hipStream_t st = ctx->stream;
+    const int64_t N = B->N, ld = B->N;
+    if(B->max_m != max_m) {
+      for(auto& g : B->graphs) (void)hipGraphExecDestroy(g.second);
+      B->graphs.clear();
+      B->max_m = max_m;
+      B->V.alloc(8 * N * (max_m + 1));
+      B->AV.alloc(8 * N * (max_m + 1));
+      B->X.alloc(8 * N);
+      B->D.alloc(8 * N);
+      B->C.alloc(8 * (size_t)B->ninst * kMaxK * kMaxK);
+      B->coef.alloc(8 * (size_t)B->ninst * kMaxK);
+      B->lam.alloc(8 * B->ninst);
+      B->tmp.alloc(8 * (size_t)B->ninst * kMaxK);
+      B->conv.alloc(4 * B->ninst);
+      B->E.alloc(8 * B->ninst);
+      B->res.alloc(8 * B->ninst);
+      B->nleft.alloc(4);
+    }
+    Timer tall(st);
+    tall.start();
+    Seg sg{B->d_off.as<int64_t>(), B->conv.as<int32_t>()};
+    double* V = B->V.as<double>();
+    double* AV = B->AV.as<double>();
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((B->ninst + 255) / 256), dim3(256), 0, st,
+                       B->conv.as<int32_t>(), (int64_t)B->ninst, 0);
+    extract_diagonal(ctx, B->H.get(), B->D.as<double>());
+    hipLaunchKernelGGL(bguess_kernel, dim3(B->ninst), dim3(256), 0, st, sg, B->D.as<double>(),
+                       B->X.as<double>());
+    MHIP_HIP_CHECK(hipMemcpyAsync(V, B->X.p, 8 * N, hipMemcpyDeviceToDevice, st));
+    launch_spmv(ctx, B, V, AV, false);
+    MHIP_HIP_CHECK(hipMemcpyAsync(V + ld, AV, 8 * N, hipMemcpyDeviceToDevice, st));
+    bgram_schmidt(ctx, B, 1, V + ld);
+    int32_t nleft = B->ninst;
+    for(int64_t rst = 0;; ++rst) {
+      for(int i = 1; i < max_m && nleft; ++i) {
+        if(use_graphs) {
+          auto it = B->graphs.find(i);
+          if(it == B->graphs.end()) {
+            hipGraph_t g;
+            MHIP_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            biteration(ctx, B, i, tol);
+            MHIP_HIP_CHECK(hipStreamEndCapture(st, &g));
+            hipGraphExec_t ge;
+            MHIP_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            it = B->graphs.emplace(i, ge).first;
+          }
+          MHIP_HIP_CHECK(hipGraphLaunch(it->second, st));
+        } else {
+          biteration(ctx, B, i, tol);
+        }
+        MHIP_HIP_CHECK(hipMemcpyAsync(&nleft, B->nleft.p, 4, hipMemcpyDeviceToHost, st));
+        MHIP_HIP_CHECK(hipStreamSynchronize(st));
+      }
+      if(!nleft || rst >= max_restarts) break;
+      // restart unconverged instances: V0 = X/|X|, V1 = V(:,max_m), AV0 = A V0
+      hipLaunchKernelGGL(copy_unconv_kernel, dim3(red_grid(N)), dim3(256), 0, st, N,
+                         B->X.as<double>(), V, B->row_inst.as<int32_t>(), B->conv.as<int32_t>());
+      hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, 1, V, ld, V,
+                         B->tmp.as<double>());
+      hipLaunchKernelGGL(bscale_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, V,
+                         B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->tmp.as<double>());
+      hipLaunchKernelGGL(copy_unconv_kernel, dim3(red_grid(N)), dim3(256), 0, st, N,
+                         V + max_m * ld, V + ld, B->row_inst.as<int32_t>(), B->conv.as<int32_t>());
+      launch_spmv(ctx, B, V, AV, true);
+      MHIP_HIP_CHECK(hipGetLastError());
+    }
+    tall.stop();
+    const double ms = tall.ms();
+    if(ms_out) *ms_out = ms;
+    std::vector<double> hE(B->ninst), hr(B->ninst);
+    MHIP_HIP_CHECK(hipMemcpyAsync(hE.data(), B->E.p, 8 * B->ninst, hipMemcpyDeviceToHost, st));
+    MHIP_HIP_CHECK(hipMemcpyAsync(hr.data(), B->res.p, 8 * B->ninst, hipMemcpyDeviceToHost, st));
+    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    for(int p = 0; p < B->ninst; ++p) {
+      E[B->order[p]] = hE[p];
+      if(residual) residual[B->order[p]] = hr[p];
+    }
+    if(nleft) throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");
+  });
+}
+
+}  // extern "C"

@@ -86,6 +86,10 @@ __device__ __forceinline__ int excitation_bound(const uint64_t (&b)[W],
 }
 
 // Structural count (FILL = false) or fill (FILL = true) pass.
+// Without a work-item table, wave w owns rows [w*R, w*R+R) and scans all
+// kets. With one (batched instances), item w = {row0, nrows, ket_lo, ket_hi}
+// names rows of one instance and the ket range of that instance; columns
+// stay absolute indices, so a batch yields one block-diagonal CSR.
 template <int W, int R, bool FILL>
 __global__ __launch_bounds__(256) void scan_pass_kernel(
     const uint64_t* __restrict__ bra, int64_t nbra,
@@ -93,8 +97,9 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
     const uint64_t* __restrict__ tand, const uint64_t* __restrict__ tor,
     int64_t ntiles, const uint64_t* __restrict__ sand,
     const uint64_t* __restrict__ sor, int64_t nsuper, int64_t nwaves,
-    int64_t nblocks, uint32_t* __restrict__ row_counts,
-    const int64_t* __restrict__ rowptr, int32_t* __restrict__ colind) {
+    int64_t nblocks, const int64_t* __restrict__ items,
+    uint32_t* __restrict__ row_counts, const int64_t* __restrict__ rowptr,
+    int32_t* __restrict__ colind) {
   // XCD-aware mapping: hardware dispatches block b to XCD b % 8; logical
   // blocks (consecutive rows, which share spin strings and therefore ket
   // tiles) are laid out so each XCD's L2 sees a contiguous range of rows.
@@ -104,7 +109,14 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
   const int64_t wave = lblk * kWavesPerBlock + (threadIdx.x >> 6);
   if(wave >= nwaves) return;
   const uint32_t lane = threadIdx.x & 63;
-  const int64_t row0 = wave * R;
+  int64_t row0 = wave * R, nrows = R, klo = 0, khi = nket;
+  if(items) {
+    row0 = (int64_t)uniform_u64((uint64_t)items[4 * wave + 0]);
+    nrows = (int64_t)uniform_u64((uint64_t)items[4 * wave + 1]);
+    klo = (int64_t)uniform_u64((uint64_t)items[4 * wave + 2]);
+    khi = (int64_t)uniform_u64((uint64_t)items[4 * wave + 3]);
+  }
+  const int64_t rend = row0 + nrows < nbra ? row0 + nrows : nbra;
 
   uint64_t b[R][W];
   uint32_t live = 0;
@@ -116,28 +128,30 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
     uint64_t nz = 0;
 #pragma unroll
     for(int w = 0; w < W; ++w) {
-      b[r][w] = row < nbra ? uniform_u64(bra[row * W + w]) : 0ull;
+      b[r][w] = row < rend ? uniform_u64(bra[row * W + w]) : 0ull;
       nz |= b[r][w];
     }
     if(nz) live |= 1u << r;  // zero bras give empty rows (double_loop.hpp:50)
     cnt[r] = 0;
     base[r] = 0;
     if constexpr(FILL) {
-      if(row < nbra) base[r] = (int64_t)uniform_u64((uint64_t)rowptr[row]);
+      if(row < rend) base[r] = (int64_t)uniform_u64((uint64_t)rowptr[row]);
     }
   }
-  if(!live) {
+  if(!live || khi <= klo) {
     if constexpr(!FILL) {
       if(lane == 0)
         for(int r = 0; r < R; ++r)
-          if(row0 + r < nbra) row_counts[row0 + r] = 0;
+          if(row0 + r < rend) row_counts[row0 + r] = 0;
     }
     return;
   }
+  const int64_t s_lo = klo / (kTile * kSuper), s_hi = (khi + kTile * kSuper - 1) / (kTile * kSuper);
+  const int64_t t_lo = klo / kTile, t_hi = (khi + kTile - 1) / kTile;
 
-  for(int64_t g = 0; g < nsuper; g += 64) {
+  for(int64_t g = s_lo; g < s_hi; g += 64) {
     const int64_t s = g + lane;
-    const bool sv = s < nsuper;
+    const bool sv = s < s_hi;
     uint64_t sa[W], so[W];
 #pragma unroll
     for(int w = 0; w < W; ++w) {
@@ -156,7 +170,7 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
       sany &= sany - 1;
       const int64_t S = g + sb;
       const int64_t t = S * kSuper + lane;
-      const bool tv = t < ntiles;
+      const bool tv = t >= t_lo && t < t_hi;
       uint64_t ta[W], to[W];
 #pragma unroll
       for(int w = 0; w < W; ++w) {
@@ -176,7 +190,7 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
         const int64_t j = (S * kSuper + tb) * kTile + lane;
         uint64_t k[W];
         uint64_t knz = 0;
-        const bool kv = j < nket;
+        const bool kv = j >= klo && j < khi;
 #pragma unroll
         for(int w = 0; w < W; ++w) {
           k[w] = kv ? ket[j * W + w] : 0ull;
@@ -204,7 +218,7 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
   if constexpr(!FILL) {
     if(lane == 0)
       for(int r = 0; r < R; ++r)
-        if(row0 + r < nbra) row_counts[row0 + r] = cnt[r];
+        if(row0 + r < rend) row_counts[row0 + r] = cnt[r];
   }
 }
 
@@ -238,11 +252,14 @@ __global__ void widen_counts_kernel(const uint32_t* __restrict__ c, int64_t n,
 
 // Diagonal element of every determinant (thread per determinant).
 template <int W>
-__global__ __launch_bounds__(256) void diag_kernel(IntsView I,
+__global__ __launch_bounds__(256) void diag_kernel(IntsView I0,
                                                     const uint64_t* __restrict__ dets,
-                                                    int64_t n, double* __restrict__ diag) {
+                                                    int64_t n, double* __restrict__ diag,
+                                                    const IntsView* __restrict__ views,
+                                                    const int32_t* __restrict__ row_inst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
+  const IntsView I = views ? views[row_inst[i]] : I0;
   const Spins<W> s = load_spins<W>(dets, i);
   diag[i] = me_diag(I, s.a, s.b);
 }
@@ -250,13 +267,15 @@ __global__ __launch_bounds__(256) void diag_kernel(IntsView I,
 // Element pass: one wavefront per bra row.
 template <int W>
 __global__ __launch_bounds__(256) void element_kernel(
-    IntsView I, const uint64_t* __restrict__ bra, int64_t nbra,
+    IntsView I0, const uint64_t* __restrict__ bra, int64_t nbra,
     const uint64_t* __restrict__ ket, const double* __restrict__ diag,
     const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     double* __restrict__ nzval, double thresh, int64_t* __restrict__ kept,
-    int* __restrict__ dropped) {
+    int* __restrict__ dropped, const IntsView* __restrict__ views,
+    const int32_t* __restrict__ row_inst) {
   const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if(row >= nbra) return;
+  const IntsView I = views ? views[row_inst[row]] : I0;
   const uint32_t lane = threadIdx.x & 63;
   const int64_t st = rowptr[row], en = rowptr[row + 1];
   if(st == en) {
@@ -306,6 +325,8 @@ __global__ __launch_bounds__(256) void compact_kernel(
   }
 }
 
+}  // namespace
+
 IntsView view(const Integrals& I) {
   IntsView v;
   v.n = (int)I.norb;
@@ -319,6 +340,7 @@ IntsView view(const Integrals& I) {
   return v;
 }
 
+namespace {
 // Exclusive scan of n+1 int64 values (in[n] must be 0) into out.
 void exclusive_scan(mhip_context* ctx, const int64_t* in, int64_t* out,
                     int64_t n1) {
@@ -332,10 +354,12 @@ void exclusive_scan(mhip_context* ctx, const int64_t* in, int64_t* out,
                                          rocprim::plus<int64_t>(), ctx->stream));
 }
 
+}  // namespace
+
 template <int W>
 void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
                 const uint64_t* d_ket, int64_t nket, double thresh,
-                mhip_csr* csr) {
+                mhip_csr* csr, const BuildExtra& ex) {
   hipStream_t st = ctx->stream;
   mhip_timings& tm = ctx->timings;
   Timer t_all(st), t(st);
@@ -360,7 +384,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   tm.ham_bounds_ms = t.ms();
 
   // 2. count
-  const int64_t nwaves = (nbra + kRows - 1) / kRows;
+  const int64_t nwaves = ex.items ? ex.nitems : (nbra + kRows - 1) / kRows;
   const int64_t nblocks = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
   const int64_t grid = (nblocks + 7) / 8 * 8;
   MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many bra rows");
@@ -369,7 +393,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   t.start();
   hipLaunchKernelGGL((scan_pass_kernel<W, kRows, false>), dim3(grid), dim3(256), 0, st,
                      d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
-                     nwaves, nblocks, counts32.as<uint32_t>(), nullptr, nullptr);
+                     nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr);
   MHIP_HIP_CHECK(hipGetLastError());
   t.stop();
   tm.ham_count_ms = t.ms();
@@ -395,7 +419,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   if(nnz_struct)
     hipLaunchKernelGGL((scan_pass_kernel<W, kRows, true>), dim3(grid), dim3(256), 0, st,
                        d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
-                       nwaves, nblocks, nullptr, rp_struct.as<int64_t>(),
+                       nwaves, nblocks, ex.items, nullptr, rp_struct.as<int64_t>(),
                        col.as<int32_t>());
   MHIP_HIP_CHECK(hipGetLastError());
   t.stop();
@@ -408,12 +432,12 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
   t.start();
   hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
-                     nbra, diag.as<double>());
+                     nbra, diag.as<double>(), ex.views, ex.row_inst);
   if(nnz_struct)
     hipLaunchKernelGGL(element_kernel<W>, dim3((nbra + 3) / 4), dim3(256), 0, st, I, d_bra,
                        nbra, d_ket, diag.as<double>(), rp_struct.as<int64_t>(),
                        col.as<int32_t>(), val.as<double>(), thresh,
-                       counts64.as<int64_t>(), flag.as<int>());
+                       counts64.as<int64_t>(), flag.as<int>(), ex.views, ex.row_inst);
   MHIP_HIP_CHECK(hipGetLastError());
   int dropped = 0;
   MHIP_HIP_CHECK(hipMemcpyAsync(&dropped, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -455,6 +479,12 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   tm.ham_nnz = csr->nnz;
 }
 
+template void build_impl<1>(mhip_context*, const uint64_t*, int64_t, const uint64_t*, int64_t,
+                            double, mhip_csr*, const BuildExtra&);
+template void build_impl<2>(mhip_context*, const uint64_t*, int64_t, const uint64_t*, int64_t,
+                            double, mhip_csr*, const BuildExtra&);
+
+namespace {
 void check_dets(int W, const uint64_t* p, int64_t n, const char* what) {
   MHIP_REQUIRE(W == 1 || W == 2, MHIP_ERR_UNSUPPORTED,
                "words_per_det must be 1 (wfn_t<64>) or 2 (wfn_t<128>)");
@@ -496,9 +526,9 @@ mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
   validate_dets(ctx, W, d_bra, nbra);
   if(d_ket != d_bra || nket != nbra) validate_dets(ctx, W, d_ket, nket);
   if(W == 1)
-    build_impl<1>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get());
+    build_impl<1>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), BuildExtra{});
   else
-    build_impl<2>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get());
+    build_impl<2>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), BuildExtra{});
   return csr.release();
 }
 
@@ -512,10 +542,10 @@ void ham_diagonal(mhip_context* ctx, int W, const uint64_t* d_dets,
   const IntsView I = view(ctx->ints);
   if(W == 1)
     hipLaunchKernelGGL(diag_kernel<1>, dim3((ndets + 255) / 256), dim3(256), 0, ctx->stream,
-                       I, d_dets, ndets, d_diag);
+                       I, d_dets, ndets, d_diag, nullptr, nullptr);
   else
     hipLaunchKernelGGL(diag_kernel<2>, dim3((ndets + 255) / 256), dim3(256), 0, ctx->stream,
-                       I, d_dets, ndets, d_diag);
+                       I, d_dets, ndets, d_diag, nullptr, nullptr);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 

@@ -164,6 +164,18 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
               int64_t max_restarts, double tol, const double* d_D, double* d_X,
               double* E, int64_t* iters);
 void validate_dets(mhip_context* ctx, int W, const uint64_t* d, int64_t n);
+struct IntsView;
+IntsView view(const Integrals& I);
+// Optional work-item table / per-instance integrals of a batched build.
+struct BuildExtra {
+  const int64_t* items = nullptr;  // DEVICE, 4 int64 per item
+  int64_t nitems = 0;
+  const IntsView* views = nullptr;  // DEVICE, one per instance
+  const int32_t* row_inst = nullptr;  // DEVICE, instance of every row
+};
+template <int W>
+void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra, const uint64_t* d_ket,
+                int64_t nket, double thresh, mhip_csr* csr, const BuildExtra& ex);
 // Host symmetric eigensolver on the lower triangle (cyclic Jacobi).
 void syev_lower(int n, double* A, int lda, double* W);
 }  // namespace mhip

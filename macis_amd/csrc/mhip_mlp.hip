@@ -1,0 +1,490 @@
+// Learned ASCI pre-screen: an MLP 128 -> 512 -> 512 -> 1 (GELU) on the
+// occupation bits of a determinant, predicting log|rv| (no counterpart in
+// the reference; the plain search of determinant_search.hpp:394-663 stays
+// the judge and the fallback).
+//
+// Compute: BF16 operands on MFMA (v_mfma_f32_32x32x16_bf16, gfx950), FP32
+// accumulation, FP32 master weights updated with Adam. One tiled GEMM
+// kernel serves the forward pass (C = A B^T, both K-contiguous), and the
+// backward products (A B with B N-contiguous, A^T B over the batch): tiles
+// are staged through LDS in K-contiguous layout whatever the global layout.
+// Epilogues fuse the bias, the GELU and the GELU derivative.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "mhip_device.hpp"
+#include "mhip_internal.hpp"
+
+namespace mhip {
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kIn = 128, kHid = 512;
+constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+
+__device__ __forceinline__ float gelu(float z) {
+  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+  return 0.5f * z * (1.f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+  const float t = tanhf(u);
+  return 0.5f * (1.f + t) +
+         0.5f * z * (1.f - t * t) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * z * z);
+}
+
+enum Epi : int {
+  EPI_F32 = 0,        // C (fp32)
+  EPI_BIAS_GELU = 1,  // Z = C + bias (fp32), A = gelu(Z) (bf16)
+  EPI_GELU_GRAD = 2,  // D = C * gelu'(Zaux) (bf16)
+};
+
+// C[m][n] = sum_k A(m,k) B(n,k); A(m,k) = AK ? A[m*lda+k] : A[k*lda+m],
+// B(n,k) = BK ? B[n*ldb+k] : B[k*ldb+n]. M, N multiples of 128, K of 32.
+template <bool AK, bool BKC, int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(
+    int M, int N, int K, const __bf16* __restrict__ A, int lda, const __bf16* __restrict__ B,
+    int ldb, const float* __restrict__ bias, const float* __restrict__ zaux,
+    float* __restrict__ Cf, __bf16* __restrict__ Cb, int ldc) {
+  __shared__ __bf16 As[BM][LDK];
+  __shared__ __bf16 Bs[BN][LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  f32x16_t acc[2][2];
+#pragma unroll
+  for(int i = 0; i < 2; ++i)
+#pragma unroll
+    for(int j = 0; j < 2; ++j)
+#pragma unroll
+      for(int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+  for(int k0 = 0; k0 < K; k0 += BK) {
+    // ---- stage A and B tiles (K-contiguous in LDS)
+#pragma unroll
+    for(int it = 0; it < 2; ++it) {
+      const int c = tid + 256 * it;
+      if constexpr(AK) {
+        const int row = c >> 2, kc = (c & 3) * 8;
+        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(m0 + row) * lda + k0 + kc);
+        *reinterpret_cast<bf16x8_t*>(&As[row][kc]) = v;
+      } else {
+        const int kk = c >> 4, mc = (c & 15) * 8;
+        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(k0 + kk) * lda + m0 + mc);
+#pragma unroll
+        for(int e = 0; e < 8; ++e) As[mc + e][kk] = v[e];
+      }
+      if constexpr(BKC) {
+        const int row = c >> 2, kc = (c & 3) * 8;
+        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(n0 + row) * ldb + k0 + kc);
+        *reinterpret_cast<bf16x8_t*>(&Bs[row][kc]) = v;
+      } else {
+        const int kk = c >> 4, nc = (c & 15) * 8;
+        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(k0 + kk) * ldb + n0 + nc);
+#pragma unroll
+        for(int e = 0; e < 8; ++e) Bs[nc + e][kk] = v[e];
+      }
+    }
+    __syncthreads();
+    // ---- MFMA: 2 x 2 tiles of 32 x 32, two K sub-steps of 16
+#pragma unroll
+    for(int ks = 0; ks < BK; ks += 16) {
+      bf16x8_t a[2], b[2];
+      const int kq = ks + (lane >> 5) * 8;
+#pragma unroll
+      for(int i = 0; i < 2; ++i)
+        a[i] = *reinterpret_cast<const bf16x8_t*>(&As[wm + i * 32 + (lane & 31)][kq]);
+#pragma unroll
+      for(int j = 0; j < 2; ++j)
+        b[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[wn + j * 32 + (lane & 31)][kq]);
+#pragma unroll
+      for(int i = 0; i < 2; ++i)
+#pragma unroll
+        for(int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // ---- epilogue; lane l, register r holds C(8*(r/4) + 4*(l/32) + r%4, l%32)
+#pragma unroll
+  for(int i = 0; i < 2; ++i)
+#pragma unroll
+    for(int j = 0; j < 2; ++j)
+#pragma unroll
+      for(int r = 0; r < 16; ++r) {
+        const int row = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int col = n0 + wn + j * 32 + (lane & 31);
+        const size_t o = (size_t)row * ldc + col;
+        float v = acc[i][j][r];
+        if constexpr(EPI == EPI_F32) {
+          Cf[o] = v;
+        } else if constexpr(EPI == EPI_BIAS_GELU) {
+          v += bias[col];
+          Cf[o] = v;
+          Cb[o] = (__bf16)gelu(v);
+        } else {
+          Cb[o] = (__bf16)(v * gelu_grad(zaux[o]));
+        }
+      }
+}
+
+// Occupation bits -> bf16 features (W words per det, 64 bits per word;
+// features beyond 64*W are zero).
+template <int W>
+__global__ void features_kernel(const uint64_t* __restrict__ dets, int64_t n, int64_t npad,
+                                __bf16* __restrict__ X) {
+  const int64_t i = (int64_t)blockIdx.x * 2 + (threadIdx.x >> 7);
+  const int f = threadIdx.x & 127;
+  if(i >= npad) return;
+  float v = 0.f;
+  if(i < n && f < 64 * W) v = (float)((dets[i * W + (f >> 6)] >> (f & 63)) & 1ull);
+  X[i * kIn + f] = (__bf16)v;
+}
+
+// y = A2 . w3 + b3; with targets, dy = 2 (y - t) / B and partial losses.
+__global__ __launch_bounds__(256) void head_kernel(const __bf16* __restrict__ A2,
+                                                   const float* __restrict__ w3,
+                                                   const float* __restrict__ b3, int64_t n,
+                                                   float* __restrict__ y,
+                                                   const float* __restrict__ t,
+                                                   float* __restrict__ dy, float inv_b,
+                                                   float* __restrict__ loss_part) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  float l = 0.f;
+  if(row < n) {
+    float s = 0.f;
+    for(int j = lane; j < kHid; j += 64) s += (float)A2[row * kHid + j] * w3[j];
+    s = wave_sum(s) + b3[0];
+    if(lane == 0) {
+      y[row] = s;
+      if(t) {
+        const float e = s - t[row];
+        dy[row] = 2.f * e * inv_b;
+        l = e * e;
+      }
+    }
+  }
+  if(t) {
+    __shared__ float red[4];
+    if(lane == 0) red[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if(threadIdx.x == 0) loss_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+// dZ2 = (dy * w3) * gelu'(Z2) as bf16.
+__global__ void head_back_kernel(const float* __restrict__ dy, const float* __restrict__ w3,
+                                 const float* __restrict__ Z2, int64_t n,
+                                 __bf16* __restrict__ dZ2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n * kHid) return;
+  const int j = (int)(i % kHid);
+  dZ2[i] = (__bf16)(dy[i / kHid] * w3[j] * gelu_grad(Z2[i]));
+}
+
+// Column sums: out[j] = sum_b M[b][j] (bf16 rows) -- bias gradients; and
+// dw3[j] = sum_b A2[b][j] dy[b].
+__global__ void colsum_kernel(const __bf16* __restrict__ Mt, int64_t n, int ncol,
+                              const float* __restrict__ wrow, float* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if(j >= ncol) return;
+  float s = 0.f;
+  for(int64_t b = 0; b < n; ++b) s += (float)Mt[b * ncol + j] * (wrow ? wrow[b] : 1.f);
+  out[j] = s;
+}
+
+__global__ void sum_kernel(const float* __restrict__ v, int64_t n, float* __restrict__ out) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for(int64_t i = threadIdx.x; i < n; i += 256) s += v[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int off = 128; off > 0; off >>= 1) {
+    if(threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) out[0] = red[0];
+}
+
+// Adam on fp32 master weights, bf16 shadow for compute.
+__global__ void adam_kernel(float* __restrict__ w, __bf16* __restrict__ wb,
+                            const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int64_t n, float lr, float b1, float b2,
+                            float c1, float c2, float eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if(wb) wb[i] = (__bf16)wi;
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ a, __bf16* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) b[i] = (__bf16)a[i];
+}
+
+}  // namespace
+
+// ---- host side -----------------------------------------------------------
+struct Mlp {
+  // parameter layout in one fp32 block: W1 (512x128) | b1 | W2 (512x512) |
+  // b2 | w3 (512) | b3 (1)
+  static constexpr int64_t oW1 = 0, ob1 = oW1 + kHid * kIn, oW2 = ob1 + kHid,
+                           ob2 = oW2 + (int64_t)kHid * kHid, ow3 = ob2 + kHid, ob3 = ow3 + kHid,
+                           nparam = ob3 + 1;
+  DevBuf p, g, m, v, pb;  // fp32 params, grads, Adam moments, bf16 params
+  int64_t step = 0;
+  // activation workspace for up to `cap` rows
+  int64_t cap = 0;
+  DevBuf X, Z1, A1, Z2, A2, y, dy, dZ2, dZ1, dA1, loss;
+  void reserve(int64_t rows) {
+    if(rows <= cap) return;
+    cap = rows;
+    X.alloc(2 * rows * kIn);
+    Z1.alloc(4 * rows * kHid);
+    A1.alloc(2 * rows * kHid);
+    Z2.alloc(4 * rows * kHid);
+    A2.alloc(2 * rows * kHid);
+    y.alloc(4 * rows);
+    dy.alloc(4 * rows);
+    dZ2.alloc(2 * rows * kHid);
+    dZ1.alloc(2 * rows * kHid);
+    dA1.alloc(4 * rows * kHid);
+    loss.alloc(4 * (rows / 4 + 2));
+  }
+};
+
+}  // namespace mhip
+
+struct mhip_mlp {
+  mhip::Mlp net;
+  int device = 0;
+};
+
+namespace mhip {
+namespace {
+
+template <bool AK, bool BKC, int EPI>
+void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
+          const float* bias, const float* zaux, float* Cf, __bf16* Cb, int ldc) {
+  MHIP_REQUIRE(M % BM == 0 && N % BN == 0 && K % BK == 0, MHIP_ERR_INVALID_ARG,
+               "gemm dims must be multiples of 128/128/32");
+  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BKC, EPI>), dim3(N / BN, M / BM), dim3(256), 0, st, M,
+                     N, K, A, lda, B, ldb, bias, zaux, Cf, Cb, ldc);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+const __bf16* bfp(const DevBuf& b, int64_t off = 0) { return b.as<__bf16>() + off; }
+
+// Forward pass for rows [0, npad) already featurised into net.X.
+void forward(hipStream_t st, Mlp& net, int64_t npad) {
+  const float* P = net.p.as<float>();
+  gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
+                                  bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr,
+                                  net.Z1.as<float>(), net.A1.as<__bf16>(), kHid);
+  gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
+                                  bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, nullptr,
+                                  net.Z2.as<float>(), net.A2.as<__bf16>(), kHid);
+}
+
+template <int W>
+void featurize(hipStream_t st, Mlp& net, const uint64_t* dets, int64_t n, int64_t npad) {
+  hipLaunchKernelGGL(features_kernel<W>, dim3((npad + 1) / 2), dim3(256), 0, st, dets, n, npad,
+                     net.X.as<__bf16>());
+}
+
+}  // namespace
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" {
+
+int mhip_mlp_create(mhip_context* ctx, uint64_t seed, mhip_mlp** out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(out, MHIP_ERR_INVALID_ARG, "null output");
+    std::unique_ptr<mhip_mlp> h(new mhip_mlp());
+    h->device = ctx->device;
+    Mlp& net = h->net;
+    // He-style init from a fixed seed (host), biases zero.
+    std::vector<float> p(Mlp::nparam, 0.f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    for(int64_t i = 0; i < kHid * kIn; ++i) p[Mlp::oW1 + i] = nd(rng) * std::sqrt(2.f / kIn);
+    for(int64_t i = 0; i < (int64_t)kHid * kHid; ++i) p[Mlp::oW2 + i] = nd(rng) * std::sqrt(2.f / kHid);
+    for(int64_t i = 0; i < kHid; ++i) p[Mlp::ow3 + i] = nd(rng) * std::sqrt(1.f / kHid);
+    net.p.alloc(4 * Mlp::nparam);
+    net.g.alloc(4 * Mlp::nparam);
+    net.m.alloc(4 * Mlp::nparam);
+    net.v.alloc(4 * Mlp::nparam);
+    net.pb.alloc(2 * Mlp::nparam);
+    MHIP_HIP_CHECK(hipMemcpyAsync(net.p.p, p.data(), 4 * Mlp::nparam, hipMemcpyHostToDevice, ctx->stream));
+    MHIP_HIP_CHECK(hipMemsetAsync(net.m.p, 0, 4 * Mlp::nparam, ctx->stream));
+    MHIP_HIP_CHECK(hipMemsetAsync(net.v.p, 0, 4 * Mlp::nparam, ctx->stream));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((Mlp::nparam + 255) / 256), dim3(256), 0, ctx->stream,
+                       net.p.as<float>(), net.pb.as<__bf16>(), Mlp::nparam);
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = h.release();
+  });
+}
+
+int mhip_mlp_destroy(mhip_mlp* mlp) {
+  return guarded([&] {
+    if(!mlp) return;
+    (void)hipSetDevice(mlp->device);
+    (void)hipDeviceSynchronize();
+    delete mlp;
+  });
+}
+
+int64_t mhip_mlp_num_params(void) { return Mlp::nparam; }
+
+int mhip_mlp_get_params(mhip_context* ctx, const mhip_mlp* mlp, float* host) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(mlp && host, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_HIP_CHECK(hipMemcpyAsync(host, mlp->net.p.p, 4 * Mlp::nparam, hipMemcpyDeviceToHost, ctx->stream));
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mhip_mlp_set_params(mhip_context* ctx, mhip_mlp* mlp, const float* host) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(mlp && host, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_HIP_CHECK(hipMemcpyAsync(mlp->net.p.p, host, 4 * Mlp::nparam, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((Mlp::nparam + 255) / 256), dim3(256), 0, ctx->stream,
+                       mlp->net.p.as<float>(), mlp->net.pb.as<__bf16>(), Mlp::nparam);
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mhip_mlp_get_grads(mhip_context* ctx, const mhip_mlp* mlp, float* host) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(mlp && host, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_HIP_CHECK(hipMemcpyAsync(host, mlp->net.g.p, 4 * Mlp::nparam, hipMemcpyDeviceToHost, ctx->stream));
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mhip_mlp_predict(mhip_context* ctx, mhip_mlp* mlp, int words_per_det, const uint64_t* d_dets,
+                     int64_t n, float* d_out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(mlp && (n == 0 || (d_dets && d_out)), MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_REQUIRE(words_per_det == 1 || words_per_det == 2, MHIP_ERR_UNSUPPORTED, "bad W");
+    Mlp& net = mlp->net;
+    const int64_t chunk = 1 << 20;
+    net.reserve(std::min<int64_t>((n + 127) / 128 * 128, chunk));
+    for(int64_t s = 0; s < n; s += chunk) {
+      const int64_t nb = std::min(chunk, n - s), npad = (nb + 127) / 128 * 128;
+      if(words_per_det == 1)
+        featurize<1>(ctx->stream, net, d_dets + s, nb, npad);
+      else
+        featurize<2>(ctx->stream, net, d_dets + 2 * s, nb, npad);
+      forward(ctx->stream, net, npad);
+      const float* P = net.p.as<float>();
+      hipLaunchKernelGGL(head_kernel, dim3((nb + 3) / 4), dim3(256), 0, ctx->stream,
+                         net.A2.as<__bf16>(), P + Mlp::ow3, P + Mlp::ob3, nb, d_out + s, nullptr,
+                         nullptr, 0.f, nullptr);
+      MHIP_HIP_CHECK(hipGetLastError());
+    }
+  });
+}
+
+// One optimisation step on a mini-batch (DEVICE dets and fp32 targets);
+// returns the batch MSE. With lr == 0 only gradients are formed.
+int mhip_mlp_train_step(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
+                        const uint64_t* d_dets, const float* d_target, int64_t n, float lr,
+                        float* loss_out) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(mlp && d_dets && d_target && n > 0, MHIP_ERR_INVALID_ARG, "bad argument");
+    MHIP_REQUIRE(words_per_det == 1 || words_per_det == 2, MHIP_ERR_UNSUPPORTED, "bad W");
+    Mlp& net = mlp->net;
+    hipStream_t st = ctx->stream;
+    const int64_t npad = (n + 127) / 128 * 128;
+    MHIP_REQUIRE(npad <= (1 << 22), MHIP_ERR_INVALID_ARG, "batch too large");
+    net.reserve(npad);
+    if(words_per_det == 1)
+      featurize<1>(st, net, d_dets, n, npad);
+    else
+      featurize<2>(st, net, d_dets, n, npad);
+    forward(st, net, npad);
+    float* P = net.p.as<float>();
+    float* G = net.g.as<float>();
+    // padded rows must not contribute: dy = 0 there
+    MHIP_HIP_CHECK(hipMemsetAsync(net.dy.p, 0, 4 * npad, st));
+    const int64_t nblk = (n + 3) / 4;
+    hipLaunchKernelGGL(head_kernel, dim3(nblk), dim3(256), 0, st, net.A2.as<__bf16>(),
+                       P + Mlp::ow3, P + Mlp::ob3, n, net.y.as<float>(), d_target,
+                       net.dy.as<float>(), 1.f / (float)n, net.loss.as<float>());
+    // head gradients
+    hipLaunchKernelGGL(colsum_kernel, dim3(kHid / 128), dim3(128), 0, st, net.A2.as<__bf16>(),
+                       npad, kHid, net.dy.as<float>(), G + Mlp::ow3);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, net.dy.as<float>(), npad, G + Mlp::ob3);
+    hipLaunchKernelGGL(head_back_kernel, dim3((npad * kHid + 255) / 256), dim3(256), 0, st,
+                       net.dy.as<float>(), P + Mlp::ow3, net.Z2.as<float>(), npad,
+                       net.dZ2.as<__bf16>());
+    // layer 2: dW2 = dZ2^T A1, db2 = colsum dZ2, dZ1 = (dZ2 W2) * gelu'(Z1)
+    gemm<false, false, EPI_F32>(st, kHid, kHid, (int)npad, bfp(net.dZ2), kHid, bfp(net.A1), kHid,
+                                nullptr, nullptr, G + Mlp::oW2, nullptr, kHid);
+    hipLaunchKernelGGL(colsum_kernel, dim3(kHid / 128), dim3(128), 0, st, net.dZ2.as<__bf16>(),
+                       npad, kHid, nullptr, G + Mlp::ob2);
+    gemm<true, false, EPI_GELU_GRAD>(st, (int)npad, kHid, kHid, bfp(net.dZ2), kHid,
+                                     bfp(net.pb, Mlp::oW2), kHid, nullptr, net.Z1.as<float>(),
+                                     nullptr, net.dZ1.as<__bf16>(), kHid);
+    // layer 1: dW1 = dZ1^T X, db1 = colsum dZ1
+    gemm<false, false, EPI_F32>(st, kHid, kIn, (int)npad, bfp(net.dZ1), kHid, bfp(net.X), kIn,
+                                nullptr, nullptr, G + Mlp::oW1, nullptr, kIn);
+    hipLaunchKernelGGL(colsum_kernel, dim3(kHid / 128), dim3(128), 0, st, net.dZ1.as<__bf16>(),
+                       npad, kHid, nullptr, G + Mlp::ob1);
+    if(lr > 0.f) {
+      net.step++;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.f - std::pow(b1, (float)net.step), c2 = 1.f - std::pow(b2, (float)net.step);
+      hipLaunchKernelGGL(adam_kernel, dim3((Mlp::nparam + 255) / 256), dim3(256), 0, st, P,
+                         net.pb.as<__bf16>(), G, net.m.as<float>(), net.v.as<float>(),
+                         Mlp::nparam, lr, b1, b2, c1, c2, 1e-8f);
+    }
+    MHIP_HIP_CHECK(hipGetLastError());
+    if(loss_out) {
+      DevBuf tot(4);
+      hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, net.loss.as<float>(), nblk,
+                         tot.as<float>());
+      float l = 0.f;
+      MHIP_HIP_CHECK(hipMemcpyAsync(&l, tot.p, 4, hipMemcpyDeviceToHost, st));
+      MHIP_HIP_CHECK(hipStreamSynchronize(st));
+      *loss_out = l / (float)n;
+    }
+  });
+}
+
+// bf16 GEMM entry for parity tests: C (fp32, M x N, row-major) = A B^T with
+// A (M x K) and B (N x K) row-major bf16 (raw 16-bit patterns).
+int mhip_gemm_bf16_nt(mhip_context* ctx, int M, int N, int K, const uint16_t* d_A,
+                      const uint16_t* d_B, float* d_C) {
+  return guarded([&] {
+    use_device(ctx);
+    gemm<true, true, EPI_F32>(ctx->stream, M, N, K, reinterpret_cast<const __bf16*>(d_A), K,
+                              reinterpret_cast<const __bf16*>(d_B), K, nullptr, nullptr, d_C,
+                              nullptr, N);
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

@@ -1194,3 +1194,105 @@ int orc_asci_refine(const orc_ham* h, int W, const orc_asci_settings* s,
   }
   return ORC_ERR_REFINE;
 }
+
+/* ===================================================================== */
+/* MLP pre-screen oracle (no reference counterpart)                        */
+/* ===================================================================== */
+/* Float32 -> bf16 -> float32 with round-to-nearest-even. */
+static double bf16r(double x) {
+  float f = (float)x;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+static double gelu_d(double z) {
+  const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
+  return 0.5 * z * (1. + tanh(u));
+}
+static double gelu_grad_d(double z) {
+  const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
+  const double t = tanh(u);
+  return 0.5 * (1. + t) +
+         0.5 * z * (1. - t * t) * 0.7978845608028654 * (1. + 3. * 0.044715 * z * z);
+}
+
+#define MLP_IN 128
+#define MLP_H 512
+int64_t orc_mlp_num_params(void) {
+  return (int64_t)MLP_H * MLP_IN + MLP_H + (int64_t)MLP_H * MLP_H + MLP_H + MLP_H + 1;
+}
+
+double orc_mlp_loss_grad(const float* P, int W, const uint64_t* dets,
+                         const float* target, int64_t n, int bf16_points,
+                         double* grad, double* pred) {
+  const int64_t oW1 = 0, ob1 = oW1 + MLP_H * MLP_IN, oW2 = ob1 + MLP_H,
+                ob2 = oW2 + (int64_t)MLP_H * MLP_H, ow3 = ob2 + MLP_H,
+                ob3 = ow3 + MLP_H;
+  const int64_t np = orc_mlp_num_params();
+#define RB(x) (bf16_points ? bf16r(x) : (x))
+  double* Wq = (double*)malloc(sizeof(double) * np);
+  for(int64_t i = 0; i < np; ++i) Wq[i] = P[i];
+  for(int64_t i = oW1; i < ob1; ++i) Wq[i] = RB(P[i]);
+  for(int64_t i = oW2; i < ob2; ++i) Wq[i] = RB(P[i]);
+  if(grad) memset(grad, 0, sizeof(double) * np);
+  double* x = (double*)malloc(sizeof(double) * MLP_IN);
+  double *z1 = (double*)malloc(sizeof(double) * MLP_H), *a1 = (double*)malloc(sizeof(double) * MLP_H);
+  double *z2 = (double*)malloc(sizeof(double) * MLP_H), *a2 = (double*)malloc(sizeof(double) * MLP_H);
+  double *dz2 = (double*)malloc(sizeof(double) * MLP_H), *dz1 = (double*)malloc(sizeof(double) * MLP_H);
+  double loss = 0.;
+  for(int64_t b = 0; b < n; ++b) {
+    for(int f = 0; f < MLP_IN; ++f)
+      x[f] = (f < 64 * W) ? (double)((dets[b * W + (f >> 6)] >> (f & 63)) & 1ULL) : 0.;
+    for(int j = 0; j < MLP_H; ++j) {
+      double s = 0.;
+      for(int f = 0; f < MLP_IN; ++f) s += Wq[oW1 + (int64_t)j * MLP_IN + f] * x[f];
+      z1[j] = s + Wq[ob1 + j];
+      a1[j] = RB(gelu_d(z1[j]));
+    }
+    for(int j = 0; j < MLP_H; ++j) {
+      double s = 0.;
+      for(int k = 0; k < MLP_H; ++k) s += Wq[oW2 + (int64_t)j * MLP_H + k] * a1[k];
+      z2[j] = s + Wq[ob2 + j];
+      a2[j] = RB(gelu_d(z2[j]));
+    }
+    double y = Wq[ob3];
+    for(int j = 0; j < MLP_H; ++j) y += a2[j] * Wq[ow3 + j];
+    if(pred) pred[b] = y;
+    if(!target) continue;
+    const double e = y - target[b];
+    loss += e * e;
+    if(!grad) continue;
+    const double dy = 2. * e / (double)n;
+    grad[ob3] += dy;
+    for(int j = 0; j < MLP_H; ++j) {
+      grad[ow3 + j] += a2[j] * dy;
+      dz2[j] = RB(dy * Wq[ow3 + j] * gelu_grad_d(z2[j]));
+    }
+    for(int j = 0; j < MLP_H; ++j) {
+      grad[ob2 + j] += dz2[j];
+      for(int k = 0; k < MLP_H; ++k) grad[oW2 + (int64_t)j * MLP_H + k] += dz2[j] * a1[k];
+    }
+    for(int k = 0; k < MLP_H; ++k) {
+      double s = 0.;
+      for(int j = 0; j < MLP_H; ++j) s += dz2[j] * Wq[oW2 + (int64_t)j * MLP_H + k];
+      dz1[k] = RB(s * gelu_grad_d(z1[k]));
+    }
+    for(int j = 0; j < MLP_H; ++j) {
+      grad[ob1 + j] += dz1[j];
+      for(int f = 0; f < MLP_IN; ++f) grad[oW1 + (int64_t)j * MLP_IN + f] += dz1[j] * x[f];
+    }
+  }
+#undef RB
+  free(Wq);
+  free(x);
+  free(z1);
+  free(a1);
+  free(z2);
+  free(a2);
+  free(dz2);
+  free(dz1);
+  return target ? loss / (double)n : 0.;
+}

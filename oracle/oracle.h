@@ -195,3 +195,20 @@ int orc_asci_refine(const orc_ham* h, int W, const orc_asci_settings* s,
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* ---- MLP pre-screen oracle (no reference counterpart) ----------------- */
+/* Forward/backward of the 128 -> 512 -> 512 -> 1 GELU(tanh) network in
+ * float64 over parameters P (layout of macis_hip.h). With bf16_points the
+ * weights, activations and back-propagated pre-activations are rounded to
+ * bf16 where the device kernels round them. Returns the mean squared error
+ * (0 without targets); grad (nparam) and pred (n) are optional. */
+int64_t orc_mlp_num_params(void);
+double orc_mlp_loss_grad(const float* P, int W, const uint64_t* dets,
+                         const float* target, int64_t n, int bf16_points,
+                         double* grad, double* pred);
+#ifdef __cplusplus
+}
+#endif

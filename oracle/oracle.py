@@ -69,6 +69,8 @@ def lib():
             "orc_asci_iter": (_i32, [_p, _i32, _p, _i64, _d, _d, _i64, _d, _p, _p, _p, _i64, _p]),
             "orc_asci_grow": (_i32, [_p, _i32, _p, _i64, _d, _d, _p, _p, _p, _p, _i64]),
             "orc_asci_refine": (_i32, [_p, _i32, _p, _i64, _d, _d, _p, _p, _p, _p, _i64]),
+            "orc_mlp_num_params": (_i64, []),
+            "orc_mlp_loss_grad": (_d, [_p, _i32, _p, _p, _i64, _i32, _p, _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -344,3 +346,16 @@ def davidson(N, max_m, rowptr, colind, nzval, D, tol, X, max_restarts=0):
     if rc:
         raise RuntimeError(f"orc_davidson rc={rc}")
     return it.value, E.value, X
+
+
+def mlp_loss_grad(P, W, dets, target=None, bf16_points=True, want_grad=True):
+    """MLP oracle: (loss, grad float64 or None, predictions)."""
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    dets = np.ascontiguousarray(dets, dtype=np.uint64)
+    n = dets.size // W
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float32)
+    g = np.zeros(lib().orc_mlp_num_params()) if want_grad else None
+    pred = np.zeros(n)
+    loss = lib().orc_mlp_loss_grad(_ptr(P), W, _ptr(dets), _ptr(t), n, int(bf16_points),
+                                   _ptr(g), _ptr(pred))
+    return loss, g, pred

@@ -1,0 +1,87 @@
+"""GPU checks of the learned pre-screen's kernels: the BF16 MFMA GEMM
+against numpy, forward predictions and backward gradients against the CPU
+oracle (orc_mlp_loss_grad, itself pinned by finite differences in
+tests/test_oracle_mlp.py), and that training reduces the loss."""
+import numpy as np
+import pytest
+
+from macis_amd import _capi
+from oracle import oracle as O
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def to_bf16_bits(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def from_bf16_bits(b):
+    return (b.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("M,N,K", [(128, 128, 32), (256, 512, 128), (384, 128, 512)])
+def test_gemm_bf16_nt(ctx, M, N, K):
+    announce(f"gemm_bf16_nt {M}x{N}x{K}")
+    rng = np.random.default_rng(M + N + K)
+    A = to_bf16_bits(rng.standard_normal((M, K)))
+    B = to_bf16_bits(rng.standard_normal((N, K)))
+    Cd = ctx.empty((M, N), np.float32)
+    _capi.check(_capi.lib().mhip_gemm_bf16_nt(ctx.h, M, N, K, ctx.to_device(A).ptr,
+                                              ctx.to_device(B).ptr, Cd.ptr), "gemm")
+    ref = from_bf16_bits(A).astype(np.float64) @ from_bf16_bits(B).astype(np.float64).T
+    got = Cd.numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(K) * np.abs(ref).max()
+
+
+def _batch(n, W, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.integers(0, 2**63, size=(n, W), dtype=np.uint64)
+    t = rng.standard_normal(n).astype(np.float32)
+    return d, t
+
+
+@pytest.mark.parametrize("W,n", [(2, 256), (2, 300), (1, 130)])
+def test_mlp_forward_backward_vs_oracle(ctx, W, n):
+    announce(f"mlp grads W={W} n={n}")
+    net = _capi.MLP(ctx, seed=7)
+    P = net.params()
+    d, t = _batch(n, W, 11 + n)
+    dd, dt = ctx.to_device(d), ctx.to_device(t)
+    pred = ctx.empty(n, np.float32)
+    net.predict(W, dd, n, pred)
+    loss_ref, g_ref, pred_ref = O.mlp_loss_grad(P, W, d, t, bf16_points=True)
+    assert np.allclose(pred.numpy(), pred_ref, rtol=0, atol=2e-3 * max(1.0, np.abs(pred_ref).max()))
+    loss = net.train_step(W, dd, dt, n, 0.0)
+    assert loss == pytest.approx(loss_ref, rel=2e-3)
+    g = net.grads().astype(np.float64)
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    assert rel < 2e-2, rel
+    # per block (W1, b1, W2, b2, w3, b3)
+    H, I = 512, 128
+    cuts = np.cumsum([0, H * I, H, H * H, H, H, 1])
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        r = np.linalg.norm(g[a:b] - g_ref[a:b]) / max(np.linalg.norm(g_ref[a:b]), 1e-30)
+        assert r < 3e-2, (a, b, r)
+    assert np.array_equal(net.params(), P)  # lr == 0 leaves weights alone
+
+
+def test_mlp_training_reduces_loss(ctx):
+    announce("mlp training on a smooth synthetic target")
+    W, n = 2, 4096
+    rng = np.random.default_rng(3)
+    d = rng.integers(0, 2**63, size=(n, W), dtype=np.uint64)
+    bits = np.unpackbits(d.view(np.uint8), bitorder="little").reshape(n, 128).astype(np.float32)
+    w = rng.standard_normal(128).astype(np.float32) / 8
+    t = (bits @ w).astype(np.float32)
+    t = (t - t.mean()) / t.std()
+    net = _capi.MLP(ctx, seed=1)
+    dd, dt = ctx.to_device(d), ctx.to_device(t)
+    first = net.train_step(W, dd, dt, n, 1e-3)
+    for _ in range(150):
+        last = net.train_step(W, dd, dt, n, 1e-3)
+    assert last < 0.1 * first, (first, last)
