@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-restarts", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="one 1 GiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the hot path has no "
@@ -91,6 +93,12 @@ def main():
     d_dets = ctx.to_device(dets)
     d_X = ctx.empty(N, np.float64)
     ctx.synchronize()
+    if args.calibrate_copy:
+        a = ctx.empty(1 << 27, np.float64).zero()
+        b = ctx.empty(1 << 27, np.float64)
+        C.check(C.lib().mhip_memcpy_d2d(ctx.h, b.ptr, a.ptr, a.nbytes), "copy")
+        ctx.synchronize()
+        del a, b
 
     def step():
         d_X.zero()

@@ -88,6 +88,8 @@ typedef struct mhip_timings {
   double asci_total_ms;
   int64_t asci_npairs;    /* raw candidates */
   int64_t asci_nunique;   /* unique candidates after accumulation */
+  double asci_nn_ms;      /* learned pre-screen (prediction + selection) */
+  int64_t asci_nsurvivors; /* candidates entering the exact top-k */
 } mhip_timings;
 
 /* Settings of asci_search (ASCISettings, determinant_search.hpp:34-51). */
@@ -277,6 +279,24 @@ int mhip_mlp_predict(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
 int mhip_mlp_train_step(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
                         const uint64_t* d_dets, const float* d_target,
                         int64_t n, float lr, float* loss);
+/* Scored candidates of asci_search before the top-k: the unique non-core
+ * determinants with their accumulated |rv| (training data of the
+ * pre-screen). DEVICE outputs of `capacity` entries; *n_out = count. */
+int mhip_asci_candidates(mhip_context* ctx, int words_per_det,
+                         const mhip_asci_settings* settings,
+                         const uint64_t* d_cdets, int64_t ncdets, double E_ASCI,
+                         const double* d_C, int64_t norb, uint64_t* d_dets_out,
+                         double* d_absrv_out, int64_t capacity, int64_t* n_out);
+/* asci_search with the learned pre-screen: all candidates are generated and
+ * accumulated as in mhip_asci_search, the network ranks them, the best
+ * keep_fraction (at least ndets_max - ncdets of them) survive and their exact
+ * |rv| picks the top-k. The plain mhip_asci_search is the fallback. */
+int mhip_asci_search_screened(mhip_context* ctx, int words_per_det,
+                              const mhip_asci_settings* settings, mhip_mlp* mlp,
+                              double keep_fraction, int64_t ndets_max,
+                              const uint64_t* d_cdets, int64_t ncdets,
+                              double E_ASCI, const double* d_C, int64_t norb,
+                              uint64_t* d_new_dets, int64_t* n_new);
 /* BF16 GEMM of the MLP kernels, for parity tests: C (FP32, M x N) = A B^T,
  * A (M x K) and B (N x K) row-major BF16 bit patterns, all DEVICE; M and N
  * multiples of 128, K of 32. */

@@ -51,6 +51,8 @@ class Timings(C.Structure):
         ("asci_total_ms", C.c_double),
         ("asci_npairs", C.c_int64),
         ("asci_nunique", C.c_int64),
+        ("asci_nn_ms", C.c_double),
+        ("asci_nsurvivors", C.c_int64),
     ]
 
     def as_dict(self):
@@ -117,6 +119,10 @@ _SIGS = {
     "mhip_mlp_predict": (_i32, [_p, _p, _i32, _p, _i64, _p]),
     "mhip_mlp_train_step": (_i32, [_p, _p, _i32, _p, _p, _i64, C.c_float, C.POINTER(C.c_float)]),
     "mhip_gemm_bf16_nt": (_i32, [_p, _i32, _i32, _i32, _p, _p, _p]),
+    "mhip_asci_candidates": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, _i64, _d, _p, _i64,
+                                    _p, _p, _i64, _pi64]),
+    "mhip_asci_search_screened": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, _d, _i64, _p,
+                                         _i64, _d, _p, _i64, _p, _pi64]),
     "mhip_batch_create": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _pp]),
     "mhip_batch_destroy": (_i32, [_p]),
     "mhip_batch_info": (_i32, [_p, _pi64, _pi64]),
@@ -341,6 +347,25 @@ class Context:
                                           C.byref(nnz), C.byref(H) if keep_H else None),
               "selected_ci_diag")
         return E.value, nnz.value, (CSR(self, H.value) if keep_H else None)
+
+    def asci_candidates(self, W, settings, d_cdets, ncdets, E_ASCI, d_C, norb, capacity):
+        """(dets (n, W) DeviceArray, |rv| DeviceArray, n) of the scored candidates."""
+        d_out = self.empty((max(capacity, 1), W), np.uint64)
+        a_out = self.empty(max(capacity, 1), np.float64)
+        n = C.c_int64()
+        check(lib().mhip_asci_candidates(self.h, W, C.byref(settings), _ptr(d_cdets), ncdets,
+                                         E_ASCI, _ptr(d_C), norb, d_out.ptr, a_out.ptr, capacity,
+                                         C.byref(n)), "asci_candidates")
+        return d_out, a_out, n.value
+
+    def asci_search_screened(self, W, settings, mlp, keep_fraction, ndets_max, d_cdets, ncdets,
+                             E_ASCI, d_C, norb, d_out):
+        n = C.c_int64()
+        check(lib().mhip_asci_search_screened(self.h, W, C.byref(settings), mlp.h, keep_fraction,
+                                              ndets_max, _ptr(d_cdets), ncdets, E_ASCI, _ptr(d_C),
+                                              norb, _ptr(d_out), C.byref(n)),
+              "asci_search_screened")
+        return n.value
 
     def asci_search(self, W, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out):
         n = C.c_int64()

@@ -344,10 +344,50 @@ __global__ void dets_to_keys_kernel(const uint64_t* __restrict__ d, int64_t n,
 
 inline int64_t host_pairs(int64_t m) { return m * (m - 1) / 2; }
 
+// Order-preserving map of float predictions to unsigned keys.
+__global__ void pred_keys_kernel(const float* __restrict__ pred, int64_t n,
+                                 uint32_t* __restrict__ key, int64_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const uint32_t u = __float_as_uint(pred[i]);
+  key[i] = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  idx[i] = i;
+}
+
+template <typename K>
+__global__ void gather_kernel(const int64_t* __restrict__ idx, int64_t n, const K* __restrict__ kin,
+                              const uint64_t* __restrict__ ain, K* __restrict__ kout,
+                              uint64_t* __restrict__ aout) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  kout[i] = kin[idx[i]];
+  aout[i] = ain[idx[i]];
+}
+
+__global__ void bits_to_double_kernel(const uint64_t* __restrict__ b, int64_t n,
+                                      double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) out[i] = __longlong_as_double((long long)b[i]);
+}
+
+// Optional stages of the search: export of the scored candidates (training
+// data of the pre-screen) or the learned pre-screen itself.
+struct SearchMode {
+  // export: unique non-core candidates with their accumulated |rv|
+  uint64_t* exp_dets = nullptr;
+  double* exp_abs = nullptr;
+  int64_t exp_cap = 0;
+  int64_t* exp_n = nullptr;
+  // pre-screen: keep the keep_fraction best-predicted candidates (at least
+  // the top-k size), then the exact |rv| picks the top-k among them
+  mhip_mlp* mlp = nullptr;
+  double keep_fraction = 1.0;
+};
+
 template <int W>
 int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndets_max,
                     const uint64_t* d_cdets, int64_t nc, double E0, const double* d_C,
-                    int64_t norb, uint64_t* d_out) {
+                    int64_t norb, uint64_t* d_out, const SearchMode& mode) {
   using K = typename std::conditional<W == 1, uint64_t, u128>::type;
   hipStream_t st = ctx->stream;
   mhip_timings& tm = ctx->timings;
@@ -497,6 +537,66 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     MHIP_HIP_CHECK(hipMemcpyAsync(&nkeep, nu_dev.as<int64_t>() + 1, 8, hipMemcpyDeviceToHost, st));
     MHIP_HIP_CHECK(hipStreamSynchronize(st));
   }
+  tm.asci_nn_ms = 0.;
+  tm.asci_nsurvivors = nkeep;
+  if(mode.exp_dets) {
+    MHIP_REQUIRE(nkeep <= mode.exp_cap, MHIP_ERR_INVALID_ARG,
+                 "candidate export capacity too small");
+    if(nkeep) {
+      hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nkeep + 255) / 256), dim3(256), 0, st,
+                         kkeys.as<K>(), nkeep, mode.exp_dets);
+      hipLaunchKernelGGL(bits_to_double_kernel, dim3((nkeep + 255) / 256), dim3(256), 0, st,
+                         kabs.as<uint64_t>(), nkeep, mode.exp_abs);
+    }
+    MHIP_HIP_CHECK(hipGetLastError());
+    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    *mode.exp_n = nkeep;
+    t.stop();
+    tm.asci_topk_ms = t.ms();
+    t_all.stop();
+    tm.asci_total_ms = t_all.ms();
+    return 0;
+  }
+  DevBuf pkeys, pidx, pkeys2, pidx2, sdets, pred;
+  if(mode.mlp && nkeep > top_k) {
+    // learned pre-screen: predict log|rv| of every candidate, keep the best
+    Timer tn(st);
+    tn.start();
+    const int64_t nsel = std::min<int64_t>(
+        nkeep, std::max<int64_t>(top_k, (int64_t)std::ceil(mode.keep_fraction * (double)nkeep)));
+    sdets.alloc(8 * W * nkeep);
+    pred.alloc(4 * nkeep);
+    hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nkeep + 255) / 256), dim3(256), 0, st,
+                       kkeys.as<K>(), nkeep, sdets.as<uint64_t>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    const int prc = mhip_mlp_predict(ctx, mode.mlp, W, sdets.as<uint64_t>(), nkeep, pred.as<float>());
+    MHIP_REQUIRE(prc == MHIP_SUCCESS, prc, "pre-screen prediction failed");
+    pkeys.alloc(4 * nkeep);
+    pidx.alloc(8 * nkeep);
+    pkeys2.alloc(4 * nkeep);
+    pidx2.alloc(8 * nkeep);
+    hipLaunchKernelGGL(pred_keys_kernel, dim3((nkeep + 255) / 256), dim3(256), 0, st,
+                       pred.as<float>(), nkeep, pkeys.as<uint32_t>(), pidx.as<int64_t>());
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, pkeys.as<uint32_t>(),
+                                                  pkeys2.as<uint32_t>(), pidx.as<int64_t>(),
+                                                  pidx2.as<int64_t>(), (size_t)nkeep, 0, 32, st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, pkeys.as<uint32_t>(),
+                                                  pkeys2.as<uint32_t>(), pidx.as<int64_t>(),
+                                                  pidx2.as<int64_t>(), (size_t)nkeep, 0, 32, st));
+    DevBuf k2(sizeof(K) * nsel), a2(8 * nsel);
+    hipLaunchKernelGGL((gather_kernel<K>), dim3((nsel + 255) / 256), dim3(256), 0, st,
+                       pidx2.as<int64_t>(), nsel, kkeys.as<K>(), kabs.as<uint64_t>(), k2.as<K>(),
+                       a2.as<uint64_t>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    kkeys = std::move(k2);
+    kabs = std::move(a2);
+    nkeep = nsel;
+    tn.stop();
+    tm.asci_nn_ms = tn.ms();
+    tm.asci_nsurvivors = nsel;
+  }
   int64_t nout = nkeep;
   const K* outkeys = kkeys.as<K>();
   DevBuf skeys, sabs;
@@ -556,9 +656,69 @@ extern "C" int mhip_asci_search(mhip_context* ctx, int words_per_det,
     if(!ncdets) return;
     if(words_per_det == 1)
       *n_new = search_impl<1>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
-                              d_new_dets);
+                              d_new_dets, SearchMode{});
     else
       *n_new = search_impl<2>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
-                              d_new_dets);
+                              d_new_dets, SearchMode{});
   });
+}
+
+namespace {
+int asci_entry(mhip_context* ctx, int W, const mhip_asci_settings* settings, int64_t ndets_max,
+               const uint64_t* d_cdets, int64_t ncdets, double E_ASCI, const double* d_C,
+               int64_t norb, uint64_t* d_out, int64_t* n_new, const SearchMode& mode) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(settings && n_new, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_REQUIRE(W == 1 || W == 2, MHIP_ERR_UNSUPPORTED, "words_per_det must be 1 or 2");
+    MHIP_REQUIRE(ctx->ints.norb > 0, MHIP_ERR_NO_INTEGRALS, "no integrals set");
+    MHIP_REQUIRE(norb > 0 && norb <= ctx->ints.norb && norb <= 32 * W, MHIP_ERR_INVALID_ARG,
+                 "bad norb");
+    MHIP_REQUIRE(ncdets > 0 && ndets_max >= ncdets && d_cdets && d_C, MHIP_ERR_INVALID_ARG,
+                 "bad core determinants");
+    *n_new = 0;
+    if(W == 1)
+      *n_new = search_impl<1>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out,
+                              mode);
+    else
+      *n_new = search_impl<2>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out,
+                              mode);
+  });
+}
+}  // namespace
+
+extern "C" int mhip_asci_candidates(mhip_context* ctx, int words_per_det,
+                                    const mhip_asci_settings* settings, const uint64_t* d_cdets,
+                                    int64_t ncdets, double E_ASCI, const double* d_C,
+                                    int64_t norb, uint64_t* d_dets_out, double* d_absrv_out,
+                                    int64_t capacity, int64_t* n_out) {
+  SearchMode m;
+  m.exp_dets = d_dets_out;
+  m.exp_abs = d_absrv_out;
+  m.exp_cap = capacity;
+  m.exp_n = n_out;
+  int64_t dummy = 0;
+  if(!n_out || !d_dets_out || !d_absrv_out) {
+    set_last_error("null output");
+    return MHIP_ERR_INVALID_ARG;
+  }
+  return asci_entry(ctx, words_per_det, settings, ncdets, d_cdets, ncdets, E_ASCI, d_C, norb,
+                    nullptr, &dummy, m);
+}
+
+extern "C" int mhip_asci_search_screened(mhip_context* ctx, int words_per_det,
+                                         const mhip_asci_settings* settings, mhip_mlp* mlp,
+                                         double keep_fraction, int64_t ndets_max,
+                                         const uint64_t* d_cdets, int64_t ncdets, double E_ASCI,
+                                         const double* d_C, int64_t norb, uint64_t* d_new_dets,
+                                         int64_t* n_new) {
+  SearchMode m;
+  m.mlp = mlp;
+  m.keep_fraction = keep_fraction;
+  if(!mlp || !(keep_fraction > 0.) || !d_new_dets) {
+    set_last_error("mhip_asci_search_screened: need a network, keep_fraction > 0, output");
+    return MHIP_ERR_INVALID_ARG;
+  }
+  return asci_entry(ctx, words_per_det, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
+                    d_new_dets, n_new, m);
 }

@@ -106,7 +106,8 @@ __global__ __launch_bounds__(256) void bgram_kernel(Seg sg, int k, const double*
   __syncthreads();
   for(int e = threadIdx.x; e < 1024; e += 256) {
     const int tile = e / 256, rem = e % 256, r = rem / 64, ln = rem % 64;
-    const int i = (tile / 2) * 16 + 4 * (ln / 16) + r, j = (tile % 2) * 16 + ln % 16;
+    // v_mfma_f64_16x16x4f64: lane ln, register r holds D(ln/16 + 4*r, ln%16)
+    const int i = (tile / 2) * 16 + ln / 16 + 4 * r, j = (tile % 2) * 16 + ln % 16;
     if(i < k && j < k)
       C[(int64_t)b * kMaxK * kMaxK + i + j * kMaxK] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
   }

@@ -88,8 +88,8 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(
 }
 
 // Sum the gram partials over blocks and scatter to column-major C (ldc).
-// Output layout of v_mfma_f64_16x16x4f64: lane l, register r holds
-// D(4*(l/16) + r, l%16).
+// Output layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l,
+// register r holds D(l/16 + 4*r, l%16).
 template <int TI, int TJ>
 __global__ void gram_finalize_kernel(const double* __restrict__ partials,
                                      int nblocks, int k, int m,
@@ -98,7 +98,7 @@ __global__ void gram_finalize_kernel(const double* __restrict__ partials,
   if(e >= TI * TJ * 256) return;
   const int tile = e / 256, rem = e % 256, r = rem / 64, lane = rem % 64;
   const int ti = tile / TJ, tj = tile % TJ;
-  const int i = ti * 16 + 4 * (lane / 16) + r, j = tj * 16 + lane % 16;
+  const int i = ti * 16 + lane / 16 + 4 * r, j = tj * 16 + lane % 16;
   if(i >= k || j >= m) return;
   double s = 0.;
   for(int b = 0; b < nblocks; ++b) s += partials[(int64_t)b * (TI * TJ * 256) + e];

@@ -75,6 +75,13 @@ __global__ __launch_bounds__(256) void bounds_kernel(
   }
 }
 
+// Value of a per-lane 64-bit register in lane l (wave-uniform result).
+__device__ __forceinline__ uint64_t lane_value(uint64_t x, uint32_t l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 template <int W>
 __device__ __forceinline__ int excitation_bound(const uint64_t (&b)[W],
                                                 const uint64_t (&a)[W],
@@ -195,6 +202,30 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
         for(int w = 0; w < W; ++w) {
           k[w] = kv ? ket[j * W + w] : 0ull;
           knz |= k[w];
+        }
+        // Tiles whose kets all share one alpha string (AND == OR on the
+        // alpha bits): the alpha excitation is a per-row scalar, so each
+        // ket costs one XOR + popcount of the beta string.
+        const uint64_t tA = lane_value(ta[0], tb), tO = lane_value(to[0], tb);
+        const uint64_t amask = W == 1 ? 0xffffffffull : ~0ull;
+        if(((tA ^ tO) & amask) == 0) {
+          const uint64_t kb = W == 1 ? (k[0] >> 32) : k[W - 1];
+#pragma unroll
+          for(int r = 0; r < R; ++r) {
+            if((tm[r] >> tb) & 1ull) {
+              const int thr = 4 - popc64((b[r][0] ^ tA) & amask);
+              const uint64_t bb = W == 1 ? (b[r][0] >> 32) : b[r][W - 1];
+              const bool hit = knz && popc64(bb ^ kb) <= thr;
+              const uint64_t m = ballot(hit);
+              if constexpr(FILL) {
+                if(hit) colind[base[r] + lanes_below(m)] = (int32_t)j;
+                base[r] += popc64(m);
+              } else {
+                cnt[r] += (uint32_t)popc64(m);
+              }
+            }
+          }
+          continue;
         }
 #pragma unroll
         for(int r = 0; r < R; ++r) {

@@ -31,8 +31,8 @@ def test_gemm_bf16_nt(ctx, M, N, K):
     A = to_bf16_bits(rng.standard_normal((M, K)))
     B = to_bf16_bits(rng.standard_normal((N, K)))
     Cd = ctx.empty((M, N), np.float32)
-    _capi.check(_capi.lib().mhip_gemm_bf16_nt(ctx.h, M, N, K, ctx.to_device(A).ptr,
-                                              ctx.to_device(B).ptr, Cd.ptr), "gemm")
+    dA, dB = ctx.to_device(A), ctx.to_device(B)  # keep both alive across the call
+    _capi.check(_capi.lib().mhip_gemm_bf16_nt(ctx.h, M, N, K, dA.ptr, dB.ptr, Cd.ptr), "gemm")
     ref = from_bf16_bits(A).astype(np.float64) @ from_bf16_bits(B).astype(np.float64).T
     got = Cd.numpy()
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(K) * np.abs(ref).max()
