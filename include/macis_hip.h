@@ -104,7 +104,7 @@ typedef struct mhip_asci_settings {
 /* ---- library / context --------------------------------------------------- */
 int mhip_get_version(void);
 const char* mhip_error_string(int code);
-/* Message of the last failing call on this host thread ("" if none). */
+/* Message of this host thread's last call ("" when it succeeded). */
 const char* mhip_last_error(void);
 
 /* Creates a context on `device` with its own HIP stream. */

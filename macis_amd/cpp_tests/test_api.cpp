@@ -229,6 +229,7 @@ static void test_asci(const H2O& h) {
   REQUIRE_NEAR(E0, -8.542926243842e+01, 1e-8);
   REQUIRE(dets.size() == 10000);
   REQUIRE(C.size() == 10000);
+  std::printf("  grown to %zu determinants, E0 = %.12f\n", dets.size(), E0);
   double n2 = 0.;
   for(auto c : C) n2 += c * c;
   REQUIRE_NEAR(n2, 1.0, 1e-10);
@@ -236,9 +237,11 @@ static void test_asci(const H2O& h) {
                                              std::move(C), gen, h.norb);
   REQUIRE_NEAR(E0, -8.542925964708e+01, 1e-8);
   REQUIRE(dets.size() == 10000);
+  std::printf("  refined, E0 = %.12f\n", E0);
 }
 
 int main(int argc, char** argv) {
+  std::setvbuf(stdout, nullptr, _IONBF, 0);  // every line reaches the log even on abort
   if(argc < 2) {
     std::printf("usage: %s <golden dir> [--cpu-only]\n", argv[0]);
     return 2;
@@ -269,5 +272,6 @@ int main(int argc, char** argv) {
     std::printf("  %s\n", g_fail == before ? "ok" : "FAILED");
   }
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
+  std::printf("[exit]\n");
   return g_fail ? 1 : 0;
 }

@@ -1,5 +1,6 @@
 // Context, device memory helpers, integral upload and error reporting of
 // the C-ABI (include/macis_hip.h).
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -9,10 +10,18 @@
 namespace mhip {
 
 namespace {
-thread_local std::string g_last_error;
-}
+// Trivially destructible on purpose: contexts may be destroyed from static
+// destructors at process exit, after the thread's thread_local objects with
+// destructors are gone, and every C-ABI call records its status here.
+constexpr size_t kMaxErr = 1024;
+thread_local char g_last_error[kMaxErr] = {0};
+}  // namespace
 
-void set_last_error(const std::string& msg) { g_last_error = msg; }
+void set_last_error(const std::string& msg) {
+  const size_t n = std::min(msg.size(), kMaxErr - 1);
+  std::memcpy(g_last_error, msg.data(), n);
+  g_last_error[n] = '\0';
+}
 
 void use_device(mhip_context* ctx) {
   MHIP_REQUIRE(ctx, MHIP_ERR_INVALID_ARG, "null context");
@@ -43,7 +52,7 @@ const char* mhip_error_string(int code) {
   }
 }
 
-const char* mhip_last_error(void) { return g_last_error.c_str(); }
+const char* mhip_last_error(void) { return g_last_error; }
 
 int mhip_context_create(int device, mhip_context** out) {
   return guarded([&] {

@@ -27,6 +27,11 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxK = 32;  // columns handled by the projection kernels
 constexpr int kRedBlocks = 512;
+constexpr int kGramBlocks = 256;  // partial Gram blocks (4 wavefronts each)
+
+int gram_blocks(int64_t N) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kGramBlocks, (N + 127) / 128));
+}
 
 int red_blocks(int64_t N) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (N + 255) / 256));
@@ -88,21 +93,26 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(
 }
 
 // Sum the gram partials over blocks and scatter to column-major C (ldc).
+// Block: 64 outputs x 4 block slices (fixed combination order).
 // Output layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l,
 // register r holds D(l/16 + 4*r, l%16).
 template <int TI, int TJ>
-__global__ void gram_finalize_kernel(const double* __restrict__ partials,
-                                     int nblocks, int k, int m,
-                                     double* __restrict__ C, int ldc) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if(e >= TI * TJ * 256) return;
+__global__ __launch_bounds__(256) void gram_finalize_kernel(const double* __restrict__ partials,
+                                                            int nblocks, int k, int m,
+                                                            double* __restrict__ C, int ldc) {
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63), slice = threadIdx.x >> 6;
+  double s = 0.;
+  for(int b = slice; b < nblocks; b += 4) s += partials[(int64_t)b * (TI * TJ * 256) + e];
+  __shared__ double red[4][64];
+  red[slice][threadIdx.x & 63] = s;
+  __syncthreads();
+  if(slice) return;
   const int tile = e / 256, rem = e % 256, r = rem / 64, lane = rem % 64;
   const int ti = tile / TJ, tj = tile % TJ;
   const int i = ti * 16 + lane / 16 + 4 * r, j = tj * 16 + lane % 16;
   if(i >= k || j >= m) return;
-  double s = 0.;
-  for(int b = 0; b < nblocks; ++b) s += partials[(int64_t)b * (TI * TJ * 256) + e];
-  C[i + (int64_t)j * ldc] = s;
+  C[i + (int64_t)j * ldc] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                            (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // inner[j] partials = V(:,j) . w for j < k (vector FMA; a GEMV is HBM bound).
@@ -136,14 +146,16 @@ __global__ __launch_bounds__(256) void dots_kernel(int64_t N, int k,
         (lds[0][threadIdx.x] + lds[1][threadIdx.x]) + (lds[2][threadIdx.x] + lds[3][threadIdx.x]);
 }
 
-// out[i] = sum_b partials[b*stride + i] in block order.
-__global__ void reduce_kernel(const double* __restrict__ partials, int nblocks,
-                              int count, int stride, double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= count) return;
+// out[i] = sum_b partials[b*stride + i]: one wavefront per output, lanes
+// stride the blocks, fixed combination order (deterministic).
+__global__ __launch_bounds__(64) void reduce_kernel(const double* __restrict__ partials,
+                                                    int nblocks, int stride,
+                                                    double* __restrict__ out) {
+  const int i = blockIdx.x, lane = threadIdx.x;
   double s = 0.;
-  for(int b = 0; b < nblocks; ++b) s += partials[(int64_t)b * stride + i];
-  out[i] = s;
+  for(int b = lane; b < nblocks; b += 64) s += partials[(int64_t)b * stride + i];
+  s = wave_sum(s);
+  if(lane == 0) out[i] = s;
 }
 
 // w = w - V(:, :k) * coef   (gram_schmidt gemm #2, davidson.hpp:116-117)
@@ -266,7 +278,7 @@ struct Ops {
   DevBuf part;   // reduction partials
   DevBuf small;  // reduced results (k inner products, norms, C)
   Ops(mhip_context* c, int64_t n) : ctx(c), N(n) {
-    part.alloc(sizeof(double) * std::max(kRedBlocks * kMaxK, 64 * 4 * 256));
+    part.alloc(sizeof(double) * std::max(kRedBlocks * kMaxK, kGramBlocks * 4 * 256));
     small.alloc(sizeof(double) * (kMaxK * kMaxK + 4 * kMaxK));
   }
   hipStream_t st() const { return ctx->stream; }
@@ -278,8 +290,8 @@ struct Ops {
     const int nb = red_blocks(N);
     hipLaunchKernelGGL(dots_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, ldv, w,
                        part.as<double>());
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, k,
-                       kMaxK, out);
+    hipLaunchKernelGGL(reduce_kernel, dim3(k), dim3(64), 0, st(), part.as<double>(), nb, kMaxK,
+                       out);
   }
   void update(const double* V, int64_t ldv, int k, const double* coef, double* w) {
     hipLaunchKernelGGL(update_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N, k, V, ldv,
@@ -299,16 +311,16 @@ struct Ops {
   }
   // C (k x k, ldc = k) = V^T AV on MFMA, copied to host.
   void gram(const double* V, const double* AV, int k, double* C_host) {
-    const int nb = std::max<int64_t>(1, std::min<int64_t>(64, (N + 127) / 128));
+    const int nb = gram_blocks(N);
     if(k <= 16) {
       hipLaunchKernelGGL((gram_mfma_kernel<1, 1>), dim3(nb), dim3(256), 0, st(), N, k, V, N, k,
                          AV, N, part.as<double>());
-      hipLaunchKernelGGL((gram_finalize_kernel<1, 1>), dim3(4), dim3(64), 0, st(),
+      hipLaunchKernelGGL((gram_finalize_kernel<1, 1>), dim3(4), dim3(256), 0, st(),
                          part.as<double>(), nb, k, k, Cdev(), k);
     } else {
       hipLaunchKernelGGL((gram_mfma_kernel<2, 2>), dim3(nb), dim3(256), 0, st(), N, k, V, N, k,
                          AV, N, part.as<double>());
-      hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(64), 0, st(),
+      hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(256), 0, st(),
                          part.as<double>(), nb, k, k, Cdev(), k);
     }
     MHIP_HIP_CHECK(hipGetLastError());
@@ -321,7 +333,7 @@ struct Ops {
     const int nb = red_blocks(N);
     hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, AV, N, rc, X, R,
                        part.as<double>());
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, 1, 1,
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, 1,
                        scalar());
     double r2 = 0.;
     MHIP_HIP_CHECK(hipMemcpyAsync(&r2, scalar(), 8, hipMemcpyDeviceToHost, st()));
@@ -508,11 +520,11 @@ void gemm_tn(mhip_context* ctx, int64_t N, int k, const double* A, int64_t lda, 
              const double* B, int64_t ldb, double* d_C, int ldc) {
   MHIP_REQUIRE(k >= 1 && k <= 32 && m >= 1 && m <= 32, MHIP_ERR_INVALID_ARG,
                "k, m must be in [1, 32]");
-  const int nb = std::max<int64_t>(1, std::min<int64_t>(64, (N + 127) / 128));
+  const int nb = gram_blocks(N);
   DevBuf part(sizeof(double) * nb * 4 * 256);
   hipLaunchKernelGGL((gram_mfma_kernel<2, 2>), dim3(nb), dim3(256), 0, ctx->stream, N, k, A,
                      lda, m, B, ldb, part.as<double>());
-  hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(64), 0, ctx->stream,
+  hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(256), 0, ctx->stream,
                      part.as<double>(), nb, k, m, d_C, ldc);
   MHIP_HIP_CHECK(hipGetLastError());
   MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -19,37 +19,70 @@ namespace mhip {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+constexpr int kXcds = 8;
 
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// One wavefront per row. The row's nonzeros are split into a head (up to 3
+// elements before the first 16-byte aligned quad), whole quads and a tail;
+// each lane streams whole quads with one 16-byte column load and two 16-byte
+// value loads (non-temporal), so the streamed bytes move in dwordx4 requests.
+// Blocks are mapped XCD-aware: hardware block b runs on XCD b % 8, and XCD
+// x owns logical blocks [x*per, (x+1)*per), a contiguous range of rows; in a
+// determinant-ordered CSR consecutive rows gather from the same region of
+// x, which then stays in that XCD's L2.
 __global__ __launch_bounds__(256) void spmv_kernel(
-    int64_t m, const int64_t* __restrict__ rowptr,
+    int64_t m, int64_t per_xcd, const int64_t* __restrict__ rowptr,
     const int32_t* __restrict__ colind, const double* __restrict__ nzval,
     int64_t indexing, double alpha, const double* __restrict__ x, double beta,
     double* __restrict__ y) {
-  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int64_t lblk = (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int64_t row = lblk * kWavesPerBlock + (threadIdx.x >> 6);
   if(row >= m) return;
   const int lane = threadIdx.x & 63;
   const int64_t st = rowptr[row] - indexing, en = rowptr[row + 1] - indexing;
   const double* xb = x - indexing;
+  const int64_t qa = (st + 3) >> 2, qb = en >> 2;  // whole quads [qa, qb)
   double acc0 = 0., acc1 = 0., acc2 = 0., acc3 = 0.;
-  int64_t j = st + lane;
-  for(; j + 192 < en; j += 256) {
-    const int32_t c0 = __builtin_nontemporal_load(colind + j);
-    const int32_t c1 = __builtin_nontemporal_load(colind + j + 64);
-    const int32_t c2 = __builtin_nontemporal_load(colind + j + 128);
-    const int32_t c3 = __builtin_nontemporal_load(colind + j + 192);
-    const double v0 = __builtin_nontemporal_load(nzval + j);
-    const double v1 = __builtin_nontemporal_load(nzval + j + 64);
-    const double v2 = __builtin_nontemporal_load(nzval + j + 128);
-    const double v3 = __builtin_nontemporal_load(nzval + j + 192);
-    acc0 = fma(v0, xb[c0], acc0);
-    acc1 = fma(v1, xb[c1], acc1);
-    acc2 = fma(v2, xb[c2], acc2);
-    acc3 = fma(v3, xb[c3], acc3);
+  // head [st, min(en, 4 qa)) and tail [max(4 qb, 4 qa), en): at most 3 each
+  {
+    const int64_t he = en < 4 * qa ? en : 4 * qa;
+    const int64_t ts = 4 * qb > 4 * qa ? 4 * qb : 4 * qa;
+    int64_t j = -1;
+    if(lane < 3) j = st + lane < he ? st + lane : -1;
+    else if(lane < 6) j = ts + (lane - 3) < en ? ts + (lane - 3) : -1;
+    if(j >= 0) acc0 = nzval[j] * xb[colind[j]];
   }
-  for(; j < en; j += 64) {
-    const int32_t c = __builtin_nontemporal_load(colind + j);
-    const double v = __builtin_nontemporal_load(nzval + j);
-    acc0 = fma(v, xb[c], acc0);
+  const i32x4* cq = reinterpret_cast<const i32x4*>(colind);
+  const f64x2* vq = reinterpret_cast<const f64x2*>(nzval);
+  int64_t q = qa + lane;
+  for(; q + 64 < qb; q += 128) {
+    const i32x4 c0 = __builtin_nontemporal_load(cq + q);
+    const i32x4 c1 = __builtin_nontemporal_load(cq + q + 64);
+    const f64x2 v0 = __builtin_nontemporal_load(vq + 2 * q);
+    const f64x2 v1 = __builtin_nontemporal_load(vq + 2 * q + 1);
+    const f64x2 v2 = __builtin_nontemporal_load(vq + 2 * (q + 64));
+    const f64x2 v3 = __builtin_nontemporal_load(vq + 2 * (q + 64) + 1);
+    const double x0 = xb[c0.x], x1 = xb[c0.y], x2 = xb[c0.z], x3 = xb[c0.w];
+    const double x4 = xb[c1.x], x5 = xb[c1.y], x6 = xb[c1.z], x7 = xb[c1.w];
+    acc0 = fma(v0.x, x0, acc0);
+    acc1 = fma(v0.y, x1, acc1);
+    acc2 = fma(v1.x, x2, acc2);
+    acc3 = fma(v1.y, x3, acc3);
+    acc0 = fma(v2.x, x4, acc0);
+    acc1 = fma(v2.y, x5, acc1);
+    acc2 = fma(v3.x, x6, acc2);
+    acc3 = fma(v3.y, x7, acc3);
+  }
+  if(q < qb) {
+    const i32x4 c0 = __builtin_nontemporal_load(cq + q);
+    const f64x2 v0 = __builtin_nontemporal_load(vq + 2 * q);
+    const f64x2 v1 = __builtin_nontemporal_load(vq + 2 * q + 1);
+    acc0 = fma(v0.x, xb[c0.x], acc0);
+    acc1 = fma(v0.y, xb[c0.y], acc1);
+    acc2 = fma(v1.x, xb[c0.z], acc2);
+    acc3 = fma(v1.y, xb[c0.w], acc3);
   }
   double acc = (acc0 + acc1) + (acc2 + acc3);
   acc = wave_sum(acc);
@@ -84,10 +117,11 @@ __global__ __launch_bounds__(256) void extract_diag_kernel(
 void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
           double beta, double* y) {
   if(A->m == 0) return;
-  hipLaunchKernelGGL(spmv_kernel, dim3((A->m + kWavesPerBlock - 1) / kWavesPerBlock),
-                     dim3(256), 0, ctx->stream, A->m, A->rowptr.as<int64_t>(),
-                     A->colind.as<int32_t>(), A->nzval.as<double>(), A->indexing, alpha,
-                     x, beta, y);
+  const int64_t nblk = (A->m + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t per = (nblk + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL(spmv_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m, per,
+                     A->rowptr.as<int64_t>(), A->colind.as<int32_t>(), A->nzval.as<double>(),
+                     A->indexing, alpha, x, beta, y);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 

@@ -79,23 +79,24 @@ class LearnedPrescreen {
     std::mt19937_64 rng(seed);
     auto& ctx = gen_.context();
     batch = std::min<int64_t>(batch, n);
-    hip::DeviceBuffer<uint64_t> bd(ctx, batch * W);
-    hip::DeviceBuffer<float> bt(ctx, batch);
-    std::vector<uint64_t> hd(batch * W);
-    std::vector<float> ht(batch);
+    // each epoch's permutation is uploaded once; the steps walk through it
+    hip::DeviceBuffer<uint64_t> bd(ctx, n * W);
+    hip::DeviceBuffer<float> bt(ctx, n);
+    std::vector<uint64_t> hd(n * W);
+    std::vector<float> ht(n);
     float loss = 0.f;
     for(int e = 0; e < epochs; ++e) {
       std::shuffle(perm.begin(), perm.end(), rng);
-      for(size_t b0 = 0; b0 + batch <= n; b0 += batch) {
-        for(int64_t i = 0; i < batch; ++i) {
-          const size_t p = perm[b0 + i];
-          std::copy_n(dets_.begin() + p * W, W, hd.begin() + i * W);
-          ht[i] = (float)((target_[p] - mu_) / sd_);
-        }
-        bd.upload(hd.data(), batch * W);
-        bt.upload(ht.data(), batch);
-        hip::check(mhip_mlp_train_step(ctx.get(), mlp_, W, bd.data(), bt.data(), batch, lr, &loss));
+      for(size_t i = 0; i < n; ++i) {
+        const size_t p = perm[i];
+        std::copy_n(dets_.begin() + p * W, W, hd.begin() + i * W);
+        ht[i] = (float)((target_[p] - mu_) / sd_);
       }
+      bd.upload(hd.data(), n * W);
+      bt.upload(ht.data(), n);
+      for(size_t b0 = 0; b0 + batch <= n; b0 += batch)
+        hip::check(mhip_mlp_train_step(ctx.get(), mlp_, W, bd.data() + b0 * W, bt.data() + b0,
+                                       batch, lr, &loss));
     }
     trained_ = true;
     recall_ = validate();

@@ -55,6 +55,34 @@ def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd):
     assert np.array_equal(Dg.numpy(), O.extract_diagonal(N, rp, ci, nz))
 
 
+@pytest.mark.parametrize("indexing", [0, 1])
+def test_spmv_ragged_rows(ctx, indexing):
+    """Rows of every length class around the kernel's 16-byte quads (0-9
+    nonzeros at all start offsets mod 4, long rows, an empty last row) against
+    the oracle's gespmbv (spmbv.hpp:37-73)."""
+    announce(f"spmv ragged rows indexing={indexing}")
+    rng = np.random.default_rng(7 + indexing)
+    lens = np.concatenate([np.tile(np.arange(10), 8), rng.integers(0, 700, 300),
+                           [1819, 4096, 3, 0]])
+    m = n = len(lens)
+    rowptr = np.zeros(m + 1, dtype=np.int64)
+    rowptr[1:] = np.cumsum(lens)
+    nnz = int(rowptr[-1])
+    colind = np.concatenate([np.sort(rng.choice(n, size=min(L, n), replace=L > n))
+                             for L in lens]).astype(np.int32)
+    nzval = rng.standard_normal(nnz)
+    A = ctx.csr_from_host(m, n, rowptr + indexing, colind + indexing, nzval, indexing=indexing)
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(m)
+    dx, dy = ctx.to_device(x), ctx.to_device(y0)
+    ctx.spmv(A, 1, 0.9, dx, n, 0.4, dy, m)
+    ref = y0.copy()
+    O.gespmbv(1, 0.9, m, rowptr + indexing, colind + indexing, nzval, indexing, x, n, 0.4, ref, m)
+    got = dy.numpy()
+    tol = 1e-14 * (np.abs(nzval).max() * np.abs(x).max() * lens.max() + 1)
+    assert np.max(np.abs(got - ref)) <= tol
+
+
 @pytest.mark.parametrize("k,m", [(1, 1), (5, 1), (16, 16), (17, 17), (21, 21), (3, 29)])
 def test_gemm_tn_mfma(ctx, k, m):
     announce(f"gemm_tn k={k} m={m}")
