@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
 N_ORB, N_ALPHA, N_BETA, SEED = 12, 6, 6, 12
 
 
@@ -63,6 +64,124 @@ def cpu_baseline(T, V, dets, budget_s=12.0):
     }
 
 
+def mlp_throughput(C, ctx):
+    """Forward (prediction) and training-step throughput of the pre-screen
+    network on random 128-bit occupation strings."""
+    W, n, nb = 2, 1 << 20, 65536
+    rng = np.random.default_rng(5)
+    d = ctx.to_device(rng.integers(0, 2**63, size=(n, W), dtype=np.uint64))
+    t = ctx.to_device(rng.standard_normal(nb).astype(np.float32))
+    out = ctx.empty(n, np.float32)
+    net = C.MLP(ctx, seed=1)
+    net.predict(W, d, n, out)
+    net.train_step(W, d, t, nb, 0.0)
+    ctx.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.predict(W, d, n, out)
+    ctx.synchronize()
+    fwd_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(W, d, t, nb, 0.0)
+    ctx.synchronize()
+    trn_s = (time.perf_counter() - t0) / reps
+    fwd_flop = 2 * (128 * 512 + 512 * 512 + 512)          # per candidate
+    bwd_flop = 2 * (2 * 512 * 512 + 128 * 512 + 2 * 512)  # dW2, dA1, dW1, head
+    f_tf = fwd_flop * n / fwd_s / 1e12
+    t_tf = (fwd_flop + bwd_flop) * nb / trn_s / 1e12
+    return {
+        "forward_rows": n, "forward_ms": fwd_s * 1e3, "forward_TFLOPs": f_tf,
+        "train_batch": nb, "train_step_ms": trn_s * 1e3, "train_TFLOPs": t_tf,
+        "mfma_util_forward": f_tf / BF16_PEAK_TFLOPS, "mfma_util_train": t_tf / BF16_PEAK_TFLOPS,
+        "note": "wall time per call incl. featurisation, head, launches; peak = dense BF16 MFMA",
+    }
+
+
+def asci_step_comparison(C, ctx):
+    """One ASCI refinement step on H2O cc-pVDZ (core 100, 6400 dets), plain
+    asci_search vs the learned pre-screen trained on the three earlier plain
+    steps (HF -> 100 -> 800 -> 6400); both re-diagonalised on the GPU."""
+    from macis_amd import dets as D
+    from macis_amd.fcidump import read_fcidump
+
+    norb, _, T, V = read_fcidump(os.path.join(ROOT, "tests", "golden", "h2o.ccpvdz.fci.dat"))
+    ctx.set_integrals(norb, T, V)
+    W, s = 1, C.ASCISettings.defaults()
+
+    def core_of(wfn, X):
+        o = np.argsort(-np.abs(X), kind="stable")[:100]
+        return wfn[o], X[o]
+
+    def diag(dets):
+        dX = ctx.empty(len(dets), np.float64).zero()
+        E, _, _ = ctx.selected_ci_diag(W, ctx.to_device(dets), len(dets), dX)
+        return E, dX.numpy()
+
+    hf = D.canonical_hf_determinant(W, 5, 5)
+    dE = ctx.empty(1, np.float64)
+    ctx.ham_diagonal(W, ctx.to_device(hf), 1, dE)
+    E0, wfn, X = float(dE.numpy()[0]), hf, np.array([1.0])
+    feats, targ = [], []
+    for nd in (100, 800, 6400):
+        core, Cc = core_of(wfn, X)
+        dcore, dC = ctx.to_device(core), ctx.to_device(Cc)
+        dd, da, n = ctx.asci_candidates(W, s, dcore, len(core), E0, dC, norb, 4_000_000)
+        feats.append(dd.numpy()[:n])
+        targ.append(np.log(da.numpy()[:n]))
+        out = ctx.empty((nd, W), np.uint64)
+        n = ctx.asci_search(W, s, nd, dcore, len(core), E0, dC, norb, out)
+        wfn = out.numpy()[:n]
+        E0, X = diag(wfn)
+    Xd = np.concatenate(feats)
+    y = np.concatenate(targ).astype(np.float32)
+    y = np.maximum(y, np.quantile(y, 0.7))
+    yn = ((y - y.mean()) / y.std()).astype(np.float32)
+    net = C.MLP(ctx, seed=3)
+    rng = np.random.default_rng(0)
+    ddX, ddy = ctx.empty((len(yn), W), np.uint64), ctx.empty(len(yn), np.float32)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        p = rng.permutation(len(yn))
+        ddX.upload(Xd[p])
+        ddy.upload(yn[p])
+        for b0 in range(0, len(yn) - 4096 + 1, 4096):
+            net.train_step(W, ddX.ptr + 8 * W * b0, ddy.ptr + 4 * b0, 4096, 1e-3)
+    ctx.synchronize()
+    train_s = time.perf_counter() - t0
+    nd = len(wfn)
+    core, Cc = core_of(wfn, X)
+    dcore, dC = ctx.to_device(core), ctx.to_device(Cc)
+    res = {}
+    for name in ("plain", "screened"):
+        out = ctx.empty((nd, W), np.uint64)
+        reps, ts = 5, []
+        for _ in range(reps):
+            if name == "plain":
+                n = ctx.asci_search(W, s, nd, dcore, len(core), E0, dC, norb, out)
+            else:
+                n = ctx.asci_search_screened(W, s, net, 0.05, nd, dcore, len(core), E0, dC, norb,
+                                             out)
+            ts.append(ctx.timings()["asci_total_ms"])
+        got = out.numpy()[:n]
+        t1 = time.perf_counter()
+        E, _ = diag(got)
+        diag_ms = (time.perf_counter() - t1) * 1e3
+        res[name] = (sorted(ts)[len(ts) // 2], got, E, diag_ms, ctx.timings())
+    plain, scr = res["plain"], res["screened"]
+    recall = len(set(scr[1][:, 0].tolist()) & set(plain[1][:, 0].tolist())) / len(plain[1])
+    return {
+        "system": "H2O cc-pVDZ (24 orbitals, 5a5b), core 100, 6400 dets (refinement step)",
+        "training_pairs": int(len(yn)), "train_s": train_s,
+        "plain_search_ms": plain[0], "screened_search_ms": scr[0],
+        "search_speedup": plain[0] / scr[0],
+        "plain_step_ms": plain[0] + plain[3], "screened_step_ms": scr[0] + scr[3],
+        "keep_fraction": 0.05, "recall": recall,
+        "dE_mHa": (scr[2] - plain[2]) * 1e3,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +189,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-restarts", type=int, default=200)
+    ap.add_argument("--no-asci", action="store_true",
+                    help="skip the learned pre-screen / ASCI step comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 1 GiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
@@ -180,6 +301,9 @@ def main():
             "traffic": None,
         },
     }
+    if not args.no_asci:
+        out["mlp"] = mlp_throughput(C, ctx)
+        out["asci"] = asci_step_comparison(C, ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(T, V, dets)
     print(json.dumps(out), flush=True)

@@ -90,7 +90,18 @@ typedef struct mhip_timings {
   int64_t asci_nunique;   /* unique candidates after accumulation */
   double asci_nn_ms;      /* learned pre-screen (prediction + selection) */
   int64_t asci_nsurvivors; /* candidates entering the exact top-k */
+  int64_t ham_path;       /* structure pass of the last build: MHIP_BUILD_* */
 } mhip_timings;
+
+/* Structure passes of the Hamiltonian build. AUTO takes the product-space
+ * walk when the determinant list is a complete product of spin strings
+ * (every run of equal outer string carries the same inner-string sequence,
+ * as the lists of generate_hilbert_space and their bitset_less-sorted form
+ * are) and the tiled XOR/popcount double loop otherwise. Both give the same
+ * CSR. */
+#define MHIP_BUILD_AUTO 0
+#define MHIP_BUILD_TILED 1
+#define MHIP_BUILD_PRODUCT 2
 
 /* Settings of asci_search (ASCISettings, determinant_search.hpp:34-51). */
 typedef struct mhip_asci_settings {
@@ -114,6 +125,11 @@ int mhip_context_destroy(mhip_context* ctx);
 int mhip_context_get_stream(mhip_context* ctx, void** stream);
 int mhip_synchronize(mhip_context* ctx);
 int mhip_get_timings(mhip_context* ctx, mhip_timings* out);
+/* Selects the structure passes of later builds on this context (no
+ * counterpart in the reference, which has one serial loop): MHIP_BUILD_AUTO
+ * (default), MHIP_BUILD_TILED, or MHIP_BUILD_PRODUCT (fails with
+ * MHIP_ERR_INVALID_ARG when a square build's list is not a product space). */
+int mhip_context_set_build_path(mhip_context* ctx, int path);
 
 /* Device memory helpers, so callers need no HIP runtime of their own. */
 int mhip_malloc(mhip_context* ctx, size_t bytes, void** dptr);

@@ -53,6 +53,7 @@ class Timings(C.Structure):
         ("asci_nunique", C.c_int64),
         ("asci_nn_ms", C.c_double),
         ("asci_nsurvivors", C.c_int64),
+        ("ham_path", C.c_int64),
     ]
 
     def as_dict(self):
@@ -88,6 +89,7 @@ _SIGS = {
     "mhip_context_get_stream": (_i32, [_p, _pp]),
     "mhip_synchronize": (_i32, [_p]),
     "mhip_get_timings": (_i32, [_p, C.POINTER(Timings)]),
+    "mhip_context_set_build_path": (_i32, [_p, _i32]),
     "mhip_malloc": (_i32, [_p, C.c_size_t, _pp]),
     "mhip_free": (_i32, [_p, _p]),
     "mhip_memcpy_h2d": (_i32, [_p, _p, _p, C.c_size_t]),
@@ -288,6 +290,11 @@ class Context:
         t = Timings()
         check(lib().mhip_get_timings(self.h, C.byref(t)), "get_timings")
         return t.as_dict()
+
+    BUILD_AUTO, BUILD_TILED, BUILD_PRODUCT = 0, 1, 2
+
+    def set_build_path(self, path):
+        check(lib().mhip_context_set_build_path(self.h, path), "set_build_path")
 
     # ---- hot path
     def set_integrals(self, norb, T, V):

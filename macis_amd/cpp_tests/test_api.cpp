@@ -4,7 +4,9 @@
 // src/sparsexx/tests/test_spmbv.cxx), with tighter tolerances.
 //
 // usage: test_api <golden dir> [--cpu-only]
+#include <algorithm>
 #include <cmath>
+#include <iterator>
 #include <cstdio>
 #include <fstream>
 #include <functional>
@@ -12,6 +14,7 @@
 #include <vector>
 
 #include <macis/asci/grow.hpp>
+#include <macis/asci/learned_prescreen.hpp>
 #include <macis/asci/refine.hpp>
 #include <macis/csr_hamiltonian.hpp>
 #include <macis/hip/hip_hamiltonian_generator.hpp>
@@ -240,6 +243,69 @@ static void test_asci(const H2O& h) {
   std::printf("  refined, E0 = %.12f\n", E0);
 }
 
+static void test_learned_prescreen(const H2O& h) {
+  // LearnedPrescreen: plain steps HF -> 100 -> 800 -> 6400 are observed, the
+  // network is trained on them, and the screened search of the next
+  // (refinement) step is judged against the plain search of that step.
+  auto T = h.T;
+  auto V = h.V;
+  macis::HipHamiltonianGenerator<64> gen(macis::matrix_span<double>(T.data(), h.norb, h.norb),
+                                         macis::rank4_span<double>(V.data(), h.norb, h.norb,
+                                                                   h.norb, h.norb));
+  macis::ASCISettings s;
+  macis::LearnedPrescreen<64> nn(gen, /*seed=*/3, /*keep_fraction=*/0.05, /*min_recall=*/0.5);
+  std::vector<macis::wfn_t<64>> wfn = {macis::canonical_hf_determinant<64>(5, 5)};
+  std::vector<double> C = {1.0};
+  double E0 = gen.matrix_element(wfn[0], wfn[0]);
+  auto step = [&](size_t nd, bool screened) {
+    macis::reorder_ci_on_coeff(wfn, C);
+    const size_t nc = std::min<size_t>(s.ncdets_max, wfn.size());
+    auto out = screened ? nn.search(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb)
+                        : macis::asci_search(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb,
+                                             gen.T(), gen.G_red(), gen.V_red(), gen.G(), gen.V(),
+                                             gen);
+    return out;
+  };
+  // untrained: the plain search is the fallback
+  {
+    auto a = step(100, true);
+    REQUIRE(nn.used_fallback());
+    auto b = step(100, false);
+    REQUIRE(a == b);
+  }
+  for(size_t nd : {100, 800, 6400}) {
+    macis::reorder_ci_on_coeff(wfn, C);
+    const size_t nc = std::min<size_t>(s.ncdets_max, wfn.size());
+    nn.observe(s, wfn.begin(), wfn.begin() + nc, E0, C, h.norb, 4000000);
+    wfn = step(nd, false);
+    std::vector<double> X;
+    E0 = macis::selected_ci_diag<64, int32_t>(wfn.begin(), wfn.end(), gen, 2.2e-16, 20, 1e-8, X);
+    C = X;
+  }
+  nn.train(20, 4096, 1e-3f);
+  std::printf("  validation recall %.4f\n", nn.recall());
+  const size_t nd = wfn.size();
+  auto plain = step(nd, false);
+  auto screened = step(nd, true);
+  REQUIRE(!nn.used_fallback());
+  REQUIRE(screened.size() == plain.size());
+  std::vector<double> Xp, Xs;
+  const double Ep =
+      macis::selected_ci_diag<64, int32_t>(plain.begin(), plain.end(), gen, 2.2e-16, 20, 1e-8, Xp);
+  const double Es = macis::selected_ci_diag<64, int32_t>(screened.begin(), screened.end(), gen,
+                                                         2.2e-16, 20, 1e-8, Xs);
+  auto key = [](const macis::wfn_t<64>& d) { return d.to_ullong(); };
+  std::vector<unsigned long long> kp, ks;
+  for(auto& d : plain) kp.push_back(key(d));
+  for(auto& d : screened) ks.push_back(key(d));
+  std::sort(kp.begin(), kp.end());
+  std::sort(ks.begin(), ks.end());
+  std::vector<unsigned long long> both;
+  std::set_intersection(kp.begin(), kp.end(), ks.begin(), ks.end(), std::back_inserter(both));
+  std::printf("  recall %.4f, dE %+.3e Ha\n", (double)both.size() / kp.size(), Es - Ep);
+  REQUIRE(std::abs(Es - Ep) <= 1e-4);
+}
+
 int main(int argc, char** argv) {
   std::setvbuf(stdout, nullptr, _IONBF, 0);  // every line reaches the log even on abort
   if(argc < 2) {
@@ -258,6 +324,7 @@ int main(int argc, char** argv) {
     cases.push_back({"spmbv (device)", [] { test_spmbv(); }});
     cases.push_back({"Davidson (device)", [&] { test_davidson(h); }});
     cases.push_back({"ASCI grow/refine (device)", [&] { test_asci(h); }});
+    cases.push_back({"learned ASCI pre-screen (device)", [&] { test_learned_prescreen(h); }});
   }
   for(auto& [name, fn] : cases) {
     std::printf("[case] %s\n", name.c_str());

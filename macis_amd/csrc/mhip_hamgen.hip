@@ -25,6 +25,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <memory>
 
 #include "mhip_device.hpp"
@@ -397,22 +398,26 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   t_all.start();
   const IntsView I = view(ctx->ints);
 
-  // 1. bounds
+  // 1. bounds (tiled path only)
   t.start();
   const int64_t ntiles = (nket + kTile - 1) / kTile;
   const int64_t nsuper = (ntiles + kSuper - 1) / kSuper;
-  DevBuf bnd(sizeof(uint64_t) * W * 2 * (ntiles + nsuper));
-  uint64_t* tand = bnd.as<uint64_t>();
-  uint64_t* tor = tand + W * ntiles;
-  uint64_t* sand = tor + W * ntiles;
-  uint64_t* sor = sand + W * nsuper;
-  hipLaunchKernelGGL(bounds_kernel<W>, dim3((ntiles + 3) / 4), dim3(256), 0, st,
-                     d_ket, d_ket, nket, ntiles, tand, tor);
-  hipLaunchKernelGGL(bounds_kernel<W>, dim3((nsuper + 3) / 4), dim3(256), 0, st,
-                     tand, tor, ntiles, nsuper, sand, sor);
-  MHIP_HIP_CHECK(hipGetLastError());
+  DevBuf bnd;
+  uint64_t *tand = nullptr, *tor = nullptr, *sand = nullptr, *sor = nullptr;
+  if(!ex.product) {
+    bnd.alloc(sizeof(uint64_t) * W * 2 * (ntiles + nsuper));
+    tand = bnd.as<uint64_t>();
+    tor = tand + W * ntiles;
+    sand = tor + W * ntiles;
+    sor = sand + W * nsuper;
+    hipLaunchKernelGGL(bounds_kernel<W>, dim3((ntiles + 3) / 4), dim3(256), 0, st,
+                       d_ket, d_ket, nket, ntiles, tand, tor);
+    hipLaunchKernelGGL(bounds_kernel<W>, dim3((nsuper + 3) / 4), dim3(256), 0, st,
+                       tand, tor, ntiles, nsuper, sand, sor);
+    MHIP_HIP_CHECK(hipGetLastError());
+  }
   t.stop();
-  tm.ham_bounds_ms = t.ms();
+  tm.ham_bounds_ms = t.ms() + ex.detect_ms;
 
   // 2. count
   const int64_t nwaves = ex.items ? ex.nitems : (nbra + kRows - 1) / kRows;
@@ -422,9 +427,12 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   DevBuf counts32(sizeof(uint32_t) * (nbra + 1));
   DevBuf counts64(sizeof(int64_t) * (nbra + 1));
   t.start();
-  hipLaunchKernelGGL((scan_pass_kernel<W, kRows, false>), dim3(grid), dim3(256), 0, st,
-                     d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
-                     nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr);
+  if(ex.product)
+    product_count(ctx, *ex.product, nbra, counts32.as<uint32_t>());
+  else
+    hipLaunchKernelGGL((scan_pass_kernel<W, kRows, false>), dim3(grid), dim3(256), 0, st,
+                       d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
+                       nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr);
   MHIP_HIP_CHECK(hipGetLastError());
   t.stop();
   tm.ham_count_ms = t.ms();
@@ -447,7 +455,9 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   DevBuf col(sizeof(int32_t) * std::max<int64_t>(nnz_struct, 1));
   DevBuf val(sizeof(double) * std::max<int64_t>(nnz_struct, 1));
   t.start();
-  if(nnz_struct)
+  if(nnz_struct && ex.product)
+    product_fill(ctx, *ex.product, nbra, rp_struct.as<int64_t>(), col.as<int32_t>());
+  else if(nnz_struct)
     hipLaunchKernelGGL((scan_pass_kernel<W, kRows, true>), dim3(grid), dim3(256), 0, st,
                        d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
                        nwaves, nblocks, ex.items, nullptr, rp_struct.as<int64_t>(),
@@ -506,8 +516,9 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
     csr->nnz = nnz;
   }
   t_all.stop();
-  tm.ham_total_ms = t_all.ms();
+  tm.ham_total_ms = t_all.ms() + ex.detect_ms;
   tm.ham_nnz = csr->nnz;
+  tm.ham_path = ex.product ? MHIP_BUILD_PRODUCT : MHIP_BUILD_TILED;
 }
 
 template void build_impl<1>(mhip_context*, const uint64_t*, int64_t, const uint64_t*, int64_t,
@@ -555,11 +566,27 @@ mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
   MHIP_REQUIRE(ctx->ints.norb <= 32 * W, MHIP_ERR_INVALID_ARG,
                "norb exceeds the spin-string width of the determinant type");
   validate_dets(ctx, W, d_bra, nbra);
-  if(d_ket != d_bra || nket != nbra) validate_dets(ctx, W, d_ket, nket);
+  const bool square = d_ket == d_bra && nket == nbra;
+  if(!square) validate_dets(ctx, W, d_ket, nket);
+  // Product-space structure passes for complete spaces (same CSR).
+  ProductSpace ps;
+  BuildExtra ex;
+  if(square && ctx->build_path != MHIP_BUILD_TILED) {
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    if(detect_product_space(ctx, W, d_bra, nbra, ps)) ex.product = &ps;
+    ex.detect_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    MHIP_REQUIRE(ex.product || ctx->build_path != MHIP_BUILD_PRODUCT, MHIP_ERR_INVALID_ARG,
+                 "determinant list is not a product space");
+  } else {
+    MHIP_REQUIRE(ctx->build_path != MHIP_BUILD_PRODUCT, MHIP_ERR_INVALID_ARG,
+                 "the product-space build needs a square block (bra list == ket list)");
+  }
   if(W == 1)
-    build_impl<1>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), BuildExtra{});
+    build_impl<1>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), ex);
   else
-    build_impl<2>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), BuildExtra{});
+    build_impl<2>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), ex);
   return csr.release();
 }
 

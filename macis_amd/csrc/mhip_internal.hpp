@@ -134,6 +134,7 @@ struct mhip_context {
   // Reusable scratch (partials, flags, temp storage of device primitives).
   mhip::DevBuf scratch_a, scratch_b, scratch_c;
   std::vector<char> host_scratch;
+  int build_path = 0;  // MHIP_BUILD_*
 };
 
 struct mhip_csr {
@@ -166,12 +167,32 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
 void validate_dets(mhip_context* ctx, int W, const uint64_t* d, int64_t n);
 struct IntsView;
 IntsView view(const Integrals& I);
+// Product-space structure of a determinant list: n = nruns * L, det
+// r*L + k = (outer string O[r], inner string I[k]). Device tables: outer
+// neighbours of every run (r' << 2 | excitation level, r' ascending), their
+// per-level counts, and the inner neighbour lists of every inner string at
+// levels <= 1 and <= 2 (ascending k').
+struct ProductSpace {
+  int64_t L = 0, nruns = 0;
+  bool outer_alpha = true;
+  DevBuf onb_ptr, onb, ocnt;         // int32 [nruns+1], int32 [..], int32 [3*nruns]
+  DevBuf ptr1, idx1, ptr2, idx2;     // int32 inner lists (levels <= 1, <= 2)
+};
+// Recognises a product-space list (host + device checks); false otherwise.
+bool detect_product_space(mhip_context* ctx, int W, const uint64_t* d, int64_t n,
+                          ProductSpace& ps);
+// Structural row counts / ascending columns of a product-space list.
+void product_count(mhip_context* ctx, const ProductSpace& ps, int64_t n, uint32_t* counts);
+void product_fill(mhip_context* ctx, const ProductSpace& ps, int64_t n, const int64_t* rowptr,
+                  int32_t* colind);
 // Optional work-item table / per-instance integrals of a batched build.
 struct BuildExtra {
   const int64_t* items = nullptr;  // DEVICE, 4 int64 per item
   int64_t nitems = 0;
   const IntsView* views = nullptr;  // DEVICE, one per instance
   const int32_t* row_inst = nullptr;  // DEVICE, instance of every row
+  const ProductSpace* product = nullptr;  // structure passes of a product list
+  double detect_ms = 0.;                  // host time spent recognising it
 };
 template <int W>
 void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra, const uint64_t* d_ket,

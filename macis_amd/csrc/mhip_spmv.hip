@@ -2,11 +2,12 @@
 // (extract_diagonal_elements, submatrix.hpp:264-295) and the CSR handle
 // entry points.
 //
-// SpMV layout: one wavefront per row; lanes stride the row, values and
-// column indices are streamed with non-temporal loads (read once, kept out
-// of L2 so the gathered x vector stays cached), x is gathered through
-// L2 / Infinity Cache, and the row sum is reduced across the wavefront.
-// Bound: HBM (2 flops per 12 streamed bytes).
+// SpMV layout: four rows per wavefront, 16 lanes per row streaming 16-byte
+// quads of column indices and values with non-temporal loads (read once,
+// kept out of L2 so the gathered x vector stays cached); x is gathered
+// through L1/L2 and each row sum is reduced across its 16 lanes. Bound: HBM
+// for the streamed 12 bytes per nonzero, with the x gathers' L2 requests as
+// the second limiter.
 #include <algorithm>
 #include <limits>
 #include <memory>
@@ -20,73 +21,78 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kXcds = 8;
+// SpMV: 4 rows per wavefront, 16 lanes per row.
+constexpr int kRowsPerWave = 4;
+constexpr int kLanesPerRow = 64 / kRowsPerWave;
+constexpr int kRowsPerBlock = kWavesPerBlock * kRowsPerWave;
 
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-// One wavefront per row. The row's nonzeros are split into a head (up to 3
-// elements before the first 16-byte aligned quad), whole quads and a tail;
-// each lane streams whole quads with one 16-byte column load and two 16-byte
-// value loads (non-temporal), so the streamed bytes move in dwordx4 requests.
-// Blocks are mapped XCD-aware: hardware block b runs on XCD b % 8, and XCD
-// x owns logical blocks [x*per, (x+1)*per), a contiguous range of rows; in a
-// determinant-ordered CSR consecutive rows gather from the same region of
-// x, which then stays in that XCD's L2.
+// Four rows share a wavefront, 16 lanes each, and every lane streams whole
+// 16-byte aligned quads of its row (one column load, two value loads,
+// non-temporal); the up-to-3 elements before the first aligned quad and
+// after the last one are taken by lanes 0-5 of the row's group. Neighbouring
+// rows of a determinant-ordered Hamiltonian gather nearly the same x lines
+// at the same nonzero position, so the four rows' gathers coalesce in one
+// instruction. Blocks are mapped XCD-aware: hardware block b runs on XCD
+// b % 8, and XCD x owns logical blocks [x*per, (x+1)*per), a contiguous
+// range of rows whose x working set then stays in that XCD's L2.
 __global__ __launch_bounds__(256) void spmv_kernel(
     int64_t m, int64_t per_xcd, const int64_t* __restrict__ rowptr,
     const int32_t* __restrict__ colind, const double* __restrict__ nzval,
     int64_t indexing, double alpha, const double* __restrict__ x, double beta,
     double* __restrict__ y) {
   const int64_t lblk = (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-  const int64_t row = lblk * kWavesPerBlock + (threadIdx.x >> 6);
-  if(row >= m) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t st = rowptr[row] - indexing, en = rowptr[row + 1] - indexing;
+  const int64_t row = lblk * kRowsPerBlock + threadIdx.x / kLanesPerRow;
+  const int l = threadIdx.x % kLanesPerRow;
   const double* xb = x - indexing;
-  const int64_t qa = (st + 3) >> 2, qb = en >> 2;  // whole quads [qa, qb)
   double acc0 = 0., acc1 = 0., acc2 = 0., acc3 = 0.;
-  // head [st, min(en, 4 qa)) and tail [max(4 qb, 4 qa), en): at most 3 each
-  {
-    const int64_t he = en < 4 * qa ? en : 4 * qa;
-    const int64_t ts = 4 * qb > 4 * qa ? 4 * qb : 4 * qa;
-    int64_t j = -1;
-    if(lane < 3) j = st + lane < he ? st + lane : -1;
-    else if(lane < 6) j = ts + (lane - 3) < en ? ts + (lane - 3) : -1;
-    if(j >= 0) acc0 = nzval[j] * xb[colind[j]];
-  }
-  const i32x4* cq = reinterpret_cast<const i32x4*>(colind);
-  const f64x2* vq = reinterpret_cast<const f64x2*>(nzval);
-  int64_t q = qa + lane;
-  for(; q + 64 < qb; q += 128) {
-    const i32x4 c0 = __builtin_nontemporal_load(cq + q);
-    const i32x4 c1 = __builtin_nontemporal_load(cq + q + 64);
-    const f64x2 v0 = __builtin_nontemporal_load(vq + 2 * q);
-    const f64x2 v1 = __builtin_nontemporal_load(vq + 2 * q + 1);
-    const f64x2 v2 = __builtin_nontemporal_load(vq + 2 * (q + 64));
-    const f64x2 v3 = __builtin_nontemporal_load(vq + 2 * (q + 64) + 1);
-    const double x0 = xb[c0.x], x1 = xb[c0.y], x2 = xb[c0.z], x3 = xb[c0.w];
-    const double x4 = xb[c1.x], x5 = xb[c1.y], x6 = xb[c1.z], x7 = xb[c1.w];
-    acc0 = fma(v0.x, x0, acc0);
-    acc1 = fma(v0.y, x1, acc1);
-    acc2 = fma(v1.x, x2, acc2);
-    acc3 = fma(v1.y, x3, acc3);
-    acc0 = fma(v2.x, x4, acc0);
-    acc1 = fma(v2.y, x5, acc1);
-    acc2 = fma(v3.x, x6, acc2);
-    acc3 = fma(v3.y, x7, acc3);
-  }
-  if(q < qb) {
-    const i32x4 c0 = __builtin_nontemporal_load(cq + q);
-    const f64x2 v0 = __builtin_nontemporal_load(vq + 2 * q);
-    const f64x2 v1 = __builtin_nontemporal_load(vq + 2 * q + 1);
-    acc0 = fma(v0.x, xb[c0.x], acc0);
-    acc1 = fma(v0.y, xb[c0.y], acc1);
-    acc2 = fma(v1.x, xb[c0.z], acc2);
-    acc3 = fma(v1.y, xb[c0.w], acc3);
+  if(row < m) {
+    const int64_t st = rowptr[row] - indexing, en = rowptr[row + 1] - indexing;
+    const int64_t qa = (st + 3) >> 2, qb = en >> 2;  // whole quads [qa, qb)
+    // head [st, min(en, 4 qa)) and tail [max(4 qb, 4 qa), en): at most 3 each
+    {
+      const int64_t he = en < 4 * qa ? en : 4 * qa;
+      const int64_t ts = 4 * qb > 4 * qa ? 4 * qb : 4 * qa;
+      int64_t j = -1;
+      if(l < 3) j = st + l < he ? st + l : -1;
+      else if(l < 6) j = ts + (l - 3) < en ? ts + (l - 3) : -1;
+      if(j >= 0) acc0 = nzval[j] * xb[colind[j]];
+    }
+    const i32x4* cq = reinterpret_cast<const i32x4*>(colind);
+    const f64x2* vq = reinterpret_cast<const f64x2*>(nzval);
+    int64_t q = qa + l;
+    for(; q + kLanesPerRow < qb; q += 2 * kLanesPerRow) {
+      const i32x4 c0 = __builtin_nontemporal_load(cq + q);
+      const i32x4 c1 = __builtin_nontemporal_load(cq + q + kLanesPerRow);
+      const f64x2 v0 = __builtin_nontemporal_load(vq + 2 * q);
+      const f64x2 v1 = __builtin_nontemporal_load(vq + 2 * q + 1);
+      const f64x2 v2 = __builtin_nontemporal_load(vq + 2 * (q + kLanesPerRow));
+      const f64x2 v3 = __builtin_nontemporal_load(vq + 2 * (q + kLanesPerRow) + 1);
+      acc0 = fma(v0.x, xb[c0.x], acc0);
+      acc1 = fma(v0.y, xb[c0.y], acc1);
+      acc2 = fma(v1.x, xb[c0.z], acc2);
+      acc3 = fma(v1.y, xb[c0.w], acc3);
+      acc0 = fma(v2.x, xb[c1.x], acc0);
+      acc1 = fma(v2.y, xb[c1.y], acc1);
+      acc2 = fma(v3.x, xb[c1.z], acc2);
+      acc3 = fma(v3.y, xb[c1.w], acc3);
+    }
+    if(q < qb) {
+      const i32x4 c0 = __builtin_nontemporal_load(cq + q);
+      const f64x2 v0 = __builtin_nontemporal_load(vq + 2 * q);
+      const f64x2 v1 = __builtin_nontemporal_load(vq + 2 * q + 1);
+      acc0 = fma(v0.x, xb[c0.x], acc0);
+      acc1 = fma(v0.y, xb[c0.y], acc1);
+      acc2 = fma(v1.x, xb[c0.z], acc2);
+      acc3 = fma(v1.y, xb[c0.w], acc3);
+    }
   }
   double acc = (acc0 + acc1) + (acc2 + acc3);
-  acc = wave_sum(acc);
-  if(lane == 0) y[row] = alpha * acc + beta * y[row];  // spmbv.hpp:71
+#pragma unroll
+  for(int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if(row < m && l == 0) y[row] = alpha * acc + beta * y[row];  // spmbv.hpp:71
 }
 
 __global__ __launch_bounds__(256) void extract_diag_kernel(
@@ -117,7 +123,7 @@ __global__ __launch_bounds__(256) void extract_diag_kernel(
 void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
           double beta, double* y) {
   if(A->m == 0) return;
-  const int64_t nblk = (A->m + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t nblk = (A->m + kRowsPerBlock - 1) / kRowsPerBlock;
   const int64_t per = (nblk + kXcds - 1) / kXcds;
   hipLaunchKernelGGL(spmv_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m, per,
                      A->rowptr.as<int64_t>(), A->colind.as<int32_t>(), A->nzval.as<double>(),

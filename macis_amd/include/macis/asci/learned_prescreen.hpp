@@ -63,15 +63,22 @@ class LearnedPrescreen {
     return n;
   }
 
-  // Adam epochs over the observed pairs (targets standardised); returns the
-  // last mini-batch loss.
-  double train(int epochs, int64_t batch, float lr, uint64_t seed = 1) {
+  // Adam epochs over the observed pairs; returns the last mini-batch loss.
+  // Targets below the clip_quantile quantile are raised to it (the ranking
+  // among candidates that cannot enter the top-k does not matter), then
+  // standardised.
+  double train(int epochs, int64_t batch, float lr, uint64_t seed = 1,
+               double clip_quantile = 0.7) {
     const size_t n = target_.size();
     if(!n) return 0.;
+    std::vector<float> sorted(target_);
+    const size_t qi = std::min(n - 1, (size_t)(clip_quantile * (double)(n - 1)));
+    std::nth_element(sorted.begin(), sorted.begin() + qi, sorted.end());
+    clip_ = sorted[qi];
     double mu = 0., var = 0.;
-    for(float t : target_) mu += t;
+    for(float t : target_) mu += std::max(t, clip_);
     mu /= n;
-    for(float t : target_) var += (t - mu) * (t - mu);
+    for(float t : target_) var += (std::max(t, clip_) - mu) * (std::max(t, clip_) - mu);
     mu_ = mu;
     sd_ = std::sqrt(var / n) + 1e-12;
     std::vector<uint32_t> perm(n);
@@ -90,7 +97,7 @@ class LearnedPrescreen {
       for(size_t i = 0; i < n; ++i) {
         const size_t p = perm[i];
         std::copy_n(dets_.begin() + p * W, W, hd.begin() + i * W);
-        ht[i] = (float)((target_[p] - mu_) / sd_);
+        ht[i] = (float)((std::max(target_[p], clip_) - mu_) / sd_);
       }
       bd.upload(hd.data(), n * W);
       bt.upload(ht.data(), n);
@@ -174,6 +181,7 @@ class LearnedPrescreen {
   std::vector<uint64_t> last_dets_;
   std::vector<double> last_logrv_;
   double mu_ = 0., sd_ = 1., recall_ = 0.;
+  float clip_ = 0.f;
   bool trained_ = false, used_fallback_ = true;
 };
 

@@ -68,7 +68,7 @@ def test_spmv_ragged_rows(ctx, indexing):
     rowptr = np.zeros(m + 1, dtype=np.int64)
     rowptr[1:] = np.cumsum(lens)
     nnz = int(rowptr[-1])
-    colind = np.concatenate([np.sort(rng.choice(n, size=min(L, n), replace=L > n))
+    colind = np.concatenate([np.sort(rng.choice(n, size=L, replace=L > n))
                              for L in lens]).astype(np.int32)
     nzval = rng.standard_normal(nnz)
     A = ctx.csr_from_host(m, n, rowptr + indexing, colind + indexing, nzval, indexing=indexing)

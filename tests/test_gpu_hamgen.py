@@ -168,3 +168,75 @@ def test_cas12_full_size_properties(ctx):
         L.mhip_memcpy_d2h(ctx.h, gv.ctypes.data, d_nz + 8 * lo, gv.nbytes)
         assert np.array_equal(gc, col)
         assert np.array_equal(gv, val)
+
+
+def _bitset_less_sorted(dets, W):
+    """The list in bitset_less order (bitset_operations.hpp:289-307): the
+    beta string is the most significant part."""
+    if W == 1:
+        key = [(int(x[0]) >> 32, int(x[0]) & 0xffffffff) for x in dets]
+    else:
+        key = [(int(x[1]), int(x[0])) for x in dets]
+    return dets[sorted(range(len(dets)), key=lambda i: key[i])]
+
+
+@pytest.mark.parametrize("W,n,na,nb,order", [(1, 8, 4, 4, "generated"), (1, 8, 4, 3, "sorted"),
+                                             (2, 10, 3, 2, "generated"), (2, 9, 4, 4, "sorted"),
+                                             (1, 10, 5, 5, "generated"), (1, 7, 0, 3, "generated")])
+def test_build_paths_agree(ctx, W, n, na, nb, order):
+    """The product-space passes and the tiled double loop give the same CSR
+    (pattern and bits) on complete spaces in both determinant orders."""
+    from macis_amd import _capi
+
+    announce(f"hamgen paths n={n} na={na} nb={nb} W={W} {order}")
+    T, V = seeded_integrals(n, 40 + n)
+    ctx.set_integrals(n, T, V)
+    dets = D.generate_hilbert_space(W, n, na, nb)
+    if order == "sorted":
+        dets = _bitset_less_sorted(dets, W)
+    dd = ctx.to_device(dets)
+    out = {}
+    for path in (_capi.Context.BUILD_TILED, _capi.Context.BUILD_PRODUCT, _capi.Context.BUILD_AUTO):
+        ctx.set_build_path(path)
+        H = ctx.ham_build(W, dd, len(dets), thresh=EPS)
+        out[path] = (H.to_host(), ctx.timings()["ham_path"])
+    ctx.set_build_path(_capi.Context.BUILD_AUTO)
+    assert out[_capi.Context.BUILD_TILED][1] == _capi.Context.BUILD_TILED
+    assert out[_capi.Context.BUILD_PRODUCT][1] == _capi.Context.BUILD_PRODUCT
+    assert out[_capi.Context.BUILD_AUTO][1] == _capi.Context.BUILD_PRODUCT
+    ref = out[_capi.Context.BUILD_TILED][0]
+    for path in (_capi.Context.BUILD_PRODUCT, _capi.Context.BUILD_AUTO):
+        assert_same_csr(out[path][0], ref)
+    if len(dets) <= 5000:
+        assert_same_csr(ref, O.Ham(T, V).csr(W, dets, thresh=EPS))
+
+
+def test_product_path_only_for_product_lists(ctx, h2o, h2o_cisd):
+    """A CISD list (not a product of spin strings), a list with a repeated
+    determinant and a rectangular block take the tiled passes; forcing the
+    product passes on them is an error."""
+    from macis_amd import _capi
+
+    announce("hamgen product-path recognition")
+    norb, _, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    d = ctx.to_device(h2o_cisd)
+    ctx.ham_build(1, d, len(h2o_cisd), thresh=1e-16)
+    assert ctx.timings()["ham_path"] == _capi.Context.BUILD_TILED
+    ctx.set_build_path(_capi.Context.BUILD_PRODUCT)
+    try:
+        with pytest.raises(_capi.MhipError):
+            ctx.ham_build(1, d, len(h2o_cisd), thresh=1e-16)
+        n = 6
+        T6, V6 = seeded_integrals(n, 3)
+        ctx.set_integrals(n, T6, V6)
+        dets = D.generate_hilbert_space(1, n, 3, 3)
+        rep = np.concatenate([dets, dets[:20]])  # 420 = 21 runs of 20: repeats
+        with pytest.raises(_capi.MhipError):
+            ctx.ham_build(1, ctx.to_device(rep), len(rep), thresh=EPS)
+        ctx.set_build_path(_capi.Context.BUILD_AUTO)
+        H = ctx.ham_build(1, ctx.to_device(rep), len(rep), thresh=EPS)
+        assert ctx.timings()["ham_path"] == _capi.Context.BUILD_TILED
+        assert_same_csr(H.to_host(), O.Ham(T6, V6).csr(1, rep, thresh=EPS))
+    finally:
+        ctx.set_build_path(_capi.Context.BUILD_AUTO)

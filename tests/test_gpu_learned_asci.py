@@ -3,7 +3,9 @@
 Plain GPU ASCI steps grow the wavefunction HF -> 100 -> 800 -> 6400
 determinants (asci_iter, iteration.hpp:16-67); every step also exports its
 scored candidates (determinant, accumulated |rv|). The network is trained
-by the project's own BF16 kernels on those (candidate, log|rv|) pairs and
+by the project's own BF16 kernels on those (candidate, log|rv|) pairs (the
+bottom 70% of targets clipped to their 70th percentile, so the network
+spends its capacity on ranking the candidates that can enter the top-k) and
 then screens the NEXT step, a refinement step at 6400 determinants from the
 6400-determinant wavefunction that it has never seen: the best-predicted 5%
 of the candidates survive and their exact scores pick the top-k. Reported:
@@ -21,7 +23,8 @@ from conftest import announce
 
 pytestmark = pytest.mark.gpu
 KEEP = 0.05
-EPOCHS, BATCH, LR = 30, 4096, 1e-3
+EPOCHS, BATCH, LR = 20, 4096, 1e-3
+CLIP_Q = 0.7  # targets below this quantile are clipped: the ranking of the top matters
 
 
 def core_of(wfn, X, ncore=100):
@@ -74,6 +77,7 @@ def test_learned_prescreen_recall_energy_and_gain(ctx, h2o):
         E0, X = diag(ctx, W, wfn)
     Xd = np.concatenate(feats)
     y = np.concatenate(targ).astype(np.float32)
+    y = np.maximum(y, np.quantile(y, CLIP_Q))
     yn = ((y - y.mean()) / y.std()).astype(np.float32)
     net = _capi.MLP(ctx, seed=3)
     untrained = _capi.MLP(ctx, seed=3)
