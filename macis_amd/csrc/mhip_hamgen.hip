@@ -22,6 +22,11 @@
 //                |h| > thresh.
 //   6. compact : only when some element fell at or below the threshold,
 //                a second scan + a stable ballot compaction drops them.
+//
+// Complete-space lists (a product of spin strings, see mhip_product.hip)
+// replace steps 1, 2, 4 and 5 by a closed-form count and a fill pass that
+// walks the spin-string neighbour tables and evaluates the elements in the
+// same pass; scan and compaction are shared.
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -451,34 +456,42 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   tm.ham_nnz_struct = nnz_struct;
   counts32.release();
 
-  // 4. fill
+  // 4. fill (the product path evaluates the elements in the same pass)
   DevBuf col(sizeof(int32_t) * std::max<int64_t>(nnz_struct, 1));
   DevBuf val(sizeof(double) * std::max<int64_t>(nnz_struct, 1));
+  DevBuf diag(sizeof(double) * nbra);
+  DevBuf flag(sizeof(int));
+  MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
   t.start();
-  if(nnz_struct && ex.product)
-    product_fill(ctx, *ex.product, nbra, rp_struct.as<int64_t>(), col.as<int32_t>());
-  else if(nnz_struct)
+  if(ex.product) {
+    hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
+                       nbra, diag.as<double>(), ex.views, ex.row_inst);
+    if(nnz_struct)
+      product_fill_elements(ctx, W, *ex.product, I, nbra, diag.as<double>(),
+                            rp_struct.as<int64_t>(), col.as<int32_t>(), val.as<double>(), thresh,
+                            counts64.as<int64_t>(), flag.as<int>());
+  } else if(nnz_struct) {
     hipLaunchKernelGGL((scan_pass_kernel<W, kRows, true>), dim3(grid), dim3(256), 0, st,
                        d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
                        nwaves, nblocks, ex.items, nullptr, rp_struct.as<int64_t>(),
                        col.as<int32_t>());
+  }
   MHIP_HIP_CHECK(hipGetLastError());
   t.stop();
   tm.ham_fill_ms = t.ms();
   bnd.release();
 
-  // 5. elements
-  DevBuf diag(sizeof(double) * nbra);
-  DevBuf flag(sizeof(int));
-  MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+  // 5. elements (tiled path)
   t.start();
-  hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
-                     nbra, diag.as<double>(), ex.views, ex.row_inst);
-  if(nnz_struct)
-    hipLaunchKernelGGL(element_kernel<W>, dim3((nbra + 3) / 4), dim3(256), 0, st, I, d_bra,
-                       nbra, d_ket, diag.as<double>(), rp_struct.as<int64_t>(),
-                       col.as<int32_t>(), val.as<double>(), thresh,
-                       counts64.as<int64_t>(), flag.as<int>(), ex.views, ex.row_inst);
+  if(!ex.product) {
+    hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
+                       nbra, diag.as<double>(), ex.views, ex.row_inst);
+    if(nnz_struct)
+      hipLaunchKernelGGL(element_kernel<W>, dim3((nbra + 3) / 4), dim3(256), 0, st, I, d_bra,
+                         nbra, d_ket, diag.as<double>(), rp_struct.as<int64_t>(),
+                         col.as<int32_t>(), val.as<double>(), thresh,
+                         counts64.as<int64_t>(), flag.as<int>(), ex.views, ex.row_inst);
+  }
   MHIP_HIP_CHECK(hipGetLastError());
   int dropped = 0;
   MHIP_HIP_CHECK(hipMemcpyAsync(&dropped, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -177,14 +177,18 @@ struct ProductSpace {
   bool outer_alpha = true;
   DevBuf onb_ptr, onb, ocnt;         // int32 [nruns+1], int32 [..], int32 [3*nruns]
   DevBuf ptr1, idx1, ptr2, idx2;     // int32 inner lists (levels <= 1, <= 2)
+  DevBuf ostr, istr;                 // uint64 outer / inner strings
 };
 // Recognises a product-space list (host + device checks); false otherwise.
 bool detect_product_space(mhip_context* ctx, int W, const uint64_t* d, int64_t n,
                           ProductSpace& ps);
-// Structural row counts / ascending columns of a product-space list.
+// Structural row counts of a product-space list.
 void product_count(mhip_context* ctx, const ProductSpace& ps, int64_t n, uint32_t* counts);
-void product_fill(mhip_context* ctx, const ProductSpace& ps, int64_t n, const int64_t* rowptr,
-                  int32_t* colind);
+// Fill + Slater-Condon elements in one pass (kept counts and dropped flag
+// as the element pass of the tiled path).
+void product_fill_elements(mhip_context* ctx, int W, const ProductSpace& ps, const IntsView& I,
+                           int64_t n, const double* diag, const int64_t* rowptr, int32_t* colind,
+                           double* nzval, double thresh, int64_t* kept, int* dropped);
 // Optional work-item table / per-instance integrals of a batched build.
 struct BuildExtra {
   const int64_t* items = nullptr;  // DEVICE, 4 int64 per item

@@ -12,9 +12,11 @@
 // and the columns of row r*L + k are, for every outer neighbour r' at
 // excitation level e = popcount(O[r] ^ O[r']) / 2 <= 2 (r' ascending), the
 // inner neighbours k' of k at level <= 2 - e (k' ascending), at column
-// r'*L + k'. The count pass is closed form, the fill pass writes those
-// segments; the element and threshold passes are shared with the tiled
-// path, so both paths give the identical CSR.
+// r'*L + k'. The count pass is closed form; the fill pass writes those
+// segments and evaluates their Slater-Condon elements with the same device
+// function and diagonal as the tiled path's element pass, and counts
+// |h| > thresh for the shared compaction, so both paths give the identical
+// CSR.
 //
 // Recognition: the first run length L is read from the head of the list, a
 // device pass checks outer(d[i]) == O[i / L] and inner(d[i]) == I[i % L] for
@@ -89,39 +91,83 @@ __global__ void product_count_kernel(int64_t n, int64_t L, const int32_t* __rest
               (uint32_t)ocnt[3 * r + 2];
 }
 
-// One wavefront per row writes its column segments in ascending order.
-__global__ __launch_bounds__(256) void product_fill_kernel(
-    int64_t n, int64_t L, const int32_t* __restrict__ onb_ptr, const int32_t* __restrict__ onb,
-    const int32_t* __restrict__ ptr1, const int32_t* __restrict__ idx1,
-    const int32_t* __restrict__ ptr2, const int32_t* __restrict__ idx2,
-    const int64_t* __restrict__ rowptr, int32_t* __restrict__ colind) {
+// Fill and elements in one pass: one wavefront per row. Lanes take 64 outer
+// neighbours at a time; a wavefront prefix sum of their segment lengths
+// gives every segment's offset. Level-2 neighbours (one column each:
+// the same inner string) are evaluated by their own lane; level-0/1
+// segments are then written by the whole wavefront, one segment at a time.
+// Elements are the Slater-Condon values of the tiled path's element pass
+// (same function, same diagonal), so the CSR is bit-identical.
+__global__ __launch_bounds__(256) void product_fill_elements_kernel(
+    IntsView I, int64_t n, int64_t L, int outer_alpha, const uint64_t* __restrict__ ostr,
+    const uint64_t* __restrict__ istr, const int32_t* __restrict__ onb_ptr,
+    const int32_t* __restrict__ onb, const int32_t* __restrict__ ptr1,
+    const int32_t* __restrict__ idx1, const int32_t* __restrict__ ptr2,
+    const int32_t* __restrict__ idx2, const double* __restrict__ diag,
+    const int64_t* __restrict__ rowptr, int32_t* __restrict__ colind, double* __restrict__ nzval,
+    double thresh, int64_t* __restrict__ kept, int* __restrict__ dropped) {
   const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if(row >= n) return;
   const int lane = threadIdx.x & 63;
   const int64_t r = row / L, k = row % L;
-  int64_t pos = rowptr[row];
+  const uint64_t bo = ostr[r], bi = istr[k];
+  const uint64_t ba = outer_alpha ? bo : bi, bb = outer_alpha ? bi : bo;
+  const double d = diag[row];
+  const int64_t st = rowptr[row];
   const int32_t s1 = ptr1[k], len1 = ptr1[k + 1] - s1;
   const int32_t s2 = ptr2[k], len2 = ptr2[k + 1] - s2;
-  // the row's inner neighbour lists, kept in registers when they fit
-  const int32_t c1 = lane < len1 ? idx1[s1 + lane] : 0;
-  for(int32_t j = onb_ptr[r]; j < onb_ptr[r + 1]; ++j) {
-    const int32_t packed = onb[j];
-    const int64_t base = (int64_t)(packed >> 2) * L;
+  int64_t pos = st, nk = 0;
+  const int32_t j0 = onb_ptr[r], j1 = onb_ptr[r + 1];
+  for(int32_t jb = j0; jb < j1; jb += 64) {
+    const int32_t j = jb + lane;
+    const bool valid = j < j1;
+    const int32_t packed = valid ? onb[j] : 0;
+    const int64_t r2 = packed >> 2;
     const int e = packed & 3;
-    if(e == 2) {
-      if(lane == 0) colind[pos] = (int32_t)(base + k);
-      pos += 1;
-    } else if(e == 1) {
-      if(len1 <= 64) {
-        if(lane < len1) colind[pos + lane] = (int32_t)(base + c1);
-      } else {
-        for(int32_t t = lane; t < len1; t += 64) colind[pos + t] = (int32_t)(base + idx1[s1 + t]);
-      }
-      pos += len1;
-    } else {
-      for(int32_t t = lane; t < len2; t += 64) colind[pos + t] = (int32_t)(base + idx2[s2 + t]);
-      pos += len2;
+    const int32_t len = !valid ? 0 : (e == 2 ? 1 : (e == 1 ? len1 : len2));
+    // inclusive wavefront prefix sum of the segment lengths
+    int64_t incl = len;
+#pragma unroll
+    for(int off = 1; off < 64; off <<= 1) {
+      const int64_t v = __shfl_up(incl, off, 64);
+      if(lane >= off) incl += v;
     }
+    const int64_t my = pos + incl - len;
+    const int64_t total = __shfl(incl, 63, 64);
+    if(valid && e == 2) {  // same inner string, outer double excitation
+      const uint64_t ko = ostr[r2];
+      const uint64_t ka = outer_alpha ? ko : bi, kb = outer_alpha ? bi : ko;
+      const double h = matrix_element(I, ba, bb, ka, kb, d);
+      colind[my] = (int32_t)(r2 * L + k);
+      nzval[my] = h;
+      nk += fabs(h) > thresh ? 1 : 0;
+    }
+    uint64_t seg = ballot(valid && e < 2);
+    while(seg) {
+      const int src = (int)ctz64(seg);
+      seg &= seg - 1;
+      const int64_t sp = __shfl(my, src, 64);
+      const int64_t sr = __shfl(r2, src, 64);
+      const int se = __shfl(e, src, 64);
+      const uint64_t ko = ostr[sr];
+      const int32_t sl = se == 1 ? len1 : len2;
+      const int32_t* lst = se == 1 ? idx1 + s1 : idx2 + s2;
+      for(int32_t t = lane; t < sl; t += 64) {
+        const int32_t k2 = lst[t];
+        const uint64_t ki = istr[k2];
+        const uint64_t ka = outer_alpha ? ko : ki, kb = outer_alpha ? ki : ko;
+        const double h = matrix_element(I, ba, bb, ka, kb, d);
+        colind[sp + t] = (int32_t)(sr * L + k2);
+        nzval[sp + t] = h;
+        nk += fabs(h) > thresh ? 1 : 0;
+      }
+    }
+    pos += total;
+  }
+  nk = wave_sum(nk);
+  if(lane == 0) {
+    kept[row] = nk;
+    if(nk != pos - st) atomicOr(dropped, 1);
   }
 }
 
@@ -226,6 +272,8 @@ bool detect(mhip_context* ctx, const uint64_t* d, int64_t n, ProductSpace& ps) {
   upload(ctx, ps.idx1, i1);
   upload(ctx, ps.ptr2, p2);
   upload(ctx, ps.idx2, i2);
+  upload(ctx, ps.ostr, O);
+  upload(ctx, ps.istr, I);
   MHIP_HIP_CHECK(hipStreamSynchronize(st));
   return true;
 }
@@ -244,12 +292,16 @@ void product_count(mhip_context* ctx, const ProductSpace& ps, int64_t n, uint32_
   MHIP_HIP_CHECK(hipGetLastError());
 }
 
-void product_fill(mhip_context* ctx, const ProductSpace& ps, int64_t n, const int64_t* rowptr,
-                  int32_t* colind) {
-  hipLaunchKernelGGL(product_fill_kernel, dim3((n + kWavesPerBlock - 1) / kWavesPerBlock),
-                     dim3(256), 0, ctx->stream, n, ps.L, ps.onb_ptr.as<int32_t>(),
-                     ps.onb.as<int32_t>(), ps.ptr1.as<int32_t>(), ps.idx1.as<int32_t>(),
-                     ps.ptr2.as<int32_t>(), ps.idx2.as<int32_t>(), rowptr, colind);
+void product_fill_elements(mhip_context* ctx, int W, const ProductSpace& ps, const IntsView& I,
+                           int64_t n, const double* diag, const int64_t* rowptr, int32_t* colind,
+                           double* nzval, double thresh, int64_t* kept, int* dropped) {
+  (void)W;  // the strings are stored split; the kernel does not read determinants
+  hipLaunchKernelGGL(product_fill_elements_kernel,
+                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, ctx->stream, I,
+                     n, ps.L, ps.outer_alpha ? 1 : 0, ps.ostr.as<uint64_t>(),
+                     ps.istr.as<uint64_t>(), ps.onb_ptr.as<int32_t>(), ps.onb.as<int32_t>(),
+                     ps.ptr1.as<int32_t>(), ps.idx1.as<int32_t>(), ps.ptr2.as<int32_t>(),
+                     ps.idx2.as<int32_t>(), diag, rowptr, colind, nzval, thresh, kept, dropped);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 
