@@ -43,15 +43,19 @@ def _gumbel_choose(rng, logw, k):
     """Per row, choose k[i] distinct columns with probability proportional to
     exp(logw) (Gumbel top-k). Returns a boolean (rows, cols) mask."""
     g = logw[None, :] - np.log(-np.log(rng.random((len(k), len(logw)))))
-    order = np.argsort(-g, axis=1)
-    rank = np.empty_like(order)
-    rows = np.arange(len(k))[:, None]
-    rank[rows, order] = np.arange(len(logw))[None, :]
-    return rank < k[:, None]
+    mask = np.zeros(g.shape, dtype=bool)
+    rows = np.arange(len(k))
+    for j in range(int(k.max(initial=0))):  # k[i] largest, one argmax at a time
+        idx = np.argmax(g, axis=1)
+        take = k > j
+        mask[rows[take], idx[take]] = True
+        g[rows, idx] = -np.inf
+    return mask
 
 
 def selected_ci_dets(n: int, nalpha: int, nbeta: int, ndets: int, seed: int,
-                     T=None, weights=(0.1, 0.5, 0.2, 0.2), batch: int = 400_000):
+                     T=None, weights=(0.1, 0.5, 0.2, 0.2), batch: int = 400_000,
+                     energies=None):
     """Seeded selected-CI determinant list for wfn_t<128> (W = 2).
 
     The HF determinant plus random 1-4-fold excitations of it: the level L
@@ -60,13 +64,20 @@ def selected_ci_dets(n: int, nalpha: int, nbeta: int, ndets: int, seed: int,
     drawn without replacement with probability proportional to
     exp(-|e_p - e_F|), e_p = h_pp (one-body diagonal, or p when T is None) and
     e_F halfway between the highest occupied and lowest virtual e_p of each
-    spin. Unique dets are collected until `ndets` (HF included), sorted by
+    spin; `energies` (length n) overrides e_p. With the seeded h_pp (spacing
+    about 2 Ha from the diagonal offset) only a few orbitals around e_F carry
+    weight and a few 1e4 determinants are reachable, so lists of millions use
+    `energies` with a finer spacing. Unique dets are collected until `ndets`
+    (HF included), sorted by
     bitset_less (beta word most significant, then alpha) and truncated.
     Returns an (ndets, 2) uint64 array [alpha, beta].
     """
     assert n <= 64
     rng = np.random.Generator(np.random.PCG64(seed))
-    e = np.arange(n, dtype=float) if T is None else np.asarray(T).reshape(n, n, order="F").diagonal().copy()
+    if energies is not None:
+        e = np.asarray(energies, dtype=float).copy()
+    else:
+        e = np.arange(n, dtype=float) if T is None else np.asarray(T).reshape(n, n, order="F").diagonal().copy()
     order = np.argsort(e, kind="stable")
     def spin_setup(ne):
         occ = order[:ne]

@@ -240,3 +240,41 @@ def test_product_path_only_for_product_lists(ctx, h2o, h2o_cisd):
         assert_same_csr(H.to_host(), O.Ham(T6, V6).csr(1, rep, thresh=EPS))
     finally:
         ctx.set_build_path(_capi.Context.BUILD_AUTO)
+
+
+def test_selected_ci_64_orbitals_properties(ctx):
+    """Selected-CI list at ASCI scale (64 orbitals, 20a20b, wfn_t<128>),
+    200,000 determinants: sampled rows equal the oracle's double loop, the
+    pattern is symmetric with ascending columns, and the lowest root of the
+    restarted Davidson is an eigenpair (Rayleigh quotient, residual)."""
+    announce("hamgen 64 orbitals 20a20b, 200000 selected dets, W=2")
+    n = 64
+    T, V = seeded_integrals(n, 64)
+    ctx.set_integrals(n, T, V)
+    dets = selected_ci_dets(n, 20, 20, 200_000, seed=64, energies=0.25 * np.arange(n))
+    N = len(dets)
+    dd = ctx.to_device(dets)
+    H = ctx.ham_build(2, dd, N, thresh=EPS)
+    rp, ci, nz = H.to_host()
+    orc = O.Ham(T, V)
+    for r0 in (0, 77_777, N - 8):
+        cnt, col, val = orc.csr_rows(2, dets, r0, r0 + 8, thresh=EPS)
+        assert np.array_equal(np.diff(rp[r0:r0 + 9]), cnt)
+        assert np.array_equal(ci[rp[r0]:rp[r0 + 8]], col)
+        assert np.array_equal(nz[rp[r0]:rp[r0 + 8]], val)
+    rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(rp))
+    assert np.all((np.diff(ci) > 0) | (np.diff(rows) > 0))
+    key = rows * N + ci
+    keyT = ci.astype(np.int64) * N + rows
+    assert np.array_equal(np.sort(key), np.sort(keyT))
+    dX = ctx.empty(N, np.float64).zero()
+    E, nnz, _ = ctx.selected_ci_diag(2, dd, N, dX, max_restarts=200)
+    assert nnz == H.nnz
+    x = dX.numpy()
+    assert abs(np.linalg.norm(x) - 1.0) < 1e-12
+    Hx = np.zeros(N)
+    np.add.at(Hx, rows, nz * x[ci])
+    assert abs(x @ Hx - E) <= 1e-12 * abs(E)
+    # the device residual is below 1e-8; the host product adds rounding of
+    # order 1e-16 |H| |x| per row
+    assert np.linalg.norm(Hx - E * x) <= 1e-8 + 1e-14 * abs(E) * np.sqrt(N)
