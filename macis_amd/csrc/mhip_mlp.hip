@@ -43,6 +43,7 @@ enum Epi : int {
   EPI_F32 = 0,        // C (fp32)
   EPI_BIAS_GELU = 1,  // Z = C + bias (fp32), A = gelu(Z) (bf16)
   EPI_GELU_GRAD = 2,  // D = C * gelu'(Zaux) (bf16)
+  EPI_BIAS_GELU_A = 3,  // A = gelu(C + bias) (bf16) only: inference
 };
 
 // C[m][n] = sum_k A(m,k) B(n,k); A(m,k) = AK ? A[m*lda+k] : A[k*lda+m],
@@ -57,6 +58,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  // split K over gridDim.z (fp32 output only): slice z owns
+  // [z*K/Z, (z+1)*K/Z) and writes its partial product at Cf + z*M*ldc
+  const int kz = K / (int)gridDim.z, kbeg = (int)blockIdx.z * kz, kend = kbeg + kz;
+  if constexpr(EPI == EPI_F32) Cf += (size_t)blockIdx.z * M * ldc;
   f32x16_t acc[2][2];
 #pragma unroll
   for(int i = 0; i < 2; ++i)
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
 #pragma unroll
       for(int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-  for(int k0 = 0; k0 < K; k0 += BK) {
+  for(int k0 = kbeg; k0 < kend; k0 += BK) {
     // ---- stage A and B tiles (K-contiguous in LDS)
 #pragma unroll
     for(int it = 0; it < 2; ++it) {
@@ -128,6 +133,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
           v += bias[col];
           Cf[o] = v;
           Cb[o] = (__bf16)gelu(v);
+        } else if constexpr(EPI == EPI_BIAS_GELU_A) {
+          Cb[o] = (__bf16)gelu(v + bias[col]);
         } else {
           Cb[o] = (__bf16)(v * gelu_grad(zaux[o]));
         }
@@ -189,14 +196,29 @@ __global__ void head_back_kernel(const float* __restrict__ dy, const float* __re
   dZ2[i] = (__bf16)(dy[i / kHid] * w3[j] * gelu_grad(Z2[i]));
 }
 
-// Column sums: out[j] = sum_b M[b][j] (bf16 rows) -- bias gradients; and
-// dw3[j] = sum_b A2[b][j] dy[b].
-__global__ void colsum_kernel(const __bf16* __restrict__ Mt, int64_t n, int ncol,
-                              const float* __restrict__ wrow, float* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if(j >= ncol) return;
+// Column sums over row slices: part[s][j] = sum_{b in slice s} M[b][j] w[b]
+// (bf16 rows; w = 1 for bias gradients, dy for dw3). Block s, thread t
+// covers columns t, t + 256, ...
+__global__ __launch_bounds__(256) void colsum_part_kernel(const __bf16* __restrict__ Mt,
+                                                          int64_t n, int ncol,
+                                                          const float* __restrict__ wrow,
+                                                          float* __restrict__ part) {
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
+  for(int j = threadIdx.x; j < ncol; j += 256) {
+    float s = 0.f;
+    for(int64_t b = b0; b < b1; ++b) s += (float)Mt[b * ncol + j] * (wrow ? wrow[b] : 1.f);
+    part[(int64_t)blockIdx.x * ncol + j] = s;
+  }
+}
+
+// out[j] = sum_s part[s * stride + j] (fixed slice order).
+__global__ void slice_sum_kernel(const float* __restrict__ part, int nslice, int64_t stride,
+                                 int64_t count, float* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(j >= count) return;
   float s = 0.f;
-  for(int64_t b = 0; b < n; ++b) s += (float)Mt[b * ncol + j] * (wrow ? wrow[b] : 1.f);
+  for(int z = 0; z < nslice; ++z) s += part[(int64_t)z * stride + j];
   out[j] = s;
 }
 
@@ -249,7 +271,10 @@ struct Mlp {
   // activation workspace for up to `cap` rows
   int64_t cap = 0;
   DevBuf X, Z1, A1, Z2, A2, y, dy, dZ2, dZ1, dA1, loss;
+  DevBuf wk;  // split-K partial weight gradients / column-sum slices
   void reserve(int64_t rows) {
+    // split-K / column-sum slices: at most 32 x 512 x 512 or 256 x 512 floats
+    wk.reserve(sizeof(float) * (size_t)32 * kHid * kHid);
     if(rows <= cap) return;
     cap = rows;
     X.alloc(2 * rows * kIn);
@@ -278,25 +303,64 @@ namespace {
 
 template <bool AK, bool BKC, int EPI>
 void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
-          const float* bias, const float* zaux, float* Cf, __bf16* Cb, int ldc) {
+          const float* bias, const float* zaux, float* Cf, __bf16* Cb, int ldc, int splits = 1) {
   MHIP_REQUIRE(M % BM == 0 && N % BN == 0 && K % BK == 0, MHIP_ERR_INVALID_ARG,
                "gemm dims must be multiples of 128/128/32");
-  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BKC, EPI>), dim3(N / BN, M / BM), dim3(256), 0, st, M,
-                     N, K, A, lda, B, ldb, bias, zaux, Cf, Cb, ldc);
+  MHIP_REQUIRE(splits >= 1 && K % (splits * BK) == 0 && (splits == 1 || EPI == EPI_F32),
+               MHIP_ERR_INVALID_ARG, "bad split-K");
+  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BKC, EPI>), dim3(N / BN, M / BM, splits), dim3(256), 0,
+                     st, M, N, K, A, lda, B, ldb, bias, zaux, Cf, Cb, ldc);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+// Weight gradient C (M x N, fp32) = A^T B over K batch rows, split over K
+// into fixed slices summed in order (parallel over the batch).
+void gemm_wgrad(hipStream_t st, Mlp& net, int M, int N, int K, const __bf16* A, int lda,
+                const __bf16* B, int ldb, float* C) {
+  int splits = 1;
+  while(splits < 32 && K % (2 * splits * BK) == 0 && K / (2 * splits) >= 256) splits *= 2;
+  net.wk.reserve(sizeof(float) * (size_t)splits * M * N);
+  gemm<false, false, EPI_F32>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, net.wk.as<float>(),
+                              nullptr, N, splits);
+  const int64_t count = (int64_t)M * N;
+  hipLaunchKernelGGL(slice_sum_kernel, dim3((count + 255) / 256), dim3(256), 0, st,
+                     net.wk.as<float>(), splits, count, count, C);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+// Column sums of a (rows x ncol) bf16 matrix, weighted by wrow if given.
+void colsum(hipStream_t st, Mlp& net, const __bf16* Mt, int64_t rows, int ncol,
+            const float* wrow, float* out) {
+  const int slices = (int)std::max<int64_t>(1, std::min<int64_t>(256, rows / 64));
+  net.wk.reserve(sizeof(float) * (size_t)slices * ncol);
+  hipLaunchKernelGGL(colsum_part_kernel, dim3(slices), dim3(256), 0, st, Mt, rows, ncol, wrow,
+                     net.wk.as<float>());
+  hipLaunchKernelGGL(slice_sum_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st,
+                     net.wk.as<float>(), slices, (int64_t)ncol, (int64_t)ncol, out);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 
 const __bf16* bfp(const DevBuf& b, int64_t off = 0) { return b.as<__bf16>() + off; }
 
-// Forward pass for rows [0, npad) already featurised into net.X.
-void forward(hipStream_t st, Mlp& net, int64_t npad) {
+// Forward pass for rows [0, npad) already featurised into net.X; training
+// keeps the pre-activations Z1, Z2 for the backward pass, inference does not.
+void forward(hipStream_t st, Mlp& net, int64_t npad, bool keep_z = true) {
   const float* P = net.p.as<float>();
-  gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
-                                  bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr,
-                                  net.Z1.as<float>(), net.A1.as<__bf16>(), kHid);
-  gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
-                                  bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, nullptr,
-                                  net.Z2.as<float>(), net.A2.as<__bf16>(), kHid);
+  if(keep_z) {
+    gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
+                                    bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr,
+                                    net.Z1.as<float>(), net.A1.as<__bf16>(), kHid);
+    gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
+                                    bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, nullptr,
+                                    net.Z2.as<float>(), net.A2.as<__bf16>(), kHid);
+  } else {
+    gemm<true, true, EPI_BIAS_GELU_A>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
+                                      bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr, nullptr,
+                                      net.A1.as<__bf16>(), kHid);
+    gemm<true, true, EPI_BIAS_GELU_A>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
+                                      bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, nullptr, nullptr,
+                                      net.A2.as<__bf16>(), kHid);
+  }
 }
 
 template <int W>
@@ -396,7 +460,7 @@ int mhip_mlp_predict(mhip_context* ctx, mhip_mlp* mlp, int words_per_det, const 
         featurize<1>(ctx->stream, net, d_dets + s, nb, npad);
       else
         featurize<2>(ctx->stream, net, d_dets + 2 * s, nb, npad);
-      forward(ctx->stream, net, npad);
+      forward(ctx->stream, net, npad, false);
       const float* P = net.p.as<float>();
       hipLaunchKernelGGL(head_kernel, dim3((nb + 3) / 4), dim3(256), 0, ctx->stream,
                          net.A2.as<__bf16>(), P + Mlp::ow3, P + Mlp::ob3, nb, d_out + s, nullptr,
@@ -434,25 +498,22 @@ int mhip_mlp_train_step(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
                        P + Mlp::ow3, P + Mlp::ob3, n, net.y.as<float>(), d_target,
                        net.dy.as<float>(), 1.f / (float)n, net.loss.as<float>());
     // head gradients
-    hipLaunchKernelGGL(colsum_kernel, dim3(kHid / 128), dim3(128), 0, st, net.A2.as<__bf16>(),
-                       npad, kHid, net.dy.as<float>(), G + Mlp::ow3);
+    colsum(st, net, net.A2.as<__bf16>(), npad, kHid, net.dy.as<float>(), G + Mlp::ow3);
     hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, net.dy.as<float>(), npad, G + Mlp::ob3);
     hipLaunchKernelGGL(head_back_kernel, dim3((npad * kHid + 255) / 256), dim3(256), 0, st,
                        net.dy.as<float>(), P + Mlp::ow3, net.Z2.as<float>(), npad,
                        net.dZ2.as<__bf16>());
     // layer 2: dW2 = dZ2^T A1, db2 = colsum dZ2, dZ1 = (dZ2 W2) * gelu'(Z1)
-    gemm<false, false, EPI_F32>(st, kHid, kHid, (int)npad, bfp(net.dZ2), kHid, bfp(net.A1), kHid,
-                                nullptr, nullptr, G + Mlp::oW2, nullptr, kHid);
-    hipLaunchKernelGGL(colsum_kernel, dim3(kHid / 128), dim3(128), 0, st, net.dZ2.as<__bf16>(),
-                       npad, kHid, nullptr, G + Mlp::ob2);
+    gemm_wgrad(st, net, kHid, kHid, (int)npad, bfp(net.dZ2), kHid, bfp(net.A1), kHid,
+               G + Mlp::oW2);
+    colsum(st, net, net.dZ2.as<__bf16>(), npad, kHid, nullptr, G + Mlp::ob2);
     gemm<true, false, EPI_GELU_GRAD>(st, (int)npad, kHid, kHid, bfp(net.dZ2), kHid,
                                      bfp(net.pb, Mlp::oW2), kHid, nullptr, net.Z1.as<float>(),
                                      nullptr, net.dZ1.as<__bf16>(), kHid);
     // layer 1: dW1 = dZ1^T X, db1 = colsum dZ1
-    gemm<false, false, EPI_F32>(st, kHid, kIn, (int)npad, bfp(net.dZ1), kHid, bfp(net.X), kIn,
-                                nullptr, nullptr, G + Mlp::oW1, nullptr, kIn);
-    hipLaunchKernelGGL(colsum_kernel, dim3(kHid / 128), dim3(128), 0, st, net.dZ1.as<__bf16>(),
-                       npad, kHid, nullptr, G + Mlp::ob1);
+    gemm_wgrad(st, net, kHid, kIn, (int)npad, bfp(net.dZ1), kHid, bfp(net.X), kIn,
+               G + Mlp::oW1);
+    colsum(st, net, net.dZ1.as<__bf16>(), npad, kHid, nullptr, G + Mlp::ob1);
     if(lr > 0.f) {
       net.step++;
       const float b1 = 0.9f, b2 = 0.999f;
