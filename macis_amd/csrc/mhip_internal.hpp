@@ -173,11 +173,20 @@ IntsView view(const Integrals& I);
 // per-level counts, and the inner neighbour lists of every inner string at
 // levels <= 1 and <= 2 (ascending k').
 struct ProductSpace {
-  int64_t L = 0, nruns = 0;
+  int64_t L = 0, nruns = 0, max_seg = 0;  // max_seg: most outer neighbours of a run
   bool outer_alpha = true;
-  DevBuf onb_ptr, onb, ocnt;         // int32 [nruns+1], int32 [..], int32 [3*nruns]
-  DevBuf ptr1, idx1, ptr2, idx2;     // int32 inner lists (levels <= 1, <= 2)
-  DevBuf ostr, istr;                 // uint64 outer / inner strings
+  DevBuf arena, lists;  // owners of the tables below
+  const uint64_t *ostr = nullptr, *istr = nullptr;  // outer / inner strings
+  const int32_t *ocnt = nullptr;                    // [3*nruns] neighbours per level
+  const int32_t *onb_ptr = nullptr, *onb = nullptr; // outer lists, r' << 2 | level
+  const int32_t *ptr1 = nullptr, *idx1 = nullptr;   // inner lists, levels <= 1
+  const int32_t *ptr2 = nullptr, *idx2 = nullptr;   // inner lists, levels <= 2
+  // element tables (filled per build from the integrals): per outer entry
+  // the double element or the single's partial sum and packed excitation;
+  // per inner entry likewise (level <= 2 list) and the packed single
+  // (level <= 1 list)
+  double *oval = nullptr, *ival2 = nullptr;
+  int32_t *oexc = nullptr, *iexc1 = nullptr, *iexc2 = nullptr;
 };
 // Recognises a product-space list (host + device checks); false otherwise.
 bool detect_product_space(mhip_context* ctx, int W, const uint64_t* d, int64_t n,

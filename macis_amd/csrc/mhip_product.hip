@@ -35,6 +35,7 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int64_t kMaxHead = 1 << 16;      // determinants read to find L
+constexpr int kMaxSeg = 512;  // outer neighbours of a run staged in LDS (more: tiled path)
 constexpr double kMaxPairTests = 4.0e8;    // host neighbour-table budget
 
 template <int W>
@@ -91,190 +92,422 @@ __global__ void product_count_kernel(int64_t n, int64_t L, const int32_t* __rest
               (uint32_t)ocnt[3 * r + 2];
 }
 
-// Fill and elements in one pass: one wavefront per row. Lanes take 64 outer
-// neighbours at a time; a wavefront prefix sum of their segment lengths
-// gives every segment's offset. Level-2 neighbours (one column each:
-// the same inner string) are evaluated by their own lane; level-0/1
-// segments are then written by the whole wavefront, one segment at a time.
-// Elements are the Slater-Condon values of the tiled path's element pass
-// (same function, same diagonal), so the CSR is bit-identical.
+// Excitation of a single (o, v, sign) packed in an int32: o | v << 8 |
+// (sign < 0) << 16, with the hole/particle/sign of me_2 (ket & ex, bra & ex).
+__device__ __forceinline__ int32_t pack_single(uint64_t bra, uint64_t ket) {
+  const uint64_t ex = bra ^ ket;
+  const uint32_t o = ctz64(ket & ex), v = ctz64(bra & ex);
+  const double sg = excitation_sign(ket, v, o);
+  return (int32_t)(o | (v << 8) | (sg < 0. ? (1u << 16) : 0u));
+}
+
+// T(v,o) + sum over the bra's same-spin occupied p of G_red(p,v,o): the
+// first two terms of me_2 in its order (matrix_elements.hpp:86-105); the
+// other-spin V_red terms are added later by the fill pass in the same order.
+__device__ __forceinline__ double single_partial(const IntsView& I, uint64_t bra_same,
+                                                 int32_t code) {
+  const int n = I.n;
+  const uint32_t o = code & 0xff, v = (code >> 8) & 0xff;
+  double h = I.T[v + n * o];
+  const double* G_ov = I.G_red + n * (v + n * o);
+  for(uint64_t m = bra_same; m; m &= m - 1) h += G_ov[ctz64(m)];
+  return h;
+}
+
+// Per outer-neighbour entry: level 2 -> the same-spin double element
+// (me_4 of the outer strings); level 1 -> packed single and its partial sum.
+__global__ __launch_bounds__(256) void outer_tables_kernel(IntsView I, int64_t nruns,
+                                                           const uint64_t* __restrict__ ostr,
+                                                           const int32_t* __restrict__ onb_ptr,
+                                                           const int32_t* __restrict__ onb,
+                                                           double* __restrict__ oval,
+                                                           int32_t* __restrict__ oexc) {
+  const int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(r >= nruns) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t b = ostr[r];
+  for(int32_t j = onb_ptr[r] + lane; j < onb_ptr[r + 1]; j += 64) {
+    const int32_t packed = onb[j];
+    const uint64_t kt = ostr[packed >> 2];
+    const int e = packed & 3;
+    double val = 0.;
+    int32_t code = 0;
+    if(e == 2) {
+      val = me_4(I, b, kt, b ^ kt);
+    } else if(e == 1) {
+      code = pack_single(b, kt);
+      val = single_partial(I, b, code);
+    }
+    oval[j] = val;
+    oexc[j] = code;
+  }
+}
+
+// Per inner-list entry of string k: the level-<=2 list gets the same-spin
+// double element (level 2) or the packed single and its partial sum
+// (level 1), with the level in bits 24+; the level-<=1 list gets the packed
+// single (level 1) or -1 (k itself).
+__global__ __launch_bounds__(256) void inner_tables_kernel(
+    IntsView I, int64_t L, const uint64_t* __restrict__ istr, const int32_t* __restrict__ ptr1,
+    const int32_t* __restrict__ idx1, const int32_t* __restrict__ ptr2,
+    const int32_t* __restrict__ idx2, int32_t* __restrict__ iexc1, double* __restrict__ ival2,
+    int32_t* __restrict__ iexc2) {
+  const int64_t k = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(k >= L) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t b = istr[k];
+  for(int32_t t = ptr2[k] + lane; t < ptr2[k + 1]; t += 64) {
+    const uint64_t kt = istr[idx2[t]];
+    const int e = popc64(b ^ kt) >> 1;
+    double val = 0.;
+    int32_t code = 0;
+    if(e == 2) {
+      val = me_4(I, b, kt, b ^ kt);
+    } else if(e == 1) {
+      code = pack_single(b, kt);
+      val = single_partial(I, b, code);
+    }
+    ival2[t] = val;
+    iexc2[t] = code | (e << 24);
+  }
+  for(int32_t t = ptr1[k] + lane; t < ptr1[k + 1]; t += 64) {
+    const uint64_t kt = istr[idx1[t]];
+    iexc1[t] = kt == b ? -1 : pack_single(b, kt);
+  }
+}
+
+__device__ __forceinline__ double code_sign(int32_t code) {
+  return (code >> 16) & 1 ? -1. : 1.;
+}
+
+// Fill and elements in one pass, one wavefront per row. Lanes take
+// consecutive output positions of the row (coalesced column and value
+// stores). The row's segment starts (prefix sums of the segment lengths over
+// its outer neighbours: 1 at level 2, the level-<=1 inner list at level 1,
+// the level-<=2 list at level 0) are staged in LDS; a lane finds its segment
+// by binary search, then its inner string. Elements are the Slater-Condon
+// values of the tiled path's element pass (same device function, same
+// diagonal), so the CSR is bit-identical; |h| > thresh is counted per row
+// for the shared compaction.
+
 __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     IntsView I, int64_t n, int64_t L, int outer_alpha, const uint64_t* __restrict__ ostr,
     const uint64_t* __restrict__ istr, const int32_t* __restrict__ onb_ptr,
     const int32_t* __restrict__ onb, const int32_t* __restrict__ ptr1,
     const int32_t* __restrict__ idx1, const int32_t* __restrict__ ptr2,
-    const int32_t* __restrict__ idx2, const double* __restrict__ diag,
-    const int64_t* __restrict__ rowptr, int32_t* __restrict__ colind, double* __restrict__ nzval,
-    double thresh, int64_t* __restrict__ kept, int* __restrict__ dropped) {
-  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if(row >= n) return;
+    const int32_t* __restrict__ idx2, const double* __restrict__ oval,
+    const int32_t* __restrict__ oexc, const int32_t* __restrict__ iexc1,
+    const double* __restrict__ ival2, const int32_t* __restrict__ iexc2,
+    const double* __restrict__ diag, const int64_t* __restrict__ rowptr,
+    int32_t* __restrict__ colind, double* __restrict__ nzval, double thresh,
+    int64_t* __restrict__ kept, int* __restrict__ dropped) {
+  __shared__ int32_t seg_start[kWavesPerBlock][kMaxSeg + 1];
+  __shared__ int32_t seg_code[kWavesPerBlock][kMaxSeg];
+  const int wv = threadIdx.x >> 6;
+  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+  if(row >= n) return;  // whole wavefront leaves together
   const int lane = threadIdx.x & 63;
   const int64_t r = row / L, k = row % L;
   const uint64_t bo = ostr[r], bi = istr[k];
-  const uint64_t ba = outer_alpha ? bo : bi, bb = outer_alpha ? bi : bo;
   const double d = diag[row];
   const int64_t st = rowptr[row];
   const int32_t s1 = ptr1[k], len1 = ptr1[k + 1] - s1;
   const int32_t s2 = ptr2[k], len2 = ptr2[k + 1] - s2;
-  int64_t pos = st, nk = 0;
-  const int32_t j0 = onb_ptr[r], j1 = onb_ptr[r + 1];
-  for(int32_t jb = j0; jb < j1; jb += 64) {
-    const int32_t j = jb + lane;
-    const bool valid = j < j1;
-    const int32_t packed = valid ? onb[j] : 0;
-    const int64_t r2 = packed >> 2;
+  const int32_t j0 = onb_ptr[r], nseg = onb_ptr[r + 1] - j0;
+  // segment starts (exclusive prefix of lengths) and codes into LDS
+  int32_t carry = 0;
+  for(int32_t b = 0; b < nseg; b += 64) {
+    const int32_t j = b + lane;
+    const bool valid = j < nseg;
+    const int32_t packed = valid ? onb[j0 + j] : 0;
     const int e = packed & 3;
     const int32_t len = !valid ? 0 : (e == 2 ? 1 : (e == 1 ? len1 : len2));
-    // inclusive wavefront prefix sum of the segment lengths
-    int64_t incl = len;
+    int32_t incl = len;
 #pragma unroll
     for(int off = 1; off < 64; off <<= 1) {
-      const int64_t v = __shfl_up(incl, off, 64);
+      const int32_t v = __shfl_up(incl, off, 64);
       if(lane >= off) incl += v;
     }
-    const int64_t my = pos + incl - len;
-    const int64_t total = __shfl(incl, 63, 64);
-    if(valid && e == 2) {  // same inner string, outer double excitation
-      const uint64_t ko = ostr[r2];
-      const uint64_t ka = outer_alpha ? ko : bi, kb = outer_alpha ? bi : ko;
-      const double h = matrix_element(I, ba, bb, ka, kb, d);
-      colind[my] = (int32_t)(r2 * L + k);
-      nzval[my] = h;
-      nk += fabs(h) > thresh ? 1 : 0;
+    if(valid) {
+      seg_start[wv][j] = carry + incl - len;
+      seg_code[wv][j] = packed;
     }
-    uint64_t seg = ballot(valid && e < 2);
-    while(seg) {
-      const int src = (int)ctz64(seg);
-      seg &= seg - 1;
-      const int64_t sp = __shfl(my, src, 64);
-      const int64_t sr = __shfl(r2, src, 64);
-      const int se = __shfl(e, src, 64);
-      const uint64_t ko = ostr[sr];
-      const int32_t sl = se == 1 ? len1 : len2;
-      const int32_t* lst = se == 1 ? idx1 + s1 : idx2 + s2;
-      for(int32_t t = lane; t < sl; t += 64) {
-        const int32_t k2 = lst[t];
-        const uint64_t ki = istr[k2];
-        const uint64_t ka = outer_alpha ? ko : ki, kb = outer_alpha ? ki : ko;
-        const double h = matrix_element(I, ba, bb, ka, kb, d);
-        colind[sp + t] = (int32_t)(sr * L + k2);
-        nzval[sp + t] = h;
-        nk += fabs(h) > thresh ? 1 : 0;
+    carry += __shfl(incl, 63, 64);
+  }
+  if(lane == 0) seg_start[wv][nseg] = carry;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const int32_t total = carry;
+  int64_t nk = 0;
+  // Windows of 64 positions: the segments starting inside the window are
+  // the next ones in order (jlo, jlo + 1, ...); their start offsets form a
+  // 64-bit mask, and a lane's segment is jlo - 1 + the starts at or below it.
+  int32_t jlo = 0;
+  // kU windows per iteration: their element chains (list, string, integral
+  // loads) are independent and overlap
+  constexpr int kU = 1;
+  for(int32_t pb = 0; pb < total; pb += 64 * kU) {
+    int32_t lo[kU];
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const int32_t p0 = pb + 64 * u;
+      const int32_t js = jlo + lane;
+      const int32_t sstart = js < nseg ? seg_start[wv][js] : INT32_MAX;
+      const bool inwin = sstart < p0 + 64;
+      uint64_t startmask = inwin ? (1ull << (sstart - p0)) : 0ull;
+#pragma unroll
+      for(int off = 1; off < 64; off <<= 1) startmask |= __shfl_xor(startmask, off, 64);
+      lo[u] = jlo - 1 + (int32_t)popc64(startmask & lowmask((uint32_t)lane + 1u));
+      jlo += (int32_t)popc64(ballot(inwin));
+    }
+    int32_t k2[kU];
+    int64_t r2[kU];
+    double h[kU];
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const int32_t p = pb + 64 * u + lane;
+      const int32_t l = p < total ? lo[u] : 0;
+      const int32_t packed = seg_code[wv][l];
+      const int32_t off = p - seg_start[wv][l];
+      const int32_t j = j0 + l;  // outer-neighbour entry of the segment
+      r2[u] = packed >> 2;
+      const int e = packed & 3;
+      double hv = d;
+      int32_t kk = (int32_t)k;
+      if(p < total) {
+        if(e == 2) {  // outer double, same inner string: me_4 of the outer pair
+          hv = oval[j];
+        } else if(e == 1) {
+          kk = idx1[s1 + off];
+          const int32_t oc = oexc[j];
+          const int32_t ic = iexc1[s1 + off];
+          if(ic < 0) {  // outer single, same inner string: me_2, other spin = inner
+            const uint32_t o = oc & 0xff, v = (oc >> 8) & 0xff;
+            const double* V_ov = I.V_red + I.n * (v + I.n * o);
+            double t = oval[j];
+            for(uint64_t m = bi; m; m &= m - 1) t += V_ov[ctz64(m)];
+            hv = code_sign(oc) * t;
+          } else {  // outer single x inner single: me_22 (alpha excitation first)
+            const int32_t ac = outer_alpha ? oc : ic, bc = outer_alpha ? ic : oc;
+            const uint32_t o1 = ac & 0xff, v1 = (ac >> 8) & 0xff;
+            const uint32_t o2 = bc & 0xff, v2 = (bc >> 8) & 0xff;
+            const double sign = code_sign(ac) * code_sign(bc);
+            hv = sign * I.V[v1 + I.n * (o1 + I.n * (v2 + I.n * o2))];
+          }
+        } else {
+          const int32_t t2 = s2 + off;
+          kk = idx2[t2];
+          const int32_t ic = iexc2[t2];
+          const int lvl = ic >> 24;
+          if(lvl == 2) {  // inner double: me_4 of the inner pair
+            hv = ival2[t2];
+          } else if(lvl == 1) {  // inner single: me_2, other spin = outer
+            const uint32_t o = ic & 0xff, v = (ic >> 8) & 0xff;
+            const double* V_ov = I.V_red + I.n * (v + I.n * o);
+            double t = ival2[t2];
+            for(uint64_t m = bo; m; m &= m - 1) t += V_ov[ctz64(m)];
+            hv = code_sign(ic) * t;
+          }
+        }
+      }
+      k2[u] = kk;
+      h[u] = hv;
+    }
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const int32_t p = pb + 64 * u + lane;
+      if(p < total) {
+        colind[st + p] = (int32_t)(r2[u] * L + k2[u]);
+        nzval[st + p] = h[u];
+        nk += fabs(h[u]) > thresh ? 1 : 0;
       }
     }
-    pos += total;
   }
   nk = wave_sum(nk);
   if(lane == 0) {
     kept[row] = nk;
-    if(nk != pos - st) atomicOr(dropped, 1);
+    if(nk != total) atomicOr(dropped, 1);
+  }
+}
+
+// Neighbour counts of string i per excitation level 0/1/2 (wavefront per i).
+__global__ __launch_bounds__(256) void nb_count_kernel(const uint64_t* __restrict__ s, int64_t m,
+                                                       int32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(i >= m) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t si = s[i];
+  int32_t c0 = 0, c1 = 0, c2 = 0;
+  for(int64_t j0 = 0; j0 < m; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const int e = j < m ? popc64(si ^ s[j]) : 99;
+    c0 += popc64(ballot(e == 0));
+    c1 += popc64(ballot(e == 2));
+    c2 += popc64(ballot(e == 4));
+  }
+  if(lane == 0) {
+    cnt[3 * i] = c0;
+    cnt[3 * i + 1] = c1;
+    cnt[3 * i + 2] = c2;
+  }
+}
+
+// Ascending neighbour list of string i at levels <= maxlvl, written at
+// ptr[i]; packed = j << 2 | level, or j.
+__global__ __launch_bounds__(256) void nb_fill_kernel(const uint64_t* __restrict__ s, int64_t m,
+                                                      int maxlvl, int packed,
+                                                      const int32_t* __restrict__ ptr,
+                                                      int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(i >= m) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t si = s[i];
+  int64_t pos = ptr[i];
+  for(int64_t j0 = 0; j0 < m; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const int e = j < m ? popc64(si ^ s[j]) : 99;
+    const bool hit = e <= 2 * maxlvl;
+    const uint64_t mask = ballot(hit);
+    if(hit) out[pos + lanes_below(mask)] = packed ? (int32_t)((j << 2) | (e >> 1)) : (int32_t)j;
+    pos += popc64(mask);
   }
 }
 
 int popc(uint64_t x) { return __builtin_popcountll(x); }
 
-// Neighbour lists of `s` within `t` at levels <= maxlvl (ascending index).
-void neighbour_lists(const std::vector<uint64_t>& s, int maxlvl, std::vector<int32_t>& ptr,
-                     std::vector<int32_t>& idx, std::vector<int32_t>* lvl) {
-  const size_t n = s.size();
-  ptr.assign(n + 1, 0);
-  idx.clear();
-  if(lvl) lvl->clear();
-  for(size_t i = 0; i < n; ++i) {
-    for(size_t j = 0; j < n; ++j) {
-      const int e = popc(s[i] ^ s[j]);  // 2 * excitation level (equal electron counts)
-      if(e <= 2 * maxlvl) {
-        idx.push_back((int32_t)j);
-        if(lvl) lvl->push_back(e / 2);
-      }
-    }
-    ptr[i + 1] = (int32_t)idx.size();
-  }
-}
-
-template <typename T>
-void upload(mhip_context* ctx, DevBuf& b, const std::vector<T>& h) {
-  b.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
-  if(!h.empty())
-    MHIP_HIP_CHECK(hipMemcpyAsync(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice,
-                                  ctx->stream));
+bool distinct_equal_weight(std::vector<uint64_t> v) {
+  for(uint64_t x : v)
+    if(popc(x) != popc(v[0])) return false;
+  std::sort(v.begin(), v.end());
+  return std::adjacent_find(v.begin(), v.end()) == v.end();
 }
 
 template <int W>
 bool detect(mhip_context* ctx, const uint64_t* d, int64_t n, ProductSpace& ps) {
   if(n < 2) return false;
   hipStream_t st = ctx->stream;
-  const int64_t nh = std::min(n, kMaxHead);
-  std::vector<uint64_t> head(nh * W);
-  MHIP_HIP_CHECK(hipMemcpyAsync(head.data(), d, 8 * W * nh, hipMemcpyDeviceToHost, st));
-  MHIP_HIP_CHECK(hipStreamSynchronize(st));
-  auto alpha = [&](int64_t i) { return W == 1 ? head[i] & 0xffffffffull : head[2 * i]; };
-  auto beta = [&](int64_t i) { return W == 1 ? head[i] >> 32 : head[2 * i + 1]; };
-  bool outer_alpha;
-  if(alpha(0) == alpha(1)) outer_alpha = true;
-  else if(beta(0) == beta(1)) outer_alpha = false;
-  else return false;
-  auto outer = [&](int64_t i) { return outer_alpha ? alpha(i) : beta(i); };
-  auto inner = [&](int64_t i) { return outer_alpha ? beta(i) : alpha(i); };
-  int64_t L = 1;
-  while(L < nh && outer(L) == outer(0)) ++L;
-  if(L == nh && nh < n) return false;  // first run longer than the head read
+  // head of the list: outer spin and first run length
+  std::vector<uint64_t> head;
+  bool outer_alpha = true;
+  int64_t L = 0, nh = 0;
+  for(int64_t want = std::min<int64_t>(n, 4096);; want = std::min<int64_t>(n, kMaxHead)) {
+    nh = want;
+    head.resize(nh * W);
+    MHIP_HIP_CHECK(hipMemcpyAsync(head.data(), d, 8 * W * nh, hipMemcpyDeviceToHost, st));
+    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    auto alpha = [&](int64_t i) { return W == 1 ? head[i] & 0xffffffffull : head[2 * i]; };
+    auto beta = [&](int64_t i) { return W == 1 ? head[i] >> 32 : head[2 * i + 1]; };
+    if(alpha(0) == alpha(1)) outer_alpha = true;
+    else if(beta(0) == beta(1)) outer_alpha = false;
+    else return false;
+    auto outer = [&](int64_t i) { return outer_alpha ? alpha(i) : beta(i); };
+    L = 1;
+    while(L < nh && outer(L) == outer(0)) ++L;
+    if(L < nh || nh == n) break;           // run end found (or the list is one run)
+    if(nh == std::min<int64_t>(n, kMaxHead)) return false;  // first run too long
+  }
   if(n % L) return false;
   const int64_t nruns = n / L;
   if((double)nruns * nruns + (double)L * L > kMaxPairTests) return false;
-  // inner strings: distinct, equal electron count
   std::vector<uint64_t> I(L);
-  for(int64_t k = 0; k < L; ++k) I[k] = inner(k);
-  {
-    std::vector<uint64_t> s(I);
-    std::sort(s.begin(), s.end());
-    if(std::adjacent_find(s.begin(), s.end()) != s.end()) return false;
-    for(uint64_t v : I)
-      if(popc(v) != popc(I[0])) return false;
-  }
-  // device check of the whole list, outer strings of every run
-  DevBuf flag(sizeof(int)), heads(8 * nruns);
-  MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+  for(int64_t k = 0; k < L; ++k)
+    I[k] = outer_alpha ? (W == 1 ? head[k] >> 32 : head[2 * k + 1])
+                       : (W == 1 ? head[k] & 0xffffffffull : head[2 * k]);
+  if(!distinct_equal_weight(I)) return false;
+
+  // arena 1: strings, per-level counts, row pointers, flag
+  const size_t o_ostr = 0, o_istr = o_ostr + 8 * nruns, o_ocnt = o_istr + 8 * L,
+               o_icnt = o_ocnt + 4 * 3 * nruns, o_optr = o_icnt + 4 * 3 * L,
+               o_p1 = o_optr + 4 * (nruns + 1), o_p2 = o_p1 + 4 * (L + 1),
+               o_flag = o_p2 + 4 * (L + 1), a1 = o_flag + 16;
+  ps.arena.alloc(a1);
+  char* A = ps.arena.as<char>();
+  uint64_t* ostr = reinterpret_cast<uint64_t*>(A + o_ostr);
+  uint64_t* istr = reinterpret_cast<uint64_t*>(A + o_istr);
+  int32_t* ocnt = reinterpret_cast<int32_t*>(A + o_ocnt);
+  int32_t* icnt = reinterpret_cast<int32_t*>(A + o_icnt);
+  int* flag = reinterpret_cast<int*>(A + o_flag);
+  MHIP_HIP_CHECK(hipMemcpyAsync(istr, I.data(), 8 * L, hipMemcpyHostToDevice, st));
+  MHIP_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), st));
   hipLaunchKernelGGL(product_check_kernel<W>, dim3((n + 255) / 256), dim3(256), 0, st, d, n, L,
-                     outer_alpha, flag.as<int>());
+                     outer_alpha, flag);
   hipLaunchKernelGGL(run_heads_kernel<W>, dim3((nruns + 255) / 256), dim3(256), 0, st, d, nruns,
-                     L, outer_alpha, heads.as<uint64_t>());
+                     L, outer_alpha, ostr);
+  hipLaunchKernelGGL(nb_count_kernel, dim3((nruns + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, st, ostr, nruns, ocnt);
+  hipLaunchKernelGGL(nb_count_kernel, dim3((L + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256),
+                     0, st, istr, L, icnt);
   MHIP_HIP_CHECK(hipGetLastError());
-  int bad = 0;
-  std::vector<uint64_t> O(nruns);
-  MHIP_HIP_CHECK(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  MHIP_HIP_CHECK(hipMemcpyAsync(O.data(), heads.p, 8 * nruns, hipMemcpyDeviceToHost, st));
+  // ostr, istr, counts and the flag are contiguous: one copy back
+  std::vector<char> h(o_flag + sizeof(int) - o_ostr);
+  MHIP_HIP_CHECK(hipMemcpyAsync(h.data(), A, h.size(), hipMemcpyDeviceToHost, st));
   MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  int bad = 0;
+  std::memcpy(&bad, h.data() + o_flag, sizeof(int));
   if(bad) return false;
-  {
-    std::vector<uint64_t> s(O);
-    std::sort(s.begin(), s.end());
-    if(std::adjacent_find(s.begin(), s.end()) != s.end()) return false;
-    for(uint64_t v : O)
-      if(popc(v) != popc(O[0])) return false;
+  std::vector<uint64_t> O(nruns);
+  std::memcpy(O.data(), h.data() + o_ostr, 8 * nruns);
+  if(!distinct_equal_weight(O)) return false;
+  const int32_t* hoc = reinterpret_cast<const int32_t*>(h.data() + o_ocnt);
+  const int32_t* hic = reinterpret_cast<const int32_t*>(h.data() + o_icnt);
+  std::vector<int32_t> ptrs(nruns + 1 + 2 * (L + 1));
+  int32_t* optr = ptrs.data();
+  int32_t* p1 = optr + nruns + 1;
+  int32_t* p2 = p1 + L + 1;
+  optr[0] = p1[0] = p2[0] = 0;
+  int64_t max_seg = 0;
+  for(int64_t r = 0; r < nruns; ++r) {
+    const int64_t c = (int64_t)hoc[3 * r] + hoc[3 * r + 1] + hoc[3 * r + 2];
+    MHIP_REQUIRE(optr[r] + c < (1ll << 31), MHIP_ERR_INDEX_OVERFLOW, "neighbour table too large");
+    optr[r + 1] = optr[r] + (int32_t)c;
+    max_seg = std::max(max_seg, c);
   }
-  // neighbour tables
-  std::vector<int32_t> optr, oidx, olvl, p1, i1, p2, i2;
-  neighbour_lists(O, 2, optr, oidx, &olvl);
-  neighbour_lists(I, 1, p1, i1, nullptr);
-  neighbour_lists(I, 2, p2, i2, nullptr);
-  std::vector<int32_t> onb(oidx.size()), ocnt(3 * nruns, 0);
-  for(size_t j = 0; j < oidx.size(); ++j) onb[j] = (oidx[j] << 2) | olvl[j];
-  for(int64_t r = 0; r < nruns; ++r)
-    for(int32_t j = optr[r]; j < optr[r + 1]; ++j) ocnt[3 * r + olvl[j]]++;
+  if(max_seg > kMaxSeg) return false;  // the fill stages a run's neighbours in LDS
+  for(int64_t k = 0; k < L; ++k) {
+    p1[k + 1] = p1[k] + hic[3 * k] + hic[3 * k + 1];
+    p2[k + 1] = p2[k] + hic[3 * k] + hic[3 * k + 1] + hic[3 * k + 2];
+  }
+  MHIP_HIP_CHECK(hipMemcpyAsync(A + o_optr, ptrs.data(), 4 * ptrs.size(), hipMemcpyHostToDevice, st));
+  // arena 2: the neighbour lists
+  const size_t n_onb = optr[nruns], n1 = p1[L], n2 = p2[L];
+  // lists (int32) then the element tables: oval, ival2 (double), oexc,
+  // iexc1, iexc2 (int32)
+  ps.lists.alloc(8 * (n_onb + n2) + 4 * (2 * n_onb + 2 * n1 + 2 * n2) + 16);
+  double* oval = ps.lists.as<double>();
+  double* ival2 = oval + n_onb;
+  int32_t* onb = reinterpret_cast<int32_t*>(ival2 + n2);
+  int32_t* idx1 = onb + n_onb;
+  int32_t* idx2 = idx1 + n1;
+  int32_t* oexc = idx2 + n2;
+  int32_t* iexc1 = oexc + n_onb;
+  int32_t* iexc2 = iexc1 + n1;
+  hipLaunchKernelGGL(nb_fill_kernel, dim3((nruns + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, st, ostr, nruns, 2, 1,
+                     reinterpret_cast<const int32_t*>(A + o_optr), onb);
+  hipLaunchKernelGGL(nb_fill_kernel, dim3((L + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256),
+                     0, st, istr, L, 1, 0, reinterpret_cast<const int32_t*>(A + o_p1), idx1);
+  hipLaunchKernelGGL(nb_fill_kernel, dim3((L + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256),
+                     0, st, istr, L, 2, 0, reinterpret_cast<const int32_t*>(A + o_p2), idx2);
+  MHIP_HIP_CHECK(hipGetLastError());
   ps.L = L;
   ps.nruns = nruns;
+  ps.max_seg = max_seg;
   ps.outer_alpha = outer_alpha;
-  upload(ctx, ps.onb_ptr, optr);
-  upload(ctx, ps.onb, onb);
-  upload(ctx, ps.ocnt, ocnt);
-  upload(ctx, ps.ptr1, p1);
-  upload(ctx, ps.idx1, i1);
-  upload(ctx, ps.ptr2, p2);
-  upload(ctx, ps.idx2, i2);
-  upload(ctx, ps.ostr, O);
-  upload(ctx, ps.istr, I);
-  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  ps.ostr = ostr;
+  ps.istr = istr;
+  ps.ocnt = ocnt;
+  ps.onb_ptr = reinterpret_cast<const int32_t*>(A + o_optr);
+  ps.ptr1 = reinterpret_cast<const int32_t*>(A + o_p1);
+  ps.ptr2 = reinterpret_cast<const int32_t*>(A + o_p2);
+  ps.onb = onb;
+  ps.idx1 = idx1;
+  ps.idx2 = idx2;
+  ps.oval = oval;
+  ps.ival2 = ival2;
+  ps.oexc = oexc;
+  ps.iexc1 = iexc1;
+  ps.iexc2 = iexc2;
   return true;
 }
 
@@ -287,7 +520,7 @@ bool detect_product_space(mhip_context* ctx, int W, const uint64_t* d, int64_t n
 
 void product_count(mhip_context* ctx, const ProductSpace& ps, int64_t n, uint32_t* counts) {
   hipLaunchKernelGGL(product_count_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                     ps.L, ps.ocnt.as<int32_t>(), ps.ptr1.as<int32_t>(), ps.ptr2.as<int32_t>(),
+                     ps.L, ps.ocnt, ps.ptr1, ps.ptr2,
                      counts);
   MHIP_HIP_CHECK(hipGetLastError());
 }
@@ -296,12 +529,19 @@ void product_fill_elements(mhip_context* ctx, int W, const ProductSpace& ps, con
                            int64_t n, const double* diag, const int64_t* rowptr, int32_t* colind,
                            double* nzval, double thresh, int64_t* kept, int* dropped) {
   (void)W;  // the strings are stored split; the kernel does not read determinants
+  MHIP_REQUIRE(ps.max_seg <= kMaxSeg, MHIP_ERR_INVALID_ARG, "product space outside LDS staging");
+  // element tables of the neighbour lists (they depend on the integrals)
+  hipLaunchKernelGGL(outer_tables_kernel, dim3((ps.nruns + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, ctx->stream, I, ps.nruns, ps.ostr, ps.onb_ptr, ps.onb, ps.oval,
+                     ps.oexc);
+  hipLaunchKernelGGL(inner_tables_kernel, dim3((ps.L + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, ctx->stream, I, ps.L, ps.istr, ps.ptr1, ps.idx1, ps.ptr2,
+                     ps.idx2, ps.iexc1, ps.ival2, ps.iexc2);
   hipLaunchKernelGGL(product_fill_elements_kernel,
                      dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, ctx->stream, I,
-                     n, ps.L, ps.outer_alpha ? 1 : 0, ps.ostr.as<uint64_t>(),
-                     ps.istr.as<uint64_t>(), ps.onb_ptr.as<int32_t>(), ps.onb.as<int32_t>(),
-                     ps.ptr1.as<int32_t>(), ps.idx1.as<int32_t>(), ps.ptr2.as<int32_t>(),
-                     ps.idx2.as<int32_t>(), diag, rowptr, colind, nzval, thresh, kept, dropped);
+                     n, ps.L, ps.outer_alpha ? 1 : 0, ps.ostr, ps.istr, ps.onb_ptr, ps.onb,
+                     ps.ptr1, ps.idx1, ps.ptr2, ps.idx2, ps.oval, ps.oexc, ps.iexc1, ps.ival2,
+                     ps.iexc2, diag, rowptr, colind, nzval, thresh, kept, dropped);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 

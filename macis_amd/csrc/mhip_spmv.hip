@@ -51,15 +51,6 @@ __global__ __launch_bounds__(256) void spmv_kernel(
   if(row < m) {
     const int64_t st = rowptr[row] - indexing, en = rowptr[row + 1] - indexing;
     const int64_t qa = (st + 3) >> 2, qb = en >> 2;  // whole quads [qa, qb)
-    // head [st, min(en, 4 qa)) and tail [max(4 qb, 4 qa), en): at most 3 each
-    {
-      const int64_t he = en < 4 * qa ? en : 4 * qa;
-      const int64_t ts = 4 * qb > 4 * qa ? 4 * qb : 4 * qa;
-      int64_t j = -1;
-      if(l < 3) j = st + l < he ? st + l : -1;
-      else if(l < 6) j = ts + (l - 3) < en ? ts + (l - 3) : -1;
-      if(j >= 0) acc0 = nzval[j] * xb[colind[j]];
-    }
     const i32x4* cq = reinterpret_cast<const i32x4*>(colind);
     const f64x2* vq = reinterpret_cast<const f64x2*>(nzval);
     int64_t q = qa + l;
@@ -88,6 +79,14 @@ __global__ __launch_bounds__(256) void spmv_kernel(
       acc2 = fma(v1.x, xb[c0.z], acc2);
       acc3 = fma(v1.y, xb[c0.w], acc3);
     }
+    // head [st, min(en, 4 qa)) and tail [max(4 qb, 4 qa), en): at most 3 each,
+    // after the quads so the streamed loads are issued first
+    const int64_t he = en < 4 * qa ? en : 4 * qa;
+    const int64_t ts = 4 * qb > 4 * qa ? 4 * qb : 4 * qa;
+    int64_t j = -1;
+    if(l < 3) j = st + l < he ? st + l : -1;
+    else if(l < 6) j = ts + (l - 3) < en ? ts + (l - 3) : -1;
+    if(j >= 0) acc0 = fma(nzval[j], xb[colind[j]], acc0);
   }
   double acc = (acc0 + acc1) + (acc2 + acc3);
 #pragma unroll
