@@ -73,7 +73,7 @@ typedef struct mhip_timings {
   double ham_fill_ms;     /* structural fill pass (ascending columns) */
   double ham_elem_ms;     /* Slater-Condon elements + threshold marks */
   double ham_compact_ms;  /* threshold compaction (0 when nothing dropped) */
-  double ham_total_ms;
+  double ham_total_ms;    /* host wall time of the whole build call */
   int64_t ham_nnz_struct; /* pairs with popcount(bra^ket) <= 4 */
   int64_t ham_nnz;        /* kept after |h| > thresh */
   double dav_spmv_ms;     /* summed SpMV time inside the last Davidson */
@@ -130,6 +130,10 @@ int mhip_get_timings(mhip_context* ctx, mhip_timings* out);
  * (default), MHIP_BUILD_TILED, or MHIP_BUILD_PRODUCT (fails with
  * MHIP_ERR_INVALID_ARG when a square build's list is not a product space). */
 int mhip_context_set_build_path(mhip_context* ctx, int path);
+/* Frees device storage the context keeps for reuse (the last internal
+ * Hamiltonian of mhip_selected_ci_diag and the Davidson subspace vectors);
+ * no counterpart in the reference. */
+int mhip_context_release_cached(mhip_context* ctx);
 
 /* Device memory helpers, so callers need no HIP runtime of their own. */
 int mhip_malloc(mhip_context* ctx, size_t bytes, void** dptr);
@@ -223,7 +227,9 @@ int mhip_gemm_tn(mhip_context* ctx, int64_t N, int k, const double* d_A,
  * DEVICE d_dets with threshold h_el_tol, extract the diagonal, guess rule
  * (reuse d_C if max|C| > 1/ndets, else unit vector at argmin D), Davidson.
  * d_C (DEVICE, length ndets) is guess in / eigenvector out. If H_out is not
- * NULL the Hamiltonian handle is returned to the caller. */
+ * NULL the Hamiltonian handle is returned to the caller; otherwise its
+ * device storage is kept by the context and reused by the next build of the
+ * same or smaller size (mhip_context_release_cached frees it). */
 int mhip_selected_ci_diag(mhip_context* ctx, int words_per_det,
                           const uint64_t* d_dets, int64_t ndets,
                           double h_el_tol, int64_t max_m, int64_t max_restarts,

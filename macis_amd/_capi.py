@@ -90,6 +90,7 @@ _SIGS = {
     "mhip_synchronize": (_i32, [_p]),
     "mhip_get_timings": (_i32, [_p, C.POINTER(Timings)]),
     "mhip_context_set_build_path": (_i32, [_p, _i32]),
+    "mhip_context_release_cached": (_i32, [_p]),
     "mhip_malloc": (_i32, [_p, C.c_size_t, _pp]),
     "mhip_free": (_i32, [_p, _p]),
     "mhip_memcpy_h2d": (_i32, [_p, _p, _p, C.c_size_t]),
@@ -295,6 +296,9 @@ class Context:
 
     def set_build_path(self, path):
         check(lib().mhip_context_set_build_path(self.h, path), "set_build_path")
+
+    def release_cached(self):
+        check(lib().mhip_context_release_cached(self.h), "release_cached")
 
     # ---- hot path
     def set_integrals(self, norb, T, V):

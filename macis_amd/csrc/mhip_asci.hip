@@ -399,8 +399,7 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
 
   // candidate counts per core det (closed form of the loop bounds)
   std::vector<uint64_t> hc(nc * W);
-  MHIP_HIP_CHECK(hipMemcpyAsync(hc.data(), d_cdets, 8 * W * nc, hipMemcpyDeviceToHost, st));
-  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  d2h(ctx, hc.data(), d_cdets, 8 * W * nc);
   std::vector<int64_t> ncand(nc), cstart(nc + 1, 0);
   for(int64_t c = 0; c < nc; ++c) {
     uint64_t a, b;
@@ -461,8 +460,7 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
                                          coff.as<int64_t>(), (int64_t)0, (size_t)(nchunks + 1),
                                          rocprim::plus<int64_t>(), st));
   int64_t P = 0;
-  MHIP_HIP_CHECK(hipMemcpyAsync(&P, coff.as<int64_t>() + nchunks, 8, hipMemcpyDeviceToHost, st));
-  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  d2h(ctx, &P, coff.as<int64_t>() + nchunks, 8);
   DevBuf keys(sizeof(K) * std::max<int64_t>(P, 1)), vals(8 * std::max<int64_t>(P, 1));
   hipLaunchKernelGGL((contrib_kernel<W, true>), dim3(grid), dim3(256), 0, st, I, A,
                      dn.as<int64_t>(), nchunks, E0, s->h_el_tol, nullptr, coff.as<int64_t>(),
@@ -498,7 +496,7 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
                                           vals2.as<double>(), (size_t)P, ukeys.as<K>(),
                                           usum.as<double>(), nu_dev.as<int64_t>(),
                                           rocprim::plus<double>(), rocprim::equal_to<K>(), st));
-    MHIP_HIP_CHECK(hipMemcpyAsync(&nu, nu_dev.p, 8, hipMemcpyDeviceToHost, st));
+    d2h(ctx, &nu, nu_dev.p, 8);
   }
   t.stop();
   tm.asci_sort_ms = t.ms();
@@ -534,8 +532,7 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, absbits.as<uint64_t>(),
                                    flags.as<uint8_t>(), kabs.as<uint64_t>(),
                                    nu_dev.as<int64_t>() + 1, (size_t)nu, st));
-    MHIP_HIP_CHECK(hipMemcpyAsync(&nkeep, nu_dev.as<int64_t>() + 1, 8, hipMemcpyDeviceToHost, st));
-    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    d2h(ctx, &nkeep, nu_dev.as<int64_t>() + 1, 8);
   }
   tm.asci_nn_ms = 0.;
   tm.asci_nsurvivors = nkeep;

@@ -688,8 +688,7 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
         } else {
           biteration(ctx, B, i, tol);
         }
-        MHIP_HIP_CHECK(hipMemcpyAsync(&nleft, B->nleft.p, 4, hipMemcpyDeviceToHost, st));
-        MHIP_HIP_CHECK(hipStreamSynchronize(st));
+        d2h(ctx, &nleft, B->nleft.p, 4);
       }
       if(!nleft || rst >= max_restarts) break;
       // restart unconverged instances: V0 = X/|X|, V1 = V(:,max_m), AV0 = A V0
@@ -708,9 +707,8 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
     const double ms = tall.ms();
     if(ms_out) *ms_out = ms;
     std::vector<double> hE(B->ninst), hr(B->ninst);
-    MHIP_HIP_CHECK(hipMemcpyAsync(hE.data(), B->E.p, 8 * B->ninst, hipMemcpyDeviceToHost, st));
-    MHIP_HIP_CHECK(hipMemcpyAsync(hr.data(), B->res.p, 8 * B->ninst, hipMemcpyDeviceToHost, st));
-    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    d2h(ctx, hE.data(), B->E.p, 8 * B->ninst);
+    d2h(ctx, hr.data(), B->res.p, 8 * B->ninst);
     for(int p = 0; p < B->ninst; ++p) {
       E[B->order[p]] = hE[p];
       if(residual) residual[B->order[p]] = hr[p];

@@ -28,6 +28,67 @@ void use_device(mhip_context* ctx) {
   MHIP_HIP_CHECK(hipSetDevice(ctx->device));
 }
 
+namespace {
+// Copies bytes (multiple of 4 plus a tail) into host-pinned memory with a
+// kernel: small reads then need neither a copy engine nor a blit launch
+// from the runtime's memcpy path.
+__global__ void stage_copy_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                  size_t words) {
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+      i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+void d2h(mhip_context* ctx, void* dst, const void* src, size_t bytes) {
+  if(!bytes) return;
+  const bool aligned = ((uintptr_t)src % 4) == 0 && bytes % 4 == 0;
+  if(bytes <= (size_t(4) << 20) && aligned) {
+    ctx->pinned.reserve(std::max<size_t>(bytes, 64 << 10));
+    const size_t words = bytes / 4;
+    const unsigned blocks = (unsigned)std::min<size_t>(1024, (words + 255) / 256);
+    hipLaunchKernelGGL(stage_copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                       static_cast<uint32_t*>(ctx->pinned.p), static_cast<const uint32_t*>(src),
+                       words);
+    MHIP_HIP_CHECK(hipGetLastError());
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(dst, ctx->pinned.p, bytes);
+  } else {
+    MHIP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+}
+
+DevBuf take_buffer(mhip_context* ctx, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 1);
+  int best = -1;
+  for(int i = 0; i < (int)ctx->spare.size(); ++i) {
+    const size_t c = ctx->spare[i].bytes;
+    if(c >= bytes && c <= 2 * bytes && (best < 0 || c < ctx->spare[best].bytes)) best = i;
+  }
+  if(best >= 0) {
+    DevBuf b = std::move(ctx->spare[best]);
+    ctx->spare.erase(ctx->spare.begin() + best);
+    return b;
+  }
+  DevBuf b;
+  try {
+    b.alloc(bytes);
+  } catch(const Error& e) {
+    if(e.code != MHIP_ERR_OUT_OF_MEMORY || ctx->spare.empty()) throw;
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->spare.clear();
+    b.alloc(bytes);
+  }
+  return b;
+}
+
+void keep_buffer(mhip_context* ctx, DevBuf&& b) {
+  if(!b.p) return;
+  ctx->spare.push_back(std::move(b));
+  if(ctx->spare.size() > 8) ctx->spare.erase(ctx->spare.begin());
+}
+
 }  // namespace mhip
 
 using namespace mhip;
@@ -105,6 +166,14 @@ int mhip_context_set_build_path(mhip_context* ctx, int path) {
   });
 }
 
+int mhip_context_release_cached(mhip_context* ctx) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->spare.clear();
+  });
+}
+
 int mhip_synchronize(mhip_context* ctx) {
   return guarded([&] {
     use_device(ctx);
@@ -155,8 +224,7 @@ int mhip_memcpy_d2h(mhip_context* ctx, void* dst, const void* src,
     use_device(ctx);
     if(!bytes) return;
     MHIP_REQUIRE(dst && src, MHIP_ERR_INVALID_ARG, "null pointer");
-    MHIP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    d2h(ctx, dst, src, bytes);
   });
 }
 

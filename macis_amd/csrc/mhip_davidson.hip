@@ -324,9 +324,7 @@ struct Ops {
                          part.as<double>(), nb, k, k, Cdev(), k);
     }
     MHIP_HIP_CHECK(hipGetLastError());
-    MHIP_HIP_CHECK(hipMemcpyAsync(C_host, Cdev(), sizeof(double) * k * k,
-                                  hipMemcpyDeviceToHost, st()));
-    MHIP_HIP_CHECK(hipStreamSynchronize(st()));
+    d2h(ctx, C_host, Cdev(), sizeof(double) * k * k);
   }
   double residual(const double* V, const double* AV, int k, const RitzCoef& rc,
                   double* X, double* R) {
@@ -336,8 +334,7 @@ struct Ops {
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, 1,
                        scalar());
     double r2 = 0.;
-    MHIP_HIP_CHECK(hipMemcpyAsync(&r2, scalar(), 8, hipMemcpyDeviceToHost, st()));
-    MHIP_HIP_CHECK(hipStreamSynchronize(st()));
+    d2h(ctx, &r2, scalar(), 8);
     return std::sqrt(r2);
   }
 };
@@ -419,7 +416,10 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   t_all.start();
   Ops ops(ctx, N);
   const int64_t ld = N;
-  DevBuf Vb(sizeof(double) * N * (max_m + 1)), AVb(sizeof(double) * N * (max_m + 1));
+  // subspace storage comes from (and goes back to) the context's kept
+  // buffers: solves of the same size reuse it without remapping memory
+  DevBuf Vb = take_buffer(ctx, sizeof(double) * N * (max_m + 1));
+  DevBuf AVb = take_buffer(ctx, sizeof(double) * N * (max_m + 1));
   double* V = Vb.as<double>();
   double* AV = AVb.as<double>();
   DevBuf Dbuf;
@@ -478,6 +478,8 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   MHIP_HIP_CHECK(hipStreamSynchronize(st));
   t_all.stop();
   tm.dav_total_ms = t_all.ms();
+  keep_buffer(ctx, std::move(Vb));
+  keep_buffer(ctx, std::move(AVb));
   tm.dav_iters = iter;
   tm.dav_restarts = restarts;
   tm.dav_residual = rnorm;
@@ -496,9 +498,7 @@ void guess_rule(mhip_context* ctx, int64_t N, const double* d_D, double* d_C) {
   hipLaunchKernelGGL(argmin_maxabs_kernel, dim3(nb), dim3(256), 0, ctx->stream, N, d_D, d_C,
                      pmin, pidx, pmax);
   std::vector<double> h(3 * nb);
-  MHIP_HIP_CHECK(hipMemcpyAsync(h.data(), p.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost,
-                                ctx->stream));
-  MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  d2h(ctx, h.data(), p.p, sizeof(double) * 3 * nb);
   double mn = INFINITY, mx = 0.;
   int64_t mi = 0;
   for(int b = 0; b < nb; ++b) {
@@ -565,7 +565,13 @@ int mhip_selected_ci_diag(mhip_context* ctx, int words_per_det,
     if(ndets) guess_rule(ctx, ndets, D.as<double>(), d_C);
     bool ok = davidson(ctx, H.get(), max_m, max_restarts, res_tol, D.as<double>(), d_C, E,
                        nullptr);
-    if(H_out) *H_out = H.release();
+    if(H_out) {
+      *H_out = H.release();
+    } else {  // keep the storage for the next build on this context
+      keep_buffer(ctx, std::move(H->rowptr));
+      keep_buffer(ctx, std::move(H->colind));
+      keep_buffer(ctx, std::move(H->nzval));
+    }
     if(!ok) throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");
   });
 }

@@ -446,19 +446,18 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   t.start();
   hipLaunchKernelGGL(widen_counts_kernel, dim3((nbra + 1 + 255) / 256), dim3(256), 0, st,
                      counts32.as<uint32_t>(), nbra, counts64.as<int64_t>());
-  DevBuf rp_struct(sizeof(int64_t) * (nbra + 1));
+  DevBuf rp_struct = take_buffer(ctx, sizeof(int64_t) * (nbra + 1));
   exclusive_scan(ctx, counts64.as<int64_t>(), rp_struct.as<int64_t>(), nbra + 1);
   int64_t nnz_struct = 0;
-  MHIP_HIP_CHECK(hipMemcpyAsync(&nnz_struct, rp_struct.as<int64_t>() + nbra, 8,
-                                hipMemcpyDeviceToHost, st));
+  d2h(ctx, &nnz_struct, rp_struct.as<int64_t>() + nbra, 8);
   t.stop();
   tm.ham_scan_ms = t.ms();  // synchronises
   tm.ham_nnz_struct = nnz_struct;
   counts32.release();
 
   // 4. fill (the product path evaluates the elements in the same pass)
-  DevBuf col(sizeof(int32_t) * std::max<int64_t>(nnz_struct, 1));
-  DevBuf val(sizeof(double) * std::max<int64_t>(nnz_struct, 1));
+  DevBuf col = take_buffer(ctx, sizeof(int32_t) * std::max<int64_t>(nnz_struct, 1));
+  DevBuf val = take_buffer(ctx, sizeof(double) * std::max<int64_t>(nnz_struct, 1));
   DevBuf diag(sizeof(double) * nbra);
   DevBuf flag(sizeof(int));
   MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
@@ -494,7 +493,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   }
   MHIP_HIP_CHECK(hipGetLastError());
   int dropped = 0;
-  MHIP_HIP_CHECK(hipMemcpyAsync(&dropped, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  d2h(ctx, &dropped, flag.p, sizeof(int));
   t.stop();
   tm.ham_elem_ms = t.ms();
 
@@ -511,9 +510,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
     DevBuf rp_new(sizeof(int64_t) * (nbra + 1));
     exclusive_scan(ctx, counts64.as<int64_t>(), rp_new.as<int64_t>(), nbra + 1);
     int64_t nnz = 0;
-    MHIP_HIP_CHECK(hipMemcpyAsync(&nnz, rp_new.as<int64_t>() + nbra, 8,
-                                  hipMemcpyDeviceToHost, st));
-    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    d2h(ctx, &nnz, rp_new.as<int64_t>() + nbra, 8);
     DevBuf col2(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
     DevBuf val2(sizeof(double) * std::max<int64_t>(nnz, 1));
     hipLaunchKernelGGL(compact_kernel, dim3((nbra + 3) / 4), dim3(256), 0, st, nbra,
@@ -557,6 +554,7 @@ mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
                     int64_t nbra, const uint64_t* d_ket, int64_t nket,
                     double thresh) {
   use_device(ctx);
+  const auto t_entry = std::chrono::steady_clock::now();
   check_dets(W, d_bra, nbra, "bra");
   check_dets(W, d_ket, nket, "ket");
   MHIP_REQUIRE(nket < (1ll << 31), MHIP_ERR_INDEX_OVERFLOW,
@@ -600,6 +598,11 @@ mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
     build_impl<1>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), ex);
   else
     build_impl<2>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), ex);
+  // end-to-end host wall time of the call (validation, recognition, all
+  // passes; build_impl has synchronised)
+  ctx->timings.ham_total_ms = std::chrono::duration<double, std::milli>(
+                                  std::chrono::steady_clock::now() - t_entry)
+                                  .count();
   return csr.release();
 }
 
@@ -637,8 +640,7 @@ void validate_dets(mhip_context* ctx, int W, const uint64_t* d, int64_t n) {
                        o.as<unsigned long long>());
   MHIP_HIP_CHECK(hipGetLastError());
   uint64_t h[2] = {0, 0};
-  MHIP_HIP_CHECK(hipMemcpyAsync(h, o.p, 8 * W, hipMemcpyDeviceToHost, ctx->stream));
-  MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  d2h(ctx, h, o.p, 8 * W);
   const int64_t norb = ctx->ints.norb;
   uint64_t a, b;
   if(W == 1) {

@@ -96,6 +96,28 @@ struct DevBuf {
   }
 };
 
+// Pinned host staging buffer (grow-only) for small device-to-host reads:
+// pageable copies go through the runtime's own staging and have been seen
+// to stall for milliseconds.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if(p) (void)hipHostFree(p);
+  }
+  void reserve(size_t b) {
+    if(b <= bytes) return;
+    if(p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    MHIP_HIP_CHECK(hipHostMalloc(&p, b, hipHostMallocDefault));
+    bytes = b;
+  }
+};
+
 // Integral tables on the device (column-major, see macis_hip.h).
 struct Integrals {
   int64_t norb = 0;
@@ -134,6 +156,11 @@ struct mhip_context {
   // Reusable scratch (partials, flags, temp storage of device primitives).
   mhip::DevBuf scratch_a, scratch_b, scratch_c;
   std::vector<char> host_scratch;
+  mhip::PinnedBuf pinned;  // staging of small device-to-host reads
+  // device buffers kept for reuse (storage of the last internal Hamiltonian
+  // of selected_ci_diag, Davidson subspaces): freeing and re-mapping large
+  // buffers on every solve stalls the next blocking call for milliseconds
+  std::vector<mhip::DevBuf> spare;
   int build_path = 0;  // MHIP_BUILD_*
 };
 
@@ -149,6 +176,15 @@ struct mhip_csr {
 namespace mhip {
 // Makes the context's device current.
 void use_device(mhip_context* ctx);
+// Device-to-host copy on the context stream, waited for; copies up to 4 MiB
+// are staged through the context's pinned buffer.
+void d2h(mhip_context* ctx, void* dst, const void* src, size_t bytes);
+// A device buffer of at least `bytes`: a kept one of capacity in
+// [bytes, 2 bytes] if the context holds one, else a new allocation (kept
+// buffers are dropped first when the device is out of memory).
+DevBuf take_buffer(mhip_context* ctx, size_t bytes);
+// Keeps `b` on the context for take_buffer (at most 8 buffers).
+void keep_buffer(mhip_context* ctx, DevBuf&& b);
 
 // Hamiltonian construction shared by mhip_ham_build and selected_ci_diag.
 mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,

@@ -528,8 +528,7 @@ int mhip_mlp_train_step(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
       hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, net.loss.as<float>(), nblk,
                          tot.as<float>());
       float l = 0.f;
-      MHIP_HIP_CHECK(hipMemcpyAsync(&l, tot.p, 4, hipMemcpyDeviceToHost, st));
-      MHIP_HIP_CHECK(hipStreamSynchronize(st));
+      d2h(ctx, &l, tot.p, 4);
       *loss_out = l / (float)n;
     }
   });

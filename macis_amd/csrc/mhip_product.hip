@@ -394,8 +394,7 @@ bool detect(mhip_context* ctx, const uint64_t* d, int64_t n, ProductSpace& ps) {
   for(int64_t want = std::min<int64_t>(n, 4096);; want = std::min<int64_t>(n, kMaxHead)) {
     nh = want;
     head.resize(nh * W);
-    MHIP_HIP_CHECK(hipMemcpyAsync(head.data(), d, 8 * W * nh, hipMemcpyDeviceToHost, st));
-    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    d2h(ctx, head.data(), d, 8 * W * nh);
     auto alpha = [&](int64_t i) { return W == 1 ? head[i] & 0xffffffffull : head[2 * i]; };
     auto beta = [&](int64_t i) { return W == 1 ? head[i] >> 32 : head[2 * i + 1]; };
     if(alpha(0) == alpha(1)) outer_alpha = true;
@@ -441,8 +440,7 @@ bool detect(mhip_context* ctx, const uint64_t* d, int64_t n, ProductSpace& ps) {
   MHIP_HIP_CHECK(hipGetLastError());
   // ostr, istr, counts and the flag are contiguous: one copy back
   std::vector<char> h(o_flag + sizeof(int) - o_ostr);
-  MHIP_HIP_CHECK(hipMemcpyAsync(h.data(), A, h.size(), hipMemcpyDeviceToHost, st));
-  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  d2h(ctx, h.data(), A, h.size());
   int bad = 0;
   std::memcpy(&bad, h.data() + o_flag, sizeof(int));
   if(bad) return false;

@@ -156,3 +156,29 @@ def test_selected_ci_diag_restarted_vs_oracle(ctx, n, ne, seed):
     assert E == pytest.approx(Eo, abs=1e-8)
     assert abs(abs(X.numpy() @ Xo) - 1.0) < 1e-6
     assert ctx.timings()["dav_residual"] < 1e-8
+
+
+def test_selected_ci_diag_reuses_kept_storage(ctx):
+    """The context keeps the internal Hamiltonian's storage between
+    selected_ci_diag calls; reusing it (same list, then a smaller one that
+    fits) and releasing it give bit-identical results."""
+    from macis_amd import dets as D
+    from macis_amd.synthetic import seeded_integrals
+
+    announce("selected_ci_diag with kept Hamiltonian storage")
+    n = 8
+    T, V = seeded_integrals(n, 21)
+    ctx.set_integrals(n, T, V)
+    big = D.generate_hilbert_space(1, n, 4, 4)
+    small = D.generate_hilbert_space(1, n, 4, 3)  # 3920 of 4900 dets: reuses the kept buffers
+    out = []
+    for dets in (big, big, small, big):
+        dX = ctx.empty(len(dets), np.float64).zero()
+        E, nnz, _ = ctx.selected_ci_diag(1, ctx.to_device(dets), len(dets), dX, max_restarts=200)
+        out.append((E, nnz, dX.numpy()))
+    ctx.release_cached()
+    dX = ctx.empty(len(small), np.float64).zero()
+    E, nnz, _ = ctx.selected_ci_diag(1, ctx.to_device(small), len(small), dX, max_restarts=200)
+    assert out[0][0] == out[1][0] == out[3][0] and np.array_equal(out[0][2], out[1][2])
+    assert np.array_equal(out[0][2], out[3][2])
+    assert E == out[2][0] and nnz == out[2][1] and np.array_equal(dX.numpy(), out[2][2])
