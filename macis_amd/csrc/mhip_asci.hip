@@ -345,6 +345,91 @@ __global__ void dets_to_keys_kernel(const uint64_t* __restrict__ d, int64_t n,
 inline int64_t host_pairs(int64_t m) { return m * (m - 1) / 2; }
 
 // Order-preserving map of float predictions to unsigned keys.
+// Histogram of 12-bit digits of |rv| bit patterns (non-negative doubles
+// order like their bits): digit = (key >> shift) & 4095, counted only for
+// keys whose bits above shift + 12 equal `prefix` (all keys when shift = 52).
+__global__ __launch_bounds__(256) void digit_hist_kernel(const uint64_t* __restrict__ keys,
+                                                         int64_t n, int shift, uint64_t prefix,
+                                                         uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[4096];
+  for(int i = threadIdx.x; i < 4096; i += 256) h[i] = 0;
+  __syncthreads();
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+      i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    const bool match = shift + 12 >= 64 || (k >> (shift + 12)) == prefix;
+    if(match) atomicAdd(&h[(k >> shift) & 4095u], 1u);
+  }
+  __syncthreads();
+  for(int i = threadIdx.x; i < 4096; i += 256)
+    if(h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Candidates equal to a core determinant are not new (the reference keeps
+// the core copy with +|C| and drops rv >= 0, determinant_search.hpp:533-544):
+// their accumulated score is zeroed, which removes them with the zero scores.
+template <typename K>
+__global__ void core_zero_kernel(const K* __restrict__ ukeys, int64_t nu,
+                                 const K* __restrict__ core, int64_t nc,
+                                 double* __restrict__ usum) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= nc) return;
+  const K k = core[c];
+  int64_t lo = 0, hi = nu;
+  while(lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if(ukeys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if(lo < nu && ukeys[lo] == k) usum[lo] = 0.;
+}
+
+// Digit histogram of |score| bit patterns straight from the accumulated
+// scores (zero scores skipped).
+__global__ __launch_bounds__(256) void score_hist_kernel(const double* __restrict__ usum,
+                                                         int64_t n, int shift, uint64_t prefix,
+                                                         uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[4096];
+  for(int i = threadIdx.x; i < 4096; i += 256) h[i] = 0;
+  __syncthreads();
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+      i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = (uint64_t)__double_as_longlong(fabs(usum[i]));
+    const bool match = k != 0 && (shift + 12 >= 64 || (k >> (shift + 12)) == prefix);
+    if(match) atomicAdd(&h[(k >> shift) & 4095u], 1u);
+  }
+  __syncthreads();
+  for(int i = threadIdx.x; i < 4096; i += 256)
+    if(h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// flags[i] = nonzero score whose |score| top 24 bits are >= thr24.
+__global__ void score_above_kernel(const double* __restrict__ usum, int64_t n, uint64_t thr24,
+                                   uint8_t* __restrict__ flags) {
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+      i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = (uint64_t)__double_as_longlong(fabs(usum[i]));
+    flags[i] = (k != 0 && (k >> 40) >= thr24) ? 1 : 0;
+  }
+}
+
+__global__ void abs_bits_kernel(const double* __restrict__ v, int64_t n,
+                                uint64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) out[i] = (uint64_t)__double_as_longlong(fabs(v[i]));
+}
+
+// flags[i] = key's top 24 bits >= threshold (the candidates that can reach
+// the top-k once the boundary 24-bit bucket is known).
+__global__ void above_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t thr24,
+                             uint8_t* __restrict__ flags) {
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+      i += (int64_t)gridDim.x * blockDim.x)
+    flags[i] = (keys[i] >> 40) >= thr24 ? 1 : 0;
+}
+
 __global__ void pred_keys_kernel(const float* __restrict__ pred, int64_t n,
                                  uint32_t* __restrict__ key, int64_t* __restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -516,6 +601,95 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   MHIP_HIP_CHECK(rocprim::radix_sort_keys(ctx->scratch_c.p, tmp, core_keys.as<K>(),
                                           core_sorted.as<K>(), (size_t)nc, 0, 32 * W * 2, st));
   const int64_t top_k = ndets_max - nc;
+  if(!mode.exp_dets && !mode.mlp) {
+    // Plain search: exclusion and top-k straight on the accumulated arrays.
+    // The k-th largest |score| is bracketed by two 12-bit digit histograms
+    // (24-bit prefix); only candidates at or above that prefix (in
+    // determinant order) are selected, and a stable descending sort of them
+    // gives the same first k as sorting all candidates (ties keep
+    // determinant order, as the reference's comparison does).
+    double* us = usum.as<double>();
+    if(nu && nc)
+      hipLaunchKernelGGL((core_zero_kernel<K>), dim3((nc + 255) / 256), dim3(256), 0, st,
+                         ukeys.as<K>(), nu, core_sorted.as<K>(), nc, us);
+    DevBuf hist(sizeof(uint32_t) * 4096);
+    std::vector<uint32_t> hh(4096);
+    const unsigned hblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (nu + 255) / 256));
+    uint64_t prefix = 0;
+    int64_t above = 0, in_bucket = 0, eligible = 0;
+    const int npass = top_k > 0 ? 2 : 0;
+    for(int pass = 0; pass < std::max(npass, 1); ++pass) {
+      const int shift = pass == 0 ? 52 : 40;
+      MHIP_HIP_CHECK(hipMemsetAsync(hist.p, 0, sizeof(uint32_t) * 4096, st));
+      if(nu)
+        hipLaunchKernelGGL(score_hist_kernel, dim3(hblocks), dim3(256), 0, st, us, nu, shift,
+                           prefix, hist.as<uint32_t>());
+      MHIP_HIP_CHECK(hipGetLastError());
+      d2h(ctx, hh.data(), hist.p, sizeof(uint32_t) * 4096);
+      if(pass == 0)
+        for(uint32_t c : hh) eligible += c;
+      if(eligible <= top_k || top_k <= 0) break;  // everything fits (or nothing is asked)
+      int d = 4095;
+      while(d > 0 && above + (int64_t)hh[d] < top_k) above += hh[d--];
+      prefix = (prefix << 12) | (uint64_t)d;
+      in_bucket = hh[d];
+    }
+    tm.asci_nn_ms = 0.;
+    tm.asci_nsurvivors = eligible;
+    int64_t nout = 0;
+    DevBuf fl, cnt(16), sk, sv, ab, skeys, sabs;
+    if(top_k > 0 && eligible > 0) {
+      const bool all = eligible <= top_k;
+      const uint64_t thr = all ? 0 : prefix;
+      const int64_t nsel_expected = all ? eligible : above + in_bucket;
+      fl.alloc(nu);
+      hipLaunchKernelGGL(score_above_kernel, dim3(hblocks), dim3(256), 0, st, us, nu, thr,
+                         fl.as<uint8_t>());
+      sk.alloc(sizeof(K) * nsel_expected);
+      sv.alloc(8 * nsel_expected);
+      tmp = 0;
+      MHIP_HIP_CHECK(rocprim::select(nullptr, tmp, ukeys.as<K>(), fl.as<uint8_t>(), sk.as<K>(),
+                                     cnt.as<int64_t>(), (size_t)nu, st));
+      ctx->scratch_c.reserve(tmp + 16);
+      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, ukeys.as<K>(), fl.as<uint8_t>(),
+                                     sk.as<K>(), cnt.as<int64_t>(), (size_t)nu, st));
+      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, us, fl.as<uint8_t>(),
+                                     sv.as<double>(), cnt.as<int64_t>(), (size_t)nu, st));
+      int64_t nsel = 0;
+      d2h(ctx, &nsel, cnt.p, 8);
+      MHIP_REQUIRE(nsel == nsel_expected, MHIP_ERR_HIP, "top-k selection count mismatch");
+      const K* outk = sk.as<K>();
+      nout = nsel;
+      if(!all) {
+        ab.alloc(8 * nsel);
+        hipLaunchKernelGGL(abs_bits_kernel, dim3((nsel + 255) / 256), dim3(256), 0, st,
+                           sv.as<double>(), nsel, ab.as<uint64_t>());
+        skeys.alloc(sizeof(K) * nsel);
+        sabs.alloc(8 * nsel);
+        tmp = 0;
+        MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, ab.as<uint64_t>(),
+                                                      sabs.as<uint64_t>(), sk.as<K>(),
+                                                      skeys.as<K>(), (size_t)nsel, 0, 64, st));
+        ctx->scratch_c.reserve(tmp + 16);
+        MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, ab.as<uint64_t>(),
+                                                      sabs.as<uint64_t>(), sk.as<K>(),
+                                                      skeys.as<K>(), (size_t)nsel, 0, 64, st));
+        outk = skeys.as<K>();
+        nout = top_k;
+      }
+      hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nout + 255) / 256), dim3(256), 0, st,
+                         outk, nout, d_out);
+    }
+    if(nc)
+      MHIP_HIP_CHECK(hipMemcpyAsync(d_out + W * nout, d_cdets, 8 * W * nc,
+                                    hipMemcpyDeviceToDevice, st));
+    MHIP_HIP_CHECK(hipGetLastError());
+    t.stop();
+    tm.asci_topk_ms = t.ms();
+    t_all.stop();
+    tm.asci_total_ms = t_all.ms();
+    return nout + nc;
+  }
   int64_t nkeep = 0;
   DevBuf flags(std::max<int64_t>(nu, 1)), absbits(8 * std::max<int64_t>(nu, 1));
   DevBuf kkeys(sizeof(K) * std::max<int64_t>(nu, 1)), kabs(8 * std::max<int64_t>(nu, 1));
@@ -596,19 +770,70 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   }
   int64_t nout = nkeep;
   const K* outkeys = kkeys.as<K>();
-  DevBuf skeys, sabs;
-  if(nkeep > top_k) {
-    // stable descending sort on |rv| bits: ties keep ascending det order
-    skeys.alloc(sizeof(K) * nkeep);
-    sabs.alloc(8 * nkeep);
+  DevBuf skeys, sabs, fkeys, fabsb, flg2, cnt2;
+  if(top_k <= 0) {
+    nout = 0;
+  } else if(nkeep > top_k) {
+    // Top-k by |rv| (stable descending: ties keep determinant order). The
+    // boundary 24-bit prefix of the k-th largest key is found from two 12-bit
+    // digit histograms; the candidates at or above it (in determinant order)
+    // are the only ones that can be in the top-k, and a stable descending sort
+    // of those gives the same first k as sorting everything.
+    const uint64_t* kb = kabs.as<uint64_t>();
+    DevBuf hist(sizeof(uint32_t) * 4096);
+    std::vector<uint32_t> hh(4096);
+    const unsigned hblocks = (unsigned)std::min<int64_t>(1024, (nkeep + 255) / 256);
+    uint64_t prefix = 0;
+    int64_t above = 0;  // keys strictly above the current prefix bucket
+    int64_t in_bucket = 0;  // keys in the boundary bucket
+    for(int pass = 0; pass < 2; ++pass) {
+      const int shift = pass == 0 ? 52 : 40;
+      MHIP_HIP_CHECK(hipMemsetAsync(hist.p, 0, sizeof(uint32_t) * 4096, st));
+      hipLaunchKernelGGL(digit_hist_kernel, dim3(hblocks), dim3(256), 0, st, kb, nkeep, shift,
+                         prefix, hist.as<uint32_t>());
+      MHIP_HIP_CHECK(hipGetLastError());
+      d2h(ctx, hh.data(), hist.p, sizeof(uint32_t) * 4096);
+      int d = 4095;
+      while(d > 0 && above + (int64_t)hh[d] < top_k) above += hh[d--];
+      prefix = (prefix << 12) | (uint64_t)d;
+      in_bucket = hh[d];
+    }
+    const int64_t nsel_expected = above + in_bucket;
+    flg2.alloc(nkeep);
+    hipLaunchKernelGGL(above_kernel, dim3(hblocks), dim3(256), 0, st, kb, nkeep, prefix,
+                       flg2.as<uint8_t>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    cnt2.alloc(16);
     tmp = 0;
-    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, kabs.as<uint64_t>(),
-                                                  sabs.as<uint64_t>(), kkeys.as<K>(),
-                                                  skeys.as<K>(), (size_t)nkeep, 0, 64, st));
+    MHIP_HIP_CHECK(rocprim::select(nullptr, tmp, kkeys.as<K>(), flg2.as<uint8_t>(),
+                                   kkeys.as<K>(), cnt2.as<int64_t>(), (size_t)nkeep, st));
+    // candidates at or above the boundary bucket
+    int64_t nsel = 0;
+    {
+      DevBuf ksel(sizeof(K) * std::max<int64_t>(nsel_expected, 1)),
+          asel(8 * std::max<int64_t>(nsel_expected, 1));
+      ctx->scratch_c.reserve(tmp + 16);
+      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, kkeys.as<K>(), flg2.as<uint8_t>(),
+                                     ksel.as<K>(), cnt2.as<int64_t>(), (size_t)nkeep, st));
+      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, kb, flg2.as<uint8_t>(),
+                                     asel.as<uint64_t>(), cnt2.as<int64_t>(), (size_t)nkeep,
+                                     st));
+      d2h(ctx, &nsel, cnt2.p, 8);
+      MHIP_REQUIRE(nsel == nsel_expected && nsel >= top_k, MHIP_ERR_HIP,
+                   "top-k selection count mismatch");
+      fkeys = std::move(ksel);
+      fabsb = std::move(asel);
+    }
+    skeys.alloc(sizeof(K) * nsel);
+    sabs.alloc(8 * nsel);
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, fabsb.as<uint64_t>(),
+                                                  sabs.as<uint64_t>(), fkeys.as<K>(),
+                                                  skeys.as<K>(), (size_t)nsel, 0, 64, st));
     ctx->scratch_c.reserve(tmp + 16);
-    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, kabs.as<uint64_t>(),
-                                                  sabs.as<uint64_t>(), kkeys.as<K>(),
-                                                  skeys.as<K>(), (size_t)nkeep, 0, 64, st));
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, fabsb.as<uint64_t>(),
+                                                  sabs.as<uint64_t>(), fkeys.as<K>(),
+                                                  skeys.as<K>(), (size_t)nsel, 0, 64, st));
     outkeys = skeys.as<K>();
     nout = std::max<int64_t>(top_k, 0);
   }
