@@ -209,7 +209,8 @@ int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
  * max_restarts == 0 the reference algorithm is followed exactly and
  * MHIP_ERR_NOT_CONVERGED is returned when max_m is exhausted; with
  * max_restarts > 0 (a labelled variant, not in the reference) an exhausted
- * subspace restarts from [X, next preconditioned residual]. E receives
+ * subspace restarts from [X, previous Ritz vector, next preconditioned
+ * residual] with A applied by combination of the stored products. E receives
  * LAM(0), iters the iteration count (davidson.hpp:224). */
 int mhip_davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
                   int64_t max_restarts, double tol, const double* d_D,

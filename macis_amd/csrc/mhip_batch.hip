@@ -114,14 +114,18 @@ __global__ __launch_bounds__(256) void bgram_kernel(Seg sg, int k, const double*
 }
 
 // Lowest eigenpair of each C_b (lower triangle) by cyclic Jacobi in LDS,
-// one wavefront per instance: lam[b], coef[b][:k].
+// one wavefront per instance: lam[b], coef[b][:k], the second-lowest
+// eigenvector in coef2[b][:k]; the previous coef[b] moves to coef_prev[b]
+// (restart coefficients of the restarted variant).
 __global__ __launch_bounds__(64) void bjacobi_kernel(Seg sg, int k, const double* __restrict__ C,
-                                                     double* __restrict__ lam,
-                                                     double* __restrict__ coef) {
+                                                     double* __restrict__ lam, double* coef,
+                                                     double* __restrict__ coef_prev,
+                                                     double* __restrict__ coef2) {
   const int b = blockIdx.x;
   if(sg.conv[b]) return;
   __shared__ double a[kMaxK][kMaxK + 1], v[kMaxK][kMaxK + 1];
   const int t = threadIdx.x;
+  if(t < kMaxK) coef_prev[(int64_t)b * kMaxK + t] = coef[(int64_t)b * kMaxK + t];
   for(int e = t; e < k * k; e += 64) {
     const int i = e % k, j = e / k;
     const double x = i >= j ? C[(int64_t)b * kMaxK * kMaxK + i + j * kMaxK]
@@ -169,8 +173,86 @@ __global__ __launch_bounds__(64) void bjacobi_kernel(Seg sg, int k, const double
   int best = 0;
   for(int i = 1; i < k; ++i)
     if(a[i][i] < a[best][best]) best = i;
+  int second = best == 0 ? 1 : 0;
+  for(int i = 0; i < k; ++i)
+    if(i != best && a[i][i] < a[second][second]) second = i;
   if(t == 0) lam[b] = a[best][best];
-  if(t < kMaxK) coef[(int64_t)b * kMaxK + t] = t < k ? v[t][best] : 0.;
+  if(t < kMaxK) {
+    coef[(int64_t)b * kMaxK + t] = t < k ? v[t][best] : 0.;
+    coef2[(int64_t)b * kMaxK + t] = t < k && k > 1 ? v[t][second] : 0.;
+  }
+}
+
+// Restart coefficients of every unconverged instance (one thread each),
+// Q[b] = [q0 | q1] with kMaxK entries per column: q0 = c/|c|; for nb == 2,
+// q1 = (c_prev, 0) orthogonalised against q0 by two classical Gram-Schmidt
+// passes and normalised, or the second Ritz vector's coefficients treated
+// the same way when prev_valid is 0 or that norm is <= 1e-10.
+__global__ void brestart_coef_kernel(int ninst, const int32_t* __restrict__ conv, int k, int nb,
+                                     int prev_valid, const double* __restrict__ coef,
+                                     const double* __restrict__ coef_prev,
+                                     const double* __restrict__ coef2, double* __restrict__ Q) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if(b >= ninst || conv[b]) return;
+  const double* c = coef + (int64_t)b * kMaxK;
+  double* q0 = Q + (int64_t)b * 2 * kMaxK;
+  double* q1 = q0 + kMaxK;
+  double cn = 0.;
+  for(int j = 0; j < k; ++j) cn += c[j] * c[j];
+  cn = 1. / sqrt(cn);
+  for(int j = 0; j < kMaxK; ++j) {
+    q0[j] = j < k ? c[j] * cn : 0.;
+    q1[j] = 0.;
+  }
+  if(nb < 2) return;
+  for(int attempt = prev_valid ? 0 : 1; attempt < 2; ++attempt) {
+    for(int j = 0; j < k; ++j)
+      q1[j] = attempt ? coef2[(int64_t)b * kMaxK + j]
+                      : (j < k - 1 ? coef_prev[(int64_t)b * kMaxK + j] : 0.);
+    for(int pass = 0; pass < 2; ++pass) {
+      double s = 0.;
+      for(int j = 0; j < k; ++j) s += q0[j] * q1[j];
+      for(int j = 0; j < k; ++j) q1[j] -= s * q0[j];
+    }
+    double pn = 0.;
+    for(int j = 0; j < k; ++j) pn += q1[j] * q1[j];
+    pn = sqrt(pn);
+    if(pn > 1e-10) {
+      for(int j = 0; j < k; ++j) q1[j] /= pn;
+      return;
+    }
+  }
+}
+
+// In-place restart of the rows of unconverged instances:
+// V(:,0..nb-1) = V(:, :k) Q_b, AV(:,0..nb-1) = AV(:, :k) Q_b,
+// V(:,nb) = V(:,tcol). Every thread reads its row before writing it.
+__global__ __launch_bounds__(256) void brestart_apply_kernel(
+    int64_t N, int k, int nb, const int32_t* __restrict__ row_inst,
+    const int32_t* __restrict__ conv, const double* __restrict__ Q, double* V, double* AV,
+    int64_t ld, int64_t tcol) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    const int b = row_inst[n];
+    if(conv[b]) continue;
+    const double* q0 = Q + (int64_t)b * 2 * kMaxK;
+    const double* q1 = q0 + kMaxK;
+    double v0 = 0., v1 = 0., a0 = 0., a1 = 0.;
+    for(int j = 0; j < k; ++j) {
+      const double x = V[n + j * ld], y = AV[n + j * ld];
+      v0 = fma(x, q0[j], v0);
+      a0 = fma(y, q0[j], a0);
+      v1 = fma(x, q1[j], v1);
+      a1 = fma(y, q1[j], a1);
+    }
+    const double t = V[n + tcol * ld];
+    V[n] = v0;
+    AV[n] = a0;
+    if(nb > 1) {
+      V[n + ld] = v1;
+      AV[n + ld] = a1;
+    }
+    V[n + nb * ld] = t;
+  }
 }
 
 // X = V c_b; R = AV c_b - lam_b X (rows of unconverged instances).
@@ -305,13 +387,6 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
   if(i < n) p[i] = v;
 }
 
-__global__ void copy_unconv_kernel(int64_t N, const double* __restrict__ src, double* __restrict__ dst,
-                                   const int32_t* __restrict__ row_inst,
-                                   const int32_t* __restrict__ conv) {
-  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256)
-    if(!conv[row_inst[n]]) dst[n] = src[n];
-}
-
 }  // namespace
 }  // namespace mhip
 
@@ -334,7 +409,7 @@ struct mhip_batch {
   bool tuned = false;
   // Davidson workspace
   int64_t max_m = 0;
-  mhip::DevBuf V, AV, X, D, C, coef, lam, tmp, conv, E, res, nleft;
+  mhip::DevBuf V, AV, X, D, C, coef, coef_prev, coef2, Q, lam, tmp, conv, E, res, nleft;
   std::map<int, hipGraphExec_t> graphs;
   ~mhip_batch() {
     for(auto& g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -401,7 +476,8 @@ void biteration(mhip_context* ctx, mhip_batch* B, int i, double tol) {
   hipLaunchKernelGGL(bgram_kernel, dim3(B->ninst), dim3(256), 0, st, sg, k, V, AV, ld,
                      B->C.as<double>());
   hipLaunchKernelGGL(bjacobi_kernel, dim3(B->ninst), dim3(64), 0, st, sg, k, B->C.as<double>(),
-                     B->lam.as<double>(), B->coef.as<double>());
+                     B->lam.as<double>(), B->coef.as<double>(), B->coef_prev.as<double>(),
+                     B->coef2.as<double>());
   double* R = V + (i + 1) * ld;
   hipLaunchKernelGGL(bresidual_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, k, V, AV, ld,
                      B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->coef.as<double>(),
@@ -648,6 +724,9 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
       B->D.alloc(8 * N);
       B->C.alloc(8 * (size_t)B->ninst * kMaxK * kMaxK);
       B->coef.alloc(8 * (size_t)B->ninst * kMaxK);
+      B->coef_prev.alloc(8 * (size_t)B->ninst * kMaxK);
+      B->coef2.alloc(8 * (size_t)B->ninst * kMaxK);
+      B->Q.alloc(8 * (size_t)B->ninst * 2 * kMaxK);
       B->lam.alloc(8 * B->ninst);
       B->tmp.alloc(8 * (size_t)B->ninst * kMaxK);
       B->conv.alloc(4 * B->ninst);
@@ -670,8 +749,10 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
     MHIP_HIP_CHECK(hipMemcpyAsync(V + ld, AV, 8 * N, hipMemcpyDeviceToDevice, st));
     bgram_schmidt(ctx, B, 1, V + ld);
     int32_t nleft = B->ninst;
+    int i0 = 1;
+    const int nb = max_m >= 3 ? 2 : 1;
     for(int64_t rst = 0;; ++rst) {
-      for(int i = 1; i < max_m && nleft; ++i) {
+      for(int i = i0; i < max_m && nleft; ++i) {
         if(use_graphs) {
           auto it = B->graphs.find(i);
           if(it == B->graphs.end()) {
@@ -691,17 +772,19 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
         d2h(ctx, &nleft, B->nleft.p, 4);
       }
       if(!nleft || rst >= max_restarts) break;
-      // restart unconverged instances: V0 = X/|X|, V1 = V(:,max_m), AV0 = A V0
-      hipLaunchKernelGGL(copy_unconv_kernel, dim3(red_grid(N)), dim3(256), 0, st, N,
-                         B->X.as<double>(), V, B->row_inst.as<int32_t>(), B->conv.as<int32_t>());
-      hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, 1, V, ld, V,
-                         B->tmp.as<double>());
-      hipLaunchKernelGGL(bscale_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, V,
-                         B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->tmp.as<double>());
-      hipLaunchKernelGGL(copy_unconv_kernel, dim3(red_grid(N)), dim3(256), 0, st, N,
-                         V + max_m * ld, V + ld, B->row_inst.as<int32_t>(), B->conv.as<int32_t>());
-      launch_spmv(ctx, B, V, AV, true);
+      // restart unconverged instances from [V c, V q1, t] (AV by the same
+      // combinations, no SpMV); the last two iterations shared the basis
+      // when the cycle ran at least two of them
+      const int k = (int)max_m;
+      hipLaunchKernelGGL(brestart_coef_kernel, dim3((B->ninst + 63) / 64), dim3(64), 0, st,
+                         B->ninst, B->conv.as<int32_t>(), k, nb, (int)(max_m - 1 > i0),
+                         B->coef.as<double>(), B->coef_prev.as<double>(), B->coef2.as<double>(),
+                         B->Q.as<double>());
+      hipLaunchKernelGGL(brestart_apply_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, k, nb,
+                         B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->Q.as<double>(), V,
+                         AV, ld, max_m);
       MHIP_HIP_CHECK(hipGetLastError());
+      i0 = nb;
     }
     tall.stop();
     const double ms = tall.ms();

@@ -212,6 +212,40 @@ __global__ __launch_bounds__(256) void residual_kernel(int64_t N, int k,
   if(threadIdx.x == 0) partials[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
+// Restart of the restarted variant, in place per row n:
+//   [V(:,0..nb-1)]  = V(:, :k) Q,   [AV(:,0..nb-1)] = AV(:, :k) Q,
+//   V(:,nb) = V(:,tcol)
+// Each thread reads its row of V and AV before writing it, so the in-place
+// update is race free (no other thread touches row n).
+struct RestartCoef {
+  double q[2][kMaxK];
+};
+__global__ __launch_bounds__(256) void restart_kernel(int64_t N, int k, int nb, RestartCoef Q,
+                                                      double* V, double* AV, int64_t ld,
+                                                      int64_t tcol) {
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256) {
+    double v0 = 0., v1 = 0., a0 = 0., a1 = 0.;
+#pragma unroll
+    for(int j = 0; j < kMaxK; ++j)
+      if(j < k) {
+        const double x = V[n + j * ld], y = AV[n + j * ld];
+        v0 = fma(x, Q.q[0][j], v0);
+        a0 = fma(y, Q.q[0][j], a0);
+        v1 = fma(x, Q.q[1][j], v1);
+        a1 = fma(y, Q.q[1][j], a1);
+      }
+    const double t = V[n + tcol * ld];
+    V[n] = v0;
+    AV[n] = a0;
+    if(nb > 1) {
+      V[n + ld] = v1;
+      AV[n + ld] = a1;
+    }
+    V[n + nb * ld] = t;
+  }
+}
+
 // R = -R / (D - lam)   (davidson.hpp:212-214)
 __global__ void precond_kernel(int64_t N, double* __restrict__ R,
                                const double* __restrict__ D, double lam) {
@@ -391,6 +425,36 @@ void syev_lower(int n, double* A, int lda, double* W) {
   }
 }
 
+// Restart coefficients (k x nb): q0 = c/|c|; for nb == 2, q1 = the previous
+// Ritz coefficients (c_prev, 0) orthogonalised against q0 by two classical
+// Gram-Schmidt passes and normalised, or the second Ritz vector's
+// coefficients c2 treated the same way when c_prev is absent (the last two
+// iterations did not share the basis) or its part orthogonal to q0 has norm
+// <= 1e-10.
+void restart_coefficients(int k, int nb, const double* c, const double* cprev, const double* c2,
+                          double* q0, double* q1) {
+  double cn = 0.;
+  for(int j = 0; j < k; ++j) cn += c[j] * c[j];
+  cn = 1. / std::sqrt(cn);
+  for(int j = 0; j < k; ++j) q0[j] = c[j] * cn;
+  if(nb < 2) return;
+  for(int attempt = cprev ? 0 : 1; attempt < 2; ++attempt) {
+    for(int j = 0; j < k; ++j) q1[j] = attempt ? c2[j] : (j < k - 1 ? cprev[j] : 0.);
+    for(int pass = 0; pass < 2; ++pass) {
+      double s = 0.;
+      for(int j = 0; j < k; ++j) s += q0[j] * q1[j];
+      for(int j = 0; j < k; ++j) q1[j] -= s * q0[j];
+    }
+    double pn = 0.;
+    for(int j = 0; j < k; ++j) pn += q1[j] * q1[j];
+    pn = std::sqrt(pn);
+    if(pn > 1e-10) {
+      for(int j = 0; j < k; ++j) q1[j] /= pn;
+      return;
+    }
+  }
+}
+
 bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
               int64_t max_restarts, double tol, const double* d_D, double* d_X,
               double* E, int64_t* iters) {
@@ -442,11 +506,14 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   ops.gram_schmidt(V, 1, V + ld);                                           // :157
 
   bool converged = false;
-  int64_t iter = 1, restarts = 0;
+  int64_t iter = 1, restarts = 0, i0 = 1;
   double C[kMaxK * kMaxK], LAM[kMaxK];
+  double ccur[kMaxK] = {}, cprev[kMaxK] = {}, c2cur[kMaxK] = {};
+  bool prev_valid = false;
+  int kcur = 0;
   double lam0 = 0., rnorm = 0.;
   for(;;) {
-    for(int64_t i = 1; i < max_m; ++i, ++iter) {
+    for(int64_t i = i0; i < max_m; ++i, ++iter) {
       const int k = (int)(i + 1);
       op(V + i * ld, AV + i * ld);                                          // :166
       ops.gram(V, AV, k, C);                                                // :172
@@ -455,6 +522,13 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
       RitzCoef rc;
       for(int j = 0; j < kMaxK; ++j) rc.c[j] = j < k ? C[j] : 0.;
       rc.lam = lam0;
+      std::memcpy(cprev, ccur, sizeof(ccur));
+      prev_valid = i > i0;  // cprev holds coefficients in this basis
+      for(int j = 0; j < k; ++j) {
+        ccur[j] = C[j];
+        c2cur[j] = k > 1 ? C[j + k] : 0.;
+      }
+      kcur = k;
       double* R = V + (i + 1) * ld;
       rnorm = ops.residual(V, AV, k, rc, d_X, R);                           // :181-193
       if(!std::isfinite(rnorm)) break;
@@ -466,14 +540,18 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
                          lam0);                                             // :212
       ops.gram_schmidt(V, k, R);                                            // :217
     }
-    if(converged || restarts >= max_restarts || !std::isfinite(rnorm)) break;
-    // Restart (labelled variant): V0 = X/||X||, V1 = V(:,max_m), AV0 = A V0.
+    if(converged || restarts >= max_restarts || max_m < 2 || !std::isfinite(rnorm)) break;
+    // Restart (labelled variant): basis [V c, V q1, t] with q1 from the
+    // previous Ritz vector; AV follows by the same combinations (no SpMV).
+    // t = V(:,max_m) is orthogonal to span(V) already.
     ++restarts;
-    MHIP_HIP_CHECK(hipMemcpyAsync(V, d_X, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-    ops.normalize(V);
-    MHIP_HIP_CHECK(hipMemcpyAsync(V + ld, V + max_m * ld, sizeof(double) * N,
-                                  hipMemcpyDeviceToDevice, st));
-    op(V, AV);
+    const int nb = max_m >= 3 ? 2 : 1;
+    RestartCoef Q{};
+    restart_coefficients(kcur, nb, ccur, prev_valid ? cprev : nullptr, c2cur, Q.q[0], Q.q[1]);
+    hipLaunchKernelGGL(restart_kernel, dim3(red_blocks(N)), dim3(256), 0, st, N, kcur, nb, Q, V,
+                       AV, ld, max_m);
+    MHIP_HIP_CHECK(hipGetLastError());
+    i0 = nb;
   }
   MHIP_HIP_CHECK(hipStreamSynchronize(st));
   t_all.stop();

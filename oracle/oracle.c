@@ -693,6 +693,37 @@ static void gram_schmidt(int64_t N, int64_t K, const double* V, int64_t LDV,
   for(int64_t n = 0; n < N; ++n) w[n] *= sc;
 }
 
+/* Coefficients of the restart basis of the restarted Davidson variant
+ * (k x nb, column-major): q0 = c/|c| and, for nb == 2, q1 = the previous
+ * Ritz coefficients (c_prev, 0) orthogonalised against q0 by two classical
+ * Gram-Schmidt passes and normalised, or c2 (the second Ritz vector's
+ * coefficients) treated the same way when c_prev is absent or its
+ * component orthogonal to q0 has norm <= 1e-10. */
+static void restart_coefficients(int64_t k, int64_t nb, const double* c, const double* cprev,
+                                 const double* c2, double* Q) {
+  double cn = 0.;
+  for(int64_t j = 0; j < k; ++j) cn += c[j] * c[j];
+  cn = 1. / sqrt(cn);
+  for(int64_t j = 0; j < k; ++j) Q[j] = c[j] * cn;
+  if(nb < 2) return;
+  double* q1 = Q + k;
+  for(int attempt = cprev ? 0 : 1; attempt < 2; ++attempt) {
+    for(int64_t j = 0; j < k; ++j) q1[j] = attempt ? c2[j] : (j < k - 1 ? cprev[j] : 0.);
+    for(int pass = 0; pass < 2; ++pass) {
+      double s = 0.;
+      for(int64_t j = 0; j < k; ++j) s += Q[j] * q1[j];
+      for(int64_t j = 0; j < k; ++j) q1[j] -= s * Q[j];
+    }
+    double pn = 0.;
+    for(int64_t j = 0; j < k; ++j) pn += q1[j] * q1[j];
+    pn = sqrt(pn);
+    if(pn > 1e-10) {
+      for(int64_t j = 0; j < k; ++j) q1[j] /= pn;
+      return;
+    }
+  }
+}
+
 int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
                            const int64_t* rp, const int32_t* ci,
                            const double* nz, const double* D, double tol,
@@ -702,18 +733,23 @@ int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
    * reference algorithm exactly: no restart, error when the subspace of
    * max_m vectors is exhausted. With max_restarts > 0 (a labelled variant,
    * not in the reference) an exhausted subspace restarts from
-   * [X, t], X the current Ritz vector and t the preconditioned, already
-   * orthogonalised residual that would have been the next basis vector. */
+   * [X, Xp, t]: X the current Ritz vector, Xp the previous iteration's Ritz
+   * vector orthogonalised against X (see restart_coefficients; dropped when
+   * max_m < 3) and t the preconditioned, orthogonalised residual that would
+   * have been the next basis vector. A*X and A*Xp are the same combinations
+   * of the stored A*V columns (no product with A). */
   if(!X) return ORC_ERR_NO_GUESS;
   if(max_m > N) max_m = N;
   if(max_m > 62) return ORC_ERR_ARG;
   const int64_t ld = max_m + 1;
   double* V = (double*)calloc((size_t)N * ld, sizeof(double));
   double* AV = (double*)calloc((size_t)N * ld, sizeof(double));
-  double C[64 * 64], LAM[64];
-  if(!V || !AV) {
+  double* tmp = (double*)calloc((size_t)N * 4, sizeof(double));  /* restart staging */
+  double C[64 * 64], LAM[64], ccur[64], cprev[64], c2cur[64];
+  if(!V || !AV || !tmp) {
     free(V);
     free(AV);
+    free(tmp);
     return ORC_ERR_NOMEM;
   }
   memcpy(V, X, sizeof(double) * N);
@@ -721,19 +757,27 @@ int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
   memcpy(V + N, AV, sizeof(double) * N);                   /* :156 */
   gram_schmidt(N, 1, V, N, V + N);                          /* :157 */
   int converged = 0;
-  int64_t iter = 1, restarts = 0;
+  int64_t iter = 1, restarts = 0, i0 = 1, kcur = 0;
+  int prev_valid = 0;
   for(;;) {
-    for(int64_t i = 1; i < max_m; ++i, ++iter) {
+    for(int64_t i = i0; i < max_m; ++i, ++iter) {
       const int64_t k = i + 1;
       orc_gespmbv(1, 1., N, rp, ci, nz, 0, V + i * N, N, 0., AV + i * N, N);
       /* rayleigh_ritz (rayleigh_ritz.hpp:70-77): C = V^T AV, syev(Lower) */
       for(int64_t b = 0; b < k; ++b)
         for(int64_t a = 0; a < k; ++a) {
-          double s = 0.;
-          for(int64_t n = 0; n < N; ++n) s += V[n + a * N] * AV[n + b * N];
-          C[a + k * b] = s;
+          double sum = 0.;
+          for(int64_t n = 0; n < N; ++n) sum += V[n + a * N] * AV[n + b * N];
+          C[a + k * b] = sum;
         }
       orc_syev_lower(k, C, k, LAM);
+      memcpy(cprev, ccur, sizeof(double) * 64);
+      prev_valid = i > i0; /* c_prev holds coefficients in this basis */
+      for(int64_t j = 0; j < k; ++j) {
+        ccur[j] = C[j];
+        c2cur[j] = k > 1 ? C[j + k] : 0.;
+      }
+      kcur = k;
       double* R = V + (i + 1) * N;
       /* X = V*C(:,0); R = AV*C(:,0) - LAM0*X; ||R||   (:181-193) */
       double rn = 0.;
@@ -753,18 +797,38 @@ int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
       for(int64_t j = 0; j < N; ++j) R[j] = -R[j] / (D[j] - LAM[0]); /* :212 */
       gram_schmidt(N, k, V, N, R);                                    /* :217 */
     }
-    if(converged || restarts >= max_restarts) break;
-    /* restart: V0 = X / ||X||, V1 = V(:,max_m), AV0 = A*V0 */
+    if(converged || restarts >= max_restarts || max_m < 2) break;
     ++restarts;
-    double xn = 0.;
-    for(int64_t n = 0; n < N; ++n) xn += X[n] * X[n];
-    xn = 1. / sqrt(xn);
-    for(int64_t n = 0; n < N; ++n) V[n] = X[n] * xn;
-    memmove(V + N, V + max_m * N, sizeof(double) * N);
-    orc_gespmbv(1, 1., N, rp, ci, nz, 0, V, N, 0., AV, N);
+    /* restart basis V*Q, Q = [q0, q1] orthonormal in the coefficient space:
+     * q0 = c, q1 = (c_prev, 0) made orthogonal to q0 (two Gram-Schmidt
+     * passes); the second Ritz vector's coefficients c2 stand in for c_prev
+     * when the last two iterations did not share this basis or c_prev is
+     * numerically parallel to c. One column only when max_m < 3. */
+    double Q[2 * 64];
+    const int64_t nb = max_m >= 3 ? 2 : 1;
+    restart_coefficients(kcur, nb, ccur, prev_valid ? cprev : NULL, c2cur, Q);
+    for(int64_t n = 0; n < N; ++n) {
+      double v[2] = {0., 0.}, av[2] = {0., 0.};
+      for(int64_t c = 0; c < nb; ++c)
+        for(int64_t j = 0; j < kcur; ++j) {
+          v[c] += V[n + j * N] * Q[j + c * kcur];
+          av[c] += AV[n + j * N] * Q[j + c * kcur];
+        }
+      tmp[n] = v[0];
+      tmp[n + N] = v[1];
+      tmp[n + 2 * N] = av[0];
+      tmp[n + 3 * N] = av[1];
+    }
+    for(int64_t c = 0; c < nb; ++c) {
+      memcpy(V + c * N, tmp + c * N, sizeof(double) * N);
+      memcpy(AV + c * N, tmp + (2 + c) * N, sizeof(double) * N);
+    }
+    memmove(V + nb * N, V + max_m * N, sizeof(double) * N);
+    i0 = nb;
   }
   free(V);
   free(AV);
+  free(tmp);
   *E = LAM[0];
   if(iters) *iters = iter;
   if(restarts_out) *restarts_out = restarts;

@@ -185,6 +185,30 @@ def test_fci_davidson_vs_dense(n, ne):
     assert E == pytest.approx(E_dense, abs=1e-10)
 
 
+@pytest.mark.parametrize("max_m", [2, 3, 4, 6])
+def test_restarted_davidson_small_subspace_reaches_ground_state(max_m):
+    """The restarted variant with tiny subspaces (one-column restart at
+    max_m 2, previous-Ritz-vector restart with the second-Ritz-vector
+    fallback at 3, regular cycles at 4 and 6) converges to the dense ground
+    state, and its vector is an eigenvector (residual from the definition)."""
+    n = 6
+    T, V = seeded_integrals(n, 3)
+    dets = D.generate_hilbert_space(1, n, 3, 3)
+    rp, ci, nz = O.Ham(T, V).csr(1, dets, thresh=1e-16)
+    N = len(dets)
+    dense = np.zeros((N, N))
+    for i in range(N):
+        dense[i, ci[rp[i]:rp[i + 1]]] = nz[rp[i]:rp[i + 1]]
+    w = np.linalg.eigvalsh(dense)
+    Dg = np.diag(dense).copy()
+    X0 = np.zeros(N)
+    X0[np.argmin(Dg)] = 1.0
+    it, E, X = O.davidson(N, max_m, rp, ci, nz, Dg, 1e-8, X0, max_restarts=20000)
+    assert E == pytest.approx(w[0], abs=1e-9)
+    assert np.linalg.norm(X) == pytest.approx(1.0, abs=1e-10)
+    assert np.linalg.norm(dense @ X - E * X) < 1e-8
+
+
 def test_synthetic_integrals_symmetry():
     n = 6
     T, V = seeded_integrals(n, 1)
