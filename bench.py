@@ -28,17 +28,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
-# HBM bytes per spmv_kernel dispatch on this workload from two rocprofv3
-# --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB
-# device copy), summarised in profiles/cas12_fci_build_davidson.md.
-SPMV_TRAFFIC_GB = 21.965
+# HBM bytes per spmv_sell_kernel dispatch on this workload from two rocprofv3
+# --pmc runs (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
+# 1 GiB device copy), summarised in profiles/cas12_fci_build_davidson.md.
+SPMV_TRAFFIC_GB = None
 N_ORB, N_ALPHA, N_BETA, SEED = 12, 6, 6, 12
 
 
-def spmv_bytes(nnz, n):
-    """Algorithmic bytes of one CSR SpMV: values + int32 columns + int64 row
-    pointers + x read + y write (y is read when beta != 0; Davidson passes 0)."""
-    return nnz * (8 + 4) + (n + 1) * 8 + 2 * n * 8
+def spmv_bytes(nnz, n, sliced):
+    """Algorithmic bytes of one SpMV: values + int32 columns + the row
+    structure (int64 row pointers of the CSR, or int64 offset + int32 length
+    per 64-row slice of the sliced copy; padding entries not counted) + x read
+    + y write (y is read when beta != 0; Davidson passes 0)."""
+    rows = 12 * ((n + 63) // 64) if sliced else (n + 1) * 8
+    return nnz * (8 + 4) + rows + 2 * n * 8
 
 
 def cpu_baseline(T, V, dets, budget_s=12.0):
@@ -245,7 +248,8 @@ def main():
     spmv_calls = sum(r[2]["dav_spmv_calls"] for r in recs)
     dav_ms = sum(r[2]["dav_total_ms"] for r in recs)
     avg_spmv_ms = spmv_ms / max(spmv_calls, 1)
-    bytes_spmv = spmv_bytes(nnz, N)
+    sliced = all(r[2]["dav_sliced"] for r in recs)
+    bytes_spmv = spmv_bytes(nnz, N, sliced)
     spmv_gbs = bytes_spmv / (avg_spmv_ms * 1e-3) / 1e9
     value = nnz * args.steps / (build_ms * 1e-3)
     last = recs[-1][2]
@@ -292,17 +296,19 @@ def main():
             "ms_per_solve": dav_ms / args.steps,
             "iterations": last["dav_iters"],
             "restarts": last["dav_restarts"],
+            "sliced_copy_ms": last["dav_layout_ms"],
             "residual": last["dav_residual"],
             "E0": E,
         },
         "roofline": {
-            "kernel": "spmv_kernel (FP64 CSR SpMV, wavefront per row)",
+            "kernel": ("spmv_sell_kernel (FP64 SpMV on the 64-row sliced copy of the CSR "
+                       "Hamiltonian)" if sliced else "spmv_kernel (FP64 CSR SpMV)"),
             "bound": "hbm",
             "achieved": spmv_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": spmv_gbs / HBM_PEAK_GBS,
-            "traffic": SPMV_TRAFFIC_GB * 1e9,
+            "traffic": SPMV_TRAFFIC_GB * 1e9 if SPMV_TRAFFIC_GB and sliced else None,
         },
     }
     if not args.no_asci:

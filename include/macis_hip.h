@@ -91,6 +91,8 @@ typedef struct mhip_timings {
   double asci_nn_ms;      /* learned pre-screen (prediction + selection) */
   int64_t asci_nsurvivors; /* candidates entering the exact top-k */
   int64_t ham_path;       /* structure pass of the last build: MHIP_BUILD_* */
+  double dav_layout_ms;   /* sliced copy made for the last Davidson's SpMVs */
+  int64_t dav_sliced;     /* 1 when those SpMVs ran on the sliced copy */
 } mhip_timings;
 
 /* Structure passes of the Hamiltonian build. AUTO takes the product-space
@@ -199,6 +201,14 @@ int mhip_csr_destroy(mhip_csr* csr);
 int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
               const double* d_V, int64_t LDV, double beta, double* d_AV,
               int64_t LDAV);
+/* Attaches a sliced (SELL-64) copy of a square matrix to the handle, used
+ * from then on by every single-vector product with it (mhip_spmv columns,
+ * Davidson). No counterpart in the reference (sparsexx multiplies its CSR
+ * directly); for callers that apply one matrix many times. The copy is made
+ * only when the device keeps 1 GiB free beside it (12 bytes per stored
+ * entry); *sliced (may be NULL) says whether the handle has one. Products
+ * equal the CSR ones up to the order of the row sums. */
+int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int* sliced);
 /* extract_diagonal_elements (submatrix.hpp:264-295) into DEVICE d_D. */
 int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
 

@@ -76,6 +76,7 @@ DevBuf take_buffer(mhip_context* ctx, size_t bytes) {
     b.alloc(bytes);
   } catch(const Error& e) {
     if(e.code != MHIP_ERR_OUT_OF_MEMORY || ctx->spare.empty()) throw;
+    (void)hipGetLastError();  // the failed hipMalloc is not a pending error
     MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->spare.clear();
     b.alloc(bytes);

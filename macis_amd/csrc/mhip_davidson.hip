@@ -12,6 +12,7 @@
 // Reductions are written as per-block partials and summed in a fixed order,
 // so results are run-to-run deterministic.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -467,6 +468,8 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   tm.dav_iters = 0;
   tm.dav_restarts = 0;
   tm.dav_residual = 0.;
+  tm.dav_layout_ms = 0.;
+  tm.dav_sliced = 0;
   if(N == 0) {
     *E = 0.;
     if(iters) *iters = 0;
@@ -491,6 +494,16 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
     Dbuf.alloc(sizeof(double) * N);
     extract_diagonal(ctx, A, Dbuf.as<double>());
     d_D = Dbuf.as<double>();
+  }
+  // sliced copy of A for the SpMVs when there is room for it (made once per
+  // matrix handle; the CSR kernel runs otherwise)
+  {
+    const bool had = (bool)A->sell;
+    const auto h0 = std::chrono::steady_clock::now();
+    tm.dav_sliced = prepare_sell(ctx, A) ? 1 : 0;
+    if(!had && tm.dav_sliced) MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    tm.dav_layout_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
   }
   auto op = [&](const double* x, double* y) {
     t_op.start();
@@ -649,6 +662,10 @@ int mhip_selected_ci_diag(mhip_context* ctx, int words_per_det,
       keep_buffer(ctx, std::move(H->rowptr));
       keep_buffer(ctx, std::move(H->colind));
       keep_buffer(ctx, std::move(H->nzval));
+      if(H->sell) {
+        keep_buffer(ctx, std::move(H->sell->cols));
+        keep_buffer(ctx, std::move(H->sell->vals));
+      }
     }
     if(!ok) throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");
   });

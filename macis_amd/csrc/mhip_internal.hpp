@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -148,6 +149,20 @@ struct Timer {
 
 }  // namespace mhip
 
+namespace mhip {
+// Sliced (SELL-64) copy of a square CSR matrix for the single-vector SpMV:
+// slice s holds rows [64 s, 64 s + 64), its entries stored slot-major
+// (entry j of row 64 s + l at soff[s] + 64 j + l), every row padded to the
+// slice's longest row with (column = row, value = 0); columns 0-based.
+struct SellCopy {
+  int64_t nsl = 0, entries = 0;
+  DevBuf meta;        // int64 soff[nsl], then int32 slen[nsl]
+  DevBuf cols, vals;  // int32 / double [entries]
+  const int64_t* soff() const { return meta.as<int64_t>(); }
+  const int32_t* slen() const { return reinterpret_cast<const int32_t*>(soff() + nsl); }
+};
+}  // namespace mhip
+
 struct mhip_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -171,6 +186,8 @@ struct mhip_csr {
   mhip::DevBuf rowptr;  // int64[m+1]
   mhip::DevBuf colind;  // int32[nnz]
   mhip::DevBuf nzval;   // double[nnz]
+  // optional sliced copy used by single-vector SpMVs (prepare_sell)
+  mutable std::unique_ptr<mhip::SellCopy> sell;
 };
 
 namespace mhip {
@@ -196,6 +213,9 @@ void ham_diagonal(mhip_context* ctx, int W, const uint64_t* d_dets,
 void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
           double beta, double* y);
 void extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
+// Attaches the sliced copy to a square matrix when the device has room for
+// it (true if the matrix has one afterwards); spmv() then uses it.
+bool prepare_sell(mhip_context* ctx, const mhip_csr* A);
 // Davidson core; returns convergence flag.
 bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
               int64_t max_restarts, double tol, const double* d_D, double* d_X,

@@ -2,12 +2,16 @@
 // (extract_diagonal_elements, submatrix.hpp:264-295) and the CSR handle
 // entry points.
 //
-// SpMV layout: four rows per wavefront, 16 lanes per row streaming 16-byte
-// quads of column indices and values with non-temporal loads (read once,
-// kept out of L2 so the gathered x vector stays cached); x is gathered
-// through L1/L2 and each row sum is reduced across its 16 lanes. Bound: HBM
-// for the streamed 12 bytes per nonzero, with the x gathers' L2 requests as
-// the second limiter.
+// Two layouts. CSR (the handle's own arrays): four rows per wavefront, 16
+// lanes per row streaming 16-byte quads of column indices and values with
+// non-temporal loads (read once, kept out of L2 so the gathered x vector
+// stays cached); x is gathered through L1/L2 and each row sum is reduced
+// across its 16 lanes. Sliced copy (SELL-64, made once per matrix by
+// prepare_sell when the device has room, then used by every single-vector
+// SpMV of that matrix): one 64-row slice per wavefront, lane = row, slots
+// streamed slot-major. Both are HBM bound on the streamed 12 bytes per
+// nonzero; the x gathers' L1/L2 requests are the second limiter, far less
+// so in the sliced layout.
 #include <algorithm>
 #include <limits>
 #include <memory>
@@ -117,11 +121,190 @@ __global__ __launch_bounds__(256) void extract_diag_kernel(
   if(lane == 0) D[row] = d;
 }
 
+
+// ---- sliced (SELL-64) copy ----------------------------------------------
+// One wavefront owns one 64-row slice and lane l one row: every load of a
+// slot is one contiguous 256-byte (columns) / 512-byte (values) access,
+// and the lanes' gathers at one slot fall in one x segment for product-space
+// matrices (rows of one outer string share their outer neighbours slot by
+// slot), so x lines are reused across the wavefront instead of across
+// loop steps.
+constexpr int kSliceRows = 64;
+constexpr int kSellChunk = 32;  // slots staged through LDS per round of the copy
+constexpr int kSellUnroll = 8;  // slots in flight per lane
+
+// slen[s] = longest row of slice s (one wavefront per slice).
+__global__ __launch_bounds__(256) void sell_lengths_kernel(int64_t m, int64_t nsl,
+                                                           const int64_t* __restrict__ rowptr,
+                                                           int32_t* __restrict__ slen) {
+  const int64_t s = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
+  if(s >= nsl) return;
+  const int64_t row = s * kSliceRows + (threadIdx.x & 63);
+  int len = row < m ? (int)(rowptr[row + 1] - rowptr[row]) : 0;
+#pragma unroll
+  for(int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(len, off, 64);
+    len = o > len ? o : len;
+  }
+  if((threadIdx.x & 63) == 0) slen[s] = len;
+}
+
+// soff = exclusive scan of 64 * slen, total entries in *total (one block).
+__global__ __launch_bounds__(1024) void sell_offsets_kernel(int64_t nsl,
+                                                            const int32_t* __restrict__ slen,
+                                                            int64_t* __restrict__ soff,
+                                                            int64_t* __restrict__ total) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t chunk = (nsl + 1023) / 1024;
+  const int64_t lo = std::min<int64_t>(nsl, t * chunk), hi = std::min<int64_t>(nsl, lo + chunk);
+  int64_t sum = 0;
+  for(int64_t i = lo; i < hi; ++i) sum += (int64_t)slen[i] * kSliceRows;
+  part[t] = sum;
+  __syncthreads();
+  for(int off = 1; off < 1024; off <<= 1) {
+    const int64_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = part[t] - sum;
+  for(int64_t i = lo; i < hi; ++i) {
+    soff[i] = run;
+    run += (int64_t)slen[i] * kSliceRows;
+  }
+  if(t == 1023) *total = part[1023];
+}
+
+// Copy of slice blockIdx.x, kSellChunk slots at a time: rows are read
+// contiguously into an LDS tile and written back slot-major.
+__global__ __launch_bounds__(256) void sell_fill_kernel(
+    int64_t m, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const double* __restrict__ nzval, int64_t indexing, const int64_t* __restrict__ soff,
+    const int32_t* __restrict__ slen, int32_t* __restrict__ cols, double* __restrict__ vals) {
+  __shared__ int64_t rst[kSliceRows];
+  __shared__ int32_t rlen[kSliceRows];
+  __shared__ int32_t tc[kSellChunk][kSliceRows + 1];
+  __shared__ double tv[kSellChunk][kSliceRows + 1];
+  const int64_t s = blockIdx.x;
+  const int t = threadIdx.x;
+  if(t < kSliceRows) {
+    const int64_t row = s * kSliceRows + t;
+    rst[t] = row < m ? rowptr[row] - indexing : 0;
+    rlen[t] = row < m ? (int32_t)(rowptr[row + 1] - rowptr[row]) : 0;
+  }
+  __syncthreads();
+  const int L = slen[s];
+  const int64_t base = soff[s];
+  for(int j0 = 0; j0 < L; j0 += kSellChunk) {
+    for(int e = t; e < kSliceRows * kSellChunk; e += 256) {
+      const int r = e / kSellChunk, jj = e % kSellChunk, j = j0 + jj;
+      const int64_t row = s * kSliceRows + r;
+      int32_t c = row < m ? (int32_t)row : 0;
+      double v = 0.;
+      if(j < rlen[r]) {
+        c = (int32_t)(colind[rst[r] + j] - indexing);
+        v = nzval[rst[r] + j];
+      }
+      tc[jj][r] = c;
+      tv[jj][r] = v;
+    }
+    __syncthreads();
+    for(int e = t; e < kSliceRows * kSellChunk; e += 256) {
+      const int jj = e / kSliceRows, r = e % kSliceRows, j = j0 + jj;
+      if(j < L) {
+        cols[base + (int64_t)j * kSliceRows + r] = tc[jj][r];
+        vals[base + (int64_t)j * kSliceRows + r] = tv[jj][r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// y = alpha A x + beta y on the sliced copy: one slice per wavefront, lane =
+// row, kSellUnroll slots per step with non-temporal column/value loads;
+// blocks mapped XCD-aware as in spmv_kernel.
+__global__ __launch_bounds__(256) void spmv_sell_kernel(
+    int64_t m, int64_t nsl, int64_t per_xcd, const int64_t* __restrict__ soff,
+    const int32_t* __restrict__ slen, const int32_t* __restrict__ cols,
+    const double* __restrict__ vals, double alpha, const double* __restrict__ x, double beta,
+    double* __restrict__ y) {
+  const int64_t lblk = (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int64_t s = lblk * kWavesPerBlock + threadIdx.x / 64;
+  if(s >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const int len = slen[s];
+  const int32_t* c = cols + soff[s] + lane;
+  const double* v = vals + soff[s] + lane;
+  double a[kSellUnroll];
+#pragma unroll
+  for(int u = 0; u < kSellUnroll; ++u) a[u] = 0.;
+  int j = 0;
+  for(; j + kSellUnroll <= len; j += kSellUnroll) {
+    int32_t cc[kSellUnroll];
+    double vv[kSellUnroll];
+#pragma unroll
+    for(int u = 0; u < kSellUnroll; ++u) {
+      cc[u] = __builtin_nontemporal_load(c + (int64_t)(j + u) * kSliceRows);
+      vv[u] = __builtin_nontemporal_load(v + (int64_t)(j + u) * kSliceRows);
+    }
+#pragma unroll
+    for(int u = 0; u < kSellUnroll; ++u) a[u] = fma(vv[u], x[cc[u]], a[u]);
+  }
+  for(; j < len; ++j) a[0] = fma(v[(int64_t)j * kSliceRows], x[c[(int64_t)j * kSliceRows]], a[0]);
+  const double acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  const int64_t row = s * kSliceRows + lane;
+  if(row < m) y[row] = alpha * acc + beta * y[row];
+}
+
 }  // namespace
+
+bool prepare_sell(mhip_context* ctx, const mhip_csr* A) {
+  if(A->sell) return true;
+  if(A->m == 0 || A->m != A->n || A->m > INT32_MAX) return false;
+  hipStream_t st = ctx->stream;
+  auto S = std::make_unique<SellCopy>();
+  S->nsl = (A->m + kSliceRows - 1) / kSliceRows;
+  S->meta.alloc(12 * S->nsl + 16);  // soff, slen (padded to 8 bytes), total
+  int64_t* soff = S->meta.as<int64_t>();
+  int32_t* slen = reinterpret_cast<int32_t*>(soff + S->nsl);
+  int64_t* d_total = reinterpret_cast<int64_t*>(slen + S->nsl + (S->nsl & 1));
+  hipLaunchKernelGGL(sell_lengths_kernel, dim3((S->nsl + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, st, A->m, S->nsl, A->rowptr.as<int64_t>(), slen);
+  hipLaunchKernelGGL(sell_offsets_kernel, dim3(1), dim3(1024), 0, st, S->nsl, slen, soff,
+                     d_total);
+  MHIP_HIP_CHECK(hipGetLastError());
+  d2h(ctx, &S->entries, d_total, 8);
+  // room for the copy (12 bytes per stored entry) plus 1 GiB for the caller
+  const size_t need = 12 * (size_t)S->entries + (size_t(1) << 30);
+  size_t free_b = 0, total_b = 0;
+  MHIP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  size_t spare_b = 0;
+  for(auto& b : ctx->spare) spare_b += b.bytes;
+  if(free_b + spare_b < need) return false;
+  S->cols = take_buffer(ctx, 4 * (size_t)S->entries);
+  S->vals = take_buffer(ctx, 8 * (size_t)S->entries);
+  hipLaunchKernelGGL(sell_fill_kernel, dim3(S->nsl), dim3(256), 0, st, A->m,
+                     A->rowptr.as<int64_t>(), A->colind.as<int32_t>(), A->nzval.as<double>(),
+                     A->indexing, soff, slen, S->cols.as<int32_t>(), S->vals.as<double>());
+  MHIP_HIP_CHECK(hipGetLastError());
+  A->sell = std::move(S);
+  return true;
+}
 
 void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
           double beta, double* y) {
   if(A->m == 0) return;
+  if(A->sell) {
+    const SellCopy& S = *A->sell;
+    const int64_t nblk = (S.nsl + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t per = (nblk + kXcds - 1) / kXcds;
+    hipLaunchKernelGGL(spmv_sell_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m,
+                       S.nsl, per, S.soff(), S.slen(), S.cols.as<int32_t>(),
+                       S.vals.as<double>(), alpha, x, beta, y);
+    MHIP_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const int64_t nblk = (A->m + kRowsPerBlock - 1) / kRowsPerBlock;
   const int64_t per = (nblk + kXcds - 1) / kXcds;
   hipLaunchKernelGGL(spmv_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m, per,
@@ -278,6 +461,15 @@ int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
     MHIP_REQUIRE(K == 0 || A->m == 0 || (d_V && d_AV), MHIP_ERR_INVALID_ARG,
                  "null vector");
     for(int64_t k = 0; k < K; ++k) spmv(ctx, A, alpha, d_V + k * LDV, beta, d_AV + k * LDAV);
+  });
+}
+
+int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int* sliced) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(A, MHIP_ERR_INVALID_ARG, "null matrix");
+    const bool ok = prepare_sell(ctx, A);
+    if(sliced) *sliced = ok ? 1 : 0;
   });
 }
 

@@ -15,13 +15,16 @@ EPS = np.finfo(float).eps
 
 
 @pytest.mark.parametrize("indexing", [0, 1])
-def test_spmv_known_answer(ctx, indexing):
-    announce(f"spmv 6x6 known answer indexing={indexing}")
+@pytest.mark.parametrize("sliced", [False, True])
+def test_spmv_known_answer(ctx, indexing, sliced):
+    announce(f"spmv 6x6 known answer indexing={indexing} sliced={sliced}")
     i = indexing
     rowptr = np.array([i, i + 2, i + 5, i + 8, i + 11, i + 14, i + 16])
     colind = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4, 5, 4, 5]) + i
     nzval = np.array([2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2], dtype=float)
     A = ctx.csr_from_host(6, 6, rowptr, colind, nzval, indexing=i)
+    if sliced:
+        assert ctx.prepare_spmv(A)
     x = ctx.to_device(np.arange(1, 7, dtype=float))
     y = ctx.to_device(np.ones(6))
     ctx.spmv(A, 1, 1.0, x, 6, 0.0, y, 6)
@@ -31,12 +34,15 @@ def test_spmv_known_answer(ctx, indexing):
     assert np.array_equal(Dg.numpy(), np.full(6, 2.0))
 
 
-def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd):
-    announce("spmv H2O CISD K=2 alpha/beta")
+@pytest.mark.parametrize("sliced", [False, True])
+def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd, sliced):
+    announce(f"spmv H2O CISD K=2 alpha/beta sliced={sliced}")
     norb, _, T, V = h2o
     ctx.set_integrals(norb, T, V)
     N = len(h2o_cisd)
     H = ctx.ham_build(1, ctx.to_device(h2o_cisd), N, thresh=1e-16)
+    if sliced:
+        assert ctx.prepare_spmv(H)
     rp, ci, nz = h2o_oracle.csr(1, h2o_cisd, thresh=1e-16)
     rng = np.random.default_rng(1)
     LD = N + 3
@@ -55,12 +61,14 @@ def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd):
     assert np.array_equal(Dg.numpy(), O.extract_diagonal(N, rp, ci, nz))
 
 
+@pytest.mark.parametrize("sliced", [False, True])
 @pytest.mark.parametrize("indexing", [0, 1])
-def test_spmv_ragged_rows(ctx, indexing):
+def test_spmv_ragged_rows(ctx, indexing, sliced):
     """Rows of every length class around the kernel's 16-byte quads (0-9
-    nonzeros at all start offsets mod 4, long rows, an empty last row) against
-    the oracle's gespmbv (spmbv.hpp:37-73)."""
-    announce(f"spmv ragged rows indexing={indexing}")
+    nonzeros at all start offsets mod 4, long rows, an empty last row; in
+    the sliced layout slices mixing empty, short and 4096-long rows and a
+    partial last slice) against the oracle's gespmbv (spmbv.hpp:37-73)."""
+    announce(f"spmv ragged rows indexing={indexing} sliced={sliced}")
     rng = np.random.default_rng(7 + indexing)
     lens = np.concatenate([np.tile(np.arange(10), 8), rng.integers(0, 700, 300),
                            [1819, 4096, 3, 0]])
@@ -72,6 +80,8 @@ def test_spmv_ragged_rows(ctx, indexing):
                              for L in lens]).astype(np.int32)
     nzval = rng.standard_normal(nnz)
     A = ctx.csr_from_host(m, n, rowptr + indexing, colind + indexing, nzval, indexing=indexing)
+    if sliced:
+        assert ctx.prepare_spmv(A)
     x = rng.standard_normal(n)
     y0 = rng.standard_normal(m)
     dx, dy = ctx.to_device(x), ctx.to_device(y0)
@@ -81,6 +91,30 @@ def test_spmv_ragged_rows(ctx, indexing):
     got = dy.numpy()
     tol = 1e-14 * (np.abs(nzval).max() * np.abs(x).max() * lens.max() + 1)
     assert np.max(np.abs(got - ref)) <= tol
+
+
+def test_spmv_sliced_copy_rules(ctx):
+    """The sliced copy is made for square matrices only; once made, a second
+    prepare keeps it, and products with the same matrix are bit-identical
+    call to call."""
+    announce("spmv sliced copy rules")
+    rng = np.random.default_rng(3)
+    rowptr = np.array([0, 2, 3], dtype=np.int64)
+    colind = np.array([0, 2, 1], dtype=np.int32)
+    R = ctx.csr_from_host(2, 3, rowptr, colind, np.array([1.0, 2.0, 3.0]))
+    assert not ctx.prepare_spmv(R)
+    m = 1000
+    lens = rng.integers(0, 40, m)
+    rp = np.zeros(m + 1, dtype=np.int64)
+    rp[1:] = np.cumsum(lens)
+    ci = np.concatenate([np.sort(rng.choice(m, size=L, replace=False)) for L in lens]).astype(np.int32)
+    A = ctx.csr_from_host(m, m, rp, ci, rng.standard_normal(int(rp[-1])))
+    assert ctx.prepare_spmv(A) and ctx.prepare_spmv(A)
+    dx = ctx.to_device(rng.standard_normal(m))
+    y1, y2 = ctx.empty(m, np.float64), ctx.empty(m, np.float64)
+    ctx.spmv(A, 1, 1.0, dx, m, 0.0, y1, m)
+    ctx.spmv(A, 1, 1.0, dx, m, 0.0, y2, m)
+    assert np.array_equal(y1.numpy(), y2.numpy())
 
 
 @pytest.mark.parametrize("k,m", [(1, 1), (5, 1), (16, 16), (17, 17), (21, 21), (3, 29)])
@@ -160,6 +194,7 @@ def test_selected_ci_diag_restarted_vs_oracle(ctx, n, ne, seed):
     # the restart forms A*[X, Xp] from the stored products: one SpMV per
     # iteration plus the initial one, none at the restarts
     assert t["dav_restarts"] > 0 and t["dav_spmv_calls"] == t["dav_iters"] + 1
+    assert t["dav_sliced"] == 1  # SpMVs on the sliced copy of the Hamiltonian
 
 
 @pytest.mark.parametrize("max_m", [2, 3, 4, 6])
