@@ -31,7 +31,7 @@ BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
 # HBM bytes per spmv_sell_kernel dispatch on this workload from two rocprofv3
 # --pmc runs (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
 # 1 GiB device copy), summarised in profiles/cas12_fci_build_davidson.md.
-SPMV_TRAFFIC_GB = None
+SPMV_TRAFFIC_GB = 19.079
 N_ORB, N_ALPHA, N_BETA, SEED = 12, 6, 6, 12
 
 

@@ -201,14 +201,19 @@ int mhip_csr_destroy(mhip_csr* csr);
 int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
               const double* d_V, int64_t LDV, double beta, double* d_AV,
               int64_t LDAV);
-/* Attaches a sliced (SELL-64) copy of a square matrix to the handle, used
- * from then on by every single-vector product with it (mhip_spmv columns,
- * Davidson). No counterpart in the reference (sparsexx multiplies its CSR
- * directly); for callers that apply one matrix many times. The copy is made
- * only when the device keeps 1 GiB free beside it (12 bytes per stored
- * entry); *sliced (may be NULL) says whether the handle has one. Products
+/* Attaches a sliced copy (SELL-64: 64-row slices stored slot-major, rows
+ * padded to their slice's longest) of a square matrix to the handle, used
+ * from then on by every single-vector product with it
+ * (mhip_spmv columns, Davidson). No counterpart in the reference (sparsexx
+ * multiplies its CSR directly); for callers that apply one matrix many
+ * times. The copy needs 12 bytes per stored entry and is made only when the
+ * device keeps 1 GiB free beside it; with MHIP_SPMV_SLICED_AUTO (what
+ * Davidson uses) also only when padding adds at most a quarter of the
+ * nonzeros. *sliced (may be NULL) says whether the handle has one. Products
  * equal the CSR ones up to the order of the row sums. */
-int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int* sliced);
+#define MHIP_SPMV_SLICED_AUTO 0
+#define MHIP_SPMV_SLICED_ALWAYS 1
+int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int policy, int* sliced);
 /* extract_diagonal_elements (submatrix.hpp:264-295) into DEVICE d_D. */
 int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
 

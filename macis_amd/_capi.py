@@ -110,7 +110,7 @@ _SIGS = {
     "mhip_csr_destroy": (_i32, [_p]),
     "mhip_spmv": (_i32, [_p, _p, _i64, _d, _p, _i64, _d, _p, _i64]),
     "mhip_extract_diagonal": (_i32, [_p, _p, _p]),
-    "mhip_csr_prepare_spmv": (_i32, [_p, _p, _p]),
+    "mhip_csr_prepare_spmv": (_i32, [_p, _p, _i32, _p]),
     "mhip_gemm_tn": (_i32, [_p, _i64, _i32, _p, _i64, _i32, _p, _i64, _p, _i32]),
     "mhip_davidson": (_i32, [_p, _p, _i64, _i64, _d, _p, _p, _pd, _pi64]),
     "mhip_selected_ci_diag": (_i32, [_p, _i32, _p, _i64, _d, _i64, _i64, _d, _p, _pd, _pi64, _pp]),
@@ -339,10 +339,12 @@ class Context:
     def extract_diagonal(self, A, d_D):
         check(lib().mhip_extract_diagonal(self.h, A.h, _ptr(d_D)), "extract_diagonal")
 
-    def prepare_spmv(self, A):
-        """Attach the sliced SpMV copy to A; True if A has one afterwards."""
+    def prepare_spmv(self, A, always=False):
+        """Attach the sliced SpMV copy to A (always: whatever the padding);
+        True if A has one afterwards."""
         out = C.c_int(0)
-        check(lib().mhip_csr_prepare_spmv(self.h, A.h, C.byref(out)), "csr_prepare_spmv")
+        check(lib().mhip_csr_prepare_spmv(self.h, A.h, 1 if always else 0, C.byref(out)),
+              "csr_prepare_spmv")
         return bool(out.value)
 
     def gemm_tn(self, N, k, d_A, lda, m, d_B, ldb, d_C, ldc):

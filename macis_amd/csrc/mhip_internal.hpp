@@ -156,10 +156,10 @@ namespace mhip {
 // slice's longest row with (column = row, value = 0); columns 0-based.
 struct SellCopy {
   int64_t nsl = 0, entries = 0;
-  DevBuf meta;        // int64 soff[nsl], then int32 slen[nsl]
+  DevBuf meta;        // int64 soff[nsl], total, then int32 slen[nsl]
   DevBuf cols, vals;  // int32 / double [entries]
   const int64_t* soff() const { return meta.as<int64_t>(); }
-  const int32_t* slen() const { return reinterpret_cast<const int32_t*>(soff() + nsl); }
+  const int32_t* slen() const { return reinterpret_cast<const int32_t*>(soff() + nsl + 1); }
 };
 }  // namespace mhip
 
@@ -214,8 +214,9 @@ void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
           double beta, double* y);
 void extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
 // Attaches the sliced copy to a square matrix when the device has room for
-// it (true if the matrix has one afterwards); spmv() then uses it.
-bool prepare_sell(mhip_context* ctx, const mhip_csr* A);
+// it and (unless any_padding) padding adds at most a quarter of the
+// nonzeros; true if the matrix has one afterwards; spmv() then uses it.
+bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding = false);
 // Davidson core; returns convergence flag.
 bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
               int64_t max_restarts, double tol, const double* d_D, double* d_X,

@@ -124,11 +124,13 @@ __global__ __launch_bounds__(256) void extract_diag_kernel(
 
 // ---- sliced (SELL-64) copy ----------------------------------------------
 // One wavefront owns one 64-row slice and lane l one row: every load of a
-// slot is one contiguous 256-byte (columns) / 512-byte (values) access,
-// and the lanes' gathers at one slot fall in one x segment for product-space
-// matrices (rows of one outer string share their outer neighbours slot by
-// slot), so x lines are reused across the wavefront instead of across
-// loop steps.
+// slot is one contiguous 256-byte (columns) / 512-byte (values) access. For
+// product-space matrices all rows have one length (no padding) and the
+// lanes' gathers at one slot fall in one x segment (rows of one outer string
+// share their outer neighbours slot by slot), so x lines are reused across
+// the wavefront. Selected-CI matrices have ragged rows and scattered
+// columns: there the CSR kernel's rows-along-lanes gathers are faster even
+// without padding, and the padding rule of prepare_sell keeps them on it.
 constexpr int kSliceRows = 64;
 constexpr int kSellChunk = 32;  // slots staged through LDS per round of the copy
 constexpr int kSellUnroll = 8;  // slots in flight per lane
@@ -177,19 +179,22 @@ __global__ __launch_bounds__(1024) void sell_offsets_kernel(int64_t nsl,
 }
 
 // Copy of slice blockIdx.x, kSellChunk slots at a time: rows are read
-// contiguously into an LDS tile and written back slot-major.
+// contiguously into an LDS tile and written back slot-major. Padding
+// entries are (column = the row itself, value 0); past the last row,
+// (0, 0).
 __global__ __launch_bounds__(256) void sell_fill_kernel(
     int64_t m, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const double* __restrict__ nzval, int64_t indexing, const int64_t* __restrict__ soff,
     const int32_t* __restrict__ slen, int32_t* __restrict__ cols, double* __restrict__ vals) {
   __shared__ int64_t rst[kSliceRows];
-  __shared__ int32_t rlen[kSliceRows];
+  __shared__ int32_t rlen[kSliceRows], rid[kSliceRows];
   __shared__ int32_t tc[kSellChunk][kSliceRows + 1];
   __shared__ double tv[kSellChunk][kSliceRows + 1];
   const int64_t s = blockIdx.x;
   const int t = threadIdx.x;
   if(t < kSliceRows) {
     const int64_t row = s * kSliceRows + t;
+    rid[t] = row < m ? (int32_t)row : 0;
     rst[t] = row < m ? rowptr[row] - indexing : 0;
     rlen[t] = row < m ? (int32_t)(rowptr[row + 1] - rowptr[row]) : 0;
   }
@@ -199,8 +204,7 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(
   for(int j0 = 0; j0 < L; j0 += kSellChunk) {
     for(int e = t; e < kSliceRows * kSellChunk; e += 256) {
       const int r = e / kSellChunk, jj = e % kSellChunk, j = j0 + jj;
-      const int64_t row = s * kSliceRows + r;
-      int32_t c = row < m ? (int32_t)row : 0;
+      int32_t c = rid[r];
       double v = 0.;
       if(j < rlen[r]) {
         c = (int32_t)(colind[rst[r] + j] - indexing);
@@ -259,22 +263,25 @@ __global__ __launch_bounds__(256) void spmv_sell_kernel(
 
 }  // namespace
 
-bool prepare_sell(mhip_context* ctx, const mhip_csr* A) {
+bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding) {
   if(A->sell) return true;
   if(A->m == 0 || A->m != A->n || A->m > INT32_MAX) return false;
   hipStream_t st = ctx->stream;
   auto S = std::make_unique<SellCopy>();
   S->nsl = (A->m + kSliceRows - 1) / kSliceRows;
-  S->meta.alloc(12 * S->nsl + 16);  // soff, slen (padded to 8 bytes), total
+  S->meta.alloc(12 * S->nsl + 8);  // soff[nsl] | total | slen[nsl]
   int64_t* soff = S->meta.as<int64_t>();
-  int32_t* slen = reinterpret_cast<int32_t*>(soff + S->nsl);
-  int64_t* d_total = reinterpret_cast<int64_t*>(slen + S->nsl + (S->nsl & 1));
+  int64_t* d_total = soff + S->nsl;
+  int32_t* slen = reinterpret_cast<int32_t*>(soff + S->nsl + 1);
   hipLaunchKernelGGL(sell_lengths_kernel, dim3((S->nsl + kWavesPerBlock - 1) / kWavesPerBlock),
                      dim3(256), 0, st, A->m, S->nsl, A->rowptr.as<int64_t>(), slen);
   hipLaunchKernelGGL(sell_offsets_kernel, dim3(1), dim3(1024), 0, st, S->nsl, slen, soff,
                      d_total);
   MHIP_HIP_CHECK(hipGetLastError());
   d2h(ctx, &S->entries, d_total, 8);
+  // padding beyond a quarter of the nonzeros costs more streamed bytes than
+  // the layout saves
+  if(!any_padding && S->entries > A->nnz + A->nnz / 4) return false;
   // room for the copy (12 bytes per stored entry) plus 1 GiB for the caller
   const size_t need = 12 * (size_t)S->entries + (size_t(1) << 30);
   size_t free_b = 0, total_b = 0;
@@ -464,11 +471,13 @@ int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
   });
 }
 
-int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int* sliced) {
+int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int policy, int* sliced) {
   return guarded([&] {
     use_device(ctx);
     MHIP_REQUIRE(A, MHIP_ERR_INVALID_ARG, "null matrix");
-    const bool ok = prepare_sell(ctx, A);
+    MHIP_REQUIRE(policy == MHIP_SPMV_SLICED_AUTO || policy == MHIP_SPMV_SLICED_ALWAYS,
+                 MHIP_ERR_INVALID_ARG, "unknown sliced-copy policy");
+    const bool ok = prepare_sell(ctx, A, policy == MHIP_SPMV_SLICED_ALWAYS);
     if(sliced) *sliced = ok ? 1 : 0;
   });
 }

@@ -24,7 +24,7 @@ def test_spmv_known_answer(ctx, indexing, sliced):
     nzval = np.array([2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2], dtype=float)
     A = ctx.csr_from_host(6, 6, rowptr, colind, nzval, indexing=i)
     if sliced:
-        assert ctx.prepare_spmv(A)
+        assert ctx.prepare_spmv(A, always=True)
     x = ctx.to_device(np.arange(1, 7, dtype=float))
     y = ctx.to_device(np.ones(6))
     ctx.spmv(A, 1, 1.0, x, 6, 0.0, y, 6)
@@ -42,7 +42,7 @@ def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd, sliced):
     N = len(h2o_cisd)
     H = ctx.ham_build(1, ctx.to_device(h2o_cisd), N, thresh=1e-16)
     if sliced:
-        assert ctx.prepare_spmv(H)
+        assert ctx.prepare_spmv(H, always=True)
     rp, ci, nz = h2o_oracle.csr(1, h2o_cisd, thresh=1e-16)
     rng = np.random.default_rng(1)
     LD = N + 3
@@ -80,8 +80,8 @@ def test_spmv_ragged_rows(ctx, indexing, sliced):
                              for L in lens]).astype(np.int32)
     nzval = rng.standard_normal(nnz)
     A = ctx.csr_from_host(m, n, rowptr + indexing, colind + indexing, nzval, indexing=indexing)
-    if sliced:
-        assert ctx.prepare_spmv(A)
+    if sliced:  # rows of mixed lengths padded within their slices
+        assert ctx.prepare_spmv(A, always=True)
     x = rng.standard_normal(n)
     y0 = rng.standard_normal(m)
     dx, dy = ctx.to_device(x), ctx.to_device(y0)
@@ -94,27 +94,46 @@ def test_spmv_ragged_rows(ctx, indexing, sliced):
 
 
 def test_spmv_sliced_copy_rules(ctx):
-    """The sliced copy is made for square matrices only; once made, a second
-    prepare keeps it, and products with the same matrix are bit-identical
+    """The sliced copy is made for square matrices only; the automatic policy
+    refuses one whose padding would exceed a quarter of the nonzeros (one
+    long row among short ones) and accepts equal rows; the forced copy of
+    the padded matrix still multiplies correctly; products are bit-identical
     call to call."""
     announce("spmv sliced copy rules")
     rng = np.random.default_rng(3)
     rowptr = np.array([0, 2, 3], dtype=np.int64)
     colind = np.array([0, 2, 1], dtype=np.int32)
     R = ctx.csr_from_host(2, 3, rowptr, colind, np.array([1.0, 2.0, 3.0]))
-    assert not ctx.prepare_spmv(R)
-    m = 1000
-    lens = rng.integers(0, 40, m)
-    rp = np.zeros(m + 1, dtype=np.int64)
-    rp[1:] = np.cumsum(lens)
-    ci = np.concatenate([np.sort(rng.choice(m, size=L, replace=False)) for L in lens]).astype(np.int32)
-    A = ctx.csr_from_host(m, m, rp, ci, rng.standard_normal(int(rp[-1])))
+    assert not ctx.prepare_spmv(R) and not ctx.prepare_spmv(R, always=True)
+
+    def make(lens):
+        m = len(lens)
+        rp = np.zeros(m + 1, dtype=np.int64)
+        rp[1:] = np.cumsum(lens)
+        ci = np.concatenate([np.sort(rng.choice(m, size=L, replace=False)) for L in lens])
+        return m, rp, ci.astype(np.int32), rng.standard_normal(int(rp[-1]))
+
+    m, rp, ci, nz = make(np.full(1000, 10))  # 10,000 nonzeros, 10,240 stored
+    A = ctx.csr_from_host(m, m, rp, ci, nz)
     assert ctx.prepare_spmv(A) and ctx.prepare_spmv(A)
-    dx = ctx.to_device(rng.standard_normal(m))
-    y1, y2 = ctx.empty(m, np.float64), ctx.empty(m, np.float64)
-    ctx.spmv(A, 1, 1.0, dx, m, 0.0, y1, m)
-    ctx.spmv(A, 1, 1.0, dx, m, 0.0, y2, m)
+    lens = np.full(2500, 4)
+    lens[1700] = 2000  # one long row: its slice stores 64 x 2000 entries
+    m, rp, ci, nz = make(lens)
+    B = ctx.csr_from_host(m, m, rp, ci, nz)
+    assert not ctx.prepare_spmv(B)
+    x = rng.standard_normal(m)
+    dx = ctx.to_device(x)
+    y0, y1, y2 = (ctx.empty(m, np.float64) for _ in range(3))
+    ctx.spmv(B, 1, 1.0, dx, m, 0.0, y0, m)  # CSR kernel
+    assert ctx.prepare_spmv(B, always=True)
+    ctx.spmv(B, 1, 1.0, dx, m, 0.0, y1, m)
+    ctx.spmv(B, 1, 1.0, dx, m, 0.0, y2, m)
     assert np.array_equal(y1.numpy(), y2.numpy())
+    ref = np.zeros(m)
+    O.gespmbv(1, 1.0, m, rp, ci, nz, 0, x, m, 0.0, ref, m)
+    tol = 1e-14 * np.abs(nz).max() * np.abs(x).max() * 2000
+    assert np.max(np.abs(y1.numpy() - ref)) <= tol
+    assert np.max(np.abs(y0.numpy() - ref)) <= tol
 
 
 @pytest.mark.parametrize("k,m", [(1, 1), (5, 1), (16, 16), (17, 17), (21, 21), (3, 29)])
