@@ -148,6 +148,19 @@ __global__ __launch_bounds__(64) void bjacobi_kernel(Seg sg, int k, const double
       for(int q = p + 1; q < k; ++q) {
         const double apq = a[p][q];
         if(apq == 0.) continue;
+        // elements below the diagonals' rounding are zeroed after four
+        // sweeps instead of rotated (Numerical Recipes, jacobi); every
+        // thread reads the same values, so the branch is uniform
+        const double g = 100. * fabs(apq), app = fabs(a[p][p]), aqq = fabs(a[q][q]);
+        if(sweep > 3 && app + g == app && aqq + g == aqq) {
+          __syncthreads();
+          if(t == 0) {
+            a[p][q] = 0.;
+            a[q][p] = 0.;
+          }
+          __syncthreads();
+          continue;
+        }
         const double tau = (a[q][q] - a[p][p]) / (2. * apq);
         const double tt = (tau >= 0. ? 1. : -1.) / (fabs(tau) + sqrt(1. + tau * tau));
         const double c = 1. / sqrt(1. + tt * tt), s = tt * c;

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void residual_kernel(int64_t N, int k,
 
 // Restart of the restarted variant, in place per row n:
 //   [V(:,0..nb-1)]  = V(:, :k) Q,   [AV(:,0..nb-1)] = AV(:, :k) Q,
-//   V(:,nb) = V(:,tcol)
+//   V(:,nb) = V(:,tcol), AV(:,nb) = AV(:,tcol)
 // Each thread reads its row of V and AV before writing it, so the in-place
 // update is race free (no other thread touches row n).
 struct RestartCoef {
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void restart_kernel(int64_t N, int k, int nb, 
         v1 = fma(x, Q.q[1][j], v1);
         a1 = fma(y, Q.q[1][j], a1);
       }
-    const double t = V[n + tcol * ld];
+    const double t = V[n + tcol * ld], at = AV[n + tcol * ld];
     V[n] = v0;
     AV[n] = a0;
     if(nb > 1) {
@@ -244,6 +244,7 @@ __global__ __launch_bounds__(256) void restart_kernel(int64_t N, int k, int nb, 
       AV[n + ld] = a1;
     }
     V[n + nb * ld] = t;
+    AV[n + nb * ld] = at;
   }
 }
 
@@ -319,6 +320,7 @@ struct Ops {
   hipStream_t st() const { return ctx->stream; }
   double* inner() { return small.as<double>(); }
   double* scalar() { return small.as<double>() + kMaxK; }
+  double* rn2() { return small.as<double>() + 2 * kMaxK; }  // residual norm^2, then C
   double* Cdev() { return small.as<double>() + 4 * kMaxK; }
 
   void dots(const double* V, int64_t ldv, int k, const double* w, double* out) {
@@ -344,8 +346,8 @@ struct Ops {
     dots(w, N, 1, w, scalar());
     hipLaunchKernelGGL(scale_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N, w, scalar());
   }
-  // C (k x k, ldc = k) = V^T AV on MFMA, copied to host.
-  void gram(const double* V, const double* AV, int k, double* C_host) {
+  // C (k x k, ldc = k) = V^T AV on MFMA into Cdev() (not waited for).
+  void gram(const double* V, const double* AV, int k) {
     const int nb = gram_blocks(N);
     if(k <= 16) {
       hipLaunchKernelGGL((gram_mfma_kernel<1, 1>), dim3(nb), dim3(256), 0, st(), N, k, V, N, k,
@@ -359,18 +361,68 @@ struct Ops {
                          part.as<double>(), nb, k, k, Cdev(), k);
     }
     MHIP_HIP_CHECK(hipGetLastError());
-    d2h(ctx, C_host, Cdev(), sizeof(double) * k * k);
   }
-  double residual(const double* V, const double* AV, int k, const RitzCoef& rc,
-                  double* X, double* R) {
+  // X = V c; R = AV c - lam X; ||R||^2 into rn2() (not waited for).
+  void residual(const double* V, const double* AV, int k, const RitzCoef& rc, double* X,
+                double* R) {
     const int nb = red_blocks(N);
     hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, AV, N, rc, X, R,
                        part.as<double>());
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, 1,
-                       scalar());
+                       rn2());
+    MHIP_HIP_CHECK(hipGetLastError());
+  }
+  // One wait for the stream: the k x k Gram matrix and the last ||R||^2.
+  void read_gram(int k, double* C_host, double* r2) {
+    double buf[2 * kMaxK + kMaxK * kMaxK];
+    d2h(ctx, buf, rn2(), sizeof(double) * (2 * kMaxK + k * k));
+    *r2 = buf[0];
+    std::memcpy(C_host, buf + 2 * kMaxK, sizeof(double) * k * k);
+  }
+  double read_r2() {
     double r2 = 0.;
-    d2h(ctx, &r2, scalar(), 8);
-    return std::sqrt(r2);
+    d2h(ctx, &r2, rn2(), 8);
+    return r2;
+  }
+};
+
+// Start/stop event pairs of the SpMVs of one solve, read after the stream
+// has been waited for (no wait per product).
+struct EventPairs {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, idle;
+  ~EventPairs() {
+    for(auto* v : {&pending, &idle})
+      for(auto& e : *v) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
+  }
+  std::pair<hipEvent_t, hipEvent_t> get() {
+    std::pair<hipEvent_t, hipEvent_t> e{nullptr, nullptr};
+    if(!idle.empty()) {
+      e = idle.back();
+      idle.pop_back();
+    } else {
+      MHIP_HIP_CHECK(hipEventCreate(&e.first));
+      if(hipEventCreate(&e.second) != hipSuccess) {
+        (void)hipEventDestroy(e.first);
+        throw Error(MHIP_ERR_HIP, "hipEventCreate failed");
+      }
+    }
+    pending.push_back(e);
+    return e;
+  }
+  // sum of the elapsed times of the pending pairs (all complete)
+  double drain() {
+    double ms = 0.;
+    for(auto& e : pending) {
+      float t = 0.f;
+      MHIP_HIP_CHECK(hipEventElapsedTime(&t, e.first, e.second));
+      ms += t;
+      idle.push_back(e);
+    }
+    pending.clear();
+    return ms;
   }
 };
 
@@ -396,6 +448,16 @@ void syev_lower(int n, double* A, int lda, double* W) {
       for(int q = p + 1; q < n; ++q) {
         const double apq = a[p + n * q];
         if(apq == 0.) continue;
+        // after four sweeps an element below the diagonals' rounding is set
+        // to zero instead of rotated (Numerical Recipes, jacobi): without
+        // it the off-diagonal floor of large-magnitude matrices never meets
+        // the relative test and every call runs the sweep cap
+        const double g = 100. * std::fabs(apq), app = std::fabs(a[p + n * p]),
+                     aqq = std::fabs(a[q + n * q]);
+        if(sweep > 3 && app + g == app && aqq + g == aqq) {
+          a[p + n * q] = a[q + n * p] = 0.;
+          continue;
+        }
         const double tau = (a[q + n * q] - a[p + n * p]) / (2. * apq);
         const double t = (tau >= 0. ? 1. : -1.) / (std::fabs(tau) + std::sqrt(1. + tau * tau));
         const double c = 1. / std::sqrt(1. + t * t), s = t * c;
@@ -479,7 +541,7 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   MHIP_REQUIRE(max_m >= 1 && max_m < kMaxK, MHIP_ERR_INVALID_ARG,
                "max_m must be in [1, 31]");
   hipStream_t st = ctx->stream;
-  Timer t_all(st), t_op(st);
+  Timer t_all(st);
   t_all.start();
   Ops ops(ctx, N);
   const int64_t ld = N;
@@ -505,11 +567,12 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
     tm.dav_layout_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
   }
+  EventPairs ev;
   auto op = [&](const double* x, double* y) {
-    t_op.start();
+    const auto e = ev.get();
+    MHIP_HIP_CHECK(hipEventRecord(e.first, st));
     spmv(ctx, A, 1., x, 0., y);
-    t_op.stop();
-    tm.dav_spmv_ms += t_op.ms();
+    MHIP_HIP_CHECK(hipEventRecord(e.second, st));
     tm.dav_spmv_calls++;
   };
   MHIP_HIP_CHECK(hipMemsetAsync(AV, 0, sizeof(double) * N * (max_m + 1), st));
@@ -518,55 +581,85 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   MHIP_HIP_CHECK(hipMemcpyAsync(V + ld, AV, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
   ops.gram_schmidt(V, 1, V + ld);                                           // :157
 
+  // Iteration i (k = i + 1 columns) as davidson.hpp:164-218. The host waits
+  // once per iteration, for the Gram matrix: the residual norm of iteration
+  // i is read together with the Gram matrix of iteration i + 1, so the
+  // correction, its product and the next Gram matrix are queued before the
+  // convergence test. When that test passes, the queued work (one product,
+  // possibly a restart) is discarded: X and the energy are those of the
+  // converged iteration, exactly as if the loop had stopped there.
   bool converged = false;
-  int64_t iter = 1, restarts = 0, i0 = 1;
+  int64_t iter = 1, restarts = 0, i0 = 1, i = 1;
   double C[kMaxK * kMaxK], LAM[kMaxK];
   double ccur[kMaxK] = {}, cprev[kMaxK] = {}, c2cur[kMaxK] = {};
-  bool prev_valid = false;
+  bool prev_valid = false, pending = false, pending_restart = false;
   int kcur = 0;
   double lam0 = 0., rnorm = 0.;
-  for(;;) {
-    for(int64_t i = i0; i < max_m; ++i, ++iter) {
-      const int k = (int)(i + 1);
-      op(V + i * ld, AV + i * ld);                                          // :166
-      ops.gram(V, AV, k, C);                                                // :172
-      syev_lower(k, C, k, LAM);
-      lam0 = LAM[0];
-      RitzCoef rc;
-      for(int j = 0; j < kMaxK; ++j) rc.c[j] = j < k ? C[j] : 0.;
-      rc.lam = lam0;
-      std::memcpy(cprev, ccur, sizeof(ccur));
-      prev_valid = i > i0;  // cprev holds coefficients in this basis
-      for(int j = 0; j < k; ++j) {
-        ccur[j] = C[j];
-        c2cur[j] = k > 1 ? C[j + k] : 0.;
-      }
-      kcur = k;
-      double* R = V + (i + 1) * ld;
-      rnorm = ops.residual(V, AV, k, rc, d_X, R);                           // :181-193
+  if(max_m >= 2) op(V + ld, AV + ld);                                       // :166
+  while(max_m >= 2) {
+    const int k = (int)(i + 1);
+    ops.gram(V, AV, k);                                                     // :172
+    double r2 = 0.;
+    ops.read_gram(k, C, &r2);
+    tm.dav_spmv_ms += ev.drain();
+    if(pending) {  // the previous iteration's residual norm (:193, :205)
+      rnorm = std::sqrt(r2);
       if(!std::isfinite(rnorm)) break;
-      if(rnorm < tol) {                                                     // :205
+      if(rnorm < tol) {
         converged = true;
+        if(pending_restart) --restarts;
         break;
       }
+      ++iter;
+    }
+    syev_lower(k, C, k, LAM);
+    lam0 = LAM[0];
+    RitzCoef rc;
+    for(int j = 0; j < kMaxK; ++j) rc.c[j] = j < k ? C[j] : 0.;
+    rc.lam = lam0;
+    std::memcpy(cprev, ccur, sizeof(ccur));
+    prev_valid = i > i0;  // cprev holds coefficients in this basis
+    for(int j = 0; j < k; ++j) {
+      ccur[j] = C[j];
+      c2cur[j] = k > 1 ? C[j + k] : 0.;
+    }
+    kcur = k;
+    double* R = V + (i + 1) * ld;
+    ops.residual(V, AV, k, rc, d_X, R);                                     // :181-193
+    pending = true;
+    pending_restart = false;
+    if(i + 1 < max_m || restarts < max_restarts) {
       hipLaunchKernelGGL(precond_kernel, dim3(red_blocks(N)), dim3(256), 0, st, N, R, d_D,
                          lam0);                                             // :212
       ops.gram_schmidt(V, k, R);                                            // :217
+      op(R, AV + (i + 1) * ld);  // the next iteration's product (:166)
     }
-    if(converged || restarts >= max_restarts || max_m < 2 || !std::isfinite(rnorm)) break;
+    if(i + 1 < max_m) {
+      ++i;
+      continue;
+    }
+    if(restarts >= max_restarts) {  // subspace exhausted: this iteration decides
+      rnorm = std::sqrt(ops.read_r2());
+      converged = std::isfinite(rnorm) && rnorm < tol;
+      if(!converged && std::isfinite(rnorm)) ++iter;
+      break;
+    }
     // Restart (labelled variant): basis [V c, V q1, t] with q1 from the
     // previous Ritz vector; AV follows by the same combinations (no SpMV).
-    // t = V(:,max_m) is orthogonal to span(V) already.
+    // t = V(:,max_m) is orthogonal to span(V) already; its product A t was
+    // just formed and moves with it.
     ++restarts;
+    pending_restart = true;
     const int nb = max_m >= 3 ? 2 : 1;
     RestartCoef Q{};
     restart_coefficients(kcur, nb, ccur, prev_valid ? cprev : nullptr, c2cur, Q.q[0], Q.q[1]);
     hipLaunchKernelGGL(restart_kernel, dim3(red_blocks(N)), dim3(256), 0, st, N, kcur, nb, Q, V,
                        AV, ld, max_m);
     MHIP_HIP_CHECK(hipGetLastError());
-    i0 = nb;
+    i0 = i = nb;
   }
   MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  tm.dav_spmv_ms += ev.drain();
   t_all.stop();
   tm.dav_total_ms = t_all.ms();
   keep_buffer(ctx, std::move(Vb));

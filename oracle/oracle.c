@@ -620,6 +620,14 @@ void orc_syev_lower(int64_t n, double* A, int64_t lda, double* W) {
       for(int64_t q = p + 1; q < n; ++q) {
         const double apq = a[p + n * q];
         if(apq == 0.) continue;
+        { /* after four sweeps an element below the diagonals' rounding is
+           * zeroed instead of rotated (Numerical Recipes, jacobi) */
+          const double g = 100. * fabs(apq), app = fabs(a[p + n * p]), aqq = fabs(a[q + n * q]);
+          if(sweep > 3 && app + g == app && aqq + g == aqq) {
+            a[p + n * q] = a[q + n * p] = 0.;
+            continue;
+          }
+        }
         const double tau = (a[q + n * q] - a[p + n * p]) / (2. * apq);
         const double t = (tau >= 0. ? 1. : -1.) / (fabs(tau) + sqrt(1. + tau * tau));
         const double c = 1. / sqrt(1. + t * t), s = t * c;

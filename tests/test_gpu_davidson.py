@@ -175,7 +175,9 @@ def test_davidson_h2o_cisd(ctx, h2o, h2o_oracle, h2o_cisd):
     O.gespmbv(1, 1.0, N, rp, ci, nz, 0, Xh, N, 0.0, AX, N)
     assert Xh @ AX == pytest.approx(E0, abs=1e-9)
     t = ctx.timings()
-    assert t["dav_spmv_calls"] == it + 1 and t["dav_restarts"] == 0
+    # one product per iteration plus the guess's; the product of the next
+    # correction is queued before the convergence test reads the norm
+    assert it + 1 <= t["dav_spmv_calls"] <= it + 2 and t["dav_restarts"] == 0
 
 
 def test_davidson_not_converged(ctx, h2o, h2o_cisd):
@@ -211,8 +213,10 @@ def test_selected_ci_diag_restarted_vs_oracle(ctx, n, ne, seed):
     t = ctx.timings()
     assert t["dav_residual"] < 1e-8
     # the restart forms A*[X, Xp] from the stored products: one SpMV per
-    # iteration plus the initial one, none at the restarts
-    assert t["dav_restarts"] > 0 and t["dav_spmv_calls"] == t["dav_iters"] + 1
+    # iteration plus the initial one (and at most one queued ahead of the
+    # convergence test), none at the restarts
+    assert t["dav_restarts"] > 0
+    assert t["dav_iters"] + 1 <= t["dav_spmv_calls"] <= t["dav_iters"] + 2
     assert t["dav_sliced"] == 1  # SpMVs on the sliced copy of the Hamiltonian
 
 
@@ -245,7 +249,7 @@ def test_davidson_restarted_small_subspace(ctx, max_m):
     Xh = X.numpy()
     assert np.linalg.norm(dense @ Xh - E * Xh) < 1e-7
     t = ctx.timings()
-    assert t["dav_spmv_calls"] == it + 1
+    assert it + 1 <= t["dav_spmv_calls"] <= it + 2
     _, Eo, _ = O.davidson(N, max_m, rp, ci, nz, Dg.numpy(), 1e-8, X0, max_restarts=20000)
     assert Eo == pytest.approx(w[0], abs=1e-9)
 
