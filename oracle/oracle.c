@@ -23,6 +23,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -562,6 +563,46 @@ int orc_csr_rows(const orc_ham* h, int W, const uint64_t* bra,
   return ORC_OK;
 }
 
+int orc_csr_rows_parallel(const orc_ham* h, int W, const uint64_t* bra, int64_t row_begin,
+                          int64_t row_end, const uint64_t* ket, int64_t nket, double thresh,
+                          int64_t* row_counts, int32_t* colind, double* nzval) {
+  /* The double loop of double_loop.hpp:45-91 with rows spread over OpenMP
+   * threads (not in the reference, whose loop is serial): a counting pass,
+   * a prefix sum, then every row filled at its offset. Same CSR as
+   * orc_csr_rows. */
+  if(!h || (W != 1 && W != 2) || row_end < row_begin || !row_counts) return ORC_ERR_ARG;
+  const int64_t n = row_end - row_begin;
+#pragma omp parallel for schedule(dynamic, 16)
+  for(int64_t i = 0; i < n; ++i) {
+    int err = 0;
+    det_t b = load_det(W, bra + W * (row_begin + i));
+    row_counts[i] = csr_row(h, W, b, ket, nket, thresh, NULL, NULL, NULL, &err);
+  }
+  if(!colind) return ORC_OK;
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+  if(!off) return ORC_ERR_NOMEM;
+  off[0] = 0;
+  for(int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + row_counts[i];
+#pragma omp parallel for schedule(dynamic, 16)
+  for(int64_t i = 0; i < n; ++i) {
+    int err = 0;
+    det_t b = load_det(W, bra + W * (row_begin + i));
+    csr_row(h, W, b, ket, nket, thresh, colind + off[i], nzval + off[i], NULL, &err);
+  }
+  free(off);
+  return ORC_OK;
+}
+
+int orc_num_threads(void) {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    n = omp_get_num_threads();
+  }
+  return n;
+}
+
 void orc_free(void* p) { free(p); }
 
 /* ===================================================================== */
@@ -571,8 +612,11 @@ void orc_gespmbv(int64_t K, double alpha, int64_t M, const int64_t* Arp,
                  const int32_t* Aci, const double* Anz, int64_t indexing,
                  const double* V, int64_t LDV, double beta, double* AV,
                  int64_t LDAV) {
-  /* spmbv.hpp:37-73 (serial dot per row in CSR order) */
+  /* spmbv.hpp:37-73: rows in parallel (the reference's omp parallel for,
+   * spmbv.hpp:54), each row a serial dot in CSR order, so the result does
+   * not depend on the thread count */
   for(int64_t k = 0; k < K; ++k)
+#pragma omp parallel for schedule(static)
     for(int64_t i = 0; i < M; ++i) {
       const int64_t j_st = Arp[i] - indexing, j_en = Arp[i + 1] - indexing;
       const double* V_k = V + k * LDV - indexing;

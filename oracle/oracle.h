@@ -96,6 +96,13 @@ int orc_csr_rows(const orc_ham* h, int W, const uint64_t* bra,
                  int64_t row_begin, int64_t row_end, const uint64_t* ket,
                  int64_t nket, double thresh, int64_t* row_counts,
                  int32_t* colind, double* nzval);
+/* orc_csr_rows with rows spread over OpenMP threads (count pass, prefix
+ * sum, fill pass); row_counts is required. Same CSR. */
+int orc_csr_rows_parallel(const orc_ham* h, int W, const uint64_t* bra, int64_t row_begin,
+                          int64_t row_end, const uint64_t* ket, int64_t nket, double thresh,
+                          int64_t* row_counts, int32_t* colind, double* nzval);
+/* Threads an OpenMP parallel region of the oracle uses. */
+int orc_num_threads(void);
 void orc_free(void* p);
 
 /* ---- sparse BLAS (spmbv.hpp:37-73; submatrix.hpp:264-295) -------------- */

@@ -54,6 +54,8 @@ def lib():
             "orc_generate_cisd_hilbert_space": (_i64, [_i32, _i64, _p, _p]),
             "orc_csr_build": (_i32, [_p, _i32, _p, _i64, _p, _i64, _d, _p, _p, _p, _p]),
             "orc_csr_rows": (_i32, [_p, _i32, _p, _i64, _i64, _p, _i64, _d, _p, _p, _p]),
+            "orc_csr_rows_parallel": (_i32, [_p, _i32, _p, _i64, _i64, _p, _i64, _d, _p, _p, _p]),
+            "orc_num_threads": (_i32, []),
             "orc_free": (None, [_p]),
             "orc_gespmbv": (None, [_i64, _d, _i64, _p, _p, _p, _i64, _p, _i64, _d, _p, _i64]),
             "orc_extract_diagonal": (None, [_i64, _p, _p, _p, _i64, _p]),
@@ -173,6 +175,40 @@ class Ham:
         for p in (rp, ci, nz):
             lib().orc_free(p)
         return rowptr, colind, nzval
+
+    def csr_rows_into(self, W, bra, row_begin, row_end, col, val, ket=None, thresh=1e-16):
+        """One pass of the serial double loop over rows [row_begin,row_end)
+        writing into caller buffers (large enough for every kept entry, as
+        the reference's push_back never reallocates more than amortised);
+        returns the row counts."""
+        bra = np.ascontiguousarray(bra, dtype=np.uint64)
+        ket = bra if ket is None else np.ascontiguousarray(ket, dtype=np.uint64)
+        counts = np.zeros(row_end - row_begin, dtype=np.int64)
+        lib().orc_csr_rows(self._h, W, _ptr(bra), row_begin, row_end, _ptr(ket), ket.size // W,
+                           thresh, _ptr(counts), _ptr(col), _ptr(val))
+        return counts
+
+    def csr_rows_parallel(self, W, bra, row_begin, row_end, ket=None, thresh=1e-16, col=None,
+                          val=None):
+        """Rows [row_begin,row_end) by the OpenMP row-parallel double loop
+        (count pass, prefix sum, fill pass): (counts, colind, nzval); with
+        caller buffers col/val (large enough) one call fills them and the
+        counts are returned."""
+        bra = np.ascontiguousarray(bra, dtype=np.uint64)
+        ket = bra if ket is None else np.ascontiguousarray(ket, dtype=np.uint64)
+        counts = np.zeros(row_end - row_begin, dtype=np.int64)
+        if col is not None:
+            lib().orc_csr_rows_parallel(self._h, W, _ptr(bra), row_begin, row_end, _ptr(ket),
+                                        ket.size // W, thresh, _ptr(counts), _ptr(col), _ptr(val))
+            return counts
+        lib().orc_csr_rows_parallel(self._h, W, _ptr(bra), row_begin, row_end, _ptr(ket),
+                                    ket.size // W, thresh, _ptr(counts), None, None)
+        tot = int(counts.sum())
+        col = np.zeros(max(tot, 1), dtype=np.int32)
+        val = np.zeros(max(tot, 1), dtype=np.float64)
+        lib().orc_csr_rows_parallel(self._h, W, _ptr(bra), row_begin, row_end, _ptr(ket),
+                                    ket.size // W, thresh, _ptr(counts), _ptr(col), _ptr(val))
+        return counts, col[:tot], val[:tot]
 
     def csr_rows(self, W, bra, row_begin, row_end, ket=None, thresh=1e-16):
         """Rows [row_begin,row_end) of the block: (counts, colind, nzval)."""
@@ -297,6 +333,11 @@ def cisd_space(W, norb, state):
     out = np.zeros(n * W, dtype=np.uint64)
     lib().orc_generate_cisd_hilbert_space(W, norb, _ptr(st), _ptr(out))
     return out.reshape(n, W)
+
+
+def num_threads():
+    """Threads of the oracle's OpenMP regions (gespmbv, csr_rows_parallel)."""
+    return int(lib().orc_num_threads())
 
 
 def gespmbv(K, alpha, m, rowptr, colind, nzval, indexing, V, LDV, beta, AV, LDAV):

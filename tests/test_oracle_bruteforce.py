@@ -209,6 +209,33 @@ def test_restarted_davidson_small_subspace_reaches_ground_state(max_m):
     assert np.linalg.norm(dense @ X - E * X) < 1e-8
 
 
+def test_oracle_parallel_and_one_pass_rows_equal_serial():
+    """The OpenMP row-parallel double loop and the one-pass variant give the
+    serial loop's CSR rows exactly; the threaded gespmbv gives the row dots
+    of a plain loop."""
+    n = 8
+    T, V = seeded_integrals(n, 5)
+    Ho = O.Ham(T, V)
+    dets = D.generate_hilbert_space(1, n, 4, 4)
+    c0, col0, val0 = Ho.csr_rows(1, dets, 100, 700, thresh=1e-16)
+    c1, col1, val1 = Ho.csr_rows_parallel(1, dets, 100, 700, thresh=1e-16)
+    assert np.array_equal(c0, c1) and np.array_equal(col0, col1) and np.array_equal(val0, val1)
+    col2 = np.zeros(len(col0) + 5, dtype=np.int32)
+    val2 = np.zeros(len(col0) + 5)
+    c2 = Ho.csr_rows_into(1, dets, 100, 700, col2, val2, thresh=1e-16)
+    assert np.array_equal(c0, c2) and np.array_equal(col0, col2[:len(col0)])
+    assert np.array_equal(val0, val2[:len(col0)])
+    c3 = Ho.csr_rows_parallel(1, dets, 100, 700, thresh=1e-16, col=col2, val=val2)
+    assert np.array_equal(c0, c3) and np.array_equal(val0, val2[:len(col0)])
+    assert O.num_threads() >= 1
+    rp, ci, nz = Ho.csr(1, dets[:600], thresh=1e-16)
+    x = np.random.default_rng(1).standard_normal(600)
+    y = np.zeros(600)
+    O.gespmbv(1, 1.0, 600, rp, ci, nz, 0, x, 600, 0.0, y, 600)
+    ref = np.array([sum(nz[j] * x[ci[j]] for j in range(rp[i], rp[i + 1])) for i in range(600)])
+    assert np.array_equal(y, ref)
+
+
 def test_synthetic_integrals_symmetry():
     n = 6
     T, V = seeded_integrals(n, 1)
