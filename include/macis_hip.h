@@ -105,7 +105,8 @@ typedef struct mhip_timings {
 #define MHIP_BUILD_TILED 1
 #define MHIP_BUILD_PRODUCT 2
 
-/* Settings of asci_search (ASCISettings, determinant_search.hpp:34-51). */
+/* Settings of asci_search (ASCISettings, determinant_search.hpp:34-51).
+ * pair_size_max / rv_prune_tol prune as the reference does (:120-137). */
 typedef struct mhip_asci_settings {
   int64_t ncdets_max;
   double h_el_tol;

@@ -23,9 +23,9 @@
 //             ndets_max - ncdets largest (:563-629); ties keep determinant
 //             order. The output is top-k followed by the core dets (:653).
 // The count/fill decomposition is fixed, so the result is run-to-run
-// deterministic. The reference's pair_size_max pruning (:120-137) is not
-// applied: candidates are all kept, which equals the reference whenever
-// its pruning does not trigger.
+// deterministic. When the candidates outnumber pair_size_max, the
+// reference's pruning (:120-137) is replayed on the sorted list before the
+// accumulation (prune_* kernels).
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_reduce_by_key.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -290,6 +290,118 @@ __global__ __launch_bounds__(256) void contrib_kernel(
   }
   if constexpr(!FILL)
     if(lane == 0) chunk_counts[chunk] = cnt;
+}
+
+// ---- pair_size_max pruning (asci_contributions_standard, :120-137) -------
+// The reference prunes its growing pair list after the core determinant at
+// which it first exceeds pair_size_max: pairs with |rv| <= rv_prune_tol are
+// erased (stage 1) and, if the list is still too long, it is sorted and
+// accumulated (stage 2); later prunes then test accumulated scores. Here the
+// whole candidate list exists at once, sorted stably by determinant (ties in
+// core-determinant order), and the prune events are replayed per
+// determinant segment: the host walks the per-core-det counts to find the
+// events, one pass per stage 2 recomputes each segment's accumulated score
+// (a left fold in core-det order, the reference's order after its stable
+// sort), and a last pass marks what the remaining events erase. Erased pairs
+// get score 0 before reduce-by-key; a determinant whose pairs are all erased
+// sums to 0 and leaves with the zero scores.
+struct RvDet {
+  double rv;
+  int32_t det;  // core determinant of the pair; -1 once erased
+  int32_t pad;
+};
+
+// Per chunk (one wavefront): (rv, core det) of its slots and the number of
+// slots with |rv| <= tol.
+__global__ __launch_bounds__(256) void prune_prep_kernel(
+    const int64_t* __restrict__ chunk_start, int64_t nc, int64_t nchunks,
+    const int64_t* __restrict__ chunk_off, const double* __restrict__ vals, double tol,
+    RvDet* __restrict__ rd, int64_t* __restrict__ chunk_tiny) {
+  const int64_t chunk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if(chunk >= nchunks) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int32_t c = (int32_t)find_core(chunk_start, nc, chunk);
+  int64_t tiny = 0;
+  for(int64_t p = chunk_off[chunk] + lane; p < chunk_off[chunk + 1]; p += 64) {
+    const double v = vals[p];
+    rd[p] = RvDet{v, c, 0};
+    tiny += fabs(v) <= tol ? 1 : 0;
+  }
+  tiny = wave_sum(tiny);
+  if(lane == 0) chunk_tiny[chunk] = tiny;
+}
+
+template <typename K>
+__device__ __forceinline__ bool seg_head(const K* __restrict__ keys, int64_t i) {
+  return i == 0 || !(keys[i] == keys[i - 1]);
+}
+
+// Stage 2 at core det q (previous stage 2 at q0, -1 if none), per
+// determinant segment (thread of its first pair): the previous accumulated
+// entry is erased when |acc| <= tol (the first prune after q0 tested it),
+// single pairs with q0 < det <= q and |rv| <= tol are erased (stage 1 of the
+// events since q0), then acc = left fold of the remaining pairs with
+// det <= q. cnt[0] += segments holding an entry, cnt[1] += entries with
+// |acc| <= tol.
+template <typename K>
+__global__ __launch_bounds__(256) void prune_stage2_kernel(
+    const K* __restrict__ keys, RvDet* __restrict__ rd, int64_t P, int32_t q0, int32_t q,
+    double tol, double* __restrict__ acc, uint8_t* __restrict__ accf,
+    unsigned long long* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  unsigned long long mine[2] = {0ull, 0ull};
+  if(i < P && seg_head(keys, i)) {
+    const bool drop_acc = accf[i] && fabs(acc[i]) <= tol;
+    double a = 0.;
+    bool any = false;
+    for(int64_t j = i; j < P && keys[j] == keys[i]; ++j) {
+      RvDet e = rd[j];
+      if(e.det < 0 || e.det > q) continue;
+      if((drop_acc && e.det <= q0) || (e.det > q0 && fabs(e.rv) <= tol)) {
+        e.det = -1;
+        rd[j] = e;
+        continue;
+      }
+      a = any ? a + e.rv : e.rv;
+      any = true;
+    }
+    acc[i] = a;
+    accf[i] = any ? 1 : 0;
+    mine[0] = any ? 1ull : 0ull;
+    mine[1] = any && fabs(a) <= tol ? 1ull : 0ull;
+  }
+  __shared__ unsigned long long red[2][4];
+  const unsigned long long s0 = wave_sum(mine[0]), s1 = wave_sum(mine[1]);
+  if((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = s0;
+    red[1][threadIdx.x >> 6] = s1;
+  }
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    atomicAdd(cnt, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+    atomicAdd(cnt + 1, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+  }
+}
+
+// The events after the last stage 2 (q0, -1 if none) up to core det last:
+// the accumulated entry of q0 is erased when |acc| <= tol, single pairs with
+// q0 < det <= last and |rv| <= tol are erased; out = rv of the kept pairs,
+// 0 for erased ones.
+template <typename K>
+__global__ __launch_bounds__(256) void prune_final_kernel(
+    const K* __restrict__ keys, const RvDet* __restrict__ rd, int64_t P, int32_t q0,
+    int32_t last, double tol, const double* __restrict__ acc, const uint8_t* __restrict__ accf,
+    double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(i >= P || !seg_head(keys, i)) return;
+  const bool drop_acc = last > q0 && q0 >= 0 && accf[i] && fabs(acc[i]) <= tol;
+  for(int64_t j = i; j < P && keys[j] == keys[i]; ++j) {
+    const RvDet e = rd[j];
+    bool dead = e.det < 0;
+    if(!dead && drop_acc && e.det <= q0) dead = true;
+    if(!dead && e.det > q0 && e.det <= last && fabs(e.rv) <= tol) dead = true;
+    out[j] = dead ? 0. : e.rv;
+  }
 }
 
 // keep = not a core det and |rv| > 0 (the -|rv| < 0 partition, :528-544);
@@ -562,7 +674,63 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   DevBuf nu_dev(16);
   MHIP_HIP_CHECK(hipMemsetAsync(nu_dev.p, 0, 16, st));
   int64_t nu = 0;
-  if(P) {
+  if(P > s->pair_size_max && nc > 0) {
+    // the reference's list would exceed pair_size_max: replay its prunes
+    MHIP_REQUIRE(nc < INT32_MAX, MHIP_ERR_INVALID_ARG, "too many core determinants");
+    DevBuf rd(sizeof(RvDet) * P), rd2(sizeof(RvDet) * P), tiny(8 * nchunks);
+    hipLaunchKernelGGL(prune_prep_kernel, dim3(grid), dim3(256), 0, st, dstart.as<int64_t>(), nc,
+                       nchunks, coff.as<int64_t>(), vals.as<double>(), s->rv_prune_tol,
+                       rd.as<RvDet>(), tiny.as<int64_t>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys.as<K>(), keys2.as<K>(),
+                                             rd.as<RvDet>(), rd2.as<RvDet>(), (size_t)P, 0,
+                                             32 * W * 2, st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, keys.as<K>(), keys2.as<K>(),
+                                             rd.as<RvDet>(), rd2.as<RvDet>(), (size_t)P, 0,
+                                             32 * W * 2, st));
+    rd.release();
+    // per core det: pairs and pairs with |rv| <= rv_prune_tol
+    std::vector<int64_t> hcnt(nchunks), htiny(nchunks), cdet(nc, 0), tdet(nc, 0);
+    d2h(ctx, hcnt.data(), ccount.p, 8 * nchunks);
+    d2h(ctx, htiny.data(), tiny.p, 8 * nchunks);
+    for(int64_t c = 0; c < nc; ++c)
+      for(int64_t ch = cstart[c]; ch < cstart[c + 1]; ++ch) {
+        cdet[c] += hcnt[ch];
+        tdet[c] += htiny[ch];
+      }
+    DevBuf acc(8 * P), accf(P), pcnt(16);
+    MHIP_HIP_CHECK(hipMemsetAsync(accf.p, 0, P, st));
+    const unsigned pgrid = (unsigned)((P + 255) / 256);
+    int64_t S = 0, tiny_since = 0, pending_acc_tiny = 0;
+    int32_t last_event = -1, last_s2 = -1;
+    for(int64_t c = 0; c < nc; ++c) {
+      S += cdet[c];
+      tiny_since += tdet[c];
+      if(S <= s->pair_size_max) continue;
+      S -= tiny_since + pending_acc_tiny;  // stage 1 (:123-128)
+      tiny_since = pending_acc_tiny = 0;
+      if(S > s->pair_size_max) {  // stage 2 (:131-135)
+        MHIP_HIP_CHECK(hipMemsetAsync(pcnt.p, 0, 16, st));
+        hipLaunchKernelGGL((prune_stage2_kernel<K>), dim3(pgrid), dim3(256), 0, st,
+                           keys2.as<K>(), rd2.as<RvDet>(), P, last_s2, (int32_t)c,
+                           s->rv_prune_tol, acc.as<double>(), accf.as<uint8_t>(),
+                           pcnt.as<unsigned long long>());
+        MHIP_HIP_CHECK(hipGetLastError());
+        int64_t hc2[2];
+        d2h(ctx, hc2, pcnt.p, 16);
+        S = hc2[0];
+        pending_acc_tiny = hc2[1];
+        last_s2 = (int32_t)c;
+      }
+      last_event = (int32_t)c;
+    }
+    hipLaunchKernelGGL((prune_final_kernel<K>), dim3(pgrid), dim3(256), 0, st, keys2.as<K>(),
+                       rd2.as<RvDet>(), P, last_s2, last_event, s->rv_prune_tol,
+                       acc.as<double>(), accf.as<uint8_t>(), vals2.as<double>());
+    MHIP_HIP_CHECK(hipGetLastError());
+  } else if(P) {
     tmp = 0;
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys.as<K>(), keys2.as<K>(),
                                              vals.as<double>(), vals2.as<double>(), (size_t)P,
@@ -571,6 +739,8 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, keys.as<K>(), keys2.as<K>(),
                                              vals.as<double>(), vals2.as<double>(), (size_t)P,
                                              0, 32 * W * 2, st));
+  }
+  if(P) {
     tmp = 0;
     MHIP_HIP_CHECK(rocprim::reduce_by_key(nullptr, tmp, keys2.as<K>(), vals2.as<double>(),
                                           (size_t)P, ukeys.as<K>(), usum.as<double>(),

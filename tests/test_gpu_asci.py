@@ -100,3 +100,82 @@ def test_asci_search_everything_fits(ctx, h2o, h2o_oracle):
     E0 = h2o_oracle.matrix_element(1, hf, hf)
     got, ref, _ = run_both(ctx, h2o_oracle, 1, norb, hf, np.array([1.0]), E0, 100000)
     assert np.array_equal(got, ref)  # det-sorted order when no top-k is needed
+
+
+def _prune_events(counts, tiny, pmax):
+    """Core-det indices of the reference's prune events up to the first one
+    whose stage 1 leaves the list above pair_size_max (stage 2), from the
+    per-core-det pair and small-score counts (determinant_search.hpp:120-137)."""
+    S, since, events = 0, 0, []
+    for d, (c, t) in enumerate(zip(counts, tiny)):
+        S += c
+        since += t
+        if S <= pmax:
+            continue
+        S -= since
+        since = 0
+        events.append(d)
+        if S > pmax:
+            return events, True
+    return events, False
+
+
+PRUNE_CASES = [(0.6, 0.6), (0.4, 0.6), (0.25, 0.3), (0.15, 0.5), (0.08, 0.3)]
+
+
+@pytest.mark.parametrize("frac,q", PRUNE_CASES)
+def test_asci_search_pair_size_max_pruning(ctx, h2o, h2o_oracle, frac, q):
+    """pair_size_max below the raw candidate count: the GPU replays the
+    reference's prunes (stage 1: erase |rv| <= rv_prune_tol; stage 2: sort
+    and accumulate; later prunes test accumulated scores) and its top-k
+    matches the oracle's asci_search, which runs the reference loop. The
+    cases cover prunes with stage 1 only and with stage 2, and the pruning
+    must change the result against the unpruned search."""
+    announce(f"asci_search pruning pair_size_max = {frac} x pairs, tol = q{q}")
+    norb, _, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    hf = D.canonical_hf_determinant(1, 5, 5)
+    E0 = h2o_oracle.matrix_element(1, hf, hf)
+    s = O.ASCISettings.defaults()
+    E1, w, X = h2o_oracle.asci_iter(1, s, 100, E0, hf, np.array([1.0]))
+    E2, w, X = h2o_oracle.asci_iter(1, s, 800, E1, w, X)
+    order = np.argsort(-np.abs(X), kind="stable")
+    core, C = w[order[:100]], X[order[:100]]
+    per = [h2o_oracle.asci_contributions(1, s, core[i:i + 1], E2, C[i:i + 1])[1]
+           for i in range(len(core))]
+    counts = [len(r) for r in per]
+    rv_all = np.abs(np.concatenate(per))
+    tol = float(np.quantile(rv_all, q))
+    pmax = int(frac * len(rv_all))
+    tiny = [int((np.abs(r) <= tol).sum()) for r in per]
+    events, stage2 = _prune_events(counts, tiny, pmax)
+    assert events, "the case must prune"
+    got, ref, ref_rv = run_both(ctx, h2o_oracle, 1, norb, core, C, E2, 3000,
+                                pair_size_max=pmax, rv_prune_tol=tol)
+    check_topk(got, ref, ref_rv, 100)
+    assert ctx.timings()["asci_npairs"] == len(rv_all)
+    full, _, _ = run_both(ctx, h2o_oracle, 1, norb, core, C, E2, 3000)
+    assert _rows(full) != _rows(got), "pruning left the top-k unchanged"
+
+
+def test_asci_pruning_cases_cover_both_stages(h2o, h2o_oracle):
+    """The parameter sets above include prunes with and without stage 2."""
+    norb, _, T, V = h2o
+    hf = D.canonical_hf_determinant(1, 5, 5)
+    E0 = h2o_oracle.matrix_element(1, hf, hf)
+    s = O.ASCISettings.defaults()
+    E1, w, X = h2o_oracle.asci_iter(1, s, 100, E0, hf, np.array([1.0]))
+    E2, w, X = h2o_oracle.asci_iter(1, s, 800, E1, w, X)
+    order = np.argsort(-np.abs(X), kind="stable")
+    core, C = w[order[:100]], X[order[:100]]
+    per = [h2o_oracle.asci_contributions(1, s, core[i:i + 1], E2, C[i:i + 1])[1]
+           for i in range(len(core))]
+    rv_all = np.abs(np.concatenate(per))
+    seen = set()
+    for frac, q in PRUNE_CASES:
+        tol = float(np.quantile(rv_all, q))
+        tiny = [int((np.abs(r) <= tol).sum()) for r in per]
+        events, stage2 = _prune_events([len(r) for r in per], tiny, int(frac * len(rv_all)))
+        assert events
+        seen.add(stage2)
+    assert seen == {False, True}
