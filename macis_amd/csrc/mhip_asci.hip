@@ -297,111 +297,134 @@ __global__ __launch_bounds__(256) void contrib_kernel(
 // which it first exceeds pair_size_max: pairs with |rv| <= rv_prune_tol are
 // erased (stage 1) and, if the list is still too long, it is sorted and
 // accumulated (stage 2); later prunes then test accumulated scores. Here the
-// whole candidate list exists at once, sorted stably by determinant (ties in
-// core-determinant order), and the prune events are replayed per
-// determinant segment: the host walks the per-core-det counts to find the
-// events, one pass per stage 2 recomputes each segment's accumulated score
-// (a left fold in core-det order, the reference's order after its stable
-// sort), and a last pass marks what the remaining events erase. Erased pairs
-// get score 0 before reduce-by-key; a determinant whose pairs are all erased
-// sums to 0 and leaves with the zero scores.
-struct RvDet {
+// whole candidate list exists at once, sorted stably by determinant with
+// each pair's slot (slots run in core-det order, so "core det <= q" is
+// "slot < first slot of det q+1"). The host walks per-core-det counts to
+// place the events (stage 1 is count arithmetic); a stage 2 at q folds only
+// the pairs of dets (q0, q] into their segments' accumulated scores (a left
+// fold in core-det order continued from the score of q0: the reference's
+// order after its stable sort) and first erases the accumulated entries
+// that the first prune after q0 found small. Erased pairs get score 0
+// before reduce-by-key; a determinant whose pairs are all erased sums to 0
+// and leaves with the zero scores.
+struct RvSlot {
   double rv;
-  int32_t det;  // core determinant of the pair; -1 once erased
-  int32_t pad;
+  int64_t slot;
 };
 
-// Per chunk (one wavefront): (rv, core det) of its slots and the number of
+// Per chunk (one wavefront): (rv, slot) of its slots and the number of
 // slots with |rv| <= tol.
-__global__ __launch_bounds__(256) void prune_prep_kernel(
-    const int64_t* __restrict__ chunk_start, int64_t nc, int64_t nchunks,
-    const int64_t* __restrict__ chunk_off, const double* __restrict__ vals, double tol,
-    RvDet* __restrict__ rd, int64_t* __restrict__ chunk_tiny) {
+__global__ __launch_bounds__(256) void prune_prep_kernel(int64_t nchunks,
+                                                         const int64_t* __restrict__ chunk_off,
+                                                         const double* __restrict__ vals, double tol,
+                                                         RvSlot* __restrict__ rs,
+                                                         int64_t* __restrict__ chunk_tiny) {
   const int64_t chunk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if(chunk >= nchunks) return;
   const uint32_t lane = threadIdx.x & 63;
-  const int32_t c = (int32_t)find_core(chunk_start, nc, chunk);
   int64_t tiny = 0;
   for(int64_t p = chunk_off[chunk] + lane; p < chunk_off[chunk + 1]; p += 64) {
     const double v = vals[p];
-    rd[p] = RvDet{v, c, 0};
+    rs[p] = RvSlot{v, p};
     tiny += fabs(v) <= tol ? 1 : 0;
   }
   tiny = wave_sum(tiny);
   if(lane == 0) chunk_tiny[chunk] = tiny;
 }
 
-template <typename K>
-__device__ __forceinline__ bool seg_head(const K* __restrict__ keys, int64_t i) {
-  return i == 0 || !(keys[i] == keys[i - 1]);
+// pos[slot] = sorted position
+__global__ void prune_pos_kernel(const RvSlot* __restrict__ rs, int64_t P,
+                                 int64_t* __restrict__ pos) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(j < P) pos[rs[j].slot] = j;
 }
 
-// Stage 2 at core det q (previous stage 2 at q0, -1 if none), per
-// determinant segment (thread of its first pair): the previous accumulated
-// entry is erased when |acc| <= tol (the first prune after q0 tested it),
-// single pairs with q0 < det <= q and |rv| <= tol are erased (stage 1 of the
-// events since q0), then acc = left fold of the remaining pairs with
-// det <= q. cnt[0] += segments holding an entry, cnt[1] += entries with
-// |acc| <= tol.
-template <typename K>
-__global__ __launch_bounds__(256) void prune_stage2_kernel(
-    const K* __restrict__ keys, RvDet* __restrict__ rd, int64_t P, int32_t q0, int32_t q,
-    double tol, double* __restrict__ acc, uint8_t* __restrict__ accf,
-    unsigned long long* __restrict__ cnt) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  unsigned long long mine[2] = {0ull, 0ull};
-  if(i < P && seg_head(keys, i)) {
-    const bool drop_acc = accf[i] && fabs(acc[i]) <= tol;
-    double a = 0.;
-    bool any = false;
-    for(int64_t j = i; j < P && keys[j] == keys[i]; ++j) {
-      RvDet e = rd[j];
-      if(e.det < 0 || e.det > q) continue;
-      if((drop_acc && e.det <= q0) || (e.det > q0 && fabs(e.rv) <= tol)) {
-        e.det = -1;
-        rd[j] = e;
-        continue;
-      }
-      a = any ? a + e.rv : e.rv;
-      any = true;
-    }
-    acc[i] = a;
-    accf[i] = any ? 1 : 0;
-    mine[0] = any ? 1ull : 0ull;
-    mine[1] = any && fabs(a) <= tol ? 1ull : 0ull;
-  }
+__device__ __forceinline__ void add_counts(unsigned long long a, unsigned long long b,
+                                           unsigned long long* __restrict__ cnt) {
   __shared__ unsigned long long red[2][4];
-  const unsigned long long s0 = wave_sum(mine[0]), s1 = wave_sum(mine[1]);
+  const unsigned long long s0 = wave_sum(a), s1 = wave_sum(b);
   if((threadIdx.x & 63) == 0) {
     red[0][threadIdx.x >> 6] = s0;
     red[1][threadIdx.x >> 6] = s1;
   }
   __syncthreads();
   if(threadIdx.x == 0) {
-    atomicAdd(cnt, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
-    atomicAdd(cnt + 1, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+    const unsigned long long t0 = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    const unsigned long long t1 = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    if(t0) atomicAdd(cnt, t0);
+    if(t1) atomicAdd(cnt + 1, t1);
   }
 }
 
-// The events after the last stage 2 (q0, -1 if none) up to core det last:
-// the accumulated entry of q0 is erased when |acc| <= tol, single pairs with
-// q0 < det <= last and |rv| <= tol are erased; out = rv of the kept pairs,
-// 0 for erased ones.
+// The small accumulated entries listed by the last stage 2 (accf == 2) are
+// erased: their pairs (slots < lim0, the dets folded into them) die.
 template <typename K>
-__global__ __launch_bounds__(256) void prune_final_kernel(
-    const K* __restrict__ keys, const RvDet* __restrict__ rd, int64_t P, int32_t q0,
-    int32_t last, double tol, const double* __restrict__ acc, const uint8_t* __restrict__ accf,
-    double* __restrict__ out) {
+__global__ __launch_bounds__(256) void prune_drop_acc_kernel(
+    const K* __restrict__ keys, const RvSlot* __restrict__ rs, int64_t P, int64_t lim0,
+    const int64_t* __restrict__ list, int64_t nlist, uint8_t* __restrict__ accf,
+    uint8_t* __restrict__ alive) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if(i >= P || !seg_head(keys, i)) return;
-  const bool drop_acc = last > q0 && q0 >= 0 && accf[i] && fabs(acc[i]) <= tol;
-  for(int64_t j = i; j < P && keys[j] == keys[i]; ++j) {
-    const RvDet e = rd[j];
-    bool dead = e.det < 0;
-    if(!dead && drop_acc && e.det <= q0) dead = true;
-    if(!dead && e.det > q0 && e.det <= last && fabs(e.rv) <= tol) dead = true;
-    out[j] = dead ? 0. : e.rv;
+  if(i >= nlist) return;
+  const int64_t h = list[i];
+  accf[h] = 0;
+  for(int64_t k = h; k < P && keys[k] == keys[h]; ++k)
+    if(rs[k].slot < lim0) alive[k] = 0;
+}
+
+// Stage 2 at core det q: the pairs of slots [lim0, lim1) (dets (q0, q])
+// are folded into their segments, erasing those with |rv| <= tol. One
+// thread per segment touched: the first such pair of the segment walks
+// back to the segment head for its accumulated score and forward over the
+// run. cnt[0] += segments gaining an entry; new entries with |acc| <= tol
+// are marked accf = 2 and listed (cnt[1] of them) for the next prune.
+template <typename K>
+__global__ __launch_bounds__(256) void prune_stage2_kernel(
+    const K* __restrict__ keys, const RvSlot* __restrict__ rs, int64_t P,
+    const int64_t* __restrict__ pos, int64_t lim0, int64_t lim1, double tol,
+    double* __restrict__ acc, uint8_t* __restrict__ accf, uint8_t* __restrict__ alive,
+    unsigned long long* __restrict__ cnt, int64_t* __restrict__ list) {
+  const int64_t sl = lim0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  unsigned long long gained = 0;
+  if(sl < lim1) {
+    const int64_t j = pos[sl];
+    int64_t h = j;
+    while(h > 0 && keys[h - 1] == keys[j]) --h;
+    // leader: the first pair of the segment inside [lim0, lim1)
+    if(j == h || rs[j - 1].slot < lim0) {
+      const uint8_t had = accf[h];
+      bool any = had != 0;
+      double a = any ? acc[h] : 0.;
+      for(int64_t k = j; k < P && keys[k] == keys[h] && rs[k].slot < lim1; ++k) {
+        const double v = rs[k].rv;
+        if(fabs(v) <= tol) {
+          alive[k] = 0;
+          continue;
+        }
+        a = any ? a + v : v;
+        any = true;
+      }
+      if(any) {
+        acc[h] = a;
+        accf[h] = fabs(a) <= tol ? 2 : 1;
+        gained = had ? 0ull : 1ull;
+        if(fabs(a) <= tol) list[atomicAdd(cnt + 1, 1ull)] = h;
+      }
+    }
   }
+  add_counts(gained, 0ull, cnt);
+}
+
+// Scores for reduce-by-key after the last event: erased pairs and pairs of
+// slots [lim0, lim1) (dets after the last stage 2 up to the last event)
+// with |rv| <= tol give 0.
+__global__ void prune_final_kernel(const RvSlot* __restrict__ rs, const uint8_t* __restrict__ alive,
+                                   int64_t P, int64_t lim0, int64_t lim1, double tol,
+                                   double* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(j >= P) return;
+  const RvSlot e = rs[j];
+  const bool dead = !alive[j] || (e.slot >= lim0 && e.slot < lim1 && fabs(e.rv) <= tol);
+  out[j] = dead ? 0. : e.rv;
 }
 
 // keep = not a core det and |rv| > 0 (the -|rv| < 0 partition, :528-544);
@@ -676,59 +699,86 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   int64_t nu = 0;
   if(P > s->pair_size_max && nc > 0) {
     // the reference's list would exceed pair_size_max: replay its prunes
-    MHIP_REQUIRE(nc < INT32_MAX, MHIP_ERR_INVALID_ARG, "too many core determinants");
-    DevBuf rd(sizeof(RvDet) * P), rd2(sizeof(RvDet) * P), tiny(8 * nchunks);
-    hipLaunchKernelGGL(prune_prep_kernel, dim3(grid), dim3(256), 0, st, dstart.as<int64_t>(), nc,
-                       nchunks, coff.as<int64_t>(), vals.as<double>(), s->rv_prune_tol,
-                       rd.as<RvDet>(), tiny.as<int64_t>());
+    DevBuf rs(sizeof(RvSlot) * P), rs2(sizeof(RvSlot) * P), tiny(8 * nchunks);
+    hipLaunchKernelGGL(prune_prep_kernel, dim3(grid), dim3(256), 0, st, nchunks,
+                       coff.as<int64_t>(), vals.as<double>(), s->rv_prune_tol, rs.as<RvSlot>(),
+                       tiny.as<int64_t>());
     MHIP_HIP_CHECK(hipGetLastError());
     tmp = 0;
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys.as<K>(), keys2.as<K>(),
-                                             rd.as<RvDet>(), rd2.as<RvDet>(), (size_t)P, 0,
+                                             rs.as<RvSlot>(), rs2.as<RvSlot>(), (size_t)P, 0,
                                              32 * W * 2, st));
     ctx->scratch_c.reserve(tmp + 16);
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, keys.as<K>(), keys2.as<K>(),
-                                             rd.as<RvDet>(), rd2.as<RvDet>(), (size_t)P, 0,
+                                             rs.as<RvSlot>(), rs2.as<RvSlot>(), (size_t)P, 0,
                                              32 * W * 2, st));
-    rd.release();
-    // per core det: pairs and pairs with |rv| <= rv_prune_tol
-    std::vector<int64_t> hcnt(nchunks), htiny(nchunks), cdet(nc, 0), tdet(nc, 0);
+    rs.release();
+    // per core det: first slot, pairs and pairs with |rv| <= rv_prune_tol
+    std::vector<int64_t> hcnt(nchunks), htiny(nchunks), dslot(nc + 1, 0), tdet(nc, 0);
     d2h(ctx, hcnt.data(), ccount.p, 8 * nchunks);
     d2h(ctx, htiny.data(), tiny.p, 8 * nchunks);
-    for(int64_t c = 0; c < nc; ++c)
+    for(int64_t c = 0; c < nc; ++c) {
+      dslot[c + 1] = dslot[c];
       for(int64_t ch = cstart[c]; ch < cstart[c + 1]; ++ch) {
-        cdet[c] += hcnt[ch];
+        dslot[c + 1] += hcnt[ch];
         tdet[c] += htiny[ch];
       }
-    DevBuf acc(8 * P), accf(P), pcnt(16);
-    MHIP_HIP_CHECK(hipMemsetAsync(accf.p, 0, P, st));
+    }
     const unsigned pgrid = (unsigned)((P + 255) / 256);
-    int64_t S = 0, tiny_since = 0, pending_acc_tiny = 0;
-    int32_t last_event = -1, last_s2 = -1;
+    DevBuf pos(8 * P), acc(8 * P), accf(P), alive(P), pcnt(16), slist(8 * P);
+    hipLaunchKernelGGL(prune_pos_kernel, dim3(pgrid), dim3(256), 0, st, rs2.as<RvSlot>(), P,
+                       pos.as<int64_t>());
+    MHIP_HIP_CHECK(hipMemsetAsync(accf.p, 0, P, st));
+    MHIP_HIP_CHECK(hipMemsetAsync(alive.p, 1, P, st));
+    // S: the reference's list length; nseg: accumulated entries (stage 2);
+    // pending_small: small accumulated entries the next stage 1 erases;
+    // marks / pending_list: those entries (listed in slist) not yet erased
+    // on the device
+    int64_t S = 0, tiny_since = 0, pending_small = 0, nseg = 0, pending_list = 0;
+    int64_t last_event = -1, last_s2 = -1;
+    bool marks = false;
+    auto drop_marked = [&] {
+      hipLaunchKernelGGL((prune_drop_acc_kernel<K>), dim3((unsigned)((pending_list + 255) / 256)),
+                         dim3(256), 0, st, keys2.as<K>(), rs2.as<RvSlot>(), P, dslot[last_s2 + 1],
+                         slist.as<int64_t>(), pending_list, accf.as<uint8_t>(),
+                         alive.as<uint8_t>());
+      MHIP_HIP_CHECK(hipGetLastError());
+      nseg -= pending_list;
+      pending_list = 0;
+      marks = false;
+    };
     for(int64_t c = 0; c < nc; ++c) {
-      S += cdet[c];
+      S += dslot[c + 1] - dslot[c];
       tiny_since += tdet[c];
       if(S <= s->pair_size_max) continue;
-      S -= tiny_since + pending_acc_tiny;  // stage 1 (:123-128)
-      tiny_since = pending_acc_tiny = 0;
+      S -= tiny_since + pending_small;  // stage 1 (:123-128)
+      tiny_since = pending_small = 0;
       if(S > s->pair_size_max) {  // stage 2 (:131-135)
+        if(marks) drop_marked();
+        const int64_t lim0 = dslot[last_s2 + 1], lim1 = dslot[c + 1];
         MHIP_HIP_CHECK(hipMemsetAsync(pcnt.p, 0, 16, st));
-        hipLaunchKernelGGL((prune_stage2_kernel<K>), dim3(pgrid), dim3(256), 0, st,
-                           keys2.as<K>(), rd2.as<RvDet>(), P, last_s2, (int32_t)c,
-                           s->rv_prune_tol, acc.as<double>(), accf.as<uint8_t>(),
-                           pcnt.as<unsigned long long>());
+        if(lim1 > lim0)
+          hipLaunchKernelGGL((prune_stage2_kernel<K>), dim3((unsigned)((lim1 - lim0 + 255) / 256)),
+                             dim3(256), 0, st, keys2.as<K>(), rs2.as<RvSlot>(), P,
+                             pos.as<int64_t>(), lim0, lim1, s->rv_prune_tol, acc.as<double>(),
+                             accf.as<uint8_t>(), alive.as<uint8_t>(),
+                             pcnt.as<unsigned long long>(), slist.as<int64_t>());
         MHIP_HIP_CHECK(hipGetLastError());
-        int64_t hc2[2];
-        d2h(ctx, hc2, pcnt.p, 16);
-        S = hc2[0];
-        pending_acc_tiny = hc2[1];
-        last_s2 = (int32_t)c;
+        int64_t h2[2] = {0, 0};
+        d2h(ctx, h2, pcnt.p, 16);
+        nseg += h2[0];
+        S = nseg;
+        pending_small = pending_list = h2[1];
+        marks = h2[1] > 0;
+        last_s2 = c;
       }
-      last_event = (int32_t)c;
+      last_event = c;
     }
-    hipLaunchKernelGGL((prune_final_kernel<K>), dim3(pgrid), dim3(256), 0, st, keys2.as<K>(),
-                       rd2.as<RvDet>(), P, last_s2, last_event, s->rv_prune_tol,
-                       acc.as<double>(), accf.as<uint8_t>(), vals2.as<double>());
+    // marks of the last stage 2 are erased only if a prune followed it
+    if(marks && last_event > last_s2) drop_marked();
+    hipLaunchKernelGGL(prune_final_kernel, dim3(pgrid), dim3(256), 0, st, rs2.as<RvSlot>(),
+                       alive.as<uint8_t>(), P, dslot[last_s2 + 1], dslot[last_event + 1],
+                       s->rv_prune_tol, vals2.as<double>());
     MHIP_HIP_CHECK(hipGetLastError());
   } else if(P) {
     tmp = 0;
