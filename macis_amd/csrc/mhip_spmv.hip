@@ -289,8 +289,14 @@ bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding) {
   size_t spare_b = 0;
   for(auto& b : ctx->spare) spare_b += b.bytes;
   if(free_b + spare_b < need) return false;
-  S->cols = take_buffer(ctx, 4 * (size_t)S->entries);
-  S->vals = take_buffer(ctx, 8 * (size_t)S->entries);
+  try {
+    S->cols = take_buffer(ctx, 4 * (size_t)S->entries);
+    S->vals = take_buffer(ctx, 8 * (size_t)S->entries);
+  } catch(const Error& e) {  // another allocation won the race: keep the CSR kernel
+    if(e.code != MHIP_ERR_OUT_OF_MEMORY) throw;
+    (void)hipGetLastError();
+    return false;
+  }
   hipLaunchKernelGGL(sell_fill_kernel, dim3(S->nsl), dim3(256), 0, st, A->m,
                      A->rowptr.as<int64_t>(), A->colind.as<int32_t>(), A->nzval.as<double>(),
                      A->indexing, soff, slen, S->cols.as<int32_t>(), S->vals.as<double>());
