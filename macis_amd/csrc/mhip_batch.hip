@@ -56,62 +56,11 @@ __global__ __launch_bounds__(256) void bspmv_kernel(
   if(active && sl == 0) y[row] = acc;
 }
 
-// ---- per-instance reductions: one workgroup per instance -----------------
+// ---- per-instance state: row offsets and convergence flags ---------------
 struct Seg {
   const int64_t* __restrict__ off;  // [ninst + 1] row offsets
   const int32_t* __restrict__ conv;  // [ninst]
 };
-
-__device__ __forceinline__ double block_sum(double v, double* lds) {
-  v = wave_sum(v);
-  if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const double s = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-  __syncthreads();
-  return s;
-}
-
-// C_b = V_b^T AV_b (k x k, ld kMaxK) on MFMA; one workgroup per instance.
-__global__ __launch_bounds__(256) void bgram_kernel(Seg sg, int k, const double* __restrict__ V,
-                                                    const double* __restrict__ AV, int64_t ld,
-                                                    double* __restrict__ C) {
-  const int b = blockIdx.x;
-  if(sg.conv[b]) return;
-  const int64_t lo = sg.off[b], hi = sg.off[b + 1];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int col = lane & 15, grp = lane >> 4;
-  d4_t acc[2][2];
-  for(int i = 0; i < 2; ++i)
-    for(int j = 0; j < 2; ++j) acc[i][j] = d4_t{0., 0., 0., 0.};
-  for(int64_t n0 = lo + wv * 32; n0 < hi; n0 += 4 * 32) {
-    const int64_t nb = n0 + grp * 8;
-    double a[2][8], bb[2][8];
-    for(int i = 0; i < 2; ++i) {
-      const int c = i * 16 + col;
-      for(int s = 0; s < 8; ++s) {
-        const bool ok = c < k && nb + s < hi;
-        a[i][s] = ok ? V[(int64_t)c * ld + nb + s] : 0.;
-        bb[i][s] = ok ? AV[(int64_t)c * ld + nb + s] : 0.;
-      }
-    }
-    for(int s = 0; s < 8; ++s)
-      for(int i = 0; i < 2; ++i)
-        for(int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], bb[j][s], acc[i][j], 0, 0, 0);
-  }
-  __shared__ double red[4][1024];
-  for(int i = 0; i < 2; ++i)
-    for(int j = 0; j < 2; ++j)
-      for(int r = 0; r < 4; ++r) red[wv][(i * 2 + j) * 256 + r * 64 + lane] = acc[i][j][r];
-  __syncthreads();
-  for(int e = threadIdx.x; e < 1024; e += 256) {
-    const int tile = e / 256, rem = e % 256, r = rem / 64, ln = rem % 64;
-    // v_mfma_f64_16x16x4f64: lane ln, register r holds D(ln/16 + 4*r, ln%16)
-    const int i = (tile / 2) * 16 + ln / 16 + 4 * r, j = (tile % 2) * 16 + ln % 16;
-    if(i < k && j < k)
-      C[(int64_t)b * kMaxK * kMaxK + i + j * kMaxK] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
-  }
-}
 
 // Lowest eigenpair of each C_b (lower triangle) by cyclic Jacobi in LDS,
 // one wavefront per instance: lam[b], coef[b][:k], the second-lowest
@@ -288,19 +237,120 @@ __global__ __launch_bounds__(256) void bresidual_kernel(
   }
 }
 
-// out[b*stride + j] = V_j . w over instance b's rows, j < k (k <= 32).
-__global__ __launch_bounds__(256) void bdots_kernel(Seg sg, int k, const double* __restrict__ V,
-                                                    int64_t ld, const double* __restrict__ w,
-                                                    double* __restrict__ out) {
-  const int b = blockIdx.x;
-  if(sg.conv[b]) return;
-  __shared__ double lds[4];
-  const int64_t lo = sg.off[b], hi = sg.off[b + 1];
+// ---- tiled per-instance reductions ----------------------------------------
+// Instances are cut into tiles of at most kTileRows rows (tile t: rows
+// [tiles[2t], tiles[2t+1]) of instance tinst[t]); a block per tile writes
+// partial results, a second kernel sums an instance's tiles in a fixed
+// order, so the large instances spread over many CUs and results stay
+// run-to-run deterministic.
+constexpr int64_t kTileRows = 4096;
+
+// part[t*kMaxK + j] = V_j . w over tile t, all j < k in one pass.
+__global__ __launch_bounds__(256) void btdots_kernel(const int64_t* __restrict__ tiles,
+                                                     const int32_t* __restrict__ tinst,
+                                                     const int32_t* __restrict__ conv, int k,
+                                                     const double* __restrict__ V, int64_t ld,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ part) {
+  const int t = blockIdx.x;
+  if(conv[tinst[t]]) return;
+  const int64_t t0 = tiles[2 * t], t1 = tiles[2 * t + 1];
+  double acc[kMaxK];
+#pragma unroll
+  for(int j = 0; j < kMaxK; ++j) acc[j] = 0.;
+  for(int64_t n = t0 + threadIdx.x; n < t1; n += 256) {
+    const double wn = w[n];
+#pragma unroll
+    for(int j = 0; j < kMaxK; ++j)
+      if(j < k) acc[j] = fma(V[n + j * ld], wn, acc[j]);
+  }
+  __shared__ double lds[4][kMaxK];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for(int j = 0; j < kMaxK; ++j)
+    if(j < k) {
+      const double v = wave_sum(acc[j]);
+      if(lane == 0) lds[wv][j] = v;
+    }
+  __syncthreads();
+  if(threadIdx.x < k)
+    part[(int64_t)t * kMaxK + threadIdx.x] =
+        (lds[0][threadIdx.x] + lds[1][threadIdx.x]) + (lds[2][threadIdx.x] + lds[3][threadIdx.x]);
+}
+
+// out[b*kMaxK + j] = sum over instance b's tiles of part[t*kMaxK + j].
+__global__ __launch_bounds__(64) void btreduce_kernel(const int32_t* __restrict__ toff,
+                                                      const int32_t* __restrict__ conv, int k,
+                                                      const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  const int b = blockIdx.x, lane = threadIdx.x;
+  if(conv[b]) return;
   for(int j = 0; j < k; ++j) {
     double s = 0.;
-    for(int64_t n = lo + threadIdx.x; n < hi; n += 256) s = fma(V[n + j * ld], w[n], s);
-    s = block_sum(s, lds);
-    if(threadIdx.x == 0) out[(int64_t)b * kMaxK + j] = s;
+    for(int t = toff[b] + lane; t < toff[b + 1]; t += 64) s += part[(int64_t)t * kMaxK + j];
+    s = wave_sum(s);
+    if(lane == 0) out[(int64_t)b * kMaxK + j] = s;
+  }
+}
+
+// Partial Gram matrix V^T AV of tile t on MFMA (k <= 32): the four waves'
+// accumulators are combined in LDS and stored in the accumulator layout.
+__global__ __launch_bounds__(256) void btgram_kernel(const int64_t* __restrict__ tiles,
+                                                     const int32_t* __restrict__ tinst,
+                                                     const int32_t* __restrict__ conv, int k,
+                                                     const double* __restrict__ V,
+                                                     const double* __restrict__ AV, int64_t ld,
+                                                     double* __restrict__ part) {
+  const int t = blockIdx.x;
+  if(conv[tinst[t]]) return;
+  const int64_t lo = tiles[2 * t], hi = tiles[2 * t + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const int ni = k > 16 ? 2 : 1;
+  d4_t acc[2][2];
+  for(int i = 0; i < 2; ++i)
+    for(int j = 0; j < 2; ++j) acc[i][j] = d4_t{0., 0., 0., 0.};
+  for(int64_t n0 = lo + wv * 32; n0 < hi; n0 += 4 * 32) {
+    const int64_t nb = n0 + grp * 8;
+    double a[2][8], bb[2][8];
+    for(int i = 0; i < 2; ++i) {
+      const int c = i * 16 + col;
+      for(int s = 0; s < 8; ++s) {
+        const bool ok = i < ni && c < k && nb + s < hi;
+        a[i][s] = ok ? V[(int64_t)c * ld + nb + s] : 0.;
+        bb[i][s] = ok ? AV[(int64_t)c * ld + nb + s] : 0.;
+      }
+    }
+    for(int s = 0; s < 8; ++s)
+      for(int i = 0; i < ni; ++i)
+        for(int j = 0; j < ni; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], bb[j][s], acc[i][j], 0, 0, 0);
+  }
+  __shared__ double red[4][1024];
+  for(int i = 0; i < 2; ++i)
+    for(int j = 0; j < 2; ++j)
+      for(int r = 0; r < 4; ++r) red[wv][(i * 2 + j) * 256 + r * 64 + lane] = acc[i][j][r];
+  __syncthreads();
+  for(int e = threadIdx.x; e < 1024; e += 256)
+    part[(int64_t)t * 1024 + e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+}
+
+// C_b (k x k, ld kMaxK) = sum over instance b's tiles of the partial Gram
+// matrices, in tile order.
+__global__ __launch_bounds__(256) void btgram_reduce_kernel(const int32_t* __restrict__ toff,
+                                                            const int32_t* __restrict__ conv, int k,
+                                                            const double* __restrict__ part,
+                                                            double* __restrict__ C) {
+  const int b = blockIdx.x;
+  if(conv[b]) return;
+  for(int e = threadIdx.x; e < 1024; e += 256) {
+    const int tile = e / 256, rem = e % 256, r = rem / 64, ln = rem % 64;
+    // v_mfma_f64_16x16x4f64: lane ln, register r holds D(ln/16 + 4*r, ln%16)
+    const int i = (tile / 2) * 16 + ln / 16 + 4 * r, j = (tile % 2) * 16 + ln % 16;
+    if(i >= k || j >= k) continue;
+    double s = 0.;
+    for(int t = toff[b]; t < toff[b + 1]; ++t) s += part[(int64_t)t * 1024 + e];
+    C[(int64_t)b * kMaxK * kMaxK + i + j * kMaxK] = s;
   }
 }
 
@@ -423,6 +473,9 @@ struct mhip_batch {
   // Davidson workspace
   int64_t max_m = 0;
   mhip::DevBuf V, AV, X, D, C, coef, coef_prev, coef2, Q, lam, tmp, conv, E, res, nleft;
+  // row tiles of the per-instance reductions and their partial results
+  int ntiles = 0;
+  mhip::DevBuf tiles, tinst, toff, part;
   std::map<int, hipGraphExec_t> graphs;
   ~mhip_batch() {
     for(auto& g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -434,6 +487,10 @@ namespace {
 
 void launch_spmv(mhip_context* ctx, mhip_batch* B, const double* x, double* y, bool masked) {
   const int32_t* conv = masked ? B->conv.as<int32_t>() : nullptr;
+  if(B->H->sell) {  // sliced copy of the block-diagonal matrix (equal-length rows)
+    spmv_sliced_masked(ctx, B->H.get(), x, y, B->row_inst.as<int32_t>(), conv);
+    return;
+  }
   for(auto& bk : B->buckets) {
     const int64_t rows = bk.r1 - bk.r0;
     if(rows <= 0) continue;
@@ -459,20 +516,27 @@ void launch_spmv(mhip_context* ctx, mhip_batch* B, const double* x, double* y, b
 
 int red_grid(int64_t N) { return (int)std::max<int64_t>(1, std::min<int64_t>(2048, (N + 255) / 256)); }
 
+// B->tmp[b*kMaxK + j] = V_j . w over instance b (j < k), tiled.
+void tdots(mhip_context* ctx, mhip_batch* B, int k, const double* V, const double* w) {
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(btdots_kernel, dim3(B->ntiles), dim3(256), 0, st, B->tiles.as<int64_t>(),
+                     B->tinst.as<int32_t>(), B->conv.as<int32_t>(), k, V, B->N, w,
+                     B->part.as<double>());
+  hipLaunchKernelGGL(btreduce_kernel, dim3(B->ninst), dim3(64), 0, st, B->toff.as<int32_t>(),
+                     B->conv.as<int32_t>(), k, B->part.as<double>(), B->tmp.as<double>());
+}
+
 // CGS2 of w against V(:, :k) per instance, then per-instance normalisation.
 void bgram_schmidt(mhip_context* ctx, mhip_batch* B, int k, double* w) {
   hipStream_t st = ctx->stream;
-  Seg sg{B->d_off.as<int64_t>(), B->conv.as<int32_t>()};
   const int64_t N = B->N, ld = B->N;
   for(int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, k, B->V.as<double>(),
-                       ld, w, B->tmp.as<double>());
+    tdots(ctx, B, k, B->V.as<double>(), w);
     hipLaunchKernelGGL(bupdate_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, k,
                        B->V.as<double>(), ld, B->row_inst.as<int32_t>(), B->conv.as<int32_t>(),
                        B->tmp.as<double>(), w);
   }
-  hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, 1, w, ld, w,
-                     B->tmp.as<double>());
+  tdots(ctx, B, 1, w, w);
   hipLaunchKernelGGL(bscale_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, w,
                      B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->tmp.as<double>());
 }
@@ -486,7 +550,11 @@ void biteration(mhip_context* ctx, mhip_batch* B, int i, double tol) {
   double* V = B->V.as<double>();
   double* AV = B->AV.as<double>();
   launch_spmv(ctx, B, V + i * ld, AV + i * ld, true);
-  hipLaunchKernelGGL(bgram_kernel, dim3(B->ninst), dim3(256), 0, st, sg, k, V, AV, ld,
+  hipLaunchKernelGGL(btgram_kernel, dim3(B->ntiles), dim3(256), 0, st, B->tiles.as<int64_t>(),
+                     B->tinst.as<int32_t>(), B->conv.as<int32_t>(), k, V, AV, ld,
+                     B->part.as<double>());
+  hipLaunchKernelGGL(btgram_reduce_kernel, dim3(B->ninst), dim3(256), 0, st,
+                     B->toff.as<int32_t>(), B->conv.as<int32_t>(), k, B->part.as<double>(),
                      B->C.as<double>());
   hipLaunchKernelGGL(bjacobi_kernel, dim3(B->ninst), dim3(64), 0, st, sg, k, B->C.as<double>(),
                      B->lam.as<double>(), B->coef.as<double>(), B->coef_prev.as<double>(),
@@ -495,8 +563,7 @@ void biteration(mhip_context* ctx, mhip_batch* B, int i, double tol) {
   hipLaunchKernelGGL(bresidual_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, k, V, AV, ld,
                      B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->coef.as<double>(),
                      B->lam.as<double>(), B->X.as<double>(), R);
-  hipLaunchKernelGGL(bdots_kernel, dim3(B->ninst), dim3(256), 0, st, sg, 1, R, ld, R,
-                     B->tmp.as<double>());
+  tdots(ctx, B, 1, R, R);
   hipLaunchKernelGGL(bcheck_kernel, dim3(1), dim3(256), 0, st, B->ninst, B->conv.as<int32_t>(),
                      B->tmp.as<double>(), B->lam.as<double>(), tol, B->E.as<double>(),
                      B->res.as<double>(), B->nleft.as<int32_t>());
@@ -747,6 +814,30 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
       B->res.alloc(8 * B->ninst);
       B->nleft.alloc(4);
     }
+    if(!B->ntiles) {
+      std::vector<int64_t> tl;
+      std::vector<int32_t> ti, to(B->ninst + 1, 0);
+      for(int b = 0; b < B->ninst; ++b) {
+        for(int64_t r = B->off[b]; r < B->off[b + 1]; r += kTileRows) {
+          tl.push_back(r);
+          tl.push_back(std::min<int64_t>(r + kTileRows, B->off[b + 1]));
+          ti.push_back(b);
+        }
+        to[b + 1] = (int32_t)ti.size();
+      }
+      B->ntiles = (int)ti.size();
+      B->tiles.alloc(8 * tl.size());
+      B->tinst.alloc(4 * ti.size());
+      B->toff.alloc(4 * to.size());
+      B->part.alloc(8 * (size_t)B->ntiles * 1024);
+      MHIP_HIP_CHECK(hipMemcpyAsync(B->tiles.p, tl.data(), 8 * tl.size(), hipMemcpyHostToDevice, st));
+      MHIP_HIP_CHECK(hipMemcpyAsync(B->tinst.p, ti.data(), 4 * ti.size(), hipMemcpyHostToDevice, st));
+      MHIP_HIP_CHECK(hipMemcpyAsync(B->toff.p, to.data(), 4 * to.size(), hipMemcpyHostToDevice, st));
+      MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    // the sliced SpMV copy when padding stays small (rows of one instance
+    // have one length) and memory allows; made once per built batch
+    prepare_sell(ctx, B->H.get());
     Timer tall(st);
     tall.start();
     Seg sg{B->d_off.as<int64_t>(), B->conv.as<int32_t>()};

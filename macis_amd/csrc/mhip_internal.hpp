@@ -217,6 +217,10 @@ void extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
 // it and (unless any_padding) padding adds at most a quarter of the
 // nonzeros; true if the matrix has one afterwards; spmv() then uses it.
 bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding = false);
+// y = A x on the sliced copy (A must have one); with conv, rows r with
+// conv[row_inst[r]] set are skipped (y keeps its value there).
+void spmv_sliced_masked(mhip_context* ctx, const mhip_csr* A, const double* x, double* y,
+                        const int32_t* row_inst, const int32_t* conv);
 // Davidson core; returns convergence flag.
 bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
               int64_t max_restarts, double tol, const double* d_D, double* d_X,

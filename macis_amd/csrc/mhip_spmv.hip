@@ -228,15 +228,24 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(
 // y = alpha A x + beta y on the sliced copy: one slice per wavefront, lane =
 // row, kSellUnroll slots per step with non-temporal column/value loads;
 // blocks mapped XCD-aware as in spmv_kernel.
+// With row_inst / conv (a block-diagonal batch), rows of converged
+// instances are neither computed nor written, and slices holding only such
+// rows are skipped.
 __global__ __launch_bounds__(256) void spmv_sell_kernel(
     int64_t m, int64_t nsl, int64_t per_xcd, const int64_t* __restrict__ soff,
     const int32_t* __restrict__ slen, const int32_t* __restrict__ cols,
     const double* __restrict__ vals, double alpha, const double* __restrict__ x, double beta,
-    double* __restrict__ y) {
+    double* __restrict__ y, const int32_t* __restrict__ row_inst,
+    const int32_t* __restrict__ conv) {
   const int64_t lblk = (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   const int64_t s = lblk * kWavesPerBlock + threadIdx.x / 64;
   if(s >= nsl) return;
   const int lane = threadIdx.x & 63;
+  bool active = s * kSliceRows + lane < m;
+  if(conv) {
+    active = active && !conv[row_inst[s * kSliceRows + lane]];
+    if(!ballot(active)) return;  // whole wavefront: nothing to do in this slice
+  }
   const int len = slen[s];
   const int32_t* c = cols + soff[s] + lane;
   const double* v = vals + soff[s] + lane;
@@ -258,7 +267,7 @@ __global__ __launch_bounds__(256) void spmv_sell_kernel(
   for(; j < len; ++j) a[0] = fma(v[(int64_t)j * kSliceRows], x[c[(int64_t)j * kSliceRows]], a[0]);
   const double acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   const int64_t row = s * kSliceRows + lane;
-  if(row < m) y[row] = alpha * acc + beta * y[row];
+  if(active) y[row] = alpha * acc + beta * y[row];
 }
 
 }  // namespace
@@ -314,7 +323,7 @@ void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
     const int64_t per = (nblk + kXcds - 1) / kXcds;
     hipLaunchKernelGGL(spmv_sell_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m,
                        S.nsl, per, S.soff(), S.slen(), S.cols.as<int32_t>(),
-                       S.vals.as<double>(), alpha, x, beta, y);
+                       S.vals.as<double>(), alpha, x, beta, y, nullptr, nullptr);
     MHIP_HIP_CHECK(hipGetLastError());
     return;
   }
@@ -323,6 +332,17 @@ void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
   hipLaunchKernelGGL(spmv_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m, per,
                      A->rowptr.as<int64_t>(), A->colind.as<int32_t>(), A->nzval.as<double>(),
                      A->indexing, alpha, x, beta, y);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+void spmv_sliced_masked(mhip_context* ctx, const mhip_csr* A, const double* x, double* y,
+                        const int32_t* row_inst, const int32_t* conv) {
+  const SellCopy& S = *A->sell;
+  const int64_t nblk = (S.nsl + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t per = (nblk + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL(spmv_sell_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m, S.nsl,
+                     per, S.soff(), S.slen(), S.cols.as<int32_t>(), S.vals.as<double>(), 1., x, 0.,
+                     y, row_inst, conv);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 
