@@ -478,11 +478,13 @@ class MLP:
     def predict(self, W, d_dets, n, d_out):
         check(lib().mhip_mlp_predict(self.ctx.h, self.h, W, _ptr(d_dets), n, _ptr(d_out)), "predict")
 
-    def train_step(self, W, d_dets, d_target, n, lr):
+    def train_step(self, W, d_dets, d_target, n, lr, want_loss=True):
+        """One Adam step; returns the mini-batch loss (None when not asked
+        for: the step then does not wait for the device)."""
         loss = C.c_float()
         check(lib().mhip_mlp_train_step(self.ctx.h, self.h, W, _ptr(d_dets), _ptr(d_target), n,
-                                        lr, C.byref(loss)), "train_step")
-        return loss.value
+                                        lr, C.byref(loss) if want_loss else None), "train_step")
+        return loss.value if want_loss else None
 
 
 def _ptr(x):

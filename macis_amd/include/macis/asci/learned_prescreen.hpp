@@ -92,6 +92,7 @@ class LearnedPrescreen {
     std::vector<uint64_t> hd(n * W);
     std::vector<float> ht(n);
     float loss = 0.f;
+    const size_t nsteps = n / (size_t)batch;
     for(int e = 0; e < epochs; ++e) {
       std::shuffle(perm.begin(), perm.end(), rng);
       for(size_t i = 0; i < n; ++i) {
@@ -101,9 +102,11 @@ class LearnedPrescreen {
       }
       bd.upload(hd.data(), n * W);
       bt.upload(ht.data(), n);
-      for(size_t b0 = 0; b0 + batch <= n; b0 += batch)
+      // the loss (a wait for the device) is read on the last step only
+      for(size_t b0 = 0, i = 0; b0 + batch <= n; b0 += batch, ++i)
         hip::check(mhip_mlp_train_step(ctx.get(), mlp_, W, bd.data() + b0 * W, bt.data() + b0,
-                                       batch, lr, &loss));
+                                       batch, lr,
+                                       e + 1 == epochs && i + 1 == nsteps ? &loss : nullptr));
     }
     trained_ = true;
     recall_ = validate();

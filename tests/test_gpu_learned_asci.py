@@ -50,7 +50,7 @@ def train(ctx, net, W, dets, y, epochs, batch, lr, seed=0):
         dd.upload(dets[perm])
         dy.upload(y[perm])
         for b0 in range(0, n - batch + 1, batch):
-            net.train_step(W, dd.ptr + 8 * W * b0, dy.ptr + 4 * b0, batch, lr)
+            net.train_step(W, dd.ptr + 8 * W * b0, dy.ptr + 4 * b0, batch, lr, want_loss=False)
 
 
 def test_learned_prescreen_recall_energy_and_gain(ctx, h2o):
