@@ -27,6 +27,7 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kIn = 128, kHid = 512;
 constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+constexpr int kStageLd = 64 + 8;  // bf16 row pitch of the epilogue staging (16-byte aligned rows)
 
 __device__ __forceinline__ float gelu(float z) {
   const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
@@ -53,8 +54,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     int M, int N, int K, const __bf16* __restrict__ A, int lda, const __bf16* __restrict__ B,
     int ldb, const float* __restrict__ bias, const float* __restrict__ zaux,
     float* __restrict__ Cf, __bf16* __restrict__ Cb, int ldc) {
-  __shared__ __bf16 As[BM][LDK];
-  __shared__ __bf16 Bs[BN][LDK];
+  // A and B tiles; after the K loop the same LDS stages bf16 outputs
+  __shared__ __attribute__((aligned(16))) __bf16 smem[(BM + BN) * LDK];
+  __bf16(*As)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem);
+  __bf16(*Bs)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem + BM * LDK);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
@@ -70,33 +73,50 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
 #pragma unroll
       for(int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
+  // A and B pieces of one K step in registers: the next step's global loads
+  // are in flight while this step's MFMAs run
+  bf16x8_t ra[2], rb[2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for(int it = 0; it < 2; ++it) {
+      const int c = tid + 256 * it;
+      if constexpr(AK)
+        ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(m0 + (c >> 2)) * lda + k0 +
+                                                    (c & 3) * 8);
+      else
+        ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(k0 + (c >> 4)) * lda + m0 +
+                                                    (c & 15) * 8);
+      if constexpr(BKC)
+        rb[it] = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(n0 + (c >> 2)) * ldb + k0 +
+                                                    (c & 3) * 8);
+      else
+        rb[it] = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(k0 + (c >> 4)) * ldb + n0 +
+                                                    (c & 15) * 8);
+    }
+  };
+  if(kbeg < kend) gload(kbeg);
   for(int k0 = kbeg; k0 < kend; k0 += BK) {
     // ---- stage A and B tiles (K-contiguous in LDS)
 #pragma unroll
     for(int it = 0; it < 2; ++it) {
       const int c = tid + 256 * it;
       if constexpr(AK) {
-        const int row = c >> 2, kc = (c & 3) * 8;
-        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(m0 + row) * lda + k0 + kc);
-        *reinterpret_cast<bf16x8_t*>(&As[row][kc]) = v;
+        *reinterpret_cast<bf16x8_t*>(&As[c >> 2][(c & 3) * 8]) = ra[it];
       } else {
         const int kk = c >> 4, mc = (c & 15) * 8;
-        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(k0 + kk) * lda + m0 + mc);
 #pragma unroll
-        for(int e = 0; e < 8; ++e) As[mc + e][kk] = v[e];
+        for(int e = 0; e < 8; ++e) As[mc + e][kk] = ra[it][e];
       }
       if constexpr(BKC) {
-        const int row = c >> 2, kc = (c & 3) * 8;
-        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(n0 + row) * ldb + k0 + kc);
-        *reinterpret_cast<bf16x8_t*>(&Bs[row][kc]) = v;
+        *reinterpret_cast<bf16x8_t*>(&Bs[c >> 2][(c & 3) * 8]) = rb[it];
       } else {
         const int kk = c >> 4, nc = (c & 15) * 8;
-        const bf16x8_t v = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(k0 + kk) * ldb + n0 + nc);
 #pragma unroll
-        for(int e = 0; e < 8; ++e) Bs[nc + e][kk] = v[e];
+        for(int e = 0; e < 8; ++e) Bs[nc + e][kk] = rb[it][e];
       }
     }
     __syncthreads();
+    if(k0 + BK < kend) gload(k0 + BK);
     // ---- MFMA: 2 x 2 tiles of 32 x 32, two K sub-steps of 16
 #pragma unroll
     for(int ks = 0; ks < BK; ks += 16) {
@@ -117,6 +137,41 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     __syncthreads();
   }
   // ---- epilogue; lane l, register r holds C(8*(r/4) + 4*(l/32) + r%4, l%32)
+  if constexpr(EPI == EPI_BIAS_GELU_A || EPI == EPI_GELU_GRAD) {
+    // bf16 outputs only: each wave stages 32 x 64 of its tile in LDS and
+    // stores whole 16-byte row pieces
+    static_assert(4 * 32 * kStageLd <= (BM + BN) * LDK, "staging exceeds the tile LDS");
+    __bf16* stg = smem + wv * 32 * kStageLd;
+#pragma unroll
+    for(int i = 0; i < 2; ++i) {
+#pragma unroll
+      for(int j = 0; j < 2; ++j)
+#pragma unroll
+        for(int r = 0; r < 16; ++r) {
+          const int lr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), lc = j * 32 + (lane & 31);
+          const int row = m0 + wm + i * 32 + lr, col = n0 + wn + lc;
+          float v = acc[i][j][r];
+          if constexpr(EPI == EPI_BIAS_GELU_A)
+            v = gelu(v + bias[col]);
+          else
+            v = v * gelu_grad(zaux[(size_t)row * ldc + col]);
+          stg[lr * kStageLd + lc] = (__bf16)v;
+        }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const int c = lane + 64 * q, lr = c >> 3, cc = (c & 7) * 8;
+        *reinterpret_cast<bf16x8_t*>(Cb + (size_t)(m0 + wm + i * 32 + lr) * ldc + n0 + wn + cc) =
+            *reinterpret_cast<const bf16x8_t*>(stg + lr * kStageLd + cc);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    return;
+  }
 #pragma unroll
   for(int i = 0; i < 2; ++i)
 #pragma unroll
@@ -142,19 +197,24 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
 }
 
 // Occupation bits -> bf16 features (W words per det, 64 bits per word;
-// features beyond 64*W are zero).
+// features beyond 64*W are zero). A thread writes 8 features (16 bytes):
+// 16 threads per row, 16 rows per block.
 template <int W>
 __global__ void features_kernel(const uint64_t* __restrict__ dets, int64_t n, int64_t npad,
                                 __bf16* __restrict__ X) {
-  const int64_t i = (int64_t)blockIdx.x * 2 + (threadIdx.x >> 7);
-  const int f = threadIdx.x & 127;
+  const int64_t i = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int f0 = (threadIdx.x & 15) * 8;
   if(i >= npad) return;
-  float v = 0.f;
-  if(i < n && f < 64 * W) v = (float)((dets[i * W + (f >> 6)] >> (f & 63)) & 1ull);
-  X[i * kIn + f] = (__bf16)v;
+  uint32_t bits = 0;
+  if(i < n && f0 < 64 * W) bits = (uint32_t)(dets[i * W + (f0 >> 6)] >> (f0 & 63)) & 0xffu;
+  bf16x8_t v;
+#pragma unroll
+  for(int e = 0; e < 8; ++e) v[e] = (__bf16)(float)((bits >> e) & 1u);
+  *reinterpret_cast<bf16x8_t*>(X + i * kIn + f0) = v;
 }
 
 // y = A2 . w3 + b3; with targets, dy = 2 (y - t) / B and partial losses.
+static_assert(kHid == 64 * 8, "head_kernel maps 8 columns to each lane");
 __global__ __launch_bounds__(256) void head_kernel(const __bf16* __restrict__ A2,
                                                    const float* __restrict__ w3,
                                                    const float* __restrict__ b3, int64_t n,
@@ -166,8 +226,11 @@ __global__ __launch_bounds__(256) void head_kernel(const __bf16* __restrict__ A2
   const int lane = threadIdx.x & 63;
   float l = 0.f;
   if(row < n) {
+    // lane l: columns 8l .. 8l+7 (one 16-byte load; kHid = 512 = 64 x 8)
+    const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(A2 + row * kHid + 8 * lane);
     float s = 0.f;
-    for(int j = lane; j < kHid; j += 64) s += (float)A2[row * kHid + j] * w3[j];
+#pragma unroll
+    for(int e = 0; e < 8; ++e) s += (float)a[e] * w3[8 * lane + e];
     s = wave_sum(s) + b3[0];
     if(lane == 0) {
       y[row] = s;
@@ -385,7 +448,7 @@ void forward(hipStream_t st, Mlp& net, int64_t npad, bool keep_z = true) {
 
 template <int W>
 void featurize(hipStream_t st, Mlp& net, const uint64_t* dets, int64_t n, int64_t npad) {
-  hipLaunchKernelGGL(features_kernel<W>, dim3((npad + 1) / 2), dim3(256), 0, st, dets, n, npad,
+  hipLaunchKernelGGL(features_kernel<W>, dim3((npad + 15) / 16), dim3(256), 0, st, dets, n, npad,
                      net.X.as<__bf16>());
 }
 
