@@ -675,6 +675,7 @@ int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
       hv[p].V_red = d + n2 + 2 * n4 + n3;
       hv[p].G2_red = d + n2 + 2 * n4 + 2 * n3;
       hv[p].V2_red = d + 2 * n2 + 2 * n4 + 2 * n3;
+      hv[p].pair_sym = 0;  // read only by the product-space fill
     }
     MHIP_HIP_CHECK(hipMemcpyAsync(B->ints.p, packed.data(), 8 * total, hipMemcpyHostToDevice, ctx->stream));
     B->views.alloc(sizeof(IntsView) * ninst);

@@ -279,8 +279,21 @@ int mhip_set_integrals(mhip_context* ctx, int64_t norb, const double* T,
         G2[i + n * j] = 0.5 * G[idx(i, i, j, j)];
         V2[i + n * j] = V[idx(i, i, j, j)];
       }
+    // V(pq|rs) stored equal to V(rs|pq) bit for bit (real integrals as the
+    // FCIDUMP reader and the seeded generator produce them): element passes
+    // may then read the pair-swapped entry where it is the contiguous one
+    bool pair_sym = true;
+    for(size_t l = 0; l < n && pair_sym; ++l)
+      for(size_t k = 0; k < n && pair_sym; ++k)
+        for(size_t j = 0; j < n && pair_sym; ++j)
+          for(size_t i = 0; i < n; ++i)
+            if(std::memcmp(&V[idx(i, j, k, l)], &V[idx(k, l, i, j)], sizeof(double))) {
+              pair_sym = false;
+              break;
+            }
     Integrals& I = ctx->ints;
     I.norb = 0;
+    I.pair_sym = pair_sym;
     I.T.alloc(n2 * 8);
     I.V.alloc(n4 * 8);
     I.G.alloc(n4 * 8);

@@ -21,6 +21,7 @@ struct IntsView {
   const double* __restrict__ V_red;
   const double* __restrict__ G2_red;
   const double* __restrict__ V2_red;
+  int pair_sym;  // V(pq|rs) == V(rs|pq) bit for bit
 };
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }

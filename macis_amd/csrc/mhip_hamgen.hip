@@ -374,6 +374,7 @@ IntsView view(const Integrals& I) {
   v.V_red = I.V_red.as<double>();
   v.G2_red = I.G2_red.as<double>();
   v.V2_red = I.V2_red.as<double>();
+  v.pair_sym = I.pair_sym ? 1 : 0;
   return v;
 }
 

@@ -122,6 +122,7 @@ struct PinnedBuf {
 // Integral tables on the device (column-major, see macis_hip.h).
 struct Integrals {
   int64_t norb = 0;
+  bool pair_sym = false;  // V(pq|rs) == V(rs|pq) bit for bit
   DevBuf T, V, G, G_red, V_red, G2_red, V2_red;
 };
 

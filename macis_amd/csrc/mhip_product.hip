@@ -293,7 +293,12 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
             const uint32_t o1 = ac & 0xff, v1 = (ac >> 8) & 0xff;
             const uint32_t o2 = bc & 0xff, v2 = (bc >> 8) & 0xff;
             const double sign = code_sign(ac) * code_sign(bc);
-            hv = sign * I.V[v1 + I.n * (o1 + I.n * (v2 + I.n * o2))];
+            // outer alpha: (o1, v1) is fixed over the segment and the lanes
+            // vary (o2, v2); with pair symmetry the swapped entry
+            // V(v2,o2,v1,o1) keeps the lanes inside one n*n block
+            hv = sign * (outer_alpha && I.pair_sym
+                             ? I.V[v2 + I.n * (o2 + I.n * (v1 + I.n * o1))]
+                             : I.V[v1 + I.n * (o1 + I.n * (v2 + I.n * o2))]);
           }
         } else {
           const int32_t t2 = s2 + off;
