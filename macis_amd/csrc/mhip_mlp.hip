@@ -23,21 +23,28 @@ namespace mhip {
 namespace {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kIn = 128, kHid = 512;
 constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
 constexpr int kStageLd = 64 + 8;  // bf16 row pitch of the epilogue staging (16-byte aligned rows)
 
-__device__ __forceinline__ float gelu(float z) {
-  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return 0.5f * z * (1.f + tanhf(u));
+// GELU, tanh form: 0.5 z (1 + tanh u) with u = sqrt(2/pi) (z + 0.044715 z^3).
+// Since 0.5 (1 + tanh u) = 1 / (1 + e^(-2u)) = s, gelu(z) = z s and
+// gelu'(z) = s + z s (1 - s) 2 du/dz. s is formed from the hardware
+// exp2 and reciprocal (a libm tanhf per element made the epilogues of the
+// 2^20-row layers VALU bound); e^(-2u) -> inf gives s = 0, as it should.
+__device__ __forceinline__ float gelu_s(float z) {
+  constexpr float k2u = 2.f * 0.7978845608028654f * 1.4426950408889634f;  // 2 sqrt(2/pi) log2(e)
+  const float e = __builtin_amdgcn_exp2f(-k2u * z * (1.f + 0.044715f * z * z));
+  return __builtin_amdgcn_rcpf(1.f + e);
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
-  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  const float t = tanhf(u);
-  return 0.5f * (1.f + t) +
-         0.5f * z * (1.f - t * t) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * z * z);
+  const float sg = gelu_s(z);
+  return sg + z * sg * (1.f - sg) * (2.f * 0.7978845608028654f) *
+                  (1.f + 3.f * 0.044715f * z * z);
 }
 
 enum Epi : int {
@@ -45,6 +52,10 @@ enum Epi : int {
   EPI_BIAS_GELU = 1,  // Z = C + bias (fp32), A = gelu(Z) (bf16)
   EPI_GELU_GRAD = 2,  // D = C * gelu'(Zaux) (bf16)
   EPI_BIAS_GELU_A = 3,  // A = gelu(C + bias) (bf16) only: inference
+  // inference head folded into the last layer: zaux = w3, Cf = partial sums
+  // part[2 * n_block + wave_column][row] = sum over the wave's 64 columns of
+  // bf16(gelu(C + bias)) * w3 (ldc = row stride of part)
+  EPI_HEAD = 4,
 };
 
 // C[m][n] = sum_k A(m,k) B(n,k); A(m,k) = AK ? A[m*lda+k] : A[k*lda+m],
@@ -54,12 +65,23 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     int M, int N, int K, const __bf16* __restrict__ A, int lda, const __bf16* __restrict__ B,
     int ldb, const float* __restrict__ bias, const float* __restrict__ zaux,
     float* __restrict__ Cf, __bf16* __restrict__ Cb, int ldc) {
-  // A and B tiles; after the K loop the same LDS stages bf16 outputs
-  __shared__ __attribute__((aligned(16))) __bf16 smem[(BM + BN) * LDK];
-  __bf16(*As)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem);
-  __bf16(*Bs)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem + BM * LDK);
+  // two buffers of A and B tiles (the next K step is staged while this one
+  // feeds the MFMAs); after the K loop the same LDS stages bf16 outputs
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * (BM + BN) * LDK];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  // workgroups reach the 8 XCDs round-robin by linear id; hand each XCD a
+  // contiguous run of tiles so that the N / BN tiles sharing one A row tile
+  // run on the same XCD (one L2) at about the same time
+  int bx = blockIdx.x, by = blockIdx.y;
+  {
+    const int nbx = gridDim.x, nb = nbx * gridDim.y;
+    if((nb & 7) == 0) {
+      const int b = bx + nbx * by, l = (b & 7) * (nb >> 3) + (b >> 3);
+      bx = l % nbx;
+      by = l / nbx;
+    }
+  }
+  const int m0 = by * BM, n0 = bx * BN;
   const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
   // split K over gridDim.z (fp32 output only): slice z owns
   // [z*K/Z, (z+1)*K/Z) and writes its partial product at Cf + z*M*ldc
@@ -94,9 +116,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
                                                     (c & 15) * 8);
     }
   };
-  if(kbeg < kend) gload(kbeg);
-  for(int k0 = kbeg; k0 < kend; k0 += BK) {
-    // ---- stage A and B tiles (K-contiguous in LDS)
+  // registers -> LDS buffer (K-contiguous whatever the global layout)
+  auto lstore = [&](int buf) {
+    __bf16(*As)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem + buf * (BM + BN) * LDK);
+    __bf16(*Bs)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem + buf * (BM + BN) * LDK + BM * LDK);
 #pragma unroll
     for(int it = 0; it < 2; ++it) {
       const int c = tid + 256 * it;
@@ -115,8 +138,20 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
         for(int e = 0; e < 8; ++e) Bs[nc + e][kk] = rb[it][e];
       }
     }
+  };
+  int cur = 0;
+  if(kbeg < kend) {
+    gload(kbeg);
+    lstore(0);
     __syncthreads();
-    if(k0 + BK < kend) gload(k0 + BK);
+  }
+  for(int k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if(more) gload(k0 + BK);
+    const __bf16(*As)[LDK] =
+        reinterpret_cast<const __bf16(*)[LDK]>(smem + cur * (BM + BN) * LDK);
+    const __bf16(*Bs)[LDK] =
+        reinterpret_cast<const __bf16(*)[LDK]>(smem + cur * (BM + BN) * LDK + BM * LDK);
     // ---- MFMA: 2 x 2 tiles of 32 x 32, two K sub-steps of 16
 #pragma unroll
     for(int ks = 0; ks < BK; ks += 16) {
@@ -132,30 +167,70 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
       for(int i = 0; i < 2; ++i)
 #pragma unroll
         for(int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
     }
+    // the other buffer was last read before the previous barrier
+    if(more) lstore(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
-  // ---- epilogue; lane l, register r holds C(8*(r/4) + 4*(l/32) + r%4, l%32)
-  if constexpr(EPI == EPI_BIAS_GELU_A || EPI == EPI_GELU_GRAD) {
-    // bf16 outputs only: each wave stages 32 x 64 of its tile in LDS and
-    // stores whole 16-byte row pieces
-    static_assert(4 * 32 * kStageLd <= (BM + BN) * LDK, "staging exceeds the tile LDS");
-    __bf16* stg = smem + wv * 32 * kStageLd;
+  // ---- epilogue. The MFMAs take the B tile as their first operand, so lane
+  // l, register r of acc[i][j] holds C(row, col) with row = wm + 32 i + l%32
+  // and col = wn + 32 j + 8 (r/4) + 4 (l/32) + r%4: each group of four
+  // registers is four consecutive columns of one row.
+  const int rlane = m0 + wm + (lane & 31);
+  auto col_of = [&](int j, int q) { return n0 + wn + j * 32 + 8 * q + 4 * (lane >> 5); };
+  if constexpr(EPI == EPI_HEAD) {
 #pragma unroll
     for(int i = 0; i < 2; ++i) {
+      float s = 0.f;
 #pragma unroll
       for(int j = 0; j < 2; ++j)
 #pragma unroll
-        for(int r = 0; r < 16; ++r) {
-          const int lr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), lc = j * 32 + (lane & 31);
-          const int row = m0 + wm + i * 32 + lr, col = n0 + wn + lc;
-          float v = acc[i][j][r];
-          if constexpr(EPI == EPI_BIAS_GELU_A)
-            v = gelu(v + bias[col]);
-          else
-            v = v * gelu_grad(zaux[(size_t)row * ldc + col]);
-          stg[lr * kStageLd + lc] = (__bf16)v;
+        for(int q = 0; q < 4; ++q) {
+          const int col = col_of(j, q);
+          const float4 bv = *reinterpret_cast<const float4*>(bias + col);
+          const float4 wv3 = *reinterpret_cast<const float4*>(zaux + col);
+          s += (float)(__bf16)gelu(acc[i][j][4 * q + 0] + bv.x) * wv3.x;
+          s += (float)(__bf16)gelu(acc[i][j][4 * q + 1] + bv.y) * wv3.y;
+          s += (float)(__bf16)gelu(acc[i][j][4 * q + 2] + bv.z) * wv3.z;
+          s += (float)(__bf16)gelu(acc[i][j][4 * q + 3] + bv.w) * wv3.w;
+        }
+      s += __shfl_xor(s, 32, 64);  // the other half holds the row's other columns
+      if(lane < 32) Cf[(size_t)(2 * bx + (wv & 1)) * ldc + rlane + i * 32] = s;
+    }
+    return;
+  }
+  if constexpr(EPI == EPI_BIAS_GELU_A || EPI == EPI_GELU_GRAD) {
+    // bf16 outputs: each wave stages 32 x 64 of its tile in LDS (8-byte
+    // writes) and stores whole 16-byte row pieces
+    static_assert(4 * 32 * kStageLd <= 2 * (BM + BN) * LDK, "staging exceeds the tile LDS");
+    __bf16* stg = smem + wv * 32 * kStageLd;
+#pragma unroll
+    for(int i = 0; i < 2; ++i) {
+      const int row = rlane + i * 32;
+#pragma unroll
+      for(int j = 0; j < 2; ++j)
+#pragma unroll
+        for(int q = 0; q < 4; ++q) {
+          const int col = col_of(j, q);
+          float4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
+                      acc[i][j][4 * q + 3]};
+          if constexpr(EPI == EPI_BIAS_GELU_A) {
+            const float4 bv = *reinterpret_cast<const float4*>(bias + col);
+            v.x = gelu(v.x + bv.x);
+            v.y = gelu(v.y + bv.y);
+            v.z = gelu(v.z + bv.z);
+            v.w = gelu(v.w + bv.w);
+          } else {
+            const float4 zv = *reinterpret_cast<const float4*>(zaux + (size_t)row * ldc + col);
+            v.x *= gelu_grad(zv.x);
+            v.y *= gelu_grad(zv.y);
+            v.z *= gelu_grad(zv.z);
+            v.w *= gelu_grad(zv.w);
+          }
+          bf16x4_t o = {(__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+          *reinterpret_cast<bf16x4_t*>(stg + (lane & 31) * kStageLd + col - n0 - wn) = o;
         }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
@@ -177,22 +252,22 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
 #pragma unroll
     for(int j = 0; j < 2; ++j)
 #pragma unroll
-      for(int r = 0; r < 16; ++r) {
-        const int row = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int col = n0 + wn + j * 32 + (lane & 31);
+      for(int q = 0; q < 4; ++q) {
+        const int row = rlane + i * 32, col = col_of(j, q);
         const size_t o = (size_t)row * ldc + col;
-        float v = acc[i][j][r];
-        if constexpr(EPI == EPI_F32) {
-          Cf[o] = v;
-        } else if constexpr(EPI == EPI_BIAS_GELU) {
-          v += bias[col];
-          Cf[o] = v;
-          Cb[o] = (__bf16)gelu(v);
-        } else if constexpr(EPI == EPI_BIAS_GELU_A) {
-          Cb[o] = (__bf16)gelu(v + bias[col]);
-        } else {
-          Cb[o] = (__bf16)(v * gelu_grad(zaux[o]));
+        float4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
+                    acc[i][j][4 * q + 3]};
+        if constexpr(EPI == EPI_BIAS_GELU) {
+          const float4 bv = *reinterpret_cast<const float4*>(bias + col);
+          v.x += bv.x;
+          v.y += bv.y;
+          v.z += bv.z;
+          v.w += bv.w;
+          bf16x4_t g = {(__bf16)gelu(v.x), (__bf16)gelu(v.y), (__bf16)gelu(v.z),
+                        (__bf16)gelu(v.w)};
+          *reinterpret_cast<bf16x4_t*>(Cb + o) = g;
         }
+        *reinterpret_cast<float4*>(Cf + o) = v;
       }
 }
 
@@ -247,6 +322,17 @@ __global__ __launch_bounds__(256) void head_kernel(const __bf16* __restrict__ A2
     __syncthreads();
     if(threadIdx.x == 0) loss_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
   }
+}
+
+// Inference head: y = sum of the np column partials of the last layer's
+// fused epilogue (fixed order) + b3.
+__global__ void head_sum_kernel(const float* __restrict__ part, int np, int64_t ld,
+                                const float* __restrict__ b3, int64_t n, float* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  float s = part[i];
+  for(int p = 1; p < np; ++p) s += part[p * ld + i];
+  y[i] = s + b3[0];
 }
 
 // dZ2 = (dy * w3) * gelu'(Z2) as bf16.
@@ -440,9 +526,12 @@ void forward(hipStream_t st, Mlp& net, int64_t npad, bool keep_z = true) {
     gemm<true, true, EPI_BIAS_GELU_A>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
                                       bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr, nullptr,
                                       net.A1.as<__bf16>(), kHid);
-    gemm<true, true, EPI_BIAS_GELU_A>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
-                                      bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, nullptr, nullptr,
-                                      net.A2.as<__bf16>(), kHid);
+    // the head rides on the last layer's epilogue: per-wave column partials
+    // (2 per 128-column tile) go into A2's storage, which is far larger
+    static_assert(2 * (kHid / BN) * sizeof(float) <= kHid * sizeof(__bf16), "head partials");
+    gemm<true, true, EPI_HEAD>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
+                               bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, P + Mlp::ow3,
+                               net.A2.as<float>(), nullptr, (int)npad);
   }
 }
 
@@ -544,10 +633,9 @@ int mhip_mlp_predict(mhip_context* ctx, mhip_mlp* mlp, int words_per_det, const 
       else
         featurize<2>(ctx->stream, net, d_dets + 2 * s, nb, npad);
       forward(ctx->stream, net, npad, false);
-      const float* P = net.p.as<float>();
-      hipLaunchKernelGGL(head_kernel, dim3((nb + 3) / 4), dim3(256), 0, ctx->stream,
-                         net.A2.as<__bf16>(), P + Mlp::ow3, P + Mlp::ob3, nb, d_out + s, nullptr,
-                         nullptr, 0.f, nullptr);
+      hipLaunchKernelGGL(head_sum_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream,
+                         net.A2.as<float>(), 2 * (kHid / BN), npad,
+                         net.p.as<float>() + Mlp::ob3, nb, d_out + s);
       MHIP_HIP_CHECK(hipGetLastError());
     }
   });
