@@ -27,7 +27,7 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kIn = 128, kHid = 512;
-constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+constexpr int BN = 128, BK = 32, LDK = BK + 8;
 constexpr int kStageLd = 64 + 8;  // bf16 row pitch of the epilogue staging (16-byte aligned rows)
 
 // GELU, tanh form: 0.5 z (1 + tanh u) with u = sqrt(2/pi) (z + 0.044715 z^3).
@@ -37,14 +37,17 @@ constexpr int kStageLd = 64 + 8;  // bf16 row pitch of the epilogue staging (16-
 // 2^20-row layers VALU bound); e^(-2u) -> inf gives s = 0, as it should.
 __device__ __forceinline__ float gelu_s(float z) {
   constexpr float k2u = 2.f * 0.7978845608028654f * 1.4426950408889634f;  // 2 sqrt(2/pi) log2(e)
-  const float e = __builtin_amdgcn_exp2f(-k2u * z * (1.f + 0.044715f * z * z));
+  // the library builds without FP contraction; these FMAs are explicit
+  const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(z * z, -k2u * 0.044715f, -k2u) * z);
   return __builtin_amdgcn_rcpf(1.f + e);
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
+  constexpr float c = 2.f * 0.7978845608028654f;
   const float sg = gelu_s(z);
-  return sg + z * sg * (1.f - sg) * (2.f * 0.7978845608028654f) *
-                  (1.f + 3.f * 0.044715f * z * z);
+  const float w = __builtin_fmaf(-sg, sg, sg);                    // s (1 - s)
+  const float du = __builtin_fmaf(z * z, 3.f * 0.044715f * c, c);  // 2 du/dz
+  return __builtin_fmaf(z * w, du, sg);
 }
 
 enum Epi : int {
@@ -59,15 +62,19 @@ enum Epi : int {
 };
 
 // C[m][n] = sum_k A(m,k) B(n,k); A(m,k) = AK ? A[m*lda+k] : A[k*lda+m],
-// B(n,k) = BK ? B[n*ldb+k] : B[k*ldb+n]. M, N multiples of 128, K of 32.
-template <bool AK, bool BKC, int EPI>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(
+// B(n,k) = BK ? B[n*ldb+k] : B[k*ldb+n]. Four waves in a 2 x 2 layout, each
+// owning TI x 2 MFMA tiles of 32 x 32: block tile (64 TI) x 128, so M is a
+// multiple of 64 TI, N of 128, K of 32. TI = 4 reads 3/4 of the LDS operand
+// bytes per MFMA that TI = 2 does, at twice the accumulator registers.
+template <bool AK, bool BKC, int EPI, int TI = 2>
+__global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
     int M, int N, int K, const __bf16* __restrict__ A, int lda, const __bf16* __restrict__ B,
     int ldb, const float* __restrict__ bias, const float* __restrict__ zaux,
     float* __restrict__ Cf, __bf16* __restrict__ Cb, int ldc) {
   // two buffers of A and B tiles (the next K step is staged while this one
   // feeds the MFMAs); after the K loop the same LDS stages bf16 outputs
-  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * (BM + BN) * LDK];
+  constexpr int BMt = 64 * TI;
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * (BMt + BN) * LDK];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // workgroups reach the 8 XCDs round-robin by linear id; hand each XCD a
   // contiguous run of tiles so that the N / BN tiles sharing one A row tile
@@ -81,33 +88,59 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
       by = l / nbx;
     }
   }
-  const int m0 = by * BM, n0 = bx * BN;
-  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  const int m0 = by * BMt, n0 = bx * BN;
+  const int wm = (wv >> 1) * 32 * TI, wn = (wv & 1) * 64;
   // split K over gridDim.z (fp32 output only): slice z owns
   // [z*K/Z, (z+1)*K/Z) and writes its partial product at Cf + z*M*ldc
   const int kz = K / (int)gridDim.z, kbeg = (int)blockIdx.z * kz, kend = kbeg + kz;
   if constexpr(EPI == EPI_F32) Cf += (size_t)blockIdx.z * M * ldc;
-  f32x16_t acc[2][2];
+  // lane l, register r of acc[i][j] holds C(row, col) with row = wm + 32 i +
+  // l%32 and col = wn + 32 j + 8 (r/4) + 4 (l/32) + r%4 (the MFMAs take the
+  // B tile as their first operand): each group of four registers is four
+  // consecutive columns of one row. Epilogues with a bias start from it,
+  // except the TI = 4 head, where the early bias loads cost register spills
+  // and the epilogue adds it instead.
+  auto col_of = [&](int j, int q) { return n0 + wn + j * 32 + 8 * q + 4 * (lane >> 5); };
+  constexpr bool kBias = EPI == EPI_BIAS_GELU || EPI == EPI_BIAS_GELU_A;
+  f32x16_t acc[TI][2];
 #pragma unroll
-  for(int i = 0; i < 2; ++i)
+  for(int j = 0; j < 2; ++j)
 #pragma unroll
-    for(int j = 0; j < 2; ++j)
+    for(int q = 0; q < 4; ++q) {
+      float4 bv = {0.f, 0.f, 0.f, 0.f};
+      if constexpr(kBias) bv = *reinterpret_cast<const float4*>(bias + col_of(j, q));
 #pragma unroll
-      for(int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+      for(int i = 0; i < TI; ++i) {
+        acc[i][j][4 * q + 0] = bv.x;
+        acc[i][j][4 * q + 1] = bv.y;
+        acc[i][j][4 * q + 2] = bv.z;
+        acc[i][j][4 * q + 3] = bv.w;
+      }
+    }
 
-  // A and B pieces of one K step in registers: the next step's global loads
-  // are in flight while this step's MFMAs run
-  bf16x8_t ra[2], rb[2];
-  auto gload = [&](int k0) {
+  // A and B pieces of a K step in registers: later steps' global loads are
+  // in flight while this step's MFMAs run (one step ahead for TI = 2, two
+  // for TI = 4, whose fewer waves per SIMD hide less latency).
+  // 16-byte chunks: A tile BMt x 32 = TI per thread, B tile 128 x 32 = 2
+  constexpr int kAC = 8 * TI;  // A chunks per K row when A is M-contiguous
+  typedef bf16x8_t RegA[TI];
+  typedef bf16x8_t RegB[2];
+  RegA ra;
+  RegB rb;
+  auto gload = [&](int k0, RegA& ra, RegB& rb) {
 #pragma unroll
-    for(int it = 0; it < 2; ++it) {
+    for(int it = 0; it < TI; ++it) {
       const int c = tid + 256 * it;
       if constexpr(AK)
         ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(m0 + (c >> 2)) * lda + k0 +
                                                     (c & 3) * 8);
       else
-        ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(k0 + (c >> 4)) * lda + m0 +
-                                                    (c & 15) * 8);
+        ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(k0 + c / kAC) * lda + m0 +
+                                                    (c % kAC) * 8);
+    }
+#pragma unroll
+    for(int it = 0; it < 2; ++it) {
+      const int c = tid + 256 * it;
       if constexpr(BKC)
         rb[it] = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(n0 + (c >> 2)) * ldb + k0 +
                                                     (c & 3) * 8);
@@ -117,19 +150,24 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     }
   };
   // registers -> LDS buffer (K-contiguous whatever the global layout)
-  auto lstore = [&](int buf) {
-    __bf16(*As)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem + buf * (BM + BN) * LDK);
-    __bf16(*Bs)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem + buf * (BM + BN) * LDK + BM * LDK);
+  auto lstore = [&](int buf, const RegA& ra, const RegB& rb) {
+    __bf16(*As)[LDK] = reinterpret_cast<__bf16(*)[LDK]>(smem + buf * (BMt + BN) * LDK);
+    __bf16(*Bs)[LDK] =
+        reinterpret_cast<__bf16(*)[LDK]>(smem + buf * (BMt + BN) * LDK + BMt * LDK);
 #pragma unroll
-    for(int it = 0; it < 2; ++it) {
+    for(int it = 0; it < TI; ++it) {
       const int c = tid + 256 * it;
       if constexpr(AK) {
         *reinterpret_cast<bf16x8_t*>(&As[c >> 2][(c & 3) * 8]) = ra[it];
       } else {
-        const int kk = c >> 4, mc = (c & 15) * 8;
+        const int kk = c / kAC, mc = (c % kAC) * 8;
 #pragma unroll
         for(int e = 0; e < 8; ++e) As[mc + e][kk] = ra[it][e];
       }
+    }
+#pragma unroll
+    for(int it = 0; it < 2; ++it) {
+      const int c = tid + 256 * it;
       if constexpr(BKC) {
         *reinterpret_cast<bf16x8_t*>(&Bs[c >> 2][(c & 3) * 8]) = rb[it];
       } else {
@@ -139,75 +177,106 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
       }
     }
   };
-  int cur = 0;
-  if(kbeg < kend) {
-    gload(kbeg);
-    lstore(0);
-    __syncthreads();
-  }
-  for(int k0 = kbeg; k0 < kend; k0 += BK) {
-    const bool more = k0 + BK < kend;
-    if(more) gload(k0 + BK);
+  // ---- MFMA on LDS buffer buf: TI x 2 tiles of 32 x 32, two K sub-steps of 16
+  auto mfma_step = [&](int buf) {
     const __bf16(*As)[LDK] =
-        reinterpret_cast<const __bf16(*)[LDK]>(smem + cur * (BM + BN) * LDK);
+        reinterpret_cast<const __bf16(*)[LDK]>(smem + buf * (BMt + BN) * LDK);
     const __bf16(*Bs)[LDK] =
-        reinterpret_cast<const __bf16(*)[LDK]>(smem + cur * (BM + BN) * LDK + BM * LDK);
-    // ---- MFMA: 2 x 2 tiles of 32 x 32, two K sub-steps of 16
+        reinterpret_cast<const __bf16(*)[LDK]>(smem + buf * (BMt + BN) * LDK + BMt * LDK);
 #pragma unroll
     for(int ks = 0; ks < BK; ks += 16) {
-      bf16x8_t a[2], b[2];
+      bf16x8_t a[TI], b[2];
       const int kq = ks + (lane >> 5) * 8;
 #pragma unroll
-      for(int i = 0; i < 2; ++i)
+      for(int i = 0; i < TI; ++i)
         a[i] = *reinterpret_cast<const bf16x8_t*>(&As[wm + i * 32 + (lane & 31)][kq]);
 #pragma unroll
       for(int j = 0; j < 2; ++j)
         b[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[wn + j * 32 + (lane & 31)][kq]);
 #pragma unroll
-      for(int i = 0; i < 2; ++i)
+      for(int i = 0; i < TI; ++i)
 #pragma unroll
         for(int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
     }
-    // the other buffer was last read before the previous barrier
-    if(more) lstore(cur ^ 1);
-    __syncthreads();
-    cur ^= 1;
+  };
+  if constexpr(TI == 4) {
+    // step k uses LDS buffer k % 2; while it runs, step k + 1 waits in
+    // registers and step k + 2 is being loaded into the set step k used
+    RegA ra1;
+    RegB rb1;
+    if(kbeg < kend) {
+      gload(kbeg, ra, rb);
+      if(kbeg + BK < kend) gload(kbeg + BK, ra1, rb1);
+      lstore(0, ra, rb);
+      __syncthreads();
+    }
+    for(int k0 = kbeg; k0 < kend;) {
+      if(k0 + 2 * BK < kend) gload(k0 + 2 * BK, ra, rb);
+      mfma_step(0);
+      // buffer 1 was last read before the previous barrier
+      if(k0 + BK < kend) lstore(1, ra1, rb1);
+      __syncthreads();
+      k0 += BK;
+      if(k0 >= kend) break;
+      if(k0 + 2 * BK < kend) gload(k0 + 2 * BK, ra1, rb1);
+      mfma_step(1);
+      if(k0 + BK < kend) lstore(0, ra, rb);
+      __syncthreads();
+      k0 += BK;
+    }
+  } else {
+    int cur = 0;
+    if(kbeg < kend) {
+      gload(kbeg, ra, rb);
+      lstore(0, ra, rb);
+      __syncthreads();
+    }
+    for(int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if(more) gload(k0 + BK, ra, rb);
+      mfma_step(cur);
+      // the other buffer was last read before the previous barrier
+      if(more) lstore(cur ^ 1, ra, rb);
+      __syncthreads();
+      cur ^= 1;
+    }
   }
-  // ---- epilogue. The MFMAs take the B tile as their first operand, so lane
-  // l, register r of acc[i][j] holds C(row, col) with row = wm + 32 i + l%32
-  // and col = wn + 32 j + 8 (r/4) + 4 (l/32) + r%4: each group of four
-  // registers is four consecutive columns of one row.
+  // ---- epilogue
   const int rlane = m0 + wm + (lane & 31);
-  auto col_of = [&](int j, int q) { return n0 + wn + j * 32 + 8 * q + 4 * (lane >> 5); };
   if constexpr(EPI == EPI_HEAD) {
+    // each bias / w3 piece serves all TI row tiles before the next is loaded
+    float s[TI];
 #pragma unroll
-    for(int i = 0; i < 2; ++i) {
-      float s = 0.f;
+    for(int i = 0; i < TI; ++i) s[i] = 0.f;
 #pragma unroll
-      for(int j = 0; j < 2; ++j)
+    for(int j = 0; j < 2; ++j)
 #pragma unroll
-        for(int q = 0; q < 4; ++q) {
-          const int col = col_of(j, q);
-          const float4 bv = *reinterpret_cast<const float4*>(bias + col);
-          const float4 wv3 = *reinterpret_cast<const float4*>(zaux + col);
-          s += (float)(__bf16)gelu(acc[i][j][4 * q + 0] + bv.x) * wv3.x;
-          s += (float)(__bf16)gelu(acc[i][j][4 * q + 1] + bv.y) * wv3.y;
-          s += (float)(__bf16)gelu(acc[i][j][4 * q + 2] + bv.z) * wv3.z;
-          s += (float)(__bf16)gelu(acc[i][j][4 * q + 3] + bv.w) * wv3.w;
+      for(int q = 0; q < 4; ++q) {
+        const float4 bv = *reinterpret_cast<const float4*>(bias + col_of(j, q));
+        const float4 wv3 = *reinterpret_cast<const float4*>(zaux + col_of(j, q));
+#pragma unroll
+        for(int i = 0; i < TI; ++i) {
+          s[i] += (float)(__bf16)gelu(acc[i][j][4 * q + 0] + bv.x) * wv3.x;
+          s[i] += (float)(__bf16)gelu(acc[i][j][4 * q + 1] + bv.y) * wv3.y;
+          s[i] += (float)(__bf16)gelu(acc[i][j][4 * q + 2] + bv.z) * wv3.z;
+          s[i] += (float)(__bf16)gelu(acc[i][j][4 * q + 3] + bv.w) * wv3.w;
         }
-      s += __shfl_xor(s, 32, 64);  // the other half holds the row's other columns
-      if(lane < 32) Cf[(size_t)(2 * bx + (wv & 1)) * ldc + rlane + i * 32] = s;
+      }
+#pragma unroll
+    for(int i = 0; i < TI; ++i) {
+      s[i] += __shfl_xor(s[i], 32, 64);  // the other half holds the row's other columns
+      if(lane < 32) Cf[(size_t)(2 * bx + (wv & 1)) * ldc + rlane + i * 32] = s[i];
     }
     return;
   }
   if constexpr(EPI == EPI_BIAS_GELU_A || EPI == EPI_GELU_GRAD) {
     // bf16 outputs: each wave stages 32 x 64 of its tile in LDS (8-byte
     // writes) and stores whole 16-byte row pieces
-    static_assert(4 * 32 * kStageLd <= 2 * (BM + BN) * LDK, "staging exceeds the tile LDS");
+    static_assert(4 * 32 * kStageLd <= 2 * (BMt + BN) * LDK, "staging exceeds the tile LDS");
     __bf16* stg = smem + wv * 32 * kStageLd;
 #pragma unroll
-    for(int i = 0; i < 2; ++i) {
+    for(int i = 0; i < TI; ++i) {
       const int row = rlane + i * 32;
 #pragma unroll
       for(int j = 0; j < 2; ++j)
@@ -217,11 +286,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
           float4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
                       acc[i][j][4 * q + 3]};
           if constexpr(EPI == EPI_BIAS_GELU_A) {
-            const float4 bv = *reinterpret_cast<const float4*>(bias + col);
-            v.x = gelu(v.x + bv.x);
-            v.y = gelu(v.y + bv.y);
-            v.z = gelu(v.z + bv.z);
-            v.w = gelu(v.w + bv.w);
+            v.x = gelu(v.x);
+            v.y = gelu(v.y);
+            v.z = gelu(v.z);
+            v.w = gelu(v.w);
           } else {
             const float4 zv = *reinterpret_cast<const float4*>(zaux + (size_t)row * ldc + col);
             v.x *= gelu_grad(zv.x);
@@ -248,7 +316,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     return;
   }
 #pragma unroll
-  for(int i = 0; i < 2; ++i)
+  for(int i = 0; i < TI; ++i)
 #pragma unroll
     for(int j = 0; j < 2; ++j)
 #pragma unroll
@@ -258,11 +326,6 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
         float4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
                     acc[i][j][4 * q + 3]};
         if constexpr(EPI == EPI_BIAS_GELU) {
-          const float4 bv = *reinterpret_cast<const float4*>(bias + col);
-          v.x += bv.x;
-          v.y += bv.y;
-          v.z += bv.z;
-          v.w += bv.w;
           bf16x4_t g = {(__bf16)gelu(v.x), (__bf16)gelu(v.y), (__bf16)gelu(v.z),
                         (__bf16)gelu(v.w)};
           *reinterpret_cast<bf16x4_t*>(Cb + o) = g;
@@ -470,14 +533,15 @@ struct mhip_mlp {
 namespace mhip {
 namespace {
 
-template <bool AK, bool BKC, int EPI>
+template <bool AK, bool BKC, int EPI, int TI = 2>
 void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
           const float* bias, const float* zaux, float* Cf, __bf16* Cb, int ldc, int splits = 1) {
-  MHIP_REQUIRE(M % BM == 0 && N % BN == 0 && K % BK == 0, MHIP_ERR_INVALID_ARG,
-               "gemm dims must be multiples of 128/128/32");
+  MHIP_REQUIRE(M % (64 * TI) == 0 && N % BN == 0 && K % BK == 0, MHIP_ERR_INVALID_ARG,
+               "gemm dims must be multiples of the block tile and of 32 in K");
   MHIP_REQUIRE(splits >= 1 && K % (splits * BK) == 0 && (splits == 1 || EPI == EPI_F32),
                MHIP_ERR_INVALID_ARG, "bad split-K");
-  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BKC, EPI>), dim3(N / BN, M / BM, splits), dim3(256), 0,
+  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BKC, EPI, TI>), dim3(N / BN, M / (64 * TI), splits),
+                     dim3(256), 0,
                      st, M, N, K, A, lda, B, ldb, bias, zaux, Cf, Cb, ldc);
   MHIP_HIP_CHECK(hipGetLastError());
 }
@@ -529,9 +593,9 @@ void forward(hipStream_t st, Mlp& net, int64_t npad, bool keep_z = true) {
     // the head rides on the last layer's epilogue: per-wave column partials
     // (2 per 128-column tile) go into A2's storage, which is far larger
     static_assert(2 * (kHid / BN) * sizeof(float) <= kHid * sizeof(__bf16), "head partials");
-    gemm<true, true, EPI_HEAD>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
-                               bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, P + Mlp::ow3,
-                               net.A2.as<float>(), nullptr, (int)npad);
+    gemm<true, true, EPI_HEAD, 4>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
+                                  bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, P + Mlp::ow3,
+                                  net.A2.as<float>(), nullptr, (int)npad);
   }
 }
 
@@ -624,10 +688,11 @@ int mhip_mlp_predict(mhip_context* ctx, mhip_mlp* mlp, int words_per_det, const 
     MHIP_REQUIRE(mlp && (n == 0 || (d_dets && d_out)), MHIP_ERR_INVALID_ARG, "null argument");
     MHIP_REQUIRE(words_per_det == 1 || words_per_det == 2, MHIP_ERR_UNSUPPORTED, "bad W");
     Mlp& net = mlp->net;
+    // rows padded to the 256-row tile of the last layer's GEMM
     const int64_t chunk = 1 << 20;
-    net.reserve(std::min<int64_t>((n + 127) / 128 * 128, chunk));
+    net.reserve(std::min<int64_t>((n + 255) / 256 * 256, chunk));
     for(int64_t s = 0; s < n; s += chunk) {
-      const int64_t nb = std::min(chunk, n - s), npad = (nb + 127) / 128 * 128;
+      const int64_t nb = std::min(chunk, n - s), npad = (nb + 255) / 256 * 256;
       if(words_per_det == 1)
         featurize<1>(ctx->stream, net, d_dets + s, nb, npad);
       else
