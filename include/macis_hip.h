@@ -174,6 +174,21 @@ int mhip_ham_build(mhip_context* ctx, int words_per_det,
 int mhip_ham_diagonal(mhip_context* ctx, int words_per_det,
                       const uint64_t* d_dets, int64_t ndets, double* d_diag);
 
+/* form_rdms (double_loop.hpp:116-165, rdm_contributions util/rdms.hpp:15-132):
+ * ADDS the spin-summed 1-RDM ordm(p,q) = d_ordm[p + norb q] and 2-RDM
+ * trdm(p,q,r,s) = d_trdm[p + norb q + norb^2 r + norb^3 s] (column-major, as
+ * the reference's matrix_span / rank4_span) of the pairs bra i x ket j with
+ * popcount(bra ^ ket) <= 4 and |C[i] C[j]| > 1e-16, norb from
+ * mhip_set_integrals. One coefficient array serves both sides, as in the
+ * reference, so d_C holds max(nbra, nket) values. d_trdm may be NULL (1-RDM
+ * only, with the reference's branch order: same-spin doubles then reach the
+ * diagonal branch). All arrays are DEVICE arrays; the caller zeroes the
+ * outputs. Sums use FP64 atomics: equal to the reference's up to summation
+ * order. */
+int mhip_form_rdms(mhip_context* ctx, int words_per_det, const uint64_t* d_bra,
+                   int64_t nbra, const uint64_t* d_ket, int64_t nket,
+                   const double* d_C, double* d_ordm, double* d_trdm);
+
 /* ---- CSR handles ----------------------------------------------------------- */
 int mhip_csr_info(const mhip_csr* csr, int64_t* m, int64_t* n, int64_t* nnz);
 /* Device arrays of the handle (borrowed; valid until mhip_csr_destroy). */

@@ -102,6 +102,7 @@ _SIGS = {
     "mhip_set_integrals": (_i32, [_p, _i64, _p, _p]),
     "mhip_ham_build": (_i32, [_p, _i32, _p, _i64, _p, _i64, _d, _pp]),
     "mhip_ham_diagonal": (_i32, [_p, _i32, _p, _i64, _p]),
+    "mhip_form_rdms": (_i32, [_p, _i32, _p, _i64, _p, _i64, _p, _p, _p]),
     "mhip_csr_info": (_i32, [_p, _pi64, _pi64, _pi64]),
     "mhip_csr_device_arrays": (_i32, [_p, _pp, _pp, _pp]),
     "mhip_csr_copy_to_host_i32": (_i32, [_p, _p, _p, _p, _p]),
@@ -320,6 +321,14 @@ class Context:
 
     def ham_diagonal(self, W, d_dets, n, d_out):
         check(lib().mhip_ham_diagonal(self.h, W, _ptr(d_dets), n, _ptr(d_out)), "ham_diagonal")
+
+    def form_rdms(self, W, d_bra, nbra, d_C, d_ordm, d_trdm=None, d_ket=None, nket=None):
+        """Adds the 1-RDM (and 2-RDM when d_trdm is given) into device arrays."""
+        d_ket = d_bra if d_ket is None else d_ket
+        nket = nbra if nket is None else nket
+        check(lib().mhip_form_rdms(self.h, W, _ptr(d_bra), nbra, _ptr(d_ket), nket, _ptr(d_C),
+                                   _ptr(d_ordm), _ptr(d_trdm) if d_trdm is not None else None),
+              "form_rdms")
 
     def csr_from_host(self, m, n, rowptr, colind, nzval, indexing=0):
         rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)

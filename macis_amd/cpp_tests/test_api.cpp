@@ -128,6 +128,63 @@ static void test_host_generator_elements(const H2O& h) {
 }
 
 // ---------------------------------------------------------------- GPU cases
+static void test_rdms(const H2O& h, const std::string& dir) {
+  // double_loop.cxx:162-177: the HF determinant's RDMs give E_HF
+  const size_t n = h.norb, n2 = n * n, n4 = n2 * n2;
+  auto T = h.T;
+  auto V = h.V;
+  macis::HipHamiltonianGenerator<64> gen(macis::matrix_span<double>(T.data(), n, n),
+                                         macis::rank4_span<double>(V.data(), n, n, n, n));
+  auto hf = macis::canonical_hf_determinant<64>(5, 5);
+  std::vector<macis::wfn_t<64>> one{hf};
+  std::vector<double> C1{1.}, ordm(n2, 0.), trdm(n4, 0.);
+  gen.form_rdms(one.begin(), one.end(), one.begin(), one.end(), C1.data(),
+                macis::matrix_span<double>(ordm.data(), n, n),
+                macis::rank4_span<double>(trdm.data(), n, n, n, n));
+  double E = 0.;
+  for(size_t i = 0; i < n2; ++i) E += ordm[i] * T[i];
+  for(size_t i = 0; i < n4; ++i) E += trdm[i] * V[i];
+  REQUIRE_NEAR(E, gen.matrix_element(hf, hf), 1e-10);
+
+  // CAS(8e,8o) with one inactive orbital against the reference's fixture
+  // h2o.ccpvdz.cas88.rdms.bin (read as in fcidump.cxx:209-251)
+  const size_t na = 8, ni = 1;
+  std::vector<double> Ta(na * na), Va(na * na * na * na);
+  auto v = [&](size_t p, size_t q, size_t r, size_t s) { return V[p + n * (q + n * (r + n * s))]; };
+  for(size_t q = 0; q < na; ++q)
+    for(size_t p = 0; p < na; ++p) {
+      double t = T[(p + ni) + n * (q + ni)];
+      for(size_t i = 0; i < ni; ++i) t += 2. * v(p + ni, q + ni, i, i) - v(p + ni, i, i, q + ni);
+      Ta[p + na * q] = t;
+      for(size_t s = 0; s < na; ++s)
+        for(size_t r = 0; r < na; ++r)
+          Va[p + na * (q + na * (r + na * s))] = v(p + ni, q + ni, r + ni, s + ni);
+    }
+  macis::HipHamiltonianGenerator<64> cas(macis::matrix_span<double>(Ta.data(), na, na),
+                                         macis::rank4_span<double>(Va.data(), na, na, na, na));
+  auto dets = macis::generate_hilbert_space<64>(na, 4, 4);
+  std::vector<double> X;
+  macis::selected_ci_diag<64, int32_t>(dets.begin(), dets.end(), cas, 1e-16, 20, 1e-8, X);
+  std::vector<double> d1(na * na, 0.), d2(na * na * na * na, 0.);
+  cas.form_rdms(dets.begin(), dets.end(), dets.begin(), dets.end(), X.data(),
+                macis::matrix_span<double>(d1.data(), na, na),
+                macis::rank4_span<double>(d2.data(), na, na, na, na));
+  std::ifstream in(dir + "/h2o.ccpvdz.cas88.rdms.bin", std::ios::binary);
+  int nread = 0;
+  in.read(reinterpret_cast<char*>(&nread), sizeof(int));
+  REQUIRE(nread == (int)na);
+  std::vector<double> f1(d1.size()), f2(d2.size());
+  in.read(reinterpret_cast<char*>(f1.data()), f1.size() * sizeof(double));
+  in.read(reinterpret_cast<char*>(f2.data()), f2.size() * sizeof(double));
+  REQUIRE(bool(in));
+  double e1 = 0., e2 = 0.;
+  for(size_t i = 0; i < d1.size(); ++i) e1 = std::max(e1, std::abs(d1[i] - f1[i]));
+  for(size_t i = 0; i < d2.size(); ++i) e2 = std::max(e2, std::abs(d2[i] - f2[i]));
+  std::printf("  CAS(8,8) RDMs vs fixture: max |dD1| %.2e, max |dD2| %.2e\n", e1, e2);
+  REQUIRE(e1 < 1e-7);
+  REQUIRE(e2 < 1e-7);
+}
+
 static void test_csr_hamiltonian(const H2O& h, const std::string& dir) {
   // csr_hamiltonian.cxx:18-71
   auto T = h.T;
@@ -323,6 +380,7 @@ int main(int argc, char** argv) {
     cases.push_back({"CSR Hamiltonian (device)", [&] { test_csr_hamiltonian(h, dir); }});
     cases.push_back({"spmbv (device)", [] { test_spmbv(); }});
     cases.push_back({"Davidson (device)", [&] { test_davidson(h); }});
+    cases.push_back({"RDMs (device)", [&] { test_rdms(h, dir); }});
     cases.push_back({"ASCI grow/refine (device)", [&] { test_asci(h); }});
     cases.push_back({"learned ASCI pre-screen (device)", [&] { test_learned_prescreen(h); }});
   }

@@ -5,6 +5,7 @@
 // mhip_ham_build; the *_bit_ overrides copy it back into the reference's
 // host csr_matrix, device_csr() keeps it in HBM.
 #pragma once
+#include <algorithm>
 #include <limits>
 
 #include <macis/hamiltonian_generator.hpp>
@@ -62,10 +63,31 @@ class HipHamiltonianGenerator : public HamiltonianGenerator<N> {
     return sparse_matrix_type<int64_t>(H.m(), H.n(), std::move(rp), std::move(ci), std::move(nz));
   }
 
-  // RDMs are outside the accelerated path.
-  void form_rdms(full_det_iterator, full_det_iterator, full_det_iterator, full_det_iterator,
-                 double*, matrix_span_t, rank4_span_t) override {
-    throw std::runtime_error("HipHamiltonianGenerator::form_rdms is not implemented");
+  // form_rdms (double_loop.hpp:116-165) on the device through mhip_form_rdms:
+  // the spans are added into, as the reference does, and a null 2-RDM handle
+  // forms the 1-RDM alone. C weights bra i with C[i] and ket j with C[j].
+  void form_rdms(full_det_iterator bra_begin, full_det_iterator bra_end,
+                 full_det_iterator ket_begin, full_det_iterator ket_end, double* C,
+                 matrix_span_t ordm, rank4_span_t trdm) override {
+    const size_t n = this->norb_;
+    const size_t nb = std::distance(bra_begin, bra_end), nk = std::distance(ket_begin, ket_end);
+    if(!nb || !nk) return;
+    if(ordm.extent(0) != n || ordm.extent(1) != n)
+      throw std::runtime_error("form_rdms: the 1-RDM span must be norb x norb");
+    const bool two = trdm.data_handle() != nullptr;
+    if(two)
+      for(size_t d = 0; d < 4; ++d)
+        if(trdm.extent(d) != n) throw std::runtime_error("form_rdms: the 2-RDM span must be norb^4");
+    auto db = hip::upload_dets<N>(*ctx_, bra_begin, bra_end);
+    auto dk = hip::upload_dets<N>(*ctx_, ket_begin, ket_end);
+    hip::DeviceBuffer<double> dC(*ctx_, C, std::max(nb, nk));
+    hip::DeviceBuffer<double> d1(*ctx_, ordm.data_handle(), n * n);
+    hip::DeviceBuffer<double> d2;
+    if(two) d2 = hip::DeviceBuffer<double>(*ctx_, trdm.data_handle(), n * n * n * n);
+    hip::check(mhip_form_rdms(ctx_->get(), W, db.data(), nb, dk.data(), nk, dC.data(), d1.data(),
+                              two ? d2.data() : nullptr));
+    d1.download(ordm.data_handle(), n * n);
+    if(two) d2.download(trdm.data_handle(), n * n * n * n);
   }
 
  protected:

@@ -18,7 +18,10 @@
  *     pattern symmetry;
  *   - gespmbv: the 6x6 known answer;
  *   - Davidson: H2O CISD E0 fixture and dense eigensolves;
- *   - ASCI grow/refine: asci.cxx energies; search: brute-force rescoring.
+ *   - ASCI grow/refine: asci.cxx energies; search: brute-force rescoring;
+ *   - form_rdms: h2o.ccpvdz.cas88.rdms.bin fixture (entry by entry), second
+ *     quantisation on small spaces, the HF identities and traces of
+ *     tests/double_loop.cxx:180-223, tr(D1 T) + tr(D2 V) = C^T H C.
  */
 #include "oracle.h"
 
@@ -1310,6 +1313,134 @@ int orc_asci_refine(const orc_ham* h, int W, const orc_asci_settings* s,
   }
   return ORC_ERR_REFINE;
 }
+
+/* ===================================================================== */
+/* RDMs: form_rdms (double_loop.hpp:116-165) + rdm_contributions          */
+/* (util/rdms.hpp:15-132). ordm(p,q) = ordm[p + n q], trdm(p,q,r,s) =     */
+/* trdm[p + n q + n^2 r + n^3 s] (column-major spans, types.hpp:24-33);   */
+/* both are accumulated into (+=), as the reference does.                 */
+/* ===================================================================== */
+#define ORC_T4(p, q, r, s) trdm[(p) + n * ((q) + n * ((r) + n * (int64_t)(s)))]
+
+/* rdm_contributions_4 (util/rdms.hpp:15-26) */
+static void rdm_4(int64_t n, uint64_t bra, uint64_t ket, uint64_t ex, double val,
+                  double* trdm) {
+  uint32_t o1, v1, o2, v2;
+  const double sign = orc_doubles_sign_indices(bra, ket, ex, &o1, &v1, &o2, &v2);
+  val *= sign * 0.5;
+  ORC_T4(v1, o1, v2, o2) += val;
+  ORC_T4(v2, o1, v1, o2) -= val;
+  ORC_T4(v1, o2, v2, o1) -= val;
+  ORC_T4(v2, o2, v1, o1) += val;
+}
+
+/* single_excitation_sign_indices (sd_operations.hpp:516-525) */
+static double single_sign_indices(uint64_t bra, uint64_t ket, uint64_t ex, uint32_t* o,
+                                  uint32_t* v) {
+  *o = ffs0(ket & ex);
+  *v = ffs0(bra & ex);
+  return orc_single_excitation_sign(ket, *v, *o);
+}
+
+/* rdm_contributions_22 (util/rdms.hpp:28-42) */
+static void rdm_22(int64_t n, det_t bra, det_t ket, uint64_t exa, uint64_t exb,
+                   double val, double* trdm) {
+  uint32_t o1, v1, o2, v2;
+  const double sa = single_sign_indices(bra.a, ket.a, exa, &o1, &v1);
+  const double sb = single_sign_indices(bra.b, ket.b, exb, &o2, &v2);
+  val *= sa * sb * 0.5;
+  ORC_T4(v1, o1, v2, o2) += val;
+  ORC_T4(v2, o2, v1, o1) += val;
+}
+
+/* rdm_contributions_2 (util/rdms.hpp:44-67): `same` / `othr` are the bra's
+ * occupied orbitals of the excited spin and of the other spin. */
+static void rdm_2(int64_t n, uint64_t bra, uint64_t ket, uint64_t ex, const uint32_t* same,
+                  int nsame, const uint32_t* othr, int nothr, double val, double* ordm,
+                  double* trdm) {
+  uint32_t o1, v1;
+  const double sign = single_sign_indices(bra, ket, ex, &o1, &v1);
+  ordm[v1 + n * o1] += sign * val;
+  if(trdm) {
+    val *= sign * 0.5;
+    for(int k = 0; k < nsame; ++k) {
+      const uint32_t p = same[k];
+      ORC_T4(v1, o1, p, p) += val;
+      ORC_T4(p, p, v1, o1) += val;
+      ORC_T4(v1, p, p, o1) -= val;
+      ORC_T4(p, o1, v1, p) -= val;
+    }
+    for(int k = 0; k < nothr; ++k) {
+      const uint32_t p = othr[k];
+      ORC_T4(v1, o1, p, p) += val;
+      ORC_T4(p, p, v1, o1) += val;
+    }
+  }
+}
+
+/* rdm_contributions_diag (util/rdms.hpp:69-101) */
+static void rdm_diag(int64_t n, const uint32_t* oa, int na, const uint32_t* ob, int nb,
+                     double val, double* ordm, double* trdm) {
+  for(int k = 0; k < na; ++k) ordm[oa[k] * (n + 1)] += val;
+  for(int k = 0; k < nb; ++k) ordm[ob[k] * (n + 1)] += val;
+  if(!trdm) return;
+  val *= 0.5;
+  for(int kq = 0; kq < na; ++kq)
+    for(int kp = 0; kp < na; ++kp) {
+      const uint32_t p = oa[kp], q = oa[kq];
+      ORC_T4(p, p, q, q) += val;
+      ORC_T4(p, q, p, q) -= val;
+    }
+  for(int kq = 0; kq < nb; ++kq)
+    for(int kp = 0; kp < nb; ++kp) {
+      const uint32_t p = ob[kp], q = ob[kq];
+      ORC_T4(p, p, q, q) += val;
+      ORC_T4(p, q, p, q) -= val;
+    }
+  for(int kq = 0; kq < nb; ++kq)
+    for(int kp = 0; kp < na; ++kp) {
+      const uint32_t p = oa[kp], q = ob[kq];
+      ORC_T4(p, p, q, q) += val;
+      ORC_T4(q, q, p, p) += val;
+    }
+}
+
+/* form_rdms (double_loop.hpp:116-165) with the dispatch of
+ * rdm_contributions (util/rdms.hpp:103-130). C[i] weights bra i and C[j]
+ * ket j (one array, as the reference indexes it); trdm may be NULL. */
+void orc_form_rdms(int64_t norb, int W, const uint64_t* bra, int64_t nbra,
+                   const uint64_t* ket, int64_t nket, const double* C, double* ordm,
+                   double* trdm) {
+  const int64_t n = norb;
+  uint32_t oa[64], ob[64];
+  for(int64_t i = 0; i < nbra; ++i) {
+    const det_t b = load_det(W, bra + i * W);
+    if(!(b.a | b.b)) continue;
+    const int na = bits_to_indices(b.a, oa), nb = bits_to_indices(b.b, ob);
+    for(int64_t j = 0; j < nket; ++j) {
+      const det_t k = load_det(W, ket + j * W);
+      if(!(k.a | k.b)) continue;
+      const uint64_t exa = b.a ^ k.a, exb = b.b ^ k.b;
+      const int ca = popc(exa), cb = popc(exb);
+      if(ca + cb > 4) continue;
+      const double val = C[i] * C[j];
+      if(!(fabs(val) > 1e-16)) continue;
+      if(ca == 4 && trdm)
+        rdm_4(n, b.a, k.a, exa, val, trdm);
+      else if(cb == 4 && trdm)
+        rdm_4(n, b.b, k.b, exb, val, trdm);
+      else if(ca == 2 && cb == 2 && trdm)
+        rdm_22(n, b, k, exa, exb, val, trdm);
+      else if(ca == 2)
+        rdm_2(n, b.a, k.a, exa, oa, na, ob, nb, val, ordm, trdm);
+      else if(cb == 2)
+        rdm_2(n, b.b, k.b, exb, ob, nb, oa, na, val, ordm, trdm);
+      else
+        rdm_diag(n, oa, na, ob, nb, val, ordm, trdm);
+    }
+  }
+}
+#undef ORC_T4
 
 /* ===================================================================== */
 /* MLP pre-screen oracle (no reference counterpart)                        */

@@ -212,6 +212,11 @@ extern "C" {
  * weights, activations and back-propagated pre-activations are rounded to
  * bf16 where the device kernels round them. Returns the mean squared error
  * (0 without targets); grad (nparam) and pred (n) are optional. */
+/* form_rdms (double_loop.hpp:116-165); accumulates into ordm / trdm
+ * (column-major, norb^2 / norb^4); trdm may be NULL. */
+void orc_form_rdms(int64_t norb, int W, const uint64_t* bra, int64_t nbra,
+                   const uint64_t* ket, int64_t nket, const double* C, double* ordm,
+                   double* trdm);
 int64_t orc_mlp_num_params(void);
 double orc_mlp_loss_grad(const float* P, int W, const uint64_t* dets,
                          const float* target, int64_t n, int bf16_points,

@@ -56,6 +56,7 @@ def lib():
             "orc_csr_rows": (_i32, [_p, _i32, _p, _i64, _i64, _p, _i64, _d, _p, _p, _p]),
             "orc_csr_rows_parallel": (_i32, [_p, _i32, _p, _i64, _i64, _p, _i64, _d, _p, _p, _p]),
             "orc_num_threads": (_i32, []),
+            "orc_form_rdms": (None, [_i64, _i32, _p, _i64, _p, _i64, _p, _p, _p]),
             "orc_free": (None, [_p]),
             "orc_gespmbv": (None, [_i64, _d, _i64, _p, _p, _p, _i64, _p, _i64, _d, _p, _i64]),
             "orc_extract_diagonal": (None, [_i64, _p, _p, _p, _i64, _p]),
@@ -308,6 +309,23 @@ class Ham:
                     matel_tol=np.finfo(float).eps):
         return self._grow_or_refine(lib().orc_asci_refine, W, settings, E0, wfn, X,
                                     max_sub, res_tol, matel_tol)
+
+
+def form_rdms(norb, W, bra, C, ket=None, want_trdm=True, ordm=None, trdm=None):
+    """form_rdms (double_loop.hpp:116-165): spin-summed 1-RDM (norb, norb)
+    and 2-RDM (norb,)*4, both column-major (Fortran-ordered views), added
+    into ordm / trdm when given."""
+    bra = np.ascontiguousarray(bra, dtype=np.uint64)
+    ket = bra if ket is None else np.ascontiguousarray(ket, dtype=np.uint64)
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    n = int(norb)
+    o = np.zeros(n * n) if ordm is None else np.ascontiguousarray(ordm).reshape(-1, order="F").copy()
+    t = None
+    if want_trdm:
+        t = np.zeros(n ** 4) if trdm is None else np.ascontiguousarray(trdm).reshape(-1, order="F").copy()
+    lib().orc_form_rdms(n, W, _ptr(bra), bra.size // W, _ptr(ket), ket.size // W, _ptr(C), _ptr(o),
+                        _ptr(t) if t is not None else None)
+    return o.reshape((n, n), order="F"), (t.reshape((n,) * 4, order="F") if t is not None else None)
 
 
 def single_excitation_sign(state, p, q):
