@@ -141,6 +141,27 @@ def cpu_spmv_baseline(ctx, T, V, d_dets, N, reps=3):
     }
 
 
+def rdm_timing(ctx, T, V, d_dets, N, nnz, d_X, E):
+    """Spin-summed 1- and 2-RDMs of the CAS(12e,12o) ground state left by the
+    last timed solve (form_rdms on the device, connected pairs = the
+    Hamiltonian's pattern), timed after the timed region; the energy from
+    the RDMs checks them."""
+    n = N_ORB
+    d1 = ctx.empty(n * n, np.float64).zero()
+    d2 = ctx.empty(n ** 4, np.float64).zero()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.form_rdms(1, d_dets, N, d_X, d1, d2)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    o, t = d1.numpy(), d2.numpy()
+    E_rdm = float(o @ T + t @ V)
+    return {"ms": ms, "pairs": nnz, "pairs_per_s": nnz / (ms * 1e-3),
+            "E_from_rdms": E_rdm, "dE": E_rdm - E,
+            "trace_D1": float(np.trace(o.reshape(n, n, order="F"))),
+            "note": "includes the structure pass that lists the pairs; FP64 atomics"}
+
+
 def mlp_throughput(C, ctx):
     """Forward (prediction) and training-step throughput of the pre-screen
     network on random 128-bit occupation strings."""
@@ -382,6 +403,7 @@ def main():
             "traffic": SPMV_TRAFFIC_GB * 1e9 if SPMV_TRAFFIC_GB and sliced else None,
         },
     }
+    out["rdms"] = rdm_timing(ctx, T, V, d_dets, N, nnz, d_X, E)
     if not args.no_asci:
         out["mlp"] = mlp_throughput(C, ctx)
         out["asci"] = asci_step_comparison(C, ctx)
