@@ -300,6 +300,56 @@ static void test_asci(const H2O& h) {
   std::printf("  refined, E0 = %.12f\n", E0);
 }
 
+static void test_asci_rot(const H2O& h, double ref_E0) {
+  // asci_grow with grow_with_rot (grow.hpp:67-160): RDMs on the device, natural
+  // orbitals, rotated integrals re-uploaded, re-diagonalisation. ref_E0 (when
+  // given) is the oracle's orc_asci_grow_rot energy for the same settings.
+  auto T = h.T;
+  auto V = h.V;
+  macis::HipHamiltonianGenerator<64> gen(macis::matrix_span<double>(T.data(), h.norb, h.norb),
+                                         macis::rank4_span<double>(V.data(), h.norb, h.norb,
+                                                                   h.norb, h.norb));
+  macis::ASCISettings asci_settings;
+  macis::MCSCFSettings mcscf_settings;
+  asci_settings.ntdets_max = 2000;
+  asci_settings.grow_with_rot = true;
+  asci_settings.rot_size_start = 100;
+  std::vector<macis::wfn_t<64>> dets = {macis::canonical_hf_determinant<64>(5, 5)};
+  std::vector<double> C = {1.0};
+  double E0 = gen.matrix_element(dets[0], dets[0]);
+  std::tie(E0, dets, C) = macis::asci_grow(asci_settings, mcscf_settings, E0, std::move(dets),
+                                           std::move(C), gen, h.norb);
+  std::printf("  grow_with_rot E0 = %.12f (%zu determinants)\n", E0, dets.size());
+  REQUIRE(dets.size() == 2000);
+  if(ref_E0 != 0.) REQUIRE_NEAR(E0, ref_E0, 1e-8);
+  // the returned vector is the ground state of the same determinants in the
+  // rotated orbitals (the last rotation follows the last selection, so this
+  // energy need not be below E0): its Rayleigh quotient with the rotated
+  // generator equals a fresh solve's energy
+  std::vector<double> Cfresh;
+  const double Efresh = macis::selected_ci_diag<64, int32_t>(dets.begin(), dets.end(), gen,
+                                                             mcscf_settings.ci_matel_tol,
+                                                             mcscf_settings.ci_max_subspace,
+                                                             mcscf_settings.ci_res_tol, Cfresh);
+  auto H = macis::make_csr_hamiltonian<int32_t>(dets.begin(), dets.end(), gen, 1e-16);
+  std::vector<double> HC(C.size());
+  sparsexx::spblas::gespmbv(1, 1., H, C.data(), H.n(), 0., HC.data(), H.n());
+  double e = 0., n2 = 0.;
+  for(size_t i = 0; i < C.size(); ++i) {
+    e += C[i] * HC[i];
+    n2 += C[i] * C[i];
+  }
+  std::printf("  re-diagonalised energy in natural orbitals %.12f\n", e / n2);
+  REQUIRE_NEAR(n2, 1.0, 1e-10);
+  REQUIRE_NEAR(e / n2, Efresh, 1e-8);
+  // the caller's integrals were rotated in place and T stays symmetric
+  double asym = 0.;
+  for(size_t p = 0; p < h.norb; ++p)
+    for(size_t q = 0; q < h.norb; ++q) asym = std::max(asym, std::abs(T[p + h.norb * q] - T[q + h.norb * p]));
+  REQUIRE(asym < 1e-12);
+  REQUIRE(std::abs(T[0] - h.T[0]) > 1e-8);
+}
+
 static void test_learned_prescreen(const H2O& h) {
   // LearnedPrescreen: plain steps HF -> 100 -> 800 -> 6400 are observed, the
   // network is trained on them, and the screened search of the next
@@ -370,7 +420,13 @@ int main(int argc, char** argv) {
     return 2;
   }
   const std::string dir = argv[1];
-  const bool cpu_only = argc > 2 && std::string(argv[2]) == "--cpu-only";
+  bool cpu_only = false;
+  double rot_ref = 0.;  // --rot-ref=<E0 of the oracle's rotating grow>
+  for(int a = 2; a < argc; ++a) {
+    const std::string arg = argv[a];
+    if(arg == "--cpu-only") cpu_only = true;
+    if(arg.rfind("--rot-ref=", 0) == 0) rot_ref = std::stod(arg.substr(10));
+  }
   auto h = load_h2o(dir);
   std::vector<std::pair<std::string, std::function<void()>>> cases = {
       {"sd_operations", [] { test_sd_operations(); }},
@@ -382,6 +438,8 @@ int main(int argc, char** argv) {
     cases.push_back({"Davidson (device)", [&] { test_davidson(h); }});
     cases.push_back({"RDMs (device)", [&] { test_rdms(h, dir); }});
     cases.push_back({"ASCI grow/refine (device)", [&] { test_asci(h); }});
+    cases.push_back({"ASCI grow with natural-orbital rotation (device)",
+                     [&] { test_asci_rot(h, rot_ref); }});
     cases.push_back({"learned ASCI pre-screen (device)", [&] { test_learned_prescreen(h); }});
   }
   for(auto& [name, fn] : cases) {

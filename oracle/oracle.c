@@ -134,6 +134,8 @@ struct orc_ham {
   double* V2_red;
 };
 
+static void ham_intermediates(orc_ham* h);
+
 orc_ham* orc_ham_create(int64_t n, const double* T, const double* V) {
   /* HamiltonianGenerator ctor + generate_integral_intermediates
    * (impl.hpp:14-70) */
@@ -156,6 +158,14 @@ orc_ham* orc_ham_create(int64_t n, const double* T, const double* V) {
   }
   memcpy(h->T, T, sizeof(double) * n2);
   memcpy(h->V, V, sizeof(double) * n4);
+  ham_intermediates(h);
+  return h;
+}
+
+/* generate_integral_intermediates (impl.hpp:25-70) from h->V */
+static void ham_intermediates(orc_ham* h) {
+  const int64_t n = h->n, n2 = n * n, n3 = n2 * n;
+  const double* V = h->V;
 #define IDX4(i, j, k, l) ((i) + n * (j) + n2 * (k) + n3 * (l))
   /* G(i,j,k,l) = V(i,j,k,l) - V(i,l,k,j)   (:36-44) */
   for(int64_t i = 0; i < n; ++i)
@@ -177,7 +187,6 @@ orc_ham* orc_ham_create(int64_t n, const double* T, const double* V) {
       h->V2_red[i + n * j] = V[IDX4(i, i, j, j)];
     }
 #undef IDX4
-  return h;
 }
 
 void orc_ham_destroy(orc_ham* h) {
@@ -1312,6 +1321,105 @@ int orc_asci_refine(const orc_ham* h, int W, const orc_asci_settings* s,
     if(fabs(dE) < s->refine_energy_tol) return ORC_OK;
   }
   return ORC_ERR_REFINE;
+}
+
+/* ===================================================================== */
+/* Orbital transforms (transform.cxx:22-115), square case norb_new ==     */
+/* norb_old = n, leading dimensions n; Y may alias X.                     */
+/* ===================================================================== */
+/* One quarter: out(c, p) = sum_i in(i, c) U(i, p) for c < ncols -- the
+ * contracted index leaves the front and the new one joins at the back. */
+static void quarter(int64_t n, int64_t ncols, const double* in, const double* U, double* out) {
+  for(int64_t p = 0; p < n; ++p)
+    for(int64_t c = 0; c < ncols; ++c) {
+      double acc = 0.;
+      for(int64_t i = 0; i < n; ++i) acc += in[i + n * c] * U[i + n * p];
+      out[c + ncols * p] = acc;
+    }
+}
+
+/* Y(p,q) = U(i,p) X(i,j) U(j,q) */
+void orc_two_index_transform(int64_t n, const double* X, const double* U, double* Y) {
+  double* t = (double*)malloc(sizeof(double) * n * n);
+  double* y = (double*)malloc(sizeof(double) * n * n);
+  quarter(n, n, X, U, t); /* t(j,p) */
+  quarter(n, n, t, U, y); /* y(p,q) */
+  memcpy(Y, y, sizeof(double) * n * n);
+  free(t);
+  free(y);
+}
+
+/* Y(p,q,r,s) = X(i,j,k,l) U(i,p) U(j,q) U(k,r) U(l,s): four quarters
+ * (i,j,k,l) -> (j,k,l,p) -> (k,l,p,q) -> (l,p,q,r) -> (p,q,r,s) */
+void orc_four_index_transform(int64_t n, const double* X, const double* U, double* Y) {
+  const int64_t n3 = n * n * n, n4 = n3 * n;
+  double* a = (double*)malloc(sizeof(double) * n4);
+  double* b = (double*)malloc(sizeof(double) * n4);
+  quarter(n, n3, X, U, a);
+  quarter(n, n3, a, U, b);
+  quarter(n, n3, b, U, a);
+  quarter(n, n3, a, U, b);
+  memcpy(Y, b, sizeof(double) * n4);
+  free(a);
+  free(b);
+}
+
+/* Natural-orbital rotation of grow.hpp:73-114: RDMs of (wfn, X), natural
+ * orbitals = eigenvectors of -ordm (ascending, i.e. occupations
+ * descending), T and V rotated in place, intermediates regenerated. */
+int orc_natural_orbital_rotation(orc_ham* h, int W, const uint64_t* wfn, int64_t nwfn,
+                                 const double* X, double* occupations) {
+  const int64_t n = h->n, n2 = n * n, n4 = n2 * n2;
+  double* ordm = (double*)calloc(n2, sizeof(double));
+  double* trdm = (double*)calloc(n4, sizeof(double));
+  double* ons = (double*)malloc(sizeof(double) * n);
+  if(!ordm || !trdm || !ons) {
+    free(ordm);
+    free(trdm);
+    free(ons);
+    return ORC_ERR_NOMEM;
+  }
+  orc_form_rdms(n, W, wfn, nwfn, wfn, nwfn, X, ordm, trdm);
+  for(int64_t i = 0; i < n2; ++i) ordm[i] *= -1.0;
+  orc_syev_lower(n, ordm, n, ons);
+  for(int64_t i = 0; i < n; ++i) ons[i] *= -1.0;
+  if(occupations) memcpy(occupations, ons, sizeof(double) * n);
+  orc_two_index_transform(n, h->T, ordm, h->T);
+  orc_four_index_transform(n, h->V, ordm, h->V);
+  ham_intermediates(h);
+  free(ordm);
+  free(trdm);
+  free(ons);
+  return ORC_OK;
+}
+
+/* asci_grow (grow.hpp:16-173) with grow_with_rot: after each asci_iter that
+ * leaves at least rot_size_start determinants, rotate to natural orbitals
+ * and re-diagonalise the same determinants; E0 takes the asci_iter energy
+ * (:162). The rotated integrals stay in h. */
+int orc_asci_grow_rot(orc_ham* h, int W, const orc_asci_settings* s, int64_t ci_max_subspace,
+                      double ci_res_tol, double ci_matel_tol, double* E0, uint64_t* wfn,
+                      double* X, int64_t* nwfn, int64_t norb, int64_t rot_size_start) {
+  while(*nwfn < s->ntdets_max) {
+    int64_t nd = *nwfn * s->grow_factor;
+    if(nd < s->ntdets_min) nd = s->ntdets_min;
+    if(nd > s->ntdets_max) nd = s->ntdets_max;
+    double E;
+    int rc = orc_asci_iter(h, W, s, ci_max_subspace, ci_res_tol, ci_matel_tol, nd, *E0, wfn, X,
+                           nwfn, norb, &E);
+    if(rc) return rc;
+    if(nd > *nwfn) return ORC_ERR_DID_NOT_GROW;
+    if(*nwfn >= rot_size_start) {
+      rc = orc_natural_orbital_rotation(h, W, wfn, *nwfn, X, NULL);
+      if(rc) return rc;
+      double Er;
+      rc = orc_selected_ci_diag(h, W, wfn, *nwfn, ci_matel_tol, ci_max_subspace, ci_res_tol, X,
+                                &Er, NULL);
+      if(rc) return rc;
+    }
+    *E0 = E;
+  }
+  return ORC_OK;
 }
 
 /* ===================================================================== */

@@ -212,6 +212,18 @@ extern "C" {
  * weights, activations and back-propagated pre-activations are rounded to
  * bf16 where the device kernels round them. Returns the mean squared error
  * (0 without targets); grad (nparam) and pred (n) are optional. */
+/* two_index_transform / four_index_transform (transform.cxx:22-115),
+ * square (norb_new == norb_old == n), leading dimension n; Y may alias X. */
+void orc_two_index_transform(int64_t n, const double* X, const double* U, double* Y);
+void orc_four_index_transform(int64_t n, const double* X, const double* U, double* Y);
+/* Natural-orbital rotation of asci_grow (grow.hpp:73-114) applied to h in
+ * place; occupations (descending) written when non-NULL. */
+int orc_natural_orbital_rotation(orc_ham* h, int W, const uint64_t* wfn, int64_t nwfn,
+                                 const double* X, double* occupations);
+/* asci_grow with grow_with_rot (grow.hpp:16-173); h is rotated in place. */
+int orc_asci_grow_rot(orc_ham* h, int W, const orc_asci_settings* s, int64_t ci_max_subspace,
+                      double ci_res_tol, double ci_matel_tol, double* E0, uint64_t* wfn,
+                      double* X, int64_t* nwfn, int64_t norb, int64_t rot_size_start);
 /* form_rdms (double_loop.hpp:116-165); accumulates into ordm / trdm
  * (column-major, norb^2 / norb^4); trdm may be NULL. */
 void orc_form_rdms(int64_t norb, int W, const uint64_t* bra, int64_t nbra,

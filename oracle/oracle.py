@@ -72,6 +72,10 @@ def lib():
             "orc_asci_iter": (_i32, [_p, _i32, _p, _i64, _d, _d, _i64, _d, _p, _p, _p, _i64, _p]),
             "orc_asci_grow": (_i32, [_p, _i32, _p, _i64, _d, _d, _p, _p, _p, _p, _i64]),
             "orc_asci_refine": (_i32, [_p, _i32, _p, _i64, _d, _d, _p, _p, _p, _p, _i64]),
+            "orc_asci_grow_rot": (_i32, [_p, _i32, _p, _i64, _d, _d, _p, _p, _p, _p, _i64, _i64]),
+            "orc_natural_orbital_rotation": (_i32, [_p, _i32, _p, _i64, _p, _p]),
+            "orc_two_index_transform": (None, [_i64, _p, _p, _p]),
+            "orc_four_index_transform": (None, [_i64, _p, _p, _p]),
             "orc_mlp_num_params": (_i64, []),
             "orc_mlp_loss_grad": (_d, [_p, _i32, _p, _p, _i64, _i32, _p, _p]),
         }
@@ -309,6 +313,43 @@ class Ham:
                     matel_tol=np.finfo(float).eps):
         return self._grow_or_refine(lib().orc_asci_refine, W, settings, E0, wfn, X,
                                     max_sub, res_tol, matel_tol)
+
+    def asci_grow_rot(self, W, settings, E0, wfn, X, rot_size_start=1000, max_sub=20,
+                      res_tol=1e-8, matel_tol=np.finfo(float).eps):
+        """asci_grow with grow_with_rot (grow.hpp:16-173); rotates this Ham."""
+        def fn(*a):
+            return lib().orc_asci_grow_rot(*a, rot_size_start)
+        return self._grow_or_refine(fn, W, settings, E0, wfn, X, max_sub, res_tol, matel_tol)
+
+    def natural_orbital_rotation(self, W, wfn, X):
+        """Rotate this Ham to the natural orbitals of (wfn, X); occupations."""
+        w = np.ascontiguousarray(wfn, dtype=np.uint64)
+        x = np.ascontiguousarray(X, dtype=np.float64)
+        occ = np.zeros(self.norb)
+        rc = lib().orc_natural_orbital_rotation(self._h, W, _ptr(w), w.size // W, _ptr(x), _ptr(occ))
+        if rc:
+            raise RuntimeError(f"rotation rc={rc}")
+        return occ
+
+
+def two_index_transform(X, U):
+    """Y(p,q) = U(i,p) X(i,j) U(j,q), flat column-major (transform.cxx:22-49)."""
+    n = U.shape[0]
+    x = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+    u = np.asfortranarray(U, dtype=np.float64).reshape(-1, order="F")
+    y = np.empty(n * n)
+    lib().orc_two_index_transform(n, _ptr(x), _ptr(u), _ptr(y))
+    return y
+
+
+def four_index_transform(X, U):
+    """Y(p,q,r,s) = X(i,j,k,l) U(i,p) U(j,q) U(k,r) U(l,s) (transform.cxx:51-115)."""
+    n = U.shape[0]
+    x = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+    u = np.asfortranarray(U, dtype=np.float64).reshape(-1, order="F")
+    y = np.empty(n ** 4)
+    lib().orc_four_index_transform(n, _ptr(x), _ptr(u), _ptr(y))
+    return y
 
 
 def form_rdms(norb, W, bra, C, ket=None, want_trdm=True, ordm=None, trdm=None):
