@@ -21,7 +21,9 @@
 #include <macis/sd_operations.hpp>
 #include <macis/solvers/davidson.hpp>
 #include <macis/solvers/selected_ci_diag.hpp>
+#include <macis/util/detail/rdm_files.hpp>
 #include <macis/util/fcidump.hpp>
+#include <macis/wavefunction_io.hpp>
 #include <sparsexx/spblas/spmbv.hpp>
 #include <sparsexx/util/submatrix.hpp>
 
@@ -123,6 +125,59 @@ static void test_host_generator_elements(const H2O& h) {
     g.make_csr_hamiltonian_block<int16_t>(d.begin(), d.end(), d.begin(), d.end(), 0.);
   } catch(const std::runtime_error& e) {
     threw = std::string(e.what()) == "Unsupported index_t";
+  }
+  REQUIRE(threw);
+}
+
+static void test_file_formats(const std::string& dir) {
+  // sd_operations.cxx:240-260: canonical strings
+  auto state = macis::canonical_hf_determinant<64>(2, 2).flip(3).flip(4 + 32);
+  REQUIRE(macis::to_canonical_string(state) == "220ud000000000000000000000000000");
+  REQUIRE(macis::from_canonical_string<64>("220ud000000000000000000000000000") == state);
+  REQUIRE(macis::from_canonical_string<64>("220ud") == state);
+  // wavefunction files: write then read gives the same determinants and
+  // coefficients bit for bit (17 significant digits)
+  auto dets = macis::generate_hilbert_space<128>(6, 3, 2);
+  std::vector<double> C(dets.size());
+  for(size_t i = 0; i < C.size(); ++i) C[i] = std::sin(1.0 + i) / (1.0 + i);
+  const std::string wf = "/tmp/macis_amd_wfn_test.dat";
+  macis::write_wavefunction(wf, 6, dets, C);
+  std::vector<std::bitset<128>> d2;
+  std::vector<double> C2;
+  macis::read_wavefunction(wf, d2, C2);
+  REQUIRE(d2 == dets);
+  REQUIRE(C2 == C);
+  std::ifstream f(wf);
+  std::string head, first;
+  std::getline(f, head);
+  std::getline(f, first);
+  REQUIRE(head == std::to_string(dets.size()) + " 6 3 2");
+  char ref[64];
+  std::snprintf(ref, sizeof ref, "%30.16e ", C[0]);
+  REQUIRE(first == std::string(ref) + macis::to_canonical_string(dets[0]) + " ");
+  std::remove(wf.c_str());
+  // binary RDMs: the reference's water CAS(8,8) file, read into padded
+  // arrays and written back byte for byte
+  const size_t n = 8, ld1 = 9, ld2 = 10;
+  std::vector<double> D1(ld1 * n, -1.), D2(ld2 * ld2 * ld2 * n, -1.);
+  macis::read_rdms_binary(dir + "/h2o.ccpvdz.cas88.rdms.bin", n, D1.data(), ld1, D2.data(), ld2);
+  double tr = 0.;
+  for(size_t p = 0; p < n; ++p) tr += D1[p * (ld1 + 1)];
+  REQUIRE_NEAR(tr, 8.0, 1e-12);
+  REQUIRE(D1[n] == -1.);  // padding beyond norb untouched (fcidump.cxx:212-222)
+  const std::string rf = "/tmp/macis_amd_rdm_test.bin";
+  macis::write_rdms_binary(rf, n, D1.data(), ld1, D2.data(), ld2);
+  auto slurp = [](const std::string& fn) {
+    std::ifstream in(fn, std::ios::binary);
+    return std::string(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+  };
+  REQUIRE(slurp(rf) == slurp(dir + "/h2o.ccpvdz.cas88.rdms.bin"));
+  std::remove(rf.c_str());
+  bool threw = false;
+  try {
+    macis::read_rdms_binary(dir + "/h2o.ccpvdz.cas88.rdms.bin", 7, D1.data(), ld1, D2.data(), ld2);
+  } catch(const std::runtime_error& e) {
+    threw = std::string(e.what()) == "NORB in RDM file doesn't match 7 8";
   }
   REQUIRE(threw);
 }
@@ -431,6 +486,7 @@ int main(int argc, char** argv) {
   std::vector<std::pair<std::string, std::function<void()>>> cases = {
       {"sd_operations", [] { test_sd_operations(); }},
       {"host generator elements", [&] { test_host_generator_elements(h); }},
+      {"file formats (wavefunction, binary RDMs)", [&] { test_file_formats(dir); }},
   };
   if(!cpu_only) {
     cases.push_back({"CSR Hamiltonian (device)", [&] { test_csr_hamiltonian(h, dir); }});

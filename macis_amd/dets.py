@@ -50,3 +50,31 @@ def split_spins(dets: np.ndarray, W: int):
     if W == 1:
         return d[:, 0] & np.uint64(0xFFFFFFFF), d[:, 0] >> np.uint64(32)
     return d[:, 0].copy(), d[:, 1].copy()
+
+
+def to_canonical_string(det, W: int) -> str:
+    """to_canonical_string (sd_operations.hpp:596-613): one character per
+    orbital of the spin-string width (32 W), '2', 'u', 'd' or '0'."""
+    d = np.asarray(det, dtype=np.uint64).reshape(-1)
+    if W == 1:
+        a, b, width = int(d[0]) & 0xFFFFFFFF, int(d[0]) >> 32, 32
+    else:
+        a, b, width = int(d[0]), int(d[1]), 64
+    out = []
+    for i in range(width):
+        x, y = (a >> i) & 1, (b >> i) & 1
+        out.append("2" if x and y else "u" if x else "d" if y else "0")
+    return "".join(out)
+
+
+def from_canonical_string(s: str, W: int) -> np.ndarray:
+    """from_canonical_string (sd_operations.hpp:615-630) -> one det (W words);
+    orbitals past the string are empty."""
+    width = 32 * W
+    if len(s) > width:
+        raise ValueError("canonical string longer than the spin-string width")
+    a = sum(1 << i for i, c in enumerate(s) if c in "2u")
+    b = sum(1 << i for i, c in enumerate(s) if c in "2d")
+    if W == 1:
+        return np.array([a | (b << 32)], dtype=np.uint64)
+    return np.array([a, b], dtype=np.uint64)

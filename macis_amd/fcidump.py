@@ -57,3 +57,23 @@ def read_fcidump(fname):
             T[p + n * q] = val
             T[q + n * p] = val
     return norb, E_core, T, V
+
+
+def read_rdms_binary(fname, norb):
+    """read_rdms_binary (fcidump.cxx:209-251): int32 norb, then the 1-RDM
+    (norb^2) and 2-RDM (norb^4) column-major doubles -> (D1, D2) flat."""
+    with open(fname, "rb") as f:
+        n = int(np.frombuffer(f.read(4), dtype=np.int32)[0])
+        if n != norb:
+            raise RuntimeError(f"NORB in RDM file doesn't match {norb} {n}")
+        d1 = np.frombuffer(f.read(8 * n * n), dtype=np.float64).copy()
+        d2 = np.frombuffer(f.read(8 * n ** 4), dtype=np.float64).copy()
+    return d1, d2
+
+
+def write_rdms_binary(fname, norb, D1, D2):
+    """write_rdms_binary (fcidump.cxx:253-280), flat column-major inputs."""
+    with open(fname, "wb") as f:
+        f.write(np.int32(norb).tobytes())
+        f.write(np.ascontiguousarray(D1, dtype=np.float64).reshape(-1)[: norb * norb].tobytes())
+        f.write(np.ascontiguousarray(D2, dtype=np.float64).reshape(-1)[: norb ** 4].tobytes())

@@ -3,6 +3,7 @@
 #pragma once
 #include <algorithm>
 #include <bitset>
+#include <string>
 #include <vector>
 
 #include <macis/bitset_operations.hpp>
@@ -106,6 +107,34 @@ std::vector<std::bitset<N>> generate_cisd_hilbert_space(size_t norb, std::bitset
   for(auto& x : sa)
     for(auto& y : sb) out.push_back(full(x, y));
   return out;
+}
+
+// to_canonical_string / from_canonical_string (sd_operations.hpp:596-630):
+// one character per spatial orbital of the spin-string width, '2' doubly
+// occupied, 'u' alpha only, 'd' beta only, '0' empty; a shorter string
+// leaves the remaining orbitals empty.
+template <size_t N>
+std::string to_canonical_string(std::bitset<N> state) {
+  static_assert((N % 2) == 0, "N Odd");
+  std::string str;
+  for(size_t i = 0; i < N / 2; ++i) {
+    const bool a = state[i], b = state[i + N / 2];
+    str.push_back(a && b ? '2' : a ? 'u' : b ? 'd' : '0');
+  }
+  return str;
+}
+
+template <size_t N>
+std::bitset<N> from_canonical_string(std::string str) {
+  // both halves are built N bits wide and combined as alpha | beta << N/2,
+  // as the reference does (a character past N/2 lands in the beta half or,
+  // past N, throws std::out_of_range from bitset::set)
+  std::bitset<N> alpha, beta;
+  for(size_t i = 0; i < str.length(); ++i) {
+    if(str[i] == '2' || str[i] == 'u') alpha.set(i);
+    if(str[i] == '2' || str[i] == 'd') beta.set(i);
+  }
+  return alpha | (beta << (N / 2));
 }
 
 }  // namespace macis
