@@ -183,8 +183,9 @@ int mhip_ham_diagonal(mhip_context* ctx, int words_per_det,
  * reference, so d_C holds max(nbra, nket) values. d_trdm may be NULL (1-RDM
  * only, with the reference's branch order: same-spin doubles then reach the
  * diagonal branch). All arrays are DEVICE arrays; the caller zeroes the
- * outputs. Sums use FP64 atomics: equal to the reference's up to summation
- * order. */
+ * outputs, which must be ordinary (coarse-grained) device memory such as
+ * mhip_malloc returns: the sums use hardware FP64 atomics, equal to the
+ * reference's up to summation order. */
 int mhip_form_rdms(mhip_context* ctx, int words_per_det, const uint64_t* d_bra,
                    int64_t nbra, const uint64_t* d_ket, int64_t nket,
                    const double* d_C, double* d_ordm, double* d_trdm);

@@ -13,10 +13,12 @@
 //   2. count   : one wavefront handles R consecutive bra rows (bras held in
 //                scalar registers); lanes screen 64 superblocks / 64 tiles at
 //                a time, then test 64 kets per step with XOR + popcount and
-//                count hits with a wavefront ballot.
+//                count hits with a wavefront ballot. By default the same walk
+//                also stages the columns in pooled chunks (kStage).
 //   3. scan    : device exclusive scan of the row counts -> row pointers.
-//   4. fill    : the same walk writes ascending column indices at
-//                rowptr + running count + ballot prefix (mbcnt).
+//   4. fill    : a gather of the staged chunks into the CSR, or (pool
+//                overflow) the same walk again writing ascending column
+//                indices at rowptr + running count + ballot prefix (mbcnt).
 //   5. element : one wavefront per row evaluates the Slater-Condon element
 //                of each stored column (full lane utilisation) and counts
 //                |h| > thresh.
@@ -98,13 +100,32 @@ __device__ __forceinline__ int excitation_bound(const uint64_t (&b)[W],
   return lb;
 }
 
-// Structural count (FILL = false) or fill (FILL = true) pass.
+// Pass modes of the tiled scan. kStage appends every row's columns, in
+// scan order (ascending), to chunks of kChunk entries drawn from a device
+// pool with one vector atomic per chunk; a row's chunks form a chain
+// (head -> next ... -> -1, `used` entries each) that stage_gather_kernel
+// copies into the CSR once the row counts are scanned. One scan instead of
+// two; when the pool runs out the pass reports it and the build falls back
+// to count + fill.
+enum ScanMode : int { kCount = 0, kFill = 1, kStage = 2 };
+constexpr int kChunk = 128;
+struct StagePool {
+  int32_t* col;                // cap * kChunk entries
+  int64_t cap;                 // chunks
+  unsigned long long* next;    // chunk allocator
+  int64_t* head;               // first chunk of each row, -1 when empty
+  int64_t* chunk_next;         // next chunk of the row, -1 at its end
+  uint32_t* used;              // entries in each chunk
+  int* overflow;               // set when a chunk was refused
+};
+
+// Structural count (kCount), fill (kFill) or staged (kStage) pass.
 // Without a work-item table, wave w owns rows [w*R, w*R+R) and scans all
 // kets. With one (batched instances), item w = {row0, nrows, ket_lo, ket_hi}
 // names rows of one instance and the ket range of that instance; columns
 // stay absolute indices, so a batch yields one block-diagonal CSR.
-template <int W, int R, bool FILL>
-__global__ __launch_bounds__(256) void scan_pass_kernel(
+template <int W, int R, int MODE>
+__global__ __launch_bounds__(256, MODE == kStage ? 3 : 1) void scan_pass_kernel(
     const uint64_t* __restrict__ bra, int64_t nbra,
     const uint64_t* __restrict__ ket, int64_t nket,
     const uint64_t* __restrict__ tand, const uint64_t* __restrict__ tor,
@@ -112,7 +133,8 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
     const uint64_t* __restrict__ sor, int64_t nsuper, int64_t nwaves,
     int64_t nblocks, const int64_t* __restrict__ items,
     uint32_t* __restrict__ row_counts, const int64_t* __restrict__ rowptr,
-    int32_t* __restrict__ colind) {
+    int32_t* __restrict__ colind, StagePool P) {
+  constexpr bool FILL = MODE == kFill;
   // XCD-aware mapping: hardware dispatches block b to XCD b % 8; logical
   // blocks (consecutive rows, which share spin strings and therefore ket
   // tiles) are laid out so each XCD's L2 sees a contiguous range of rows.
@@ -135,6 +157,9 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
   uint32_t live = 0;
   int64_t base[R];
   uint32_t cnt[R];
+  int64_t cur[R];   // kStage: the row's current chunk (-1 before its first)
+  uint32_t fil[R];  // kStage: entries used in it
+  bool dead = false;  // kStage: the pool ran out; counting goes on
 #pragma unroll
   for(int r = 0; r < R; ++r) {
     const int64_t row = row0 + r;
@@ -147,6 +172,8 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
     if(nz) live |= 1u << r;  // zero bras give empty rows (double_loop.hpp:50)
     cnt[r] = 0;
     base[r] = 0;
+    cur[r] = -1;
+    fil[r] = 0;
     if constexpr(FILL) {
       if(row < rend) base[r] = (int64_t)uniform_u64((uint64_t)rowptr[row]);
     }
@@ -155,10 +182,55 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
     if constexpr(!FILL) {
       if(lane == 0)
         for(int r = 0; r < R; ++r)
-          if(row0 + r < rend) row_counts[row0 + r] = 0;
+          if(row0 + r < rend) {
+            row_counts[row0 + r] = 0;
+            if constexpr(MODE == kStage) P.head[row0 + r] = -1;
+          }
     }
     return;
   }
+  // kStage: record one ballot's hits of row r (m = ballot, hit = this lane's
+  // bit): the part that fits the current chunk goes there, the rest opens
+  // the next chunk of the row's chain
+  auto stage = [&](int r, uint64_t m, bool hit, int32_t j) {
+    const uint32_t k = popc64(m);
+    if(!k) return;
+    cnt[r] += k;
+    if(dead) return;
+    const uint32_t room = cur[r] < 0 ? 0u : (uint32_t)kChunk - fil[r];
+    const uint32_t pos = lanes_below(m);
+    int64_t nc = -1;
+    if(k > room) {
+      unsigned long long got = 0;
+      if(lane == 0) got = atomicAdd(P.next, 1ull);
+      nc = (int64_t)uniform_u64((uint64_t)__shfl(got, 0, 64));
+      if(nc >= P.cap) {
+        if(lane == 0) P.overflow[0] = 1;
+        dead = true;
+        return;
+      }
+      if(lane == 0) {
+        if(cur[r] < 0) {
+          P.head[row0 + r] = nc;
+        } else {
+          P.chunk_next[cur[r]] = nc;
+          P.used[cur[r]] = (uint32_t)kChunk;
+        }
+      }
+    }
+    if(hit) {
+      if(pos < room)
+        P.col[cur[r] * kChunk + fil[r] + pos] = j;
+      else
+        P.col[nc * kChunk + (pos - room)] = j;
+    }
+    if(k > room) {
+      cur[r] = nc;
+      fil[r] = k - room;
+    } else {
+      fil[r] += k;
+    }
+  };
   const int64_t s_lo = klo / (kTile * kSuper), s_hi = (khi + kTile * kSuper - 1) / (kTile * kSuper);
   const int64_t t_lo = klo / kTile, t_hi = (khi + kTile - 1) / kTile;
 
@@ -226,6 +298,8 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
               if constexpr(FILL) {
                 if(hit) colind[base[r] + lanes_below(m)] = (int32_t)j;
                 base[r] += popc64(m);
+              } else if constexpr(MODE == kStage) {
+                stage(r, m, hit, (int32_t)j);
               } else {
                 cnt[r] += (uint32_t)popc64(m);
               }
@@ -244,6 +318,8 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
             if constexpr(FILL) {
               if(hit) colind[base[r] + lanes_below(m)] = (int32_t)j;
               base[r] += popc64(m);
+            } else if constexpr(MODE == kStage) {
+              stage(r, m, hit, (int32_t)j);
             } else {
               cnt[r] += (uint32_t)popc64(m);
             }
@@ -255,7 +331,33 @@ __global__ __launch_bounds__(256) void scan_pass_kernel(
   if constexpr(!FILL) {
     if(lane == 0)
       for(int r = 0; r < R; ++r)
-        if(row0 + r < rend) row_counts[row0 + r] = cnt[r];
+        if(row0 + r < rend) {
+          row_counts[row0 + r] = cnt[r];
+          if constexpr(MODE == kStage) {
+            if(cur[r] < 0) {
+              P.head[row0 + r] = -1;
+            } else {
+              P.used[cur[r]] = fil[r];
+              P.chunk_next[cur[r]] = -1;
+            }
+          }
+        }
+  }
+}
+
+// Staged columns -> CSR: one wavefront per row walks the row's chunk chain.
+__global__ __launch_bounds__(256) void stage_gather_kernel(
+    const int64_t* __restrict__ head, const int64_t* __restrict__ chunk_next,
+    const uint32_t* __restrict__ used, const int32_t* __restrict__ pool,
+    const int64_t* __restrict__ rowptr, int64_t nrows, int32_t* __restrict__ colind) {
+  const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(row >= nrows) return;
+  const uint32_t lane = threadIdx.x & 63;
+  int64_t out = rowptr[row];
+  for(int64_t c = head[row]; c >= 0; c = chunk_next[c]) {
+    const uint32_t u = used[c];
+    for(uint32_t e = lane; e < u; e += 64) colind[out + e] = pool[c * kChunk + e];
+    out += u;
   }
 }
 
@@ -432,37 +534,118 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many bra rows");
   DevBuf counts32(sizeof(uint32_t) * (nbra + 1));
   DevBuf counts64(sizeof(int64_t) * (nbra + 1));
+  // tiled path: one staged scan with a pool of 2,048 entries per bra row (at
+  // most a third of the free memory; rows longer on average overflow it and
+  // take count + fill). The pool is released before the values are
+  // allocated, so the peak stays that of count + fill.
+  DevBuf s_pool, s_head, s_next, s_used, s_ctl;
+  StagePool P{};
+  bool staged = false;
+  if(!ex.product) {
+    size_t free_b = 0, total_b = 0;
+    MHIP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const int64_t per_chunk = (int64_t)kChunk * 4 + 8 + 4;
+    const int64_t cap = std::min<int64_t>({(int64_t)(free_b / 3) / per_chunk,
+                                           nbra * (2048 / kChunk) + 1024, 1ll << 27});
+    if(cap >= 4096) {
+      try {
+        s_pool.alloc(sizeof(int32_t) * kChunk * cap);
+        s_next.alloc(sizeof(int64_t) * cap);
+        s_used.alloc(sizeof(uint32_t) * cap);
+        s_head.alloc(sizeof(int64_t) * nbra);
+        s_ctl.alloc(16);
+        staged = true;
+      } catch(const Error& e) {
+        if(e.code != MHIP_ERR_OUT_OF_MEMORY) throw;
+        (void)hipGetLastError();  // a refused hipMalloc leaves no pending error
+        s_pool.release();
+        s_next.release();
+        s_used.release();
+        s_head.release();
+      }
+    }
+    if(staged) {
+      P.col = s_pool.as<int32_t>();
+      P.cap = cap;
+      P.next = s_ctl.as<unsigned long long>();
+      P.head = s_head.as<int64_t>();
+      P.chunk_next = s_next.as<int64_t>();
+      P.used = s_used.as<uint32_t>();
+      P.overflow = reinterpret_cast<int*>(s_ctl.as<unsigned char>() + 8);
+      MHIP_HIP_CHECK(hipMemsetAsync(s_ctl.p, 0, 16, st));
+    }
+  }
+  auto count_pass = [&](bool stage) {
+    if(ex.product)
+      product_count(ctx, *ex.product, nbra, counts32.as<uint32_t>());
+    else if(stage)
+      hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kStage>), dim3(grid), dim3(256), 0, st,
+                         d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
+                         nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr,
+                         P);
+    else
+      hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kCount>), dim3(grid), dim3(256), 0, st,
+                         d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
+                         nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr,
+                         StagePool{});
+    MHIP_HIP_CHECK(hipGetLastError());
+  };
   t.start();
-  if(ex.product)
-    product_count(ctx, *ex.product, nbra, counts32.as<uint32_t>());
-  else
-    hipLaunchKernelGGL((scan_pass_kernel<W, kRows, false>), dim3(grid), dim3(256), 0, st,
-                       d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
-                       nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr);
-  MHIP_HIP_CHECK(hipGetLastError());
+  count_pass(staged);
   t.stop();
   tm.ham_count_ms = t.ms();
 
   // 3. scan
-  t.start();
-  hipLaunchKernelGGL(widen_counts_kernel, dim3((nbra + 1 + 255) / 256), dim3(256), 0, st,
-                     counts32.as<uint32_t>(), nbra, counts64.as<int64_t>());
   DevBuf rp_struct = take_buffer(ctx, sizeof(int64_t) * (nbra + 1));
-  exclusive_scan(ctx, counts64.as<int64_t>(), rp_struct.as<int64_t>(), nbra + 1);
   int64_t nnz_struct = 0;
-  d2h(ctx, &nnz_struct, rp_struct.as<int64_t>() + nbra, 8);
+  auto scan_counts = [&] {
+    hipLaunchKernelGGL(widen_counts_kernel, dim3((nbra + 1 + 255) / 256), dim3(256), 0, st,
+                       counts32.as<uint32_t>(), nbra, counts64.as<int64_t>());
+    exclusive_scan(ctx, counts64.as<int64_t>(), rp_struct.as<int64_t>(), nbra + 1);
+    d2h(ctx, &nnz_struct, rp_struct.as<int64_t>() + nbra, 8);
+  };
+  t.start();
+  scan_counts();
+  if(staged) {
+    int overflow = 0;
+    d2h(ctx, &overflow, P.overflow, sizeof(int));
+    if(overflow) {  // the pool was too small: count + fill after all
+      staged = false;
+      s_pool.release();
+      s_next.release();
+      s_used.release();
+      s_head.release();
+      count_pass(false);
+      scan_counts();
+    }
+  }
   t.stop();
   tm.ham_scan_ms = t.ms();  // synchronises
   tm.ham_nnz_struct = nnz_struct;
   counts32.release();
 
-  // 4. fill (the product path evaluates the elements in the same pass)
+  // 4. fill (the product path evaluates the elements in the same pass; a
+  // staged scan only gathers its chunks)
   DevBuf col = take_buffer(ctx, sizeof(int32_t) * std::max<int64_t>(nnz_struct, 1));
+  if(staged) {
+    t.start();
+    if(nnz_struct)
+      hipLaunchKernelGGL(stage_gather_kernel, dim3((nbra + kWavesPerBlock - 1) / kWavesPerBlock),
+                         dim3(256), 0, st, P.head, P.chunk_next, P.used, P.col,
+                         rp_struct.as<int64_t>(), nbra, col.as<int32_t>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    s_pool.release();
+    s_next.release();
+    s_used.release();
+    s_head.release();
+    t.stop();
+  }
   DevBuf val = take_buffer(ctx, sizeof(double) * std::max<int64_t>(nnz_struct, 1));
   DevBuf diag(sizeof(double) * nbra);
   DevBuf flag(sizeof(int));
   MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
-  t.start();
+  if(!staged) t.start();
   if(ex.product) {
     hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
                        nbra, diag.as<double>(), ex.views, ex.row_inst);
@@ -470,14 +653,14 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
       product_fill_elements(ctx, W, *ex.product, I, nbra, diag.as<double>(),
                             rp_struct.as<int64_t>(), col.as<int32_t>(), val.as<double>(), thresh,
                             counts64.as<int64_t>(), flag.as<int>());
-  } else if(nnz_struct) {
-    hipLaunchKernelGGL((scan_pass_kernel<W, kRows, true>), dim3(grid), dim3(256), 0, st,
+  } else if(nnz_struct && !staged) {
+    hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kFill>), dim3(grid), dim3(256), 0, st,
                        d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
                        nwaves, nblocks, ex.items, nullptr, rp_struct.as<int64_t>(),
-                       col.as<int32_t>());
+                       col.as<int32_t>(), StagePool{});
   }
   MHIP_HIP_CHECK(hipGetLastError());
-  t.stop();
+  if(!staged) t.stop();
   tm.ham_fill_ms = t.ms();
   bnd.release();
 
