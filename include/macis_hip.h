@@ -165,6 +165,9 @@ int mhip_set_integrals(mhip_context* ctx, int64_t norb, const double* T,
  * DEVICE arrays of words_per_det uint64 per determinant. Zero determinants
  * give empty rows and are never columns. nbra == 0 or nket == 0 gives an
  * empty matrix (csr_hamiltonian.hpp:30-35). The CSR is owned by *out. */
+/* The tiled passes stage columns in a device pool; the environment variable
+ * MHIP_STAGE_POOL_CHUNKS caps it (128-entry chunks; 0 disables staging), the
+ * build falling back to a count and a fill walk when it overflows. */
 int mhip_ham_build(mhip_context* ctx, int words_per_det,
                    const uint64_t* d_bra, int64_t nbra, const uint64_t* d_ket,
                    int64_t nket, double thresh, mhip_csr** out);

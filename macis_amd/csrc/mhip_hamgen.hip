@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <memory>
 
 #include "mhip_device.hpp"
@@ -545,9 +546,13 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
     size_t free_b = 0, total_b = 0;
     MHIP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     const int64_t per_chunk = (int64_t)kChunk * 4 + 8 + 4;
-    const int64_t cap = std::min<int64_t>({(int64_t)(free_b / 3) / per_chunk,
-                                           nbra * (2048 / kChunk) + 1024, 1ll << 27});
-    if(cap >= 4096) {
+    int64_t cap = std::min<int64_t>({(int64_t)(free_b / 3) / per_chunk,
+                                     nbra * (2048 / kChunk) + 1024, 1ll << 27});
+    // MHIP_STAGE_POOL_CHUNKS caps the pool (0: no staging), so tests can
+    // force the overflow fallback and the two-pass walk
+    if(const char* env = std::getenv("MHIP_STAGE_POOL_CHUNKS"))
+      cap = std::min<int64_t>(cap, std::max<int64_t>(0, std::atoll(env)));
+    if(cap >= 1024) {
       try {
         s_pool.alloc(sizeof(int32_t) * kChunk * cap);
         s_next.alloc(sizeof(int64_t) * cap);

@@ -278,3 +278,22 @@ def test_selected_ci_64_orbitals_properties(ctx):
     # the device residual is below 1e-8; the host product adds rounding of
     # order 1e-16 |H| |x| per row
     assert np.linalg.norm(Hx - E * x) <= 1e-8 + 1e-14 * abs(E) * np.sqrt(N)
+
+
+@pytest.mark.parametrize("pool", ["0", "1024"])
+def test_tiled_build_fallback_paths(ctx, pool, monkeypatch):
+    # MHIP_STAGE_POOL_CHUNKS=0: no staging (count + fill walks); 1024 chunks
+    # (131,072 entries) overflow on this list and the build redoes count +
+    # fill. Both must give the staged build's CSR exactly.
+    announce(f"hamgen tiled fallback pool={pool}")
+    n = 16
+    T, V = seeded_integrals(n, 77)
+    ctx.set_integrals(n, T, V)
+    dets = selected_ci_dets(n, 5, 5, 4000, seed=77)
+    dd = ctx.to_device(dets)
+    ref = ctx.ham_build(2, dd, len(dets), thresh=EPS).to_host()
+    assert ref[0][-1] > 131072
+    monkeypatch.setenv("MHIP_STAGE_POOL_CHUNKS", pool)
+    got = ctx.ham_build(2, dd, len(dets), thresh=EPS).to_host()
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
