@@ -110,6 +110,7 @@ __device__ __forceinline__ int excitation_bound(const uint64_t (&b)[W],
 // to count + fill.
 enum ScanMode : int { kCount = 0, kFill = 1, kStage = 2 };
 constexpr int kChunk = 128;
+constexpr int kChunksPerGrab = 8;  // chunks a wavefront takes per atomic
 struct StagePool {
   int32_t* col;                // cap * kChunk entries
   int64_t cap;                 // chunks
@@ -161,6 +162,9 @@ __global__ __launch_bounds__(256, MODE == kStage ? 3 : 1) void scan_pass_kernel(
   int64_t cur[R];   // kStage: the row's current chunk (-1 before its first)
   uint32_t fil[R];  // kStage: entries used in it
   bool dead = false;  // kStage: the pool ran out; counting goes on
+  // kStage: chunks [grab, grab_end) taken from the pool and not yet used (one
+  // atomic per kChunksPerGrab chunks keeps the shared counter uncontended)
+  int64_t grab = 0, grab_end = 0;
 #pragma unroll
   for(int r = 0; r < R; ++r) {
     const int64_t row = row0 + r;
@@ -202,14 +206,18 @@ __global__ __launch_bounds__(256, MODE == kStage ? 3 : 1) void scan_pass_kernel(
     const uint32_t pos = lanes_below(m);
     int64_t nc = -1;
     if(k > room) {
-      unsigned long long got = 0;
-      if(lane == 0) got = atomicAdd(P.next, 1ull);
-      nc = (int64_t)uniform_u64((uint64_t)__shfl(got, 0, 64));
-      if(nc >= P.cap) {
-        if(lane == 0) P.overflow[0] = 1;
-        dead = true;
-        return;
+      if(grab >= grab_end) {
+        unsigned long long got = 0;
+        if(lane == 0) got = atomicAdd(P.next, (unsigned long long)kChunksPerGrab);
+        grab = (int64_t)uniform_u64((uint64_t)__shfl(got, 0, 64));
+        grab_end = grab + kChunksPerGrab;
+        if(grab_end > P.cap) {
+          if(lane == 0) P.overflow[0] = 1;
+          dead = true;
+          return;
+        }
       }
+      nc = grab++;
       if(lane == 0) {
         if(cur[r] < 0) {
           P.head[row0 + r] = nc;
@@ -535,14 +543,16 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many bra rows");
   DevBuf counts32(sizeof(uint32_t) * (nbra + 1));
   DevBuf counts64(sizeof(int64_t) * (nbra + 1));
-  // tiled path: one staged scan with a pool of 2,048 entries per bra row (at
-  // most a third of the free memory; rows longer on average overflow it and
-  // take count + fill). The pool is released before the values are
-  // allocated, so the peak stays that of count + fill.
+  // tiled path over one list: one staged scan with a pool of 2,048 entries
+  // per bra row (at most a third of the free memory; rows longer on average
+  // overflow it and take count + fill). The pool is released before the
+  // values are allocated, so the peak stays that of count + fill. Batched
+  // builds (work items over small per-instance ket ranges) scan cheaply and
+  // keep count + fill, which runs at a higher occupancy.
   DevBuf s_pool, s_head, s_next, s_used, s_ctl;
   StagePool P{};
   bool staged = false;
-  if(!ex.product) {
+  if(!ex.product && !ex.items) {
     size_t free_b = 0, total_b = 0;
     MHIP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     const int64_t per_chunk = (int64_t)kChunk * 4 + 8 + 4;
