@@ -398,14 +398,24 @@ __global__ void head_sum_kernel(const float* __restrict__ part, int np, int64_t 
   y[i] = s + b3[0];
 }
 
-// dZ2 = (dy * w3) * gelu'(Z2) as bf16.
+// dZ2 = (dy * w3) * gelu'(Z2) as bf16; a thread takes 8 consecutive
+// elements of a row (two 16-byte loads of Z2, one 16-byte store).
 __global__ void head_back_kernel(const float* __restrict__ dy, const float* __restrict__ w3,
                                  const float* __restrict__ Z2, int64_t n,
                                  __bf16* __restrict__ dZ2) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
   if(i >= n * kHid) return;
   const int j = (int)(i % kHid);
-  dZ2[i] = (__bf16)(dy[i / kHid] * w3[j] * gelu_grad(Z2[i]));
+  const float d = dy[i / kHid];
+  const float4 z0 = *reinterpret_cast<const float4*>(Z2 + i);
+  const float4 z1 = *reinterpret_cast<const float4*>(Z2 + i + 4);
+  const float4 w0 = *reinterpret_cast<const float4*>(w3 + j);
+  const float4 w1 = *reinterpret_cast<const float4*>(w3 + j + 4);
+  const bf16x8_t o = {(__bf16)(d * w0.x * gelu_grad(z0.x)), (__bf16)(d * w0.y * gelu_grad(z0.y)),
+                      (__bf16)(d * w0.z * gelu_grad(z0.z)), (__bf16)(d * w0.w * gelu_grad(z0.w)),
+                      (__bf16)(d * w1.x * gelu_grad(z1.x)), (__bf16)(d * w1.y * gelu_grad(z1.y)),
+                      (__bf16)(d * w1.z * gelu_grad(z1.z)), (__bf16)(d * w1.w * gelu_grad(z1.w))};
+  *reinterpret_cast<bf16x8_t*>(dZ2 + i) = o;
 }
 
 // Column sums over row slices: part[s][j] = sum_{b in slice s} M[b][j] w[b]
@@ -763,7 +773,7 @@ int mhip_mlp_train_step(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
     // head gradients
     colsum(st, net, net.A2.as<__bf16>(), npad, net.dy.as<float>(), G + Mlp::ow3);
     hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, st, net.dy.as<float>(), npad, G + Mlp::ob3);
-    hipLaunchKernelGGL(head_back_kernel, dim3((npad * kHid + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL(head_back_kernel, dim3((npad * kHid / 8 + 255) / 256), dim3(256), 0, st,
                        net.dy.as<float>(), P + Mlp::ow3, net.Z2.as<float>(), npad,
                        net.dZ2.as<__bf16>());
     // layer 2: dW2 = dZ2^T A1, db2 = colsum dZ2, dZ1 = (dZ2 W2) * gelu'(Z1)
