@@ -66,7 +66,11 @@ enum Epi : int {
 // owning TI x 2 MFMA tiles of 32 x 32: block tile (64 TI) x 128, so M is a
 // multiple of 64 TI, N of 128, K of 32. TI = 4 reads 3/4 of the LDS operand
 // bytes per MFMA that TI = 2 does, at twice the accumulator registers.
-template <bool AK, bool BKC, int EPI, int TI = 2>
+// FW > 0 (with AK): A is not read from memory but formed from determinants
+// of FW uint64 words (passed as A; lda = number of valid rows): A(m, k) =
+// bit k of determinant m as bf16 0/1 (zero past 64 FW bits and for rows m >=
+// lda) -- the featurisation folded into the first layer's loads.
+template <bool AK, bool BKC, int EPI, int TI = 2, int FW = 0>
 __global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
     int M, int N, int K, const __bf16* __restrict__ A, int lda, const __bf16* __restrict__ B,
     int ldb, const float* __restrict__ bias, const float* __restrict__ zaux,
@@ -131,7 +135,16 @@ __global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
 #pragma unroll
     for(int it = 0; it < TI; ++it) {
       const int c = tid + 256 * it;
-      if constexpr(AK)
+      if constexpr(AK && FW > 0) {
+        const int64_t row = m0 + (c >> 2);
+        const int kq = k0 + (c & 3) * 8;
+        uint32_t bits = 0;
+        if(row < lda && kq < 64 * FW)
+          bits = (uint32_t)(reinterpret_cast<const uint64_t*>(A)[row * FW + (kq >> 6)] >>
+                            (kq & 63)) & 0xffu;
+#pragma unroll
+        for(int e = 0; e < 8; ++e) ra[it][e] = (__bf16)(float)((bits >> e) & 1u);
+      } else if constexpr(AK)
         ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(m0 + (c >> 2)) * lda + k0 +
                                                     (c & 3) * 8);
       else
@@ -570,14 +583,15 @@ struct mhip_mlp {
 namespace mhip {
 namespace {
 
-template <bool AK, bool BKC, int EPI, int TI = 2>
+template <bool AK, bool BKC, int EPI, int TI = 2, int FW = 0>
 void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
           const float* bias, const float* zaux, float* Cf, __bf16* Cb, int ldc, int splits = 1) {
   MHIP_REQUIRE(M % (64 * TI) == 0 && N % BN == 0 && K % BK == 0, MHIP_ERR_INVALID_ARG,
                "gemm dims must be multiples of the block tile and of 32 in K");
   MHIP_REQUIRE(splits >= 1 && K % (splits * BK) == 0 && (splits == 1 || EPI == EPI_F32),
                MHIP_ERR_INVALID_ARG, "bad split-K");
-  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BKC, EPI, TI>), dim3(N / BN, M / (64 * TI), splits),
+  static_assert(FW == 0 || AK, "determinant-formed A needs the K-contiguous layout");
+  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BKC, EPI, TI, FW>), dim3(N / BN, M / (64 * TI), splits),
                      dim3(256), 0,
                      st, M, N, K, A, lda, B, ldb, bias, zaux, Cf, Cb, ldc);
   MHIP_HIP_CHECK(hipGetLastError());
@@ -612,28 +626,36 @@ void colsum(hipStream_t st, Mlp& net, const __bf16* Mt, int64_t rows, const floa
 
 const __bf16* bfp(const DevBuf& b, int64_t off = 0) { return b.as<__bf16>() + off; }
 
-// Forward pass for rows [0, npad) already featurised into net.X; training
-// keeps the pre-activations Z1, Z2 for the backward pass, inference does not.
-void forward(hipStream_t st, Mlp& net, int64_t npad, bool keep_z = true) {
+// Training forward pass for rows [0, npad) already featurised into net.X;
+// keeps the pre-activations Z1, Z2 for the backward pass.
+void forward(hipStream_t st, Mlp& net, int64_t npad) {
   const float* P = net.p.as<float>();
-  if(keep_z) {
-    gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
-                                    bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr,
-                                    net.Z1.as<float>(), net.A1.as<__bf16>(), kHid);
-    gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
-                                    bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, nullptr,
-                                    net.Z2.as<float>(), net.A2.as<__bf16>(), kHid);
-  } else {
-    gemm<true, true, EPI_BIAS_GELU_A>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
-                                      bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr, nullptr,
-                                      net.A1.as<__bf16>(), kHid);
-    // the head rides on the last layer's epilogue: per-wave column partials
-    // (2 per 128-column tile) go into A2's storage, which is far larger
-    static_assert(2 * (kHid / BN) * sizeof(float) <= kHid * sizeof(__bf16), "head partials");
-    gemm<true, true, EPI_HEAD, 4>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
-                                  bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, P + Mlp::ow3,
-                                  net.A2.as<float>(), nullptr, (int)npad);
-  }
+  gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kIn, bfp(net.X), kIn,
+                                  bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr,
+                                  net.Z1.as<float>(), net.A1.as<__bf16>(), kHid);
+  gemm<true, true, EPI_BIAS_GELU>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
+                                  bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, nullptr,
+                                  net.Z2.as<float>(), net.A2.as<__bf16>(), kHid);
+}
+
+// Inference for n determinants (rows padded to npad): the first layer forms
+// its inputs from the determinant bits as it loads them, the head rides on
+// the last layer's epilogue (per-wave column partials, 2 per 128-column
+// tile, in A2's storage, which is far larger) and a small kernel adds them.
+template <int W>
+void infer(hipStream_t st, Mlp& net, const uint64_t* dets, int64_t n, int64_t npad, float* out) {
+  const float* P = net.p.as<float>();
+  gemm<true, true, EPI_BIAS_GELU_A, 2, W>(st, (int)npad, kHid, kIn,
+                                          reinterpret_cast<const __bf16*>(dets), (int)n,
+                                          bfp(net.pb, Mlp::oW1), kIn, P + Mlp::ob1, nullptr,
+                                          nullptr, net.A1.as<__bf16>(), kHid);
+  static_assert(2 * (kHid / BN) * sizeof(float) <= kHid * sizeof(__bf16), "head partials");
+  gemm<true, true, EPI_HEAD, 4>(st, (int)npad, kHid, kHid, bfp(net.A1), kHid,
+                                bfp(net.pb, Mlp::oW2), kHid, P + Mlp::ob2, P + Mlp::ow3,
+                                net.A2.as<float>(), nullptr, (int)npad);
+  hipLaunchKernelGGL(head_sum_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
+                     net.A2.as<float>(), 2 * (kHid / BN), npad, P + Mlp::ob3, n, out);
+  MHIP_HIP_CHECK(hipGetLastError());
 }
 
 template <int W>
@@ -731,14 +753,9 @@ int mhip_mlp_predict(mhip_context* ctx, mhip_mlp* mlp, int words_per_det, const 
     for(int64_t s = 0; s < n; s += chunk) {
       const int64_t nb = std::min(chunk, n - s), npad = (nb + 255) / 256 * 256;
       if(words_per_det == 1)
-        featurize<1>(ctx->stream, net, d_dets + s, nb, npad);
+        infer<1>(ctx->stream, net, d_dets + s, nb, npad, d_out + s);
       else
-        featurize<2>(ctx->stream, net, d_dets + 2 * s, nb, npad);
-      forward(ctx->stream, net, npad, false);
-      hipLaunchKernelGGL(head_sum_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream,
-                         net.A2.as<float>(), 2 * (kHid / BN), npad,
-                         net.p.as<float>() + Mlp::ob3, nb, d_out + s);
-      MHIP_HIP_CHECK(hipGetLastError());
+        infer<2>(ctx->stream, net, d_dets + 2 * s, nb, npad, d_out + s);
     }
   });
 }
