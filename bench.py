@@ -247,8 +247,9 @@ def mlp_throughput(C, ctx):
         "forward_rows": n, "forward_ms": fwd_s * 1e3, "forward_TFLOPs": f_tf,
         "train_batch": nb, "train_step_ms": trn_s * 1e3, "train_TFLOPs": t_tf,
         "mfma_util_forward": f_tf / BF16_PEAK_TFLOPS, "mfma_util_train": t_tf / BF16_PEAK_TFLOPS,
-        "note": "wall time per call incl. featurisation, head, launches (training steps queued "
-                "back to back, loss not read); peak = dense BF16 MFMA",
+        "note": "wall time per call incl. the head and launches (inference forms the first "
+                "layer's inputs from the determinant bits; training featurises, its steps "
+                "queued back to back, loss not read); peak = dense BF16 MFMA",
     }
 
 
