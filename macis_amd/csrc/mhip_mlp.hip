@@ -126,7 +126,6 @@ __global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
   // in flight while this step's MFMAs run (one step ahead for TI = 2, two
   // for TI = 4, whose fewer waves per SIMD hide less latency).
   // 16-byte chunks: A tile BMt x 32 = TI per thread, B tile 128 x 32 = 2
-  constexpr int kAC = 8 * TI;  // A chunks per K row when A is M-contiguous
   typedef bf16x8_t RegA[TI];
   typedef bf16x8_t RegB[2];
   RegA ra;
@@ -148,8 +147,8 @@ __global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
         ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(m0 + (c >> 2)) * lda + k0 +
                                                     (c & 3) * 8);
       else
-        ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(k0 + c / kAC) * lda + m0 +
-                                                    (c % kAC) * 8);
+        ra[it] = *reinterpret_cast<const bf16x8_t*>(A + (size_t)(k0 + (c & 31)) * lda + m0 +
+                                                    (c >> 5) * 8);
     }
 #pragma unroll
     for(int it = 0; it < 2; ++it) {
@@ -158,8 +157,8 @@ __global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
         rb[it] = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(n0 + (c >> 2)) * ldb + k0 +
                                                     (c & 3) * 8);
       else
-        rb[it] = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(k0 + (c >> 4)) * ldb + n0 +
-                                                    (c & 15) * 8);
+        rb[it] = *reinterpret_cast<const bf16x8_t*>(B + (size_t)(k0 + (c & 31)) * ldb + n0 +
+                                                    (c >> 5) * 8);
     }
   };
   // registers -> LDS buffer (K-contiguous whatever the global layout)
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
       if constexpr(AK) {
         *reinterpret_cast<bf16x8_t*>(&As[c >> 2][(c & 3) * 8]) = ra[it];
       } else {
-        const int kk = c / kAC, mc = (c % kAC) * 8;
+        const int kk = c & 31, mc = (c >> 5) * 8;
 #pragma unroll
         for(int e = 0; e < 8; ++e) As[mc + e][kk] = ra[it][e];
       }
@@ -184,7 +183,7 @@ __global__ __launch_bounds__(256, TI == 4 ? 2 : 1) void gemm_bf16_kernel(
       if constexpr(BKC) {
         *reinterpret_cast<bf16x8_t*>(&Bs[c >> 2][(c & 3) * 8]) = rb[it];
       } else {
-        const int kk = c >> 4, nc = (c & 15) * 8;
+        const int kk = c & 31, nc = (c >> 5) * 8;
 #pragma unroll
         for(int e = 0; e < 8; ++e) Bs[nc + e][kk] = rb[it][e];
       }
