@@ -85,6 +85,29 @@ def cpu_baseline(T, V, dets, budget_s=12.0):
     }
 
 
+def cpu_rdm_baseline(dets, X, budget_s=5.0):
+    """The reference's serial form_rdms (the oracle's restatement of
+    double_loop.hpp:116-165) for the first bra rows of the list against all
+    kets, with the ground state of the timed solve, sized to about budget_s;
+    the full-list time is extrapolated per bra row."""
+    from oracle import oracle as O
+
+    n = N_ORB
+    rows = 32
+    t0 = time.perf_counter()
+    O.form_rdms(n, 1, dets[:rows], X, ket=dets)
+    dt = time.perf_counter() - t0
+    rows = int(min(len(dets), max(rows, rows * budget_s / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    O.form_rdms(n, 1, dets[:rows], X, ket=dets)
+    dt = time.perf_counter() - t0
+    full_ms = dt / rows * len(dets) * 1e3
+    return {"value": full_ms, "unit": "ms (full list, extrapolated)", "cores": 1,
+            "kind": "port",
+            "sample": f"{rows} bra rows x {len(dets)} kets of CAS(12e,12o), serial double loop "
+                      f"with rdm_contributions, {dt:.1f} s"}
+
+
 def cpu_fair_baseline(T, V, dets, budget_s=6.0):
     """The same double loop with rows spread over the host's OpenMP threads
     (count pass + fill pass), not in the reference: the multi-core CPU
@@ -472,6 +495,8 @@ def main():
         out["cpu_spmv"] = cpu_spmv_baseline(ctx, T, V, d_dets, N)
         assert out["cpu_spmv"]["sample"].startswith(f"full CAS(12e,12o) SpMV ({nnz} nonzeros")
         out["cpu_spmv"]["gpu_speedup"] = out["cpu_spmv"]["ms_per_spmv"] / avg_spmv_ms
+        out["cpu_rdms"] = cpu_rdm_baseline(dets, d_X.numpy())
+        out["cpu_rdms"]["gpu_speedup"] = out["cpu_rdms"]["value"] / out["rdms"]["ms"]
     out["host"] = {"cpu": cpu_model(), "nproc": os.cpu_count(),
                    "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
     print(json.dumps(out), flush=True)
