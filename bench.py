@@ -185,7 +185,7 @@ def rdm_timing(ctx, T, V, d_dets, N, nnz, d_X, E):
             "note": "includes the structure pass that lists the pairs; FP64 atomics"}
 
 
-def selected_ci_section(C, ctx):
+def selected_ci_section(C, ctx, cpu=True):
     """The 64-orbital 20a20b selected-CI configuration: 2,000,000 determinants
     from the seeded generator (HF + 1-4-fold excitations, level weights
     0.1/0.5/0.2/0.2, holes/particles ~ exp(-|e_p - e_F|) with e_p = 0.25 p,
@@ -209,6 +209,8 @@ def selected_ci_section(C, ctx):
     build_ms = (time.perf_counter() - t0) * 1e3
     tm = ctx.timings()
     nnz = H.nnz
+    rowptr = np.zeros(nd + 1, dtype=np.int64)
+    C.check(C.lib().mhip_csr_copy_to_host_i64(ctx.h, H.h, rowptr.ctypes.data, None, None), "rp")
     x = ctx.to_device(np.random.default_rng(0).standard_normal(nd))
     y = ctx.empty(nd, np.float64)
     ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
@@ -225,6 +227,32 @@ def selected_ci_section(C, ctx):
     E, _, _ = ctx.selected_ci_diag(2, dd, nd, dX, max_restarts=200)
     diag_s = time.perf_counter() - t0
     td = ctx.timings()
+    cpu_leg = None
+    if cpu:
+        # the reference's serial double loop (oracle restatement) on
+        # contiguous bra rows from a third of the way in x all kets; its
+        # buffers take the exact row counts of the device CSR (equal by the
+        # parity tests)
+        from oracle import oracle as O
+
+        orc = O.Ham(T, V)
+        start, chunk, rows, nnz_c, t_used = nd // 3, 16, 0, 0, 0.0
+        while t_used < 8.0 and rows < 20000:
+            r0, r1 = start + rows, start + rows + chunk
+            need = int(rowptr[r1] - rowptr[r0])
+            col = np.zeros(max(need, 1), dtype=np.int32)
+            val = np.zeros(max(need, 1))
+            t1 = time.perf_counter()
+            cnt = orc.csr_rows_into(2, dets, r0, r1, col, val, thresh=eps)
+            t_used += time.perf_counter() - t1
+            assert int(cnt.sum()) == need
+            rows += chunk
+            nnz_c += need
+        cpu_leg = {"value": nnz_c / t_used, "unit": "nnz/s", "cores": 1, "kind": "port",
+                   "sample": f"{rows} contiguous bra rows x {nd} kets, serial double loop, "
+                             f"{t_used:.1f} s",
+                   "gpu_speedup": (nnz / (build_ms * 1e-3)) / (nnz_c / t_used)}
+        del orc
     return {
         "workload": "64 orbitals, 20a20b, 2,000,000 selected determinants (wfn_t<128>)",
         "nnz": nnz, "build_ms": build_ms, "nnz_per_s": nnz / (build_ms * 1e-3),
@@ -233,6 +261,7 @@ def selected_ci_section(C, ctx):
         "spmv_ms": spmv_ms, "spmv_GBs": spmv_bytes(nnz, nd, False) / (spmv_ms * 1e-3) / 1e9,
         "selected_ci_diag_s": diag_s, "E0": E, "davidson_iterations": td["dav_iters"],
         "davidson_restarts": td["dav_restarts"], "setup_s_host": setup_s,
+        "cpu_baseline": cpu_leg,
         "note": "build = host wall time of mhip_ham_build (second call); spmv on the CSR "
                 "kernel (the automatic policy keeps CSR for this list); solve includes its "
                 "own build",
@@ -488,7 +517,7 @@ def main():
         out["mlp"] = mlp_throughput(C, ctx)
         out["asci"] = asci_step_comparison(C, ctx)
     if not args.no_selected_ci:
-        out["selected_ci"] = selected_ci_section(C, ctx)
+        out["selected_ci"] = selected_ci_section(C, ctx, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(T, V, dets)
         out["cpu_fair"] = cpu_fair_baseline(T, V, dets)
