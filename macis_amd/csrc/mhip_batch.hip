@@ -295,6 +295,7 @@ __global__ __launch_bounds__(64) void btreduce_kernel(const int32_t* __restrict_
 
 // Partial Gram matrix V^T AV of tile t on MFMA (k <= 32): the four waves'
 // accumulators are combined in LDS and stored in the accumulator layout.
+template <int TI>  // 16-column tiles per side: 1 for k <= 16, else 2
 __global__ __launch_bounds__(256) void btgram_kernel(const int64_t* __restrict__ tiles,
                                                      const int32_t* __restrict__ tinst,
                                                      const int32_t* __restrict__ conv, int k,
@@ -306,33 +307,45 @@ __global__ __launch_bounds__(256) void btgram_kernel(const int64_t* __restrict__
   const int64_t lo = tiles[2 * t], hi = tiles[2 * t + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
-  const int ni = k > 16 ? 2 : 1;
-  d4_t acc[2][2];
-  for(int i = 0; i < 2; ++i)
-    for(int j = 0; j < 2; ++j) acc[i][j] = d4_t{0., 0., 0., 0.};
+  d4_t acc[TI][TI];
+#pragma unroll
+  for(int i = 0; i < TI; ++i)
+#pragma unroll
+    for(int j = 0; j < TI; ++j) acc[i][j] = d4_t{0., 0., 0., 0.};
   for(int64_t n0 = lo + wv * 32; n0 < hi; n0 += 4 * 32) {
     const int64_t nb = n0 + grp * 8;
-    double a[2][8], bb[2][8];
-    for(int i = 0; i < 2; ++i) {
+    double a[TI][8], bb[TI][8];
+#pragma unroll
+    for(int i = 0; i < TI; ++i) {
       const int c = i * 16 + col;
+#pragma unroll
       for(int s = 0; s < 8; ++s) {
-        const bool ok = i < ni && c < k && nb + s < hi;
+        const bool ok = c < k && nb + s < hi;
         a[i][s] = ok ? V[(int64_t)c * ld + nb + s] : 0.;
         bb[i][s] = ok ? AV[(int64_t)c * ld + nb + s] : 0.;
       }
     }
+#pragma unroll
     for(int s = 0; s < 8; ++s)
-      for(int i = 0; i < ni; ++i)
-        for(int j = 0; j < ni; ++j)
+#pragma unroll
+      for(int i = 0; i < TI; ++i)
+#pragma unroll
+        for(int j = 0; j < TI; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], bb[j][s], acc[i][j], 0, 0, 0);
   }
+  // partial layout of a tile: 2 x 2 blocks of 256 (only TI x TI written;
+  // the reduction reads rows and columns below k)
   __shared__ double red[4][1024];
-  for(int i = 0; i < 2; ++i)
-    for(int j = 0; j < 2; ++j)
+#pragma unroll
+  for(int i = 0; i < TI; ++i)
+#pragma unroll
+    for(int j = 0; j < TI; ++j)
+#pragma unroll
       for(int r = 0; r < 4; ++r) red[wv][(i * 2 + j) * 256 + r * 64 + lane] = acc[i][j][r];
   __syncthreads();
   for(int e = threadIdx.x; e < 1024; e += 256)
-    part[(int64_t)t * 1024 + e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+    if((e >> 8) / 2 < TI && (e >> 8) % 2 < TI)
+      part[(int64_t)t * 1024 + e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
 }
 
 // C_b (k x k, ld kMaxK) = sum over instance b's tiles of the partial Gram
@@ -550,7 +563,12 @@ void biteration(mhip_context* ctx, mhip_batch* B, int i, double tol) {
   double* V = B->V.as<double>();
   double* AV = B->AV.as<double>();
   launch_spmv(ctx, B, V + i * ld, AV + i * ld, true);
-  hipLaunchKernelGGL(btgram_kernel, dim3(B->ntiles), dim3(256), 0, st, B->tiles.as<int64_t>(),
+  if(k <= 16)
+    hipLaunchKernelGGL(btgram_kernel<1>, dim3(B->ntiles), dim3(256), 0, st, B->tiles.as<int64_t>(),
+                     B->tinst.as<int32_t>(), B->conv.as<int32_t>(), k, V, AV, ld,
+                     B->part.as<double>());
+  else
+    hipLaunchKernelGGL(btgram_kernel<2>, dim3(B->ntiles), dim3(256), 0, st, B->tiles.as<int64_t>(),
                      B->tinst.as<int32_t>(), B->conv.as<int32_t>(), k, V, AV, ld,
                      B->part.as<double>());
   hipLaunchKernelGGL(btgram_reduce_kernel, dim3(B->ninst), dim3(256), 0, st,
