@@ -600,8 +600,13 @@ void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const _
 // into fixed slices summed in order (parallel over the batch).
 void gemm_wgrad(hipStream_t st, Mlp& net, int M, int N, int K, const __bf16* A, int lda,
                 const __bf16* B, int ldb, float* C) {
+  // split K until the launch has 512 blocks (two per CU), each slice at
+  // least 256 batch rows
+  const int tiles = (M / 128) * (N / BN);
   int splits = 1;
-  while(splits < 32 && K % (2 * splits * BK) == 0 && K / (2 * splits) >= 256) splits *= 2;
+  while(splits < 128 && tiles * splits < 512 && K % (2 * splits * BK) == 0 &&
+        K / (2 * splits) >= 256)
+    splits *= 2;
   net.wk.reserve(sizeof(float) * (size_t)splits * M * N);
   gemm<false, false, EPI_F32>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, net.wk.as<float>(),
                               nullptr, N, splits);
