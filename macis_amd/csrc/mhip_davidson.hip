@@ -1,5 +1,5 @@
 // K3-K5: device-resident Davidson (davidson.hpp:61-225) and the serial
-// selected_ci_diag driver (selected_ci_diag.hpp:75-207).
+// selected_ci_diag routine (selected_ci_diag.hpp:75-207).
 //
 // Per iteration everything N-long stays in HBM:
 //   SpMV AV(:,i) = A V(:,i)                                      (K2)

@@ -1,7 +1,7 @@
 // Learned ASCI pre-screen: an MLP 128 -> 512 -> 512 -> 1 (GELU) on the
 // occupation bits of a determinant, predicting log|rv| (no counterpart in
 // the reference; the plain search of determinant_search.hpp:394-663 stays
-// the judge and the fallback).
+// the arbiter of what is kept and the fallback).
 //
 // Compute: BF16 operands on MFMA (v_mfma_f32_32x32x16_bf16, gfx950), FP32
 // accumulation, FP32 master weights updated with Adam. One tiled GEMM

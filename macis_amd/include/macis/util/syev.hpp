@@ -1,5 +1,5 @@
 // Symmetric eigensolver for the small (norb x norb) matrices of the host
-// drivers, in place of lapack::syev(Job::Vec, Uplo::Lower, ...) as
+// loops (asci_grow), in place of lapack::syev(Job::Vec, Uplo::Lower, ...) as
 // asci_grow calls it (grow.hpp:90-92): cyclic Jacobi rotations on a full
 // copy of the lower triangle, eigenvalues ascending in W, eigenvectors
 // overwriting the columns of A. Eigenvector signs follow from the

@@ -4,7 +4,7 @@ The oracle restates DoubleLoopHamiltonianGenerator::make_csr_hamiltonian_block_
 (double_loop.hpp:30-98); the reference test csr_hamiltonian.cxx:18-71 pins
 the H2O CISD row pointers. Pattern (rowptr, colind) must be identical and
 values bit-identical (the device evaluates the same sums in the same order
-without FMA contraction; the north-star bound is 1e-12 relative).
+without FMA contraction; the required bound is 1e-12 relative).
 """
 import os
 
