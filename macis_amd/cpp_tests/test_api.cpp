@@ -23,6 +23,7 @@
 #include <macis/solvers/selected_ci_diag.hpp>
 #include <macis/util/detail/rdm_files.hpp>
 #include <macis/util/fcidump.hpp>
+#include <macis/util/transform.hpp>
 #include <macis/wavefunction_io.hpp>
 #include <sparsexx/spblas/spmbv.hpp>
 #include <sparsexx/util/submatrix.hpp>
@@ -180,6 +181,56 @@ static void test_file_formats(const std::string& dir) {
     threw = std::string(e.what()) == "NORB in RDM file doesn't match 7 8";
   }
   REQUIRE(threw);
+}
+
+static void test_transforms() {
+  // tests/transform.cxx: two-index literals, four-index against the defining sum
+  const size_t n = 4;
+  const std::vector<double> A = {1, 5, 9, 13, 2, 6, 10, 14, 3, 7, 11, 15, 4, 8, 12, 16};
+  const std::vector<double> I4 = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  const std::vector<double> P4 = {1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 1, 0, 0};
+  const std::vector<double> S43 = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  const std::vector<double> SP43 = {1, 0, 0, 0, 0, 0, 0, 1, 0, 1, 0, 0};
+  std::vector<double> B(n * n);
+  macis::two_index_transform(n, n, A.data(), n, I4.data(), n, B.data(), n);
+  REQUIRE(B == A);
+  macis::two_index_transform(n, n, A.data(), n, P4.data(), n, B.data(), n);
+  REQUIRE(B == std::vector<double>({1, 9, 13, 5, 3, 11, 15, 7, 4, 12, 16, 8, 2, 10, 14, 6}));
+  std::vector<double> B3(9);
+  macis::two_index_transform(n, 3, A.data(), n, S43.data(), n, B3.data(), 3);
+  REQUIRE(B3 == std::vector<double>({1, 5, 9, 2, 6, 10, 3, 7, 11}));
+  macis::two_index_transform(n, 3, A.data(), n, SP43.data(), n, B3.data(), 3);
+  REQUIRE(B3 == std::vector<double>({1, 13, 5, 4, 16, 8, 2, 14, 6}));
+  // four-index: Y(p,q,r,s) = sum X(i,j,k,l) C(i,p) C(j,q) C(k,r) C(l,s)
+  std::vector<double> X(n * n * n * n);
+  for(size_t i = 0; i < X.size(); ++i) X[i] = std::sin(0.37 * i + 1.0);
+  auto check4 = [&](const std::vector<double>& Cm, size_t m) {
+    std::vector<double> Y(m * m * m * m), ref(m * m * m * m, 0.);
+    macis::four_index_transform(n, m, X.data(), n, Cm.data(), n, Y.data(), m);
+    auto c = [&](size_t i, size_t p) { return Cm[i + n * p]; };
+    for(size_t p = 0; p < m; ++p)
+      for(size_t q = 0; q < m; ++q)
+        for(size_t r = 0; r < m; ++r)
+          for(size_t t = 0; t < m; ++t)
+            for(size_t i = 0; i < n; ++i)
+              for(size_t j = 0; j < n; ++j)
+                for(size_t k = 0; k < n; ++k)
+                  for(size_t l = 0; l < n; ++l)
+                    ref[p + m * (q + m * (r + m * t))] += c(i, p) * c(j, q) * c(k, r) * c(l, t) *
+                                                         X[i + n * (j + n * (k + n * l))];
+    double e = 0.;
+    for(size_t i = 0; i < Y.size(); ++i) e = std::max(e, std::abs(Y[i] - ref[i]));
+    REQUIRE(e < 1e-13);
+  };
+  check4(I4, 4);
+  check4(P4, 4);
+  check4(S43, 3);
+  check4(SP43, 3);
+  // in place, as asci_grow calls it
+  std::vector<double> Xi = X, Yo(X.size());
+  macis::four_index_transform(n, n, X.data(), n, P4.data(), n, Yo.data(), n);
+  macis::four_index_transform(n, n, Xi.data(), n, P4.data(), n, Xi.data(), n);
+  REQUIRE(Xi == Yo);
 }
 
 // ---------------------------------------------------------------- GPU cases
@@ -487,6 +538,7 @@ int main(int argc, char** argv) {
       {"sd_operations", [] { test_sd_operations(); }},
       {"host generator elements", [&] { test_host_generator_elements(h); }},
       {"file formats (wavefunction, binary RDMs)", [&] { test_file_formats(dir); }},
+      {"orbital transforms", [] { test_transforms(); }},
   };
   if(!cpu_only) {
     cases.push_back({"CSR Hamiltonian (device)", [&] { test_csr_hamiltonian(h, dir); }});

@@ -197,6 +197,15 @@ def test_index_transforms_vs_einsum():
     assert np.abs(O.four_index_transform(X4, U) - ref4.reshape(-1, order="F")).max() < 1e-12
 
 
+def test_two_index_transform_literals():
+    # tests/transform.cxx: identity and permutation of a 4 x 4 matrix
+    A = np.array([1, 5, 9, 13, 2, 6, 10, 14, 3, 7, 11, 15, 4, 8, 12, 16], dtype=float)
+    P = np.array([1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 1, 0, 0], dtype=float).reshape(4, 4, order="F")
+    assert np.array_equal(O.two_index_transform(A, np.eye(4)), A)
+    assert np.array_equal(O.two_index_transform(A, P),
+                          np.array([1, 9, 13, 5, 3, 11, 15, 7, 4, 12, 16, 8, 2, 10, 14, 6], dtype=float))
+
+
 def test_natural_orbital_rotation_fci_invariants():
     # In a complete space the rotation changes no energy, and the ground
     # state's 1-RDM in the rotated orbitals is diagonal with the occupations
