@@ -127,7 +127,7 @@ struct StagePool {
 // names rows of one instance and the ket range of that instance; columns
 // stay absolute indices, so a batch yields one block-diagonal CSR.
 template <int W, int R, int MODE>
-__global__ __launch_bounds__(256, MODE == kStage ? 3 : 1) void scan_pass_kernel(
+__global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void scan_pass_kernel(
     const uint64_t* __restrict__ bra, int64_t nbra,
     const uint64_t* __restrict__ ket, int64_t nket,
     const uint64_t* __restrict__ tand, const uint64_t* __restrict__ tor,
