@@ -136,6 +136,10 @@ _SIGS = {
     "mhip_batch_build": (_i32, [_p, _p, _d]),
     "mhip_batch_tune": (_i32, [_p, _p, _p]),
     "mhip_batch_davidson": (_i32, [_p, _p, _i64, _i64, _d, _i32, _p, _p, _pd]),
+    "mhip_batch_diagonal": (_i32, [_p, _p, _p]),
+    "mhip_batch_set_guess": (_i32, [_p, _p, _p]),
+    "mhip_batch_vectors": (_i32, [_p, _p, _p]),
+    "mhip_batch_instance_csr": (_i32, [_p, _p, _i32, _p, _p, _p, _pi64]),
 }
 
 _lib = None
@@ -245,6 +249,20 @@ class CSR:
         nz = np.empty(max(self.nnz, 1), dtype=np.float64)
         check(lib().mhip_csr_copy_to_host_i32(self.ctx.h, self.h, rp.ctypes.data,
                                               ci.ctypes.data, nz.ctypes.data), "copy_to_host_i32")
+        return rp, ci[: self.nnz], nz[: self.nnz]
+
+    def to_host_native(self):
+        """(rowptr int64, colind int32, nzval float64): the device arrays
+        copied as they are stored (no index widening)."""
+        d_rp, d_ci, d_nz = self.device_arrays()
+        rp = np.empty(self.m + 1, dtype=np.int64)
+        ci = np.empty(max(self.nnz, 1), dtype=np.int32)
+        nz = np.empty(max(self.nnz, 1), dtype=np.float64)
+        L = lib()
+        check(L.mhip_memcpy_d2h(self.ctx.h, rp.ctypes.data, d_rp, rp.nbytes), "copy rowptr")
+        if self.nnz:
+            check(L.mhip_memcpy_d2h(self.ctx.h, ci.ctypes.data, d_ci, 4 * self.nnz), "copy colind")
+            check(L.mhip_memcpy_d2h(self.ctx.h, nz.ctypes.data, d_nz, 8 * self.nnz), "copy nzval")
         return rp, ci[: self.nnz], nz[: self.nnz]
 
     def device_arrays(self):
@@ -426,6 +444,8 @@ class Batch:
                                       dets.ctypes.data, Tp, Vp, C.byref(p)), "batch_create")
         self.h = p.value
         self.ninst = n
+        self.offsets = offs
+        self.ndets = int(offs[-1])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -444,6 +464,40 @@ class Batch:
         lanes = np.zeros(self.ninst, np.int32)
         check(lib().mhip_batch_tune(self.ctx.h, self.h, lanes.ctypes.data), "batch_tune")
         return lanes
+
+    def diagonal(self):
+        """H_ii of all instances, caller order (concatenated)."""
+        D = np.zeros(self.ndets)
+        check(lib().mhip_batch_diagonal(self.ctx.h, self.h, D.ctypes.data), "batch_diagonal")
+        return D
+
+    def set_guess(self, X):
+        """Davidson guess (concatenated, caller order); None: diagonal guess."""
+        if X is None:
+            check(lib().mhip_batch_set_guess(self.ctx.h, self.h, None), "batch_set_guess")
+            return
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        assert X.size == self.ndets
+        check(lib().mhip_batch_set_guess(self.ctx.h, self.h, X.ctypes.data), "batch_set_guess")
+
+    def vectors(self):
+        X = np.zeros(self.ndets)
+        check(lib().mhip_batch_vectors(self.ctx.h, self.h, X.ctypes.data), "batch_vectors")
+        return X
+
+    def instance_csr(self, inst):
+        """(rowptr int64, colind int32, nzval) of one instance's block."""
+        nnz = C.c_int64()
+        check(lib().mhip_batch_instance_csr(self.ctx.h, self.h, inst, None, None, None,
+                                            C.byref(nnz)), "batch_instance_csr")
+        m = int(self.offsets[inst + 1] - self.offsets[inst])
+        rp = np.zeros(m + 1, np.int64)
+        ci = np.zeros(max(nnz.value, 1), np.int32)
+        nz = np.zeros(max(nnz.value, 1), np.float64)
+        check(lib().mhip_batch_instance_csr(self.ctx.h, self.h, inst, rp.ctypes.data,
+                                            ci.ctypes.data, nz.ctypes.data, C.byref(nnz)),
+              "batch_instance_csr")
+        return rp, ci[: nnz.value], nz[: nnz.value]
 
     def davidson(self, max_m=20, max_restarts=0, tol=1e-8, use_graphs=True):
         E = np.zeros(self.ninst)

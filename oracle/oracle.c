@@ -605,6 +605,76 @@ int orc_csr_rows_parallel(const orc_ham* h, int W, const uint64_t* bra, int64_t 
   return ORC_OK;
 }
 
+/* One bra row of the double loop (double_loop.hpp:45-89) compared, entry by
+ * entry in ascending ket order, against a given CSR row col[0..len) /
+ * val[0..len): 0 when the row holds exactly the kept kets with bitwise equal
+ * values, else 1. The connection test is the same popcount(bra ^ ket) <= 4
+ * as csr_row; for one-word determinants it is done on the whole word (the
+ * sum of the two spin popcounts). */
+static int verify_row(const orc_ham* h, int W, det_t bra, const uint64_t* ket,
+                      int64_t nket, double thresh, const int32_t* col,
+                      const double* val, int64_t len) {
+  int64_t pos = 0;
+  if(!(popc(bra.a) + popc(bra.b))) return len != 0; /* zero bra: empty row */
+  uint32_t oa[64], ob[64];
+  int na = bits_to_indices(bra.a, oa), nb = bits_to_indices(bra.b, ob);
+  if(W == 1) {
+    const uint64_t bw = bra.a | (bra.b << 32);
+    for(int64_t j = 0; j < nket; ++j) {
+      const uint64_t kw = ket[j];
+      if(popc(bw ^ kw) > 4 || !kw) continue;
+      det_t k = load_det(1, ket + j);
+      double h_el = me_dispatch(h, bra, k, oa, na, ob, nb);
+      if(fabs(h_el) > thresh) {
+        if(pos >= len || col[pos] != (int32_t)j ||
+           memcmp(&val[pos], &h_el, sizeof(double)) != 0)
+          return 1;
+        ++pos;
+      }
+    }
+  } else {
+    for(int64_t j = 0; j < nket; ++j) {
+      const uint64_t ka = ket[2 * j], kb = ket[2 * j + 1];
+      if(popc(bra.a ^ ka) + popc(bra.b ^ kb) > 4 || !(ka | kb)) continue;
+      det_t k = {ka, kb};
+      double h_el = me_dispatch(h, bra, k, oa, na, ob, nb);
+      if(fabs(h_el) > thresh) {
+        if(pos >= len || col[pos] != (int32_t)j ||
+           memcmp(&val[pos], &h_el, sizeof(double)) != 0)
+          return 1;
+        ++pos;
+      }
+    }
+  }
+  return pos != len;
+}
+
+int64_t orc_csr_verify_rows(const orc_ham* h, int W, const uint64_t* bra,
+                            int64_t row_begin, int64_t row_end,
+                            const uint64_t* ket, int64_t nket, double thresh,
+                            const int64_t* rowptr, const int32_t* colind,
+                            const double* nzval, int64_t* first_bad) {
+  /* Checks rows [row_begin,row_end) of a CSR (rowptr indexed by absolute
+   * row, 0-based, entries colind/nzval at rowptr[i] .. rowptr[i+1]) against
+   * the reference double loop without storing the loop's output: rows are
+   * spread over OpenMP threads, each walks all kets as double_loop.hpp:45-89
+   * does and compares as it goes. Returns the number of rows that differ;
+   * *first_bad receives the lowest such row (-1 if none). */
+  if(!h || (W != 1 && W != 2) || row_end < row_begin) return -1;
+  int64_t bad = 0, first = INT64_MAX;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad) reduction(min : first)
+  for(int64_t i = row_begin; i < row_end; ++i) {
+    det_t b = load_det(W, bra + W * i);
+    const int64_t lo = rowptr[i], len = rowptr[i + 1] - rowptr[i];
+    if(len < 0 || verify_row(h, W, b, ket, nket, thresh, colind + lo, nzval + lo, len)) {
+      ++bad;
+      if(i < first) first = i;
+    }
+  }
+  if(first_bad) *first_bad = bad ? first : -1;
+  return bad;
+}
+
 int orc_num_threads(void) {
   int n = 1;
 #pragma omp parallel

@@ -101,6 +101,14 @@ int orc_csr_rows(const orc_ham* h, int W, const uint64_t* bra,
 int orc_csr_rows_parallel(const orc_ham* h, int W, const uint64_t* bra, int64_t row_begin,
                           int64_t row_end, const uint64_t* ket, int64_t nket, double thresh,
                           int64_t* row_counts, int32_t* colind, double* nzval);
+/* Rows [row_begin,row_end) of a given CSR (rowptr indexed by absolute row)
+ * checked against the double loop entry by entry, rows over OpenMP threads;
+ * returns the number of differing rows, *first_bad the lowest (-1 if none). */
+int64_t orc_csr_verify_rows(const orc_ham* h, int W, const uint64_t* bra,
+                            int64_t row_begin, int64_t row_end,
+                            const uint64_t* ket, int64_t nket, double thresh,
+                            const int64_t* rowptr, const int32_t* colind,
+                            const double* nzval, int64_t* first_bad);
 /* Threads an OpenMP parallel region of the oracle uses. */
 int orc_num_threads(void);
 void orc_free(void* p);

@@ -56,6 +56,8 @@ def lib():
             "orc_csr_rows": (_i32, [_p, _i32, _p, _i64, _i64, _p, _i64, _d, _p, _p, _p]),
             "orc_csr_rows_parallel": (_i32, [_p, _i32, _p, _i64, _i64, _p, _i64, _d, _p, _p, _p]),
             "orc_num_threads": (_i32, []),
+            "orc_csr_verify_rows": (_i64, [_p, _i32, _p, _i64, _i64, _p, _i64, _d, _p, _p, _p,
+                                           _p]),
             "orc_form_rdms": (None, [_i64, _i32, _p, _i64, _p, _i64, _p, _p, _p]),
             "orc_free": (None, [_p]),
             "orc_gespmbv": (None, [_i64, _d, _i64, _p, _p, _p, _i64, _p, _i64, _d, _p, _i64]),
@@ -229,6 +231,23 @@ class Ham:
         lib().orc_csr_rows(self._h, W, _ptr(bra), row_begin, row_end, _ptr(ket), nk,
                            thresh, _ptr(counts), _ptr(col), _ptr(val))
         return counts, col[:tot], val[:tot]
+
+    def csr_verify_rows(self, W, bra, row_begin, row_end, rowptr, colind, nzval, ket=None,
+                        thresh=1e-16):
+        """Rows [row_begin,row_end) of a given CSR (rowptr int64 indexed by
+        absolute row, colind int32, nzval) against the double loop, rows over
+        OpenMP threads: (number of differing rows, first such row or -1)."""
+        bra = np.ascontiguousarray(bra, dtype=np.uint64)
+        ket = bra if ket is None else np.ascontiguousarray(ket, dtype=np.uint64)
+        assert rowptr.dtype == np.int64 and colind.dtype == np.int32 and nzval.dtype == np.float64
+        assert len(rowptr) >= row_end + 1
+        first = C.c_int64()
+        bad = lib().orc_csr_verify_rows(self._h, W, _ptr(bra), row_begin, row_end, _ptr(ket),
+                                        ket.size // W, thresh, _ptr(rowptr), _ptr(colind),
+                                        _ptr(nzval), C.byref(first))
+        if bad < 0:
+            raise RuntimeError("orc_csr_verify_rows: bad arguments")
+        return int(bad), int(first.value)
 
     def selected_ci_diag(self, W, dets, h_el_tol=np.finfo(float).eps, max_m=20,
                          res_tol=1e-8, C0=None, max_restarts=0):
