@@ -314,6 +314,23 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* batch, int64_t max_m,
                         int64_t max_restarts, double tol, int use_graphs,
                         double* E, double* residual, double* ms_out);
 
+/* Diagonal H_ii of every instance of the built batch (HOST, caller order:
+ * det_offsets). */
+int mhip_batch_diagonal(mhip_context* ctx, mhip_batch* batch, double* D);
+/* Davidson guess for later mhip_batch_davidson calls (HOST, caller order,
+ * each instance's part normalised by the caller, as davidson.hpp:151 copies
+ * the guess into V(:,0) unchanged); NULL returns to the diagonal guess of
+ * davidson.hpp:61-68 (unit vector at each instance's first minimum of D). */
+int mhip_batch_set_guess(mhip_context* ctx, mhip_batch* batch, const double* X);
+/* Ritz vectors of the last mhip_batch_davidson (HOST, caller order). */
+int mhip_batch_vectors(mhip_context* ctx, mhip_batch* batch, double* X);
+/* Instance `inst` (caller index) of the built block-diagonal Hamiltonian
+ * as a HOST CSR with instance-local indices (rowptr int64[ndets_b + 1] from
+ * 0, colind int32, nzval): *nnz receives its nonzero count; with rowptr NULL
+ * only the count is returned. */
+int mhip_batch_instance_csr(mhip_context* ctx, mhip_batch* batch, int inst, int64_t* rowptr,
+                            int32_t* colind, double* nzval, int64_t* nnz);
+
 /* ---- learned ASCI pre-screen (no counterpart in the reference) ---------------
  * An MLP 128 -> 512 -> 512 -> 1 with GELU on the occupation bits of a
  * determinant (bits 0-63 alpha, 64-127 beta for wfn_t<128>; wfn_t<64> uses

@@ -490,6 +490,10 @@ struct mhip_batch {
   int ntiles = 0;
   mhip::DevBuf tiles, tinst, toff, part;
   std::map<int, hipGraphExec_t> graphs;
+  // caller-supplied Davidson guess (internal order); used instead of the
+  // diagonal guess while has_guess is set
+  mhip::DevBuf guess;
+  bool has_guess = false;
   ~mhip_batch() {
     for(auto& g : graphs) (void)hipGraphExecDestroy(g.second);
   }
@@ -865,8 +869,11 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
     hipLaunchKernelGGL(fill_i32_kernel, dim3((B->ninst + 255) / 256), dim3(256), 0, st,
                        B->conv.as<int32_t>(), (int64_t)B->ninst, 0);
     extract_diagonal(ctx, B->H.get(), B->D.as<double>());
-    hipLaunchKernelGGL(bguess_kernel, dim3(B->ninst), dim3(256), 0, st, sg, B->D.as<double>(),
-                       B->X.as<double>());
+    if(B->has_guess)
+      MHIP_HIP_CHECK(hipMemcpyAsync(B->X.p, B->guess.p, 8 * N, hipMemcpyDeviceToDevice, st));
+    else
+      hipLaunchKernelGGL(bguess_kernel, dim3(B->ninst), dim3(256), 0, st, sg, B->D.as<double>(),
+                         B->X.as<double>());
     MHIP_HIP_CHECK(hipMemcpyAsync(V, B->X.p, 8 * N, hipMemcpyDeviceToDevice, st));
     launch_spmv(ctx, B, V, AV, false);
     MHIP_HIP_CHECK(hipMemcpyAsync(V + ld, AV, 8 * N, hipMemcpyDeviceToDevice, st));
@@ -920,6 +927,91 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
       if(residual) residual[B->order[p]] = hr[p];
     }
     if(nleft) throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");
+  });
+}
+
+// Position of caller instance `inst` in the internal (size-sorted) order.
+static int internal_pos(const mhip_batch* B, int inst) {
+  for(int p = 0; p < B->ninst; ++p)
+    if(B->order[p] == inst) return p;
+  return -1;
+}
+
+// Caller-order host copy <-> internal-order device array of N doubles.
+static void batch_to_host(mhip_context* ctx, const mhip_batch* B, const double* d, double* h) {
+  std::vector<double> tmp(B->N);
+  d2h(ctx, tmp.data(), d, 8 * B->N);
+  int64_t o = 0;
+  for(int b = 0; b < B->ninst; ++b) {  // caller offsets follow caller order
+    const int p = internal_pos(B, b);
+    const int64_t n = B->off[p + 1] - B->off[p];
+    std::memcpy(h + o, tmp.data() + B->off[p], 8 * n);
+    o += n;
+  }
+}
+
+int mhip_batch_diagonal(mhip_context* ctx, mhip_batch* B, double* D) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B && B->H && D, MHIP_ERR_INVALID_ARG, "build the batch first");
+    DevBuf d(8 * B->N);
+    extract_diagonal(ctx, B->H.get(), d.as<double>());
+    batch_to_host(ctx, B, d.as<double>(), D);
+  });
+}
+
+int mhip_batch_set_guess(mhip_context* ctx, mhip_batch* B, const double* X) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B, MHIP_ERR_INVALID_ARG, "null batch");
+    if(!X) {
+      B->has_guess = false;
+      return;
+    }
+    std::vector<double> tmp(B->N);
+    int64_t o = 0;
+    for(int b = 0; b < B->ninst; ++b) {
+      const int p = internal_pos(B, b);
+      const int64_t n = B->off[p + 1] - B->off[p];
+      std::memcpy(tmp.data() + B->off[p], X + o, 8 * n);
+      o += n;
+    }
+    B->guess.alloc(8 * B->N);
+    MHIP_HIP_CHECK(hipMemcpyAsync(B->guess.p, tmp.data(), 8 * B->N, hipMemcpyHostToDevice,
+                                  ctx->stream));
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    B->has_guess = true;
+  });
+}
+
+int mhip_batch_vectors(mhip_context* ctx, mhip_batch* B, double* X) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B && X, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_REQUIRE(B->max_m > 0, MHIP_ERR_INVALID_ARG, "run mhip_batch_davidson first");
+    batch_to_host(ctx, B, B->X.as<double>(), X);
+  });
+}
+
+int mhip_batch_instance_csr(mhip_context* ctx, mhip_batch* B, int inst, int64_t* rowptr,
+                            int32_t* colind, double* nzval, int64_t* nnz) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B && B->H && nnz, MHIP_ERR_INVALID_ARG, "build the batch first");
+    MHIP_REQUIRE(inst >= 0 && inst < B->ninst, MHIP_ERR_INVALID_ARG, "instance out of range");
+    const int p = internal_pos(B, inst);
+    const int64_t r0 = B->off[p], m = B->off[p + 1] - r0;
+    std::vector<int64_t> rp(m + 1);
+    d2h(ctx, rp.data(), B->H->rowptr.as<int64_t>() + r0, 8 * (m + 1));
+    const int64_t base = rp[0], cnt = rp[m] - base;
+    *nnz = cnt;
+    if(!rowptr) return;
+    MHIP_REQUIRE(colind && nzval, MHIP_ERR_INVALID_ARG, "null output array");
+    for(int64_t i = 0; i <= m; ++i) rowptr[i] = rp[i] - base;
+    if(!cnt) return;
+    d2h(ctx, colind, B->H->colind.as<int32_t>() + base, 4 * cnt);
+    d2h(ctx, nzval, B->H->nzval.as<double>() + base, 8 * cnt);
+    for(int64_t j = 0; j < cnt; ++j) colind[j] -= (int32_t)r0;
   });
 }
 

@@ -134,3 +134,45 @@ def selected_ci_dets(n: int, nalpha: int, nbeta: int, ndets: int, seed: int,
     b = np.concatenate(b_all)[:ndets]
     srt = np.lexsort((a, b))
     return np.ascontiguousarray(np.stack([a[srt], b[srt]], axis=1))
+
+
+def cas_batch_instances(count: int = 256, seed: int = 2024, sizes_from=(6, 8, 10)):
+    """The batched CAS-CI workload: `count` instances of CAS(n e, n o) with n
+    drawn uniformly (seeded) from `sizes_from`, n/2 alpha + n/2 beta
+    electrons, the complete determinant list of generate_hilbert_space
+    (wfn_t<64> words) and integrals seeded_integrals(n, 1000 + i) for
+    instance i. Returns (norbs, det_lists, Ts, Vs)."""
+    from . import dets as D
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice(list(sizes_from), size=count)
+    cache, norbs, dl, Ts, Vs = {}, [], [], [], []
+    for i, n in enumerate(sizes):
+        n = int(n)
+        T, V = seeded_integrals(n, 1000 + i)
+        if n not in cache:
+            cache[n] = D.generate_hilbert_space(1, n, n // 2, n // 2)[:, 0]
+        norbs.append(n)
+        dl.append(cache[n])
+        Ts.append(T)
+        Vs.append(V)
+    return norbs, dl, Ts, Vs
+
+
+def perturbed_diagonal_guess(D, delta: float = 1e-3, seed: int = 0):
+    """Davidson guess that reaches the lowest eigenvalue of any spin sector:
+    the unit vector at the first minimum of D (diagonal_guess,
+    davidson.hpp:61-68) plus `delta` times a seeded uniform unit vector,
+    normalised. With equal alpha and beta counts a closed-shell unit vector
+    is even under the alpha <-> beta exchange that commutes with H, and the
+    diagonal preconditioner keeps that parity, so from the plain diagonal
+    guess Davidson converges to the lowest even state; when the ground state
+    is odd, only round-off can lead it there. The seeded component gives
+    every sector a weight of order delta, far above any residual tolerance.
+    """
+    D = np.asarray(D, dtype=np.float64)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = rng.uniform(-1.0, 1.0, size=D.size)
+    X = delta * r / np.linalg.norm(r)
+    X[int(np.argmin(D))] += 1.0
+    return X / np.linalg.norm(X)
