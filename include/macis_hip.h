@@ -259,6 +259,18 @@ int mhip_gemm_tn(mhip_context* ctx, int64_t N, int k, const double* d_A,
                  int64_t lda, int m, const double* d_B, int64_t ldb,
                  double* d_C, int ldc);
 
+/* mhip_gemm_tn without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The block partials
+ * (at most 256 blocks x 1024 doubles = 2 MiB, the maximum of every N) live in
+ * a static 2 MiB device array per GPU, set up when a context is created on
+ * that GPU and held until the process ends. Same arguments, limits and
+ * result as mhip_gemm_tn. Calls that overlap on one GPU (two contexts or
+ * threads at once) share the array and race. */
+int mhip_gemm_tn_noalloc(mhip_context* ctx, int64_t N, int k, const double* d_A,
+                         int64_t lda, int m, const double* d_B, int64_t ldb,
+                         double* d_C, int ldc);
+
 /* selected_ci_diag (selected_ci_diag.hpp:124-207, serial path): build H over
  * DEVICE d_dets with threshold h_el_tol, extract the diagonal, guess rule
  * (reuse d_C if max|C| > 1/ndets, else unit vector at argmin D), Davidson.

@@ -113,6 +113,7 @@ _SIGS = {
     "mhip_extract_diagonal": (_i32, [_p, _p, _p]),
     "mhip_csr_prepare_spmv": (_i32, [_p, _p, _i32, _p]),
     "mhip_gemm_tn": (_i32, [_p, _i64, _i32, _p, _i64, _i32, _p, _i64, _p, _i32]),
+    "mhip_gemm_tn_noalloc": (_i32, [_p, _i64, _i32, _p, _i64, _i32, _p, _i64, _p, _i32]),
     "mhip_davidson": (_i32, [_p, _p, _i64, _i64, _d, _p, _p, _pd, _pi64]),
     "mhip_selected_ci_diag": (_i32, [_p, _i32, _p, _i64, _d, _i64, _i64, _d, _p, _pd, _pi64, _pp]),
     "mhip_asci_search": (_i32, [_p, _i32, C.POINTER(ASCISettings), _i64, _p, _i64, _d, _p, _i64,

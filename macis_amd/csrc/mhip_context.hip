@@ -2,12 +2,42 @@
 // the C-ABI (include/macis_hip.h).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "mhip_internal.hpp"
 
+// Device scratch of the allocation-free entry points (mhip_gemm_tn_noalloc):
+// one array per GPU, held until the process ends. Host code never uses this
+// name as an address: the per-device address comes from hipGetSymbolAddress
+// at context creation and is kept in g_scratch_addr.
+static __device__ __align__(256) unsigned char scratch[mhip::kNoAllocScratchBytes];
+
 namespace mhip {
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {nullptr};
+
+// Looks up this device's copy of `scratch` (the device must be current).
+void init_noalloc_scratch(int device) {
+  MHIP_REQUIRE(device >= 0 && device < kMaxDevices, MHIP_ERR_INVALID_ARG,
+               "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if(g_scratch_addr[device]) return;
+  void* p = nullptr;
+  MHIP_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  MHIP_REQUIRE(p, MHIP_ERR_HIP, "hipGetSymbolAddress returned no address");
+  g_scratch_addr[device] = p;
+}
+}  // namespace
+
+void* noalloc_scratch(int device) {
+  if(device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_addr[device];
+}
 
 namespace {
 // Trivially destructible on purpose: contexts may be destroyed from static
@@ -129,6 +159,7 @@ int mhip_context_create(int device, mhip_context** out) {
     ctx->device = device;
     try {
       MHIP_HIP_CHECK(hipSetDevice(device));
+      init_noalloc_scratch(device);
       // Default priority stream (non-blocking with respect to stream 0).
       MHIP_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     } catch(...) {

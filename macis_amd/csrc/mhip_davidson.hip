@@ -700,18 +700,25 @@ void guess_rule(mhip_context* ctx, int64_t N, const double* d_D, double* d_C) {
 }
 
 // Tall-skinny GEMM entry used by tests and the C++ layer: C = A^T B.
+// C = A^T B through block partials in `part` (gram_blocks(N) * 1024
+// doubles), waited for.
 void gemm_tn(mhip_context* ctx, int64_t N, int k, const double* A, int64_t lda, int m,
-             const double* B, int64_t ldb, double* d_C, int ldc) {
-  MHIP_REQUIRE(k >= 1 && k <= 32 && m >= 1 && m <= 32, MHIP_ERR_INVALID_ARG,
-               "k, m must be in [1, 32]");
+             const double* B, int64_t ldb, double* d_C, int ldc, double* part) {
   const int nb = gram_blocks(N);
-  DevBuf part(sizeof(double) * nb * 4 * 256);
   hipLaunchKernelGGL((gram_mfma_kernel<2, 2>), dim3(nb), dim3(256), 0, ctx->stream, N, k, A,
-                     lda, m, B, ldb, part.as<double>());
+                     lda, m, B, ldb, part);
   hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(256), 0, ctx->stream,
-                     part.as<double>(), nb, k, m, d_C, ldc);
+                     part, nb, k, m, d_C, ldc);
   MHIP_HIP_CHECK(hipGetLastError());
   MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void check_gemm_tn_args(int64_t N, int k, const double* A, int64_t lda, int m, const double* B,
+                        int64_t ldb, const double* C, int ldc) {
+  MHIP_REQUIRE(A && B && C && lda >= N && ldb >= N && ldc >= k,
+               MHIP_ERR_INVALID_ARG, "bad arguments");
+  MHIP_REQUIRE(k >= 1 && k <= 32 && m >= 1 && m <= 32, MHIP_ERR_INVALID_ARG,
+               "k, m must be in [1, 32]");
 }
 
 }  // namespace mhip
@@ -769,9 +776,26 @@ int mhip_gemm_tn(mhip_context* ctx, int64_t N, int k, const double* d_A,
                  int ldc) {
   return guarded([&] {
     use_device(ctx);
-    MHIP_REQUIRE(d_A && d_B && d_C && lda >= N && ldb >= N && ldc >= k,
-                 MHIP_ERR_INVALID_ARG, "bad arguments");
-    gemm_tn(ctx, N, k, d_A, lda, m, d_B, ldb, d_C, ldc);
+    check_gemm_tn_args(N, k, d_A, lda, m, d_B, ldb, d_C, ldc);
+    DevBuf part(sizeof(double) * gram_blocks(N) * 4 * 256);
+    gemm_tn(ctx, N, k, d_A, lda, m, d_B, ldb, d_C, ldc, part.as<double>());
+  });
+}
+
+// The same product with its block partials in the device's static scratch
+// array: no allocation. Calls that overlap on one GPU share that array and
+// race.
+int mhip_gemm_tn_noalloc(mhip_context* ctx, int64_t N, int k, const double* d_A, int64_t lda,
+                         int m, const double* d_B, int64_t ldb, double* d_C, int ldc) {
+  return guarded([&] {
+    use_device(ctx);
+    check_gemm_tn_args(N, k, d_A, lda, m, d_B, ldb, d_C, ldc);
+    double* part = static_cast<double*>(noalloc_scratch(ctx->device));
+    MHIP_REQUIRE(part, MHIP_ERR_HIP, "device scratch not set up");
+    const size_t need = sizeof(double) * (size_t)gram_blocks(N) * 4 * 256;
+    MHIP_REQUIRE(need <= kNoAllocScratchBytes, MHIP_ERR_INVALID_ARG,
+                 "partials exceed the 2 MiB device scratch");
+    gemm_tn(ctx, N, k, d_A, lda, m, d_B, ldb, d_C, ldc, part);
   });
 }
 

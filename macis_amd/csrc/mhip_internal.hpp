@@ -194,6 +194,12 @@ struct mhip_csr {
 namespace mhip {
 // Makes the context's device current.
 void use_device(mhip_context* ctx);
+// Scratch array of the allocation-free entry points: its size, the devices
+// the address table covers, and this device's address of it (nullptr until a
+// context was created on the device).
+constexpr size_t kNoAllocScratchBytes = size_t(2) << 20;
+constexpr int kMaxDevices = 64;
+void* noalloc_scratch(int device);
 // Device-to-host copy on the context stream, waited for; copies up to 4 MiB
 // are staged through the context's pinned buffer.
 void d2h(mhip_context* ctx, void* dst, const void* src, size_t bytes);
