@@ -246,7 +246,9 @@ int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
  * max_restarts > 0 (a labelled variant, not in the reference) an exhausted
  * subspace restarts from [X, previous Ritz vector, next preconditioned
  * residual] with A applied by combination of the stored products. E receives
- * LAM(0), iters the iteration count (davidson.hpp:224). */
+ * LAM(0), iters the iteration count (davidson.hpp:224). max_m is clamped to
+ * N (davidson.hpp:140) and may be up to 1024; subspaces wider than 32
+ * columns are projected in 32-column blocks. */
 int mhip_davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
                   int64_t max_restarts, double tol, const double* d_D,
                   double* d_X, double* E, int64_t* iters);

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <fstream>
 #include <functional>
+#include <random>
+#include <unistd.h>
 #include <string>
 #include <vector>
 
@@ -130,6 +132,222 @@ static void test_host_generator_elements(const H2O& h) {
   REQUIRE(threw);
 }
 
+// A generator of the reference's base class with no CSR of its own: the
+// per-pair members are all it is used for.
+template <size_t N>
+struct HostOnly : macis::HamiltonianGenerator<N> {
+  using base = macis::HamiltonianGenerator<N>;
+  using base::base;
+  using typename base::full_det_iterator;
+  using typename base::matrix_span_t;
+  using typename base::rank4_span_t;
+  typename base::template sparse_matrix_type<int32_t> make_csr_hamiltonian_block_32bit_(
+      full_det_iterator, full_det_iterator, full_det_iterator, full_det_iterator,
+      double) override {
+    throw std::runtime_error("unused");
+  }
+  typename base::template sparse_matrix_type<int64_t> make_csr_hamiltonian_block_64bit_(
+      full_det_iterator, full_det_iterator, full_det_iterator, full_det_iterator,
+      double) override {
+    throw std::runtime_error("unused");
+  }
+  void form_rdms(full_det_iterator, full_det_iterator, full_det_iterator, full_det_iterator,
+                 double*, matrix_span_t, rank4_span_t) override {}
+};
+
+// Host RDMs of a wavefunction through the generator's rdm_contributions, the
+// loop of double_loop.hpp:116-165 (pairs with popcount(bra^ket) <= 4 and
+// |C_i C_j| > 1e-16).
+template <size_t N, typename Gen>
+static void host_rdms(Gen& g, const std::vector<macis::wfn_t<N>>& dets, const double* C,
+                      macis::matrix_span<double> ordm, macis::rank4_span<double> trdm) {
+  for(size_t i = 0; i < dets.size(); ++i) {
+    const auto ba = g.alpha_string(dets[i]), bb = g.beta_string(dets[i]);
+    const auto oa = macis::bits_to_indices(ba), ob = macis::bits_to_indices(bb);
+    for(size_t j = 0; j < dets.size(); ++j) {
+      if((dets[i] ^ dets[j]).count() > 4) continue;
+      const double val = C[i] * C[j];
+      if(std::abs(val) <= 1e-16) continue;
+      const auto ka = g.alpha_string(dets[j]), kb = g.beta_string(dets[j]);
+      g.rdm_contributions(ba, ka, ba ^ ka, bb, kb, bb ^ kb, oa, ob, val, ordm, trdm);
+    }
+  }
+}
+
+static void test_generator_members(const H2O& h) {
+  // tests/double_loop.cxx:55-87: fast diagonals of excited determinants
+  // from the HF diagonal, called as the reference's test calls them
+  auto T = h.T;
+  auto V = h.V;
+  const size_t norb = h.norb, nocc = 5;
+  HostOnly<64> ham_gen(macis::matrix_span<double>(T.data(), norb, norb),
+                       macis::rank4_span<double>(V.data(), norb, norb, norb, norb));
+  const auto hf_det = macis::canonical_hf_determinant<64>(nocc, nocc);
+  const auto EHF = ham_gen.matrix_element(hf_det, hf_det);
+  std::vector<uint32_t> occ = {0, 1, 2, 3, 4};
+  {
+    auto state = hf_det;
+    state.flip(0).flip(nocc);
+    const auto ES = ham_gen.matrix_element(state, state);
+    REQUIRE_NEAR(ES, -6.488097259228e+01, 1e-10);
+    REQUIRE_NEAR(ham_gen.fast_diag_single(occ, occ, 0, nocc, EHF), ES, 1e-10);
+  }
+  {
+    auto state = hf_det;
+    state.flip(0).flip(nocc).flip(1).flip(nocc + 1);
+    const auto ED = ham_gen.matrix_element(state, state);
+    REQUIRE_NEAR(ED, -6.314093508151e+01, 1e-10);
+    REQUIRE_NEAR(ham_gen.fast_diag_ss_double(occ, occ, 0, 1, nocc, nocc + 1, EHF), ED, 1e-10);
+  }
+  {
+    auto state = hf_det;
+    state.flip(0).flip(nocc).flip(1 + 32).flip(nocc + 1 + 32);
+    const auto ED = ham_gen.matrix_element(state, state);
+    REQUIRE_NEAR(ED, -6.304547887231e+01, 1e-10);
+    REQUIRE_NEAR(ham_gen.fast_diag_os_double(occ, occ, 0, 1, nocc, nocc + 1, EHF), ED, 1e-10);
+  }
+  // orbital energies: the vector form equals the scalar form; the
+  // energy-argument fast diagonals equal the occupation-argument ones
+  const auto eps = ham_gen.single_orbital_ens(norb, occ, occ);
+  REQUIRE(eps.size() == norb);
+  bool same = true;
+  for(uint32_t p = 0; p < norb; ++p) same &= eps[p] == ham_gen.single_orbital_en(p, occ, occ);
+  REQUIRE(same);
+  REQUIRE(ham_gen.fast_diag_single(eps[2], eps[7], 2, 7, EHF) ==
+          ham_gen.fast_diag_single(occ, occ, 2, 7, EHF));
+  // every single and double of the HF determinant: fast diagonal == exact
+  double worst = 0.;
+  for(uint32_t i = 0; i < nocc; ++i)
+    for(uint32_t a = nocc; a < norb; ++a) {
+      auto s1 = hf_det;
+      s1.flip(i).flip(a);
+      worst = std::max(worst, std::abs(ham_gen.fast_diag_single(eps[i], eps[a], i, a, EHF) -
+                                       ham_gen.matrix_element(s1, s1)));
+      for(uint32_t j = 0; j < nocc; ++j)
+        for(uint32_t b = nocc; b < norb; ++b) {
+          auto s2 = s1;
+          s2.flip(j + 32).flip(b + 32);
+          worst = std::max(worst, std::abs(ham_gen.fast_diag_os_double(eps[i], eps[j], eps[a],
+                                                                       eps[b], i, j, a, b, EHF) -
+                                           ham_gen.matrix_element(s2, s2)));
+          if(j > i && b > a) {
+            auto s3 = s1;
+            s3.flip(j).flip(b);
+            worst = std::max(worst,
+                             std::abs(ham_gen.fast_diag_ss_double(eps[i], eps[j], eps[a], eps[b],
+                                                                  i, j, a, b, EHF) -
+                                      ham_gen.matrix_element(s3, s3)));
+          }
+        }
+    }
+  REQUIRE(worst < 1e-10);
+  // the per-class elements and the 8-argument dispatch equal the full-det
+  // matrix_element bit for bit (matrix_elements.hpp:14-62)
+  std::mt19937_64 rng(5);
+  bool bits = true;
+  int classes[5] = {0, 0, 0, 0, 0};
+  auto dets = macis::generate_cisd_hilbert_space<64>(norb, hf_det);
+  for(int t = 0; t < 200000; ++t) {
+    // bras among the first 400 (HF, singles, the first doubles), kets anywhere
+    const auto bra = dets[rng() % 400], ket = dets[rng() % dets.size()];
+    const auto ba = ham_gen.alpha_string(bra), ka = ham_gen.alpha_string(ket);
+    const auto bb = ham_gen.beta_string(bra), kb = ham_gen.beta_string(ket);
+    const auto ea = ba ^ ka, eb = bb ^ kb;
+    const auto oa = macis::bits_to_indices(ba), ob = macis::bits_to_indices(bb);
+    const double ref = ham_gen.matrix_element(bra, ket);
+    bits &= ham_gen.matrix_element(ba, ka, ea, bb, kb, eb, oa, ob) == ref;
+    const size_t ca = ea.count(), cb = eb.count();
+    if(ca + cb > 4) continue;
+    double cls;
+    if(ca == 4) cls = ham_gen.matrix_element_4(ba, ka, ea), ++classes[0];
+    else if(cb == 4) cls = ham_gen.matrix_element_4(bb, kb, eb), ++classes[0];
+    else if(ca == 2 && cb == 2) cls = ham_gen.matrix_element_22(ba, ka, ea, bb, kb, eb), ++classes[1];
+    else if(ca == 2) cls = ham_gen.matrix_element_2(ba, ka, ea, oa, ob), ++classes[2];
+    else if(cb == 2) cls = ham_gen.matrix_element_2(bb, kb, eb, ob, oa), ++classes[3];
+    else cls = ham_gen.matrix_element_diag(oa, ob), ++classes[4];
+    bits &= cls == ref;
+  }
+  REQUIRE(bits);
+  std::printf("  pairs per class (4, 22, 2a, 2b, diag): %d %d %d %d %d\n", classes[0], classes[1],
+              classes[2], classes[3], classes[4]);
+  REQUIRE(classes[0] > 0 && classes[1] > 0 && classes[2] > 0 && classes[3] > 0 && classes[4] > 0);
+  REQUIRE(ham_gen.alpha_string(hf_det).to_ullong() == 0x1Full);
+  REQUIRE(ham_gen.beta_string(hf_det).to_ullong() == 0x1Full);
+  // public intermediate spans over the generator's storage (impl.hpp:25-70)
+  REQUIRE(&ham_gen.G_red_(0, 0, 0) == ham_gen.G_red());
+  REQUIRE(ham_gen.G2_red_(3, 2) == 0.5 * ham_gen.G_pqrs_(3, 3, 2, 2));
+  REQUIRE(ham_gen.V2_red_(3, 2) == ham_gen.V_pqrs_(3, 3, 2, 2));
+
+  // double_loop.cxx:162-177 through the host rdm_contributions: the HF
+  // determinant's RDMs reproduce E_HF
+  {
+    const size_t n2 = norb * norb, n4 = n2 * n2;
+    std::vector<double> ordm(n2, 0.), trdm(n4, 0.);
+    std::vector<macis::wfn_t<64>> one{hf_det};
+    const double C1 = 1.;
+    host_rdms<64>(ham_gen, one, &C1, macis::matrix_span<double>(ordm.data(), norb, norb),
+                  macis::rank4_span<double>(trdm.data(), norb, norb, norb, norb));
+    double E = 0.;
+    for(size_t i = 0; i < n2; ++i) E += ordm[i] * T[i];
+    for(size_t i = 0; i < n4; ++i) E += trdm[i] * V[i];
+    REQUIRE_NEAR(E, EHF, 1e-10);
+  }
+  // double_loop.cxx:179-215 ("RDMS", HF section): wfn_t<128>, 34 orbitals,
+  // D2 of a single determinant is the antisymmetrised product of D1
+  {
+    const size_t n = 34, n2 = n * n;
+    std::vector<double> T0(n2, 0.), V0(n2 * n2, 0.), ordm(n2, 0.), trdm(n2 * n2, 0.);
+    HostOnly<128> g(macis::matrix_span<double>(T0.data(), n, n),
+                    macis::rank4_span<double>(V0.data(), n, n, n, n));
+    macis::matrix_span<double> D1(ordm.data(), n, n);
+    macis::rank4_span<double> D2(trdm.data(), n, n, n, n);
+    std::vector<macis::wfn_t<128>> one{macis::canonical_hf_determinant<128>(nocc, nocc)};
+    const double C1 = 1.;
+    host_rdms<128>(g, one, &C1, D1, D2);
+    for(size_t i = 0; i < nocc; ++i)
+      for(size_t j = 0; j < nocc; ++j)
+        for(size_t k = 0; k < nocc; ++k)
+          for(size_t l = 0; l < nocc; ++l) {
+            D2(i, j, l, k) -= 0.5 * D1(i, j) * D1(k, l);
+            D2(i, j, l, k) += 0.25 * D1(i, l) * D1(k, j);
+          }
+    double sum = 0.;
+    for(auto x : trdm) sum += std::abs(x);
+    REQUIRE(sum < 1e-15);
+    for(size_t i = 0; i < nocc; ++i) D1(i, i) -= 2.0;
+    sum = 0.;
+    for(auto x : ordm) sum += std::abs(x);
+    REQUIRE(sum < 1e-15);
+  }
+  // rotate_hamiltonian_ordm: a diagonal 1-RDM with ascending occupations
+  // rotates to the reversed orbital order (natural orbitals by descending
+  // occupation), so T'(p,q) = T(n-1-p, n-1-q) and V likewise
+  {
+    const size_t n = 6;
+    std::vector<double> Tr(n * n), Vr(n * n * n * n), D(n * n, 0.);
+    for(size_t i = 0; i < Tr.size(); ++i) Tr[i] = std::sin(1.0 + i);
+    for(size_t i = 0; i < Vr.size(); ++i) Vr[i] = std::cos(0.5 * i);
+    const auto T0 = Tr, V0 = Vr;
+    for(size_t p = 0; p < n; ++p) D[p + n * p] = 0.1 * (p + 1);
+    HostOnly<64> g(macis::matrix_span<double>(Tr.data(), n, n),
+                   macis::rank4_span<double>(Vr.data(), n, n, n, n));
+    g.rotate_hamiltonian_ordm(D.data());
+    double eT = 0., eV = 0.;
+    const size_t m = n - 1;
+    for(size_t q = 0; q < n; ++q)
+      for(size_t p = 0; p < n; ++p) {
+        eT = std::max(eT, std::abs(Tr[p + n * q] - T0[(m - p) + n * (m - q)]));
+        for(size_t r = 0; r < n; ++r)
+          for(size_t s = 0; s < n; ++s)
+            eV = std::max(eV, std::abs(Vr[p + n * (q + n * (r + n * s))] -
+                                       V0[(m - p) + n * ((m - q) + n * ((m - r) + n * (m - s)))]));
+      }
+    REQUIRE(eT < 1e-14);
+    REQUIRE(eV < 1e-14);
+    REQUIRE(g.G2_red_(0, 1) == 0.5 * g.G_pqrs_(0, 0, 1, 1));  // intermediates regenerated
+  }
+}
+
 static void test_file_formats(const std::string& dir) {
   // sd_operations.cxx:240-260: canonical strings
   auto state = macis::canonical_hf_determinant<64>(2, 2).flip(3).flip(4 + 32);
@@ -181,6 +399,60 @@ static void test_file_formats(const std::string& dir) {
     threw = std::string(e.what()) == "NORB in RDM file doesn't match 7 8";
   }
   REQUIRE(threw);
+
+  // write_fcidump (fcidump.cxx:184-207) of the water integrals, read back:
+  // 17 significant digits make the round trip exact
+  {
+    const std::string f = dir + "/h2o.ccpvdz.fci.dat";
+    const size_t n = macis::read_fcidump_norb(f);
+    const double Ec = macis::read_fcidump_core(f);
+    std::vector<double> T(n * n, 0.), V(n * n * n * n, 0.);
+    macis::read_fcidump_1body(f, T.data(), n);
+    macis::read_fcidump_2body(f, V.data(), n);
+    char tmpl[] = "/tmp/mhip_fcidump_XXXXXX";
+    const int fd = mkstemp(tmpl);
+    REQUIRE(fd >= 0);
+    close(fd);
+    std::remove(tmpl);  // write_fcidump appends, as the reference's sink does
+    macis::write_fcidump(tmpl, n, T.data(), n, V.data(), n, Ec);
+    std::ifstream in(tmpl);
+    std::string first;
+    std::getline(in, first);
+    char ref[128];
+    std::snprintf(ref, sizeof ref, "%8d %8d %8d %8d %25.16e", 1, 1, 1, 1, V[0]);
+    REQUIRE(first == ref);
+    size_t lines = 1;
+    for(std::string l; std::getline(in, l);) ++lines;
+    REQUIRE(lines == n * n * n * n + n * n + 1);
+    REQUIRE(macis::read_fcidump_norb(tmpl) == n);
+    REQUIRE(macis::read_fcidump_core(tmpl) == Ec);
+    std::vector<double> T2(n * n, 0.), V2(n * n * n * n, 0.);
+    macis::read_fcidump_1body(tmpl, T2.data(), n);
+    macis::read_fcidump_2body(tmpl, V2.data(), n);
+    REQUIRE(T2 == T);
+    REQUIRE(V2 == V);
+    // span readers (fcidump.hpp:54-71) into views wider than norb: the same
+    // entries at the view's strides, the padding untouched
+    const size_t m = n + 3;
+    std::vector<double> Tp(m * m, -7.), Vp(m * m * m * m, -7.);
+    macis::read_fcidump_1body(tmpl, macis::col_major_span<double, 2>(Tp.data(), m, m));
+    macis::read_fcidump_2body(tmpl, macis::col_major_span<double, 4>(Vp.data(), m, m, m, m));
+    bool same = true, pad = true;
+    for(size_t q = 0; q < m; ++q)
+      for(size_t p = 0; p < m; ++p) {
+        if(p < n && q < n) same &= Tp[p + m * q] == T[p + n * q];
+        else pad &= Tp[p + m * q] == -7.;
+      }
+    for(size_t l = 0; l < n; ++l)
+      for(size_t k = 0; k < n; ++k)
+        for(size_t j = 0; j < n; ++j)
+          for(size_t i = 0; i < n; ++i)
+            same &= Vp[i + m * (j + m * (k + m * l))] == V[i + n * (j + n * (k + n * l))];
+    pad &= Vp[n + m * (n + m * (n + m * n))] == -7.;
+    REQUIRE(same);
+    REQUIRE(pad);
+    std::remove(tmpl);
+  }
 }
 
 static void test_transforms() {
@@ -289,6 +561,72 @@ static void test_rdms(const H2O& h, const std::string& dir) {
   std::printf("  CAS(8,8) RDMs vs fixture: max |dD1| %.2e, max |dD2| %.2e\n", e1, e2);
   REQUIRE(e1 < 1e-7);
   REQUIRE(e2 < 1e-7);
+  // spans wider than norb (the reference indexes any extents): the corner
+  // receives the same RDMs, the padding is untouched
+  {
+    const size_t m = na + 2;
+    std::vector<double> w1(m * m, -1.), w2(m * m * m * m, -1.);
+    for(size_t q = 0; q < na; ++q)
+      for(size_t p = 0; p < na; ++p) w1[p + m * q] = 0.;
+    for(size_t l = 0; l < na; ++l)
+      for(size_t k = 0; k < na; ++k)
+        for(size_t j = 0; j < na; ++j)
+          for(size_t i = 0; i < na; ++i) w2[i + m * (j + m * (k + m * l))] = 0.;
+    cas.form_rdms(dets.begin(), dets.end(), dets.begin(), dets.end(), X.data(),
+                  macis::matrix_span<double>(w1.data(), m, m),
+                  macis::rank4_span<double>(w2.data(), m, m, m, m));
+    double e = 0.;
+    bool pad = true;
+    for(size_t q = 0; q < m; ++q)
+      for(size_t p = 0; p < m; ++p) {
+        if(p < na && q < na) e = std::max(e, std::abs(w1[p + m * q] - d1[p + na * q]));
+        else pad &= w1[p + m * q] == -1.;
+      }
+    for(size_t l = 0; l < na; ++l)
+      for(size_t k = 0; k < na; ++k)
+        for(size_t j = 0; j < na; ++j)
+          for(size_t i = 0; i < na; ++i)
+            e = std::max(e, std::abs(w2[i + m * (j + m * (k + m * l))] -
+                                     d2[i + na * (j + na * (k + na * l))]));
+    pad &= w2[na + m * (0 + m * (0 + m * 0))] == -1. && w2[m * m * m * m - 1] == -1.;
+    REQUIRE(e < 1e-12);
+    REQUIRE(pad);
+  }
+  // the generator's host rdm_contributions over the same pairs equal the
+  // device form_rdms up to summation order
+  std::vector<double> h1(na * na, 0.), h2(na * na * na * na, 0.);
+  host_rdms<64>(cas, dets, X.data(), macis::matrix_span<double>(h1.data(), na, na),
+                macis::rank4_span<double>(h2.data(), na, na, na, na));
+  double g1 = 0., g2 = 0.;
+  for(size_t i = 0; i < d1.size(); ++i) g1 = std::max(g1, std::abs(d1[i] - h1[i]));
+  for(size_t i = 0; i < d2.size(); ++i) g2 = std::max(g2, std::abs(d2[i] - h2[i]));
+  std::printf("  host rdm_contributions vs device form_rdms: %.2e %.2e\n", g1, g2);
+  REQUIRE(g1 < 1e-12);
+  REQUIRE(g2 < 1e-12);
+  // rotate_hamiltonian_ordm to the natural orbitals keeps the CAS-CI energy
+  // and makes the new 1-RDM diagonal with descending occupations
+  double E0 = 0.;
+  {
+    std::vector<double> Xr;
+    E0 = macis::selected_ci_diag<64, int32_t>(dets.begin(), dets.end(), cas, 1e-16, 20, 1e-8, Xr);
+  }
+  cas.rotate_hamiltonian_ordm(d1.data());
+  std::vector<double> Xn;
+  const double E1 =
+      macis::selected_ci_diag<64, int32_t>(dets.begin(), dets.end(), cas, 1e-16, 20, 1e-8, Xn);
+  REQUIRE_NEAR(E1, E0, 1e-9);
+  std::vector<double> n1(na * na, 0.), nt(na * na * na * na, 0.);
+  cas.form_rdms(dets.begin(), dets.end(), dets.begin(), dets.end(), Xn.data(),
+                macis::matrix_span<double>(n1.data(), na, na),
+                macis::rank4_span<double>(nt.data(), na, na, na, na));
+  double off = 0.;
+  for(size_t q = 0; q < na; ++q)
+    for(size_t p = 0; p < na; ++p)
+      if(p != q) off = std::max(off, std::abs(n1[p + na * q]));
+  REQUIRE(off < 1e-6);
+  bool desc = true;
+  for(size_t p = 1; p < na; ++p) desc &= n1[p + na * p] <= n1[(p - 1) + na * (p - 1)] + 1e-9;
+  REQUIRE(desc);
 }
 
 static void test_csr_hamiltonian(const H2O& h, const std::string& dir) {
@@ -374,6 +712,19 @@ static void test_spmbv() {
     sparsexx::spblas::gespmbv(1, 1., A, V.data(), 6, 0., AV.data(), 6);
     std::vector<double> ref = {4, 8, 12, 16, 20, 17};
     REQUIRE(AV == ref);
+    // the device-resident overload: matrix uploaded once, device blocks
+    auto& ctx = macis::hip::default_context();
+    std::vector<int64_t> rp(A.rowptr().begin(), A.rowptr().end());
+    mhip_csr* h = nullptr;
+    macis::hip::check(mhip_csr_create_from_host(ctx.get(), 6, 6, 16, indexing, rp.data(),
+                                                A.colind().data(), A.nzval().data(), &h));
+    macis::hip::DeviceCSR dA(h);
+    macis::hip::DeviceBuffer<double> dV(ctx, V.data(), 6), dAV(ctx, AV.data(), 6);
+    sparsexx::spblas::gespmbv(1, 2., dA, dV.data(), 6, -1., dAV.data(), 6);
+    std::vector<double> out(6);
+    dAV.download(out.data(), 6);
+    std::vector<double> ref2 = {4, 8, 12, 16, 20, 17};  // 2 A V - A V
+    REQUIRE(out == ref2);
   }
 }
 
@@ -404,6 +755,21 @@ static void test_asci(const H2O& h) {
   REQUIRE_NEAR(E0, -8.542925964708e+01, 1e-8);
   REQUIRE(dets.size() == 10000);
   std::printf("  refined, E0 = %.12f\n", E0);
+  // asci_search over a generator of the reference's base class (no device
+  // copy of its own): the integral pointers of the signature go to the
+  // device, and the result equals the device generator's
+  auto T2 = h.T;
+  auto V2 = h.V;
+  HostOnly<64> host_gen(macis::matrix_span<double>(T2.data(), h.norb, h.norb),
+                        macis::rank4_span<double>(V2.data(), h.norb, h.norb, h.norb, h.norb));
+  macis::reorder_ci_on_coeff(dets, C);
+  auto a = macis::asci_search(asci_settings, 3000, dets.begin(), dets.begin() + 100, E0, C,
+                              h.norb, gen.T(), gen.G_red(), gen.V_red(), gen.G(), gen.V(), gen);
+  auto b = macis::asci_search(asci_settings, 3000, dets.begin(), dets.begin() + 100, E0, C,
+                              h.norb, host_gen.T(), host_gen.G_red(), host_gen.V_red(),
+                              host_gen.G(), host_gen.V(), host_gen);
+  REQUIRE(a.size() == 3000);
+  REQUIRE(a == b);
 }
 
 static void test_asci_rot(const H2O& h, double ref_E0) {
@@ -537,6 +903,8 @@ int main(int argc, char** argv) {
   std::vector<std::pair<std::string, std::function<void()>>> cases = {
       {"sd_operations", [] { test_sd_operations(); }},
       {"host generator elements", [&] { test_host_generator_elements(h); }},
+      {"generator members (fast diagonals, elements, RDM contributions, rotation)",
+       [&] { test_generator_members(h); }},
       {"file formats (wavefunction, binary RDMs)", [&] { test_file_formats(dir); }},
       {"orbital transforms", [] { test_transforms(); }},
   };

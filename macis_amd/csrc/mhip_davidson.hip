@@ -26,7 +26,8 @@ namespace {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kMaxK = 32;  // columns handled by the projection kernels
+constexpr int kMaxK = 32;  // columns handled by one launch of the projection kernels
+constexpr int64_t kMaxSubspace = 1024;  // widest Davidson subspace (max_m)
 constexpr int kRedBlocks = 512;
 constexpr int kGramBlocks = 256;  // partial Gram blocks (4 wavefronts each)
 
@@ -248,6 +249,70 @@ __global__ __launch_bounds__(256) void restart_kernel(int64_t N, int k, int nb, 
   }
 }
 
+// residual_kernel for subspaces wider than kMaxK: the Ritz coefficients
+// c[0..k) come from device memory through LDS (k * 8 bytes of dynamic LDS).
+__global__ __launch_bounds__(256) void residual_wide_kernel(int64_t N, int k,
+                                                             const double* __restrict__ V,
+                                                             const double* __restrict__ AV,
+                                                             int64_t ld,
+                                                             const double* __restrict__ c,
+                                                             double lam, double* __restrict__ X,
+                                                             double* __restrict__ R,
+                                                             double* __restrict__ partials) {
+  extern __shared__ double cs[];
+  for(int j = threadIdx.x; j < k; j += 256) cs[j] = c[j];
+  __syncthreads();
+  double acc = 0.;
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256) {
+    double x = 0., r = 0.;
+    for(int j = 0; j < k; ++j) {
+      x = fma(V[n + j * ld], cs[j], x);
+      r = fma(AV[n + j * ld], cs[j], r);
+    }
+    X[n] = x;
+    const double rr = r + (-lam) * x;
+    R[n] = rr;
+    acc = fma(rr, rr, acc);
+  }
+  __shared__ double lds[4];
+  const double v = wave_sum(acc);
+  if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if(threadIdx.x == 0) partials[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+// restart_kernel for subspaces wider than kMaxK: q[0..k) and q[k..2k) (the
+// two restart coefficient vectors) from device memory through LDS.
+__global__ __launch_bounds__(256) void restart_wide_kernel(int64_t N, int k, int nb,
+                                                            const double* __restrict__ q,
+                                                            double* V, double* AV, int64_t ld,
+                                                            int64_t tcol) {
+  extern __shared__ double qs[];
+  for(int j = threadIdx.x; j < 2 * k; j += 256) qs[j] = q[j];
+  __syncthreads();
+  for(int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+      n += (int64_t)gridDim.x * 256) {
+    double v0 = 0., v1 = 0., a0 = 0., a1 = 0.;
+    for(int j = 0; j < k; ++j) {
+      const double x = V[n + j * ld], y = AV[n + j * ld];
+      v0 = fma(x, qs[j], v0);
+      a0 = fma(y, qs[j], a0);
+      v1 = fma(x, qs[k + j], v1);
+      a1 = fma(y, qs[k + j], a1);
+    }
+    const double t = V[n + tcol * ld], at = AV[n + tcol * ld];
+    V[n] = v0;
+    AV[n] = a0;
+    if(nb > 1) {
+      V[n + ld] = v1;
+      AV[n + ld] = a1;
+    }
+    V[n + nb * ld] = t;
+    AV[n + nb * ld] = at;
+  }
+}
+
 // R = -R / (D - lam)   (davidson.hpp:212-214)
 __global__ void precond_kernel(int64_t N, double* __restrict__ R,
                                const double* __restrict__ D, double lam) {
@@ -311,28 +376,37 @@ __global__ void set_unit_kernel(double* X, int64_t i) { X[i] = 1.; }
 struct Ops {
   mhip_context* ctx;
   int64_t N;
+  int KS;        // stride of the small arrays: max(kMaxK, widest subspace)
   DevBuf part;   // reduction partials
   DevBuf small;  // reduced results (k inner products, norms, C)
-  Ops(mhip_context* c, int64_t n) : ctx(c), N(n) {
+  DevBuf wide;   // Ritz / restart coefficients of subspaces wider than kMaxK
+  Ops(mhip_context* c, int64_t n, int kmax) : ctx(c), N(n), KS(std::max(kMaxK, kmax)) {
     part.alloc(sizeof(double) * std::max(kRedBlocks * kMaxK, kGramBlocks * 4 * 256));
-    small.alloc(sizeof(double) * (kMaxK * kMaxK + 4 * kMaxK));
+    small.alloc(sizeof(double) * ((size_t)KS * KS + 4 * (size_t)KS));
+    if(KS > kMaxK) wide.alloc(sizeof(double) * 2 * (size_t)KS);
   }
   hipStream_t st() const { return ctx->stream; }
   double* inner() { return small.as<double>(); }
-  double* scalar() { return small.as<double>() + kMaxK; }
-  double* rn2() { return small.as<double>() + 2 * kMaxK; }  // residual norm^2, then C
-  double* Cdev() { return small.as<double>() + 4 * kMaxK; }
+  double* scalar() { return small.as<double>() + KS; }
+  double* rn2() { return small.as<double>() + 2 * KS; }  // residual norm^2, then C
+  double* Cdev() { return small.as<double>() + 4 * KS; }
 
+  // out[j] = V(:,j) . w, j < k, in column blocks of kMaxK
   void dots(const double* V, int64_t ldv, int k, const double* w, double* out) {
     const int nb = red_blocks(N);
-    hipLaunchKernelGGL(dots_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, ldv, w,
-                       part.as<double>());
-    hipLaunchKernelGGL(reduce_kernel, dim3(k), dim3(64), 0, st(), part.as<double>(), nb, kMaxK,
-                       out);
+    for(int j0 = 0; j0 < k; j0 += kMaxK) {
+      const int kc = std::min(kMaxK, k - j0);
+      hipLaunchKernelGGL(dots_kernel, dim3(nb), dim3(256), 0, st(), N, kc, V + j0 * ldv, ldv, w,
+                         part.as<double>());
+      hipLaunchKernelGGL(reduce_kernel, dim3(kc), dim3(64), 0, st(), part.as<double>(), nb,
+                         kMaxK, out + j0);
+    }
   }
+  // w -= V(:, :k) coef, in column blocks of kMaxK
   void update(const double* V, int64_t ldv, int k, const double* coef, double* w) {
-    hipLaunchKernelGGL(update_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N, k, V, ldv,
-                       coef, w);
+    for(int j0 = 0; j0 < k; j0 += kMaxK)
+      hipLaunchKernelGGL(update_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N,
+                         std::min(kMaxK, k - j0), V + j0 * ldv, ldv, coef + j0, w);
   }
   // gram_schmidt (davidson.hpp:111-126): CGS2 against V(:, :k), normalise.
   void gram_schmidt(const double* V, int k, double* w) {
@@ -346,7 +420,8 @@ struct Ops {
     dots(w, N, 1, w, scalar());
     hipLaunchKernelGGL(scale_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N, w, scalar());
   }
-  // C (k x k, ldc = k) = V^T AV on MFMA into Cdev() (not waited for).
+  // C (k x k, ldc = k) = V^T AV on MFMA into Cdev() (not waited for); wider
+  // subspaces in 32 x 32 tiles.
   void gram(const double* V, const double* AV, int k) {
     const int nb = gram_blocks(N);
     if(k <= 16) {
@@ -355,29 +430,67 @@ struct Ops {
       hipLaunchKernelGGL((gram_finalize_kernel<1, 1>), dim3(4), dim3(256), 0, st(),
                          part.as<double>(), nb, k, k, Cdev(), k);
     } else {
-      hipLaunchKernelGGL((gram_mfma_kernel<2, 2>), dim3(nb), dim3(256), 0, st(), N, k, V, N, k,
-                         AV, N, part.as<double>());
-      hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(256), 0, st(),
-                         part.as<double>(), nb, k, k, Cdev(), k);
+      for(int i0 = 0; i0 < k; i0 += kMaxK)
+        for(int j0 = 0; j0 < k; j0 += kMaxK) {
+          const int ki = std::min(kMaxK, k - i0), kj = std::min(kMaxK, k - j0);
+          hipLaunchKernelGGL((gram_mfma_kernel<2, 2>), dim3(nb), dim3(256), 0, st(), N, ki,
+                             V + i0 * N, N, kj, AV + j0 * N, N, part.as<double>());
+          hipLaunchKernelGGL((gram_finalize_kernel<2, 2>), dim3(16), dim3(256), 0, st(),
+                             part.as<double>(), nb, ki, kj, Cdev() + i0 + (int64_t)j0 * k, k);
+        }
     }
     MHIP_HIP_CHECK(hipGetLastError());
   }
   // X = V c; R = AV c - lam X; ||R||^2 into rn2() (not waited for).
-  void residual(const double* V, const double* AV, int k, const RitzCoef& rc, double* X,
+  void residual(const double* V, const double* AV, int k, const double* c, double lam, double* X,
                 double* R) {
     const int nb = red_blocks(N);
-    hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, AV, N, rc, X, R,
-                       part.as<double>());
+    if(k <= kMaxK) {
+      RitzCoef rc;
+      for(int j = 0; j < kMaxK; ++j) rc.c[j] = j < k ? c[j] : 0.;
+      rc.lam = lam;
+      hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, st(), N, k, V, AV, N, rc, X, R,
+                         part.as<double>());
+    } else {
+      MHIP_HIP_CHECK(hipMemcpyAsync(wide.p, c, sizeof(double) * k, hipMemcpyHostToDevice, st()));
+      hipLaunchKernelGGL(residual_wide_kernel, dim3(nb), dim3(256), sizeof(double) * k, st(), N,
+                         k, V, AV, N, wide.as<double>(), lam, X, R, part.as<double>());
+    }
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(64), 0, st(), part.as<double>(), nb, 1,
                        rn2());
     MHIP_HIP_CHECK(hipGetLastError());
   }
+  // In-place restart [V Q, t] (restart_kernel) with q0 / q1 of length k.
+  void restart(int k, int nb, const double* q0, const double* q1, double* V, double* AV,
+               int64_t tcol) {
+    if(k <= kMaxK) {
+      RestartCoef Q{};
+      for(int j = 0; j < k; ++j) {
+        Q.q[0][j] = q0[j];
+        Q.q[1][j] = q1[j];
+      }
+      hipLaunchKernelGGL(restart_kernel, dim3(red_blocks(N)), dim3(256), 0, st(), N, k, nb, Q,
+                         V, AV, N, tcol);
+    } else {
+      std::vector<double> q(2 * (size_t)k);
+      std::copy_n(q0, k, q.begin());
+      std::copy_n(q1, k, q.begin() + k);
+      MHIP_HIP_CHECK(hipMemcpyAsync(wide.p, q.data(), sizeof(double) * 2 * k,
+                                    hipMemcpyHostToDevice, st()));
+      // the host vector must outlive the copy
+      MHIP_HIP_CHECK(hipStreamSynchronize(st()));
+      hipLaunchKernelGGL(restart_wide_kernel, dim3(red_blocks(N)), dim3(256),
+                         sizeof(double) * 2 * k, st(), N, k, nb, wide.as<double>(), V, AV, N,
+                         tcol);
+    }
+    MHIP_HIP_CHECK(hipGetLastError());
+  }
   // One wait for the stream: the k x k Gram matrix and the last ||R||^2.
   void read_gram(int k, double* C_host, double* r2) {
-    double buf[2 * kMaxK + kMaxK * kMaxK];
-    d2h(ctx, buf, rn2(), sizeof(double) * (2 * kMaxK + k * k));
+    std::vector<double> buf(2 * (size_t)KS + (size_t)k * k);
+    d2h(ctx, buf.data(), rn2(), sizeof(double) * buf.size());
     *r2 = buf[0];
-    std::memcpy(C_host, buf + 2 * kMaxK, sizeof(double) * k * k);
+    std::memcpy(C_host, buf.data() + 2 * KS, sizeof(double) * k * k);
   }
   double read_r2() {
     double r2 = 0.;
@@ -538,12 +651,13 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
     return false;
   }
   max_m = std::min(max_m, N);  // davidson.hpp:140
-  MHIP_REQUIRE(max_m >= 1 && max_m < kMaxK, MHIP_ERR_INVALID_ARG,
-               "max_m must be in [1, 31]");
+  MHIP_REQUIRE(max_m >= 1 && max_m <= kMaxSubspace, MHIP_ERR_INVALID_ARG,
+               "max_m must be in [1, 1024]");
   hipStream_t st = ctx->stream;
   Timer t_all(st);
   t_all.start();
-  Ops ops(ctx, N);
+  const int kmax = (int)max_m + 1;
+  Ops ops(ctx, N, kmax);
   const int64_t ld = N;
   // subspace storage comes from (and goes back to) the context's kept
   // buffers: solves of the same size reuse it without remapping memory
@@ -590,8 +704,10 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   // converged iteration, exactly as if the loop had stopped there.
   bool converged = false;
   int64_t iter = 1, restarts = 0, i0 = 1, i = 1;
-  double C[kMaxK * kMaxK], LAM[kMaxK];
-  double ccur[kMaxK] = {}, cprev[kMaxK] = {}, c2cur[kMaxK] = {};
+  std::vector<double> Cv((size_t)kmax * kmax), LAMv(kmax), ccurv(kmax, 0.), cprevv(kmax, 0.),
+      c2curv(kmax, 0.), q0(kmax, 0.), q1(kmax, 0.);
+  double *C = Cv.data(), *LAM = LAMv.data(), *ccur = ccurv.data(), *cprev = cprevv.data(),
+         *c2cur = c2curv.data();
   bool prev_valid = false, pending = false, pending_restart = false;
   int kcur = 0;
   double lam0 = 0., rnorm = 0.;
@@ -614,10 +730,7 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
     }
     syev_lower(k, C, k, LAM);
     lam0 = LAM[0];
-    RitzCoef rc;
-    for(int j = 0; j < kMaxK; ++j) rc.c[j] = j < k ? C[j] : 0.;
-    rc.lam = lam0;
-    std::memcpy(cprev, ccur, sizeof(ccur));
+    std::copy(ccurv.begin(), ccurv.end(), cprevv.begin());
     prev_valid = i > i0;  // cprev holds coefficients in this basis
     for(int j = 0; j < k; ++j) {
       ccur[j] = C[j];
@@ -625,7 +738,7 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
     }
     kcur = k;
     double* R = V + (i + 1) * ld;
-    ops.residual(V, AV, k, rc, d_X, R);                                     // :181-193
+    ops.residual(V, AV, k, C, lam0, d_X, R);                                // :181-193
     pending = true;
     pending_restart = false;
     if(i + 1 < max_m || restarts < max_restarts) {
@@ -651,11 +764,11 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
     ++restarts;
     pending_restart = true;
     const int nb = max_m >= 3 ? 2 : 1;
-    RestartCoef Q{};
-    restart_coefficients(kcur, nb, ccur, prev_valid ? cprev : nullptr, c2cur, Q.q[0], Q.q[1]);
-    hipLaunchKernelGGL(restart_kernel, dim3(red_blocks(N)), dim3(256), 0, st, N, kcur, nb, Q, V,
-                       AV, ld, max_m);
-    MHIP_HIP_CHECK(hipGetLastError());
+    std::fill(q0.begin(), q0.end(), 0.);
+    std::fill(q1.begin(), q1.end(), 0.);
+    restart_coefficients(kcur, nb, ccur, prev_valid ? cprev : nullptr, c2cur, q0.data(),
+                         q1.data());
+    ops.restart(kcur, nb, q0.data(), q1.data(), V, AV, max_m);
     i0 = i = nb;
   }
   MHIP_HIP_CHECK(hipStreamSynchronize(st));

@@ -1,8 +1,13 @@
 // ASCI search (restating include/macis/asci/determinant_search.hpp:26-51,
 // 394-663, world_size == 1). The candidate generation, accumulation and
-// top-k run on the device (mhip_asci_search); the integral pointers of the
-// reference signature are taken from the generator's device copy.
+// top-k run on the device (mhip_asci_search). With a HipHamiltonianGenerator
+// the device copy of its integrals is used; with any other generator the
+// reference signature's own integral pointers are uploaded: T_pq (norb x
+// norb) and V_pqrs (norb^4), from which the device forms G, G_red, V_red and
+// the diagonal intermediates by the same formulas (impl.hpp:25-70), so the
+// scores equal those the generator's members would give.
 #pragma once
+#include <memory>
 #include <vector>
 
 #include <macis/hip/hip_hamiltonian_generator.hpp>
@@ -32,10 +37,15 @@ std::vector<wfn_t<N>> asci_search(ASCISettings asci_settings, size_t ndets_max,
                                   const std::vector<double>& C, size_t norb, const double* T_pq,
                                   const double* G_red, const double* V_red, const double* G_pqrs,
                                   const double* V_pqrs, HamiltonianGenerator<N>& ham_gen) {
-  (void)T_pq, (void)G_red, (void)V_red, (void)G_pqrs, (void)V_pqrs;
+  (void)G_red, (void)V_red, (void)G_pqrs;
   auto* hg = dynamic_cast<HipHamiltonianGenerator<N>*>(&ham_gen);
-  if(!hg) throw std::runtime_error("asci_search: needs a HipHamiltonianGenerator");
-  auto& ctx = hg->context();
+  std::unique_ptr<hip::Context> own;
+  if(!hg) {
+    if(!T_pq || !V_pqrs) throw std::runtime_error("asci_search: T_pq and V_pqrs are required");
+    own = std::make_unique<hip::Context>(0);
+    hip::check(mhip_set_integrals(own->get(), (int64_t)norb, T_pq, V_pqrs));
+  }
+  auto& ctx = hg ? hg->context() : *own;
   const size_t ncdets = std::distance(cdets_begin, cdets_end);
   mhip_asci_settings s;
   s.ncdets_max = asci_settings.ncdets_max;

@@ -66,28 +66,59 @@ class HipHamiltonianGenerator : public HamiltonianGenerator<N> {
   // form_rdms (double_loop.hpp:116-165) on the device through mhip_form_rdms:
   // the spans are added into, as the reference does, and a null 2-RDM handle
   // forms the 1-RDM alone. C weights bra i with C[i] and ket j with C[j].
+  // Spans wider than norb (any extents >= norb, as the reference indexes
+  // them only below norb) are staged through dense norb-sized copies.
   void form_rdms(full_det_iterator bra_begin, full_det_iterator bra_end,
                  full_det_iterator ket_begin, full_det_iterator ket_end, double* C,
                  matrix_span_t ordm, rank4_span_t trdm) override {
     const size_t n = this->norb_;
     const size_t nb = std::distance(bra_begin, bra_end), nk = std::distance(ket_begin, ket_end);
     if(!nb || !nk) return;
-    if(ordm.extent(0) != n || ordm.extent(1) != n)
-      throw std::runtime_error("form_rdms: the 1-RDM span must be norb x norb");
+    if(ordm.extent(0) < n || ordm.extent(1) < n)
+      throw std::runtime_error("form_rdms: the 1-RDM span is smaller than norb x norb");
     const bool two = trdm.data_handle() != nullptr;
+    bool dense = ordm.extent(0) == n && ordm.extent(1) == n;
     if(two)
-      for(size_t d = 0; d < 4; ++d)
-        if(trdm.extent(d) != n) throw std::runtime_error("form_rdms: the 2-RDM span must be norb^4");
+      for(size_t d = 0; d < 4; ++d) {
+        if(trdm.extent(d) < n)
+          throw std::runtime_error("form_rdms: the 2-RDM span is smaller than norb^4");
+        dense = dense && trdm.extent(d) == n;
+      }
+    std::vector<double> h1, h2;
+    double *p1 = ordm.data_handle(), *p2 = trdm.data_handle();
+    if(!dense) {  // gather the norb-sized corners, added into below
+      h1.resize(n * n);
+      for(size_t q = 0; q < n; ++q)
+        for(size_t p = 0; p < n; ++p) h1[p + n * q] = ordm(p, q);
+      p1 = h1.data();
+      if(two) {
+        h2.resize(n * n * n * n);
+        for(size_t l = 0; l < n; ++l)
+          for(size_t k = 0; k < n; ++k)
+            for(size_t j = 0; j < n; ++j)
+              for(size_t i = 0; i < n; ++i) h2[i + n * (j + n * (k + n * l))] = trdm(i, j, k, l);
+        p2 = h2.data();
+      }
+    }
     auto db = hip::upload_dets<N>(*ctx_, bra_begin, bra_end);
     auto dk = hip::upload_dets<N>(*ctx_, ket_begin, ket_end);
     hip::DeviceBuffer<double> dC(*ctx_, C, std::max(nb, nk));
-    hip::DeviceBuffer<double> d1(*ctx_, ordm.data_handle(), n * n);
+    hip::DeviceBuffer<double> d1(*ctx_, p1, n * n);
     hip::DeviceBuffer<double> d2;
-    if(two) d2 = hip::DeviceBuffer<double>(*ctx_, trdm.data_handle(), n * n * n * n);
+    if(two) d2 = hip::DeviceBuffer<double>(*ctx_, p2, n * n * n * n);
     hip::check(mhip_form_rdms(ctx_->get(), W, db.data(), nb, dk.data(), nk, dC.data(), d1.data(),
                               two ? d2.data() : nullptr));
-    d1.download(ordm.data_handle(), n * n);
-    if(two) d2.download(trdm.data_handle(), n * n * n * n);
+    d1.download(p1, n * n);
+    if(two) d2.download(p2, n * n * n * n);
+    if(!dense) {
+      for(size_t q = 0; q < n; ++q)
+        for(size_t p = 0; p < n; ++p) ordm(p, q) = h1[p + n * q];
+      if(two)
+        for(size_t l = 0; l < n; ++l)
+          for(size_t k = 0; k < n; ++k)
+            for(size_t j = 0; j < n; ++j)
+              for(size_t i = 0; i < n; ++i) trdm(i, j, k, l) = h2[i + n * (j + n * (k + n * l))];
+    }
   }
 
  protected:

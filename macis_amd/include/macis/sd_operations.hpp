@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <bitset>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include <macis/bitset_operations.hpp>
@@ -107,6 +108,50 @@ std::vector<std::bitset<N>> generate_cisd_hilbert_space(size_t norb, std::bitset
   for(auto& x : sa)
     for(auto& y : sb) out.push_back(full(x, y));
   return out;
+}
+
+// Excitation primitives (sd_operations.hpp:497-551). For bra and ket that
+// differ by one or two electron moves within one spin string (ex = bra ^
+// ket): the hole o is the lowest flipped orbital occupied in the ket, the
+// particle v the lowest flipped orbital occupied in the bra, and the sign is
+// the parity of the ket's occupied orbitals strictly between them; a double
+// is two such moves, the second applied to the updated ket.
+template <size_t N>
+uint32_t first_occupied_flipped(std::bitset<N> state, std::bitset<N> ex) {
+  return ffs(state & ex) - 1u;
+}
+
+template <size_t N>
+double single_excitation_sign(std::bitset<N> state, unsigned p, unsigned q) {
+  const unsigned lo = std::min(p, q), hi = std::max(p, q);
+  const auto between = full_mask<N>(hi) ^ full_mask<N>(lo + 1);
+  return ((state & between).count() % 2) ? -1. : 1.;
+}
+
+template <size_t N>
+inline auto single_excitation_sign_indices(std::bitset<N> bra, std::bitset<N> ket,
+                                           std::bitset<N> ex) {
+  const uint32_t o1 = first_occupied_flipped(ket, ex);
+  const uint32_t v1 = first_occupied_flipped(bra, ex);
+  return std::make_tuple(o1, v1, single_excitation_sign(ket, v1, o1));
+}
+
+template <size_t N>
+inline auto doubles_sign_indices(std::bitset<N> bra, std::bitset<N> ket, std::bitset<N> ex) {
+  const uint32_t o1 = first_occupied_flipped(ket, ex);
+  const uint32_t v1 = first_occupied_flipped(bra, ex);
+  double sign = single_excitation_sign(ket, v1, o1);
+  ket.flip(o1).flip(v1);
+  ex.flip(o1).flip(v1);
+  const uint32_t o2 = first_occupied_flipped(ket, ex);
+  const uint32_t v2 = first_occupied_flipped(bra, ex);
+  sign *= single_excitation_sign(ket, v2, o2);
+  return std::make_tuple(o1, v1, o2, v2, sign);
+}
+
+template <size_t N>
+inline double doubles_sign(std::bitset<N> bra, std::bitset<N> ket, std::bitset<N> ex) {
+  return std::get<4>(doubles_sign_indices(bra, ket, ex));
 }
 
 // to_canonical_string / from_canonical_string (sd_operations.hpp:596-630):

@@ -1,16 +1,20 @@
-// FCIDUMP reading (restating src/macis/fcidump.cxx:78-182): 5-token lines,
-// either "p q r s value" or "value p q r s", 1-based indices, all-zero
-// indices = core energy, (p,q,0,0) one-body, four nonzero two-body with
-// 8-fold unpacking. Arrays are column-major with leading dimension norb.
+// FCIDUMP reading and writing (restating src/macis/fcidump.cxx:78-207):
+// 5-token lines, either "p q r s value" or "value p q r s", 1-based indices,
+// all-zero indices = core energy, (p,q,0,0) one-body, four nonzero two-body
+// with 8-fold unpacking. Arrays are column-major with leading dimension
+// norb, or col_major_span views of any extents.
 #pragma once
 #include <algorithm>
 #include <cctype>
+#include <cstdio>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
 #include <vector>
+
+#include <macis/types.hpp>
 
 namespace macis {
 namespace detail {
@@ -84,6 +88,51 @@ inline void read_fcidump_2body(std::string fname, double* V, size_t LDV) {
         V[a + L * b + L2 * c + L3 * d] = v;
     }
   });
+}
+
+// Span variants (fcidump.hpp:54-71): the entries land at the view's own
+// column-major positions.
+inline void read_fcidump_1body(std::string fname, col_major_span<double, 2> T) {
+  detail::fcidump_for_each(fname, [&](int p, int q, int r, int s, double v) {
+    if((p || q || r || s) && !(p && q && r && s)) {
+      T(p - 1, q - 1) = v;
+      T(q - 1, p - 1) = v;
+    }
+  });
+}
+
+inline void read_fcidump_2body(std::string fname, col_major_span<double, 4> V) {
+  detail::fcidump_for_each(fname, [&](int p, int q, int r, int s, double v) {
+    if(p && q && r && s) {
+      --p, --q, --r, --s;
+      for(auto [a, b, c, d] : {std::tuple{p, q, r, s}, {p, q, s, r}, {q, p, r, s}, {q, p, s, r},
+                               {r, s, p, q}, {s, r, p, q}, {r, s, q, p}, {s, r, q, p}})
+        V(a, b, c, d) = v;
+    }
+  });
+}
+
+// write_fcidump (fcidump.cxx:184-207): every two-body entry (i,j,k,l) in
+// the reference's loop order (i outermost, l innermost), then every one-body
+// (i,j,0,0), then the core line (0,0,0,0), each as four width-8 integers and
+// a width-25, 16-digit scientific value. Lines are appended to the file, as
+// the reference's logger sink does.
+inline void write_fcidump(std::string fname, size_t norb, const double* T, size_t LDT,
+                          const double* V, size_t LDV, double E_core) {
+  std::FILE* f = std::fopen(fname.c_str(), "a");
+  if(!f) throw std::runtime_error("cannot open " + fname);
+  auto line = [&](size_t p, size_t q, size_t r, size_t s, double v) {
+    std::fprintf(f, "%8zu %8zu %8zu %8zu %25.16e\n", p, q, r, s, v);
+  };
+  for(size_t i = 0; i < norb; ++i)
+    for(size_t j = 0; j < norb; ++j)
+      for(size_t k = 0; k < norb; ++k)
+        for(size_t l = 0; l < norb; ++l)
+          line(i + 1, j + 1, k + 1, l + 1, V[i + j * LDV + k * LDV * LDV + l * LDV * LDV * LDV]);
+  for(size_t i = 0; i < norb; ++i)
+    for(size_t j = 0; j < norb; ++j) line(i + 1, j + 1, 0, 0, T[i + j * LDT]);
+  line(0, 0, 0, 0, E_core);
+  if(std::fclose(f) != 0) throw std::runtime_error("cannot write " + fname);
 }
 
 }  // namespace macis

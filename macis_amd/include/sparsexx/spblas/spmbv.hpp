@@ -2,8 +2,8 @@
 // src/sparsexx/include/sparsexx/spblas/spmbv.hpp:37-73):
 // AV = ALPHA * A * V + BETA * AV over K columns. The product runs on the
 // device (mhip_spmv on the default context): the host matrix and blocks are
-// staged through HBM for the call. Device-resident callers use
-// macis::HipSparseMatrixOperator instead.
+// staged through HBM for the call. Device-resident callers use the
+// DeviceCSR overload below or macis::HipSparseMatrixOperator.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -30,6 +30,18 @@ void gespmbv(int64_t K, ALPHAT ALPHA, const SpMatType& A,
   macis::hip::DeviceBuffer<double> dAV(ctx, AV, (K - 1) * LDAV + A.m());
   macis::hip::check(mhip_spmv(ctx.get(), dA.get(), K, ALPHA, dV.data(), LDV, BETA, dAV.data(), LDAV));
   dAV.download(AV, (K - 1) * LDAV + A.m());
+}
+
+// The same product on a matrix already resident in HBM (a DeviceCSR from
+// HipHamiltonianGenerator::device_csr or mhip_csr_create_from_host) with
+// DEVICE blocks V and AV: nothing is staged, the call runs on the matrix's
+// context and returns when the product is done.
+template <typename ALPHAT, typename BETAT>
+void gespmbv(int64_t K, ALPHAT ALPHA, const macis::hip::DeviceCSR& A, const double* V,
+             int64_t LDV, BETAT BETA, double* AV, int64_t LDAV,
+             macis::hip::Context& ctx = macis::hip::default_context()) {
+  macis::hip::check(mhip_spmv(ctx.get(), A.get(), K, ALPHA, V, LDV, BETA, AV, LDAV));
+  macis::hip::check(mhip_synchronize(ctx.get()));
 }
 
 }  // namespace sparsexx::spblas

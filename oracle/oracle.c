@@ -605,48 +605,78 @@ int orc_csr_rows_parallel(const orc_ham* h, int W, const uint64_t* bra, int64_t 
   return ORC_OK;
 }
 
-/* One bra row of the double loop (double_loop.hpp:45-89) compared, entry by
- * entry in ascending ket order, against a given CSR row col[0..len) /
- * val[0..len): 0 when the row holds exactly the kept kets with bitwise equal
- * values, else 1. The connection test is the same popcount(bra ^ ket) <= 4
- * as csr_row; for one-word determinants it is done on the whole word (the
- * sum of the two spin popcounts). */
-static int verify_row(const orc_ham* h, int W, det_t bra, const uint64_t* ket,
-                      int64_t nket, double thresh, const int32_t* col,
-                      const double* val, int64_t len) {
-  int64_t pos = 0;
-  if(!(popc(bra.a) + popc(bra.b))) return len != 0; /* zero bra: empty row */
-  uint32_t oa[64], ob[64];
-  int na = bits_to_indices(bra.a, oa), nb = bits_to_indices(bra.b, ob);
+/* Connection flags of a block of kets against one bra: flags[j] = 1 when
+ * popcount(bra ^ ket_j) <= 4 (the test of double_loop.hpp:66-67, summed over
+ * the two spin strings, which for one-word determinants is the popcount of
+ * the whole word). An AVX-512 build of the same loop is used when the CPU
+ * has vector popcounts; both give the same flags. */
+#define VBLK 2048
+__attribute__((target("avx512f,avx512bw,avx512vl,avx512vpopcntdq"))) static void
+conn_flags_v(int W, uint64_t a, uint64_t b, const uint64_t* k, int64_t n, unsigned char* f) {
   if(W == 1) {
-    const uint64_t bw = bra.a | (bra.b << 32);
-    for(int64_t j = 0; j < nket; ++j) {
-      const uint64_t kw = ket[j];
-      if(popc(bw ^ kw) > 4 || !kw) continue;
-      det_t k = load_det(1, ket + j);
-      double h_el = me_dispatch(h, bra, k, oa, na, ob, nb);
-      if(fabs(h_el) > thresh) {
-        if(pos >= len || col[pos] != (int32_t)j ||
-           memcmp(&val[pos], &h_el, sizeof(double)) != 0)
-          return 1;
-        ++pos;
-      }
-    }
+    const uint64_t w = a | (b << 32);
+    for(int64_t j = 0; j < n; ++j) f[j] = __builtin_popcountll(w ^ k[j]) <= 4;
   } else {
-    for(int64_t j = 0; j < nket; ++j) {
-      const uint64_t ka = ket[2 * j], kb = ket[2 * j + 1];
-      if(popc(bra.a ^ ka) + popc(bra.b ^ kb) > 4 || !(ka | kb)) continue;
-      det_t k = {ka, kb};
-      double h_el = me_dispatch(h, bra, k, oa, na, ob, nb);
-      if(fabs(h_el) > thresh) {
-        if(pos >= len || col[pos] != (int32_t)j ||
-           memcmp(&val[pos], &h_el, sizeof(double)) != 0)
-          return 1;
-        ++pos;
+    for(int64_t j = 0; j < n; ++j)
+      f[j] = (__builtin_popcountll(a ^ k[2 * j]) + __builtin_popcountll(b ^ k[2 * j + 1])) <= 4;
+  }
+}
+static void conn_flags_s(int W, uint64_t a, uint64_t b, const uint64_t* k, int64_t n,
+                         unsigned char* f) {
+  if(W == 1) {
+    const uint64_t w = a | (b << 32);
+    for(int64_t j = 0; j < n; ++j) f[j] = popc(w ^ k[j]) <= 4;
+  } else {
+    for(int64_t j = 0; j < n; ++j) f[j] = (popc(a ^ k[2 * j]) + popc(b ^ k[2 * j + 1])) <= 4;
+  }
+}
+
+/* State of one bra row of the double loop (double_loop.hpp:45-89) while it
+ * is compared, entry by entry in ascending ket order, against a given CSR
+ * row col[0..len) / val[0..len). */
+typedef struct {
+  det_t bra;
+  uint32_t oa[64], ob[64];
+  int na, nb, bad, zero;
+  int64_t pos, len;
+  const int32_t* col;
+  const double* val;
+} vrow_t;
+
+/* Kets j0 .. j0+n of the loop for one row: flagged kets go through the
+ * loop's zero-ket test, matrix element and threshold as in csr_row, and each
+ * kept entry must be the row's next stored one (same column, bitwise equal
+ * value). */
+static void verify_block(const orc_ham* h, int W, vrow_t* r, const uint64_t* ket, int64_t j0,
+                         int64_t n, double thresh, int vec, unsigned char* flags) {
+  if(r->bad || r->zero) return;
+  if(vec)
+    conn_flags_v(W, r->bra.a, r->bra.b, ket + W * j0, n, flags);
+  else
+    conn_flags_s(W, r->bra.a, r->bra.b, ket + W * j0, n, flags);
+  for(int64_t q = 0; q < n; q += 8) {
+    uint64_t m8;
+    if(q + 8 <= n) {
+      memcpy(&m8, flags + q, 8);
+      if(!m8) continue;
+    }
+    const int64_t qe = q + 8 < n ? q + 8 : n;
+    for(int64_t jj = q; jj < qe; ++jj) {
+      if(!flags[jj]) continue;
+      const int64_t j = j0 + jj;
+      det_t k = load_det(W, ket + W * j);
+      if(!(popc(k.a) + popc(k.b))) continue; /* zero ket (:62) */
+      double h_el = me_dispatch(h, r->bra, k, r->oa, r->na, r->ob, r->nb);
+      if(fabs(h_el) > thresh) { /* :76-80 */
+        if(r->pos >= r->len || r->col[r->pos] != (int32_t)j ||
+           memcmp(&r->val[r->pos], &h_el, sizeof(double)) != 0) {
+          r->bad = 1;
+          return;
+        }
+        ++r->pos;
       }
     }
   }
-  return pos != len;
 }
 
 int64_t orc_csr_verify_rows(const orc_ham* h, int W, const uint64_t* bra,
@@ -656,19 +686,47 @@ int64_t orc_csr_verify_rows(const orc_ham* h, int W, const uint64_t* bra,
                             const double* nzval, int64_t* first_bad) {
   /* Checks rows [row_begin,row_end) of a CSR (rowptr indexed by absolute
    * row, 0-based, entries colind/nzval at rowptr[i] .. rowptr[i+1]) against
-   * the reference double loop without storing the loop's output: rows are
-   * spread over OpenMP threads, each walks all kets as double_loop.hpp:45-89
-   * does and compares as it goes. Returns the number of rows that differ;
-   * *first_bad receives the lowest such row (-1 if none). */
+   * the reference double loop without storing the loop's output. Threads
+   * take 16 consecutive rows at a time and walk the kets in blocks of VBLK,
+   * each block serving all 16 rows from cache; every row still sees the kets
+   * in ascending order, as double_loop.hpp:45-89 does. Returns the number
+   * of rows that differ; *first_bad receives the lowest such row (-1 if
+   * none). */
   if(!h || (W != 1 && W != 2) || row_end < row_begin) return -1;
+  enum { RB = 16 };
+  const int64_t nrow = row_end - row_begin, ngrp = (nrow + RB - 1) / RB;
   int64_t bad = 0, first = INT64_MAX;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad) reduction(min : first)
-  for(int64_t i = row_begin; i < row_end; ++i) {
-    det_t b = load_det(W, bra + W * i);
-    const int64_t lo = rowptr[i], len = rowptr[i + 1] - rowptr[i];
-    if(len < 0 || verify_row(h, W, b, ket, nket, thresh, colind + lo, nzval + lo, len)) {
-      ++bad;
-      if(i < first) first = i;
+  const int vec = __builtin_cpu_supports("avx512vpopcntdq") && __builtin_cpu_supports("avx512bw") &&
+                  __builtin_cpu_supports("avx512vl");
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : bad) reduction(min : first)
+  for(int64_t g = 0; g < ngrp; ++g) {
+    vrow_t rows[RB];
+    unsigned char flags[VBLK];
+    const int64_t i0 = row_begin + g * RB;
+    const int nr = (int)(row_end - i0 < RB ? row_end - i0 : RB);
+    for(int q = 0; q < nr; ++q) {
+      vrow_t* r = &rows[q];
+      const int64_t i = i0 + q;
+      r->bra = load_det(W, bra + W * i);
+      r->na = bits_to_indices(r->bra.a, r->oa);
+      r->nb = bits_to_indices(r->bra.b, r->ob);
+      r->zero = !(r->na + r->nb); /* zero bra: empty row (:50) */
+      r->len = rowptr[i + 1] - rowptr[i];
+      r->bad = r->len < 0;
+      r->pos = 0;
+      r->col = colind + rowptr[i];
+      r->val = nzval + rowptr[i];
+    }
+    for(int64_t j0 = 0; j0 < nket; j0 += VBLK) {
+      const int64_t n = nket - j0 < VBLK ? nket - j0 : VBLK;
+      for(int q = 0; q < nr; ++q) verify_block(h, W, &rows[q], ket, j0, n, thresh, vec, flags);
+    }
+    for(int q = 0; q < nr; ++q) {
+      const vrow_t* r = &rows[q];
+      if(r->bad || r->pos != (r->zero ? 0 : r->len) || (r->zero && r->len)) {
+        ++bad;
+        if(i0 + q < first) first = i0 + q;
+      }
     }
   }
   if(first_bad) *first_bad = bad ? first : -1;
@@ -807,7 +865,7 @@ void orc_diagonal_guess(int64_t N, const double* D, double* X) {
 /* gram_schmidt (:111-126): CGS2 then normalise */
 static void gram_schmidt(int64_t N, int64_t K, const double* V, int64_t LDV,
                          double* w) {
-  double inner[64];
+  double inner[1025]; /* K <= max_m + 1 <= 1025 */
   for(int pass = 0; pass < 2; ++pass) {
     for(int64_t j = 0; j < K; ++j) {
       double s = 0.;
@@ -874,18 +932,23 @@ int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
    * of the stored A*V columns (no product with A). */
   if(!X) return ORC_ERR_NO_GUESS;
   if(max_m > N) max_m = N;
-  if(max_m > 62) return ORC_ERR_ARG;
+  if(max_m > 1024) return ORC_ERR_ARG;
   const int64_t ld = max_m + 1;
   double* V = (double*)calloc((size_t)N * ld, sizeof(double));
   double* AV = (double*)calloc((size_t)N * ld, sizeof(double));
   double* tmp = (double*)calloc((size_t)N * 4, sizeof(double));  /* restart staging */
-  double C[64 * 64], LAM[64], ccur[64], cprev[64], c2cur[64];
-  if(!V || !AV || !tmp) {
+  /* k x k Ritz problem, eigenvalues, current / previous / second Ritz
+   * coefficients and the two restart columns, for k <= max_m + 1 */
+  double* small = (double*)calloc((size_t)ld * ld + 6 * (size_t)ld, sizeof(double));
+  if(!V || !AV || !tmp || !small) {
     free(V);
     free(AV);
     free(tmp);
+    free(small);
     return ORC_ERR_NOMEM;
   }
+  double *C = small, *LAM = C + ld * ld, *ccur = LAM + ld, *cprev = ccur + ld,
+         *c2cur = cprev + ld, *Q = c2cur + ld;
   memcpy(V, X, sizeof(double) * N);
   orc_gespmbv(1, 1., N, rp, ci, nz, 0, V, N, 0., AV, N); /* :153 */
   memcpy(V + N, AV, sizeof(double) * N);                   /* :156 */
@@ -905,7 +968,7 @@ int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
           C[a + k * b] = sum;
         }
       orc_syev_lower(k, C, k, LAM);
-      memcpy(cprev, ccur, sizeof(double) * 64);
+      memcpy(cprev, ccur, sizeof(double) * ld);
       prev_valid = i > i0; /* c_prev holds coefficients in this basis */
       for(int64_t j = 0; j < k; ++j) {
         ccur[j] = C[j];
@@ -938,7 +1001,6 @@ int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
      * passes); the second Ritz vector's coefficients c2 stand in for c_prev
      * when the last two iterations did not share this basis or c_prev is
      * numerically parallel to c. One column only when max_m < 3. */
-    double Q[2 * 64];
     const int64_t nb = max_m >= 3 ? 2 : 1;
     restart_coefficients(kcur, nb, ccur, prev_valid ? cprev : NULL, c2cur, Q);
     for(int64_t n = 0; n < N; ++n) {
@@ -964,6 +1026,7 @@ int orc_davidson_restarted(int64_t N, int64_t max_m, int64_t max_restarts,
   free(AV);
   free(tmp);
   *E = LAM[0];
+  free(small);
   if(iters) *iters = iter;
   if(restarts_out) *restarts_out = restarts;
   return converged ? ORC_OK : ORC_ERR_NOT_CONVERGED;

@@ -179,3 +179,86 @@ def test_asci_pruning_cases_cover_both_stages(h2o, h2o_oracle):
         assert events
         seen.add(stage2)
     assert seen == {False, True}
+
+
+def _as_void(a):
+    """Rows of a (n, W) uint64 array as single opaque items (for unique/isin)."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a.view(np.dtype((np.void, 8 * a.shape[1]))).ravel()
+
+
+def _accumulate(states, rv):
+    """Unique determinants (as opaque rows, sorted), the sum of their scores
+    in generation order and the sum of the terms' magnitudes."""
+    keys, inv = np.unique(_as_void(states), return_inverse=True)
+    inv = inv.reshape(-1)
+    return keys, np.bincount(inv, weights=rv), np.bincount(inv, weights=np.abs(rv))
+
+
+@pytest.mark.parametrize("W", [1, 2])
+def test_asci_candidate_scores_match_oracle(ctx, h2o, h2o_oracle, W):
+    """mhip_asci_candidates' accumulated |rv| per unique non-core
+    determinant equals the oracle's asci_contributions
+    (determinant_contributions.hpp:25-160) summed per determinant
+    (determinant_sort.hpp:39-83): the same determinants, and each score to
+    1e-12 of the sum of its terms' magnitudes (the order of a sum is the
+    only freedom)."""
+    if W == 1:
+        announce("asci candidate scores, H2O 100-det core, W=1")
+        norb, _, T, V = h2o
+        orc = h2o_oracle
+        hf = D.canonical_hf_determinant(1, 5, 5)
+        E0 = orc.matrix_element(1, hf, hf)
+        s = O.ASCISettings.defaults()
+        E1, w, X = orc.asci_iter(1, s, 100, E0, hf, np.array([1.0]))
+        E2, w, X = orc.asci_iter(1, s, 800, E1, w, X)
+        order = np.argsort(-np.abs(X), kind="stable")
+        core, C, E = w[order[:100]], X[order[:100]], E2
+    else:
+        announce("asci candidate scores, 64 orbitals 20a20b, 12 core dets, W=2")
+        norb = 64
+        T, V = seeded_integrals(norb, 31)
+        orc = O.Ham(T, V)
+        core = selected_ci_dets(norb, 20, 20, 12, seed=9, T=T)
+        rng = np.random.default_rng(11)
+        C = rng.standard_normal(len(core))
+        C /= np.linalg.norm(C)
+        E = orc.matrix_element(2, core[0], core[0]) - 0.5
+    ctx.set_integrals(norb, T, V)
+    states, rv = orc.asci_contributions(W, O.ASCISettings.defaults(), core, E, C)
+    keys, acc, mag = _accumulate(states, rv)
+    keep = ~np.isin(keys, _as_void(core)) & (acc != 0.0)
+    keys, ref, mag = keys[keep], np.abs(acc[keep]), mag[keep]
+    cap = len(keys) + 16
+    d_out, a_out, n = ctx.asci_candidates(W, _capi.ASCISettings.defaults(), ctx.to_device(core),
+                                          len(core), E, ctx.to_device(C), norb, cap)
+    dets = d_out.numpy()[:n]
+    sc = a_out.numpy()[:n]
+    gk, gi = np.unique(_as_void(dets), return_index=True)
+    assert len(gk) == n, "candidates are unique"
+    assert np.array_equal(gk, keys), "candidate determinants differ"
+    err = np.max(np.abs(sc[gi] - ref) / mag)
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("tol", [np.inf, 0.0])
+@pytest.mark.parametrize("frac", [0.1, 0.45])
+def test_asci_search_pruning_limits(ctx, h2o, h2o_oracle, tol, frac):
+    """pair_size_max pruning at the two tolerances whose outcome the loop's
+    definition fixes (the oracle side is pinned in
+    test_oracle_asci_pruning.py): +inf erases every pair at each event,
+    0 erases none and only regroups the sums."""
+    announce(f"asci_search pruning limits tol={tol} pair_size_max={frac} x pairs")
+    norb, _, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    hf = D.canonical_hf_determinant(1, 5, 5)
+    E0 = h2o_oracle.matrix_element(1, hf, hf)
+    s = O.ASCISettings.defaults()
+    E1, w, X = h2o_oracle.asci_iter(1, s, 100, E0, hf, np.array([1.0]))
+    E2, w, X = h2o_oracle.asci_iter(1, s, 800, E1, w, X)
+    order = np.argsort(-np.abs(X), kind="stable")
+    core, C = w[order[:100]], X[order[:100]]
+    npairs = len(h2o_oracle.asci_contributions(1, s, core, E2, C)[1])
+    got, ref, ref_rv = run_both(ctx, h2o_oracle, 1, norb, core, C, E2, 3000,
+                                pair_size_max=int(frac * npairs), rv_prune_tol=tol)
+    check_topk(got, ref, ref_rv, 100)

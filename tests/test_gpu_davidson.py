@@ -278,3 +278,58 @@ def test_selected_ci_diag_reuses_kept_storage(ctx):
     assert out[0][0] == out[1][0] == out[3][0] and np.array_equal(out[0][2], out[1][2])
     assert np.array_equal(out[0][2], out[3][2])
     assert E == out[2][0] and nnz == out[2][1] and np.array_equal(dX.numpy(), out[2][2])
+
+
+def _cas_csr(ctx, n, ne, seed):
+    T, V = seeded_integrals(n, seed)
+    ctx.set_integrals(n, T, V)
+    dets = D.generate_hilbert_space(1, n, ne, ne)
+    M = len(dets)
+    H = ctx.ham_build(1, ctx.to_device(dets), M, thresh=1e-16)
+    Dg = ctx.empty(M, np.float64)
+    ctx.extract_diagonal(H, Dg)
+    d = Dg.numpy()
+    g = np.zeros(M)
+    g[np.argmin(d)] = 1.0
+    return H, Dg, d, g, O.Ham(T, V).csr(1, dets, thresh=1e-16)
+
+
+def test_davidson_wide_subspace(ctx):
+    """Subspaces wider than one 32-column projection block (davidson.hpp
+    sizes V and AV for any max_m, :145-147). The synthetic CAS(8e,8o) of seed
+    108 needs 73 iterations without restarts: with max_m 100 the reference
+    algorithm converges on the device as in the oracle (subspaces up to 74
+    columns, three column blocks), with max_m 70 both report non-convergence,
+    and the restarted variant at max_m 35 agrees with the oracle's."""
+    from macis_amd import _capi
+
+    announce("davidson wide subspace, CAS(8,8) max_m 100 / 70 / restarted 35")
+    H, Dg, d, g, (rp, ci, nz) = _cas_csr(ctx, 8, 4, 108)
+    M = len(d)
+    X = ctx.to_device(g)
+    it, E = ctx.davidson(H, 100, 1e-8, X, Dg)
+    ito, Eo, Xo = O.davidson(M, 100, rp, ci, nz, d, 1e-8, g)
+    assert ito == 73 and abs(it - ito) <= 1
+    assert E == pytest.approx(Eo, abs=1e-10)
+    assert abs(abs(X.numpy() @ Xo) - 1.0) < 1e-8
+    with pytest.raises(_capi.MhipError, match="Did Not Converge"):
+        ctx.davidson(H, 70, 1e-8, ctx.to_device(g), Dg)
+    Xr = ctx.to_device(g)
+    itr, Er = ctx.davidson(H, 35, 1e-9, Xr, Dg, max_restarts=50)
+    assert ctx.timings()["dav_restarts"] >= 1
+    itro, Ero, _ = O.davidson(M, 35, rp, ci, nz, d, 1e-9, g, max_restarts=50)
+    assert abs(itr - itro) <= 1
+    assert Er == pytest.approx(Ero, abs=1e-10)
+
+
+def test_davidson_subspace_limit(ctx, h2o, h2o_cisd):
+    from macis_amd import _capi
+
+    announce("davidson max_m above the 1024 limit is refused")
+    norb, _, T, V = h2o
+    ctx.set_integrals(norb, T, V)
+    N = len(h2o_cisd)
+    H = ctx.ham_build(1, ctx.to_device(h2o_cisd), N, thresh=1e-16)
+    X = ctx.to_device(np.eye(1, N, 0)[0])
+    with pytest.raises(_capi.MhipError, match=r"\[1, 1024\]"):
+        ctx.davidson(H, 1025, 1e-8, X)
