@@ -228,6 +228,18 @@ def selected_ci_section(C, ctx, cpu=True):
     for _ in range(reps):
         ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
     ctx.synchronize()
+    csr_spmv_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the layout the Davidson uses for this matrix: the column-panel split
+    t0 = time.perf_counter()
+    assert ctx.prepare_spmv(H, panels=True)
+    ctx.synchronize()
+    panel_prep_ms = (time.perf_counter() - t0) * 1e3
+    ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
+    ctx.synchronize()
     spmv_ms = (time.perf_counter() - t0) * 1e3 / reps
     del H, x, y
     dX = ctx.empty(nd, np.float64).zero()
@@ -267,12 +279,16 @@ def selected_ci_section(C, ctx, cpu=True):
         "build_scan_ms": tm["ham_count_ms"], "build_gather_ms": tm["ham_fill_ms"],
         "build_elements_ms": tm["ham_elem_ms"],
         "spmv_ms": spmv_ms, "spmv_GBs": spmv_bytes(nnz, nd, False) / (spmv_ms * 1e-3) / 1e9,
+        "spmv_layout": "column panels (8 nnz-balanced panels, one per XCD)",
+        "panel_split_ms": panel_prep_ms, "csr_spmv_ms": csr_spmv_ms,
+        "csr_spmv_GBs": spmv_bytes(nnz, nd, False) / (csr_spmv_ms * 1e-3) / 1e9,
+        "davidson_layout": td["dav_sliced"],
         "selected_ci_diag_s": diag_s, "E0": E, "davidson_iterations": td["dav_iters"],
         "davidson_restarts": td["dav_restarts"], "setup_s_host": setup_s,
         "cpu_baseline": cpu_leg,
-        "note": "build = host wall time of mhip_ham_build (second call); spmv on the CSR "
-                "kernel (the automatic policy keeps CSR for this list); solve includes its "
-                "own build",
+        "note": "build = host wall time of mhip_ham_build (second call); spmv on the "
+                "column-panel split the Davidson attaches to this matrix (csr_spmv_* the plain "
+                "CSR kernel); solve includes its own build",
     }
 
 
@@ -466,6 +482,9 @@ def main():
                     help="skip the learned pre-screen / ASCI step comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 1 GiB device-to-device copy first (counter calibration)")
+    ap.add_argument("--only", choices=["selected_ci", "batch", "asci", "mlp"],
+                    help="run only this section and print its JSON object (for profiling; "
+                         "not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the hot path has no "
@@ -477,6 +496,14 @@ def main():
     from macis_amd.synthetic import seeded_integrals
 
     C = capi()
+    if args.only:
+        ctx = C.Context(0)
+        run = {"selected_ci": lambda: selected_ci_section(C, ctx, cpu=not args.no_cpu_baseline),
+               "batch": lambda: batch_section(C, ctx),
+               "asci": lambda: asci_step_comparison(C, ctx),
+               "mlp": lambda: mlp_throughput(C, ctx)}[args.only]
+        print(json.dumps({"section": args.only, args.only: run()}), flush=True)
+        return
     T, V = seeded_integrals(N_ORB, SEED)
     dets = D.generate_hilbert_space(1, N_ORB, N_ALPHA, N_BETA)
     N = len(dets)

@@ -91,8 +91,8 @@ typedef struct mhip_timings {
   double asci_nn_ms;      /* learned pre-screen (prediction + selection) */
   int64_t asci_nsurvivors; /* candidates entering the exact top-k */
   int64_t ham_path;       /* structure pass of the last build: MHIP_BUILD_* */
-  double dav_layout_ms;   /* sliced copy made for the last Davidson's SpMVs */
-  int64_t dav_sliced;     /* 1 when those SpMVs ran on the sliced copy */
+  double dav_layout_ms;   /* sliced copy / panel split made for the last Davidson */
+  int64_t dav_sliced;     /* 1: its SpMVs ran on the sliced copy, 2: on column panels */
 } mhip_timings;
 
 /* Structure passes of the Hamiltonian build. AUTO takes the product-space
@@ -233,6 +233,15 @@ int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
  * equal the CSR ones up to the order of the row sums. */
 #define MHIP_SPMV_SLICED_AUTO 0
 #define MHIP_SPMV_SLICED_ALWAYS 1
+/* MHIP_SPMV_COLUMN_PANELS attaches instead a column-panel split of a square
+ * matrix without a sliced copy: the columns are cut into 8 panels of about
+ * equal nonzero counts and each of the GPU's 8 XCDs multiplies one panel's
+ * entries of every row, so its L2 holds one panel of x (for x vectors larger
+ * than an L2; per row 8 int32 panel offsets and 8 partial sums, 96 bytes,
+ * no copy of the entries). Davidson attaches it by itself to matrices that
+ * get no sliced copy and have at least 262,144 columns. *sliced receives 2.
+ * Products equal the CSR ones up to the order of the row sums. */
+#define MHIP_SPMV_COLUMN_PANELS 2
 int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int policy, int* sliced);
 /* extract_diagonal_elements (submatrix.hpp:264-295) into DEVICE d_D. */
 int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);

@@ -674,9 +674,9 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   // sliced copy of A for the SpMVs when there is room for it (made once per
   // matrix handle; the CSR kernel runs otherwise)
   {
-    const bool had = (bool)A->sell;
+    const bool had = A->sell || A->panels;
     const auto h0 = std::chrono::steady_clock::now();
-    tm.dav_sliced = prepare_sell(ctx, A) ? 1 : 0;
+    tm.dav_sliced = prepare_sell(ctx, A) ? 1 : prepare_panels(ctx, A) ? 2 : 0;
     if(!had && tm.dav_sliced) MHIP_HIP_CHECK(hipStreamSynchronize(st));
     tm.dav_layout_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();

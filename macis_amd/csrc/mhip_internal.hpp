@@ -162,6 +162,18 @@ struct SellCopy {
   const int64_t* soff() const { return meta.as<int64_t>(); }
   const int32_t* slen() const { return reinterpret_cast<const int32_t*>(soff() + nsl + 1); }
 };
+// Column-panel split of a CSR for single-vector SpMVs whose x vector
+// outgrows one XCD's L2: columns [bounds[p], bounds[p+1]) form panel p
+// (np panels of about equal nonzero counts); off holds, per row, the
+// start of each panel's entries relative to the row start (columns are
+// ascending, so a panel's entries are one contiguous run of the row).
+constexpr int kMaxPanels = 32;
+struct PanelSplit {
+  int np = 8;                   // panels, a multiple of 8 (XCD x takes x, x + 8, ...)
+  std::vector<int64_t> bounds;  // np + 1 column bounds
+  DevBuf off;                   // int32[m * np]
+  DevBuf part;                  // double[np * m] partial sums
+};
 }  // namespace mhip
 
 struct mhip_context {
@@ -189,6 +201,8 @@ struct mhip_csr {
   mhip::DevBuf nzval;   // double[nnz]
   // optional sliced copy used by single-vector SpMVs (prepare_sell)
   mutable std::unique_ptr<mhip::SellCopy> sell;
+  // optional column-panel split used by single-vector SpMVs (prepare_panels)
+  mutable std::unique_ptr<mhip::PanelSplit> panels;
 };
 
 namespace mhip {
@@ -224,6 +238,11 @@ void extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);
 // it and (unless any_padding) padding adds at most a quarter of the
 // nonzeros; true if the matrix has one afterwards; spmv() then uses it.
 bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding = false);
+// Attaches the column-panel split to a square matrix (when it has no sliced
+// copy); with any_size false only when x exceeds kPanelMinCols entries.
+// True if the matrix has one afterwards; spmv() then uses it.
+constexpr int64_t kPanelMinCols = 262144;
+bool prepare_panels(mhip_context* ctx, const mhip_csr* A, bool any_size = false);
 // y = A x on the sliced copy (A must have one); with conv, rows r with
 // conv[row_inst[r]] set are skipped (y keeps its value there).
 void spmv_sliced_masked(mhip_context* ctx, const mhip_csr* A, const double* x, double* y,

@@ -323,8 +323,10 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     for(int u = 0; u < kU; ++u) {
       const int32_t p = pb + 64 * u + lane;
       if(p < total) {
-        colind[st + p] = (int32_t)(r2[u] * L + k2[u]);
-        nzval[st + p] = h[u];
+        // streamed out once: non-temporal, so the 19 GB of stores of a large
+        // build do not evict the element tables and integrals from L2
+        __builtin_nontemporal_store((int32_t)(r2[u] * L + k2[u]), colind + st + p);
+        __builtin_nontemporal_store(h[u], nzval + st + p);
         nk += fabs(h[u]) > thresh ? 1 : 0;
       }
     }

@@ -68,11 +68,20 @@ def test_c3_symmetric_and_lowest_root_is_eigenpair(ctx, c3):
     a, b = float(x @ dHy.numpy()), float(y @ dHx.numpy())
     scale = float(np.abs(nz).max()) * np.sqrt(len(nz))
     assert abs(a - b) <= 1e-12 * scale, (a, b)
+    announce("C3 SpMV on the column-panel split vs the host product")
+    ref = np.zeros(N)
+    O.gespmbv(1, 1.0, N, rp, ci, nz, 0, x, N, 0.0, ref, N)
+    assert ctx.prepare_spmv(H, panels=True)
+    ctx.spmv(H, 1, 1.0, dx, N, 0.0, dHx, N)
+    lens = np.diff(rp)
+    tol = 1e-13 * np.abs(nz).max() * np.abs(x).max() * lens.max()
+    assert np.max(np.abs(dHx.numpy() - ref)) <= tol
     announce("C3 restarted Davidson, residual on the host matrix")
     ctx.set_integrals(N_ORB, T, V)
     dX = ctx.empty(N, np.float64).zero()
     E, nnz, _ = ctx.selected_ci_diag(2, dd, N, dX, max_restarts=200)
     assert nnz == len(nz)
+    assert ctx.timings()["dav_sliced"] == 2  # the Davidson multiplied on column panels
     X = dX.numpy()
     assert abs(np.linalg.norm(X) - 1.0) < 1e-12
     HX = np.zeros(N)

@@ -14,17 +14,27 @@ pytestmark = pytest.mark.gpu
 EPS = np.finfo(float).eps
 
 
+LAYOUTS = ["csr", "sliced", "panels"]
+
+
+def _layout(ctx, A, layout):
+    """Attach the requested product layout to A (the CSR kernel: nothing)."""
+    if layout == "sliced":
+        assert ctx.prepare_spmv(A, always=True)
+    elif layout == "panels":
+        assert ctx.prepare_spmv(A, panels=True)
+
+
 @pytest.mark.parametrize("indexing", [0, 1])
-@pytest.mark.parametrize("sliced", [False, True])
-def test_spmv_known_answer(ctx, indexing, sliced):
-    announce(f"spmv 6x6 known answer indexing={indexing} sliced={sliced}")
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_spmv_known_answer(ctx, indexing, layout):
+    announce(f"spmv 6x6 known answer indexing={indexing} layout={layout}")
     i = indexing
     rowptr = np.array([i, i + 2, i + 5, i + 8, i + 11, i + 14, i + 16])
     colind = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4, 5, 4, 5]) + i
     nzval = np.array([2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2], dtype=float)
     A = ctx.csr_from_host(6, 6, rowptr, colind, nzval, indexing=i)
-    if sliced:
-        assert ctx.prepare_spmv(A, always=True)
+    _layout(ctx, A, layout)
     x = ctx.to_device(np.arange(1, 7, dtype=float))
     y = ctx.to_device(np.ones(6))
     ctx.spmv(A, 1, 1.0, x, 6, 0.0, y, 6)
@@ -34,15 +44,14 @@ def test_spmv_known_answer(ctx, indexing, sliced):
     assert np.array_equal(Dg.numpy(), np.full(6, 2.0))
 
 
-@pytest.mark.parametrize("sliced", [False, True])
-def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd, sliced):
-    announce(f"spmv H2O CISD K=2 alpha/beta sliced={sliced}")
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd, layout):
+    announce(f"spmv H2O CISD K=2 alpha/beta layout={layout}")
     norb, _, T, V = h2o
     ctx.set_integrals(norb, T, V)
     N = len(h2o_cisd)
     H = ctx.ham_build(1, ctx.to_device(h2o_cisd), N, thresh=1e-16)
-    if sliced:
-        assert ctx.prepare_spmv(H, always=True)
+    _layout(ctx, H, layout)
     rp, ci, nz = h2o_oracle.csr(1, h2o_cisd, thresh=1e-16)
     rng = np.random.default_rng(1)
     LD = N + 3
@@ -61,14 +70,16 @@ def test_spmv_block_alpha_beta(ctx, h2o, h2o_oracle, h2o_cisd, sliced):
     assert np.array_equal(Dg.numpy(), O.extract_diagonal(N, rp, ci, nz))
 
 
-@pytest.mark.parametrize("sliced", [False, True])
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("indexing", [0, 1])
-def test_spmv_ragged_rows(ctx, indexing, sliced):
+def test_spmv_ragged_rows(ctx, indexing, layout):
     """Rows of every length class around the kernel's 16-byte quads (0-9
     nonzeros at all start offsets mod 4, long rows, an empty last row; in
     the sliced layout slices mixing empty, short and 4096-long rows and a
-    partial last slice) against the oracle's gespmbv (spmbv.hpp:37-73)."""
-    announce(f"spmv ragged rows indexing={indexing} sliced={sliced}")
+    partial last slice; in the panel split rows whose runs in a panel are
+    empty or a few entries long) against the oracle's gespmbv
+    (spmbv.hpp:37-73)."""
+    announce(f"spmv ragged rows indexing={indexing} layout={layout}")
     rng = np.random.default_rng(7 + indexing)
     lens = np.concatenate([np.tile(np.arange(10), 8), rng.integers(0, 700, 300),
                            [1819, 4096, 3, 0]])
@@ -80,8 +91,7 @@ def test_spmv_ragged_rows(ctx, indexing, sliced):
                              for L in lens]).astype(np.int32)
     nzval = rng.standard_normal(nnz)
     A = ctx.csr_from_host(m, n, rowptr + indexing, colind + indexing, nzval, indexing=indexing)
-    if sliced:  # rows of mixed lengths padded within their slices
-        assert ctx.prepare_spmv(A, always=True)
+    _layout(ctx, A, layout)
     x = rng.standard_normal(n)
     y0 = rng.standard_normal(m)
     dx, dy = ctx.to_device(x), ctx.to_device(y0)
