@@ -297,3 +297,31 @@ def test_tiled_build_fallback_paths(ctx, pool, monkeypatch):
     got = ctx.ham_build(2, dd, len(dets), thresh=EPS).to_host()
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
+
+
+def test_product_fill_non_uniform_inner_strings(ctx):
+    """A product list whose inner strings are not a complete space (every
+    third beta string dropped) has uneven inner neighbour counts; the
+    product path still recognises it, and its CSR equals the oracle's and
+    the tiled loop's."""
+    from macis_amd import _capi
+
+    announce("product fill, incomplete inner strings")
+    n = 8
+    T, V = seeded_integrals(n, 88)
+    ctx.set_integrals(n, T, V)
+    a = D.generate_combs(n, 4)
+    b = D.generate_combs(n, 3)
+    b = b[np.arange(len(b)) % 3 != 1]
+    dets = np.array([[int(x) | (int(y) << 32)] for x in a for y in b], dtype=np.uint64)
+    dd = ctx.to_device(dets)
+    ctx.set_build_path(_capi.Context.BUILD_PRODUCT)
+    try:
+        prod = ctx.ham_build(1, dd, len(dets), thresh=EPS).to_host()
+        assert ctx.timings()["ham_path"] == _capi.Context.BUILD_PRODUCT
+        ctx.set_build_path(_capi.Context.BUILD_TILED)
+        tiled = ctx.ham_build(1, dd, len(dets), thresh=EPS).to_host()
+    finally:
+        ctx.set_build_path(_capi.Context.BUILD_AUTO)
+    assert_same_csr(prod, tiled)
+    assert_same_csr(prod, O.Ham(T, V).csr(1, dets, thresh=EPS))
