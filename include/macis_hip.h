@@ -310,7 +310,8 @@ int mhip_asci_search(mhip_context* ctx, int words_per_det,
 
 /* ---- batched CAS-CI (no counterpart in the reference, which solves one
  * instance per compute_casci_rdms call, util/cas.hpp:32-64) ------------------
- * ninst independent problems, each with its own norb (<= 32, wfn_t<64>),
+ * ninst independent problems, each with its own norb (<= 32 for wfn_t<64>,
+ * <= 64 for wfn_t<128> through mhip_batch_create_w),
  * HOST determinants dets[det_offsets[b] .. det_offsets[b+1]) and HOST
  * integrals T[b] (norb^2), V[b] (norb^4). All instances share one
  * block-diagonal device CSR built by one launch per stage. */
@@ -319,6 +320,14 @@ int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
                       const int64_t* det_offsets, const uint64_t* dets,
                       const double* const* T, const double* const* V,
                       mhip_batch** out);
+/* mhip_batch_create for any determinant width: words_per_det 1 (wfn_t<64>,
+ * norb <= 32) or 2 (wfn_t<128>, norb <= 64); dets holds words_per_det
+ * uint64 per determinant (word 0 alpha, word 1 beta for wfn_t<128>), as in
+ * the per-instance path (compute_casci_rdms works for any N, cas.hpp:32-64). */
+int mhip_batch_create_w(mhip_context* ctx, int words_per_det, int ninst,
+                        const int32_t* norb, const int64_t* det_offsets,
+                        const uint64_t* dets, const double* const* T,
+                        const double* const* V, mhip_batch** out);
 int mhip_batch_destroy(mhip_batch* batch);
 int mhip_batch_info(const mhip_batch* batch, int64_t* ndets, int64_t* nnz);
 /* Hamiltonians of all instances with |h| > thresh (double_loop.hpp:30-98

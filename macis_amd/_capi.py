@@ -132,6 +132,7 @@ _SIGS = {
     "mhip_asci_search_screened": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, _d, _i64, _p,
                                          _i64, _d, _p, _i64, _p, _pi64]),
     "mhip_batch_create": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _pp]),
+    "mhip_batch_create_w": (_i32, [_p, _i32, _i32, _p, _p, _p, _p, _p, _pp]),
     "mhip_batch_destroy": (_i32, [_p]),
     "mhip_batch_info": (_i32, [_p, _pi64, _pi64]),
     "mhip_batch_build": (_i32, [_p, _p, _d]),
@@ -429,12 +430,14 @@ class Context:
 class Batch:
     """Batched CAS-CI instances (mhip_batch)."""
 
-    def __init__(self, ctx, norbs, det_lists, Ts, Vs):
+    def __init__(self, ctx, norbs, det_lists, Ts, Vs, W=1):
+        """det_lists: per instance (ndets,) uint64 words (W=1) or (ndets, 2)
+        [alpha, beta] words (W=2)."""
         self.ctx = ctx
         n = len(norbs)
         norb = np.ascontiguousarray(norbs, dtype=np.int32)
         offs = np.zeros(n + 1, dtype=np.int64)
-        offs[1:] = np.cumsum([len(d) for d in det_lists])
+        offs[1:] = np.cumsum([np.asarray(d).size // W for d in det_lists])
         dets = np.ascontiguousarray(np.concatenate([np.asarray(d).reshape(-1) for d in det_lists]),
                                     dtype=np.uint64)
         self._T = [np.ascontiguousarray(t, dtype=np.float64) for t in Ts]
@@ -442,8 +445,8 @@ class Batch:
         Tp = (C.c_void_p * n)(*[t.ctypes.data for t in self._T])
         Vp = (C.c_void_p * n)(*[v.ctypes.data for v in self._V])
         p = C.c_void_p()
-        check(lib().mhip_batch_create(ctx.h, n, norb.ctypes.data, offs.ctypes.data,
-                                      dets.ctypes.data, Tp, Vp, C.byref(p)), "batch_create")
+        check(lib().mhip_batch_create_w(ctx.h, W, n, norb.ctypes.data, offs.ctypes.data,
+                                        dets.ctypes.data, Tp, Vp, C.byref(p)), "batch_create")
         self.h = p.value
         self.ninst = n
         self.offsets = offs

@@ -468,6 +468,7 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 
 struct mhip_batch {
   int device = 0;
+  int W = 1;  // words per determinant (1: wfn_t<64>, 2: wfn_t<128>)
   int ninst = 0;
   int64_t N = 0;
   std::vector<int> order;          // internal position -> caller's instance index
@@ -602,16 +603,20 @@ using namespace mhip;
 
 extern "C" {
 
-int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
-                      const int64_t* det_offsets, const uint64_t* dets, const double* const* T,
-                      const double* const* V, mhip_batch** out) {
+int mhip_batch_create_w(mhip_context* ctx, int words_per_det, int ninst, const int32_t* norb,
+                        const int64_t* det_offsets, const uint64_t* dets,
+                        const double* const* T, const double* const* V, mhip_batch** out) {
   return guarded([&] {
     use_device(ctx);
     MHIP_REQUIRE(out && ninst > 0 && norb && det_offsets && dets && T && V,
                  MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_REQUIRE(words_per_det == 1 || words_per_det == 2, MHIP_ERR_UNSUPPORTED,
+                 "words_per_det must be 1 (wfn_t<64>) or 2 (wfn_t<128>)");
+    const int W = words_per_det;
     *out = nullptr;
     std::unique_ptr<mhip_batch> B(new mhip_batch());
     B->device = ctx->device;
+    B->W = W;
     B->ninst = ninst;
     // internal order: by size, largest first (contiguous size buckets)
     B->order.resize(ninst);
@@ -625,30 +630,33 @@ int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
       const int b = B->order[p];
       const int64_t n = det_offsets[b + 1] - det_offsets[b];
       MHIP_REQUIRE(n > 0, MHIP_ERR_INVALID_ARG, "empty instance");
-      MHIP_REQUIRE(norb[b] >= 1 && norb[b] <= 32, MHIP_ERR_INVALID_ARG,
-                   "batched instances use wfn_t<64> (norb <= 32)");
+      MHIP_REQUIRE(norb[b] >= 1 && norb[b] <= 32 * W, MHIP_ERR_INVALID_ARG,
+                   W == 1 ? "wfn_t<64> instances need norb <= 32"
+                          : "wfn_t<128> instances need norb <= 64");
       B->off[p + 1] = B->off[p] + n;
       B->norb[p] = norb[b];
     }
     const int64_t N = B->N = B->off[ninst];
     MHIP_REQUIRE(N < (1ll << 31), MHIP_ERR_INDEX_OVERFLOW, "too many determinants");
-    std::vector<uint64_t> hd(N);
+    std::vector<uint64_t> hd(W * N);
     std::vector<int32_t> ri(N);
     for(int p = 0; p < ninst; ++p) {
       const int b = B->order[p];
-      const uint64_t bad = ~((1ull << B->norb[p]) - 1ull);
+      const uint64_t bad = B->norb[p] >= 64 ? 0ull : ~((1ull << B->norb[p]) - 1ull);
       for(int64_t i = 0; i < B->off[p + 1] - B->off[p]; ++i) {
-        const uint64_t d = dets[det_offsets[b] + i];
-        MHIP_REQUIRE(!(((d & 0xffffffffull) | (d >> 32)) & bad), MHIP_ERR_INVALID_ARG,
+        const uint64_t* d = dets + W * (det_offsets[b] + i);
+        const uint64_t spins = W == 1 ? ((d[0] & 0xffffffffull) | (d[0] >> 32)) : (d[0] | d[1]);
+        MHIP_REQUIRE(!(spins & bad), MHIP_ERR_INVALID_ARG,
                      "determinant occupies an orbital >= norb");
-        hd[B->off[p] + i] = d;
+        for(int w = 0; w < W; ++w) hd[W * (B->off[p] + i) + w] = d[w];
         ri[B->off[p] + i] = p;
       }
     }
-    B->dets.alloc(8 * N);
+    B->dets.alloc(8 * W * N);
     B->row_inst.alloc(4 * N);
     B->d_off.alloc(8 * (ninst + 1));
-    MHIP_HIP_CHECK(hipMemcpyAsync(B->dets.p, hd.data(), 8 * N, hipMemcpyHostToDevice, ctx->stream));
+    MHIP_HIP_CHECK(hipMemcpyAsync(B->dets.p, hd.data(), 8 * W * N, hipMemcpyHostToDevice,
+                                  ctx->stream));
     MHIP_HIP_CHECK(hipMemcpyAsync(B->row_inst.p, ri.data(), 4 * N, hipMemcpyHostToDevice, ctx->stream));
     MHIP_HIP_CHECK(hipMemcpyAsync(B->d_off.p, B->off.data(), 8 * (ninst + 1), hipMemcpyHostToDevice,
                                   ctx->stream));
@@ -715,6 +723,12 @@ int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
   });
 }
 
+int mhip_batch_create(mhip_context* ctx, int ninst, const int32_t* norb,
+                      const int64_t* det_offsets, const uint64_t* dets, const double* const* T,
+                      const double* const* V, mhip_batch** out) {
+  return mhip_batch_create_w(ctx, 1, ninst, norb, det_offsets, dets, T, V, out);
+}
+
 int mhip_batch_destroy(mhip_batch* B) {
   return guarded([&] {
     if(!B) return;
@@ -758,8 +772,12 @@ int mhip_batch_build(mhip_context* ctx, mhip_batch* B, double thresh) {
     B->H.reset(new mhip_csr());
     B->H->device = ctx->device;
     B->H->m = B->H->n = B->N;
-    build_impl<1>(ctx, B->dets.as<uint64_t>(), B->N, B->dets.as<uint64_t>(), B->N, thresh,
-                  B->H.get(), ex);
+    if(B->W == 1)
+      build_impl<1>(ctx, B->dets.as<uint64_t>(), B->N, B->dets.as<uint64_t>(), B->N, thresh,
+                    B->H.get(), ex);
+    else
+      build_impl<2>(ctx, B->dets.as<uint64_t>(), B->N, B->dets.as<uint64_t>(), B->N, thresh,
+                    B->H.get(), ex);
   });
 }
 

@@ -190,3 +190,45 @@ def test_c4_all_energies_match_per_instance_solves(ctx, c4):
     E1, res1, _ = B.davidson(max_m=20, max_restarts=200, tol=1e-8, use_graphs=True)
     B.set_guess(None)
     assert np.all(np.abs(E1 - E) <= 1e-8), float(np.abs(E1 - E).max())
+
+
+def test_batch_128bit_instances(ctx):
+    """wfn_t<128> instances (mhip_batch_create_w, words_per_det 2): complete
+    CAS(6,6) / CAS(8,8) spaces and selected lists over 36 and 40 orbitals
+    (norb > 32). Every block equals the oracle's CSR and every energy the
+    per-instance selected_ci_diag solve from the same guess."""
+    from macis_amd.synthetic import perturbed_diagonal_guess, selected_ci_dets
+
+    announce("batch of wfn_t<128> instances, norb up to 40")
+    norbs, dl, Ts, Vs = [], [], [], []
+    for i, (n, kind) in enumerate([(6, "cas"), (36, "sel"), (8, "cas"), (40, "sel"), (36, "sel")]):
+        T, V = seeded_integrals(n, 700 + i)
+        if kind == "cas":
+            d = D.generate_hilbert_space(2, n, n // 2, n // 2)
+        else:
+            d = selected_ci_dets(n, 4, 4, 1500 + 300 * i, seed=i, energies=0.5 * np.arange(n))
+        norbs.append(n)
+        dl.append(d)
+        Ts.append(T)
+        Vs.append(V)
+    B = _capi.Batch(ctx, norbs, dl, Ts, Vs, W=2)
+    B.build(EPS)
+    for i in range(len(norbs)):
+        rp, ci, nz = B.instance_csr(i)
+        ref = O.Ham(Ts[i], Vs[i]).csr(2, dl[i], thresh=EPS)
+        assert np.array_equal(rp, ref[0]) and np.array_equal(ci, ref[1]), i
+        assert np.array_equal(nz.view(np.uint64), ref[2].view(np.uint64)), i
+    Dg = B.diagonal()
+    seg = [slice(int(B.offsets[i]), int(B.offsets[i + 1])) for i in range(len(norbs))]
+    X0 = np.concatenate([perturbed_diagonal_guess(Dg[seg[i]], seed=i) for i in range(len(norbs))])
+    B.set_guess(X0)
+    E, res, _ = B.davidson(max_m=20, max_restarts=200, tol=1e-8)
+    assert np.all(res < 1e-8)
+    for i in range(len(norbs)):
+        ctx.set_integrals(norbs[i], Ts[i], Vs[i])
+        Ep, _, _ = ctx.selected_ci_diag(2, ctx.to_device(dl[i]), len(dl[i]),
+                                        ctx.to_device(X0[seg[i]]), max_restarts=200)
+        assert abs(Ep - E[i]) <= 1e-8, (i, E[i], Ep)
+    # the one-word entry point refuses norb > 32
+    with pytest.raises(_capi.MhipError, match="norb <= 32"):
+        _capi.Batch(ctx, [36], [dl[1][:, 0]], [Ts[1]], [Vs[1]], W=1)
