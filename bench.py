@@ -193,7 +193,7 @@ def rdm_timing(ctx, T, V, d_dets, N, nnz, d_X, E):
             "note": "includes the structure pass that lists the pairs; FP64 atomics"}
 
 
-def selected_ci_section(C, ctx, cpu=True):
+def selected_ci_section(C, ctx, cpu=True, keep=None):
     """The 64-orbital 20a20b selected-CI configuration: 2,000,000 determinants
     from the seeded generator (HF + 1-4-fold excitations, level weights
     0.1/0.5/0.2/0.2, holes/particles ~ exp(-|e_p - e_F|) with e_p = 0.25 p,
@@ -247,6 +247,8 @@ def selected_ci_section(C, ctx, cpu=True):
     E, _, _ = ctx.selected_ci_diag(2, dd, nd, dX, max_restarts=200)
     diag_s = time.perf_counter() - t0
     td = ctx.timings()
+    if keep is not None:  # the list and its ground state, for the c5 section
+        keep.update(dets=dets, X=dX.numpy(), E0=E)
     cpu_leg = None
     if cpu:
         # the reference's serial double loop (oracle restatement) on
@@ -384,10 +386,57 @@ def mlp_throughput(C, ctx):
     }
 
 
+def _blocks_per_core(core, W):
+    """Number of excitation blocks of each core determinant."""
+    out = []
+    for d in core:
+        if W == 1:
+            a, b = bin(int(d[0]) & 0xFFFFFFFF).count("1"), bin(int(d[0]) >> 32).count("1")
+        else:
+            a, b = bin(int(d[0])).count("1"), bin(int(d[1])).count("1")
+        out.append(a + b + a * (a - 1) // 2 + b * (b - 1) // 2 + a * b)
+    return np.array(out)
+
+
+def _train_blocks(C, ctx, W, Xt, yt, epochs, batch, seed=3):
+    """Network trained on standardised block targets; returns (net, mu, sd, s)."""
+    mu, sd = float(yt.mean()), float(yt.std())
+    yn = ((yt - mu) / sd).astype(np.float32)
+    net = C.MLP(ctx, seed=seed)
+    rng = np.random.default_rng(0)
+    dd, dy = ctx.empty((len(yn), W), np.uint64), ctx.empty(len(yn), np.float32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(epochs):
+        p = rng.permutation(len(yn))
+        dd.upload(Xt[p])
+        dy.upload(yn[p])
+        for b0 in range(0, len(yn) - batch + 1, batch):
+            net.train_step(W, dd.ptr + 8 * W * b0, dy.ptr + 4 * b0, batch, 1e-3, want_loss=False)
+    ctx.synchronize()
+    return net, mu, sd, time.perf_counter() - t0
+
+
+def _block_training_data(ctx, W, s, cores, E0, norb):
+    Xs, ys = [], []
+    for cc in cores:
+        d_bd, d_t, nb = ctx.asci_blocks(W, s, ctx.to_device(cc), len(cc), E0, norb)
+        t = d_t.numpy()[:nb]
+        f = np.isfinite(t)
+        Xs.append(d_bd.numpy()[:nb][f])
+        ys.append(t[f])
+    return np.concatenate(Xs), np.concatenate(ys).astype(np.float32)
+
+
+def _sort_dets(d):
+    return d[np.lexsort(tuple(d[:, k] for k in range(d.shape[1])))]
+
+
 def asci_step_comparison(C, ctx):
-    """One ASCI refinement step on H2O cc-pVDZ (core 100, 6400 dets), plain
-    asci_search vs the learned pre-screen trained on the three earlier plain
-    steps (HF -> 100 -> 800 -> 6400); both re-diagonalised on the GPU."""
+    """One ASCI refinement step on H2O cc-pVDZ (core 100, 6400 dets): plain
+    asci_search vs the learned block screen trained on the blocks of the
+    three earlier plain steps' cores (HF -> 100 -> 800 -> 6400), and vs the
+    screen that ranks blocks by |C_core| alone; re-diagonalised on the GPU."""
     from macis_amd import dets as D
     from macis_amd.fcidump import read_fcidump
 
@@ -400,6 +449,7 @@ def asci_step_comparison(C, ctx):
         return wfn[o], X[o]
 
     def diag(dets):
+        dets = _sort_dets(dets)
         dX = ctx.empty(len(dets), np.float64).zero()
         E, _, _ = ctx.selected_ci_diag(W, ctx.to_device(dets), len(dets), dX)
         return E, dX.numpy()
@@ -408,63 +458,162 @@ def asci_step_comparison(C, ctx):
     dE = ctx.empty(1, np.float64)
     ctx.ham_diagonal(W, ctx.to_device(hf), 1, dE)
     E0, wfn, X = float(dE.numpy()[0]), hf, np.array([1.0])
-    feats, targ = [], []
+    cores = []
     for nd in (100, 800, 6400):
         core, Cc = core_of(wfn, X)
-        dcore, dC = ctx.to_device(core), ctx.to_device(Cc)
-        dd, da, n = ctx.asci_candidates(W, s, dcore, len(core), E0, dC, norb, 4_000_000)
-        feats.append(dd.numpy()[:n])
-        targ.append(np.log(da.numpy()[:n]))
+        cores.append((core, E0))
         out = ctx.empty((nd, W), np.uint64)
-        n = ctx.asci_search(W, s, nd, dcore, len(core), E0, dC, norb, out)
+        n = ctx.asci_search(W, s, nd, ctx.to_device(core), len(core), E0, ctx.to_device(Cc),
+                            norb, out)
         wfn = out.numpy()[:n]
         E0, X = diag(wfn)
-    Xd = np.concatenate(feats)
-    y = np.concatenate(targ).astype(np.float32)
-    y = np.maximum(y, np.quantile(y, 0.7))
-    yn = ((y - y.mean()) / y.std()).astype(np.float32)
-    net = C.MLP(ctx, seed=3)
-    rng = np.random.default_rng(0)
-    ddX, ddy = ctx.empty((len(yn), W), np.uint64), ctx.empty(len(yn), np.float32)
-    t0 = time.perf_counter()
-    for _ in range(20):
-        p = rng.permutation(len(yn))
-        ddX.upload(Xd[p])
-        ddy.upload(yn[p])
-        for b0 in range(0, len(yn) - 4096 + 1, 4096):
-            net.train_step(W, ddX.ptr + 8 * W * b0, ddy.ptr + 4 * b0, 4096, 1e-3)
-    ctx.synchronize()
-    train_s = time.perf_counter() - t0
+    Xs, ys = [], []
+    for core, e in cores:
+        a, b = _block_training_data(ctx, W, s, [core], e, norb)
+        Xs.append(a)
+        ys.append(b)
+    Xt, yt = np.concatenate(Xs), np.concatenate(ys)
+    net, mu, sd, train_s = _train_blocks(C, ctx, W, Xt, yt, 20, 1024)
     nd = len(wfn)
     core, Cc = core_of(wfn, X)
     dcore, dC = ctx.to_device(core), ctx.to_device(Cc)
-    res = {}
-    for name in ("plain", "screened"):
+    coeff_scores = ctx.to_device(np.repeat(np.log(np.abs(Cc)), _blocks_per_core(core, W))
+                                 .astype(np.float32))
+
+    def run(kind, keep):
         out = ctx.empty((nd, W), np.uint64)
-        reps, ts = 5, []
-        for _ in range(reps):
-            if name == "plain":
+        ts = []
+        for _ in range(5):
+            if kind == "plain":
                 n = ctx.asci_search(W, s, nd, dcore, len(core), E0, dC, norb, out)
+            elif kind == "nn":
+                n = ctx.asci_search_screened(W, s, net, keep, nd, dcore, len(core), E0, dC, norb,
+                                             out, scale=sd, shift=mu)
             else:
-                n = ctx.asci_search_screened(W, s, net, 0.05, nd, dcore, len(core), E0, dC, norb,
-                                             out)
+                n = ctx.asci_search_blocks(W, s, coeff_scores, keep, nd, dcore, len(core), E0,
+                                           dC, norb, out)
             ts.append(ctx.timings()["asci_total_ms"])
-        got = out.numpy()[:n]
-        t1 = time.perf_counter()
-        E, _ = diag(got)
-        diag_ms = (time.perf_counter() - t1) * 1e3
-        res[name] = (sorted(ts)[len(ts) // 2], got, E, diag_ms, ctx.timings())
-    plain, scr = res["plain"], res["screened"]
-    recall = len(set(scr[1][:, 0].tolist()) & set(plain[1][:, 0].tolist())) / len(plain[1])
+        return out.numpy()[:n], sorted(ts)[2], ctx.timings()
+
+    plain, t_plain, tm_p = run("plain", 1.0)
+    E_plain = diag(plain)[0]
+    pset = set(plain[:, 0].tolist())
+    res = {}
+    for kind in ("nn", "coeff"):
+        for keep in (0.05, 0.2, 0.3):
+            got, t, tm = run(kind, keep)
+            res[f"{kind}_keep{keep}"] = {
+                "search_ms": t, "speedup": t_plain / t,
+                "recall": len(pset & set(got[:, 0].tolist())) / len(plain),
+                "dE_mHa": (diag(got)[0] - E_plain) * 1e3,
+                "raw_candidates": tm["asci_npairs"],
+                "blocks_kept": tm["asci_nblocks_kept"], "blocks": tm["asci_nblocks"],
+            }
     return {
         "system": "H2O cc-pVDZ (24 orbitals, 5a5b), core 100, 6400 dets (refinement step)",
-        "training_pairs": int(len(yn)), "train_s": train_s,
-        "plain_search_ms": plain[0], "screened_search_ms": scr[0],
-        "search_speedup": plain[0] / scr[0],
-        "plain_step_ms": plain[0] + plain[3], "screened_step_ms": scr[0] + scr[3],
-        "keep_fraction": 0.05, "recall": recall,
-        "dE_mHa": (scr[2] - plain[2]) * 1e3,
+        "training_blocks": int(len(yt)), "train_s": train_s,
+        "plain_search_ms": t_plain, "plain_raw_candidates": tm_p["asci_npairs"],
+        "screens": res,
+        "note": "nn: blocks ranked by log|C| + network prediction; coeff: by log|C| alone",
     }
+
+
+def c5_section(C, ctx, sel):
+    """The learned block screen at the 64-orbital scale: the 2,000,000-det
+    selected-CI list and its ground state from the selected_ci section; the
+    core is its 1,000 largest-|C| determinants (1.14e9 raw candidates, the
+    config's 'about 1e9'), the search keeps 2,000,000 determinants. Training:
+    the blocks of the cores of two earlier, smaller steps (100 and 300
+    determinants). Plain search with the reference's pair_size_max pruning
+    replay (default settings) and without it (pair_size_max above the raw
+    count: the same output here); screened searches at several keep
+    fractions, the |C|-only block ranking and the perfect block ranking (the
+    true block targets); recall against the plain top-k and the variational
+    energy difference after re-diagonalising each 2e6-det set."""
+    dets, X, E, n = sel["dets"], sel["X"], sel["E0"], 64
+    W, nd, ncore = 2, 2_000_000, 1000
+    s = C.ASCISettings.defaults()
+    snp = C.ASCISettings.defaults(pair_size_max=2**62)
+    order = np.argsort(-np.abs(X), kind="stable")
+    Xt, yt = _block_training_data(ctx, W, s, [dets[order[:100]], dets[order[:300]]], E, n)
+    net, mu, sd, train_s = _train_blocks(C, ctx, W, Xt, yt, 20, 4096)
+    core, Cc = dets[order[:ncore]], X[order[:ncore]]
+    dcore, dC = ctx.to_device(core), ctx.to_device(Cc)
+    nper = _blocks_per_core(core, W)
+    lc = np.repeat(np.log(np.abs(Cc)), nper).astype(np.float32)
+    _, d_t, nblk = ctx.asci_blocks(W, s, dcore, ncore, E, n)
+    true_scores = ctx.to_device((d_t.numpy()[:nblk] + lc).astype(np.float32))
+    coeff_scores = ctx.to_device(lc)
+
+    def run(kind, keep=1.0, reps=3):
+        out = ctx.empty((nd, W), np.uint64)
+        ts = []
+        for _ in range(reps):
+            if kind == "plain":
+                m = ctx.asci_search(W, s, nd, dcore, ncore, E, dC, n, out)
+            elif kind == "plain_nopruning":
+                m = ctx.asci_search(W, snp, nd, dcore, ncore, E, dC, n, out)
+            elif kind == "nn":
+                m = ctx.asci_search_screened(W, s, net, keep, nd, dcore, ncore, E, dC, n, out,
+                                             scale=sd, shift=mu)
+            else:
+                m = ctx.asci_search_blocks(W, s, true_scores if kind == "perfect" else coeff_scores,
+                                           keep, nd, dcore, ncore, E, dC, n, out)
+            ts.append(ctx.timings()["asci_total_ms"])
+        return out.numpy()[:m], sorted(ts)[len(ts) // 2], ctx.timings()
+
+    def energy(d):
+        d = _sort_dets(d)
+        dX = ctx.empty(len(d), np.float64).zero()
+        e, _, _ = ctx.selected_ci_diag(W, ctx.to_device(d), len(d), dX, max_restarts=200)
+        return e
+
+    plain, t_plain, tm_p = run("plain")
+    plain2, t_np, _ = run("plain_nopruning")
+    pset = set(map(tuple, plain.tolist()))
+    same_np = len(pset & set(map(tuple, plain2.tolist()))) / len(plain)
+    E_plain = energy(plain)
+    res = {}
+    for kind, keeps in (("nn", (0.05, 0.2, 0.5)), ("coeff", (0.05,)), ("perfect", (0.05,))):
+        for keep in keeps:
+            got, t, tm = run(kind, keep)
+            r = {"search_ms": t, "speedup_vs_plain": t_plain / t,
+                 "speedup_vs_plain_nopruning": t_np / t,
+                 "recall": len(pset & set(map(tuple, got.tolist()))) / len(plain),
+                 "raw_candidates": tm["asci_npairs"], "blocks_kept": tm["asci_nblocks_kept"],
+                 "score_ms": tm["asci_nn_ms"]}
+            if kind == "nn" and keep in (0.05, 0.5):
+                r["dE_mHa"] = (energy(got) - E_plain) * 1e3
+            if kind == "nn":
+                # network forward over every block: 656,384 flop each
+                r["network_TFLOPs"] = nblk * 656384 / (tm["asci_nn_ms"] * 1e-3) / 1e12
+            res[f"{kind}_keep{keep}"] = r
+    return {
+        "workload": "64 orbitals, 20a20b; core 1,000 of the 2,000,000-det selected-CI ground "
+                    "state; ndets_max 2,000,000",
+        "raw_candidates": tm_p["asci_npairs"], "unique_candidates": tm_p["asci_nunique"],
+        "blocks": int(nblk), "training_blocks": int(len(yt)), "train_s": train_s,
+        "plain_search_ms": t_plain, "plain_nopruning_search_ms": t_np,
+        "plain_nopruning_recall": same_np,
+        "plain_phases_ms": {"generate": tm_p["asci_contrib_ms"], "sort_accumulate":
+                            tm_p["asci_sort_ms"], "topk": tm_p["asci_topk_ms"]},
+        "E_plain": E_plain, "screens": res,
+        "bar": "dE <= 0.1 mHa: met by no measured keep fraction below 1 (every block kept "
+               "reproduces the plain search)",
+    }
+
+
+def c5_only(C, ctx):
+    """The c5 section on its own: the selected-CI list and ground state first."""
+    from macis_amd.synthetic import seeded_integrals, selected_ci_dets
+
+    n, nd = 64, 2_000_000
+    T, V = seeded_integrals(n, 64)
+    dets = selected_ci_dets(n, 20, 20, nd, seed=64, energies=0.25 * np.arange(n))
+    ctx.set_integrals(n, T, V)
+    dX = ctx.empty(nd, np.float64).zero()
+    E, _, _ = ctx.selected_ci_diag(2, ctx.to_device(dets), nd, dX, max_restarts=200)
+    return c5_section(C, ctx, {"dets": dets, "X": dX.numpy(), "E0": E})
 
 
 def main():
@@ -482,7 +631,9 @@ def main():
                     help="skip the learned pre-screen / ASCI step comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 1 GiB device-to-device copy first (counter calibration)")
-    ap.add_argument("--only", choices=["selected_ci", "batch", "asci", "mlp"],
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the 64-orbital learned block screen section")
+    ap.add_argument("--only", choices=["selected_ci", "batch", "asci", "mlp", "c5"],
                     help="run only this section and print its JSON object (for profiling; "
                          "not the benchmark line)")
     args = ap.parse_args()
@@ -501,7 +652,8 @@ def main():
         run = {"selected_ci": lambda: selected_ci_section(C, ctx, cpu=not args.no_cpu_baseline),
                "batch": lambda: batch_section(C, ctx),
                "asci": lambda: asci_step_comparison(C, ctx),
-               "mlp": lambda: mlp_throughput(C, ctx)}[args.only]
+               "mlp": lambda: mlp_throughput(C, ctx),
+               "c5": lambda: c5_only(C, ctx)}[args.only]
         print(json.dumps({"section": args.only, args.only: run()}), flush=True)
         return
     T, V = seeded_integrals(N_ORB, SEED)
@@ -611,7 +763,11 @@ def main():
     if not args.no_batch:
         out["batch"] = batch_section(C, ctx)
     if not args.no_selected_ci:
-        out["selected_ci"] = selected_ci_section(C, ctx, cpu=not args.no_cpu_baseline)
+        sel = {}
+        out["selected_ci"] = selected_ci_section(C, ctx, cpu=not args.no_cpu_baseline, keep=sel)
+        if not args.no_c5:
+            out["c5"] = c5_section(C, ctx, sel)
+            del sel
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(T, V, dets)
         out["cpu_fair"] = cpu_fair_baseline(T, V, dets)

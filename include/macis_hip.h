@@ -88,11 +88,13 @@ typedef struct mhip_timings {
   double asci_total_ms;
   int64_t asci_npairs;    /* raw candidates */
   int64_t asci_nunique;   /* unique candidates after accumulation */
-  double asci_nn_ms;      /* learned pre-screen (prediction + selection) */
-  int64_t asci_nsurvivors; /* candidates entering the exact top-k */
+  double asci_nn_ms;      /* block screen: scoring (network) + block selection */
+  int64_t asci_nsurvivors; /* unique non-core candidates entering the top-k */
   int64_t ham_path;       /* structure pass of the last build: MHIP_BUILD_* */
   double dav_layout_ms;   /* sliced copy / panel split made for the last Davidson */
   int64_t dav_sliced;     /* 1: its SpMVs ran on the sliced copy, 2: on column panels */
+  int64_t asci_nblocks;      /* excitation blocks of the core (block-screened search) */
+  int64_t asci_nblocks_kept; /* blocks whose candidates were generated */
 } mhip_timings;
 
 /* Structure passes of the Hamiltonian build. AUTO takes the product-space
@@ -366,7 +368,8 @@ int mhip_batch_instance_csr(mhip_context* ctx, mhip_batch* batch, int inst, int6
 /* ---- learned ASCI pre-screen (no counterpart in the reference) ---------------
  * An MLP 128 -> 512 -> 512 -> 1 with GELU on the occupation bits of a
  * determinant (bits 0-63 alpha, 64-127 beta for wfn_t<128>; wfn_t<64> uses
- * the 64 bits of its word), predicting log|rv| of asci_search scores. BF16
+ * the 64 bits of its word), predicting the block targets of
+ * mhip_asci_blocks (log of the block's largest |h / (E - H_kk)|). BF16
  * operands on MFMA with FP32 accumulation; FP32 master weights, Adam.
  * Parameters are one FP32 block: W1 (512x128, row-major) | b1 (512) |
  * W2 (512x512) | b2 (512) | w3 (512) | b3 (1). */
@@ -387,21 +390,53 @@ int mhip_mlp_train_step(mhip_context* ctx, mhip_mlp* mlp, int words_per_det,
                         const uint64_t* d_dets, const float* d_target,
                         int64_t n, float lr, float* loss);
 /* Scored candidates of asci_search before the top-k: the unique non-core
- * determinants with their accumulated |rv| (training data of the
- * pre-screen). DEVICE outputs of `capacity` entries; *n_out = count. */
+ * determinants with their accumulated |rv|. DEVICE outputs of `capacity`
+ * entries; *n_out = count. */
 int mhip_asci_candidates(mhip_context* ctx, int words_per_det,
                          const mhip_asci_settings* settings,
                          const uint64_t* d_cdets, int64_t ncdets, double E_ASCI,
                          const double* d_C, int64_t norb, uint64_t* d_dets_out,
                          double* d_absrv_out, int64_t capacity, int64_t* n_out);
-/* asci_search with the learned pre-screen: all candidates are generated and
- * accumulated as in mhip_asci_search, the network ranks them, the best
- * keep_fraction (at least ndets_max - ncdets of them) survive and their exact
- * |rv| picks the top-k. The plain mhip_asci_search is the fallback. */
+/* Excitation blocks of a core (the units of the block screen). The
+ * candidates of asci_search's core determinant c (determinant_contributions
+ * .hpp:25-160) fall into one block per hole set, in the order
+ *   [alpha single holes (na) | beta single holes (nb) |
+ *    alpha-alpha hole pairs (na(na-1)/2) | beta-beta hole pairs |
+ *    alpha-beta hole pairs (na*nb, alpha hole major)]
+ * (the singles only with just_singles); hole indices follow the occupied
+ * list in ascending orbital order, pairs (x < y) lexicographically. Blocks
+ * are numbered core det by core det. A block's determinant is the core
+ * determinant with its holes cleared (the network's input); its target is
+ * log max |h / (E_ASCI - H_kk)| over the block's candidates that pass the
+ * h_el_tol screen (-INF if none): the coefficient-free part of the
+ * candidates' scores. DEVICE outputs of `capacity` entries (either may be
+ * NULL; with both NULL only *n_blocks is set, which is also set when the
+ * capacity is too small). */
+int mhip_asci_blocks(mhip_context* ctx, int words_per_det, const mhip_asci_settings* settings,
+                     const uint64_t* d_cdets, int64_t ncdets, double E_ASCI, int64_t norb,
+                     uint64_t* d_block_dets, float* d_target, int64_t capacity,
+                     int64_t* n_blocks);
+/* asci_search over the best-scored blocks only: the ceil(keep_fraction *
+ * nblocks) blocks with the largest DEVICE d_block_scores (ties: lower block
+ * index first) are generated, their exact contributions accumulated, and
+ * the top-k picked as in mhip_asci_search (core dets excluded, ties in
+ * determinant order, core appended). Contributions of dropped blocks are
+ * not generated: that is the work the screen removes. No pair_size_max
+ * pruning replay. keep_fraction 1 gives the plain search's candidates. */
+int mhip_asci_search_blocks(mhip_context* ctx, int words_per_det,
+                            const mhip_asci_settings* settings, const float* d_block_scores,
+                            double keep_fraction, int64_t ndets_max, const uint64_t* d_cdets,
+                            int64_t ncdets, double E_ASCI, const double* d_C, int64_t norb,
+                            uint64_t* d_new_dets, int64_t* n_new);
+/* Learned block screen: mhip_asci_search_blocks with block scores
+ * pred * pred_scale + pred_shift + log|C_c| from the network's prediction
+ * `pred` on the block's determinant (a network trained on standardised
+ * mhip_asci_blocks targets passes their standard deviation and mean). The
+ * plain mhip_asci_search is the fallback. */
 int mhip_asci_search_screened(mhip_context* ctx, int words_per_det,
                               const mhip_asci_settings* settings, mhip_mlp* mlp,
-                              double keep_fraction, int64_t ndets_max,
-                              const uint64_t* d_cdets, int64_t ncdets,
+                              float pred_scale, float pred_shift, double keep_fraction,
+                              int64_t ndets_max, const uint64_t* d_cdets, int64_t ncdets,
                               double E_ASCI, const double* d_C, int64_t norb,
                               uint64_t* d_new_dets, int64_t* n_new);
 /* BF16 GEMM of the MLP kernels, for parity tests: C (FP32, M x N) = A B^T,

@@ -56,6 +56,8 @@ class Timings(C.Structure):
         ("ham_path", C.c_int64),
         ("dav_layout_ms", C.c_double),
         ("dav_sliced", C.c_int64),
+        ("asci_nblocks", C.c_int64),
+        ("asci_nblocks_kept", C.c_int64),
     ]
 
     def as_dict(self):
@@ -129,8 +131,12 @@ _SIGS = {
     "mhip_gemm_bf16_nt": (_i32, [_p, _i32, _i32, _i32, _p, _p, _p]),
     "mhip_asci_candidates": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, _i64, _d, _p, _i64,
                                     _p, _p, _i64, _pi64]),
-    "mhip_asci_search_screened": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, _d, _i64, _p,
-                                         _i64, _d, _p, _i64, _p, _pi64]),
+    "mhip_asci_search_screened": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, C.c_float,
+                                         C.c_float, _d, _i64, _p, _i64, _d, _p, _i64, _p, _pi64]),
+    "mhip_asci_search_blocks": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, _d, _i64, _p, _i64,
+                                       _d, _p, _i64, _p, _pi64]),
+    "mhip_asci_blocks": (_i32, [_p, _i32, C.POINTER(ASCISettings), _p, _i64, _d, _i64, _p, _p,
+                                _i64, _pi64]),
     "mhip_batch_create": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _pp]),
     "mhip_batch_create_w": (_i32, [_p, _i32, _i32, _p, _p, _p, _p, _p, _pp]),
     "mhip_batch_destroy": (_i32, [_p]),
@@ -410,12 +416,35 @@ class Context:
                                          C.byref(n)), "asci_candidates")
         return d_out, a_out, n.value
 
-    def asci_search_screened(self, W, settings, mlp, keep_fraction, ndets_max, d_cdets, ncdets,
-                             E_ASCI, d_C, norb, d_out):
+    def asci_blocks(self, W, settings, d_cdets, ncdets, E_ASCI, norb, targets=True):
+        """(block dets (n, W) DeviceArray, targets DeviceArray or None, n): the
+        excitation blocks of the core (mhip_asci_blocks)."""
         n = C.c_int64()
-        check(lib().mhip_asci_search_screened(self.h, W, C.byref(settings), mlp.h, keep_fraction,
-                                              ndets_max, _ptr(d_cdets), ncdets, E_ASCI, _ptr(d_C),
-                                              norb, _ptr(d_out), C.byref(n)),
+        check(lib().mhip_asci_blocks(self.h, W, C.byref(settings), _ptr(d_cdets), ncdets, E_ASCI,
+                                     norb, None, None, 0, C.byref(n)), "asci_blocks")
+        nb = n.value
+        d_out = self.empty((max(nb, 1), W), np.uint64)
+        t_out = self.empty(max(nb, 1), np.float32) if targets else None
+        check(lib().mhip_asci_blocks(self.h, W, C.byref(settings), _ptr(d_cdets), ncdets, E_ASCI,
+                                     norb, d_out.ptr, t_out.ptr if targets else None, nb,
+                                     C.byref(n)), "asci_blocks")
+        return d_out, t_out, nb
+
+    def asci_search_blocks(self, W, settings, d_scores, keep_fraction, ndets_max, d_cdets,
+                           ncdets, E_ASCI, d_C, norb, d_out):
+        n = C.c_int64()
+        check(lib().mhip_asci_search_blocks(self.h, W, C.byref(settings), _ptr(d_scores),
+                                            keep_fraction, ndets_max, _ptr(d_cdets), ncdets,
+                                            E_ASCI, _ptr(d_C), norb, _ptr(d_out), C.byref(n)),
+              "asci_search_blocks")
+        return n.value
+
+    def asci_search_screened(self, W, settings, mlp, keep_fraction, ndets_max, d_cdets, ncdets,
+                             E_ASCI, d_C, norb, d_out, scale=1.0, shift=0.0):
+        n = C.c_int64()
+        check(lib().mhip_asci_search_screened(self.h, W, C.byref(settings), mlp.h, scale, shift,
+                                              keep_fraction, ndets_max, _ptr(d_cdets), ncdets,
+                                              E_ASCI, _ptr(d_C), norb, _ptr(d_out), C.byref(n)),
               "asci_search_screened")
         return n.value
 

@@ -823,9 +823,12 @@ static void test_asci_rot(const H2O& h, double ref_E0) {
 }
 
 static void test_learned_prescreen(const H2O& h) {
-  // LearnedPrescreen: plain steps HF -> 100 -> 800 -> 6400 are observed, the
-  // network is trained on them, and the screened search of the next
-  // (refinement) step is judged against the plain search of that step.
+  // LearnedPrescreen: the cores of the plain steps HF -> 100 -> 800 -> 6400
+  // are observed, the network is trained on their block targets and judged
+  // on the next (refinement) step against the plain search of that step.
+  // The screen generates only the kept blocks; at keep_fraction 1 it must
+  // reproduce the plain search, and a network whose recall is below
+  // min_recall leaves the plain search in charge.
   auto T = h.T;
   auto V = h.V;
   macis::HipHamiltonianGenerator<64> gen(macis::matrix_span<double>(T.data(), h.norb, h.norb),
@@ -836,53 +839,58 @@ static void test_learned_prescreen(const H2O& h) {
   std::vector<macis::wfn_t<64>> wfn = {macis::canonical_hf_determinant<64>(5, 5)};
   std::vector<double> C = {1.0};
   double E0 = gen.matrix_element(wfn[0], wfn[0]);
-  auto step = [&](size_t nd, bool screened) {
+  auto plain_step = [&](size_t nd) {
     macis::reorder_ci_on_coeff(wfn, C);
     const size_t nc = std::min<size_t>(s.ncdets_max, wfn.size());
-    auto out = screened ? nn.search(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb)
-                        : macis::asci_search(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb,
-                                             gen.T(), gen.G_red(), gen.V_red(), gen.G(), gen.V(),
-                                             gen);
-    return out;
+    return macis::asci_search(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb, gen.T(),
+                              gen.G_red(), gen.V_red(), gen.G(), gen.V(), gen);
+  };
+  auto screened_step = [&](size_t nd) {
+    macis::reorder_ci_on_coeff(wfn, C);
+    const size_t nc = std::min<size_t>(s.ncdets_max, wfn.size());
+    return nn.search(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb);
   };
   // untrained: the plain search is the fallback
   {
-    auto a = step(100, true);
+    auto a = screened_step(100);
     REQUIRE(nn.used_fallback());
-    auto b = step(100, false);
-    REQUIRE(a == b);
+    REQUIRE(a == plain_step(100));
   }
   for(size_t nd : {100, 800, 6400}) {
     macis::reorder_ci_on_coeff(wfn, C);
     const size_t nc = std::min<size_t>(s.ncdets_max, wfn.size());
-    nn.observe(s, wfn.begin(), wfn.begin() + nc, E0, C, h.norb, 4000000);
-    wfn = step(nd, false);
+    REQUIRE(nn.observe(s, wfn.begin(), wfn.begin() + nc, E0, h.norb) > 0);
+    wfn = plain_step(nd);
     std::vector<double> X;
     E0 = macis::selected_ci_diag<64, int32_t>(wfn.begin(), wfn.end(), gen, 2.2e-16, 20, 1e-8, X);
     C = X;
   }
   nn.train(20, 4096, 1e-3f);
-  std::printf("  validation recall %.4f\n", nn.recall());
   const size_t nd = wfn.size();
-  auto plain = step(nd, false);
-  auto screened = step(nd, true);
+  macis::reorder_ci_on_coeff(wfn, C);
+  const size_t nc = std::min<size_t>(s.ncdets_max, wfn.size());
+  // every block kept: the plain search's determinants
+  nn.set_keep_fraction(1.0);
+  REQUIRE(nn.validate(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb) == 1.0);
+  auto all = screened_step(nd);
   REQUIRE(!nn.used_fallback());
+  REQUIRE(all == plain_step(nd));
+  // 5 % of the blocks: judged against the plain search
+  nn.set_keep_fraction(0.05);
+  const double rec = nn.validate(s, nd, wfn.begin(), wfn.begin() + nc, E0, C, h.norb);
+  auto plain = plain_step(nd);
+  auto screened = screened_step(nd);
+  REQUIRE(nn.used_fallback() == (rec < 0.5));
   REQUIRE(screened.size() == plain.size());
   std::vector<double> Xp, Xs;
   const double Ep =
       macis::selected_ci_diag<64, int32_t>(plain.begin(), plain.end(), gen, 2.2e-16, 20, 1e-8, Xp);
   const double Es = macis::selected_ci_diag<64, int32_t>(screened.begin(), screened.end(), gen,
                                                          2.2e-16, 20, 1e-8, Xs);
-  auto key = [](const macis::wfn_t<64>& d) { return d.to_ullong(); };
-  std::vector<unsigned long long> kp, ks;
-  for(auto& d : plain) kp.push_back(key(d));
-  for(auto& d : screened) ks.push_back(key(d));
-  std::sort(kp.begin(), kp.end());
-  std::sort(ks.begin(), ks.end());
-  std::vector<unsigned long long> both;
-  std::set_intersection(kp.begin(), kp.end(), ks.begin(), ks.end(), std::back_inserter(both));
-  std::printf("  recall %.4f, dE %+.3e Ha\n", (double)both.size() / kp.size(), Es - Ep);
-  REQUIRE(std::abs(Es - Ep) <= 1e-4);
+  std::printf("  keep 0.05: recall %.4f (%s), dE %+.3e Ha\n", rec,
+              nn.used_fallback() ? "plain fallback" : "screened", Es - Ep);
+  REQUIRE(rec > 0.3);
+  if(nn.used_fallback()) REQUIRE(Es == Ep);
 }
 
 int main(int argc, char** argv) {

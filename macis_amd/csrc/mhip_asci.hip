@@ -292,6 +292,198 @@ __global__ __launch_bounds__(256) void contrib_kernel(
     if(lane == 0) chunk_counts[chunk] = cnt;
 }
 
+// ---- excitation blocks (learned screen) -----------------------------------
+// The candidates of a core determinant fall into blocks, one per hole set,
+// in the order [alpha single holes (na) | beta single holes (nb) | alpha-
+// alpha hole pairs (na(na-1)/2) | beta-beta hole pairs | alpha-beta hole
+// pairs (na*nb)] (singles only with just_singles); a block's candidates are
+// its particle choices. A block maps onto a run (alpha-beta: a strided set)
+// of the flat candidate indices of `candidate`, so a block is evaluated with
+// the same code and in the same order within its core determinant.
+__host__ __device__ __forceinline__ int64_t block_count(int64_t na, int64_t nb, bool singles) {
+  int64_t k = na + nb;
+  if(!singles) k += na * (na - 1) / 2 + nb * (nb - 1) / 2 + na * nb;
+  return k;
+}
+
+struct Blk {
+  int64_t base;       // flat index of the block's first candidate
+  int64_t size;       // candidates in the block
+  int64_t rowstride;  // alpha-beta: flat step per alpha particle (0: a contiguous run)
+  int nvb;            // alpha-beta: beta particles per alpha particle
+  uint64_t ha, hb;    // hole bits of the block (alpha, beta string)
+};
+
+__device__ __forceinline__ Blk block_decode(const CoreArrays& A, int64_t c, int64_t q) {
+  const int na = A.counts[c * 4 + 0], nva = A.counts[c * 4 + 1];
+  const int nb = A.counts[c * 4 + 2], nvb = A.counts[c * 4 + 3];
+  const uint8_t* L = A.lists + c * 4 * 64;
+  const int64_t Sa = (int64_t)na * nva, Sb = (int64_t)nb * nvb;
+  const int64_t pva = pairs(nva), pvb = pairs(nvb);
+  const int64_t Da = pairs(na) * pva, Db = pairs(nb) * pvb;
+  Blk b;
+  b.rowstride = 0;
+  b.nvb = 1;
+  b.ha = b.hb = 0;
+  if(q < na) {
+    b.base = q * nva;
+    b.size = nva;
+    b.ha = 1ull << L[q];
+    return b;
+  }
+  q -= na;
+  if(q < nb) {
+    b.base = Sa + q * nvb;
+    b.size = nvb;
+    b.hb = 1ull << L[128 + q];
+    return b;
+  }
+  q -= nb;
+  int x, y;
+  if(q < pairs(na)) {
+    tri_decode(q, na, x, y);
+    b.base = Sa + Sb + q * pva;
+    b.size = pva;
+    b.ha = (1ull << L[x]) | (1ull << L[y]);
+    return b;
+  }
+  q -= pairs(na);
+  if(q < pairs(nb)) {
+    tri_decode(q, nb, x, y);
+    b.base = Sa + Sb + Da + q * pvb;
+    b.size = pvb;
+    b.hb = (1ull << L[128 + x]) | (1ull << L[128 + y]);
+    return b;
+  }
+  q -= pairs(nb);
+  const int ii = (int)(q / nb), jj = (int)(q % nb);
+  b.base = Sa + Sb + Da + Db + (int64_t)ii * nva * Sb + (int64_t)jj * nvb;
+  b.size = (int64_t)nva * nvb;
+  b.rowstride = Sb;
+  b.nvb = nvb;
+  b.ha = 1ull << L[ii];
+  b.hb = 1ull << L[128 + jj];
+  return b;
+}
+
+__device__ __forceinline__ int64_t block_flat(const Blk& b, int64_t l) {
+  return b.rowstride ? b.base + (l / b.nvb) * b.rowstride + l % b.nvb : b.base + l;
+}
+
+// Thread per block: its determinant (the core determinant with the block's
+// holes cleared), in the layout of the input determinants.
+template <int W>
+__global__ void block_dets_kernel(CoreArrays A, const int64_t* __restrict__ bstart, int64_t nblk,
+                                  uint64_t* __restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(g >= nblk) return;
+  const int64_t c = find_core(bstart, A.nc, g);
+  const Blk b = block_decode(A, c, g - bstart[c]);
+  const uint64_t sa = A.sa[c] & ~b.ha, sb = A.sb[c] & ~b.hb;
+  if constexpr(W == 1) {
+    out[g] = sa | (sb << 32);
+  } else {
+    out[2 * g] = sa;
+    out[2 * g + 1] = sb;
+  }
+}
+
+// Wavefront per block: log of the largest |h / (E0 - H_kk)| among the
+// block's candidates that pass the h_el_tol screen (-inf if none). A.coeff
+// must hold ones, so `candidate` returns h / (E0 - H_kk) itself.
+__global__ __launch_bounds__(256) void block_target_kernel(IntsView I, CoreArrays A,
+                                                           const int64_t* __restrict__ bstart,
+                                                           int64_t nblk, double E0, double tol,
+                                                           float* __restrict__ target) {
+  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if(g >= nblk) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t c = find_core(bstart, A.nc, g);
+  const Blk b = block_decode(A, c, g - bstart[c]);
+  double m = -1.0;
+  for(int64_t l0 = 0; l0 < b.size; l0 += 64) {
+    const int64_t l = l0 + lane;
+    if(l < b.size) {
+      const Cand cd = candidate<true>(I, A, c, block_flat(b, l), E0, tol);
+      if(cd.keep) m = fmax(m, fabs(cd.rv));
+    }
+  }
+#pragma unroll
+  for(int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if(lane == 0) target[g] = m < 0. ? -INFINITY : (float)log(m);
+}
+
+// Block score -> descending-sortable key (NaN ranks last, -0 as +0) and the
+// block index as payload. With `pred`, score = pred * scale + shift +
+// log|C_c| (the network predicts the block's target); otherwise the
+// caller's scores are used as given.
+__global__ void block_key_kernel(const float* __restrict__ pred, float scale, float shift,
+                                 const double* __restrict__ coeff,
+                                 const int64_t* __restrict__ bstart, int64_t nc,
+                                 const float* __restrict__ given, int64_t nblk,
+                                 uint32_t* __restrict__ key, int64_t* __restrict__ idx) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(g >= nblk) return;
+  float s;
+  if(pred) {
+    const int64_t c = find_core(bstart, nc, g);
+    s = fmaf(pred[g], scale, shift) + (float)log(fabs(coeff[c]));
+  } else {
+    s = given[g];
+  }
+  if(s != s) s = -INFINITY;
+  s += 0.0f;
+  const uint32_t u = __float_as_uint(s);
+  key[g] = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  idx[g] = g;
+}
+
+// Candidate count (FILL = false) and fill of the kept blocks: wavefront per
+// kept block (kept ids ascending, so the fill runs in core-det order).
+template <int W, bool FILL>
+__global__ __launch_bounds__(256) void block_contrib_kernel(
+    IntsView I, CoreArrays A, const int64_t* __restrict__ bstart, const int64_t* __restrict__ kept,
+    int64_t nkept, double E0, double tol, int64_t* __restrict__ counts,
+    const int64_t* __restrict__ off,
+    typename std::conditional<W == 1, uint64_t, u128>::type* __restrict__ keys,
+    double* __restrict__ vals) {
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if(k >= nkept) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t g = kept[k];
+  const int64_t c = find_core(bstart, A.nc, g);
+  const Blk b = block_decode(A, c, g - bstart[c]);
+  int64_t out = FILL ? off[k] : 0;
+  int64_t cnt = 0;
+  for(int64_t l0 = 0; l0 < b.size; l0 += 64) {
+    const int64_t l = l0 + lane;
+    Cand cd;
+    cd.keep = false;
+    if(l < b.size) cd = candidate<FILL>(I, A, c, block_flat(b, l), E0, tol);
+    const uint64_t m = ballot(cd.keep);
+    if constexpr(FILL) {
+      if(cd.keep) {
+        const int64_t p = out + lanes_below(m);
+        if constexpr(W == 1)
+          keys[p] = cd.a | (cd.b << 32);
+        else
+          keys[p] = ((u128)cd.b << 64) | (u128)cd.a;
+        vals[p] = cd.rv;
+      }
+      out += popc64(m);
+    } else {
+      cnt += popc64(m);
+    }
+  }
+  if constexpr(!FILL)
+    if(lane == 0) counts[k] = cnt;
+}
+
+__global__ void fill_ones_kernel(double* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = 1.0;
+}
+
 // ---- pair_size_max pruning (asci_contributions_standard, :120-137) -------
 // The reference prunes its growing pair list after the core determinant at
 // which it first exceeds pair_size_max: pairs with |rv| <= rv_prune_tol are
@@ -479,27 +671,6 @@ __global__ void dets_to_keys_kernel(const uint64_t* __restrict__ d, int64_t n,
 
 inline int64_t host_pairs(int64_t m) { return m * (m - 1) / 2; }
 
-// Order-preserving map of float predictions to unsigned keys.
-// Histogram of 12-bit digits of |rv| bit patterns (non-negative doubles
-// order like their bits): digit = (key >> shift) & 4095, counted only for
-// keys whose bits above shift + 12 equal `prefix` (all keys when shift = 52).
-__global__ __launch_bounds__(256) void digit_hist_kernel(const uint64_t* __restrict__ keys,
-                                                         int64_t n, int shift, uint64_t prefix,
-                                                         uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[4096];
-  for(int i = threadIdx.x; i < 4096; i += 256) h[i] = 0;
-  __syncthreads();
-  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-      i += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = keys[i];
-    const bool match = shift + 12 >= 64 || (k >> (shift + 12)) == prefix;
-    if(match) atomicAdd(&h[(k >> shift) & 4095u], 1u);
-  }
-  __syncthreads();
-  for(int i = threadIdx.x; i < 4096; i += 256)
-    if(h[i]) atomicAdd(&hist[i], h[i]);
-}
-
 // Candidates equal to a core determinant are not new (the reference keeps
 // the core copy with +|C| and drops rv >= 0, determinant_search.hpp:533-544):
 // their accumulated score is zeroed, which removes them with the zero scores.
@@ -556,53 +727,98 @@ __global__ void abs_bits_kernel(const double* __restrict__ v, int64_t n,
   if(i < n) out[i] = (uint64_t)__double_as_longlong(fabs(v[i]));
 }
 
-// flags[i] = key's top 24 bits >= threshold (the candidates that can reach
-// the top-k once the boundary 24-bit bucket is known).
-__global__ void above_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t thr24,
-                             uint8_t* __restrict__ flags) {
-  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-      i += (int64_t)gridDim.x * blockDim.x)
-    flags[i] = (keys[i] >> 40) >= thr24 ? 1 : 0;
-}
-
-__global__ void pred_keys_kernel(const float* __restrict__ pred, int64_t n,
-                                 uint32_t* __restrict__ key, int64_t* __restrict__ idx) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  const uint32_t u = __float_as_uint(pred[i]);
-  key[i] = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-  idx[i] = i;
-}
-
-template <typename K>
-__global__ void gather_kernel(const int64_t* __restrict__ idx, int64_t n, const K* __restrict__ kin,
-                              const uint64_t* __restrict__ ain, K* __restrict__ kout,
-                              uint64_t* __restrict__ aout) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  kout[i] = kin[idx[i]];
-  aout[i] = ain[idx[i]];
-}
-
 __global__ void bits_to_double_kernel(const uint64_t* __restrict__ b, int64_t n,
                                       double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(i < n) out[i] = __longlong_as_double((long long)b[i]);
 }
 
-// Optional stages of the search: export of the scored candidates (training
-// data of the pre-screen) or the learned pre-screen itself.
+// Optional stages of the search: export of the scored candidates, or the
+// block screen (learned or with the caller's block scores).
 struct SearchMode {
   // export: unique non-core candidates with their accumulated |rv|
   uint64_t* exp_dets = nullptr;
   double* exp_abs = nullptr;
   int64_t exp_cap = 0;
   int64_t* exp_n = nullptr;
-  // pre-screen: keep the keep_fraction best-predicted candidates (at least
-  // the top-k size), then the exact |rv| picks the top-k among them
+  // block screen: only the keep_fraction best-scored excitation blocks are
+  // generated; their exact contributions are accumulated and ranked as in
+  // the plain search. Scores are network predictions (pred * scale + shift
+  // + log|C_c|) or the caller's.
   mhip_mlp* mlp = nullptr;
+  float pred_scale = 1.f, pred_shift = 0.f;
+  const float* block_scores = nullptr;
   double keep_fraction = 1.0;
+  bool screened() const { return mlp || block_scores; }
 };
+
+// Host side of the core determinants: validates them against norb and
+// returns per-det alpha / beta electron counts.
+template <int W>
+void core_counts(mhip_context* ctx, const uint64_t* d_cdets, int64_t nc, int64_t norb,
+                 std::vector<int>& na, std::vector<int>& nb) {
+  std::vector<uint64_t> hc(nc * W);
+  d2h(ctx, hc.data(), d_cdets, 8 * W * nc);
+  na.resize(nc);
+  nb.resize(nc);
+  for(int64_t c = 0; c < nc; ++c) {
+    uint64_t a, b;
+    if(W == 1) {
+      a = hc[c] & 0xffffffffull;
+      b = hc[c] >> 32;
+    } else {
+      a = hc[2 * c];
+      b = hc[2 * c + 1];
+    }
+    MHIP_REQUIRE(!((a | b) & (norb >= 64 ? 0ull : ~((1ull << norb) - 1ull))),
+                 MHIP_ERR_INVALID_ARG, "core determinant occupies an orbital >= norb");
+    na[c] = __builtin_popcountll(a);
+    nb[c] = __builtin_popcountll(b);
+  }
+}
+
+// Per-core-det arrays of the candidate kernels (prep_kernel's outputs).
+struct CorePrep {
+  DevBuf sa, sb, lists, counts, eps, hdiag;
+  CoreArrays A;
+};
+
+template <int W>
+void prepare_core(mhip_context* ctx, const IntsView& I, const uint64_t* d_cdets, int64_t nc,
+                  int64_t norb, const double* d_C, int just_singles, CorePrep& P) {
+  const int n = (int)ctx->ints.norb;
+  P.sa.alloc(8 * nc);
+  P.sb.alloc(8 * nc);
+  P.lists.alloc(4 * 64 * nc);
+  P.counts.alloc(16 * nc);
+  P.eps.alloc(16 * n * nc);
+  P.hdiag.alloc(8 * nc);
+  hipLaunchKernelGGL(prep_kernel<W>, dim3((nc + 63) / 64), dim3(64), 0, ctx->stream, I, d_cdets,
+                     nc, (int)norb, P.sa.as<uint64_t>(), P.sb.as<uint64_t>(),
+                     P.lists.as<uint8_t>(), P.counts.as<int32_t>(), P.eps.as<double>(),
+                     P.hdiag.as<double>());
+  MHIP_HIP_CHECK(hipGetLastError());
+  CoreArrays& A = P.A;
+  A.sa = P.sa.as<uint64_t>();
+  A.sb = P.sb.as<uint64_t>();
+  A.lists = P.lists.as<uint8_t>();
+  A.counts = P.counts.as<int32_t>();
+  A.eps = P.eps.as<double>();
+  A.hdiag = P.hdiag.as<double>();
+  A.coeff = d_C;
+  A.chunk_start = nullptr;
+  A.nc = nc;
+  A.n = n;
+  A.just_singles = just_singles;
+}
+
+// First block of every core det (bstart[nc] = number of blocks).
+inline std::vector<int64_t> block_starts(const std::vector<int>& na, const std::vector<int>& nb,
+                                         bool singles) {
+  std::vector<int64_t> bs(na.size() + 1, 0);
+  for(size_t c = 0; c < na.size(); ++c) bs[c + 1] = bs[c] + block_count(na[c], nb[c], singles);
+  return bs;
+}
 
 template <int W>
 int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndets_max,
@@ -615,24 +831,14 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   t_all.start();
   t.start();
   const IntsView I = device_view(ctx->ints);
-  const int n = (int)ctx->ints.norb;
+  const bool screened = mode.screened();
 
   // candidate counts per core det (closed form of the loop bounds)
-  std::vector<uint64_t> hc(nc * W);
-  d2h(ctx, hc.data(), d_cdets, 8 * W * nc);
+  std::vector<int> hna, hnb;
+  core_counts<W>(ctx, d_cdets, nc, norb, hna, hnb);
   std::vector<int64_t> ncand(nc), cstart(nc + 1, 0);
   for(int64_t c = 0; c < nc; ++c) {
-    uint64_t a, b;
-    if(W == 1) {
-      a = hc[c] & 0xffffffffull;
-      b = hc[c] >> 32;
-    } else {
-      a = hc[2 * c];
-      b = hc[2 * c + 1];
-    }
-    MHIP_REQUIRE(!((a | b) & (norb >= 64 ? 0ull : ~((1ull << norb) - 1ull))),
-                 MHIP_ERR_INVALID_ARG, "core determinant occupies an orbital >= norb");
-    const int64_t na = __builtin_popcountll(a), nb = __builtin_popcountll(b);
+    const int64_t na = hna[c], nb = hnb[c];
     const int64_t nva = norb - na, nvb = norb - nb;
     int64_t k = na * nva + nb * nvb;
     if(!s->just_singles)
@@ -641,51 +847,131 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     ncand[c] = k;
     cstart[c + 1] = cstart[c] + (k + kChunk - 1) / kChunk;
   }
-  const int64_t nchunks = cstart[nc];
+  const int64_t nchunks = screened ? 0 : cstart[nc];
 
-  DevBuf sa(8 * nc), sb(8 * nc), lists(4 * 64 * nc), counts(16 * nc), eps(16 * n * nc),
-      hdiag(8 * nc), dn(8 * nc), dstart(8 * (nc + 1));
+  CorePrep core;
+  prepare_core<W>(ctx, I, d_cdets, nc, norb, d_C, s->just_singles, core);
+  CoreArrays& A = core.A;
+  DevBuf dn(8 * nc), dstart(8 * (nc + 1));
   MHIP_HIP_CHECK(hipMemcpyAsync(dn.p, ncand.data(), 8 * nc, hipMemcpyHostToDevice, st));
   MHIP_HIP_CHECK(hipMemcpyAsync(dstart.p, cstart.data(), 8 * (nc + 1), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(prep_kernel<W>, dim3((nc + 63) / 64), dim3(64), 0, st, I, d_cdets, nc,
-                     (int)norb, sa.as<uint64_t>(), sb.as<uint64_t>(), lists.as<uint8_t>(),
-                     counts.as<int32_t>(), eps.as<double>(), hdiag.as<double>());
-  CoreArrays A;
-  A.sa = sa.as<uint64_t>();
-  A.sb = sb.as<uint64_t>();
-  A.lists = lists.as<uint8_t>();
-  A.counts = counts.as<int32_t>();
-  A.eps = eps.as<double>();
-  A.hdiag = hdiag.as<double>();
-  A.coeff = d_C;
   A.chunk_start = dstart.as<int64_t>();
-  A.nc = nc;
-  A.n = n;
-  A.just_singles = s->just_singles;
 
-  DevBuf ccount(8 * (nchunks + 1)), coff(8 * (nchunks + 1));
-  MHIP_HIP_CHECK(hipMemsetAsync(ccount.p, 0, 8 * (nchunks + 1), st));
-  const int64_t grid = (nchunks + 3) / 4;
-  MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many candidates");
-  hipLaunchKernelGGL((contrib_kernel<W, false>), dim3(grid), dim3(256), 0, st, I, A,
-                     dn.as<int64_t>(), nchunks, E0, s->h_el_tol, ccount.as<int64_t>(), nullptr,
-                     nullptr, nullptr);
-  MHIP_HIP_CHECK(hipGetLastError());
+  DevBuf ccount, coff, keys, vals;
   size_t tmp = 0;
-  MHIP_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp, ccount.as<int64_t>(), coff.as<int64_t>(),
-                                         (int64_t)0, (size_t)(nchunks + 1),
-                                         rocprim::plus<int64_t>(), st));
-  ctx->scratch_c.reserve(tmp + 16);
-  MHIP_HIP_CHECK(rocprim::exclusive_scan(ctx->scratch_c.p, tmp, ccount.as<int64_t>(),
-                                         coff.as<int64_t>(), (int64_t)0, (size_t)(nchunks + 1),
-                                         rocprim::plus<int64_t>(), st));
   int64_t P = 0;
-  d2h(ctx, &P, coff.as<int64_t>() + nchunks, 8);
-  DevBuf keys(sizeof(K) * std::max<int64_t>(P, 1)), vals(8 * std::max<int64_t>(P, 1));
-  hipLaunchKernelGGL((contrib_kernel<W, true>), dim3(grid), dim3(256), 0, st, I, A,
-                     dn.as<int64_t>(), nchunks, E0, s->h_el_tol, nullptr, coff.as<int64_t>(),
-                     keys.as<K>(), vals.as<double>());
-  MHIP_HIP_CHECK(hipGetLastError());
+  const int64_t grid = (nchunks + 3) / 4;
+  tm.asci_nn_ms = 0.;
+  tm.asci_nsurvivors = 0;
+  tm.asci_nblocks = tm.asci_nblocks_kept = 0;
+  if(!screened) {
+    ccount.alloc(8 * (nchunks + 1));
+    coff.alloc(8 * (nchunks + 1));
+    MHIP_HIP_CHECK(hipMemsetAsync(ccount.p, 0, 8 * (nchunks + 1), st));
+    MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many candidates");
+    hipLaunchKernelGGL((contrib_kernel<W, false>), dim3(grid), dim3(256), 0, st, I, A,
+                       dn.as<int64_t>(), nchunks, E0, s->h_el_tol, ccount.as<int64_t>(), nullptr,
+                       nullptr, nullptr);
+    MHIP_HIP_CHECK(hipGetLastError());
+    MHIP_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp, ccount.as<int64_t>(), coff.as<int64_t>(),
+                                           (int64_t)0, (size_t)(nchunks + 1),
+                                           rocprim::plus<int64_t>(), st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::exclusive_scan(ctx->scratch_c.p, tmp, ccount.as<int64_t>(),
+                                           coff.as<int64_t>(), (int64_t)0, (size_t)(nchunks + 1),
+                                           rocprim::plus<int64_t>(), st));
+    d2h(ctx, &P, coff.as<int64_t>() + nchunks, 8);
+    keys.alloc(sizeof(K) * std::max<int64_t>(P, 1));
+    vals.alloc(8 * std::max<int64_t>(P, 1));
+    hipLaunchKernelGGL((contrib_kernel<W, true>), dim3(grid), dim3(256), 0, st, I, A,
+                       dn.as<int64_t>(), nchunks, E0, s->h_el_tol, nullptr, coff.as<int64_t>(),
+                       keys.as<K>(), vals.as<double>());
+    MHIP_HIP_CHECK(hipGetLastError());
+  } else {
+    // ---- block screen: score every excitation block, keep the best
+    // keep_fraction of them (ties: lower block index first), generate only
+    // their candidates
+    Timer tn(st);
+    tn.start();
+    const std::vector<int64_t> bs = block_starts(hna, hnb, s->just_singles != 0);
+    const int64_t nblk = bs[nc];
+    DevBuf dbs(8 * (nc + 1));
+    MHIP_HIP_CHECK(hipMemcpyAsync(dbs.p, bs.data(), 8 * (nc + 1), hipMemcpyHostToDevice, st));
+    DevBuf bkey(4 * std::max<int64_t>(nblk, 1)), bidx(8 * std::max<int64_t>(nblk, 1));
+    const unsigned bgrid = (unsigned)((nblk + 255) / 256);
+    if(mode.mlp) {
+      DevBuf bdets(8 * W * std::max<int64_t>(nblk, 1)), pred(4 * std::max<int64_t>(nblk, 1));
+      if(nblk)
+        hipLaunchKernelGGL(block_dets_kernel<W>, dim3(bgrid), dim3(256), 0, st, A,
+                           dbs.as<int64_t>(), nblk, bdets.as<uint64_t>());
+      MHIP_HIP_CHECK(hipGetLastError());
+      const int prc =
+          mhip_mlp_predict(ctx, mode.mlp, W, bdets.as<uint64_t>(), nblk, pred.as<float>());
+      MHIP_REQUIRE(prc == MHIP_SUCCESS, prc, "block prediction failed");
+      if(nblk)
+        hipLaunchKernelGGL(block_key_kernel, dim3(bgrid), dim3(256), 0, st, pred.as<float>(),
+                           mode.pred_scale, mode.pred_shift, d_C, dbs.as<int64_t>(), nc, nullptr,
+                           nblk, bkey.as<uint32_t>(), bidx.as<int64_t>());
+    } else if(nblk) {
+      hipLaunchKernelGGL(block_key_kernel, dim3(bgrid), dim3(256), 0, st, nullptr, 1.f, 0.f,
+                         d_C, dbs.as<int64_t>(), nc, mode.block_scores, nblk,
+                         bkey.as<uint32_t>(), bidx.as<int64_t>());
+    }
+    MHIP_HIP_CHECK(hipGetLastError());
+    const int64_t nkept = std::min<int64_t>(
+        nblk, std::max<int64_t>(1, (int64_t)std::ceil(mode.keep_fraction * (double)nblk)));
+    DevBuf kept(8 * std::max<int64_t>(nkept, 1));
+    if(nblk) {
+      DevBuf bkey2(4 * nblk), bidx2(8 * nblk);
+      tmp = 0;
+      MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, bkey.as<uint32_t>(),
+                                                    bkey2.as<uint32_t>(), bidx.as<int64_t>(),
+                                                    bidx2.as<int64_t>(), (size_t)nblk, 0, 32, st));
+      ctx->scratch_c.reserve(tmp + 16);
+      MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, bkey.as<uint32_t>(),
+                                                    bkey2.as<uint32_t>(), bidx.as<int64_t>(),
+                                                    bidx2.as<int64_t>(), (size_t)nblk, 0, 32, st));
+      // kept ids back in ascending order: the fill then runs in core-det order
+      int bits = 1;
+      while(bits < 63 && (int64_t(1) << bits) < nblk) ++bits;
+      tmp = 0;
+      MHIP_HIP_CHECK(rocprim::radix_sort_keys(nullptr, tmp, bidx2.as<int64_t>(),
+                                              kept.as<int64_t>(), (size_t)nkept, 0, bits, st));
+      ctx->scratch_c.reserve(tmp + 16);
+      MHIP_HIP_CHECK(rocprim::radix_sort_keys(ctx->scratch_c.p, tmp, bidx2.as<int64_t>(),
+                                              kept.as<int64_t>(), (size_t)nkept, 0, bits, st));
+    }
+    tn.stop();
+    tm.asci_nn_ms = tn.ms();
+    const int64_t kgrid = (nkept + 3) / 4;
+    MHIP_REQUIRE(kgrid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many blocks");
+    ccount.alloc(8 * (nkept + 1));
+    coff.alloc(8 * (nkept + 1));
+    MHIP_HIP_CHECK(hipMemsetAsync(ccount.p, 0, 8 * (nkept + 1), st));
+    if(nblk)
+      hipLaunchKernelGGL((block_contrib_kernel<W, false>), dim3(kgrid), dim3(256), 0, st, I, A,
+                         dbs.as<int64_t>(), kept.as<int64_t>(), nkept, E0, s->h_el_tol,
+                         ccount.as<int64_t>(), nullptr, nullptr, nullptr);
+    MHIP_HIP_CHECK(hipGetLastError());
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp, ccount.as<int64_t>(), coff.as<int64_t>(),
+                                           (int64_t)0, (size_t)(nkept + 1),
+                                           rocprim::plus<int64_t>(), st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::exclusive_scan(ctx->scratch_c.p, tmp, ccount.as<int64_t>(),
+                                           coff.as<int64_t>(), (int64_t)0, (size_t)(nkept + 1),
+                                           rocprim::plus<int64_t>(), st));
+    d2h(ctx, &P, coff.as<int64_t>() + nkept, 8);
+    keys.alloc(sizeof(K) * std::max<int64_t>(P, 1));
+    vals.alloc(8 * std::max<int64_t>(P, 1));
+    if(nblk)
+      hipLaunchKernelGGL((block_contrib_kernel<W, true>), dim3(kgrid), dim3(256), 0, st, I, A,
+                         dbs.as<int64_t>(), kept.as<int64_t>(), nkept, E0, s->h_el_tol, nullptr,
+                         coff.as<int64_t>(), keys.as<K>(), vals.as<double>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    tm.asci_nblocks = nblk;
+    tm.asci_nblocks_kept = nkept;
+  }
   t.stop();
   tm.asci_contrib_ms = t.ms();
   tm.asci_npairs = P;
@@ -697,7 +983,7 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   DevBuf nu_dev(16);
   MHIP_HIP_CHECK(hipMemsetAsync(nu_dev.p, 0, 16, st));
   int64_t nu = 0;
-  if(P > s->pair_size_max && nc > 0) {
+  if(P > s->pair_size_max && nc > 0 && !screened) {
     // the reference's list would exceed pair_size_max: replay its prunes
     DevBuf rs(sizeof(RvSlot) * P), rs2(sizeof(RvSlot) * P), tiny(8 * nchunks);
     hipLaunchKernelGGL(prune_prep_kernel, dim3(grid), dim3(256), 0, st, nchunks,
@@ -821,8 +1107,8 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   MHIP_HIP_CHECK(rocprim::radix_sort_keys(ctx->scratch_c.p, tmp, core_keys.as<K>(),
                                           core_sorted.as<K>(), (size_t)nc, 0, 32 * W * 2, st));
   const int64_t top_k = ndets_max - nc;
-  if(!mode.exp_dets && !mode.mlp) {
-    // Plain search: exclusion and top-k straight on the accumulated arrays.
+  if(!mode.exp_dets) {
+    // Plain and block-screened search: exclusion and top-k straight on the accumulated arrays.
     // The k-th largest |score| is bracketed by two 12-bit digit histograms
     // (24-bit prefix); only candidates at or above that prefix (in
     // determinant order) are selected, and a stable descending sort of them
@@ -854,7 +1140,6 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
       prefix = (prefix << 12) | (uint64_t)d;
       in_bucket = hh[d];
     }
-    tm.asci_nn_ms = 0.;
     tm.asci_nsurvivors = eligible;
     int64_t nout = 0;
     DevBuf fl, cnt(16), sk, sv, ab, skeys, sabs;
@@ -928,147 +1213,23 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
                                    nu_dev.as<int64_t>() + 1, (size_t)nu, st));
     d2h(ctx, &nkeep, nu_dev.as<int64_t>() + 1, 8);
   }
-  tm.asci_nn_ms = 0.;
+  // export of the scored candidates
   tm.asci_nsurvivors = nkeep;
-  if(mode.exp_dets) {
-    MHIP_REQUIRE(nkeep <= mode.exp_cap, MHIP_ERR_INVALID_ARG,
-                 "candidate export capacity too small");
-    if(nkeep) {
-      hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nkeep + 255) / 256), dim3(256), 0, st,
-                         kkeys.as<K>(), nkeep, mode.exp_dets);
-      hipLaunchKernelGGL(bits_to_double_kernel, dim3((nkeep + 255) / 256), dim3(256), 0, st,
-                         kabs.as<uint64_t>(), nkeep, mode.exp_abs);
-    }
-    MHIP_HIP_CHECK(hipGetLastError());
-    MHIP_HIP_CHECK(hipStreamSynchronize(st));
-    *mode.exp_n = nkeep;
-    t.stop();
-    tm.asci_topk_ms = t.ms();
-    t_all.stop();
-    tm.asci_total_ms = t_all.ms();
-    return 0;
-  }
-  DevBuf pkeys, pidx, pkeys2, pidx2, sdets, pred;
-  if(mode.mlp && nkeep > top_k) {
-    // learned pre-screen: predict log|rv| of every candidate, keep the best
-    Timer tn(st);
-    tn.start();
-    const int64_t nsel = std::min<int64_t>(
-        nkeep, std::max<int64_t>(top_k, (int64_t)std::ceil(mode.keep_fraction * (double)nkeep)));
-    sdets.alloc(8 * W * nkeep);
-    pred.alloc(4 * nkeep);
+  MHIP_REQUIRE(nkeep <= mode.exp_cap, MHIP_ERR_INVALID_ARG, "candidate export capacity too small");
+  if(nkeep) {
     hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nkeep + 255) / 256), dim3(256), 0, st,
-                       kkeys.as<K>(), nkeep, sdets.as<uint64_t>());
-    MHIP_HIP_CHECK(hipGetLastError());
-    const int prc = mhip_mlp_predict(ctx, mode.mlp, W, sdets.as<uint64_t>(), nkeep, pred.as<float>());
-    MHIP_REQUIRE(prc == MHIP_SUCCESS, prc, "pre-screen prediction failed");
-    pkeys.alloc(4 * nkeep);
-    pidx.alloc(8 * nkeep);
-    pkeys2.alloc(4 * nkeep);
-    pidx2.alloc(8 * nkeep);
-    hipLaunchKernelGGL(pred_keys_kernel, dim3((nkeep + 255) / 256), dim3(256), 0, st,
-                       pred.as<float>(), nkeep, pkeys.as<uint32_t>(), pidx.as<int64_t>());
-    tmp = 0;
-    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, pkeys.as<uint32_t>(),
-                                                  pkeys2.as<uint32_t>(), pidx.as<int64_t>(),
-                                                  pidx2.as<int64_t>(), (size_t)nkeep, 0, 32, st));
-    ctx->scratch_c.reserve(tmp + 16);
-    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, pkeys.as<uint32_t>(),
-                                                  pkeys2.as<uint32_t>(), pidx.as<int64_t>(),
-                                                  pidx2.as<int64_t>(), (size_t)nkeep, 0, 32, st));
-    DevBuf k2(sizeof(K) * nsel), a2(8 * nsel);
-    hipLaunchKernelGGL((gather_kernel<K>), dim3((nsel + 255) / 256), dim3(256), 0, st,
-                       pidx2.as<int64_t>(), nsel, kkeys.as<K>(), kabs.as<uint64_t>(), k2.as<K>(),
-                       a2.as<uint64_t>());
-    MHIP_HIP_CHECK(hipGetLastError());
-    kkeys = std::move(k2);
-    kabs = std::move(a2);
-    nkeep = nsel;
-    tn.stop();
-    tm.asci_nn_ms = tn.ms();
-    tm.asci_nsurvivors = nsel;
+                       kkeys.as<K>(), nkeep, mode.exp_dets);
+    hipLaunchKernelGGL(bits_to_double_kernel, dim3((nkeep + 255) / 256), dim3(256), 0, st,
+                       kabs.as<uint64_t>(), nkeep, mode.exp_abs);
   }
-  int64_t nout = nkeep;
-  const K* outkeys = kkeys.as<K>();
-  DevBuf skeys, sabs, fkeys, fabsb, flg2, cnt2;
-  if(top_k <= 0) {
-    nout = 0;
-  } else if(nkeep > top_k) {
-    // Top-k by |rv| (stable descending: ties keep determinant order). The
-    // boundary 24-bit prefix of the k-th largest key is found from two 12-bit
-    // digit histograms; the candidates at or above it (in determinant order)
-    // are the only ones that can be in the top-k, and a stable descending sort
-    // of those gives the same first k as sorting everything.
-    const uint64_t* kb = kabs.as<uint64_t>();
-    DevBuf hist(sizeof(uint32_t) * 4096);
-    std::vector<uint32_t> hh(4096);
-    const unsigned hblocks = (unsigned)std::min<int64_t>(1024, (nkeep + 255) / 256);
-    uint64_t prefix = 0;
-    int64_t above = 0;  // keys strictly above the current prefix bucket
-    int64_t in_bucket = 0;  // keys in the boundary bucket
-    for(int pass = 0; pass < 2; ++pass) {
-      const int shift = pass == 0 ? 52 : 40;
-      MHIP_HIP_CHECK(hipMemsetAsync(hist.p, 0, sizeof(uint32_t) * 4096, st));
-      hipLaunchKernelGGL(digit_hist_kernel, dim3(hblocks), dim3(256), 0, st, kb, nkeep, shift,
-                         prefix, hist.as<uint32_t>());
-      MHIP_HIP_CHECK(hipGetLastError());
-      d2h(ctx, hh.data(), hist.p, sizeof(uint32_t) * 4096);
-      int d = 4095;
-      while(d > 0 && above + (int64_t)hh[d] < top_k) above += hh[d--];
-      prefix = (prefix << 12) | (uint64_t)d;
-      in_bucket = hh[d];
-    }
-    const int64_t nsel_expected = above + in_bucket;
-    flg2.alloc(nkeep);
-    hipLaunchKernelGGL(above_kernel, dim3(hblocks), dim3(256), 0, st, kb, nkeep, prefix,
-                       flg2.as<uint8_t>());
-    MHIP_HIP_CHECK(hipGetLastError());
-    cnt2.alloc(16);
-    tmp = 0;
-    MHIP_HIP_CHECK(rocprim::select(nullptr, tmp, kkeys.as<K>(), flg2.as<uint8_t>(),
-                                   kkeys.as<K>(), cnt2.as<int64_t>(), (size_t)nkeep, st));
-    // candidates at or above the boundary bucket
-    int64_t nsel = 0;
-    {
-      DevBuf ksel(sizeof(K) * std::max<int64_t>(nsel_expected, 1)),
-          asel(8 * std::max<int64_t>(nsel_expected, 1));
-      ctx->scratch_c.reserve(tmp + 16);
-      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, kkeys.as<K>(), flg2.as<uint8_t>(),
-                                     ksel.as<K>(), cnt2.as<int64_t>(), (size_t)nkeep, st));
-      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, kb, flg2.as<uint8_t>(),
-                                     asel.as<uint64_t>(), cnt2.as<int64_t>(), (size_t)nkeep,
-                                     st));
-      d2h(ctx, &nsel, cnt2.p, 8);
-      MHIP_REQUIRE(nsel == nsel_expected && nsel >= top_k, MHIP_ERR_HIP,
-                   "top-k selection count mismatch");
-      fkeys = std::move(ksel);
-      fabsb = std::move(asel);
-    }
-    skeys.alloc(sizeof(K) * nsel);
-    sabs.alloc(8 * nsel);
-    tmp = 0;
-    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, tmp, fabsb.as<uint64_t>(),
-                                                  sabs.as<uint64_t>(), fkeys.as<K>(),
-                                                  skeys.as<K>(), (size_t)nsel, 0, 64, st));
-    ctx->scratch_c.reserve(tmp + 16);
-    MHIP_HIP_CHECK(rocprim::radix_sort_pairs_desc(ctx->scratch_c.p, tmp, fabsb.as<uint64_t>(),
-                                                  sabs.as<uint64_t>(), fkeys.as<K>(),
-                                                  skeys.as<K>(), (size_t)nsel, 0, 64, st));
-    outkeys = skeys.as<K>();
-    nout = std::max<int64_t>(top_k, 0);
-  }
-  if(nout)
-    hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nout + 255) / 256), dim3(256), 0, st,
-                       outkeys, nout, d_out);
-  if(nc)
-    MHIP_HIP_CHECK(hipMemcpyAsync(d_out + W * nout, d_cdets, 8 * W * nc,
-                                  hipMemcpyDeviceToDevice, st));
   MHIP_HIP_CHECK(hipGetLastError());
+  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  *mode.exp_n = nkeep;
   t.stop();
   tm.asci_topk_ms = t.ms();
   t_all.stop();
   tm.asci_total_ms = t_all.ms();
-  return nout + nc;
+  return 0;
 }
 
 }  // namespace
@@ -1150,17 +1311,101 @@ extern "C" int mhip_asci_candidates(mhip_context* ctx, int words_per_det,
 
 extern "C" int mhip_asci_search_screened(mhip_context* ctx, int words_per_det,
                                          const mhip_asci_settings* settings, mhip_mlp* mlp,
-                                         double keep_fraction, int64_t ndets_max,
-                                         const uint64_t* d_cdets, int64_t ncdets, double E_ASCI,
-                                         const double* d_C, int64_t norb, uint64_t* d_new_dets,
-                                         int64_t* n_new) {
+                                         float pred_scale, float pred_shift, double keep_fraction,
+                                         int64_t ndets_max, const uint64_t* d_cdets,
+                                         int64_t ncdets, double E_ASCI, const double* d_C,
+                                         int64_t norb, uint64_t* d_new_dets, int64_t* n_new) {
   SearchMode m;
   m.mlp = mlp;
+  m.pred_scale = pred_scale;
+  m.pred_shift = pred_shift;
   m.keep_fraction = keep_fraction;
-  if(!mlp || !(keep_fraction > 0.) || !d_new_dets) {
-    set_last_error("mhip_asci_search_screened: need a network, keep_fraction > 0, output");
+  if(!mlp || !(keep_fraction > 0. && keep_fraction <= 1.) || !d_new_dets ||
+     !std::isfinite(pred_scale) || !std::isfinite(pred_shift)) {
+    set_last_error(
+        "mhip_asci_search_screened: need a network, finite scale/shift, 0 < keep_fraction <= 1, "
+        "an output");
     return MHIP_ERR_INVALID_ARG;
   }
   return asci_entry(ctx, words_per_det, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
                     d_new_dets, n_new, m);
+}
+
+extern "C" int mhip_asci_search_blocks(mhip_context* ctx, int words_per_det,
+                                       const mhip_asci_settings* settings,
+                                       const float* d_block_scores, double keep_fraction,
+                                       int64_t ndets_max, const uint64_t* d_cdets, int64_t ncdets,
+                                       double E_ASCI, const double* d_C, int64_t norb,
+                                       uint64_t* d_new_dets, int64_t* n_new) {
+  SearchMode m;
+  m.block_scores = d_block_scores;
+  m.keep_fraction = keep_fraction;
+  if(!d_block_scores || !(keep_fraction > 0. && keep_fraction <= 1.) || !d_new_dets) {
+    set_last_error("mhip_asci_search_blocks: need block scores, 0 < keep_fraction <= 1, an output");
+    return MHIP_ERR_INVALID_ARG;
+  }
+  return asci_entry(ctx, words_per_det, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
+                    d_new_dets, n_new, m);
+}
+
+namespace {
+template <int W>
+int64_t blocks_impl(mhip_context* ctx, const mhip_asci_settings* s, const uint64_t* d_cdets,
+                    int64_t nc, double E0, int64_t norb, uint64_t* d_dets, float* d_target,
+                    int64_t capacity, int64_t* n_out) {
+  hipStream_t st = ctx->stream;
+  const IntsView I = device_view(ctx->ints);
+  std::vector<int> hna, hnb;
+  core_counts<W>(ctx, d_cdets, nc, norb, hna, hnb);
+  const std::vector<int64_t> bs = block_starts(hna, hnb, s->just_singles != 0);
+  const int64_t nblk = bs[nc];
+  *n_out = nblk;  // reported also when the capacity check below fails
+  if(!d_dets && !d_target) return nblk;
+  MHIP_REQUIRE(nblk <= capacity, MHIP_ERR_INVALID_ARG, "block export capacity too small");
+  if(!nblk) return 0;
+  DevBuf ones(8 * nc);
+  hipLaunchKernelGGL(fill_ones_kernel, dim3((nc + 255) / 256), dim3(256), 0, st,
+                     ones.as<double>(), nc);
+  CorePrep core;
+  prepare_core<W>(ctx, I, d_cdets, nc, norb, ones.as<double>(), s->just_singles, core);
+  DevBuf dbs(8 * (nc + 1));
+  MHIP_HIP_CHECK(hipMemcpyAsync(dbs.p, bs.data(), 8 * (nc + 1), hipMemcpyHostToDevice, st));
+  if(d_dets)
+    hipLaunchKernelGGL(block_dets_kernel<W>, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0,
+                       st, core.A, dbs.as<int64_t>(), nblk, d_dets);
+  const int64_t grid = (nblk + 3) / 4;
+  MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many blocks");
+  if(d_target)
+    hipLaunchKernelGGL(block_target_kernel, dim3((unsigned)grid), dim3(256), 0, st, I, core.A,
+                       dbs.as<int64_t>(), nblk, E0, s->h_el_tol, d_target);
+  MHIP_HIP_CHECK(hipGetLastError());
+  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  return nblk;
+}
+}  // namespace
+
+extern "C" int mhip_asci_blocks(mhip_context* ctx, int words_per_det,
+                                const mhip_asci_settings* settings, const uint64_t* d_cdets,
+                                int64_t ncdets, double E_ASCI, int64_t norb,
+                                uint64_t* d_block_dets, float* d_target, int64_t capacity,
+                                int64_t* n_blocks) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(settings && n_blocks, MHIP_ERR_INVALID_ARG, "null argument");
+    MHIP_REQUIRE(words_per_det == 1 || words_per_det == 2, MHIP_ERR_UNSUPPORTED,
+                 "words_per_det must be 1 or 2");
+    MHIP_REQUIRE(ctx->ints.norb > 0, MHIP_ERR_NO_INTEGRALS, "no integrals set");
+    MHIP_REQUIRE(norb > 0 && norb <= ctx->ints.norb && norb <= 32 * words_per_det,
+                 MHIP_ERR_INVALID_ARG, "bad norb");
+    MHIP_REQUIRE(ncdets >= 0 && (ncdets == 0 || d_cdets), MHIP_ERR_INVALID_ARG,
+                 "bad core determinants");
+    *n_blocks = 0;
+    if(!ncdets) return;
+    if(words_per_det == 1)
+      *n_blocks = blocks_impl<1>(ctx, settings, d_cdets, ncdets, E_ASCI, norb, d_block_dets,
+                                 d_target, capacity, n_blocks);
+    else
+      *n_blocks = blocks_impl<2>(ctx, settings, d_cdets, ncdets, E_ASCI, norb, d_block_dets,
+                                 d_target, capacity, n_blocks);
+  });
 }

@@ -2,8 +2,10 @@
 // the C-ABI (include/macis_hip.h).
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mhip_internal.hpp"
@@ -32,6 +34,99 @@ void init_noalloc_scratch(int device) {
   g_scratch_addr[device] = p;
 }
 }  // namespace
+
+namespace {
+constexpr size_t kCacheMinBytes = size_t(64) << 20;
+struct BlockCache {
+  std::mutex mu;
+  std::multimap<size_t, void*> idle[kMaxDevices];  // size -> block, per device
+  struct Live {
+    size_t bytes;
+    int device;
+  };
+  std::unordered_map<void*, Live> live;  // cacheable blocks in use
+};
+// Never destroyed: blocks still cached at process exit go back with the
+// process, not through hipFree calls from static destructors.
+BlockCache& block_cache() {
+  static BlockCache* c = new BlockCache;
+  return *c;
+}
+int current_device() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  return d;
+}
+}  // namespace
+
+void* device_alloc(size_t bytes) {
+  if(!bytes) return nullptr;
+  void* p = nullptr;
+  if(bytes < kCacheMinBytes) {
+    MHIP_HIP_CHECK(hipMalloc(&p, bytes));
+    return p;
+  }
+  BlockCache& c = block_cache();
+  const int dev = current_device();
+  MHIP_REQUIRE(dev >= 0 && dev < kMaxDevices, MHIP_ERR_INVALID_ARG, "device index out of range");
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    auto& idle = c.idle[dev];
+    auto it = idle.lower_bound(bytes);
+    if(it != idle.end() && it->first <= bytes + bytes / 8) {
+      p = it->second;
+      c.live[p] = {it->first, dev};
+      idle.erase(it);
+      return p;
+    }
+  }
+  hipError_t e = hipMalloc(&p, bytes);
+  if(e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();  // not a pending error: retried after the trim
+    device_cache_trim(dev);
+    e = hipMalloc(&p, bytes);
+  }
+  MHIP_HIP_CHECK(e);
+  std::lock_guard<std::mutex> lock(c.mu);
+  c.live[p] = {bytes, dev};
+  return p;
+}
+
+void device_free(void* p) noexcept {
+  if(!p) return;
+  BlockCache& c = block_cache();
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    auto it = c.live.find(p);
+    if(it != c.live.end()) {
+      const BlockCache::Live l = it->second;
+      c.live.erase(it);
+      const int cur = current_device();
+      if(cur != l.device) (void)hipSetDevice(l.device);
+      (void)hipDeviceSynchronize();  // no kernel may still use the block
+      if(cur != l.device) (void)hipSetDevice(cur);
+      c.idle[l.device].emplace(l.bytes, p);
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+void device_cache_trim(int device) noexcept {
+  if(device < 0 || device >= kMaxDevices) return;
+  std::vector<void*> blocks;
+  {
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    for(auto& kv : c.idle[device]) blocks.push_back(kv.second);
+    c.idle[device].clear();
+  }
+  if(blocks.empty()) return;
+  const int cur = current_device();
+  if(cur != device) (void)hipSetDevice(device);
+  for(void* b : blocks) (void)hipFree(b);
+  if(cur != device) (void)hipSetDevice(cur);
+}
 
 void* noalloc_scratch(int device) {
   if(device < 0 || device >= kMaxDevices) return nullptr;
@@ -178,7 +273,9 @@ int mhip_context_destroy(mhip_context* ctx) {
       (void)hipStreamSynchronize(ctx->stream);
       (void)hipStreamDestroy(ctx->stream);
     }
+    const int device = ctx->device;
     delete ctx;
+    device_cache_trim(device);
   });
 }
 
@@ -225,7 +322,7 @@ int mhip_malloc(mhip_context* ctx, size_t bytes, void** dptr) {
     MHIP_REQUIRE(dptr, MHIP_ERR_INVALID_ARG, "null output pointer");
     use_device(ctx);
     *dptr = nullptr;
-    if(bytes) MHIP_HIP_CHECK(hipMalloc(dptr, bytes));
+    if(bytes) *dptr = device_alloc(bytes);
   });
 }
 
@@ -234,7 +331,7 @@ int mhip_free(mhip_context* ctx, void* dptr) {
     use_device(ctx);
     if(dptr) {
       MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      MHIP_HIP_CHECK(hipFree(dptr));
+      device_free(dptr);
     }
   });
 }

@@ -1,10 +1,17 @@
-// Learned ASCI pre-screen (no counterpart in the reference): an MLP on the
-// occupation bits predicts log|rv| of asci_search candidates; the screened
-// search keeps the best-predicted fraction and lets the exact scores pick
-// the top-k. Training pairs come from plain searches (observe), training
-// runs on the device (BF16 MFMA kernels), and the plain asci_search is the
-// fallback whenever the network is untrained or its recall on the last
-// observed search is below `min_recall`.
+// Learned ASCI block screen (no counterpart in the reference). The
+// candidates of a core determinant fall into excitation blocks, one per hole
+// set (mhip_asci_blocks); an MLP on the block's determinant (the core
+// determinant with the holes cleared) predicts the log of the block's
+// largest |h / (E - H_kk)|, and the screened search generates only the
+// keep_fraction blocks with the best predicted log|C_core| + prediction.
+// Their exact contributions are accumulated and ranked as in asci_search.
+//
+// Training pairs are the block targets of the cores of earlier steps
+// (observe); training runs on the device (BF16 MFMA kernels). A network is
+// trusted only after validate() has compared its screened search with the
+// plain asci_search on a core and found a recall of at least min_recall;
+// until then, and whenever that check fails, search() is the plain
+// asci_search.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -32,53 +39,57 @@ class LearnedPrescreen {
   LearnedPrescreen(const LearnedPrescreen&) = delete;
   LearnedPrescreen& operator=(const LearnedPrescreen&) = delete;
 
-  // Scores all candidates of the core with the plain search and keeps them
-  // (determinant, log|rv|) as training data; returns the number added.
+  // Keeps the (block determinant, target) pairs of the core as training
+  // data (blocks without a screened-in candidate are skipped); returns the
+  // number added.
   size_t observe(const ASCISettings& s, wavefunction_iterator_t<N> cbeg,
-                 wavefunction_iterator_t<N> cend, double E0, const std::vector<double>& C,
-                 size_t norb, size_t capacity) {
+                 wavefunction_iterator_t<N> cend, double E0, size_t norb) {
     auto& ctx = gen_.context();
     const size_t nc = std::distance(cbeg, cend);
+    if(!nc) return 0;
     auto dc = hip::upload_dets<N>(ctx, cbeg, cend);
-    hip::DeviceBuffer<double> dC(ctx, C.data(), std::max<size_t>(nc, 1));
-    hip::DeviceBuffer<uint64_t> dd(ctx, std::max<size_t>(capacity, 1) * W);
-    hip::DeviceBuffer<double> da(ctx, std::max<size_t>(capacity, 1));
     const mhip_asci_settings ms = to_c(s);
-    int64_t n = 0;
-    hip::check(mhip_asci_candidates(ctx.get(), W, &ms, dc.data(), nc, E0, dC.data(), norb,
-                                    dd.data(), da.data(), capacity, &n));
-    std::vector<uint64_t> w(n * W);
-    std::vector<double> a(n);
-    if(n) {
-      dd.download(w.data(), n * W);
-      da.download(a.data(), n);
+    int64_t nb = 0;
+    hip::check(mhip_asci_blocks(ctx.get(), W, &ms, dc.data(), nc, E0, norb, nullptr, nullptr, 0,
+                                &nb));
+    hip::DeviceBuffer<uint64_t> dd(ctx, std::max<int64_t>(nb, 1) * W);
+    hip::DeviceBuffer<float> dt(ctx, std::max<int64_t>(nb, 1));
+    hip::check(mhip_asci_blocks(ctx.get(), W, &ms, dc.data(), nc, E0, norb, dd.data(), dt.data(),
+                                nb, &nb));
+    std::vector<uint64_t> w(nb * W);
+    std::vector<float> t(nb);
+    if(nb) {
+      dd.download(w.data(), nb * W);
+      dt.download(t.data(), nb);
     }
-    dets_.insert(dets_.end(), w.begin(), w.end());
-    last_dets_ = std::move(w);
-    last_logrv_.resize(n);
-    for(int64_t i = 0; i < n; ++i) {
-      last_logrv_[i] = std::log(a[i]);
-      target_.push_back((float)last_logrv_[i]);
+    size_t added = 0;
+    for(int64_t i = 0; i < nb; ++i) {
+      if(!std::isfinite(t[i])) continue;
+      dets_.insert(dets_.end(), w.begin() + i * W, w.begin() + (i + 1) * W);
+      target_.push_back(t[i]);
+      ++added;
     }
-    return n;
+    return added;
   }
 
   // Adam epochs over the observed pairs; returns the last mini-batch loss.
-  // Targets below the clip_quantile quantile are raised to it (the ranking
-  // among candidates that cannot enter the top-k does not matter), then
-  // standardised.
+  // Targets below the clip_quantile quantile are raised to it, then
+  // standardised; the screen undoes the standardisation (scale, shift).
   double train(int epochs, int64_t batch, float lr, uint64_t seed = 1,
-               double clip_quantile = 0.7) {
+               double clip_quantile = 0.0) {
     const size_t n = target_.size();
     if(!n) return 0.;
-    std::vector<float> sorted(target_);
-    const size_t qi = std::min(n - 1, (size_t)(clip_quantile * (double)(n - 1)));
-    std::nth_element(sorted.begin(), sorted.begin() + qi, sorted.end());
-    clip_ = sorted[qi];
+    float clip = -INFINITY;
+    if(clip_quantile > 0.) {
+      std::vector<float> sorted(target_);
+      const size_t qi = std::min(n - 1, (size_t)(clip_quantile * (double)(n - 1)));
+      std::nth_element(sorted.begin(), sorted.begin() + qi, sorted.end());
+      clip = sorted[qi];
+    }
     double mu = 0., var = 0.;
-    for(float t : target_) mu += std::max(t, clip_);
+    for(float t : target_) mu += std::max(t, clip);
     mu /= n;
-    for(float t : target_) var += (std::max(t, clip_) - mu) * (std::max(t, clip_) - mu);
+    for(float t : target_) var += (std::max(t, clip) - mu) * (std::max(t, clip) - mu);
     mu_ = mu;
     sd_ = std::sqrt(var / n) + 1e-12;
     std::vector<uint32_t> perm(n);
@@ -98,7 +109,7 @@ class LearnedPrescreen {
       for(size_t i = 0; i < n; ++i) {
         const size_t p = perm[i];
         std::copy_n(dets_.begin() + p * W, W, hd.begin() + i * W);
-        ht[i] = (float)((std::max(target_[p], clip_) - mu_) / sd_);
+        ht[i] = (float)((std::max(target_[p], clip) - mu_) / sd_);
       }
       bd.upload(hd.data(), n * W);
       bt.upload(ht.data(), n);
@@ -109,40 +120,33 @@ class LearnedPrescreen {
                                        e + 1 == epochs && i + 1 == nsteps ? &loss : nullptr));
     }
     trained_ = true;
-    recall_ = validate();
+    recall_ = 0.;  // a retrained network is judged again
     return loss;
   }
 
-  // Recall of the top-k (k = the last search's size minus the core) among
-  // the best-predicted keep_fraction of the last observed candidates.
-  double validate(size_t k = 0) {
-    const size_t n = last_logrv_.size();
-    if(!n) return 0.;
-    if(!k) k = std::max<size_t>(1, (size_t)(0.01 * n));
-    auto& ctx = gen_.context();
-    hip::DeviceBuffer<uint64_t> dd(ctx, last_dets_.data(), n * W);
-    hip::DeviceBuffer<float> dp(ctx, n);
-    hip::check(mhip_mlp_predict(ctx.get(), mlp_, W, dd.data(), n, dp.data()));
-    std::vector<float> pred(n);
-    dp.download(pred.data(), n);
-    std::vector<uint32_t> byrv(n), bypred(n);
-    std::iota(byrv.begin(), byrv.end(), 0);
-    std::iota(bypred.begin(), bypred.end(), 0);
-    k = std::min(k, n);
-    std::partial_sort(byrv.begin(), byrv.begin() + k, byrv.end(),
-                      [&](uint32_t a, uint32_t b) { return last_logrv_[a] > last_logrv_[b]; });
-    const size_t keep = std::min(n, std::max(k, (size_t)std::ceil(keep_ * n)));
-    std::nth_element(bypred.begin(), bypred.begin() + keep - 1, bypred.end(),
-                     [&](uint32_t a, uint32_t b) { return pred[a] > pred[b]; });
-    std::vector<char> kept(n, 0);
-    for(size_t i = 0; i < keep; ++i) kept[bypred[i]] = 1;
-    size_t hit = 0;
-    for(size_t i = 0; i < k; ++i) hit += kept[byrv[i]];
-    return (double)hit / k;
+  // Runs the screened and the plain search on one core and returns the
+  // fraction of the plain search's new determinants that the screened one
+  // also selects; the network is trusted when it is >= min_recall.
+  double validate(const ASCISettings& s, size_t ndets_max, wavefunction_iterator_t<N> cbeg,
+                  wavefunction_iterator_t<N> cend, double E0, const std::vector<double>& C,
+                  size_t norb) {
+    if(!trained_) return 0.;
+    const size_t nc = std::distance(cbeg, cend);
+    auto plain = asci_search(s, ndets_max, cbeg, cend, E0, C, norb, gen_.T(), gen_.G_red(),
+                             gen_.V_red(), gen_.G(), gen_.V(), gen_);
+    auto scr = screened(s, ndets_max, cbeg, cend, E0, C, norb);
+    const size_t np = plain.size() - nc;
+    std::vector<wfn_t<N>> a(plain.begin(), plain.begin() + np), b(scr.begin(), scr.end() - nc);
+    auto less = bitset_less_comparator<N>{};
+    std::sort(a.begin(), a.end(), less);
+    std::sort(b.begin(), b.end(), less);
+    std::vector<wfn_t<N>> both;
+    std::set_intersection(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(both), less);
+    recall_ = np ? (double)both.size() / np : 1.;
+    return recall_;
   }
 
-  // Screened search, or the plain asci_search when the network is not
-  // trusted (untrained, or validation recall below min_recall).
+  // Screened search when the network is trusted, else the plain asci_search.
   std::vector<wfn_t<N>> search(const ASCISettings& s, size_t ndets_max,
                                wavefunction_iterator_t<N> cbeg, wavefunction_iterator_t<N> cend,
                                double E0, const std::vector<double>& C, size_t norb) {
@@ -150,6 +154,23 @@ class LearnedPrescreen {
     if(used_fallback_)
       return asci_search(s, ndets_max, cbeg, cend, E0, C, norb, gen_.T(), gen_.G_red(),
                          gen_.V_red(), gen_.G(), gen_.V(), gen_);
+    return screened(s, ndets_max, cbeg, cend, E0, C, norb);
+  }
+
+  bool used_fallback() const { return used_fallback_; }
+  double recall() const { return recall_; }
+  double keep_fraction() const { return keep_; }
+  void set_keep_fraction(double k) {
+    keep_ = k;
+    recall_ = 0.;  // judged again at the new fraction
+  }
+  mhip_mlp* network() const { return mlp_; }
+  size_t num_pairs() const { return target_.size(); }
+
+ private:
+  std::vector<wfn_t<N>> screened(const ASCISettings& s, size_t ndets_max,
+                                 wavefunction_iterator_t<N> cbeg, wavefunction_iterator_t<N> cend,
+                                 double E0, const std::vector<double>& C, size_t norb) {
     auto& ctx = gen_.context();
     const size_t nc = std::distance(cbeg, cend);
     auto dc = hip::upload_dets<N>(ctx, cbeg, cend);
@@ -157,16 +178,11 @@ class LearnedPrescreen {
     hip::DeviceBuffer<uint64_t> out(ctx, std::max<size_t>(ndets_max, 1) * W);
     const mhip_asci_settings ms = to_c(s);
     int64_t nnew = 0;
-    hip::check(mhip_asci_search_screened(ctx.get(), W, &ms, mlp_, keep_, ndets_max, dc.data(), nc,
-                                         E0, dC.data(), norb, out.data(), &nnew));
+    hip::check(mhip_asci_search_screened(ctx.get(), W, &ms, mlp_, (float)sd_, (float)mu_, keep_,
+                                         ndets_max, dc.data(), nc, E0, dC.data(), norb,
+                                         out.data(), &nnew));
     return hip::download_dets<N>(out, nnew);
   }
-
-  bool used_fallback() const { return used_fallback_; }
-  double recall() const { return recall_; }
-  mhip_mlp* network() const { return mlp_; }
-
- private:
   static mhip_asci_settings to_c(const ASCISettings& s) {
     mhip_asci_settings m;
     m.ncdets_max = s.ncdets_max;
@@ -181,10 +197,7 @@ class LearnedPrescreen {
   double keep_, min_recall_;
   std::vector<uint64_t> dets_;
   std::vector<float> target_;
-  std::vector<uint64_t> last_dets_;
-  std::vector<double> last_logrv_;
   double mu_ = 0., sd_ = 1., recall_ = 0.;
-  float clip_ = 0.f;
   bool trained_ = false, used_fallback_ = true;
 };
 

@@ -1171,121 +1171,144 @@ static void keep_largest(pair_buf* b) {
   b->n = w + 1;
 }
 
+/* Per-core-det data of the candidate formulas. */
+typedef struct {
+  det_t s;
+  uint32_t oa[64], va[64], ob[64], vb[64];
+  int na, nva, nb, nvb;
+  double *eps_a, *eps_b;
+  double h_diag;
+} core_t;
+
+static void core_load(const orc_ham* h, int W, const uint64_t* det, int64_t norb, core_t* k) {
+  k->s = load_det(W, det);
+  occ_vir(norb, k->s.a, k->oa, &k->na, k->va, &k->nva);
+  occ_vir(norb, k->s.b, k->ob, &k->nb, k->vb, &k->nvb);
+  orc_single_orbital_ens(h, norb, k->oa, k->na, k->ob, k->nb, k->eps_a);
+  orc_single_orbital_ens(h, norb, k->ob, k->nb, k->oa, k->na, k->eps_b);
+  k->h_diag = me_diag(h, k->oa, k->na, k->ob, k->nb);
+}
+
+/* One single i -> a of spin `spin` (append_singles_asci_contributions,
+ * determinant_contributions.hpp:25-66): returns 0 when screened out by
+ * |h| < tol (before the sign, :47), else the excited det and
+ * val = sign * h / (E0 - H_kk^fast). */
+static int cand_single(const orc_ham* h, const core_t* k, int spin, uint32_t i, uint32_t a,
+                       double E0, double tol, det_t* ex, double* val) {
+  const int64_t n = h->n, n2 = n * n;
+  const uint64_t ss = spin ? k->s.b : k->s.a;
+  const uint32_t* occ = spin ? k->ob : k->oa;
+  const uint32_t* oth = spin ? k->oa : k->ob;
+  const int no = spin ? k->nb : k->na, nt = spin ? k->na : k->nb;
+  const double* eps = spin ? k->eps_b : k->eps_a;
+  double h_el = h->T[a + i * n];
+  const double* G_ov = h->G_red + a * n + i * n2;
+  const double* V_ov = h->V_red + a * n + i * n2;
+  for(int p = 0; p < no; ++p) h_el += G_ov[occ[p]];
+  for(int p = 0; p < nt; ++p) h_el += V_ov[oth[p]];
+  if(fabs(h_el) < tol) return 0;
+  *ex = k->s;
+  if(spin)
+    ex->b ^= (1ULL << i) | (1ULL << a);
+  else
+    ex->a ^= (1ULL << i) | (1ULL << a);
+  h_el *= orc_single_excitation_sign(ss, a, i);
+  const double hd = orc_fast_diag_single(h, eps[i], eps[a], i, a, k->h_diag);
+  *val = h_el / (E0 - hd);
+  return 1;
+}
+
+/* One same-spin double (i, j) -> (a, b) (append_ss_doubles_asci_contributions,
+ * :68-120). */
+static int cand_ss(const orc_ham* h, const core_t* k, int spin, uint32_t i, uint32_t j,
+                   uint32_t a, uint32_t b, double E0, double tol, det_t* ex, double* val) {
+  const int64_t n = h->n, n2 = n * n;
+  const uint64_t ss = spin ? k->s.b : k->s.a;
+  const double* eps = spin ? k->eps_b : k->eps_a;
+  const double G_aibj = h->G[(a + i * n) * n2 + b + j * n];
+  if(fabs(G_aibj) < tol) return 0;
+  const uint64_t fex = (1ULL << i) | (1ULL << j) | (1ULL << a) | (1ULL << b);
+  uint32_t o1, v1, o2, v2;
+  const double sign = orc_doubles_sign_indices(ss, ss ^ fex, fex, &o1, &v1, &o2, &v2);
+  *ex = k->s;
+  if(spin)
+    ex->b ^= fex;
+  else
+    ex->a ^= fex;
+  const double h_el = sign * G_aibj;
+  const double hd =
+      orc_fast_diag_ss_double(h, eps[i], eps[j], eps[a], eps[b], i, j, a, b, k->h_diag);
+  *val = h_el / (E0 - hd);
+  return 1;
+}
+
+/* One opposite-spin double alpha i -> a, beta j -> b
+ * (append_os_doubles_asci_contributions, :122-160). */
+static int cand_os(const orc_ham* h, const core_t* k, uint32_t i, uint32_t a, uint32_t j,
+                   uint32_t b, double E0, double tol, det_t* ex, double* val) {
+  const int64_t n = h->n, n2 = n * n;
+  const double V_aibj = h->V[a + i * n + (b + j * n) * n2];
+  if(fabs(V_aibj) < tol) return 0;
+  const double sign =
+      orc_single_excitation_sign(k->s.a, a, i) * orc_single_excitation_sign(k->s.b, b, j);
+  *ex = k->s;
+  ex->a ^= (1ULL << a) | (1ULL << i);
+  ex->b ^= (1ULL << j) | (1ULL << b);
+  const double h_el = sign * V_aibj;
+  const double hd = orc_fast_diag_os_double(h, k->eps_a[i], k->eps_b[j], k->eps_a[a],
+                                            k->eps_b[b], i, j, a, b, k->h_diag);
+  *val = h_el / (E0 - hd);
+  return 1;
+}
+
 /* asci_contributions_standard (determinant_search.hpp:53-141) with the
- * append_* kernels of determinant_contributions.hpp:25-160. When `prune`
- * is set, applies the pair_size_max pruning of :120-137. */
+ * append_* kernels of determinant_contributions.hpp:25-160, in their loop
+ * order. When `prune` is set, applies the pair_size_max pruning of
+ * :120-137. */
 static int asci_contribs(const orc_ham* h, int W, const orc_asci_settings* st,
                          const uint64_t* cdets, int64_t ncdets, double E0,
                          const double* C, int64_t norb, pair_buf* out,
                          int prune) {
-  const int64_t n = h->n, n2 = n * n;
-  const int NB = spin_bits(W);
   const double tol = st->h_el_tol;
-  uint32_t oa[64], va[64], ob[64], vb[64];
-  int na, nva, nb, nvb;
-  double *eps_a = (double*)malloc(sizeof(double) * norb),
-         *eps_b = (double*)malloc(sizeof(double) * norb);
+  core_t k;
+  k.eps_a = (double*)malloc(sizeof(double) * norb);
+  k.eps_b = (double*)malloc(sizeof(double) * norb);
+  det_t ex;
+  double v;
   for(int64_t c = 0; c < ncdets; ++c) {
-    det_t s = load_det(W, cdets + W * c);
+    core_load(h, W, cdets + W * c, norb, &k);
     const double coeff = C[c];
-    occ_vir(norb, s.a, oa, &na, va, &nva);
-    occ_vir(norb, s.b, ob, &nb, vb, &nvb);
-    orc_single_orbital_ens(h, norb, oa, na, ob, nb, eps_a);
-    orc_single_orbital_ens(h, norb, ob, nb, oa, na, eps_b);
-    const double h_diag = me_diag(h, oa, na, ob, nb);
-    /* singles AA then BB (append_singles_asci_contributions :25-66) */
+    /* singles AA then BB */
     for(int spin = 0; spin < 2; ++spin) {
-      const uint64_t ss = spin ? s.b : s.a;
-      const uint32_t* occ = spin ? ob : oa;
-      const uint32_t* vir = spin ? vb : va;
-      const uint32_t* oth = spin ? oa : ob;
-      const int no = spin ? nb : na, nv = spin ? nvb : nva,
-                nt = spin ? na : nb;
-      const double* eps = spin ? eps_b : eps_a;
+      const uint32_t* occ = spin ? k.ob : k.oa;
+      const uint32_t* vir = spin ? k.vb : k.va;
+      const int no = spin ? k.nb : k.na, nv = spin ? k.nvb : k.nva;
       for(int ii = 0; ii < no; ++ii)
-        for(int aa = 0; aa < nv; ++aa) {
-          const uint32_t i = occ[ii], a = vir[aa];
-          double h_el = h->T[a + i * n];
-          const double* G_ov = h->G_red + a * n + i * n2;
-          const double* V_ov = h->V_red + a * n + i * n2;
-          for(int p = 0; p < no; ++p) h_el += G_ov[occ[p]];
-          for(int p = 0; p < nt; ++p) h_el += V_ov[oth[p]];
-          if(fabs(h_el) < tol) continue;
-          det_t ex = s;
-          if(spin)
-            ex.b ^= (1ULL << i) | (1ULL << a);
-          else
-            ex.a ^= (1ULL << i) | (1ULL << a);
-          const double sign = orc_single_excitation_sign(ss, a, i);
-          h_el *= sign;
-          const double hd =
-              orc_fast_diag_single(h, eps[i], eps[a], i, a, h_diag);
-          h_el /= (E0 - hd);
-          if(pb_push(out, ex, coeff * h_el)) goto nomem;
-        }
+        for(int aa = 0; aa < nv; ++aa)
+          if(cand_single(h, &k, spin, occ[ii], vir[aa], E0, tol, &ex, &v))
+            if(pb_push(out, ex, coeff * v)) goto nomem;
     }
     if(!st->just_singles) {
-      /* same-spin doubles AAAA then BBBB (:68-120) */
+      /* same-spin doubles AAAA then BBBB */
       for(int spin = 0; spin < 2; ++spin) {
-        const uint64_t ss = spin ? s.b : s.a;
-        const uint32_t* occ = spin ? ob : oa;
-        const uint32_t* vir = spin ? vb : va;
-        const int no = spin ? nb : na, nv = spin ? nvb : nva;
-        const double* eps = spin ? eps_b : eps_a;
+        const uint32_t* occ = spin ? k.ob : k.oa;
+        const uint32_t* vir = spin ? k.vb : k.va;
+        const int no = spin ? k.nb : k.na, nv = spin ? k.nvb : k.nva;
         for(int ii = 0; ii < no; ++ii)
-          for(int aa = 0; aa < nv; ++aa) {
-            const uint32_t i = occ[ii], a = vir[aa];
-            const double* G_ai = h->G + (a + i * n) * n2;
+          for(int aa = 0; aa < nv; ++aa)
             for(int jj = ii + 1; jj < no; ++jj)
-              for(int bb = aa + 1; bb < nv; ++bb) {
-                const uint32_t j = occ[jj], b = vir[bb];
-                const double G_aibj = G_ai[b + j * n];
-                if(fabs(G_aibj) < tol) continue;
-                const uint64_t fex =
-                    (1ULL << i) | (1ULL << j) | (1ULL << a) | (1ULL << b);
-                const uint64_t exs = ss ^ fex;
-                uint32_t o1, v1, o2, v2;
-                const double sign =
-                    orc_doubles_sign_indices(ss, exs, fex, &o1, &v1, &o2, &v2);
-                det_t ex = s;
-                if(spin)
-                  ex.b ^= fex;
-                else
-                  ex.a ^= fex;
-                double h_el = sign * G_aibj;
-                const double hd = orc_fast_diag_ss_double(
-                    h, eps[i], eps[j], eps[a], eps[b], i, j, a, b, h_diag);
-                h_el /= (E0 - hd);
-                if(pb_push(out, ex, coeff * h_el)) goto nomem;
-              }
-          }
+              for(int bb = aa + 1; bb < nv; ++bb)
+                if(cand_ss(h, &k, spin, occ[ii], occ[jj], vir[aa], vir[bb], E0, tol, &ex, &v))
+                  if(pb_push(out, ex, coeff * v)) goto nomem;
       }
-      /* opposite-spin doubles AABB (:122-160) */
-      for(int ii = 0; ii < na; ++ii)
-        for(int aa = 0; aa < nva; ++aa) {
-          const uint32_t i = oa[ii], a = va[aa];
-          const double* V_ai = h->V + a + i * n;
-          const double sa = orc_single_excitation_sign(s.a, a, i);
-          for(int jj = 0; jj < nb; ++jj)
-            for(int bb = 0; bb < nvb; ++bb) {
-              const uint32_t j = ob[jj], b = vb[bb];
-              const double V_aibj = V_ai[(b + j * n) * n2];
-              if(fabs(V_aibj) < tol) continue;
-              const double sb = orc_single_excitation_sign(s.b, b, j);
-              const double sign = sa * sb;
-              det_t ex = s;
-              ex.a ^= (1ULL << a) | (1ULL << i);
-              ex.b ^= (1ULL << j) | (1ULL << b);
-              double h_el = sign * V_aibj;
-              const double hd = orc_fast_diag_os_double(
-                  h, eps_a[i], eps_b[j], eps_a[a], eps_b[b], i, j, a, b,
-                  h_diag);
-              h_el /= (E0 - hd);
-              if(pb_push(out, ex, coeff * h_el)) goto nomem;
-            }
-        }
+      /* opposite-spin doubles AABB */
+      for(int ii = 0; ii < k.na; ++ii)
+        for(int aa = 0; aa < k.nva; ++aa)
+          for(int jj = 0; jj < k.nb; ++jj)
+            for(int bb = 0; bb < k.nvb; ++bb)
+              if(cand_os(h, &k, k.oa[ii], k.va[aa], k.ob[jj], k.vb[bb], E0, tol, &ex, &v))
+                if(pb_push(out, ex, coeff * v)) goto nomem;
     }
-    (void)NB;
     if(prune && out->n > st->pair_size_max) { /* :120-137 */
       int64_t w = 0;
       for(int64_t i = 0; i < out->n; ++i)
@@ -1295,12 +1318,12 @@ static int asci_contribs(const orc_ham* h, int W, const orc_asci_settings* st,
       for(int64_t i = 0; i < out->n; ++i) out->p[i].idx = i;
     }
   }
-  free(eps_a);
-  free(eps_b);
+  free(k.eps_a);
+  free(k.eps_b);
   return ORC_OK;
 nomem:
-  free(eps_a);
-  free(eps_b);
+  free(k.eps_a);
+  free(k.eps_b);
   return ORC_ERR_NOMEM;
 }
 
@@ -1324,6 +1347,37 @@ int64_t orc_asci_contributions(const orc_ham* h, int W,
   return n;
 }
 
+/* The end of asci_search after the contributions (determinant_search.hpp:
+ * 498-653): accumulate, rv <- -|rv|, core dets appended with +|C|, keep the
+ * largest copy, keep rv < 0, top (ndets_max - ncdets) by |rv|, core last. */
+static int asci_finish(int W, pair_buf* b, int64_t ndets_max, const uint64_t* cdets,
+                       int64_t ncdets, const double* C, uint64_t* new_dets, int64_t* n_new,
+                       double* rv_out) {
+  sort_and_accumulate(b);                                     /* :498 */
+  for(int64_t i = 0; i < b->n; ++i) b->p[i].rv = -fabs(b->p[i].rv); /* :530 */
+  for(int64_t i = 0; i < ncdets; ++i)                          /* :533-536 */
+    if(pb_push(b, load_det(W, cdets + W * i), fabs(C[i]))) return ORC_ERR_NOMEM;
+  keep_largest(b); /* :540 */
+  {                /* keep rv < 0 (:542-544) */
+    int64_t w = 0;
+    for(int64_t i = 0; i < b->n; ++i)
+      if(b->p[i].rv < 0.0) b->p[w++] = b->p[i];
+    b->n = w;
+  }
+  const int64_t top_k = ndets_max - ncdets; /* :547 */
+  if(b->n > top_k) {                        /* :565-629 */
+    qsort(b->p, b->n, sizeof(pair_t), cmp_absrv_desc);
+    b->n = top_k;
+  }
+  for(int64_t i = 0; i < b->n; ++i) {
+    store_det(W, b->p[i].s, new_dets + W * i);
+    if(rv_out) rv_out[i] = b->p[i].rv;
+  }
+  memcpy(new_dets + W * b->n, cdets, sizeof(uint64_t) * W * ncdets); /* :653 */
+  *n_new = b->n + ncdets;
+  return ORC_OK;
+}
+
 int orc_asci_search(const orc_ham* h, int W, const orc_asci_settings* s,
                     int64_t ndets_max, const uint64_t* cdets, int64_t ncdets,
                     double E_ASCI, const double* C, int64_t norb,
@@ -1331,37 +1385,174 @@ int orc_asci_search(const orc_ham* h, int W, const orc_asci_settings* s,
   /* asci_search (determinant_search.hpp:394-663), world_size == 1 */
   pair_buf b = {NULL, 0, 0};
   int rc = asci_contribs(h, W, s, cdets, ncdets, E_ASCI, C, norb, &b, 1);
-  if(rc) {
-    free(b.p);
-    return rc;
-  }
-  sort_and_accumulate(&b);                                    /* :498 */
-  for(int64_t i = 0; i < b.n; ++i) b.p[i].rv = -fabs(b.p[i].rv); /* :530 */
-  for(int64_t i = 0; i < ncdets; ++i) /* :533-536 */
-    if(pb_push(&b, load_det(W, cdets + W * i), fabs(C[i]))) {
-      free(b.p);
-      return ORC_ERR_NOMEM;
-    }
-  keep_largest(&b); /* :540 */
-  {                 /* keep rv < 0 (:542-544) */
-    int64_t w = 0;
-    for(int64_t i = 0; i < b.n; ++i)
-      if(b.p[i].rv < 0.0) b.p[w++] = b.p[i];
-    b.n = w;
-  }
-  const int64_t top_k = ndets_max - ncdets; /* :547 */
-  if(b.n > top_k) {                         /* :565-629 */
-    qsort(b.p, b.n, sizeof(pair_t), cmp_absrv_desc);
-    b.n = top_k;
-  }
-  for(int64_t i = 0; i < b.n; ++i) {
-    store_det(W, b.p[i].s, new_dets + W * i);
-    if(rv_out) rv_out[i] = b.p[i].rv;
-  }
-  memcpy(new_dets + W * b.n, cdets, sizeof(uint64_t) * W * ncdets); /* :653 */
-  *n_new = b.n + ncdets;
+  if(!rc) rc = asci_finish(W, &b, ndets_max, cdets, ncdets, C, new_dets, n_new, rv_out);
   free(b.p);
+  return rc;
+}
+
+/* ---- excitation blocks (the block screen; no reference counterpart) ----
+ * The candidates of a core det, grouped by hole set: [alpha single holes |
+ * beta single holes | alpha-alpha hole pairs | beta-beta hole pairs |
+ * alpha-beta hole pairs (alpha hole major)]; pairs (x < y) of the occupied
+ * lists in lexicographic order; particles in the loop order of the
+ * append_* kernels. */
+static int64_t block_total(const core_t* k, int singles) {
+  int64_t n = k->na + k->nb;
+  if(!singles)
+    n += (int64_t)k->na * (k->na - 1) / 2 + (int64_t)k->nb * (k->nb - 1) / 2 +
+         (int64_t)k->na * k->nb;
+  return n;
+}
+
+static void pair_of(int64_t q, int m, int* x, int* y) {
+  int a = 0;
+  while(q >= m - 1 - a) {
+    q -= m - 1 - a;
+    ++a;
+  }
+  *x = a;
+  *y = a + 1 + (int)q;
+}
+
+/* Walks block q of core k: the block's determinant (holes cleared), the
+ * largest |h / (E0 - H_kk)| of its screened-in candidates (-1 if none), and,
+ * with `out`, pushes (candidate, coeff * h / (E0 - H_kk)) in block order. */
+static int block_walk(const orc_ham* h, const core_t* k, int64_t q, double E0, double tol,
+                      double coeff, pair_buf* out, double* maxabs, det_t* bdet) {
+  det_t ex;
+  double v, m = -1.0;
+  *bdet = k->s;
+  int x, y;
+#define TAKE                                          \
+  do {                                                \
+    if(fabs(v) > m) m = fabs(v);                      \
+    if(out && pb_push(out, ex, coeff * v)) return ORC_ERR_NOMEM; \
+  } while(0)
+  if(q < k->na) {
+    bdet->a &= ~(1ULL << k->oa[q]);
+    for(int aa = 0; aa < k->nva; ++aa)
+      if(cand_single(h, k, 0, k->oa[q], k->va[aa], E0, tol, &ex, &v)) TAKE;
+    *maxabs = m;
+    return ORC_OK;
+  }
+  q -= k->na;
+  if(q < k->nb) {
+    bdet->b &= ~(1ULL << k->ob[q]);
+    for(int bb = 0; bb < k->nvb; ++bb)
+      if(cand_single(h, k, 1, k->ob[q], k->vb[bb], E0, tol, &ex, &v)) TAKE;
+    *maxabs = m;
+    return ORC_OK;
+  }
+  q -= k->nb;
+  for(int spin = 0; spin < 2; ++spin) {
+    const int no = spin ? k->nb : k->na, nv = spin ? k->nvb : k->nva;
+    const uint32_t* occ = spin ? k->ob : k->oa;
+    const uint32_t* vir = spin ? k->vb : k->va;
+    const int64_t np = (int64_t)no * (no - 1) / 2;
+    if(q < np) {
+      pair_of(q, no, &x, &y);
+      const uint64_t hb = (1ULL << occ[x]) | (1ULL << occ[y]);
+      if(spin)
+        bdet->b &= ~hb;
+      else
+        bdet->a &= ~hb;
+      for(int aa = 0; aa < nv; ++aa)
+        for(int bb = aa + 1; bb < nv; ++bb)
+          if(cand_ss(h, k, spin, occ[x], occ[y], vir[aa], vir[bb], E0, tol, &ex, &v)) TAKE;
+      *maxabs = m;
+      return ORC_OK;
+    }
+    q -= np;
+  }
+  {
+    const int ii = (int)(q / k->nb), jj = (int)(q % k->nb);
+    bdet->a &= ~(1ULL << k->oa[ii]);
+    bdet->b &= ~(1ULL << k->ob[jj]);
+    for(int aa = 0; aa < k->nva; ++aa)
+      for(int bb = 0; bb < k->nvb; ++bb)
+        if(cand_os(h, k, k->oa[ii], k->va[aa], k->ob[jj], k->vb[bb], E0, tol, &ex, &v)) TAKE;
+  }
+#undef TAKE
+  *maxabs = m;
   return ORC_OK;
+}
+
+int64_t orc_asci_blocks(const orc_ham* h, int W, const orc_asci_settings* s,
+                        const uint64_t* cdets, int64_t ncdets, double E_ASCI, int64_t norb,
+                        uint64_t* block_dets, double* target) {
+  core_t k;
+  k.eps_a = (double*)malloc(sizeof(double) * norb);
+  k.eps_b = (double*)malloc(sizeof(double) * norb);
+  int64_t g = 0;
+  for(int64_t c = 0; c < ncdets; ++c) {
+    core_load(h, W, cdets + W * c, norb, &k);
+    const int64_t nb = block_total(&k, s->just_singles);
+    if(block_dets || target)
+      for(int64_t q = 0; q < nb; ++q) {
+        double m;
+        det_t bd;
+        block_walk(h, &k, q, E_ASCI, s->h_el_tol, 1.0, NULL, &m, &bd);
+        if(block_dets) store_det(W, bd, block_dets + W * (g + q));
+        if(target) target[g + q] = m < 0.0 ? -INFINITY : log(m);
+      }
+    g += nb;
+  }
+  free(k.eps_a);
+  free(k.eps_b);
+  return g;
+}
+
+typedef struct {
+  float s;
+  int64_t i;
+} bscore_t;
+static int cmp_bscore(const void* x_, const void* y_) {
+  const bscore_t *x = (const bscore_t*)x_, *y = (const bscore_t*)y_;
+  if(x->s != y->s) return x->s > y->s ? -1 : 1;
+  return (x->i > y->i) - (x->i < y->i);
+}
+
+int orc_asci_search_blocks(const orc_ham* h, int W, const orc_asci_settings* s,
+                           const float* scores, double keep_fraction, int64_t ndets_max,
+                           const uint64_t* cdets, int64_t ncdets, double E_ASCI, const double* C,
+                           int64_t norb, uint64_t* new_dets, int64_t* n_new, double* rv_out) {
+  const int64_t nblk = orc_asci_blocks(h, W, s, cdets, ncdets, E_ASCI, norb, NULL, NULL);
+  int64_t nkeep = (int64_t)ceil(keep_fraction * (double)nblk);
+  if(nkeep < 1) nkeep = 1;
+  if(nkeep > nblk) nkeep = nblk;
+  bscore_t* o = (bscore_t*)malloc(sizeof(bscore_t) * (nblk + 1));
+  char* kept = (char*)calloc(nblk + 1, 1);
+  for(int64_t g = 0; g < nblk; ++g) {
+    float v = scores[g];
+    o[g].s = v != v ? -INFINITY : v;
+    o[g].i = g;
+  }
+  qsort(o, nblk, sizeof(bscore_t), cmp_bscore);
+  for(int64_t t = 0; t < nkeep; ++t) kept[o[t].i] = 1;
+  free(o);
+  core_t k;
+  k.eps_a = (double*)malloc(sizeof(double) * norb);
+  k.eps_b = (double*)malloc(sizeof(double) * norb);
+  pair_buf b = {NULL, 0, 0};
+  int rc = ORC_OK;
+  int64_t g = 0;
+  for(int64_t c = 0; c < ncdets && !rc; ++c) {
+    core_load(h, W, cdets + W * c, norb, &k);
+    const int64_t nb = block_total(&k, s->just_singles);
+    for(int64_t q = 0; q < nb && !rc; ++q)
+      if(kept[g + q]) {
+        double m;
+        det_t bd;
+        rc = block_walk(h, &k, q, E_ASCI, s->h_el_tol, C[c], &b, &m, &bd);
+      }
+    g += nb;
+  }
+  if(!rc) rc = asci_finish(W, &b, ndets_max, cdets, ncdets, C, new_dets, n_new, rv_out);
+  free(b.p);
+  free(kept);
+  free(k.eps_a);
+  free(k.eps_b);
+  return rc;
 }
 
 typedef struct {

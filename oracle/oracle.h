@@ -189,6 +189,23 @@ int orc_asci_search(const orc_ham* h, int W, const orc_asci_settings* s,
                     double E_ASCI, const double* C, int64_t norb,
                     uint64_t* new_dets, int64_t* n_new, double* rv_out);
 
+/* Excitation blocks of a core (the block screen; no reference counterpart):
+ * per core det, one block per hole set in the order [alpha single holes |
+ * beta single holes | alpha-alpha hole pairs | beta-beta hole pairs |
+ * alpha-beta hole pairs]. block_dets (W words each) = the core det with the
+ * holes cleared; target = log max |h / (E_ASCI - H_kk)| over the block's
+ * screened-in candidates (-INFINITY if none). Either may be NULL; returns
+ * the number of blocks. */
+int64_t orc_asci_blocks(const orc_ham* h, int W, const orc_asci_settings* s,
+                        const uint64_t* cdets, int64_t ncdets, double E_ASCI, int64_t norb,
+                        uint64_t* block_dets, double* target);
+/* asci_search over the ceil(keep_fraction * nblocks) blocks with the largest
+ * scores (ties: lower block index first); no pair_size_max pruning. */
+int orc_asci_search_blocks(const orc_ham* h, int W, const orc_asci_settings* s,
+                           const float* scores, double keep_fraction, int64_t ndets_max,
+                           const uint64_t* cdets, int64_t ncdets, double E_ASCI, const double* C,
+                           int64_t norb, uint64_t* new_dets, int64_t* n_new, double* rv_out);
+
 /* asci_iter (iteration.hpp:16-67). wfn/X are in/out, capacity ndets_max;
  * *nwfn is the current size in and the new size out. */
 int orc_asci_iter(const orc_ham* h, int W, const orc_asci_settings* s,

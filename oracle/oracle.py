@@ -71,6 +71,9 @@ def lib():
             "orc_asci_default_settings": (None, [_p]),
             "orc_asci_contributions": (_i64, [_p, _i32, _p, _p, _i64, _d, _p, _i64, _p, _p]),
             "orc_asci_search": (_i32, [_p, _i32, _p, _i64, _p, _i64, _d, _p, _i64, _p, _p, _p]),
+            "orc_asci_blocks": (_i64, [_p, _i32, _p, _p, _i64, _d, _i64, _p, _p]),
+            "orc_asci_search_blocks": (_i32, [_p, _i32, _p, _p, _d, _i64, _p, _i64, _d, _p, _i64,
+                                              _p, _p, _p]),
             "orc_asci_iter": (_i32, [_p, _i32, _p, _i64, _d, _d, _i64, _d, _p, _p, _p, _i64, _p]),
             "orc_asci_grow": (_i32, [_p, _i32, _p, _i64, _d, _d, _p, _p, _p, _p, _i64]),
             "orc_asci_refine": (_i32, [_p, _i32, _p, _i64, _d, _d, _p, _p, _p, _p, _i64]),
@@ -288,6 +291,34 @@ class Ham:
                                    E0, _ptr(C0), self.norb, _ptr(out), C.byref(nn), _ptr(rv))
         if rc:
             raise RuntimeError(f"orc_asci_search rc={rc}")
+        n = nn.value
+        return out[: n * W].reshape(n, W), rv[: n - nc]
+
+    def asci_blocks(self, W, settings, cdets, E0):
+        """(block determinants [nblk, W], targets [nblk]) of the core."""
+        cdets = np.ascontiguousarray(cdets, dtype=np.uint64)
+        nc = cdets.size // W
+        n = lib().orc_asci_blocks(self._h, W, C.byref(settings), _ptr(cdets), nc, E0, self.norb,
+                                  None, None)
+        bd = np.zeros(max(n, 1) * W, dtype=np.uint64)
+        t = np.zeros(max(n, 1))
+        lib().orc_asci_blocks(self._h, W, C.byref(settings), _ptr(cdets), nc, E0, self.norb,
+                              _ptr(bd), _ptr(t))
+        return bd[: n * W].reshape(n, W), t[:n]
+
+    def asci_search_blocks(self, W, settings, scores, keep_fraction, ndets_max, cdets, E0, C0):
+        cdets = np.ascontiguousarray(cdets, dtype=np.uint64)
+        C0 = np.ascontiguousarray(C0, dtype=np.float64)
+        sc = np.ascontiguousarray(scores, dtype=np.float32)
+        nc = cdets.size // W
+        out = np.zeros(ndets_max * W, dtype=np.uint64)
+        rv = np.zeros(ndets_max)
+        nn = C.c_int64()
+        rc = lib().orc_asci_search_blocks(self._h, W, C.byref(settings), _ptr(sc), keep_fraction,
+                                          ndets_max, _ptr(cdets), nc, E0, _ptr(C0), self.norb,
+                                          _ptr(out), C.byref(nn), _ptr(rv))
+        if rc:
+            raise RuntimeError(f"orc_asci_search_blocks rc={rc}")
         n = nn.value
         return out[: n * W].reshape(n, W), rv[: n - nc]
 
