@@ -44,6 +44,7 @@ namespace {
 
 constexpr int kTile = 64;   // kets per tile (one per lane)
 constexpr int kSuper = 64;  // tiles per superblock
+constexpr int kXcdChunk = 64;  // scan blocks per chunk dealt round-robin to the 8 XCDs
 constexpr int kRows = 8;    // bra rows per wavefront in the scan passes
 constexpr int kWavesPerBlock = 4;
 
@@ -137,11 +138,14 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
     uint32_t* __restrict__ row_counts, const int64_t* __restrict__ rowptr,
     int32_t* __restrict__ colind, StagePool P) {
   constexpr bool FILL = MODE == kFill;
-  // XCD-aware mapping: hardware dispatches block b to XCD b % 8; logical
+  // XCD-aware mapping: hardware dispatches block b to XCD b % 8. Logical
   // blocks (consecutive rows, which share spin strings and therefore ket
-  // tiles) are laid out so each XCD's L2 sees a contiguous range of rows.
-  const int64_t per = gridDim.x / 8;
-  const int64_t lblk = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  // tiles) come in chunks of kXcdChunk blocks, chunk c on XCD c % 8: each
+  // XCD's L2 sees contiguous rows, and every XCD samples the whole list (the
+  // work per row varies along a selected-CI list: one contiguous eighth per
+  // XCD left the first XCD with twice the mean).
+  const int64_t xcd = blockIdx.x % 8, k = blockIdx.x / 8;
+  const int64_t lblk = (xcd + 8 * (k / kXcdChunk)) * kXcdChunk + k % kXcdChunk;
   if(lblk >= nblocks) return;
   const int64_t wave = lblk * kWavesPerBlock + (threadIdx.x >> 6);
   if(wave >= nwaves) return;
@@ -539,7 +543,9 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   // 2. count
   const int64_t nwaves = ex.items ? ex.nitems : (nbra + kRows - 1) / kRows;
   const int64_t nblocks = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int64_t grid = (nblocks + 7) / 8 * 8;
+  // whole rounds of 8 chunks (the scan kernel's XCD mapping)
+  const int64_t nchunks = (nblocks + kXcdChunk - 1) / kXcdChunk;
+  const int64_t grid = (nchunks + 7) / 8 * 8 * kXcdChunk;
   MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many bra rows");
   DevBuf counts32(sizeof(uint32_t) * (nbra + 1));
   DevBuf counts64(sizeof(int64_t) * (nbra + 1));
