@@ -603,6 +603,56 @@ def c5_section(C, ctx, sel):
     }
 
 
+def c1_section(C, ctx):
+    """CAS(10e,10o) (63,504 determinants, seeded integrals n = 10, seed 10) timed
+    in full on both sides: the GPU's selected_ci_diag (build + restarted
+    Davidson to 1e-8) and, on the host, the oracle's serial double loop
+    (double_loop.hpp:30-98, one pass per row as the reference pushes its
+    entries) alone, then its selected_ci_diag (that build + the restarted
+    Davidson with the reference's row-parallel gespmbv). No extrapolation."""
+    from macis_amd import dets as D
+    from macis_amd.synthetic import seeded_integrals
+    from oracle import oracle as O
+
+    n = 10
+    T, V = seeded_integrals(n, 10)
+    dets = D.generate_hilbert_space(1, n, 5, 5)
+    N = len(dets)
+    eps = np.finfo(float).eps
+    ctx.set_integrals(n, T, V)
+    dd = ctx.to_device(dets)
+    dX = ctx.empty(N, np.float64)
+    runs = []
+    for _ in range(4):
+        dX.zero()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        E, nnz, _ = ctx.selected_ci_diag(1, dd, N, dX, h_el_tol=eps, max_m=20, res_tol=1e-8,
+                                         max_restarts=200)
+        ctx.synchronize()
+        runs.append(((time.perf_counter() - t0) * 1e3, ctx.timings()["ham_total_ms"]))
+    gpu_ms, gpu_build_ms = sorted(runs[1:])[1]
+    orc = O.Ham(T, V)
+    t0 = time.perf_counter()
+    _, ci, _ = orc.csr(1, dets, thresh=eps)
+    cpu_build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E_cpu, _, _ = orc.selected_ci_diag(1, dets, max_restarts=200)
+    cpu_solve_s = time.perf_counter() - t0
+    return {
+        "workload": "CAS(10e,10o), 63,504 determinants, H build + Davidson (subspace 20, "
+                    "restarted, tol 1e-8), seeded integrals n=10 seed 10",
+        "nnz": nnz, "E0": E, "E0_cpu": E_cpu, "abs_dE": abs(E - E_cpu),
+        "gpu_solve_ms": gpu_ms, "gpu_build_ms": gpu_build_ms,
+        "gpu_build_nnz_per_s": nnz / (gpu_build_ms * 1e-3),
+        "cpu_build_s": cpu_build_s, "cpu_build_nnz_per_s": len(ci) / cpu_build_s,
+        "cpu_build_threads": 1, "cpu_solve_s": cpu_solve_s,
+        "cpu_solve_threads": O.lib().orc_num_threads(),
+        "solve_speedup": cpu_solve_s / (gpu_ms * 1e-3),
+        "note": "both sides timed in full (no sampling); cpu_solve includes its own serial build",
+    }
+
+
 def c5_only(C, ctx):
     """The c5 section on its own: the selected-CI list and ground state first."""
     from macis_amd.synthetic import seeded_integrals, selected_ci_dets
@@ -633,7 +683,7 @@ def main():
                     help="one 1 GiB device-to-device copy first (counter calibration)")
     ap.add_argument("--no-c5", action="store_true",
                     help="skip the 64-orbital learned block screen section")
-    ap.add_argument("--only", choices=["selected_ci", "batch", "asci", "mlp", "c5"],
+    ap.add_argument("--only", choices=["selected_ci", "batch", "asci", "mlp", "c5", "c1"],
                     help="run only this section and print its JSON object (for profiling; "
                          "not the benchmark line)")
     args = ap.parse_args()
@@ -653,7 +703,8 @@ def main():
                "batch": lambda: batch_section(C, ctx),
                "asci": lambda: asci_step_comparison(C, ctx),
                "mlp": lambda: mlp_throughput(C, ctx),
-               "c5": lambda: c5_only(C, ctx)}[args.only]
+               "c5": lambda: c5_only(C, ctx),
+               "c1": lambda: c1_section(C, ctx)}[args.only]
         print(json.dumps({"section": args.only, args.only: run()}), flush=True)
         return
     T, V = seeded_integrals(N_ORB, SEED)
@@ -774,6 +825,7 @@ def main():
         out["cpu_spmv"] = cpu_spmv_baseline(ctx, T, V, d_dets, N)
         assert out["cpu_spmv"]["sample"].startswith(f"full CAS(12e,12o) SpMV ({nnz} nonzeros")
         out["cpu_spmv"]["gpu_speedup"] = out["cpu_spmv"]["ms_per_spmv"] / avg_spmv_ms
+        out["c1"] = c1_section(C, ctx)
         out["davidson"]["E0_oracle"] = out["cpu_spmv"]["oracle_E0"]
         out["davidson"]["abs_dE_vs_oracle"] = abs(E - out["cpu_spmv"]["oracle_E0"])
         out["cpu_rdms"] = cpu_rdm_baseline(dets, d_X.numpy())

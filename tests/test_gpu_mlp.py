@@ -60,13 +60,18 @@ def test_mlp_forward_backward_vs_oracle(ctx, W, n):
     assert loss == pytest.approx(loss_ref, rel=2e-3)
     g = net.grads().astype(np.float64)
     rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
-    assert rel < 2e-2, rel
     # per block (W1, b1, W2, b2, w3, b3)
     H, I = 512, 128
     cuts = np.cumsum([0, H * I, H, H * H, H, H, 1])
-    for a, b in zip(cuts[:-1], cuts[1:]):
-        r = np.linalg.norm(g[a:b] - g_ref[a:b]) / max(np.linalg.norm(g_ref[a:b]), 1e-30)
-        assert r < 3e-2, (a, b, r)
+    rb = [np.linalg.norm(g[a:b] - g_ref[a:b]) / max(np.linalg.norm(g_ref[a:b]), 1e-30)
+          for a, b in zip(cuts[:-1], cuts[1:])]
+    print(f"\n  gradient rel. error: all {rel:.2e}; blocks " + " ".join(f"{r:.2e}" for r in rb))
+    # the oracle rounds to bf16 at the same points, so what remains is FP32
+    # accumulation order and the odd bf16 tie flipped by it (measured: at
+    # most 1.8e-4 per block)
+    assert rel < 1e-3, rel
+    for r in rb:
+        assert r < 2e-3, rb
     assert np.array_equal(net.params(), P)  # lr == 0 leaves weights alone
 
 
