@@ -320,7 +320,8 @@ def batch_section(C, ctx):
     X0 = np.concatenate([perturbed_diagonal_guess(Dg[B.offsets[i]:B.offsets[i + 1]], seed=i)
                          for i in range(len(norbs))])
     B.set_guess(X0)
-    B.tune()
+    lanes = B.tune()
+    layout = B.spmv_layout()
     B.davidson(max_m=20, max_restarts=200, tol=1e-8, use_graphs=True)  # graphs recorded
     t0 = time.perf_counter()
     E, res, _ = B.davidson(max_m=20, max_restarts=200, tol=1e-8, use_graphs=True)
@@ -340,6 +341,8 @@ def batch_section(C, ctx):
                     f"{int((sizes == 10).sum())}), own seeded integrals",
         "ndets": int(nd), "nnz": int(nnz), "build_ms": build_ms,
         "build_nnz_per_s": nnz / (build_ms * 1e-3), "davidson_ms": dav_ms,
+        "spmv_layout": "sliced copy" if layout == 1 else "CSR, autotuned lanes per row",
+        "tuned_lanes_per_bucket": [int(v) for v in lanes[:3]],
         "max_residual": float(res.max()), "sequential_solves_ms": seq_ms,
         "speedup_vs_sequential": seq_ms / (build_ms + dav_ms),
         "instances_within_1e-8": int((dE <= 1e-8).sum()),

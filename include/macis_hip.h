@@ -337,8 +337,16 @@ int mhip_batch_info(const mhip_batch* batch, int64_t* ndets, int64_t* nnz);
 int mhip_batch_build(mhip_context* ctx, mhip_batch* batch, double thresh);
 /* Autotuner: picks the SpMV lanes per row (8/16/32/64) and the rows per
  * Hamiltonian work item for every instance-size bucket (largest first);
- * lanes_out (optional) receives one entry per bucket. */
+ * lanes_out (optional) receives one entry per bucket. It then times that CSR
+ * schedule against the sliced copy of the block-diagonal matrix (when the
+ * copy's padding rule allows one) and keeps the faster layout. */
 int mhip_batch_tune(mhip_context* ctx, mhip_batch* batch, int32_t* lanes_out);
+/* SpMV layout of the batched Davidson: layout 1 forces the sliced copy
+ * (MHIP_ERR_UNSUPPORTED when the padding rule refuses it), 0 the CSR
+ * schedule, -1 back to the default (the sliced copy when allowed), -2 only
+ * queries. *current (optional) receives 1 (sliced), 0 (CSR schedule) or -1
+ * (undecided: the default applies at the next Davidson). */
+int mhip_batch_spmv_layout(mhip_context* ctx, mhip_batch* batch, int layout, int* current);
 /* Davidson of every instance in lockstep (davidson.hpp:128-225 per
  * instance; max_restarts > 0 selects the restarted variant), iterations
  * captured as hipGraphs when use_graphs != 0. E and residual (optional) are

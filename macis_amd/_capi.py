@@ -143,6 +143,7 @@ _SIGS = {
     "mhip_batch_info": (_i32, [_p, _pi64, _pi64]),
     "mhip_batch_build": (_i32, [_p, _p, _d]),
     "mhip_batch_tune": (_i32, [_p, _p, _p]),
+    "mhip_batch_spmv_layout": (_i32, [_p, _p, _i32, C.POINTER(C.c_int)]),
     "mhip_batch_davidson": (_i32, [_p, _p, _i64, _i64, _d, _i32, _p, _p, _pd]),
     "mhip_batch_diagonal": (_i32, [_p, _p, _p]),
     "mhip_batch_set_guess": (_i32, [_p, _p, _p]),
@@ -498,6 +499,14 @@ class Batch:
         lanes = np.zeros(self.ninst, np.int32)
         check(lib().mhip_batch_tune(self.ctx.h, self.h, lanes.ctypes.data), "batch_tune")
         return lanes
+
+    def spmv_layout(self, layout=-2):
+        """Forces (1 sliced, 0 CSR schedule, -1 default) or, with -2, only
+        queries the SpMV layout; returns the current one."""
+        cur = C.c_int()
+        check(lib().mhip_batch_spmv_layout(self.ctx.h, self.h, layout, C.byref(cur)),
+              "batch_spmv_layout")
+        return cur.value
 
     def diagonal(self):
         """H_ii of all instances, caller order (concatenated)."""

@@ -484,6 +484,10 @@ struct mhip_batch {
   };
   std::vector<Bucket> buckets;
   bool tuned = false;
+  // SpMV layout: 1 the sliced copy of the block-diagonal matrix, 0 the CSR
+  // kernels with the bucket schedule, -1 undecided (the sliced copy when its
+  // padding rule allows it). The autotuner times both and sets it.
+  int layout = -1;
   // Davidson workspace
   int64_t max_m = 0;
   mhip::DevBuf V, AV, X, D, C, coef, coef_prev, coef2, Q, lam, tmp, conv, E, res, nleft;
@@ -505,7 +509,7 @@ namespace {
 
 void launch_spmv(mhip_context* ctx, mhip_batch* B, const double* x, double* y, bool masked) {
   const int32_t* conv = masked ? B->conv.as<int32_t>() : nullptr;
-  if(B->H->sell) {  // sliced copy of the block-diagonal matrix (equal-length rows)
+  if(B->H->sell && B->layout != 0) {  // sliced copy of the block-diagonal matrix
     spmv_sliced_masked(ctx, B->H.get(), x, y, B->row_inst.as<int32_t>(), conv);
     return;
   }
@@ -783,7 +787,9 @@ int mhip_batch_build(mhip_context* ctx, mhip_batch* B, double thresh) {
 
 // Autotuner: per size bucket, time the SpMV with 8/16/32/64 lanes per row
 // on the built matrix and keep the fastest. Rows per scan work item are set
-// from the same choice (short rows -> more rows per item).
+// from the same choice (short rows -> more rows per item). Then the whole
+// tuned CSR schedule is timed against the sliced copy (when its padding rule
+// allows one) and the faster layout is kept for the Davidson.
 int mhip_batch_tune(mhip_context* ctx, mhip_batch* B, int32_t* lanes_out) {
   return guarded([&] {
     use_device(ctx);
@@ -791,6 +797,7 @@ int mhip_batch_tune(mhip_context* ctx, mhip_batch* B, int32_t* lanes_out) {
     DevBuf x(8 * B->N), y(8 * B->N);
     MHIP_HIP_CHECK(hipMemsetAsync(x.p, 0, 8 * B->N, ctx->stream));
     Timer t(ctx->stream);
+    B->layout = 0;  // the buckets are timed on the CSR kernels
     int bi = 0;
     for(auto& bk : B->buckets) {
       double best = 1e30;
@@ -815,7 +822,34 @@ int mhip_batch_tune(mhip_context* ctx, mhip_batch* B, int32_t* lanes_out) {
       if(lanes_out) lanes_out[bi] = bestl;
       ++bi;
     }
+    auto time3 = [&] {
+      launch_spmv(ctx, B, x.as<double>(), y.as<double>(), false);  // warm
+      t.start();
+      for(int r = 0; r < 3; ++r) launch_spmv(ctx, B, x.as<double>(), y.as<double>(), false);
+      t.stop();
+      return t.ms();
+    };
+    const double csr_ms = time3();
+    B->layout = 0;
+    if(prepare_sell(ctx, B->H.get())) {
+      B->layout = 1;
+      const double sell_ms = time3();
+      B->layout = sell_ms < csr_ms ? 1 : 0;
+    }
     B->tuned = true;
+  });
+}
+
+int mhip_batch_spmv_layout(mhip_context* ctx, mhip_batch* B, int layout, int* current) {
+  return guarded([&] {
+    use_device(ctx);
+    MHIP_REQUIRE(B && B->H, MHIP_ERR_INVALID_ARG, "build the batch first");
+    MHIP_REQUIRE(layout >= -2 && layout <= 1, MHIP_ERR_INVALID_ARG, "bad layout");
+    if(layout == 1)
+      MHIP_REQUIRE(prepare_sell(ctx, B->H.get()), MHIP_ERR_UNSUPPORTED,
+                   "no sliced copy for this batch (padding rule or memory)");
+    if(layout != -2) B->layout = layout;
+    if(current) *current = B->layout == 0 ? 0 : (B->H->sell ? 1 : -1);
   });
 }
 
@@ -877,8 +911,9 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
       MHIP_HIP_CHECK(hipStreamSynchronize(st));
     }
     // the sliced SpMV copy when padding stays small (rows of one instance
-    // have one length) and memory allows; made once per built batch
-    prepare_sell(ctx, B->H.get());
+    // have one length) and memory allows, unless the autotuner chose the
+    // CSR schedule; made once per built batch
+    if(B->layout != 0) prepare_sell(ctx, B->H.get());
     Timer tall(st);
     tall.start();
     Seg sg{B->d_off.as<int64_t>(), B->conv.as<int32_t>()};
@@ -899,40 +934,54 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
     int32_t nleft = B->ninst;
     int i0 = 1;
     const int nb = max_m >= 3 ? 2 : 1;
-    for(int64_t rst = 0;; ++rst) {
-      for(int i = i0; i < max_m && nleft; ++i) {
-        if(use_graphs) {
-          auto it = B->graphs.find(i);
-          if(it == B->graphs.end()) {
-            hipGraph_t g;
-            MHIP_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            biteration(ctx, B, i, tol);
-            MHIP_HIP_CHECK(hipStreamEndCapture(st, &g));
-            hipGraphExec_t ge;
-            MHIP_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
-            it = B->graphs.emplace(i, ge).first;
-          }
-          MHIP_HIP_CHECK(hipGraphLaunch(it->second, st));
-        } else {
-          biteration(ctx, B, i, tol);
-        }
-        d2h(ctx, &nleft, B->nleft.p, 4);
-      }
-      if(!nleft || rst >= max_restarts) break;
-      // restart unconverged instances from [V c, V q1, t] (AV by the same
-      // combinations, no SpMV); the last two iterations shared the basis
-      // when the cycle ran at least two of them
+    // restart unconverged instances from [V c, V q1, t] (AV by the same
+    // combinations, no SpMV); the last two iterations shared the basis when
+    // the cycle ran at least two of them. Converged instances are frozen.
+    auto restart = [&](int first) {
       const int k = (int)max_m;
       hipLaunchKernelGGL(brestart_coef_kernel, dim3((B->ninst + 63) / 64), dim3(64), 0, st,
-                         B->ninst, B->conv.as<int32_t>(), k, nb, (int)(max_m - 1 > i0),
+                         B->ninst, B->conv.as<int32_t>(), k, nb, (int)(max_m - 1 > first),
                          B->coef.as<double>(), B->coef_prev.as<double>(), B->coef2.as<double>(),
                          B->Q.as<double>());
       hipLaunchKernelGGL(brestart_apply_kernel, dim3(red_grid(N)), dim3(256), 0, st, N, k, nb,
                          B->row_inst.as<int32_t>(), B->conv.as<int32_t>(), B->Q.as<double>(), V,
                          AV, ld, max_m);
-      MHIP_HIP_CHECK(hipGetLastError());
-      i0 = nb;
+    };
+    if(use_graphs) {
+      // one graph per restart cycle: its iterations and the restart after
+      // them, replayed with one host read of the unconverged count per
+      // cycle. Every kernel skips converged instances, so iterations after
+      // the last convergence, and a restart after the last cycle, change no
+      // instance's result.
+      for(int64_t rst = 0; nleft; ++rst) {
+        auto it = B->graphs.find(i0);
+        if(it == B->graphs.end()) {
+          hipGraph_t g;
+          MHIP_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          for(int i = i0; i < max_m; ++i) biteration(ctx, B, i, tol);
+          restart(i0);
+          MHIP_HIP_CHECK(hipStreamEndCapture(st, &g));
+          hipGraphExec_t ge;
+          MHIP_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+          (void)hipGraphDestroy(g);
+          it = B->graphs.emplace(i0, ge).first;
+        }
+        MHIP_HIP_CHECK(hipGraphLaunch(it->second, st));
+        d2h(ctx, &nleft, B->nleft.p, 4);
+        if(rst >= max_restarts) break;
+        i0 = nb;
+      }
+    } else {
+      for(int64_t rst = 0;; ++rst) {
+        for(int i = i0; i < max_m && nleft; ++i) {
+          biteration(ctx, B, i, tol);
+          d2h(ctx, &nleft, B->nleft.p, 4);
+        }
+        if(!nleft || rst >= max_restarts) break;
+        restart(i0);
+        MHIP_HIP_CHECK(hipGetLastError());
+        i0 = nb;
+      }
     }
     tall.stop();
     const double ms = tall.ms();

@@ -49,14 +49,21 @@ def test_batch_build_matches_per_instance_oracle(ctx):
     assert nnz == ref_nnz
 
 
-@pytest.mark.parametrize("use_graphs", [True, False])
-def test_batch_davidson_energies(ctx, use_graphs):
-    announce(f"batch davidson use_graphs={use_graphs}")
+@pytest.mark.parametrize("use_graphs,layout", [(True, None), (False, None), (True, 0),
+                                               (True, 1)])
+def test_batch_davidson_energies(ctx, use_graphs, layout):
+    """Energies against the per-instance oracle solve with the autotuner's
+    SpMV layout (None) and with each layout forced (0: tuned CSR schedule,
+    1: sliced copy); graphs hold one restart cycle each."""
+    announce(f"batch davidson use_graphs={use_graphs} layout={layout}")
     sizes = [6, 8, 4, 6, 8, 6, 4, 8, 6, 6]
     norbs, dl, Ts, Vs = make_instances(sizes, 900)
     B = _capi.Batch(ctx, norbs, dl, Ts, Vs)
     B.build(EPS)
     B.tune()
+    assert B.spmv_layout() in (0, 1)  # the tuner decided
+    if layout is not None:
+        assert B.spmv_layout(layout) == layout
     E, res, ms = B.davidson(max_m=20, max_restarts=200, tol=1e-8, use_graphs=use_graphs)
     assert np.all(res < 1e-8)
     for i, (n, d, T, V) in enumerate(zip(norbs, dl, Ts, Vs)):
