@@ -63,7 +63,13 @@ void* device_alloc(size_t bytes) {
   if(!bytes) return nullptr;
   void* p = nullptr;
   if(bytes < kCacheMinBytes) {
-    MHIP_HIP_CHECK(hipMalloc(&p, bytes));
+    hipError_t e = hipMalloc(&p, bytes);
+    if(e == hipErrorOutOfMemory) {  // kept blocks may hold the memory
+      (void)hipGetLastError();
+      device_cache_trim(current_device());
+      e = hipMalloc(&p, bytes);
+    }
+    MHIP_HIP_CHECK(e);
     return p;
   }
   BlockCache& c = block_cache();
@@ -126,6 +132,15 @@ void device_cache_trim(int device) noexcept {
   if(cur != device) (void)hipSetDevice(device);
   for(void* b : blocks) (void)hipFree(b);
   if(cur != device) (void)hipSetDevice(cur);
+}
+
+void device_mem_info(size_t* free_b, size_t* total_b) {
+  MHIP_HIP_CHECK(hipMemGetInfo(free_b, total_b));
+  const int dev = current_device();
+  if(dev < 0 || dev >= kMaxDevices) return;
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  for(auto& kv : c.idle[dev]) *free_b += kv.first;
 }
 
 void* noalloc_scratch(int device) {
@@ -212,7 +227,16 @@ DevBuf take_buffer(mhip_context* ctx, size_t bytes) {
 void keep_buffer(mhip_context* ctx, DevBuf&& b) {
   if(!b.p) return;
   ctx->spare.push_back(std::move(b));
-  if(ctx->spare.size() > 8) ctx->spare.erase(ctx->spare.begin());
+  // at most 8 buffers and a quarter of the device's memory, oldest dropped
+  // first: kept Hamiltonians must not starve the next, differently sized one
+  size_t free_b = 0, total_b = 0;
+  MHIP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  size_t held = 0;
+  for(auto& k : ctx->spare) held += k.bytes;
+  while(ctx->spare.size() > 1 && (ctx->spare.size() > 8 || held > total_b / 4)) {
+    held -= ctx->spare.front().bytes;
+    ctx->spare.erase(ctx->spare.begin());
+  }
 }
 
 }  // namespace mhip

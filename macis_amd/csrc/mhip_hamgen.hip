@@ -549,7 +549,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many bra rows");
   DevBuf counts32(sizeof(uint32_t) * (nbra + 1));
   DevBuf counts64(sizeof(int64_t) * (nbra + 1));
-  // tiled path over one list: one staged scan with a pool of 2,048 entries
+  // tiled path over one list: one staged scan with a pool of 4,096 entries
   // per bra row (at most a third of the free memory; rows longer on average
   // overflow it and take count + fill). The pool is released before the
   // values are allocated, so the peak stays that of count + fill. Batched
@@ -560,10 +560,10 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   bool staged = false;
   if(!ex.product && !ex.items) {
     size_t free_b = 0, total_b = 0;
-    MHIP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    device_mem_info(&free_b, &total_b);
     const int64_t per_chunk = (int64_t)kChunk * 4 + 8 + 4;
     int64_t cap = std::min<int64_t>({(int64_t)(free_b / 3) / per_chunk,
-                                     nbra * (2048 / kChunk) + 1024, 1ll << 27});
+                                     nbra * (4096 / kChunk) + 1024, 1ll << 27});
     // MHIP_STAGE_POOL_CHUNKS caps the pool (0: no staging), so tests can
     // force the overflow fallback and the two-pass walk
     if(const char* env = std::getenv("MHIP_STAGE_POOL_CHUNKS"))

@@ -66,6 +66,10 @@ int guarded(F&& f) {
 void* device_alloc(size_t bytes);
 void device_free(void* p) noexcept;
 void device_cache_trim(int device) noexcept;
+// Free device memory of the current device as hipMemGetInfo reports it plus
+// the kept blocks, which an allocation would release: what sizing decisions
+// (staging pools, layout copies) may plan with.
+void device_mem_info(size_t* free_b, size_t* total_b);
 
 // Owning device buffer.
 struct DevBuf {

@@ -434,7 +434,7 @@ bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding) {
   // room for the copy (12 bytes per stored entry) plus 1 GiB for the caller
   const size_t need = 12 * (size_t)S->entries + (size_t(1) << 30);
   size_t free_b = 0, total_b = 0;
-  MHIP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  device_mem_info(&free_b, &total_b);
   size_t spare_b = 0;
   for(auto& b : ctx->spare) spare_b += b.bytes;
   if(free_b + spare_b < need) return false;
