@@ -21,6 +21,7 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <unordered_map>
 #include <vector>
 
 #include "mhip_device.hpp"
@@ -495,6 +496,12 @@ struct mhip_batch {
   int ntiles = 0;
   mhip::DevBuf tiles, tinst, toff, part;
   std::map<int, hipGraphExec_t> graphs;
+  // list_id[p]: first internal instance with the same determinant list as p
+  // (p itself when none); product-space structures of the instances, made
+  // at the first build (product_tried: made or found impossible)
+  std::vector<int> list_id;
+  std::unique_ptr<mhip::ProductBatch> product;
+  bool product_tried = false;
   // caller-supplied Davidson guess (internal order); used instead of the
   // diagonal guess while has_guess is set
   mhip::DevBuf guess;
@@ -656,6 +663,26 @@ int mhip_batch_create_w(mhip_context* ctx, int words_per_det, int ninst, const i
         ri[B->off[p] + i] = p;
       }
     }
+    // instances with identical determinant lists (the usual case for
+    // complete spaces of one size) share one product-space structure
+    {
+      B->list_id.assign(ninst, 0);
+      std::unordered_map<uint64_t, std::vector<int>> seen;
+      for(int p = 0; p < ninst; ++p) {
+        const uint64_t* a = hd.data() + W * B->off[p];
+        const int64_t len = W * (B->off[p + 1] - B->off[p]);
+        uint64_t h = 1469598103934665603ull ^ (uint64_t)len;
+        for(int64_t i = 0; i < len; ++i) h = (h ^ a[i]) * 1099511628211ull;
+        B->list_id[p] = p;
+        for(int q : seen[h])
+          if(B->off[q + 1] - B->off[q] == B->off[p + 1] - B->off[p] &&
+             std::equal(a, a + len, hd.data() + W * B->off[q])) {
+            B->list_id[p] = q;
+            break;
+          }
+        if(B->list_id[p] == p) seen[h].push_back(p);
+      }
+    }
     B->dets.alloc(8 * W * N);
     B->row_inst.alloc(4 * N);
     B->d_off.alloc(8 * (ninst + 1));
@@ -709,7 +736,14 @@ int mhip_batch_create_w(mhip_context* ctx, int words_per_det, int ninst, const i
       hv[p].V_red = d + n2 + 2 * n4 + n3;
       hv[p].G2_red = d + n2 + 2 * n4 + 2 * n3;
       hv[p].V2_red = d + 2 * n2 + 2 * n4 + 2 * n3;
-      hv[p].pair_sym = 0;  // read only by the product-space fill
+      // V(pq|rs) stored equal to V(rs|pq) bit for bit: the product-space
+      // fill may read the pair-swapped (contiguous) entry
+      bool sym = true;
+      for(size_t q = 0; q < n4 && sym; ++q) {
+        const size_t i = q % n, j = (q / n) % n, k = (q / n2) % n, l = q / n3;
+        sym = std::memcmp(&hV[q], &hV[k + n * (l + n * (i + n * j))], sizeof(double)) == 0;
+      }
+      hv[p].pair_sym = sym ? 1 : 0;
     }
     MHIP_HIP_CHECK(hipMemcpyAsync(B->ints.p, packed.data(), 8 * total, hipMemcpyHostToDevice, ctx->stream));
     B->views.alloc(sizeof(IntsView) * ninst);
@@ -755,6 +789,34 @@ int mhip_batch_build(mhip_context* ctx, mhip_batch* B, double thresh) {
   return guarded([&] {
     use_device(ctx);
     MHIP_REQUIRE(B, MHIP_ERR_INVALID_ARG, "null batch");
+    B->H.reset(new mhip_csr());
+    B->H->device = ctx->device;
+    B->H->m = B->H->n = B->N;
+    // product-space instances (complete active spaces): the neighbour-table
+    // walk of every instance in one launch per stage, unless the tiled path
+    // is forced
+    if(!B->product_tried && ctx->build_path != MHIP_BUILD_TILED) {
+      B->product_tried = true;
+      std::unique_ptr<ProductBatch> pb(new ProductBatch());
+      if(prepare_batch_product(ctx, B->W, B->dets.as<uint64_t>(), B->off, B->list_id,
+                               B->views.as<IntsView>(), *pb))
+        B->product = std::move(pb);
+    }
+    if(B->product && ctx->build_path != MHIP_BUILD_TILED) {
+      BuildExtra ex;
+      ex.views = B->views.as<IntsView>();
+      ex.row_inst = B->row_inst.as<int32_t>();
+      ex.bproduct = B->product.get();
+      if(B->W == 1)
+        build_impl<1>(ctx, B->dets.as<uint64_t>(), B->N, B->dets.as<uint64_t>(), B->N, thresh,
+                      B->H.get(), ex);
+      else
+        build_impl<2>(ctx, B->dets.as<uint64_t>(), B->N, B->dets.as<uint64_t>(), B->N, thresh,
+                      B->H.get(), ex);
+      return;
+    }
+    MHIP_REQUIRE(ctx->build_path != MHIP_BUILD_PRODUCT, MHIP_ERR_INVALID_ARG,
+                 "product-space build forced but an instance is not a product space");
     // work items: rows of one instance, in groups of the bucket's size
     std::vector<int64_t> items;
     for(auto& bk : B->buckets)
@@ -773,9 +835,6 @@ int mhip_batch_build(mhip_context* ctx, mhip_batch* B, double thresh) {
     ex.nitems = (int64_t)items.size() / 4;
     ex.views = B->views.as<IntsView>();
     ex.row_inst = B->row_inst.as<int32_t>();
-    B->H.reset(new mhip_csr());
-    B->H->device = ctx->device;
-    B->H->m = B->H->n = B->N;
     if(B->W == 1)
       build_impl<1>(ctx, B->dets.as<uint64_t>(), B->N, B->dets.as<uint64_t>(), B->N, thresh,
                     B->H.get(), ex);

@@ -520,12 +520,13 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   const IntsView I = view(ctx->ints);
 
   // 1. bounds (tiled path only)
+  const bool prod = ex.product || ex.bproduct;
   t.start();
   const int64_t ntiles = (nket + kTile - 1) / kTile;
   const int64_t nsuper = (ntiles + kSuper - 1) / kSuper;
   DevBuf bnd;
   uint64_t *tand = nullptr, *tor = nullptr, *sand = nullptr, *sor = nullptr;
-  if(!ex.product) {
+  if(!prod) {
     bnd.alloc(sizeof(uint64_t) * W * 2 * (ntiles + nsuper));
     tand = bnd.as<uint64_t>();
     tor = tand + W * ntiles;
@@ -558,7 +559,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   DevBuf s_pool, s_head, s_next, s_used, s_ctl;
   StagePool P{};
   bool staged = false;
-  if(!ex.product && !ex.items) {
+  if(!prod && !ex.items) {
     size_t free_b = 0, total_b = 0;
     device_mem_info(&free_b, &total_b);
     const int64_t per_chunk = (int64_t)kChunk * 4 + 8 + 4;
@@ -599,6 +600,8 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   auto count_pass = [&](bool stage) {
     if(ex.product)
       product_count(ctx, *ex.product, nbra, counts32.as<uint32_t>());
+    else if(ex.bproduct)
+      batch_product_count(ctx, *ex.bproduct, ex.row_inst, nbra, counts32.as<uint32_t>());
     else if(stage)
       hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kStage>), dim3(grid), dim3(256), 0, st,
                          d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
@@ -667,10 +670,14 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   DevBuf flag(sizeof(int));
   MHIP_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
   if(!staged) t.start();
-  if(ex.product) {
+  if(prod) {
     hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
                        nbra, diag.as<double>(), ex.views, ex.row_inst);
-    if(nnz_struct)
+    if(nnz_struct && ex.bproduct)
+      batch_product_fill_elements(ctx, *ex.bproduct, ex.row_inst, nbra, diag.as<double>(),
+                                  rp_struct.as<int64_t>(), col.as<int32_t>(), val.as<double>(),
+                                  thresh, counts64.as<int64_t>(), flag.as<int>());
+    else if(nnz_struct)
       product_fill_elements(ctx, W, *ex.product, I, nbra, diag.as<double>(),
                             rp_struct.as<int64_t>(), col.as<int32_t>(), val.as<double>(), thresh,
                             counts64.as<int64_t>(), flag.as<int>());
@@ -687,7 +694,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
 
   // 5. elements (tiled path)
   t.start();
-  if(!ex.product) {
+  if(!prod) {
     hipLaunchKernelGGL(diag_kernel<W>, dim3((nbra + 255) / 256), dim3(256), 0, st, I, d_bra,
                        nbra, diag.as<double>(), ex.views, ex.row_inst);
     if(nnz_struct)
@@ -733,7 +740,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   t_all.stop();
   tm.ham_total_ms = t_all.ms() + ex.detect_ms;
   tm.ham_nnz = csr->nnz;
-  tm.ham_path = ex.product ? MHIP_BUILD_PRODUCT : MHIP_BUILD_TILED;
+  tm.ham_path = prod ? MHIP_BUILD_PRODUCT : MHIP_BUILD_TILED;
 }
 
 template void build_impl<1>(mhip_context*, const uint64_t*, int64_t, const uint64_t*, int64_t,

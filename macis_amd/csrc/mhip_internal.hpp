@@ -302,6 +302,27 @@ void product_count(mhip_context* ctx, const ProductSpace& ps, int64_t n, uint32_
 void product_fill_elements(mhip_context* ctx, int W, const ProductSpace& ps, const IntsView& I,
                            int64_t n, const double* diag, const int64_t* rowptr, int32_t* colind,
                            double* nzval, double thresh, int64_t* kept, int* dropped);
+// Product spaces of the instances of a batched build: instances with the
+// same determinant list share one ProductSpace; every instance has its own
+// element tables and device view (first row, structure, integrals).
+struct ProductBatch {
+  std::vector<std::unique_ptr<ProductSpace>> spaces;
+  std::vector<int> space_of;  // instance -> spaces index
+  DevBuf tables, views, run_off, inner_off;
+  int ninst = 0;
+  int64_t total_runs = 0, total_inner = 0;
+};
+// list_id[i]: the first instance whose determinant list equals instance i's
+// (i itself when none). False when an instance is not a product space.
+bool prepare_batch_product(mhip_context* ctx, int W, const uint64_t* d_dets,
+                           const std::vector<int64_t>& off, const std::vector<int>& list_id,
+                           const IntsView* d_views, ProductBatch& pb);
+void batch_product_count(mhip_context* ctx, const ProductBatch& pb, const int32_t* row_inst,
+                         int64_t n, uint32_t* counts);
+void batch_product_fill_elements(mhip_context* ctx, const ProductBatch& pb,
+                                 const int32_t* row_inst, int64_t n, const double* diag,
+                                 const int64_t* rowptr, int32_t* colind, double* nzval,
+                                 double thresh, int64_t* kept, int* dropped);
 // Optional work-item table / per-instance integrals of a batched build.
 struct BuildExtra {
   const int64_t* items = nullptr;  // DEVICE, 4 int64 per item
@@ -309,6 +330,7 @@ struct BuildExtra {
   const IntsView* views = nullptr;  // DEVICE, one per instance
   const int32_t* row_inst = nullptr;  // DEVICE, instance of every row
   const ProductSpace* product = nullptr;  // structure passes of a product list
+  const ProductBatch* bproduct = nullptr;  // per-instance product spaces of a batch
   double detect_ms = 0.;                  // host time spent recognising it
 };
 template <int W>

@@ -24,6 +24,8 @@
 // list with repeated determinants goes to the tiled path). The neighbour
 // tables come from XOR/popcount over the string pairs on the host.
 #include <algorithm>
+#include <array>
+#include <memory>
 #include <cstring>
 #include <vector>
 
@@ -48,6 +50,36 @@ __device__ __forceinline__ void split(const uint64_t* d, int64_t i, uint64_t& a,
     a = d[2 * i];
     b = d[2 * i + 1];
   }
+}
+
+}  // namespace
+
+// Device view of one product space with its element tables and integrals.
+// A single build passes one by value; a batched build keeps one per
+// instance (row0 = the instance's first row, columns offset by it).
+struct PSView {
+  IntsView I;
+  int64_t row0, L, nruns;
+  int outer_alpha;
+  const uint64_t *ostr, *istr;
+  const int32_t *ocnt, *onb_ptr, *onb, *ptr1, *idx1, *ptr2, *idx2;
+  double *oval, *ival2;
+  int32_t *oexc, *iexc1, *iexc2;
+};
+
+namespace {
+
+// Instance of global item g for a prefix array off[0..ninst] (off[0] = 0).
+__device__ __forceinline__ int inst_of(const int64_t* __restrict__ off, int ninst, int64_t g) {
+  int lo = 0, hi = ninst;  // last i with off[i] <= g
+  while(hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if(off[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
 }
 
 // Flags any determinant that breaks the product layout.
@@ -79,17 +111,20 @@ __global__ void run_heads_kernel(const uint64_t* __restrict__ d, int64_t nruns, 
 }
 
 // counts[r*L+k] = n0(r) m2(k) + n1(r) m1(k) + n2(r)   (m0 = n0 = 1)
-__global__ void product_count_kernel(int64_t n, int64_t L, const int32_t* __restrict__ ocnt,
-                                     const int32_t* __restrict__ ptr1,
-                                     const int32_t* __restrict__ ptr2,
+// (batched: the row's instance view; rows of one instance are contiguous)
+template <bool BATCH>
+__global__ void product_count_kernel(PSView one, const PSView* __restrict__ many,
+                                     const int32_t* __restrict__ row_inst, int64_t n,
                                      uint32_t* __restrict__ counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
-  const int64_t r = i / L, k = i % L;
-  const uint32_t m1 = (uint32_t)(ptr1[k + 1] - ptr1[k]);
-  const uint32_t m2 = (uint32_t)(ptr2[k + 1] - ptr2[k]);
-  counts[i] = (uint32_t)ocnt[3 * r] * m2 + (uint32_t)ocnt[3 * r + 1] * m1 +
-              (uint32_t)ocnt[3 * r + 2];
+  const PSView& P = BATCH ? many[row_inst[i]] : one;
+  const int64_t li = i - P.row0;
+  const int64_t r = li / P.L, k = li % P.L;
+  const uint32_t m1 = (uint32_t)(P.ptr1[k + 1] - P.ptr1[k]);
+  const uint32_t m2 = (uint32_t)(P.ptr2[k + 1] - P.ptr2[k]);
+  counts[i] = (uint32_t)P.ocnt[3 * r] * m2 + (uint32_t)P.ocnt[3 * r + 1] * m1 +
+              (uint32_t)P.ocnt[3 * r + 2];
 }
 
 // Excitation of a single (o, v, sign) packed in an int32: o | v << 8 |
@@ -116,14 +151,24 @@ __device__ __forceinline__ double single_partial(const IntsView& I, uint64_t bra
 
 // Per outer-neighbour entry: level 2 -> the same-spin double element
 // (me_4 of the outer strings); level 1 -> packed single and its partial sum.
-__global__ __launch_bounds__(256) void outer_tables_kernel(IntsView I, int64_t nruns,
-                                                           const uint64_t* __restrict__ ostr,
-                                                           const int32_t* __restrict__ onb_ptr,
-                                                           const int32_t* __restrict__ onb,
-                                                           double* __restrict__ oval,
-                                                           int32_t* __restrict__ oexc) {
-  const int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if(r >= nruns) return;
+// (batched: wavefront g over the runs of all instances, run_off = prefix of
+// the instances' run counts)
+template <bool BATCH>
+__global__ __launch_bounds__(256) void outer_tables_kernel(PSView one,
+                                                           const PSView* __restrict__ many,
+                                                           const int64_t* __restrict__ run_off,
+                                                           int ninst, int64_t total) {
+  const int64_t g = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(g >= total) return;
+  const int inst = BATCH ? inst_of(run_off, ninst, g) : 0;
+  const PSView& P = BATCH ? many[inst] : one;
+  const int64_t r = BATCH ? g - run_off[inst] : g;
+  const IntsView& I = P.I;
+  const uint64_t* __restrict__ ostr = P.ostr;
+  const int32_t* __restrict__ onb_ptr = P.onb_ptr;
+  const int32_t* __restrict__ onb = P.onb;
+  double* __restrict__ oval = P.oval;
+  int32_t* __restrict__ oexc = P.oexc;
   const int lane = threadIdx.x & 63;
   const uint64_t b = ostr[r];
   for(int32_t j = onb_ptr[r] + lane; j < onb_ptr[r + 1]; j += 64) {
@@ -147,13 +192,25 @@ __global__ __launch_bounds__(256) void outer_tables_kernel(IntsView I, int64_t n
 // double element (level 2) or the packed single and its partial sum
 // (level 1), with the level in bits 24+; the level-<=1 list gets the packed
 // single (level 1) or -1 (k itself).
-__global__ __launch_bounds__(256) void inner_tables_kernel(
-    IntsView I, int64_t L, const uint64_t* __restrict__ istr, const int32_t* __restrict__ ptr1,
-    const int32_t* __restrict__ idx1, const int32_t* __restrict__ ptr2,
-    const int32_t* __restrict__ idx2, int32_t* __restrict__ iexc1, double* __restrict__ ival2,
-    int32_t* __restrict__ iexc2) {
-  const int64_t k = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if(k >= L) return;
+template <bool BATCH>
+__global__ __launch_bounds__(256) void inner_tables_kernel(PSView one,
+                                                           const PSView* __restrict__ many,
+                                                           const int64_t* __restrict__ inner_off,
+                                                           int ninst, int64_t total) {
+  const int64_t g = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if(g >= total) return;
+  const int inst = BATCH ? inst_of(inner_off, ninst, g) : 0;
+  const PSView& P = BATCH ? many[inst] : one;
+  const int64_t k = BATCH ? g - inner_off[inst] : g;
+  const IntsView& I = P.I;
+  const uint64_t* __restrict__ istr = P.istr;
+  const int32_t* __restrict__ ptr1 = P.ptr1;
+  const int32_t* __restrict__ idx1 = P.idx1;
+  const int32_t* __restrict__ ptr2 = P.ptr2;
+  const int32_t* __restrict__ idx2 = P.idx2;
+  int32_t* __restrict__ iexc1 = P.iexc1;
+  double* __restrict__ ival2 = P.ival2;
+  int32_t* __restrict__ iexc2 = P.iexc2;
   const int lane = threadIdx.x & 63;
   const uint64_t b = istr[k];
   for(int32_t t = ptr2[k] + lane; t < ptr2[k + 1]; t += 64) {
@@ -190,15 +247,10 @@ __device__ __forceinline__ double code_sign(int32_t code) {
 // diagonal), so the CSR is bit-identical; |h| > thresh is counted per row
 // for the shared compaction.
 
+template <bool BATCH>
 __global__ __launch_bounds__(256) void product_fill_elements_kernel(
-    IntsView I, int64_t n, int64_t L, int outer_alpha, const uint64_t* __restrict__ ostr,
-    const uint64_t* __restrict__ istr, const int32_t* __restrict__ onb_ptr,
-    const int32_t* __restrict__ onb, const int32_t* __restrict__ ptr1,
-    const int32_t* __restrict__ idx1, const int32_t* __restrict__ ptr2,
-    const int32_t* __restrict__ idx2, const double* __restrict__ oval,
-    const int32_t* __restrict__ oexc, const int32_t* __restrict__ iexc1,
-    const double* __restrict__ ival2, const int32_t* __restrict__ iexc2,
-    const double* __restrict__ diag, const int64_t* __restrict__ rowptr,
+    PSView one, const PSView* __restrict__ many, const int32_t* __restrict__ row_inst,
+    int64_t n, const double* __restrict__ diag, const int64_t* __restrict__ rowptr,
     int32_t* __restrict__ colind, double* __restrict__ nzval, double thresh,
     int64_t* __restrict__ kept, int* __restrict__ dropped) {
   __shared__ int32_t seg_start[kWavesPerBlock][kMaxSeg + 1];
@@ -207,7 +259,24 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + wv;
   if(row >= n) return;  // whole wavefront leaves together
   const int lane = threadIdx.x & 63;
-  const int64_t r = row / L, k = row % L;
+  const PSView& P = BATCH ? many[row_inst[row]] : one;
+  const IntsView& I = P.I;
+  const int64_t L = P.L, row0 = P.row0;
+  const int outer_alpha = P.outer_alpha;
+  const uint64_t* __restrict__ ostr = P.ostr;
+  const uint64_t* __restrict__ istr = P.istr;
+  const int32_t* __restrict__ onb_ptr = P.onb_ptr;
+  const int32_t* __restrict__ onb = P.onb;
+  const int32_t* __restrict__ ptr1 = P.ptr1;
+  const int32_t* __restrict__ idx1 = P.idx1;
+  const int32_t* __restrict__ ptr2 = P.ptr2;
+  const int32_t* __restrict__ idx2 = P.idx2;
+  const double* __restrict__ oval = P.oval;
+  const int32_t* __restrict__ oexc = P.oexc;
+  const int32_t* __restrict__ iexc1 = P.iexc1;
+  const double* __restrict__ ival2 = P.ival2;
+  const int32_t* __restrict__ iexc2 = P.iexc2;
+  const int64_t r = (row - row0) / L, k = (row - row0) % L;
   const uint64_t bo = ostr[r], bi = istr[k];
   const double d = diag[row];
   const int64_t st = rowptr[row];
@@ -325,7 +394,7 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
       if(p < total) {
         // streamed out once: non-temporal, so the 19 GB of stores of a large
         // build do not evict the element tables and integrals from L2
-        __builtin_nontemporal_store((int32_t)(r2[u] * L + k2[u]), colind + st + p);
+        __builtin_nontemporal_store((int32_t)(row0 + r2[u] * L + k2[u]), colind + st + p);
         __builtin_nontemporal_store(h[u], nzval + st + p);
         nk += fabs(h[u]) > thresh ? 1 : 0;
       }
@@ -523,10 +592,35 @@ bool detect_product_space(mhip_context* ctx, int W, const uint64_t* d, int64_t n
   return W == 1 ? detect<1>(ctx, d, n, ps) : detect<2>(ctx, d, n, ps);
 }
 
+namespace {
+PSView make_view(const ProductSpace& ps, const IntsView& I, int64_t row0) {
+  PSView v;
+  v.I = I;
+  v.row0 = row0;
+  v.L = ps.L;
+  v.nruns = ps.nruns;
+  v.outer_alpha = ps.outer_alpha ? 1 : 0;
+  v.ostr = ps.ostr;
+  v.istr = ps.istr;
+  v.ocnt = ps.ocnt;
+  v.onb_ptr = ps.onb_ptr;
+  v.onb = ps.onb;
+  v.ptr1 = ps.ptr1;
+  v.idx1 = ps.idx1;
+  v.ptr2 = ps.ptr2;
+  v.idx2 = ps.idx2;
+  v.oval = ps.oval;
+  v.ival2 = ps.ival2;
+  v.oexc = ps.oexc;
+  v.iexc1 = ps.iexc1;
+  v.iexc2 = ps.iexc2;
+  return v;
+}
+}  // namespace
+
 void product_count(mhip_context* ctx, const ProductSpace& ps, int64_t n, uint32_t* counts) {
-  hipLaunchKernelGGL(product_count_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                     ps.L, ps.ocnt, ps.ptr1, ps.ptr2,
-                     counts);
+  hipLaunchKernelGGL(product_count_kernel<false>, dim3((n + 255) / 256), dim3(256), 0,
+                     ctx->stream, make_view(ps, IntsView{}, 0), nullptr, nullptr, n, counts);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 
@@ -535,18 +629,127 @@ void product_fill_elements(mhip_context* ctx, int W, const ProductSpace& ps, con
                            double* nzval, double thresh, int64_t* kept, int* dropped) {
   (void)W;  // the strings are stored split; the kernel does not read determinants
   MHIP_REQUIRE(ps.max_seg <= kMaxSeg, MHIP_ERR_INVALID_ARG, "product space outside LDS staging");
+  const PSView v = make_view(ps, I, 0);
   // element tables of the neighbour lists (they depend on the integrals)
-  hipLaunchKernelGGL(outer_tables_kernel, dim3((ps.nruns + kWavesPerBlock - 1) / kWavesPerBlock),
-                     dim3(256), 0, ctx->stream, I, ps.nruns, ps.ostr, ps.onb_ptr, ps.onb, ps.oval,
-                     ps.oexc);
-  hipLaunchKernelGGL(inner_tables_kernel, dim3((ps.L + kWavesPerBlock - 1) / kWavesPerBlock),
-                     dim3(256), 0, ctx->stream, I, ps.L, ps.istr, ps.ptr1, ps.idx1, ps.ptr2,
-                     ps.idx2, ps.iexc1, ps.ival2, ps.iexc2);
-  hipLaunchKernelGGL(product_fill_elements_kernel,
-                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, ctx->stream, I,
-                     n, ps.L, ps.outer_alpha ? 1 : 0, ps.ostr, ps.istr, ps.onb_ptr, ps.onb,
-                     ps.ptr1, ps.idx1, ps.ptr2, ps.idx2, ps.oval, ps.oexc, ps.iexc1, ps.ival2,
-                     ps.iexc2, diag, rowptr, colind, nzval, thresh, kept, dropped);
+  hipLaunchKernelGGL(outer_tables_kernel<false>,
+                     dim3((ps.nruns + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0,
+                     ctx->stream, v, nullptr, nullptr, 1, ps.nruns);
+  hipLaunchKernelGGL(inner_tables_kernel<false>, dim3((ps.L + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3(256), 0, ctx->stream, v, nullptr, nullptr, 1, ps.L);
+  hipLaunchKernelGGL(product_fill_elements_kernel<false>,
+                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, ctx->stream, v,
+                     nullptr, nullptr, n, diag, rowptr, colind, nzval, thresh, kept, dropped);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+// ---- batched instances -----------------------------------------------------
+// Instances with the same determinant list (list_id) share one structure;
+// every instance has its own element tables (its integrals) and view.
+bool prepare_batch_product(mhip_context* ctx, int W, const uint64_t* d_dets,
+                           const std::vector<int64_t>& off, const std::vector<int>& list_id,
+                           const IntsView* d_views, ProductBatch& pb) {
+  const int ninst = (int)list_id.size();
+  pb.spaces.clear();
+  pb.space_of.assign(ninst, -1);
+  std::vector<int> first_of;  // structure index -> first instance with it
+  for(int i = 0; i < ninst; ++i) {
+    const int first = list_id[i];
+    if(first != i) {  // same list as an earlier instance
+      MHIP_REQUIRE(first >= 0 && first < i && pb.space_of[first] >= 0, MHIP_ERR_INVALID_ARG,
+                   "bad list id");
+      pb.space_of[i] = pb.space_of[first];
+      continue;
+    }
+    std::unique_ptr<ProductSpace> ps(new ProductSpace());
+    if(!detect_product_space(ctx, W, d_dets + W * off[i], off[i + 1] - off[i], *ps)) {
+      pb.spaces.clear();
+      return false;
+    }
+    pb.space_of[i] = (int)pb.spaces.size();
+    pb.spaces.push_back(std::move(ps));
+  }
+  // per-instance element tables: oval, ival2 (double), oexc, iexc1, iexc2
+  std::vector<size_t> tab_off(ninst + 1, 0);
+  std::vector<int64_t> run_off(ninst + 1, 0), inner_off(ninst + 1, 0);
+  for(int i = 0; i < ninst; ++i) {
+    const ProductSpace& ps = *pb.spaces[pb.space_of[i]];
+    run_off[i + 1] = run_off[i] + ps.nruns;
+    inner_off[i + 1] = inner_off[i] + ps.L;
+  }
+  // table sizes need the list lengths (device row pointers): read them once
+  // per structure
+  std::vector<std::array<int64_t, 3>> sizes(pb.spaces.size());
+  for(size_t q = 0; q < pb.spaces.size(); ++q) {
+    const ProductSpace& ps = *pb.spaces[q];
+    int32_t a = 0, b = 0, c = 0;
+    d2h(ctx, &a, ps.onb_ptr + ps.nruns, 4);
+    d2h(ctx, &b, ps.ptr1 + ps.L, 4);
+    d2h(ctx, &c, ps.ptr2 + ps.L, 4);
+    sizes[q] = {a, b, c};
+  }
+  for(int i = 0; i < ninst; ++i) {
+    const auto& z = sizes[pb.space_of[i]];
+    tab_off[i + 1] = tab_off[i] + 8 * (size_t)(z[0] + z[2]) + 4 * (size_t)(z[0] + z[1] + z[2]);
+    tab_off[i + 1] = (tab_off[i + 1] + 15) / 16 * 16;
+  }
+  pb.tables.alloc(std::max<size_t>(tab_off[ninst], 16));
+  std::vector<IntsView> hv(ninst);
+  d2h(ctx, hv.data(), d_views, sizeof(IntsView) * ninst);
+  std::vector<PSView> views(ninst);
+  int64_t max_seg = 0;
+  for(int i = 0; i < ninst; ++i) {
+    const ProductSpace& ps = *pb.spaces[pb.space_of[i]];
+    max_seg = std::max(max_seg, ps.max_seg);
+    PSView v = make_view(ps, hv[i], off[i]);
+    const auto& z = sizes[pb.space_of[i]];
+    char* base = pb.tables.as<char>() + tab_off[i];
+    v.oval = reinterpret_cast<double*>(base);
+    v.ival2 = v.oval + z[0];
+    v.oexc = reinterpret_cast<int32_t*>(v.ival2 + z[2]);
+    v.iexc1 = v.oexc + z[0];
+    v.iexc2 = v.iexc1 + z[1];
+    views[i] = v;
+  }
+  MHIP_REQUIRE(max_seg <= kMaxSeg, MHIP_ERR_INVALID_ARG, "product space outside LDS staging");
+  pb.views.alloc(sizeof(PSView) * ninst);
+  pb.run_off.alloc(8 * (ninst + 1));
+  pb.inner_off.alloc(8 * (ninst + 1));
+  hipStream_t st = ctx->stream;
+  MHIP_HIP_CHECK(hipMemcpyAsync(pb.views.p, views.data(), sizeof(PSView) * ninst,
+                                hipMemcpyHostToDevice, st));
+  MHIP_HIP_CHECK(hipMemcpyAsync(pb.run_off.p, run_off.data(), 8 * (ninst + 1),
+                                hipMemcpyHostToDevice, st));
+  MHIP_HIP_CHECK(hipMemcpyAsync(pb.inner_off.p, inner_off.data(), 8 * (ninst + 1),
+                                hipMemcpyHostToDevice, st));
+  MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  pb.ninst = ninst;
+  pb.total_runs = run_off[ninst];
+  pb.total_inner = inner_off[ninst];
+  return true;
+}
+
+void batch_product_count(mhip_context* ctx, const ProductBatch& pb, const int32_t* row_inst,
+                         int64_t n, uint32_t* counts) {
+  hipLaunchKernelGGL(product_count_kernel<true>, dim3((n + 255) / 256), dim3(256), 0,
+                     ctx->stream, PSView{}, pb.views.as<PSView>(), row_inst, n, counts);
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+
+void batch_product_fill_elements(mhip_context* ctx, const ProductBatch& pb,
+                                 const int32_t* row_inst, int64_t n, const double* diag,
+                                 const int64_t* rowptr, int32_t* colind, double* nzval,
+                                 double thresh, int64_t* kept, int* dropped) {
+  hipStream_t st = ctx->stream;
+  const PSView* v = pb.views.as<PSView>();
+  hipLaunchKernelGGL(outer_tables_kernel<true>,
+                     dim3((pb.total_runs + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, st,
+                     PSView{}, v, pb.run_off.as<int64_t>(), pb.ninst, pb.total_runs);
+  hipLaunchKernelGGL(inner_tables_kernel<true>,
+                     dim3((pb.total_inner + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0,
+                     st, PSView{}, v, pb.inner_off.as<int64_t>(), pb.ninst, pb.total_inner);
+  hipLaunchKernelGGL(product_fill_elements_kernel<true>,
+                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, st, PSView{},
+                     v, row_inst, n, diag, rowptr, colind, nzval, thresh, kept, dropped);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 

@@ -239,3 +239,39 @@ def test_batch_128bit_instances(ctx):
     # the one-word entry point refuses norb > 32
     with pytest.raises(_capi.MhipError, match="norb <= 32"):
         _capi.Batch(ctx, [36], [dl[1][:, 0]], [Ts[1]], [Vs[1]], W=1)
+
+
+def test_batch_product_and_tiled_builds_agree(ctx):
+    """The batched build walks the instances' spin-string neighbour tables
+    (complete spaces; instances with the same list share the tables) and
+    must give the block-diagonal CSR of the tiled per-instance double loop
+    bit for bit; a batch with one list that is not a product space falls
+    back to the tiled walk."""
+    announce("batched build: product-space path vs tiled path")
+    norbs, dl, Ts, Vs = make_instances([6, 8, 4, 6, 8, 4, 6], 321)
+    B = _capi.Batch(ctx, norbs, dl, Ts, Vs)
+    try:
+        ctx.set_build_path(ctx.BUILD_TILED)
+        B.build(EPS)
+        assert ctx.timings()["ham_path"] == ctx.BUILD_TILED
+        tiled = [B.instance_csr(i) for i in range(len(norbs))]
+        ctx.set_build_path(ctx.BUILD_AUTO)
+        B.build(EPS)
+        assert ctx.timings()["ham_path"] == ctx.BUILD_PRODUCT
+        for i in range(len(norbs)):
+            got = B.instance_csr(i)
+            assert np.array_equal(got[0], tiled[i][0]), i
+            assert np.array_equal(got[1], tiled[i][1]), i
+            assert np.array_equal(got[2].view(np.uint64), tiled[i][2].view(np.uint64)), i
+        # a list that is not a complete product of spin strings (a truncated
+        # complete space) keeps the whole batch on the tiled walk
+        dl2 = list(dl)
+        dl2[1] = dl2[1][: len(dl2[1]) - 3]
+        B2 = _capi.Batch(ctx, norbs, dl2, Ts, Vs)
+        B2.build(EPS)
+        assert ctx.timings()["ham_path"] == ctx.BUILD_TILED
+        rp, ci, nz = B2.instance_csr(1)
+        ref = O.Ham(Ts[1], Vs[1]).csr(1, dl2[1].reshape(-1, 1), thresh=EPS)
+        assert np.array_equal(rp, ref[0]) and np.array_equal(ci, ref[1])
+    finally:
+        ctx.set_build_path(ctx.BUILD_AUTO)
