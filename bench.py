@@ -606,6 +606,30 @@ def c5_section(C, ctx, sel):
     }
 
 
+def stream_read_ceiling():
+    """Practical HBM read rate of this GPU: a plain device reduction over
+    16 GiB (torch.sum, 10 reps), the reference the SpMV's measured traffic
+    rate is compared with. None when torch has no GPU."""
+    try:
+        import torch
+
+        if not torch.cuda.is_available():
+            return None
+        x = torch.ones(2**31, dtype=torch.float64, device="cuda")
+        x.sum()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            x.sum()
+        torch.cuda.synchronize()
+        gbs = 2**31 * 8 / ((time.perf_counter() - t0) / 10) / 1e9
+        del x
+        torch.cuda.empty_cache()
+        return gbs
+    except Exception:  # noqa: BLE001 - a reference number only
+        return None
+
+
 def c1_section(C, ctx):
     """CAS(10e,10o) (63,504 determinants, seeded integrals n = 10, seed 10) timed
     in full on both sides: the GPU's selected_ci_diag (build + restarted
@@ -811,6 +835,11 @@ def main():
         },
     }
     out["rdms"] = rdm_timing(ctx, T, V, d_dets, N, nnz, d_X, E)
+    out["spmv"]["stream_read_GBs"] = stream_read_ceiling()
+    if out["spmv"]["stream_read_GBs"]:
+        out["spmv"]["frac_of_stream_read"] = (
+            SPMV_TRAFFIC_GB * 1e9 / (avg_spmv_ms * 1e-3) / 1e9 / out["spmv"]["stream_read_GBs"]
+            if SPMV_TRAFFIC_GB and sliced else None)
     if not args.no_asci:
         out["mlp"] = mlp_throughput(C, ctx)
         out["asci"] = asci_step_comparison(C, ctx)
