@@ -606,28 +606,29 @@ def c5_section(C, ctx, sel):
     }
 
 
-def stream_read_ceiling():
-    """Practical HBM read rate of this GPU: a plain device reduction over
-    16 GiB (torch.sum, 10 reps), the reference the SpMV's measured traffic
-    rate is compared with. None when torch has no GPU."""
-    try:
-        import torch
-
-        if not torch.cuda.is_available():
-            return None
-        x = torch.ones(2**31, dtype=torch.float64, device="cuda")
-        x.sum()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(10):
-            x.sum()
-        torch.cuda.synchronize()
-        gbs = 2**31 * 8 / ((time.perf_counter() - t0) / 10) / 1e9
-        del x
-        torch.cuda.empty_cache()
-        return gbs
-    except Exception:  # noqa: BLE001 - a reference number only
-        return None
+def spmv_stream_floor(ctx, N, row_len, reps=10):
+    """The roofline kernel's own streaming floor: the sliced SpMV on a matrix
+    of the C2 shape (N rows of row_len entries) whose column indices all point
+    at x[0], so every x gather hits cache and only the 12 streamed bytes per
+    entry remain. Returns (ms per launch, GB/s of its algorithmic bytes)."""
+    nnz = N * row_len
+    rowptr = np.arange(0, nnz + 1, row_len, dtype=np.int64)
+    colind = np.zeros(nnz, dtype=np.int32)
+    nzval = np.ones(nnz, dtype=np.float64)
+    A = ctx.csr_from_host(N, N, rowptr, colind, nzval)
+    del colind, nzval
+    assert ctx.prepare_spmv(A, always=True)
+    x = ctx.to_device(np.ones(N))
+    y = ctx.empty(N, np.float64)
+    ctx.spmv(A, 1, 1.0, x, N, 0.0, y, N)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.spmv(A, 1, 1.0, x, N, 0.0, y, N)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    del A, x, y
+    return ms, spmv_bytes(nnz, N, True) / (ms * 1e-3) / 1e9
 
 
 def c1_section(C, ctx):
@@ -835,11 +836,12 @@ def main():
         },
     }
     out["rdms"] = rdm_timing(ctx, T, V, d_dets, N, nnz, d_X, E)
-    out["spmv"]["stream_read_GBs"] = stream_read_ceiling()
-    if out["spmv"]["stream_read_GBs"]:
-        out["spmv"]["frac_of_stream_read"] = (
-            SPMV_TRAFFIC_GB * 1e9 / (avg_spmv_ms * 1e-3) / 1e9 / out["spmv"]["stream_read_GBs"]
-            if SPMV_TRAFFIC_GB and sliced else None)
+    if sliced and nnz % N == 0:
+        # every row of the complete space has the same length
+        fl_ms, fl_gbs = spmv_stream_floor(ctx, N, nnz // N)
+        out["spmv"]["stream_floor_ms"] = fl_ms
+        out["spmv"]["stream_floor_GBs"] = fl_gbs
+        out["spmv"]["frac_of_stream_floor"] = fl_ms / avg_spmv_ms
     if not args.no_asci:
         out["mlp"] = mlp_throughput(C, ctx)
         out["asci"] = asci_step_comparison(C, ctx)
