@@ -815,6 +815,10 @@ def main():
             "algorithmic_GB": bytes_spmv / 1e9,
             "hbm_GBps": spmv_gbs,
             "frac_of_peak": spmv_gbs / HBM_PEAK_GBS,
+            # what the layout actually streams: 16-bit column offsets in the
+            # narrow slices of the sliced copy (x read and y written added)
+            "streamed_GB": (last["dav_spmv_matrix_bytes"] + 2 * N * 8) / 1e9,
+            "streamed_GBps": (last["dav_spmv_matrix_bytes"] + 2 * N * 8) / (avg_spmv_ms * 1e6),
         },
         "davidson": {
             "ms_per_solve": dav_ms / args.steps,

@@ -95,6 +95,11 @@ typedef struct mhip_timings {
   int64_t dav_sliced;     /* 1: its SpMVs ran on the sliced copy, 2: on column panels */
   int64_t asci_nblocks;      /* excitation blocks of the core (block-screened search) */
   int64_t asci_nblocks_kept; /* blocks whose candidates were generated */
+  /* matrix bytes one SpMV of the last Davidson streams from its layout: the
+   * sliced copy's values, column words (16-bit offsets in narrow slices,
+   * 32-bit columns elsewhere) and slot bases, or the CSR's 12 bytes per
+   * nonzero and its row pointers (x and y not counted) */
+  double dav_spmv_matrix_bytes;
 } mhip_timings;
 
 /* Structure passes of the Hamiltonian build. AUTO takes the product-space

@@ -58,6 +58,7 @@ class Timings(C.Structure):
         ("dav_sliced", C.c_int64),
         ("asci_nblocks", C.c_int64),
         ("asci_nblocks_kept", C.c_int64),
+        ("dav_spmv_matrix_bytes", C.c_double),
     ]
 
     def as_dict(self):

@@ -678,6 +678,10 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
     const auto h0 = std::chrono::steady_clock::now();
     tm.dav_sliced = prepare_sell(ctx, A) ? 1 : prepare_panels(ctx, A) ? 2 : 0;
     if(!had && tm.dav_sliced) MHIP_HIP_CHECK(hipStreamSynchronize(st));
+    tm.dav_spmv_matrix_bytes =
+        A->sell ? A->sell->streamed_bytes()
+                : 12. * (double)A->nnz + 8. * (double)(A->m + 1) +
+                      (A->panels ? 4. * (double)A->m * A->panels->np : 0.);
     tm.dav_layout_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
   }
@@ -878,6 +882,7 @@ int mhip_selected_ci_diag(mhip_context* ctx, int words_per_det,
       if(H->sell) {
         keep_buffer(ctx, std::move(H->sell->cols));
         keep_buffer(ctx, std::move(H->sell->vals));
+        keep_buffer(ctx, std::move(H->sell->base));
       }
     }
     if(!ok) throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");

@@ -170,14 +170,29 @@ struct Timer {
 namespace mhip {
 // Sliced (SELL-64) copy of a square CSR matrix for the single-vector SpMV:
 // slice s holds rows [64 s, 64 s + 64), its entries stored slot-major
-// (entry j of row 64 s + l at soff[s] + 64 j + l), every row padded to the
-// slice's longest row with (column = row, value = 0); columns 0-based.
+// (value of entry j of row 64 s + l at soff[s] + 64 j + l), every row
+// padded to the slice's longest row with value 0; columns 0-based.
+// A slice whose columns at every slot lie within 65,536 of each other is
+// stored narrow: per slot a base column (the smallest at that slot) and per
+// entry a 16-bit offset from it, two slots packed per 32-bit word (word
+// soff[s] + 64 p + l holds slot 2p in its low half and slot 2p + 1 in its
+// high half for row 64 s + l), padding at offset 0. Other slices keep
+// 32-bit columns at soff[s] + 64 j + l, padding at column = row.
 struct SellCopy {
   int64_t nsl = 0, entries = 0;
+  int64_t narrow_entries = 0;  // entries of the narrow slices
   DevBuf meta;        // int64 soff[nsl], total, then int32 slen[nsl]
-  DevBuf cols, vals;  // int32 / double [entries]
+  DevBuf cols, vals;  // column words (4 bytes reserved per entry) / double [entries]
+  DevBuf narrow;      // int32[nsl]: 1 when slice s is stored narrow
+  DevBuf base;        // int32[entries / 64]: slot bases, slot j of slice s at soff[s] / 64 + j
   const int64_t* soff() const { return meta.as<int64_t>(); }
   const int32_t* slen() const { return reinterpret_cast<const int32_t*>(soff() + nsl + 1); }
+  // bytes one single-vector SpMV streams from the copy: values, column
+  // words and the narrow slices' slot bases
+  double streamed_bytes() const {
+    const double nw = (double)narrow_entries;
+    return 8. * (double)entries + 4. * ((double)entries - nw) + 2. * nw + nw / 16.;
+  }
 };
 // Column-panel split of a CSR for single-vector SpMVs whose x vector
 // outgrows one XCD's L2: columns [bounds[p], bounds[p+1]) form panel p

@@ -318,18 +318,41 @@ __global__ __launch_bounds__(1024) void sell_offsets_kernel(int64_t nsl,
   if(t == 1023) *total = part[1023];
 }
 
+// *total = entries of the narrow slices (one block).
+__global__ __launch_bounds__(1024) void sell_narrow_count_kernel(int64_t nsl,
+                                                                 const int32_t* __restrict__ slen,
+                                                                 const int32_t* __restrict__ narrow,
+                                                                 int64_t* __restrict__ total) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  int64_t sum = 0;
+  for(int64_t i = t; i < nsl; i += 1024) sum += narrow[i] ? (int64_t)slen[i] * kSliceRows : 0;
+  part[t] = sum;
+  __syncthreads();
+  for(int off = 512; off > 0; off >>= 1) {
+    if(t < off) part[t] += part[t + off];
+    __syncthreads();
+  }
+  if(t == 0) *total = part[0];
+}
+
 // Copy of slice blockIdx.x, kSellChunk slots at a time: rows are read
-// contiguously into an LDS tile and written back slot-major. Padding
-// entries are (column = the row itself, value 0); past the last row,
-// (0, 0).
+// contiguously into an LDS tile and written back slot-major. A first pass
+// over the slice's columns finds every slot's smallest and largest column
+// (padding excluded), stores the smallest as the slot base and decides the
+// slice's width: narrow when every slot spans less than 65,536 columns.
+// Padding entries get value 0 and, narrow, offset 0 (the base column),
+// wide, the row itself as column (past the last row, column 0).
 __global__ __launch_bounds__(256) void sell_fill_kernel(
     int64_t m, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const double* __restrict__ nzval, int64_t indexing, const int64_t* __restrict__ soff,
-    const int32_t* __restrict__ slen, int32_t* __restrict__ cols, double* __restrict__ vals) {
+    const int32_t* __restrict__ slen, int32_t* __restrict__ cols, double* __restrict__ vals,
+    int32_t* __restrict__ narrow, int32_t* __restrict__ sbase) {
   __shared__ int64_t rst[kSliceRows];
   __shared__ int32_t rlen[kSliceRows], rid[kSliceRows];
   __shared__ int32_t tc[kSellChunk][kSliceRows + 1];
   __shared__ double tv[kSellChunk][kSliceRows + 1];
+  __shared__ int32_t tb[kSellChunk];
   const int64_t s = blockIdx.x;
   const int t = threadIdx.x;
   if(t < kSliceRows) {
@@ -341,10 +364,41 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(
   __syncthreads();
   const int L = slen[s];
   const int64_t base = soff[s];
+  int32_t* __restrict__ bs = sbase + base / kSliceRows;
+  // pass 1: slot bases and spans; thread t takes slot t / 8 of the chunk and
+  // rows 8 (t % 8) .. 8 (t % 8) + 7
+  int wide = 0;
   for(int j0 = 0; j0 < L; j0 += kSellChunk) {
+    const int jj = t >> 3, j = j0 + jj, r0 = (t & 7) * 8;
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    if(j < L) {
+      for(int r = r0; r < r0 + 8; ++r) {
+        if(j < rlen[r]) {
+          const int32_t c = (int32_t)(colind[rst[r] + j] - indexing);
+          lo = c < lo ? c : lo;
+          hi = c > hi ? c : hi;
+        }
+      }
+    }
+#pragma unroll
+    for(int off = 1; off < 8; off <<= 1) {
+      const int32_t ol = __shfl_xor(lo, off, 64), oh = __shfl_xor(hi, off, 64);
+      lo = ol < lo ? ol : lo;
+      hi = oh > hi ? oh : hi;
+    }
+    if(j < L && (t & 7) == 0) {
+      bs[j] = lo;  // every slot below L holds at least one entry
+      if((int64_t)hi - (int64_t)lo >= 65536) wide = 1;
+    }
+  }
+  const bool nar = !__syncthreads_or(wide);
+  if(t == 0) narrow[s] = nar ? 1 : 0;
+  uint32_t* __restrict__ cw = reinterpret_cast<uint32_t*>(cols) + base;
+  for(int j0 = 0; j0 < L; j0 += kSellChunk) {
+    if(nar && t < kSellChunk) tb[t] = j0 + t < L ? bs[j0 + t] : 0;
     for(int e = t; e < kSliceRows * kSellChunk; e += 256) {
       const int r = e / kSellChunk, jj = e % kSellChunk, j = j0 + jj;
-      int32_t c = rid[r];
+      int32_t c = nar ? -1 : rid[r];  // narrow padding: -1 marks offset 0
       double v = 0.;
       if(j < rlen[r]) {
         c = (int32_t)(colind[rst[r] + j] - indexing);
@@ -354,12 +408,26 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(
       tv[jj][r] = v;
     }
     __syncthreads();
+    if(nar) {
+      // one 32-bit word per row and slot pair (kSellChunk is even)
+      for(int e = t; e < kSliceRows * kSellChunk / 2; e += 256) {
+        const int pj = e / kSliceRows, r = e % kSliceRows, jj = 2 * pj, j = j0 + jj;
+        if(j < L) {
+          const int32_t c0 = tc[jj][r], c1 = j + 1 < L ? tc[jj + 1][r] : -1;
+          const uint32_t o0 = c0 < 0 ? 0u : (uint32_t)(c0 - tb[jj]);
+          const uint32_t o1 = c1 < 0 ? 0u : (uint32_t)(c1 - tb[jj + 1]);
+          cw[(int64_t)(j >> 1) * kSliceRows + r] = o0 | (o1 << 16);
+        }
+      }
+    } else {
+      for(int e = t; e < kSliceRows * kSellChunk; e += 256) {
+        const int jj = e / kSliceRows, r = e % kSliceRows, j = j0 + jj;
+        if(j < L) cols[base + (int64_t)j * kSliceRows + r] = tc[jj][r];
+      }
+    }
     for(int e = t; e < kSliceRows * kSellChunk; e += 256) {
       const int jj = e / kSliceRows, r = e % kSliceRows, j = j0 + jj;
-      if(j < L) {
-        cols[base + (int64_t)j * kSliceRows + r] = tc[jj][r];
-        vals[base + (int64_t)j * kSliceRows + r] = tv[jj][r];
-      }
+      if(j < L) vals[base + (int64_t)j * kSliceRows + r] = tv[jj][r];
     }
     __syncthreads();
   }
@@ -374,11 +442,14 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(
 __global__ __launch_bounds__(256) void spmv_sell_kernel(
     int64_t m, int64_t nsl, int64_t per_xcd, const int64_t* __restrict__ soff,
     const int32_t* __restrict__ slen, const int32_t* __restrict__ cols,
-    const double* __restrict__ vals, double alpha, const double* __restrict__ x, double beta,
+    const double* __restrict__ vals, const int32_t* __restrict__ narrow,
+    const int32_t* __restrict__ sbase, double alpha, const double* __restrict__ x, double beta,
     double* __restrict__ y, const int32_t* __restrict__ row_inst,
     const int32_t* __restrict__ conv) {
   const int64_t lblk = (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-  const int64_t s = lblk * kWavesPerBlock + threadIdx.x / 64;
+  // the slice index is uniform over the wavefront: slice data and slot
+  // bases come through scalar loads
+  const int64_t s = lblk * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   if(s >= nsl) return;
   const int lane = threadIdx.x & 63;
   bool active = s * kSliceRows + lane < m;
@@ -387,24 +458,52 @@ __global__ __launch_bounds__(256) void spmv_sell_kernel(
     if(!ballot(active)) return;  // whole wavefront: nothing to do in this slice
   }
   const int len = slen[s];
-  const int32_t* c = cols + soff[s] + lane;
-  const double* v = vals + soff[s] + lane;
+  const int64_t so = soff[s];
+  const double* v = vals + so + lane;
   double a[kSellUnroll];
 #pragma unroll
   for(int u = 0; u < kSellUnroll; ++u) a[u] = 0.;
   int j = 0;
-  for(; j + kSellUnroll <= len; j += kSellUnroll) {
-    int32_t cc[kSellUnroll];
-    double vv[kSellUnroll];
+  if(narrow[s]) {
+    // 16-bit offsets from the slot bases, two slots per 32-bit word
+    const uint32_t* c = reinterpret_cast<const uint32_t*>(cols) + so + lane;
+    const int32_t* b = sbase + so / kSliceRows;
+    for(; j + kSellUnroll <= len; j += kSellUnroll) {
+      uint32_t w[kSellUnroll / 2];
+      double vv[kSellUnroll];
 #pragma unroll
-    for(int u = 0; u < kSellUnroll; ++u) {
-      cc[u] = __builtin_nontemporal_load(c + (int64_t)(j + u) * kSliceRows);
-      vv[u] = __builtin_nontemporal_load(v + (int64_t)(j + u) * kSliceRows);
+      for(int u = 0; u < kSellUnroll / 2; ++u)
+        w[u] = __builtin_nontemporal_load(c + (int64_t)((j >> 1) + u) * kSliceRows);
+#pragma unroll
+      for(int u = 0; u < kSellUnroll; ++u)
+        vv[u] = __builtin_nontemporal_load(v + (int64_t)(j + u) * kSliceRows);
+#pragma unroll
+      for(int u = 0; u < kSellUnroll; ++u) {
+        const int32_t cc = b[j + u] + (int32_t)((u & 1) ? w[u >> 1] >> 16 : w[u >> 1] & 0xffffu);
+        a[u] = fma(vv[u], x[cc], a[u]);
+      }
     }
+    for(; j < len; ++j) {
+      const uint32_t w1 = c[(int64_t)(j >> 1) * kSliceRows];
+      const int32_t cc = b[j] + (int32_t)((j & 1) ? w1 >> 16 : w1 & 0xffffu);
+      a[0] = fma(v[(int64_t)j * kSliceRows], x[cc], a[0]);
+    }
+  } else {
+    const int32_t* c = cols + so + lane;
+    for(; j + kSellUnroll <= len; j += kSellUnroll) {
+      int32_t cc[kSellUnroll];
+      double vv[kSellUnroll];
 #pragma unroll
-    for(int u = 0; u < kSellUnroll; ++u) a[u] = fma(vv[u], x[cc[u]], a[u]);
+      for(int u = 0; u < kSellUnroll; ++u) {
+        cc[u] = __builtin_nontemporal_load(c + (int64_t)(j + u) * kSliceRows);
+        vv[u] = __builtin_nontemporal_load(v + (int64_t)(j + u) * kSliceRows);
+      }
+#pragma unroll
+      for(int u = 0; u < kSellUnroll; ++u) a[u] = fma(vv[u], x[cc[u]], a[u]);
+    }
+    for(; j < len; ++j)
+      a[0] = fma(v[(int64_t)j * kSliceRows], x[c[(int64_t)j * kSliceRows]], a[0]);
   }
-  for(; j < len; ++j) a[0] = fma(v[(int64_t)j * kSliceRows], x[c[(int64_t)j * kSliceRows]], a[0]);
   const double acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   const int64_t row = s * kSliceRows + lane;
   if(active) y[row] = alpha * acc + beta * y[row];
@@ -446,10 +545,16 @@ bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding) {
     (void)hipGetLastError();
     return false;
   }
+  S->narrow = take_buffer(ctx, 4 * (size_t)S->nsl);
+  S->base = take_buffer(ctx, 4 * (size_t)std::max<int64_t>(1, S->entries / kSliceRows));
   hipLaunchKernelGGL(sell_fill_kernel, dim3(S->nsl), dim3(256), 0, st, A->m,
                      A->rowptr.as<int64_t>(), A->colind.as<int32_t>(), A->nzval.as<double>(),
-                     A->indexing, soff, slen, S->cols.as<int32_t>(), S->vals.as<double>());
+                     A->indexing, soff, slen, S->cols.as<int32_t>(), S->vals.as<double>(),
+                     S->narrow.as<int32_t>(), S->base.as<int32_t>());
+  hipLaunchKernelGGL(sell_narrow_count_kernel, dim3(1), dim3(1024), 0, st, S->nsl, slen,
+                     S->narrow.as<int32_t>(), d_total);
   MHIP_HIP_CHECK(hipGetLastError());
+  d2h(ctx, &S->narrow_entries, d_total, 8);
   A->sell = std::move(S);
   return true;
 }
@@ -532,7 +637,8 @@ void spmv(mhip_context* ctx, const mhip_csr* A, double alpha, const double* x,
     const int64_t per = (nblk + kXcds - 1) / kXcds;
     hipLaunchKernelGGL(spmv_sell_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m,
                        S.nsl, per, S.soff(), S.slen(), S.cols.as<int32_t>(),
-                       S.vals.as<double>(), alpha, x, beta, y, nullptr, nullptr);
+                       S.vals.as<double>(), S.narrow.as<int32_t>(), S.base.as<int32_t>(), alpha,
+                       x, beta, y, nullptr, nullptr);
     MHIP_HIP_CHECK(hipGetLastError());
     return;
   }
@@ -550,8 +656,8 @@ void spmv_sliced_masked(mhip_context* ctx, const mhip_csr* A, const double* x, d
   const int64_t nblk = (S.nsl + kWavesPerBlock - 1) / kWavesPerBlock;
   const int64_t per = (nblk + kXcds - 1) / kXcds;
   hipLaunchKernelGGL(spmv_sell_kernel, dim3(per * kXcds), dim3(256), 0, ctx->stream, A->m, S.nsl,
-                     per, S.soff(), S.slen(), S.cols.as<int32_t>(), S.vals.as<double>(), 1., x, 0.,
-                     y, row_inst, conv);
+                     per, S.soff(), S.slen(), S.cols.as<int32_t>(), S.vals.as<double>(),
+                     S.narrow.as<int32_t>(), S.base.as<int32_t>(), 1., x, 0., y, row_inst, conv);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 

@@ -146,6 +146,75 @@ def test_spmv_sliced_copy_rules(ctx):
     assert np.max(np.abs(y0.numpy() - ref)) <= tol
 
 
+def test_spmv_sliced_narrow_and_wide_slices(ctx):
+    """The sliced copy stores a slice's columns as 16-bit offsets from
+    per-slot bases when every slot spans fewer than 65,536 columns and as
+    32-bit columns otherwise. Banded rows of a 131,072-column matrix (all
+    slices narrow) against the same matrix with a far zero-valued entry
+    appended to the shortest row of every other slice (most of those slices
+    wide; the slot counts and summation order unchanged): the products
+    agree bit for bit and with the oracle's gespmbv; slots that span
+    exactly 65,535 (narrow) and 65,536 (wide) columns are present."""
+    announce("spmv sliced narrow / wide slices")
+    rng = np.random.default_rng(11)
+    m = n = 131072
+    lens = rng.integers(0, 40, m)
+
+    def row_cols(i, L):
+        lo, hi = max(0, i - 20000), min(n, i + 20000)
+        return np.unique(rng.integers(lo, hi, L))
+
+    cols = [row_cols(i, L) for i, L in enumerate(lens)]
+    vals = [rng.standard_normal(len(c)) for c in cols]
+    # boundary spans: in slice 100, slot 0 spans 65,535 columns; in slice
+    # 300, 65,536 (row 64 s + 1 starts at the slot's smallest column + span)
+    for s, span in ((100, 65535), (300, 65536)):
+        r0, r1 = 64 * s, 64 * s + 1
+        c0 = 1000
+        cols[r0] = np.array([c0, c0 + 5])
+        cols[r1] = np.array([c0 + span, c0 + span + 3])
+        vals[r0] = rng.standard_normal(2)
+        vals[r1] = rng.standard_normal(2)
+        for r in range(64 * s + 2, 64 * s + 64):
+            cols[r] = np.array([c0 + 10 + r % 50, c0 + 100 + r % 50])
+            vals[r] = rng.standard_normal(2)
+
+    def csr(cols, vals):
+        lens = np.array([len(c) for c in cols])
+        rp = np.zeros(m + 1, dtype=np.int64)
+        rp[1:] = np.cumsum(lens)
+        return rp, np.concatenate(cols).astype(np.int32), np.concatenate(vals)
+
+    rp, ci, nz = csr(cols, vals)
+    cols_w, vals_w = list(cols), list(vals)
+    for s in range(0, m // 64, 2):
+        # the slice's shortest row, so its slot count (and every row's
+        # summation order) stays; the new entry replaces a zero padding term
+        ls = [len(cols[r]) for r in range(64 * s, 64 * s + 64)]
+        r = 64 * s + int(np.argmin(ls))
+        if min(ls) < max(ls) and (len(cols[r]) == 0 or cols[r][-1] < n - 1):
+            cols_w[r] = np.append(cols[r], n - 1)
+            vals_w[r] = np.append(vals[r], 0.0)
+    rpw, ciw, nzw = csr(cols_w, vals_w)
+    A = ctx.csr_from_host(m, n, rp, ci, nz)
+    B = ctx.csr_from_host(m, n, rpw, ciw, nzw)
+    assert ctx.prepare_spmv(A, always=True) and ctx.prepare_spmv(B, always=True)
+    x = rng.standard_normal(n)
+    dx = ctx.to_device(x)
+    ya, yb = ctx.empty(m, np.float64), ctx.empty(m, np.float64)
+    ctx.spmv(A, 1, 1.0, dx, n, 0.0, ya, m)
+    ctx.spmv(B, 1, 1.0, dx, n, 0.0, yb, m)
+    ref = np.zeros(m)
+    O.gespmbv(1, 1.0, m, rp, ci, nz, 0, x, n, 0.0, ref, m)
+    got = ya.numpy()
+    assert np.array_equal(got, yb.numpy())
+    tol = 1e-14 * np.abs(nz).max() * np.abs(x).max() * 64
+    assert np.max(np.abs(got - ref)) <= tol
+    for s in (100, 300):  # the boundary rows, exactly as the oracle sums two terms
+        for r in (64 * s, 64 * s + 1):
+            assert abs(got[r] - ref[r]) <= 1e-15 * (abs(ref[r]) + 1)
+
+
 @pytest.mark.parametrize("k,m", [(1, 1), (5, 1), (16, 16), (17, 17), (21, 21), (3, 29)])
 def test_gemm_tn_mfma(ctx, k, m):
     announce(f"gemm_tn k={k} m={m}")
