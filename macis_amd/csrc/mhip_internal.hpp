@@ -325,6 +325,7 @@ struct ProductBatch {
   std::vector<int> space_of;  // instance -> spaces index
   DevBuf tables, views, run_off, inner_off;
   int ninst = 0;
+  int64_t max_seg = 0;  // most outer neighbours of a run over the instances
   int64_t total_runs = 0, total_inner = 0;
 };
 // list_id[i]: the first instance whose determinant list equals instance i's

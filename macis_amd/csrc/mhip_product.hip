@@ -239,26 +239,51 @@ __device__ __forceinline__ double code_sign(int32_t code) {
 
 // Fill and elements in one pass, one wavefront per row. Lanes take
 // consecutive output positions of the row (coalesced column and value
-// stores). The row's segment starts (prefix sums of the segment lengths over
-// its outer neighbours: 1 at level 2, the level-<=1 inner list at level 1,
-// the level-<=2 list at level 0) are staged in LDS; a lane finds its segment
-// by binary search, then its inner string. Elements are the Slater-Condon
-// values of the tiled path's element pass (same device function, same
-// diagonal), so the CSR is bit-identical; |h| > thresh is counted per row
-// for the shared compaction.
+// stores). The row's segments (one per outer neighbour: 1 entry at level 2,
+// the level-<=1 inner list at level 1, the level-<=2 list at level 0) are
+// staged in LDS with their start (prefix sums of the lengths), packed
+// neighbour, excitation code and a value: at level 2 the same-spin double
+// element, at level 1 the element of the segment's entry with the row's own
+// inner string (me_2 with that string as spectator), so the window loop
+// reads it from LDS instead of summing integrals for one or two lanes of a
+// window. A lane finds its segment from a 64-bit mask of the segment starts
+// inside its 64-position window; the level-1 inner list of the row (at most
+// 64 entries: registers, read across lanes) gives the inner neighbour and
+// the inner single of an alpha-beta double. Elements are the Slater-Condon
+// values of the tiled path's element pass (the same operations in the same
+// order), so the CSR is bit-identical; |h| > thresh is counted per row for
+// the shared compaction. LDS per wavefront: 18 bytes per outer neighbour
+// (max_seg of the launch) + 4, so that at CAS(12e,12o) (262 outer
+// neighbours) the 62-VGPR kernel keeps 8 wavefronts per SIMD.
+__host__ __device__ constexpr size_t fill_lds_per_wave(int max_seg) {
+  return ((size_t)18 * max_seg + 4 + 7) / 8 * 8;
+}
+
+// A packed single (o | v << 8 | (sign < 0) << 16) in 16 bits (orbitals < 64)
+// and back.
+__device__ __forceinline__ int16_t code16(int32_t c) {
+  return (int16_t)((c & 0x3f) | (((c >> 8) & 0x3f) << 6) | (((c >> 16) & 1) << 12));
+}
+__device__ __forceinline__ int32_t code32(int16_t x) {
+  return (x & 0x3f) | (((x >> 6) & 0x3f) << 8) | (((x >> 12) & 1) << 16);
+}
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     PSView one, const PSView* __restrict__ many, const int32_t* __restrict__ row_inst,
-    int64_t n, const double* __restrict__ diag, const int64_t* __restrict__ rowptr,
+    int64_t n, int max_seg, const double* __restrict__ diag, const int64_t* __restrict__ rowptr,
     int32_t* __restrict__ colind, double* __restrict__ nzval, double thresh,
     int64_t* __restrict__ kept, int* __restrict__ dropped) {
-  __shared__ int32_t seg_start[kWavesPerBlock][kMaxSeg + 1];
-  __shared__ int32_t seg_code[kWavesPerBlock][kMaxSeg];
+  extern __shared__ double fill_lds[];
   const int wv = threadIdx.x >> 6;
   const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + wv;
   if(row >= n) return;  // whole wavefront leaves together
   const int lane = threadIdx.x & 63;
+  // this wavefront's staging: values, then starts, packed neighbours, codes
+  double* seg_val = fill_lds + wv * (fill_lds_per_wave(max_seg) / 8);
+  int32_t* seg_start = reinterpret_cast<int32_t*>(seg_val + max_seg);
+  int32_t* seg_code = seg_start + max_seg + 1;
+  int16_t* seg_exc = reinterpret_cast<int16_t*>(seg_code + max_seg);
   const PSView& P = BATCH ? many[row_inst[row]] : one;
   const IntsView& I = P.I;
   const int64_t L = P.L, row0 = P.row0;
@@ -283,7 +308,14 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   const int32_t s1 = ptr1[k], len1 = ptr1[k + 1] - s1;
   const int32_t s2 = ptr2[k], len2 = ptr2[k + 1] - s2;
   const int32_t j0 = onb_ptr[r], nseg = onb_ptr[r + 1] - j0;
-  // segment starts (exclusive prefix of lengths) and codes into LDS
+  // the level-<=1 inner list in registers (lane t: entry t) when it fits
+  const bool inreg = len1 <= 64;
+  int32_t reg_k1 = 0, reg_c1 = 0;
+  if(inreg && lane < len1) {
+    reg_k1 = idx1[s1 + lane];
+    reg_c1 = iexc1[s1 + lane];
+  }
+  // segment starts (exclusive prefix of lengths), neighbours, codes, values
   int32_t carry = 0;
   for(int32_t b = 0; b < nseg; b += 64) {
     const int32_t j = b + lane;
@@ -298,12 +330,26 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
       if(lane >= off) incl += v;
     }
     if(valid) {
-      seg_start[wv][j] = carry + incl - len;
-      seg_code[wv][j] = packed;
+      seg_start[j] = carry + incl - len;
+      seg_code[j] = packed;
+      int32_t oc = 0;
+      double val = 0.;
+      if(e == 2) {  // outer double, same inner string: me_4 of the outer pair
+        val = oval[j0 + j];
+      } else if(e == 1) {  // outer single, same inner string: me_2, other spin = inner
+        oc = oexc[j0 + j];
+        const uint32_t o = oc & 0xff, v = (oc >> 8) & 0xff;
+        const double* V_ov = I.V_red + I.n * (v + I.n * o);
+        double t = oval[j0 + j];
+        for(uint64_t m = bi; m; m &= m - 1) t += V_ov[ctz64(m)];
+        val = code_sign(oc) * t;
+      }
+      seg_exc[j] = code16(oc);
+      seg_val[j] = val;
     }
     carry += __shfl(incl, 63, 64);
   }
-  if(lane == 0) seg_start[wv][nseg] = carry;
+  if(lane == 0) seg_start[nseg] = carry;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -313,91 +359,72 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   // the next ones in order (jlo, jlo + 1, ...); their start offsets form a
   // 64-bit mask, and a lane's segment is jlo - 1 + the starts at or below it.
   int32_t jlo = 0;
-  // kU windows per iteration: their element chains (list, string, integral
-  // loads) are independent and overlap
-  constexpr int kU = 1;
-  for(int32_t pb = 0; pb < total; pb += 64 * kU) {
-    int32_t lo[kU];
+  for(int32_t p0 = 0; p0 < total; p0 += 64) {
+    const int32_t js = jlo + lane;
+    const int32_t sstart = js < nseg ? seg_start[js] : INT32_MAX;
+    const bool inwin = sstart < p0 + 64;
+    uint64_t startmask = inwin ? (1ull << (sstart - p0)) : 0ull;
 #pragma unroll
-    for(int u = 0; u < kU; ++u) {
-      const int32_t p0 = pb + 64 * u;
-      const int32_t js = jlo + lane;
-      const int32_t sstart = js < nseg ? seg_start[wv][js] : INT32_MAX;
-      const bool inwin = sstart < p0 + 64;
-      uint64_t startmask = inwin ? (1ull << (sstart - p0)) : 0ull;
-#pragma unroll
-      for(int off = 1; off < 64; off <<= 1) startmask |= __shfl_xor(startmask, off, 64);
-      lo[u] = jlo - 1 + (int32_t)popc64(startmask & lowmask((uint32_t)lane + 1u));
-      jlo += (int32_t)popc64(ballot(inwin));
+    for(int off = 1; off < 64; off <<= 1) startmask |= __shfl_xor(startmask, off, 64);
+    const int32_t lo = jlo - 1 + (int32_t)popc64(startmask & lowmask((uint32_t)lane + 1u));
+    jlo += (int32_t)popc64(ballot(inwin));
+    const int32_t p = p0 + lane;
+    const bool act = p < total;
+    const int32_t l = act ? lo : 0;
+    const int32_t packed = seg_code[l];
+    const int32_t off = p - seg_start[l];
+    const int64_t r2 = packed >> 2;
+    const int e = packed & 3;
+    const double sv = seg_val[l];
+    const int32_t oc = code32(seg_exc[l]);
+    // the lane's entry of the level-<=1 inner list (meaningful at level 1)
+    int32_t k1 = __shfl(reg_k1, off & 63, 64);
+    int32_t c1 = __shfl(reg_c1, off & 63, 64);
+    if(!inreg && act && e == 1) {
+      k1 = idx1[s1 + off];
+      c1 = iexc1[s1 + off];
     }
-    int32_t k2[kU];
-    int64_t r2[kU];
-    double h[kU];
-#pragma unroll
-    for(int u = 0; u < kU; ++u) {
-      const int32_t p = pb + 64 * u + lane;
-      const int32_t l = p < total ? lo[u] : 0;
-      const int32_t packed = seg_code[wv][l];
-      const int32_t off = p - seg_start[wv][l];
-      const int32_t j = j0 + l;  // outer-neighbour entry of the segment
-      r2[u] = packed >> 2;
-      const int e = packed & 3;
-      double hv = d;
-      int32_t kk = (int32_t)k;
-      if(p < total) {
-        if(e == 2) {  // outer double, same inner string: me_4 of the outer pair
-          hv = oval[j];
-        } else if(e == 1) {
-          kk = idx1[s1 + off];
-          const int32_t oc = oexc[j];
-          const int32_t ic = iexc1[s1 + off];
-          if(ic < 0) {  // outer single, same inner string: me_2, other spin = inner
-            const uint32_t o = oc & 0xff, v = (oc >> 8) & 0xff;
-            const double* V_ov = I.V_red + I.n * (v + I.n * o);
-            double t = oval[j];
-            for(uint64_t m = bi; m; m &= m - 1) t += V_ov[ctz64(m)];
-            hv = code_sign(oc) * t;
-          } else {  // outer single x inner single: me_22 (alpha excitation first)
-            const int32_t ac = outer_alpha ? oc : ic, bc = outer_alpha ? ic : oc;
-            const uint32_t o1 = ac & 0xff, v1 = (ac >> 8) & 0xff;
-            const uint32_t o2 = bc & 0xff, v2 = (bc >> 8) & 0xff;
-            const double sign = code_sign(ac) * code_sign(bc);
-            // outer alpha: (o1, v1) is fixed over the segment and the lanes
-            // vary (o2, v2); with pair symmetry the swapped entry
-            // V(v2,o2,v1,o1) keeps the lanes inside one n*n block
-            hv = sign * (outer_alpha && I.pair_sym
-                             ? I.V[v2 + I.n * (o2 + I.n * (v1 + I.n * o1))]
-                             : I.V[v1 + I.n * (o1 + I.n * (v2 + I.n * o2))]);
-          }
-        } else {
-          const int32_t t2 = s2 + off;
-          kk = idx2[t2];
-          const int32_t ic = iexc2[t2];
-          const int lvl = ic >> 24;
-          if(lvl == 2) {  // inner double: me_4 of the inner pair
-            hv = ival2[t2];
-          } else if(lvl == 1) {  // inner single: me_2, other spin = outer
-            const uint32_t o = ic & 0xff, v = (ic >> 8) & 0xff;
-            const double* V_ov = I.V_red + I.n * (v + I.n * o);
-            double t = ival2[t2];
-            for(uint64_t m = bo; m; m &= m - 1) t += V_ov[ctz64(m)];
-            hv = code_sign(ic) * t;
-          }
+    double hv = d;
+    int32_t kk = (int32_t)k;
+    if(act) {
+      if(e == 2) {
+        hv = sv;
+      } else if(e == 1) {
+        kk = k1;
+        if(c1 < 0) {
+          hv = sv;
+        } else {  // outer single x inner single: me_22 (alpha excitation first)
+          const int32_t ac = outer_alpha ? oc : c1, bc = outer_alpha ? c1 : oc;
+          const uint32_t o1 = ac & 0xff, v1 = (ac >> 8) & 0xff;
+          const uint32_t o2 = bc & 0xff, v2 = (bc >> 8) & 0xff;
+          const double sign = code_sign(ac) * code_sign(bc);
+          // outer alpha: (o1, v1) is fixed over the segment and the lanes
+          // vary (o2, v2); with pair symmetry the swapped entry
+          // V(v2,o2,v1,o1) keeps the lanes inside one n*n block
+          hv = sign * (outer_alpha && I.pair_sym
+                           ? I.V[v2 + I.n * (o2 + I.n * (v1 + I.n * o1))]
+                           : I.V[v1 + I.n * (o1 + I.n * (v2 + I.n * o2))]);
+        }
+      } else {
+        const int32_t t2 = s2 + off;
+        kk = idx2[t2];
+        const int32_t ic = iexc2[t2];
+        const int lvl = ic >> 24;
+        if(lvl == 2) {  // inner double: me_4 of the inner pair
+          hv = ival2[t2];
+        } else if(lvl == 1) {  // inner single: me_2, other spin = outer
+          const uint32_t o = ic & 0xff, v = (ic >> 8) & 0xff;
+          const double* V_ov = I.V_red + I.n * (v + I.n * o);
+          double t = ival2[t2];
+          for(uint64_t m = bo; m; m &= m - 1) t += V_ov[ctz64(m)];
+          hv = code_sign(ic) * t;
         }
       }
-      k2[u] = kk;
-      h[u] = hv;
-    }
-#pragma unroll
-    for(int u = 0; u < kU; ++u) {
-      const int32_t p = pb + 64 * u + lane;
-      if(p < total) {
-        // streamed out once: non-temporal, so the 19 GB of stores of a large
-        // build do not evict the element tables and integrals from L2
-        __builtin_nontemporal_store((int32_t)(row0 + r2[u] * L + k2[u]), colind + st + p);
-        __builtin_nontemporal_store(h[u], nzval + st + p);
-        nk += fabs(h[u]) > thresh ? 1 : 0;
-      }
+      // streamed out once: non-temporal, so the 19 GB of stores of a large
+      // build do not evict the element tables and integrals from L2
+      __builtin_nontemporal_store((int32_t)(row0 + r2 * L + kk), colind + st + p);
+      __builtin_nontemporal_store(hv, nzval + st + p);
+      nk += fabs(hv) > thresh ? 1 : 0;
     }
   }
   nk = wave_sum(nk);
@@ -636,9 +663,11 @@ void product_fill_elements(mhip_context* ctx, int W, const ProductSpace& ps, con
                      ctx->stream, v, nullptr, nullptr, 1, ps.nruns);
   hipLaunchKernelGGL(inner_tables_kernel<false>, dim3((ps.L + kWavesPerBlock - 1) / kWavesPerBlock),
                      dim3(256), 0, ctx->stream, v, nullptr, nullptr, 1, ps.L);
+  const int ms = (int)std::max<int64_t>(1, ps.max_seg);
   hipLaunchKernelGGL(product_fill_elements_kernel<false>,
-                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, ctx->stream, v,
-                     nullptr, nullptr, n, diag, rowptr, colind, nzval, thresh, kept, dropped);
+                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256),
+                     kWavesPerBlock * fill_lds_per_wave(ms), ctx->stream, v, nullptr, nullptr, n,
+                     ms, diag, rowptr, colind, nzval, thresh, kept, dropped);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 
@@ -723,6 +752,7 @@ bool prepare_batch_product(mhip_context* ctx, int W, const uint64_t* d_dets,
                                 hipMemcpyHostToDevice, st));
   MHIP_HIP_CHECK(hipStreamSynchronize(st));
   pb.ninst = ninst;
+  pb.max_seg = max_seg;
   pb.total_runs = run_off[ninst];
   pb.total_inner = inner_off[ninst];
   return true;
@@ -747,9 +777,11 @@ void batch_product_fill_elements(mhip_context* ctx, const ProductBatch& pb,
   hipLaunchKernelGGL(inner_tables_kernel<true>,
                      dim3((pb.total_inner + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0,
                      st, PSView{}, v, pb.inner_off.as<int64_t>(), pb.ninst, pb.total_inner);
+  const int ms = (int)std::max<int64_t>(1, pb.max_seg);
   hipLaunchKernelGGL(product_fill_elements_kernel<true>,
-                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, st, PSView{},
-                     v, row_inst, n, diag, rowptr, colind, nzval, thresh, kept, dropped);
+                     dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256),
+                     kWavesPerBlock * fill_lds_per_wave(ms), st, PSView{}, v, row_inst, n, ms, diag,
+                     rowptr, colind, nzval, thresh, kept, dropped);
   MHIP_HIP_CHECK(hipGetLastError());
 }
 
