@@ -179,6 +179,27 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// OR of a 64-bit value over the wavefront (every lane active), uniform
+// result: DPP row shifts by 1, 2, 4, 8 leave each 16-lane row's OR in its
+// lane 15, two row broadcasts carry rows 0-2 into lane 63, which is read.
+// Both halves go through VALU DPP operations; no LDS instruction.
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+  uint32_t a = (uint32_t)v, b = (uint32_t)(v >> 32);
+#define MHIP_OR_DPP(ctrl, rows, bound)                                                  \
+  a |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)a, ctrl, rows, 0xf, bound);       \
+  b |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)b, ctrl, rows, 0xf, bound);
+  MHIP_OR_DPP(0x111, 0xf, true)   // row_shr:1
+  MHIP_OR_DPP(0x112, 0xf, true)   // row_shr:2
+  MHIP_OR_DPP(0x114, 0xf, true)   // row_shr:4
+  MHIP_OR_DPP(0x118, 0xf, true)   // row_shr:8
+  MHIP_OR_DPP(0x142, 0xa, false)  // row_bcast:15 into rows 1 and 3
+  MHIP_OR_DPP(0x143, 0xc, false)  // row_bcast:31 into rows 2 and 3
+#undef MHIP_OR_DPP
+  // readlane returns a signed int: widen through uint32_t (no sign extension)
+  return (uint64_t)(uint32_t)__builtin_amdgcn_readlane(a, 63) |
+         ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(b, 63) << 32);
+}
+
 __device__ __forceinline__ int64_t uniform(int64_t x) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));

@@ -363,9 +363,7 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     const int32_t js = jlo + lane;
     const int32_t sstart = js < nseg ? seg_start[js] : INT32_MAX;
     const bool inwin = sstart < p0 + 64;
-    uint64_t startmask = inwin ? (1ull << (sstart - p0)) : 0ull;
-#pragma unroll
-    for(int off = 1; off < 64; off <<= 1) startmask |= __shfl_xor(startmask, off, 64);
+    const uint64_t startmask = wave_or64(inwin ? (1ull << (sstart - p0)) : 0ull);
     const int32_t lo = jlo - 1 + (int32_t)popc64(startmask & lowmask((uint32_t)lane + 1u));
     jlo += (int32_t)popc64(ballot(inwin));
     const int32_t p = p0 + lane;
