@@ -253,10 +253,10 @@ __device__ __forceinline__ double code_sign(int32_t code) {
 // values of the tiled path's element pass (the same operations in the same
 // order), so the CSR is bit-identical; |h| > thresh is counted per row for
 // the shared compaction. LDS per wavefront: 18 bytes per outer neighbour
-// (max_seg of the launch) + 4, so that at CAS(12e,12o) (262 outer
-// neighbours) the 62-VGPR kernel keeps 8 wavefronts per SIMD.
+// (max_seg of the launch) + 132, so that at CAS(12e,12o) (262 outer
+// neighbours) the 64-VGPR kernel keeps 8 wavefronts per SIMD.
 __host__ __device__ constexpr size_t fill_lds_per_wave(int max_seg) {
-  return ((size_t)18 * max_seg + 4 + 7) / 8 * 8;
+  return ((size_t)18 * max_seg + 4 + 128 + 7) / 8 * 8;
 }
 
 // A packed single (o | v << 8 | (sign < 0) << 16) in 16 bits (orbitals < 64)
@@ -284,6 +284,7 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   int32_t* seg_start = reinterpret_cast<int32_t*>(seg_val + max_seg);
   int32_t* seg_code = seg_start + max_seg + 1;
   int16_t* seg_exc = reinterpret_cast<int16_t*>(seg_code + max_seg);
+  int16_t* lv1 = seg_exc + max_seg;  // level-1 segments, in order (at most 64)
   const PSView& P = BATCH ? many[row_inst[row]] : one;
   const IntsView& I = P.I;
   const int64_t L = P.L, row0 = P.row0;
@@ -315,7 +316,13 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     reg_k1 = idx1[s1 + lane];
     reg_c1 = iexc1[s1 + lane];
   }
-  // segment starts (exclusive prefix of lengths), neighbours, codes, values
+  // segment starts (exclusive prefix of lengths), neighbours, codes, values;
+  // with at most 64 level-1 segments their own-inner-string elements are
+  // formed afterwards in one round (a round per 64 segments would run the
+  // integral sum for the few level-1 lanes of each)
+  const int32_t n1 = P.ocnt[3 * r + 1];
+  const bool compact1 = n1 <= 64;
+  int32_t seen1 = 0;
   int32_t carry = 0;
   for(int32_t b = 0; b < nseg; b += 64) {
     const int32_t j = b + lane;
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
       double val = 0.;
       if(e == 2) {  // outer double, same inner string: me_4 of the outer pair
         val = oval[j0 + j];
-      } else if(e == 1) {  // outer single, same inner string: me_2, other spin = inner
+      } else if(e == 1 && !compact1) {  // outer single, same inner string: me_2
         oc = oexc[j0 + j];
         const uint32_t o = oc & 0xff, v = (oc >> 8) & 0xff;
         const double* V_ov = I.V_red + I.n * (v + I.n * o);
@@ -347,9 +354,27 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
       seg_exc[j] = code16(oc);
       seg_val[j] = val;
     }
+    if(compact1) {
+      const uint64_t m1 = ballot(valid && e == 1);
+      if(valid && e == 1) lv1[seen1 + (int32_t)lanes_below(m1)] = (int16_t)j;
+      seen1 += popc64(m1);
+    }
     carry += __shfl(incl, 63, 64);
   }
   if(lane == 0) seg_start[nseg] = carry;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  if(compact1 && lane < n1) {  // outer single, same inner string: me_2, other spin = inner
+    const int32_t j = lv1[lane];
+    const int32_t oc = oexc[j0 + j];
+    const uint32_t o = oc & 0xff, v = (oc >> 8) & 0xff;
+    const double* V_ov = I.V_red + I.n * (v + I.n * o);
+    double t = oval[j0 + j];
+    for(uint64_t m = bi; m; m &= m - 1) t += V_ov[ctz64(m)];
+    seg_exc[j] = code16(oc);
+    seg_val[j] = code_sign(oc) * t;
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
