@@ -27,6 +27,7 @@
 #include <array>
 #include <memory>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "mhip_device.hpp"
@@ -275,7 +276,8 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     int32_t* __restrict__ colind, double* __restrict__ nzval, double thresh,
     int64_t* __restrict__ kept, int* __restrict__ dropped) {
   extern __shared__ double fill_lds[];
-  const int wv = threadIdx.x >> 6;
+  // the row, and what is derived from it, is uniform over the wavefront
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t row = (int64_t)blockIdx.x * kWavesPerBlock + wv;
   if(row >= n) return;  // whole wavefront leaves together
   const int lane = threadIdx.x & 63;
@@ -384,72 +386,119 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   // the next ones in order (jlo, jlo + 1, ...); their start offsets form a
   // 64-bit mask, and a lane's segment is jlo - 1 + the starts at or below it.
   int32_t jlo = 0;
-  for(int32_t p0 = 0; p0 < total; p0 += 64) {
-    const int32_t js = jlo + lane;
+  // lane's segment in the window at p0; advances jl past the window's starts
+  auto locate = [&](int32_t p0, int32_t& jl) -> int32_t {
+    const int32_t js = jl + lane;
     const int32_t sstart = js < nseg ? seg_start[js] : INT32_MAX;
     const bool inwin = sstart < p0 + 64;
     const uint64_t startmask = wave_or64(inwin ? (1ull << (sstart - p0)) : 0ull);
-    const int32_t lo = jlo - 1 + (int32_t)popc64(startmask & lowmask((uint32_t)lane + 1u));
-    jlo += (int32_t)popc64(ballot(inwin));
-    const int32_t p = p0 + lane;
+    const int32_t lo = jl - 1 + (int32_t)popc64(startmask & lowmask((uint32_t)lane + 1u));
+    jl += (int32_t)popc64(ballot(inwin));
+    return lo;
+  };
+  const bool swap_pairs = outer_alpha && I.pair_sym;
+  // Column and element of position p (segment lo), without branches for
+  // the level-2 and level-1 segments: `gather` issues the alpha-beta
+  // integral load of every lane (index 0 where it is not needed), `finish`
+  // forms the element, so that two windows' loads are in flight together.
+  // Level-0 entries (the row's own outer string, a few windows per row)
+  // are formed by `level0` afterwards. e: 3 past the row end.
+  struct Win {
+    int32_t col, off;
+    int e;
+    bool ab;
+    double sign, sv, vv;
+  };
+  auto gather = [&](auto in_regs, int32_t p, int32_t lo, Win& w) {
     const bool act = p < total;
     const int32_t l = act ? lo : 0;
     const int32_t packed = seg_code[l];
-    const int32_t off = p - seg_start[l];
+    w.off = p - seg_start[l];
     const int64_t r2 = packed >> 2;
-    const int e = packed & 3;
-    const double sv = seg_val[l];
+    w.e = act ? (packed & 3) : 3;
+    w.sv = seg_val[l];
     const int32_t oc = code32(seg_exc[l]);
     // the lane's entry of the level-<=1 inner list (meaningful at level 1)
-    int32_t k1 = __shfl(reg_k1, off & 63, 64);
-    int32_t c1 = __shfl(reg_c1, off & 63, 64);
-    if(!inreg && act && e == 1) {
-      k1 = idx1[s1 + off];
-      c1 = iexc1[s1 + off];
+    int32_t k1 = __shfl(reg_k1, w.off & 63, 64);
+    int32_t c1 = __shfl(reg_c1, w.off & 63, 64);
+    if constexpr(!decltype(in_regs)::value) {  // every lane reads an entry of the row's list
+      const int32_t t1 = s1 + (w.off < len1 ? w.off : len1 - 1);
+      k1 = idx1[t1];
+      c1 = iexc1[t1];
     }
-    double hv = d;
-    int32_t kk = (int32_t)k;
-    if(act) {
-      if(e == 2) {
-        hv = sv;
-      } else if(e == 1) {
-        kk = k1;
-        if(c1 < 0) {
-          hv = sv;
-        } else {  // outer single x inner single: me_22 (alpha excitation first)
-          const int32_t ac = outer_alpha ? oc : c1, bc = outer_alpha ? c1 : oc;
-          const uint32_t o1 = ac & 0xff, v1 = (ac >> 8) & 0xff;
-          const uint32_t o2 = bc & 0xff, v2 = (bc >> 8) & 0xff;
-          const double sign = code_sign(ac) * code_sign(bc);
-          // outer alpha: (o1, v1) is fixed over the segment and the lanes
-          // vary (o2, v2); with pair symmetry the swapped entry
-          // V(v2,o2,v1,o1) keeps the lanes inside one n*n block
-          hv = sign * (outer_alpha && I.pair_sym
-                           ? I.V[v2 + I.n * (o2 + I.n * (v1 + I.n * o1))]
-                           : I.V[v1 + I.n * (o1 + I.n * (v2 + I.n * o2))]);
-        }
-      } else {
-        const int32_t t2 = s2 + off;
-        kk = idx2[t2];
-        const int32_t ic = iexc2[t2];
-        const int lvl = ic >> 24;
-        if(lvl == 2) {  // inner double: me_4 of the inner pair
-          hv = ival2[t2];
-        } else if(lvl == 1) {  // inner single: me_2, other spin = outer
-          const uint32_t o = ic & 0xff, v = (ic >> 8) & 0xff;
-          const double* V_ov = I.V_red + I.n * (v + I.n * o);
-          double t = ival2[t2];
-          for(uint64_t m = bo; m; m &= m - 1) t += V_ov[ctz64(m)];
-          hv = code_sign(ic) * t;
-        }
+    // outer single x inner single: me_22 (alpha excitation first); outer
+    // alpha: (o1, v1) is fixed over the segment and the lanes vary (o2, v2),
+    // and with pair symmetry the swapped entry V(v2,o2,v1,o1) keeps the
+    // lanes inside one n*n block
+    w.ab = w.e == 1 && c1 >= 0;
+    const int32_t ac = outer_alpha ? oc : c1, bc = outer_alpha ? c1 : oc;
+    const uint32_t o1 = ac & 0xff, v1 = (ac >> 8) & 0xff;
+    const uint32_t o2 = bc & 0xff, v2 = (bc >> 8) & 0xff;
+    w.sign = code_sign(ac) * code_sign(bc);
+    const uint32_t n = (uint32_t)I.n;
+    const uint32_t pa = v1 + n * o1, pb = v2 + n * o2;  // V(pa | pb) = V(pb | pa)
+    const uint32_t vi = swap_pairs ? pb + n * n * pa : pa + n * n * pb;
+    w.vv = I.V[w.ab ? vi : 0u];
+    w.col = (int32_t)(row0 + r2 * L + (w.e == 1 ? k1 : (int32_t)k));
+  };
+  // level 2: me_4 of the outer pair; level 1 with the row's own inner
+  // string: me_2 (both staged); level 0: the diagonal until level0()
+  auto finish = [&](const Win& w) -> double {
+    return w.ab ? w.sign * w.vv : (w.e == 1 || w.e == 2) ? w.sv : d;
+  };
+  auto level0 = [&](int32_t off, int32_t& col, double& hv) {
+    const int32_t t2 = s2 + off;
+    col = (int32_t)(row0 + r * L + idx2[t2]);
+    const int32_t ic = iexc2[t2];
+    const int lvl = ic >> 24;
+    if(lvl == 2) {  // inner double: me_4 of the inner pair
+      hv = ival2[t2];
+    } else if(lvl == 1) {  // inner single: me_2, other spin = outer
+      const uint32_t o = ic & 0xff, v = (ic >> 8) & 0xff;
+      const double* V_ov = I.V_red + I.n * (v + I.n * o);
+      double t = ival2[t2];
+      for(uint64_t m = bo; m; m &= m - 1) t += V_ov[ctz64(m)];
+      hv = code_sign(ic) * t;
+    }
+  };
+  // two windows per iteration: their element chains (cross-lane reads, the
+  // integral gather) are independent and overlap
+  auto windows = [&](auto in_regs) {
+    for(int32_t p0 = 0; p0 < total; p0 += 128) {
+      const int32_t loA = locate(p0, jlo);
+      const int32_t loB = locate(p0 + 64, jlo);
+      const int32_t pA = p0 + lane, pB = p0 + 64 + lane;
+      Win wA, wB;
+      gather(in_regs, pA, loA, wA);
+      gather(in_regs, pB, loB, wB);
+      // both integral loads are issued before either is waited for (the
+      // empty statement consumes them here, so neither load sinks into the
+      // branch that uses it)
+      asm volatile("" : "+v"(wA.vv), "+v"(wB.vv));
+      int32_t cA = wA.col, cB = wB.col;
+      double hA = finish(wA), hB = finish(wB);
+      if(ballot(wA.e == 0) | ballot(wB.e == 0)) {
+        if(wA.e == 0) level0(wA.off, cA, hA);
+        if(wB.e == 0) level0(wB.off, cB, hB);
       }
       // streamed out once: non-temporal, so the 19 GB of stores of a large
       // build do not evict the element tables and integrals from L2
-      __builtin_nontemporal_store((int32_t)(row0 + r2 * L + kk), colind + st + p);
-      __builtin_nontemporal_store(hv, nzval + st + p);
-      nk += fabs(hv) > thresh ? 1 : 0;
+      if(pA < total) {
+        __builtin_nontemporal_store(cA, colind + st + pA);
+        __builtin_nontemporal_store(hA, nzval + st + pA);
+        nk += fabs(hA) > thresh ? 1 : 0;
+      }
+      if(pB < total) {
+        __builtin_nontemporal_store(cB, colind + st + pB);
+        __builtin_nontemporal_store(hB, nzval + st + pB);
+        nk += fabs(hB) > thresh ? 1 : 0;
+      }
     }
-  }
+  };
+  if(inreg)
+    windows(std::integral_constant<bool, true>());
+  else
+    windows(std::integral_constant<bool, false>());
   nk = wave_sum(nk);
   if(lane == 0) {
     kept[row] = nk;
