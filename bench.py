@@ -30,8 +30,9 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
 # HBM bytes per spmv_sell_kernel dispatch on this workload from two rocprofv3
 # --pmc runs (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
-# 1 GiB device copy), summarised in profiles/cas12_fci_build_davidson.md.
-SPMV_TRAFFIC_GB = 19.079
+# 1 GiB device copy), summarised in profiles/cas12_fci_build_davidson.md:
+# the sliced copy streams 16-bit column offsets (15.64 GB per product).
+SPMV_TRAFFIC_GB = 16.14
 N_ORB, N_ALPHA, N_BETA, SEED = 12, 6, 6, 12
 
 
