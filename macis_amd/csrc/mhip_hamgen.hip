@@ -120,13 +120,24 @@ struct StagePool {
   int64_t* chunk_next;         // next chunk of the row, -1 at its end
   uint32_t* used;              // entries in each chunk
   int* overflow;               // set when a chunk was refused
+  // state carried between the launches of a ket-chunked scan: per row the
+  // count, current chunk and its fill; per wavefront the grabbed chunks
+  uint32_t* st_cnt;
+  int64_t* st_cur;
+  uint32_t* st_fil;
+  int64_t* st_grab;
 };
 
 // Structural count (kCount), fill (kFill) or staged (kStage) pass.
-// Without a work-item table, wave w owns rows [w*R, w*R+R) and scans all
-// kets. With one (batched instances), item w = {row0, nrows, ket_lo, ket_hi}
-// names rows of one instance and the ket range of that instance; columns
-// stay absolute indices, so a batch yields one block-diagonal CSR.
+// Without a work-item table, wave w owns rows [w*R, w*R+R) and scans kets
+// [kc_lo, kc_hi). With one (batched instances), item w = {row0, nrows,
+// ket_lo, ket_hi} names rows of one instance and the ket range of that
+// instance; columns stay absolute indices, so a batch yields one
+// block-diagonal CSR. A staged scan of a long list runs as several launches
+// over consecutive ket chunks sized to one XCD's L2 (chunk_flags: 1 = first,
+// 2 = last launch): every row's hits of chunk c are appended after those of
+// chunk c - 1, so columns stay ascending, and the row and wavefront state
+// (count, chain end, grabbed chunks) is stored between launches.
 template <int W, int R, int MODE>
 __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void scan_pass_kernel(
     const uint64_t* __restrict__ bra, int64_t nbra,
@@ -136,8 +147,9 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
     const uint64_t* __restrict__ sor, int64_t nsuper, int64_t nwaves,
     int64_t nblocks, const int64_t* __restrict__ items,
     uint32_t* __restrict__ row_counts, const int64_t* __restrict__ rowptr,
-    int32_t* __restrict__ colind, StagePool P) {
+    int32_t* __restrict__ colind, StagePool P, int64_t kc_lo, int64_t kc_hi, int chunk_flags) {
   constexpr bool FILL = MODE == kFill;
+  const bool first = chunk_flags & 1, last = chunk_flags & 2;
   // XCD-aware mapping: hardware dispatches block b to XCD b % 8. Logical
   // blocks (consecutive rows, which share spin strings and therefore ket
   // tiles) come in chunks of kXcdChunk blocks, chunk c on XCD c % 8: each
@@ -150,7 +162,7 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
   const int64_t wave = lblk * kWavesPerBlock + (threadIdx.x >> 6);
   if(wave >= nwaves) return;
   const uint32_t lane = threadIdx.x & 63;
-  int64_t row0 = wave * R, nrows = R, klo = 0, khi = nket;
+  int64_t row0 = wave * R, nrows = R, klo = kc_lo, khi = kc_hi;
   if(items) {
     row0 = (int64_t)uniform_u64((uint64_t)items[4 * wave + 0]);
     nrows = (int64_t)uniform_u64((uint64_t)items[4 * wave + 1]);
@@ -189,7 +201,7 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
   }
   if(!live || khi <= klo) {
     if constexpr(!FILL) {
-      if(lane == 0)
+      if(lane == 0 && first)
         for(int r = 0; r < R; ++r)
           if(row0 + r < rend) {
             row_counts[row0 + r] = 0;
@@ -197,6 +209,22 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
           }
     }
     return;
+  }
+  if constexpr(MODE == kStage) {
+    if(!first) {  // resume the rows' chains and the wavefront's grabbed chunks
+#pragma unroll
+      for(int r = 0; r < R; ++r) {
+        const int64_t row = row0 + r;
+        if(row < rend) {
+          cnt[r] = (uint32_t)uniform_u64((uint64_t)P.st_cnt[row]);
+          cur[r] = (int64_t)uniform_u64((uint64_t)P.st_cur[row]);
+          fil[r] = (uint32_t)uniform_u64((uint64_t)P.st_fil[row]);
+        }
+      }
+      grab = (int64_t)uniform_u64((uint64_t)P.st_grab[2 * wave]);
+      grab_end = (int64_t)uniform_u64((uint64_t)P.st_grab[2 * wave + 1]);
+      dead = P.overflow[0] != 0;
+    }
   }
   // kStage: record one ballot's hits of row r (m = ballot, hit = this lane's
   // bit): the part that fits the current chunk goes there, the rest opens
@@ -339,6 +367,21 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
           }
         }
       }
+    }
+  }
+  if constexpr(MODE == kStage) {
+    if(!last) {  // keep the state for the next chunk's launch
+      if(lane == 0) {
+        for(int r = 0; r < R; ++r)
+          if(row0 + r < rend) {
+            P.st_cnt[row0 + r] = cnt[r];
+            P.st_cur[row0 + r] = cur[r];
+            P.st_fil[row0 + r] = fil[r];
+          }
+        P.st_grab[2 * wave] = grab;
+        P.st_grab[2 * wave + 1] = grab_end;
+      }
+      return;
     }
   }
   if constexpr(!FILL) {
@@ -556,9 +599,18 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   // values are allocated, so the peak stays that of count + fill. Batched
   // builds (work items over small per-instance ket ranges) scan cheaply and
   // keep count + fill, which runs at a higher occupancy.
-  DevBuf s_pool, s_head, s_next, s_used, s_ctl;
+  DevBuf s_pool, s_head, s_next, s_used, s_ctl, s_state;
   StagePool P{};
   bool staged = false;
+  // ket chunks of the staged scan: one XCD's L2 (4 MB) holds a chunk of
+  // kets (16 bytes each at W = 2) with its tile bounds; a list of more than
+  // two chunks is scanned one chunk per launch (MHIP_SCAN_KET_CHUNK, a
+  // multiple of 4,096 kets, overrides the chunk so tests reach the path on
+  // short lists; 0 disables chunking)
+  int64_t ket_chunk = W == 1 ? 262144 : 131072;
+  if(const char* env = std::getenv("MHIP_SCAN_KET_CHUNK"))
+    ket_chunk = std::max<int64_t>(0, std::atoll(env)) / (kTile * kSuper) * (kTile * kSuper);
+  const bool chunked = !prod && !ex.items && ket_chunk > 0 && nket > 2 * ket_chunk;
   if(!prod && !ex.items) {
     size_t free_b = 0, total_b = 0;
     device_mem_info(&free_b, &total_b);
@@ -576,6 +628,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
         s_used.alloc(sizeof(uint32_t) * cap);
         s_head.alloc(sizeof(int64_t) * nbra);
         s_ctl.alloc(16);
+        if(chunked) s_state.alloc(16 * (size_t)nbra + 16 * (size_t)nwaves);
         staged = true;
       } catch(const Error& e) {
         if(e.code != MHIP_ERR_OUT_OF_MEMORY) throw;
@@ -584,6 +637,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
         s_next.release();
         s_used.release();
         s_head.release();
+        s_state.release();
       }
     }
     if(staged) {
@@ -594,6 +648,12 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
       P.chunk_next = s_next.as<int64_t>();
       P.used = s_used.as<uint32_t>();
       P.overflow = reinterpret_cast<int*>(s_ctl.as<unsigned char>() + 8);
+      if(chunked) {
+        P.st_cur = s_state.as<int64_t>();
+        P.st_grab = P.st_cur + nbra;
+        P.st_cnt = reinterpret_cast<uint32_t*>(P.st_grab + 2 * nwaves);
+        P.st_fil = P.st_cnt + nbra;
+      }
       MHIP_HIP_CHECK(hipMemsetAsync(s_ctl.p, 0, 16, st));
     }
   }
@@ -602,16 +662,22 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
       product_count(ctx, *ex.product, nbra, counts32.as<uint32_t>());
     else if(ex.bproduct)
       batch_product_count(ctx, *ex.bproduct, ex.row_inst, nbra, counts32.as<uint32_t>());
-    else if(stage)
-      hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kStage>), dim3(grid), dim3(256), 0, st,
-                         d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
-                         nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr,
-                         P);
-    else
+    else if(stage) {
+      const int64_t step = chunked ? ket_chunk : nket;
+      for(int64_t c0 = 0; c0 < nket || c0 == 0; c0 += step) {
+        const int64_t c1 = std::min(nket, c0 + step);
+        const int flags = (c0 == 0 ? 1 : 0) | (c1 >= nket ? 2 : 0);
+        hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kStage>), dim3(grid), dim3(256), 0, st,
+                           d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
+                           nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr,
+                           nullptr, P, c0, c1, flags);
+        if(c1 >= nket) break;
+      }
+    } else
       hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kCount>), dim3(grid), dim3(256), 0, st,
                          d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
                          nwaves, nblocks, ex.items, counts32.as<uint32_t>(), nullptr, nullptr,
-                         StagePool{});
+                         StagePool{}, (int64_t)0, nket, 3);
     MHIP_HIP_CHECK(hipGetLastError());
   };
   t.start();
@@ -639,6 +705,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
       s_next.release();
       s_used.release();
       s_head.release();
+      s_state.release();
       count_pass(false);
       scan_counts();
     }
@@ -663,6 +730,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
     s_next.release();
     s_used.release();
     s_head.release();
+    s_state.release();
     t.stop();
   }
   DevBuf val = take_buffer(ctx, sizeof(double) * std::max<int64_t>(nnz_struct, 1));
@@ -685,7 +753,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
     hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kFill>), dim3(grid), dim3(256), 0, st,
                        d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
                        nwaves, nblocks, ex.items, nullptr, rp_struct.as<int64_t>(),
-                       col.as<int32_t>(), StagePool{});
+                       col.as<int32_t>(), StagePool{}, (int64_t)0, nket, 3);
   }
   MHIP_HIP_CHECK(hipGetLastError());
   if(!staged) t.stop();

@@ -280,23 +280,56 @@ def test_selected_ci_64_orbitals_properties(ctx):
     assert np.linalg.norm(Hx - E * x) <= 1e-8 + 1e-14 * abs(E) * np.sqrt(N)
 
 
+@pytest.mark.parametrize("W", [1, 2])
 @pytest.mark.parametrize("pool", ["0", "1024"])
-def test_tiled_build_fallback_paths(ctx, pool, monkeypatch):
+def test_tiled_build_fallback_paths(ctx, pool, W, monkeypatch):
     # MHIP_STAGE_POOL_CHUNKS=0: no staging (count + fill walks); 1024 chunks
     # (131,072 entries) overflow on this list and the build redoes count +
-    # fill. Both must give the staged build's CSR exactly.
-    announce(f"hamgen tiled fallback pool={pool}")
+    # fill. Both must give the staged build's CSR exactly, for one-word
+    # (W = 1, alpha | beta << 32, the same bitset_less order) and two-word
+    # determinants.
+    announce(f"hamgen tiled fallback pool={pool} W={W}")
     n = 16
     T, V = seeded_integrals(n, 77)
     ctx.set_integrals(n, T, V)
     dets = selected_ci_dets(n, 5, 5, 4000, seed=77)
+    if W == 1:
+        dets = (dets[:, 0] | (dets[:, 1] << np.uint64(32))).reshape(-1, 1)
     dd = ctx.to_device(dets)
-    ref = ctx.ham_build(2, dd, len(dets), thresh=EPS).to_host()
+    ref = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
     assert ref[0][-1] > 131072
     monkeypatch.setenv("MHIP_STAGE_POOL_CHUNKS", pool)
-    got = ctx.ham_build(2, dd, len(dets), thresh=EPS).to_host()
+    got = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
+    assert_same_csr(got, O.Ham(T, V).csr(W, dets, thresh=EPS))
+
+
+@pytest.mark.parametrize("W", [1, 2])
+@pytest.mark.parametrize("chunk,pool", [("4096", None), ("12288", None), ("4096", "1024")])
+def test_tiled_scan_ket_chunks(ctx, W, chunk, pool, monkeypatch):
+    """A long list's staged scan runs one launch per ket chunk, each row's
+    hits appended chunk after chunk and its state stored between launches.
+    MHIP_SCAN_KET_CHUNK forces chunks of 4,096 / 12,288 kets on a 20,000-det
+    list (5 / 2 launches, the last chunk partial); with a 1,024-chunk pool
+    the staging overflows during the chunked launches and the build redoes
+    count + fill. The CSR equals the unchunked build's and the oracle's."""
+    announce(f"hamgen ket-chunked scan W={W} chunk={chunk} pool={pool}")
+    n = 16
+    T, V = seeded_integrals(n, 79)
+    ctx.set_integrals(n, T, V)
+    dets = selected_ci_dets(n, 5, 5, 20000, seed=79, energies=0.25 * np.arange(n))
+    if W == 1:
+        dets = (dets[:, 0] | (dets[:, 1] << np.uint64(32))).reshape(-1, 1)
+    dd = ctx.to_device(dets)
+    ref = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
+    monkeypatch.setenv("MHIP_SCAN_KET_CHUNK", chunk)
+    if pool:
+        monkeypatch.setenv("MHIP_STAGE_POOL_CHUNKS", pool)
+    got = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    assert_same_csr(got, O.Ham(T, V).csr(W, dets, thresh=EPS))
 
 
 def test_product_fill_non_uniform_inner_strings(ctx):
