@@ -291,8 +291,10 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
       smask[r] = ((live >> r) & 1u) ? ballot(sv && excitation_bound<W>(b[r], sa, so) <= 4) : 0ull;
       sany |= smask[r];
     }
+    // the masks are wavefront-uniform (ballots): kept in scalar registers
+    sany = uniform_u64(sany);
     while(sany) {
-      const uint32_t sb = ctz64(sany);
+      const uint32_t sb = __builtin_amdgcn_readfirstlane(ctz64(sany));
       sany &= sany - 1;
       const int64_t S = g + sb;
       const int64_t t = S * kSuper + lane;
@@ -310,51 +312,39 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
         tm[r] = ((smask[r] >> sb) & 1ull) ? ballot(tv && excitation_bound<W>(b[r], ta, to) <= 4) : 0ull;
         tany |= tm[r];
       }
-      while(tany) {
-        const uint32_t tb = ctz64(tany);
-        tany &= tany - 1;
+      // kets of surviving tile tb (one per lane; zero outside [klo, khi))
+      auto load_tile = [&](uint32_t tb, uint64_t (&k)[W]) {
         const int64_t j = (S * kSuper + tb) * kTile + lane;
-        uint64_t k[W];
-        uint64_t knz = 0;
         const bool kv = j >= klo && j < khi;
 #pragma unroll
-        for(int w = 0; w < W; ++w) {
-          k[w] = kv ? ket[j * W + w] : 0ull;
-          knz |= k[w];
-        }
+        for(int w = 0; w < W; ++w) k[w] = kv ? ket[j * W + w] : 0ull;
+      };
+      auto test_tile = [&](uint32_t tb, const uint64_t (&k)[W]) {
+        const int64_t j = (S * kSuper + tb) * kTile + lane;
+        uint64_t knz = 0;
+#pragma unroll
+        for(int w = 0; w < W; ++w) knz |= k[w];
         // Tiles whose kets all share one alpha string (AND == OR on the
         // alpha bits): the alpha excitation is a per-row scalar, so each
         // ket costs one XOR + popcount of the beta string.
         const uint64_t tA = lane_value(ta[0], tb), tO = lane_value(to[0], tb);
         const uint64_t amask = W == 1 ? 0xffffffffull : ~0ull;
-        if(((tA ^ tO) & amask) == 0) {
-          const uint64_t kb = W == 1 ? (k[0] >> 32) : k[W - 1];
-#pragma unroll
-          for(int r = 0; r < R; ++r) {
-            if((tm[r] >> tb) & 1ull) {
-              const int thr = 4 - popc64((b[r][0] ^ tA) & amask);
-              const uint64_t bb = W == 1 ? (b[r][0] >> 32) : b[r][W - 1];
-              const bool hit = knz && popc64(bb ^ kb) <= thr;
-              const uint64_t m = ballot(hit);
-              if constexpr(FILL) {
-                if(hit) colind[base[r] + lanes_below(m)] = (int32_t)j;
-                base[r] += popc64(m);
-              } else if constexpr(MODE == kStage) {
-                stage(r, m, hit, (int32_t)j);
-              } else {
-                cnt[r] += (uint32_t)popc64(m);
-              }
-            }
-          }
-          continue;
-        }
+        const bool one_alpha = ((tA ^ tO) & amask) == 0;
 #pragma unroll
         for(int r = 0; r < R; ++r) {
           if((tm[r] >> tb) & 1ull) {
-            int pc = 0;
+            bool hit;
+            if(one_alpha) {
+              const int thr = 4 - popc64((b[r][0] ^ tA) & amask);
+              const uint64_t bb = W == 1 ? (b[r][0] >> 32) : b[r][W - 1];
+              const uint64_t kb = W == 1 ? (k[0] >> 32) : k[W - 1];
+              hit = knz && popc64(bb ^ kb) <= thr;
+            } else {
+              int pc = 0;
 #pragma unroll
-            for(int w = 0; w < W; ++w) pc += popc64(b[r][w] ^ k[w]);
-            const bool hit = knz && pc <= 4;
+              for(int w = 0; w < W; ++w) pc += popc64(b[r][w] ^ k[w]);
+              hit = knz && pc <= 4;
+            }
             const uint64_t m = ballot(hit);
             if constexpr(FILL) {
               if(hit) colind[base[r] + lanes_below(m)] = (int32_t)j;
@@ -365,6 +355,26 @@ __global__ __launch_bounds__(256, MODE == kStage ? (W == 1 ? 4 : 3) : 1) void sc
               cnt[r] += (uint32_t)popc64(m);
             }
           }
+        }
+      };
+      // the next surviving tile's kets are loaded while this one is tested
+      tany = uniform_u64(tany);
+      if(tany) {
+        uint32_t tb = __builtin_amdgcn_readfirstlane(ctz64(tany));
+        tany &= tany - 1;
+        uint64_t k[W];
+        load_tile(tb, k);
+        while(true) {
+          const bool more = tany != 0;
+          const uint32_t tn = more ? __builtin_amdgcn_readfirstlane(ctz64(tany)) : 0u;
+          uint64_t kn[W];
+          if(more) load_tile(tn, kn);
+          test_tile(tb, k);
+          if(!more) break;
+          tany &= tany - 1;
+          tb = tn;
+#pragma unroll
+          for(int w = 0; w < W; ++w) k[w] = kn[w];
         }
       }
     }
