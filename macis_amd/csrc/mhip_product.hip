@@ -260,13 +260,22 @@ __host__ __device__ constexpr size_t fill_lds_per_wave(int max_seg) {
   return ((size_t)18 * max_seg + 4 + 128 + 7) / 8 * 8;
 }
 
-// A packed single (o | v << 8 | (sign < 0) << 16) in 16 bits (orbitals < 64)
-// and back.
-__device__ __forceinline__ int16_t code16(int32_t c) {
-  return (int16_t)((c & 0x3f) | (((c >> 8) & 0x3f) << 6) | (((c >> 16) & 1) << 12));
+// The alpha-beta double of an outer single x inner single is
+// sign_out * sign_in * V(vi) with vi = P_in + n^2 P_out when the outer
+// string is beta or the integrals are pair-symmetric (the pair-swapped
+// entry), else P_out + n^2 P_in, where P = v + n o of the single (packed
+// as o | v << 8 | (sign < 0) << 16). The fill stages the outer part as
+// P_out | sign << 15 (16 bits, P < n^2 <= 4096) and holds the inner part
+// of the row's level-1 list in registers as its term of vi | sign << 30
+// (-1: the row's own inner string), so a lane forms vi with one
+// multiply-add and the sign by one bit flip of the gathered integral
+// (the same double as the product with +-1.0).
+__device__ __forceinline__ uint16_t outer_pair(int32_t c, uint32_t n) {
+  return (uint16_t)(((c & 0xff) * n + ((c >> 8) & 0xff)) | (((c >> 16) & 1) << 15));
 }
-__device__ __forceinline__ int32_t code32(int16_t x) {
-  return (x & 0x3f) | (((x >> 6) & 0x3f) << 8) | (((x >> 12) & 1) << 16);
+__device__ __forceinline__ int32_t inner_pair(int32_t c, uint32_t n, uint32_t mult_in) {
+  if(c < 0) return -1;
+  return (int32_t)((((c & 0xff) * n + ((c >> 8) & 0xff)) * mult_in) | (((c >> 16) & 1) << 30));
 }
 
 template <bool BATCH>
@@ -285,8 +294,8 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   double* seg_val = fill_lds + wv * (fill_lds_per_wave(max_seg) / 8);
   int32_t* seg_start = reinterpret_cast<int32_t*>(seg_val + max_seg);
   int32_t* seg_code = seg_start + max_seg + 1;
-  int16_t* seg_exc = reinterpret_cast<int16_t*>(seg_code + max_seg);
-  int16_t* lv1 = seg_exc + max_seg;  // level-1 segments, in order (at most 64)
+  uint16_t* seg_exc = reinterpret_cast<uint16_t*>(seg_code + max_seg);
+  int16_t* lv1 = reinterpret_cast<int16_t*>(seg_exc + max_seg);  // level-1 segments (<= 64)
   const PSView& P = BATCH ? many[row_inst[row]] : one;
   const IntsView& I = P.I;
   const int64_t L = P.L, row0 = P.row0;
@@ -311,12 +320,16 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   const int32_t s1 = ptr1[k], len1 = ptr1[k + 1] - s1;
   const int32_t s2 = ptr2[k], len2 = ptr2[k + 1] - s2;
   const int32_t j0 = onb_ptr[r], nseg = onb_ptr[r + 1] - j0;
+  // alpha-beta integral index: vi = inner term + P_out * mult_out
+  const uint32_t nn = (uint32_t)I.n;
+  const bool swap_pairs = !outer_alpha || I.pair_sym;
+  const uint32_t mult_out = swap_pairs ? nn * nn : 1u, mult_in = swap_pairs ? 1u : nn * nn;
   // the level-<=1 inner list in registers (lane t: entry t) when it fits
   const bool inreg = len1 <= 64;
   int32_t reg_k1 = 0, reg_c1 = 0;
   if(inreg && lane < len1) {
     reg_k1 = idx1[s1 + lane];
-    reg_c1 = iexc1[s1 + lane];
+    reg_c1 = inner_pair(iexc1[s1 + lane], nn, mult_in);
   }
   // segment starts (exclusive prefix of lengths), neighbours, codes, values;
   // with at most 64 level-1 segments their own-inner-string elements are
@@ -353,7 +366,7 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
         for(uint64_t m = bi; m; m &= m - 1) t += V_ov[ctz64(m)];
         val = code_sign(oc) * t;
       }
-      seg_exc[j] = code16(oc);
+      seg_exc[j] = outer_pair(oc, nn);
       seg_val[j] = val;
     }
     if(compact1) {
@@ -374,7 +387,7 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     const double* V_ov = I.V_red + I.n * (v + I.n * o);
     double t = oval[j0 + j];
     for(uint64_t m = bi; m; m &= m - 1) t += V_ov[ctz64(m)];
-    seg_exc[j] = code16(oc);
+    seg_exc[j] = outer_pair(oc, nn);
     seg_val[j] = code_sign(oc) * t;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -396,59 +409,58 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
     jl += (int32_t)popc64(ballot(inwin));
     return lo;
   };
-  const bool swap_pairs = outer_alpha && I.pair_sym;
   // Column and element of position p (segment lo), without branches for
   // the level-2 and level-1 segments: `gather` issues the alpha-beta
   // integral load of every lane (index 0 where it is not needed), `finish`
   // forms the element, so that two windows' loads are in flight together.
   // Level-0 entries (the row's own outer string, a few windows per row)
   // are formed by `level0` afterwards. e: 3 past the row end.
+  const double* __restrict__ Vg = I.V;
+  const int32_t colrow = (int32_t)(row0 + r * L);  // first column of the row's own run
   struct Win {
     int32_t col, off;
     int e;
     bool ab;
-    double sign, sv, vv;
+    uint32_t neg;
+    double sv, vv;
   };
   auto gather = [&](auto in_regs, int32_t p, int32_t lo, Win& w) {
     const bool act = p < total;
     const int32_t l = act ? lo : 0;
     const int32_t packed = seg_code[l];
     w.off = p - seg_start[l];
-    const int64_t r2 = packed >> 2;
     w.e = act ? (packed & 3) : 3;
     w.sv = seg_val[l];
-    const int32_t oc = code32(seg_exc[l]);
+    const uint32_t xo = seg_exc[l];
     // the lane's entry of the level-<=1 inner list (meaningful at level 1)
     int32_t k1 = __shfl(reg_k1, w.off & 63, 64);
     int32_t c1 = __shfl(reg_c1, w.off & 63, 64);
     if constexpr(!decltype(in_regs)::value) {  // every lane reads an entry of the row's list
       const int32_t t1 = s1 + (w.off < len1 ? w.off : len1 - 1);
       k1 = idx1[t1];
-      c1 = iexc1[t1];
+      c1 = inner_pair(iexc1[t1], nn, mult_in);
     }
     // outer single x inner single: me_22 (alpha excitation first); outer
-    // alpha: (o1, v1) is fixed over the segment and the lanes vary (o2, v2),
-    // and with pair symmetry the swapped entry V(v2,o2,v1,o1) keeps the
-    // lanes inside one n*n block
+    // alpha with pair symmetry reads the swapped entry V(v2,o2,v1,o1), which
+    // keeps a segment's lanes (fixed (o1, v1), varying (o2, v2)) inside one
+    // n*n block
     w.ab = w.e == 1 && c1 >= 0;
-    const int32_t ac = outer_alpha ? oc : c1, bc = outer_alpha ? c1 : oc;
-    const uint32_t o1 = ac & 0xff, v1 = (ac >> 8) & 0xff;
-    const uint32_t o2 = bc & 0xff, v2 = (bc >> 8) & 0xff;
-    w.sign = code_sign(ac) * code_sign(bc);
-    const uint32_t n = (uint32_t)I.n;
-    const uint32_t pa = v1 + n * o1, pb = v2 + n * o2;  // V(pa | pb) = V(pb | pa)
-    const uint32_t vi = swap_pairs ? pb + n * n * pa : pa + n * n * pb;
-    w.vv = I.V[w.ab ? vi : 0u];
-    w.col = (int32_t)(row0 + r2 * L + (w.e == 1 ? k1 : (int32_t)k));
+    const uint32_t vi = (uint32_t)(c1 & 0xffffff) + (xo & 0xfffu) * mult_out;
+    w.neg = ((uint32_t)c1 >> 30 ^ xo >> 15) & 1u;
+    w.vv = Vg[w.ab ? vi : 0u];
+    w.col = (int32_t)(row0 + (packed >> 2) * (int32_t)L) + (w.e == 1 ? k1 : (int32_t)k);
   };
   // level 2: me_4 of the outer pair; level 1 with the row's own inner
-  // string: me_2 (both staged); level 0: the diagonal until level0()
+  // string: me_2 (both staged); level 0: the diagonal until level0();
+  // alpha-beta: the integral with the product of the two signs
   auto finish = [&](const Win& w) -> double {
-    return w.ab ? w.sign * w.vv : (w.e == 1 || w.e == 2) ? w.sv : d;
+    const double sgn = __hiloint2double(__double2hiint(w.vv) ^ (int)(w.neg << 31),
+                                        __double2loint(w.vv));
+    return w.ab ? sgn : (w.e == 1 || w.e == 2) ? w.sv : d;
   };
   auto level0 = [&](int32_t off, int32_t& col, double& hv) {
     const int32_t t2 = s2 + off;
-    col = (int32_t)(row0 + r * L + idx2[t2]);
+    col = colrow + idx2[t2];
     const int32_t ic = iexc2[t2];
     const int lvl = ic >> 24;
     if(lvl == 2) {  // inner double: me_4 of the inner pair
