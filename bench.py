@@ -210,8 +210,8 @@ def selected_ci_section(C, ctx, cpu=True, keep=None):
     ctx.set_integrals(n, T, V)
     dd = ctx.to_device(dets)
     eps = np.finfo(float).eps
-    H = ctx.ham_build(2, dd, nd, thresh=eps)  # warm-up (pool, scan storage)
-    del H
+    # warm-up build (pool, scan storage); its matrix serves the 8-panel split
+    H8 = ctx.ham_build(2, dd, nd, thresh=eps)
     ctx.synchronize()
     t0 = time.perf_counter()
     H = ctx.ham_build(2, dd, nd, thresh=eps)
@@ -230,19 +230,25 @@ def selected_ci_section(C, ctx, cpu=True, keep=None):
         ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
     ctx.synchronize()
     csr_spmv_ms = (time.perf_counter() - t0) * 1e3 / reps
-    # the layout the Davidson uses for this matrix: the column-panel split
-    t0 = time.perf_counter()
-    assert ctx.prepare_spmv(H, panels=True)
-    ctx.synchronize()
-    panel_prep_ms = (time.perf_counter() - t0) * 1e3
-    ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
-    ctx.synchronize()
-    spmv_ms = (time.perf_counter() - t0) * 1e3 / reps
-    del H, x, y
+    def layout_ms(A, **kw):
+        """(prepare ms, ms per product) of a layout attached to A"""
+        t0 = time.perf_counter()
+        assert ctx.prepare_spmv(A, **kw)
+        ctx.synchronize()
+        prep = (time.perf_counter() - t0) * 1e3
+        ctx.spmv(A, 1, 1.0, x, nd, 0.0, y, nd)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.spmv(A, 1, 1.0, x, nd, 0.0, y, nd)
+        ctx.synchronize()
+        return prep, (time.perf_counter() - t0) * 1e3 / reps
+
+    # the layout the Davidson uses for this matrix: the 8-panel split (one
+    # panel per XCD); beside it the panel-major copy with x tiles in LDS
+    panel_prep_ms, spmv_ms = layout_ms(H8, panels=True)
+    xtile_prep_ms, xtile_spmv_ms = layout_ms(H, xtiles=True)
+    del H, H8, x, y
     dX = ctx.empty(nd, np.float64).zero()
     t0 = time.perf_counter()
     E, _, _ = ctx.selected_ci_diag(2, dd, nd, dX, max_restarts=200)
@@ -283,15 +289,20 @@ def selected_ci_section(C, ctx, cpu=True, keep=None):
         "build_elements_ms": tm["ham_elem_ms"],
         "spmv_ms": spmv_ms, "spmv_GBs": spmv_bytes(nnz, nd, False) / (spmv_ms * 1e-3) / 1e9,
         "spmv_layout": "column panels (8 nnz-balanced panels, one per XCD)",
-        "panel_split_ms": panel_prep_ms, "csr_spmv_ms": csr_spmv_ms,
+        "panel_split_ms": panel_prep_ms,
+        "xtile_spmv_ms": xtile_spmv_ms,
+        "xtile_spmv_GBs": spmv_bytes(nnz, nd, False) / (xtile_spmv_ms * 1e-3) / 1e9,
+        "xtile_prepare_ms": xtile_prep_ms,
+        "csr_spmv_ms": csr_spmv_ms,
         "csr_spmv_GBs": spmv_bytes(nnz, nd, False) / (csr_spmv_ms * 1e-3) / 1e9,
         "davidson_layout": td["dav_sliced"],
         "selected_ci_diag_s": diag_s, "E0": E, "davidson_iterations": td["dav_iters"],
         "davidson_restarts": td["dav_restarts"], "setup_s_host": setup_s,
         "cpu_baseline": cpu_leg,
         "note": "build = host wall time of mhip_ham_build (second call); spmv on the "
-                "column-panel split the Davidson attaches to this matrix (csr_spmv_* the plain "
-                "CSR kernel); solve includes its own build",
+                "column-panel split the Davidson attaches to this matrix (xtile_* the "
+                "panel-major copy with x in LDS, csr_spmv_* the plain CSR kernel; GB/s of the "
+                "CSR's algorithmic bytes); solve includes its own build",
     }
 
 
@@ -773,7 +784,7 @@ def main():
     spmv_calls = sum(r[2]["dav_spmv_calls"] for r in recs)
     dav_ms = sum(r[2]["dav_total_ms"] for r in recs)
     avg_spmv_ms = spmv_ms / max(spmv_calls, 1)
-    sliced = all(r[2]["dav_sliced"] for r in recs)
+    sliced = all(r[2]["dav_sliced"] == 1 for r in recs)
     bytes_spmv = spmv_bytes(nnz, N, sliced)
     spmv_gbs = bytes_spmv / (avg_spmv_ms * 1e-3) / 1e9
     value = nnz * args.steps / (build_ms * 1e-3)

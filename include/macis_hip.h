@@ -92,7 +92,8 @@ typedef struct mhip_timings {
   int64_t asci_nsurvivors; /* unique non-core candidates entering the top-k */
   int64_t ham_path;       /* structure pass of the last build: MHIP_BUILD_* */
   double dav_layout_ms;   /* sliced copy / panel split made for the last Davidson */
-  int64_t dav_sliced;     /* 1: its SpMVs ran on the sliced copy, 2: on column panels */
+  int64_t dav_sliced;     /* 1: its SpMVs ran on the sliced copy, 2: on column panels,
+                             3: on the x-tile panels */
   int64_t asci_nblocks;      /* excitation blocks of the core (block-screened search) */
   int64_t asci_nblocks_kept; /* blocks whose candidates were generated */
   /* matrix bytes one SpMV of the last Davidson streams from its layout: the
@@ -249,6 +250,18 @@ int mhip_spmv(mhip_context* ctx, const mhip_csr* A, int64_t K, double alpha,
  * get no sliced copy and have at least 262,144 columns. *sliced receives 2.
  * Products equal the CSR ones up to the order of the row sums. */
 #define MHIP_SPMV_COLUMN_PANELS 2
+/* MHIP_SPMV_X_TILES attaches instead a panel-major copy of a square matrix
+ * without a sliced copy: columns cut into panels of 12,288 (96 KB of x, held
+ * in LDS while a workgroup streams the panel's entries of its 2,048 rows),
+ * entries stored as 16-bit panel-local columns and values (10 bytes per
+ * nonzero, plus 4 bytes per row and panel). Only on request: on the
+ * 2,000,000-determinant selected-CI matrix it is slower than the column
+ * panels (one 96 KB tile fits a CU's LDS, so a CU's waves synchronise on
+ * every piece of entries); a Davidson on a matrix that has it uses it.
+ * *sliced receives 3. Products equal the CSR ones up to the order of the
+ * row sums (per row: panels in order, each panel's entries in a fixed
+ * order) and are the same bits call to call. */
+#define MHIP_SPMV_X_TILES 3
 int mhip_csr_prepare_spmv(mhip_context* ctx, const mhip_csr* A, int policy, int* sliced);
 /* extract_diagonal_elements (submatrix.hpp:264-295) into DEVICE d_D. */
 int mhip_extract_diagonal(mhip_context* ctx, const mhip_csr* A, double* d_D);

@@ -376,11 +376,12 @@ class Context:
     def extract_diagonal(self, A, d_D):
         check(lib().mhip_extract_diagonal(self.h, A.h, _ptr(d_D)), "extract_diagonal")
 
-    def prepare_spmv(self, A, always=False, panels=False):
+    def prepare_spmv(self, A, always=False, panels=False, xtiles=False):
         """Attach the sliced SpMV copy to A (always: whatever the padding), or
-        with panels=True the column-panel split; True if A has it afterwards."""
+        with panels=True the column-panel split, with xtiles=True the
+        panel-major copy with LDS-staged x tiles; True if A has it afterwards."""
         out = C.c_int(0)
-        policy = 2 if panels else (1 if always else 0)
+        policy = 3 if xtiles else 2 if panels else (1 if always else 0)
         check(lib().mhip_csr_prepare_spmv(self.h, A.h, policy, C.byref(out)),
               "csr_prepare_spmv")
         return bool(out.value)

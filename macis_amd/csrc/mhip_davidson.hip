@@ -676,12 +676,19 @@ bool davidson(mhip_context* ctx, const mhip_csr* A, int64_t max_m,
   {
     const bool had = A->sell || A->panels;
     const auto h0 = std::chrono::steady_clock::now();
-    tm.dav_sliced = prepare_sell(ctx, A) ? 1 : prepare_panels(ctx, A) ? 2 : 0;
+    // (the x-tile copy is used when a caller attached it; on the
+    // 2,000,000-determinant selected-CI matrix it measured 24.9 ms per
+    // product against 14.5 ms for the 8-panel split)
+    tm.dav_sliced = A->xtiles                ? 3
+                    : prepare_sell(ctx, A)   ? 1
+                    : prepare_panels(ctx, A) ? 2
+                                             : 0;
     if(!had && tm.dav_sliced) MHIP_HIP_CHECK(hipStreamSynchronize(st));
     tm.dav_spmv_matrix_bytes =
-        A->sell ? A->sell->streamed_bytes()
-                : 12. * (double)A->nnz + 8. * (double)(A->m + 1) +
-                      (A->panels ? 4. * (double)A->m * A->panels->np : 0.);
+        A->sell     ? A->sell->streamed_bytes()
+        : A->xtiles ? 10. * (double)A->nnz + 4. * (double)A->xtiles->np * (double)(A->m + 1)
+                    : 12. * (double)A->nnz + 8. * (double)(A->m + 1) +
+                          (A->panels ? 4. * (double)A->m * A->panels->np : 0.);
     tm.dav_layout_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
   }
@@ -883,6 +890,10 @@ int mhip_selected_ci_diag(mhip_context* ctx, int words_per_det,
         keep_buffer(ctx, std::move(H->sell->cols));
         keep_buffer(ctx, std::move(H->sell->vals));
         keep_buffer(ctx, std::move(H->sell->base));
+      }
+      if(H->xtiles) {
+        keep_buffer(ctx, std::move(H->xtiles->cols));
+        keep_buffer(ctx, std::move(H->xtiles->vals));
       }
     }
     if(!ok) throw Error(MHIP_ERR_NOT_CONVERGED, "Davidson Did Not Converge!");

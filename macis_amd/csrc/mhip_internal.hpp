@@ -206,6 +206,23 @@ struct PanelSplit {
   DevBuf off;                   // int32[m * np]
   DevBuf part;                  // double[np * m] partial sums
 };
+// Panel-major copy of a square CSR for single-vector SpMVs whose x vector
+// is too large for an XCD's L2: panel p holds columns [p T, (p + 1) T) with
+// T = kTileCols, its entries stored contiguously row after row (columns
+// ascending within a row) as 16-bit local columns and values; rowoff[p][r]
+// (m + 1 per panel) is the first entry of row r in panel p relative to the
+// panel's first entry base[p]. A workgroup stages a panel's share of x in
+// LDS and streams that panel's entries of its rows.
+constexpr int kTileCols = 12288;  // 96 KB of x per panel
+struct XTiles {
+  int np = 0;
+  int64_t entries = 0;
+  std::vector<int64_t> base_h;  // host copy of base (np + 1)
+  DevBuf base;                  // int64[np + 1]
+  DevBuf rowoff;                // uint32[np * (m + 1)]
+  DevBuf cols;                  // uint16[entries]
+  DevBuf vals;                  // double[entries]
+};
 }  // namespace mhip
 
 struct mhip_context {
@@ -235,6 +252,8 @@ struct mhip_csr {
   mutable std::unique_ptr<mhip::SellCopy> sell;
   // optional column-panel split used by single-vector SpMVs (prepare_panels)
   mutable std::unique_ptr<mhip::PanelSplit> panels;
+  // optional panel-major copy with LDS-staged x tiles (prepare_xtiles)
+  mutable std::unique_ptr<mhip::XTiles> xtiles;
 };
 
 namespace mhip {
@@ -275,6 +294,12 @@ bool prepare_sell(mhip_context* ctx, const mhip_csr* A, bool any_padding = false
 // True if the matrix has one afterwards; spmv() then uses it.
 constexpr int64_t kPanelMinCols = 262144;
 bool prepare_panels(mhip_context* ctx, const mhip_csr* A, bool any_size = false);
+// Attaches the panel-major copy with LDS-staged x tiles to a square matrix
+// (no sliced copy) when the device has room for it (10 bytes per nonzero
+// and 4 per row and panel, plus 1 GiB); with any_size false only when x
+// exceeds kPanelMinCols entries. True if the matrix has one afterwards;
+// spmv() then uses it.
+bool prepare_xtiles(mhip_context* ctx, const mhip_csr* A, bool any_size = false);
 // y = A x on the sliced copy (A must have one); with conv, rows r with
 // conv[row_inst[r]] set are skipped (y keeps its value there).
 void spmv_sliced_masked(mhip_context* ctx, const mhip_csr* A, const double* x, double* y,

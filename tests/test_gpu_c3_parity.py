@@ -76,6 +76,12 @@ def test_c3_symmetric_and_lowest_root_is_eigenpair(ctx, c3):
     lens = np.diff(rp)
     tol = 1e-13 * np.abs(nz).max() * np.abs(x).max() * lens.max()
     assert np.max(np.abs(dHx.numpy() - ref)) <= tol
+    announce("C3 SpMV on the x-tile layout vs the host product")
+    H2 = ctx.ham_build(2, dd, N, thresh=EPS)
+    assert ctx.prepare_spmv(H2, xtiles=True)
+    ctx.spmv(H2, 1, 1.0, dx, N, 0.0, dHx, N)
+    assert np.max(np.abs(dHx.numpy() - ref)) <= tol
+    del H2
     announce("C3 restarted Davidson, residual on the host matrix")
     ctx.set_integrals(N_ORB, T, V)
     dX = ctx.empty(N, np.float64).zero()

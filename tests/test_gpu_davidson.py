@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 EPS = np.finfo(float).eps
 
 
-LAYOUTS = ["csr", "sliced", "panels"]
+LAYOUTS = ["csr", "sliced", "panels", "xtiles"]
 
 
 def _layout(ctx, A, layout):
@@ -23,6 +23,8 @@ def _layout(ctx, A, layout):
         assert ctx.prepare_spmv(A, always=True)
     elif layout == "panels":
         assert ctx.prepare_spmv(A, panels=True)
+    elif layout == "xtiles":
+        assert ctx.prepare_spmv(A, xtiles=True)
 
 
 @pytest.mark.parametrize("indexing", [0, 1])
@@ -99,6 +101,56 @@ def test_spmv_ragged_rows(ctx, indexing, layout):
     ref = y0.copy()
     O.gespmbv(1, 0.9, m, rowptr + indexing, colind + indexing, nzval, indexing, x, n, 0.4, ref, m)
     got = dy.numpy()
+    tol = 1e-14 * (np.abs(nzval).max() * np.abs(x).max() * lens.max() + 1)
+    assert np.max(np.abs(got - ref)) <= tol
+
+
+@pytest.mark.parametrize("indexing", [0, 1])
+def test_spmv_xtiles_many_panels(ctx, indexing):
+    """The x-tile layout over 61,440 columns (5 panels of 12,288 staged in
+    LDS, the last one partial) and 4,100 rows (3 row blocks of 2,048, the
+    last partial): ragged rows, empty rows, rows with no entry in some
+    panels, a 30,000-entry row whose runs span several 4,096-entry pieces
+    and rows that begin at a piece boundary; alpha/beta and 0/1-based
+    indices against the oracle's gespmbv, and the same bits call to call."""
+    announce(f"spmv x tiles, many panels, indexing={indexing}")
+    rng = np.random.default_rng(21 + indexing)
+    m, n = 4100, 61440
+    lens = rng.integers(0, 400, m)
+    lens[::7] = 0
+    lens[1000] = 30000
+    lens[2047:2050] = [4096, 1, 4095]
+    rowptr = np.zeros(m + 1, dtype=np.int64)
+    rowptr[1:] = np.cumsum(lens)
+    nnz = int(rowptr[-1])
+
+    def cols(i, L):
+        if i % 5 == 1:  # only the first and last panels
+            pool = np.concatenate([np.arange(0, 12288), np.arange(49152, n)])
+            return np.sort(rng.choice(pool, size=L, replace=False))
+        return np.sort(rng.choice(n, size=L, replace=False))
+
+    colind = np.concatenate([cols(i, L) for i, L in enumerate(lens)]).astype(np.int32)
+    nzval = rng.standard_normal(nnz)
+    A = ctx.csr_from_host(m, n, rowptr + indexing, colind + indexing, nzval, indexing=indexing)
+    from macis_amd._capi import MhipError
+
+    with pytest.raises(MhipError, match="square"):  # m != n: no x tiles
+        ctx.prepare_spmv(A, xtiles=True)
+    # a square matrix: the same rows padded with empty rows up to n
+    rp2 = np.concatenate([rowptr, np.full(n - m, rowptr[-1], dtype=np.int64)])
+    B = ctx.csr_from_host(n, n, rp2 + indexing, colind + indexing, nzval, indexing=indexing)
+    assert ctx.prepare_spmv(B, xtiles=True)
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    dx = ctx.to_device(x)
+    dy1, dy2 = ctx.to_device(y0), ctx.to_device(y0)
+    ctx.spmv(B, 1, 0.7, dx, n, -0.3, dy1, n)
+    ctx.spmv(B, 1, 0.7, dx, n, -0.3, dy2, n)
+    got = dy1.numpy()
+    assert np.array_equal(got, dy2.numpy())
+    ref = y0.copy()
+    O.gespmbv(1, 0.7, n, rp2 + indexing, colind + indexing, nzval, indexing, x, n, -0.3, ref, n)
     tol = 1e-14 * (np.abs(nzval).max() * np.abs(x).max() * lens.max() + 1)
     assert np.max(np.abs(got - ref)) <= tol
 
