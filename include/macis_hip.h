@@ -107,11 +107,17 @@ typedef struct mhip_timings {
  * walk when the determinant list is a complete product of spin strings
  * (every run of equal outer string carries the same inner-string sequence,
  * as the lists of generate_hilbert_space and their bitset_less-sorted form
- * are) and the tiled XOR/popcount double loop otherwise. Both give the same
- * CSR. */
+ * are); otherwise, for square lists of at least 32,768 determinants with one
+ * alpha and one beta electron count, the spin-string group scan (each row
+ * scans the determinants sharing its alpha or its beta string and those of
+ * the single-excitation neighbour strings, then its columns are sorted);
+ * otherwise the tiled XOR/popcount double loop. All give the same CSR.
+ * SPIN_GROUPS forces the group scan (MHIP_ERR_INVALID_ARG when the list is
+ * not square or its electron counts vary). */
 #define MHIP_BUILD_AUTO 0
 #define MHIP_BUILD_TILED 1
 #define MHIP_BUILD_PRODUCT 2
+#define MHIP_BUILD_SPIN_GROUPS 3
 
 /* Settings of asci_search (ASCISettings, determinant_search.hpp:34-51).
  * pair_size_max / rv_prune_tol prune as the reference does (:120-137). */
@@ -138,8 +144,9 @@ int mhip_synchronize(mhip_context* ctx);
 int mhip_get_timings(mhip_context* ctx, mhip_timings* out);
 /* Selects the structure passes of later builds on this context (no
  * counterpart in the reference, which has one serial loop): MHIP_BUILD_AUTO
- * (default), MHIP_BUILD_TILED, or MHIP_BUILD_PRODUCT (fails with
- * MHIP_ERR_INVALID_ARG when a square build's list is not a product space). */
+ * (default), MHIP_BUILD_TILED, MHIP_BUILD_PRODUCT (fails with
+ * MHIP_ERR_INVALID_ARG when a square build's list is not a product space) or
+ * MHIP_BUILD_SPIN_GROUPS (see above). */
 int mhip_context_set_build_path(mhip_context* ctx, int path);
 /* Frees device storage the context keeps for reuse (the last internal
  * Hamiltonian of mhip_selected_ci_diag and the Davidson subspace vectors);

@@ -324,7 +324,7 @@ class Context:
         check(lib().mhip_get_timings(self.h, C.byref(t)), "get_timings")
         return t.as_dict()
 
-    BUILD_AUTO, BUILD_TILED, BUILD_PRODUCT = 0, 1, 2
+    BUILD_AUTO, BUILD_TILED, BUILD_PRODUCT, BUILD_SPIN_GROUPS = 0, 1, 2, 3
 
     def set_build_path(self, path):
         check(lib().mhip_context_set_build_path(self.h, path), "set_build_path")

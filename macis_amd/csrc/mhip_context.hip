@@ -313,7 +313,8 @@ int mhip_context_get_stream(mhip_context* ctx, void** stream) {
 int mhip_context_set_build_path(mhip_context* ctx, int path) {
   return guarded([&] {
     MHIP_REQUIRE(ctx, MHIP_ERR_INVALID_ARG, "null context");
-    MHIP_REQUIRE(path == MHIP_BUILD_AUTO || path == MHIP_BUILD_TILED || path == MHIP_BUILD_PRODUCT,
+    MHIP_REQUIRE(path == MHIP_BUILD_AUTO || path == MHIP_BUILD_TILED ||
+                     path == MHIP_BUILD_PRODUCT || path == MHIP_BUILD_SPIN_GROUPS,
                  MHIP_ERR_INVALID_ARG, "unknown build path");
     ctx->build_path = path;
   });

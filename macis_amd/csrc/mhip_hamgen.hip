@@ -47,6 +47,7 @@ constexpr int kSuper = 64;  // tiles per superblock
 constexpr int kXcdChunk = 64;  // scan blocks per chunk dealt round-robin to the 8 XCDs
 constexpr int kRows = 8;    // bra rows per wavefront in the scan passes
 constexpr int kWavesPerBlock = 4;
+constexpr int64_t kMinGroupRows = 32768;  // AUTO: spin-group scan from this list length
 
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
@@ -574,12 +575,13 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
 
   // 1. bounds (tiled path only)
   const bool prod = ex.product || ex.bproduct;
+  const bool grp = ex.groups != nullptr;
   t.start();
   const int64_t ntiles = (nket + kTile - 1) / kTile;
   const int64_t nsuper = (ntiles + kSuper - 1) / kSuper;
   DevBuf bnd;
   uint64_t *tand = nullptr, *tor = nullptr, *sand = nullptr, *sor = nullptr;
-  if(!prod) {
+  if(!prod && !grp) {
     bnd.alloc(sizeof(uint64_t) * W * 2 * (ntiles + nsuper));
     tand = bnd.as<uint64_t>();
     tor = tand + W * ntiles;
@@ -620,8 +622,8 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   int64_t ket_chunk = W == 1 ? 262144 : 131072;
   if(const char* env = std::getenv("MHIP_SCAN_KET_CHUNK"))
     ket_chunk = std::max<int64_t>(0, std::atoll(env)) / (kTile * kSuper) * (kTile * kSuper);
-  const bool chunked = !prod && !ex.items && ket_chunk > 0 && nket > 2 * ket_chunk;
-  if(!prod && !ex.items) {
+  const bool chunked = !prod && !grp && !ex.items && ket_chunk > 0 && nket > 2 * ket_chunk;
+  if(!prod && !grp && !ex.items) {
     size_t free_b = 0, total_b = 0;
     device_mem_info(&free_b, &total_b);
     const int64_t per_chunk = (int64_t)kChunk * 4 + 8 + 4;
@@ -672,6 +674,8 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
       product_count(ctx, *ex.product, nbra, counts32.as<uint32_t>());
     else if(ex.bproduct)
       batch_product_count(ctx, *ex.bproduct, ex.row_inst, nbra, counts32.as<uint32_t>());
+    else if(grp)
+      spin_group_count(ctx, W, *ex.groups, d_bra, nbra, counts32.as<uint32_t>());
     else if(stage) {
       const int64_t step = chunked ? ket_chunk : nket;
       for(int64_t c0 = 0; c0 < nket || c0 == 0; c0 += step) {
@@ -759,6 +763,10 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
       product_fill_elements(ctx, W, *ex.product, I, nbra, diag.as<double>(),
                             rp_struct.as<int64_t>(), col.as<int32_t>(), val.as<double>(), thresh,
                             counts64.as<int64_t>(), flag.as<int>());
+  } else if(nnz_struct && grp) {
+    // scan-order columns, then every row sorted ascending
+    spin_group_fill(ctx, W, *ex.groups, d_bra, nbra, rp_struct.as<int64_t>(), col.as<int32_t>());
+    sort_row_columns(ctx, nbra, rp_struct.as<int64_t>(), nnz_struct, nket, col);
   } else if(nnz_struct && !staged) {
     hipLaunchKernelGGL((scan_pass_kernel<W, kRows, kFill>), dim3(grid), dim3(256), 0, st,
                        d_bra, nbra, d_ket, nket, tand, tor, ntiles, sand, sor, nsuper,
@@ -818,7 +826,7 @@ void build_impl(mhip_context* ctx, const uint64_t* d_bra, int64_t nbra,
   t_all.stop();
   tm.ham_total_ms = t_all.ms() + ex.detect_ms;
   tm.ham_nnz = csr->nnz;
-  tm.ham_path = prod ? MHIP_BUILD_PRODUCT : MHIP_BUILD_TILED;
+  tm.ham_path = prod ? MHIP_BUILD_PRODUCT : grp ? MHIP_BUILD_SPIN_GROUPS : MHIP_BUILD_TILED;
 }
 
 template void build_impl<1>(mhip_context*, const uint64_t*, int64_t, const uint64_t*, int64_t,
@@ -869,21 +877,40 @@ mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
   validate_dets(ctx, W, d_bra, nbra);
   const bool square = d_ket == d_bra && nket == nbra;
   if(!square) validate_dets(ctx, W, d_ket, nket);
-  // Product-space structure passes for complete spaces (same CSR).
+  // Product-space structure passes for complete spaces, spin-string group
+  // scan for other long lists (same CSR).
   ProductSpace ps;
+  SpinGroups sg;
   BuildExtra ex;
-  if(square && ctx->build_path != MHIP_BUILD_TILED) {
+  const int path = ctx->build_path;
+  if(square && (path == MHIP_BUILD_AUTO || path == MHIP_BUILD_PRODUCT)) {
     MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     const auto t0 = std::chrono::steady_clock::now();
     if(detect_product_space(ctx, W, d_bra, nbra, ps)) ex.product = &ps;
     ex.detect_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    MHIP_REQUIRE(ex.product || ctx->build_path != MHIP_BUILD_PRODUCT, MHIP_ERR_INVALID_ARG,
+    MHIP_REQUIRE(ex.product || path != MHIP_BUILD_PRODUCT, MHIP_ERR_INVALID_ARG,
                  "determinant list is not a product space");
   } else {
-    MHIP_REQUIRE(ctx->build_path != MHIP_BUILD_PRODUCT, MHIP_ERR_INVALID_ARG,
+    MHIP_REQUIRE(path != MHIP_BUILD_PRODUCT, MHIP_ERR_INVALID_ARG,
                  "the product-space build needs a square block (bra list == ket list)");
   }
+  if(!ex.product && square &&
+     (path == MHIP_BUILD_SPIN_GROUPS || (path == MHIP_BUILD_AUTO && nbra >= kMinGroupRows))) {
+    MHIP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    // flat lists within a quarter of the free memory (the CSR comes after)
+    size_t free_b = 0, total_b = 0;
+    device_mem_info(&free_b, &total_b);
+    if(prepare_spin_groups(ctx, W, d_bra, nbra, 0.25 * (double)free_b, sg)) ex.groups = &sg;
+    ex.detect_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    MHIP_REQUIRE(ex.groups || path != MHIP_BUILD_SPIN_GROUPS, MHIP_ERR_INVALID_ARG,
+                 "spin-group build: electron counts vary over the list (or its lists exceed "
+                 "the memory budget)");
+  }
+  MHIP_REQUIRE(square || path != MHIP_BUILD_SPIN_GROUPS, MHIP_ERR_INVALID_ARG,
+               "the spin-group build needs a square block (bra list == ket list)");
   if(W == 1)
     build_impl<1>(ctx, d_bra, nbra, d_ket, nket, thresh, csr.get(), ex);
   else

@@ -365,6 +365,33 @@ void batch_product_fill_elements(mhip_context* ctx, const ProductBatch& pb,
                                  const int64_t* rowptr, int32_t* colind, double* nzval,
                                  double thresh, int64_t* kept, int* dropped);
 // Optional work-item table / per-instance integrals of a batched build.
+// Spin-string groups of a square list with fixed alpha and beta electron
+// counts (mhip_spingroup.hip): per spin, the groups of equal string (start
+// of each group's members in the sorted order), the total membership of
+// each group's single-excitation neighbour groups, and each group's flat
+// list (own members, then the neighbour groups' members) as other-spin
+// string + determinant index.
+struct GroupSide {
+  int64_t ng = 0, total = 0;
+  DevBuf start, nsize, foff, gof, fw, fi;
+};
+struct SpinGroups {
+  int64_t n = 0;
+  GroupSide A, B;
+};
+// False when the counts differ between determinants or the flat lists
+// would exceed max_bytes.
+bool prepare_spin_groups(mhip_context* ctx, int W, const uint64_t* d, int64_t n,
+                         double max_bytes, SpinGroups& g);
+void spin_group_count(mhip_context* ctx, int W, const SpinGroups& g, const uint64_t* d,
+                      int64_t n, uint32_t* counts);
+void spin_group_fill(mhip_context* ctx, int W, const SpinGroups& g, const uint64_t* d, int64_t n,
+                     const int64_t* rowptr, int32_t* col);
+// Ascending columns within every row (segmented radix sort over the bits
+// of ncols); col may be swapped for the sorted buffer.
+void sort_row_columns(mhip_context* ctx, int64_t n, const int64_t* rowptr, int64_t nnz,
+                      int64_t ncols, DevBuf& col);
+
 struct BuildExtra {
   const int64_t* items = nullptr;  // DEVICE, 4 int64 per item
   int64_t nitems = 0;
@@ -372,6 +399,7 @@ struct BuildExtra {
   const int32_t* row_inst = nullptr;  // DEVICE, instance of every row
   const ProductSpace* product = nullptr;  // structure passes of a product list
   const ProductBatch* bproduct = nullptr;  // per-instance product spaces of a batch
+  const SpinGroups* groups = nullptr;      // spin-string groups of a selected-CI list
   double detect_ms = 0.;                  // host time spent recognising it
 };
 template <int W>

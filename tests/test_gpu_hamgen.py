@@ -296,10 +296,10 @@ def test_tiled_build_fallback_paths(ctx, pool, W, monkeypatch):
     if W == 1:
         dets = (dets[:, 0] | (dets[:, 1] << np.uint64(32))).reshape(-1, 1)
     dd = ctx.to_device(dets)
-    ref = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
+    ref = _build_with(ctx, ctx.BUILD_TILED, W, dd, len(dets))
     assert ref[0][-1] > 131072
     monkeypatch.setenv("MHIP_STAGE_POOL_CHUNKS", pool)
-    got = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
+    got = _build_with(ctx, ctx.BUILD_TILED, W, dd, len(dets))
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
     assert_same_csr(got, O.Ham(T, V).csr(W, dets, thresh=EPS))
@@ -322,11 +322,11 @@ def test_tiled_scan_ket_chunks(ctx, W, chunk, pool, monkeypatch):
     if W == 1:
         dets = (dets[:, 0] | (dets[:, 1] << np.uint64(32))).reshape(-1, 1)
     dd = ctx.to_device(dets)
-    ref = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
+    ref = _build_with(ctx, ctx.BUILD_TILED, W, dd, len(dets))
     monkeypatch.setenv("MHIP_SCAN_KET_CHUNK", chunk)
     if pool:
         monkeypatch.setenv("MHIP_STAGE_POOL_CHUNKS", pool)
-    got = ctx.ham_build(W, dd, len(dets), thresh=EPS).to_host()
+    got = _build_with(ctx, ctx.BUILD_TILED, W, dd, len(dets))
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
     assert_same_csr(got, O.Ham(T, V).csr(W, dets, thresh=EPS))
@@ -358,3 +358,109 @@ def test_product_fill_non_uniform_inner_strings(ctx):
         ctx.set_build_path(_capi.Context.BUILD_AUTO)
     assert_same_csr(prod, tiled)
     assert_same_csr(prod, O.Ham(T, V).csr(1, dets, thresh=EPS))
+
+
+def _build_with(ctx, path, W, dd, n):
+    ctx.set_build_path(path)
+    try:
+        H = ctx.ham_build(W, dd, n, thresh=EPS)
+        got = H.to_host()
+        assert ctx.timings()["ham_path"] == path
+    finally:
+        ctx.set_build_path(ctx.BUILD_AUTO)
+    return got
+
+
+def _one_word(dets):
+    return (dets[:, 0] | (dets[:, 1] << np.uint64(32))).reshape(-1, 1)
+
+
+@pytest.mark.parametrize("case", ["sci64", "sci16_w1", "shuffled_w1", "duplicates", "cas_w1"])
+def test_spin_group_build_matches_tiled(ctx, case):
+    """The spin-string group scan (rows scan the determinants that share
+    their alpha or beta string and those of the single-excitation neighbour
+    strings; columns sorted per row afterwards) gives the tiled double
+    loop's CSR bit for bit: selected-CI lists at 64 and 16 orbitals (W = 2
+    and W = 1), a list in random order, a list with repeated determinants
+    and a complete space in generation order; the small lists also against
+    the oracle."""
+    rng = np.random.default_rng(5)
+    if case == "sci64":
+        n, W = 64, 2
+        dets = selected_ci_dets(n, 20, 20, 12000, seed=11, energies=0.25 * np.arange(n))
+    elif case == "sci16_w1":
+        n, W = 16, 1
+        dets = _one_word(selected_ci_dets(n, 5, 5, 4000, seed=12, energies=0.25 * np.arange(n)))
+    elif case == "shuffled_w1":
+        n, W = 16, 1
+        dets = _one_word(selected_ci_dets(n, 6, 5, 6000, seed=13, energies=0.25 * np.arange(n)))
+        dets = dets[rng.permutation(len(dets))]
+    elif case == "duplicates":
+        n, W = 32, 2
+        dets = selected_ci_dets(n, 8, 8, 3000, seed=14, energies=0.25 * np.arange(n))
+        dets = np.concatenate([dets, dets[rng.choice(len(dets), 40, replace=False)]])
+        dets = dets[rng.permutation(len(dets))]
+    else:
+        n, W = 10, 1
+        dets = D.generate_hilbert_space(1, n, 4, 5)
+    announce(f"hamgen spin-group scan vs tiled: {case} ({len(dets)} dets, W={W})")
+    T, V = seeded_integrals(n, 90 + n)
+    ctx.set_integrals(n, T, V)
+    dd = ctx.to_device(np.ascontiguousarray(dets))
+    tiled = _build_with(ctx, ctx.BUILD_TILED, W, dd, len(dets))
+    grp = _build_with(ctx, ctx.BUILD_SPIN_GROUPS, W, dd, len(dets))
+    for a, b in zip(grp, tiled):
+        assert np.array_equal(a, b)
+    if len(dets) <= 6000:
+        assert_same_csr(grp, O.Ham(T, V).csr(W, dets, thresh=EPS))
+
+
+@pytest.mark.parametrize("piece", ["50000", "333333"])
+def test_spin_group_sort_in_pieces(ctx, piece, monkeypatch):
+    """The per-row column sort runs in pieces of whole rows when the CSR
+    holds more entries than one sort call indexes (2^31 - 1);
+    MHIP_ROW_SORT_PIECE lowers the piece so a short list takes that path.
+    The CSR equals the one-piece sort's."""
+    announce(f"hamgen spin-group row sort in pieces of {piece}")
+    n = 64
+    T, V = seeded_integrals(n, 93)
+    ctx.set_integrals(n, T, V)
+    dets = selected_ci_dets(n, 20, 20, 20000, seed=15, energies=0.25 * np.arange(n))
+    dd = ctx.to_device(dets)
+    ref = _build_with(ctx, ctx.BUILD_SPIN_GROUPS, 2, dd, len(dets))
+    assert ref[0][-1] > 3 * int(piece)
+    monkeypatch.setenv("MHIP_ROW_SORT_PIECE", piece)
+    got = _build_with(ctx, ctx.BUILD_SPIN_GROUPS, 2, dd, len(dets))
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+def test_spin_group_refuses_varying_counts(ctx):
+    """A list whose alpha electron count varies is not grouped: the forced
+    path is an error, AUTO on a long such list takes the tiled loop, and a
+    rectangular block cannot take the group scan."""
+    from macis_amd import _capi
+
+    announce("hamgen spin-group scan: varying electron counts, rectangular block")
+    n = 12
+    T, V = seeded_integrals(n, 94)
+    ctx.set_integrals(n, T, V)
+    a = D.generate_hilbert_space(1, n, 6, 6)
+    b = D.generate_hilbert_space(1, n, 5, 6)
+    dets = np.concatenate([a[:30000], b[:8000]])
+    dd = ctx.to_device(dets)
+    ctx.set_build_path(ctx.BUILD_SPIN_GROUPS)
+    try:
+        with pytest.raises(_capi.MhipError, match="electron counts"):
+            ctx.ham_build(1, dd, len(dets), thresh=EPS)
+    finally:
+        ctx.set_build_path(ctx.BUILD_AUTO)
+    H = ctx.ham_build(1, dd, len(dets), thresh=EPS)
+    assert ctx.timings()["ham_path"] == ctx.BUILD_TILED
+    del H
+    ctx.set_build_path(ctx.BUILD_SPIN_GROUPS)
+    try:
+        with pytest.raises(_capi.MhipError, match="square"):
+            ctx.ham_build(1, dd, 100, dd, 200, thresh=EPS)
+    finally:
+        ctx.set_build_path(ctx.BUILD_AUTO)
