@@ -374,10 +374,12 @@ void batch_product_fill_elements(mhip_context* ctx, const ProductBatch& pb,
 struct GroupSide {
   int64_t ng = 0, total = 0;
   DevBuf start, nsize, foff, gof, fw, fi;
+  DevBuf perm;  // determinants in group order (ascending index within a group)
 };
 struct SpinGroups {
   int64_t n = 0;
   GroupSide A, B;
+  DevBuf cnt_b;  // per row: hits of its beta-side scan
 };
 // False when the counts differ between determinants or the flat lists
 // would exceed max_bytes.

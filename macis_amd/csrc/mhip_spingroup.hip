@@ -224,78 +224,100 @@ struct SideView {
   const int32_t* gof;    // group of each determinant
   const uint64_t* fw;    // other-spin string per flat entry
   const int32_t* fi;     // determinant index per flat entry
+  const int32_t* perm;   // determinants in group order
 };
 
-// One wavefront per row: the two list scans described at the top of the
-// file. Count pass: counts[row]; fill pass: columns at rowptr[row] + hits
-// so far, in scan order.
-template <int W, bool FILL>
+constexpr int kXcds = 8;
+constexpr int kChunkBlocks = 16;  // consecutive blocks (64 rows) kept on one XCD
+
+// One side's list scan for every row (one wavefront per row), rows taken in
+// that side's group order so rows sharing a flat list run together; blocks
+// come in chunks of 16 dealt round-robin to the 8 XCDs (the hardware deals
+// single blocks), so a chunk's rows meet in one L2. SIDE 1 (beta): the beta
+// group's list, count into cnt_b / columns from rowptr[row]; SIDE 0 (alpha):
+// the alpha group's list, total count cnt_b + own into counts / columns from
+// rowptr[row] + cnt_b[row]. The row's class C goes with the side whose
+// neighbour part is shorter (alpha on ties), the same choice in both passes.
+template <int W, bool FILL, int SIDE>
 __global__ __launch_bounds__(256) void group_scan_kernel(const uint64_t* __restrict__ d,
-                                                         int64_t n, SideView A, SideView B,
+                                                         int64_t n, int64_t nblocks, SideView A,
+                                                         SideView B,
+                                                         uint32_t* __restrict__ cnt_b,
                                                          uint32_t* __restrict__ counts,
                                                          const int64_t* __restrict__ rowptr,
                                                          int32_t* __restrict__ col) {
-  const int64_t i = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-  if(i >= n) return;
+  const int64_t hb = blockIdx.x, x = hb % kXcds, j = hb / kXcds;
+  const int64_t lb = ((j / kChunkBlocks) * kXcds + x) * kChunkBlocks + j % kChunkBlocks;
+  const int64_t k = lb * kWaves + (threadIdx.x >> 6);
+  if(lb >= nblocks || k >= n) return;  // whole wavefront leaves together
   const uint32_t lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane((int32_t)i);
+  const SideView& S = SIDE ? B : A;
+  const int64_t row = __builtin_amdgcn_readfirstlane(S.perm[k]);
   const uint64_t a = alpha_word<W>(d, row), b = beta_word<W>(d, row);
   const int32_t ga = A.gof[row], gb = B.gof[row];
-  const int64_t aown = A.start[ga + 1] - A.start[ga], anb = A.nsize[ga], ao = A.foff[ga];
-  const int64_t bown = B.start[gb + 1] - B.start[gb], bnb = B.nsize[gb], bo = B.foff[gb];
+  const int64_t anb = A.nsize[ga], bnb = B.nsize[gb];
   const bool via_alpha = anb <= bnb;
-  const int64_t base = FILL ? rowptr[row] : 0;
+  const int32_t g = SIDE ? gb : ga;
+  const int64_t own = S.start[g + 1] - S.start[g], o = S.foff[g];
+  const int64_t len = own + (SIDE ? (via_alpha ? 0 : bnb) : (via_alpha ? anb : 0));
+  // own part: beta side e <= 4 (class B), alpha side e in {2, 4} (class A);
+  // neighbour part: e == 2 (class C)
+  const uint64_t xw = SIDE ? a : b;
+  const int emin = SIDE ? 0 : 2;
+  const uint64_t* __restrict__ fw = S.fw;
+  const int32_t* __restrict__ fi = S.fi;
+  const int64_t base = FILL ? rowptr[row] + (SIDE ? 0 : (int64_t)cnt_b[row]) : 0;
   int64_t cnt = 0;
-  // own part: test own(e); neighbour part: e == 2. Four steps of 64
-  // entries per iteration, their four loads issued before any is tested
-  // (the scan is bound by load latency with one load in flight); in the
-  // fill pass the index loads of a group's hits are issued together too.
-  auto scan = [&](const uint64_t* __restrict__ fw, const int32_t* __restrict__ fi, int64_t o,
-                  int64_t own, int64_t len, uint64_t x, int emin) {
-    constexpr int U = 4;
-    int64_t q0 = 0;
-    for(; q0 + 64 * U <= len; q0 += 64 * U) {
-      uint64_t w[U];
+  // Four steps of 64 entries per iteration, their four loads issued before
+  // any is tested (the scan is bound by load latency with one load in
+  // flight); in the fill pass the index loads of a group's hits are issued
+  // together too.
+  constexpr int U = 4;
+  int64_t q0 = 0;
+  for(; q0 + 64 * U <= len; q0 += 64 * U) {
+    uint64_t w[U];
 #pragma unroll
-      for(int u = 0; u < U; ++u) w[u] = fw[o + q0 + 64 * u + lane];
-      bool hit[U];
-      uint64_t m[U];
+    for(int u = 0; u < U; ++u) w[u] = fw[o + q0 + 64 * u + lane];
+    bool hit[U];
+    uint64_t m[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      const int64_t q = q0 + 64 * u + lane;
+      const int e = popc64(w[u] ^ xw);
+      hit[u] = q < own ? (e >= emin && e <= 4) : e == 2;
+      m[u] = ballot(hit[u]);
+    }
+    if(FILL && (m[0] | m[1] | m[2] | m[3])) {
+      int32_t id[U];
+#pragma unroll
+      for(int u = 0; u < U; ++u) id[u] = hit[u] ? fi[o + q0 + 64 * u + lane] : 0;
+      int64_t c = cnt;
 #pragma unroll
       for(int u = 0; u < U; ++u) {
-        const int64_t q = q0 + 64 * u + lane;
-        const int e = popc64(w[u] ^ x);
-        hit[u] = q < own ? (e >= emin && e <= 4) : e == 2;
-        m[u] = ballot(hit[u]);
+        if(hit[u]) col[base + c + lanes_below(m[u])] = id[u];
+        c += popc64(m[u]);
       }
-      if(FILL && (m[0] | m[1] | m[2] | m[3])) {
-        int32_t id[U];
-#pragma unroll
-        for(int u = 0; u < U; ++u) id[u] = hit[u] ? fi[o + q0 + 64 * u + lane] : 0;
-        int64_t c = cnt;
-#pragma unroll
-        for(int u = 0; u < U; ++u) {
-          if(hit[u]) col[base + c + lanes_below(m[u])] = id[u];
-          c += popc64(m[u]);
-        }
-      }
-#pragma unroll
-      for(int u = 0; u < U; ++u) cnt += popc64(m[u]);
     }
-    for(; q0 < len; q0 += 64) {
-      const int64_t q = q0 + lane;
-      bool hit = false;
-      if(q < len) {
-        const int e = popc64(fw[o + q] ^ x);
-        hit = q < own ? (e >= emin && e <= 4) : e == 2;
-      }
-      const uint64_t m = ballot(hit);
-      if(FILL && hit) col[base + cnt + lanes_below(m)] = fi[o + q];
-      cnt += popc64(m);
+#pragma unroll
+    for(int u = 0; u < U; ++u) cnt += popc64(m[u]);
+  }
+  for(; q0 < len; q0 += 64) {
+    const int64_t q = q0 + lane;
+    bool hit = false;
+    if(q < len) {
+      const int e = popc64(fw[o + q] ^ xw);
+      hit = q < own ? (e >= emin && e <= 4) : e == 2;
     }
-  };
-  scan(B.fw, B.fi, bo, bown, bown + (via_alpha ? 0 : bnb), a, 0);
-  scan(A.fw, A.fi, ao, aown, aown + (via_alpha ? anb : 0), b, 2);
-  if(!FILL && lane == 0) counts[row] = (uint32_t)cnt;
+    const uint64_t m = ballot(hit);
+    if(FILL && hit) col[base + cnt + lanes_below(m)] = fi[o + q];
+    cnt += popc64(m);
+  }
+  if(!FILL && lane == 0) {
+    if(SIDE)
+      cnt_b[row] = (uint32_t)cnt;
+    else
+      counts[row] = cnt_b[row] + (uint32_t)cnt;
+  }
 }
 
 // begin offsets of rows [r0, r0 + m] relative to rowptr[r0]
@@ -386,7 +408,8 @@ void build_side(mhip_context* ctx, int W, int64_t n, const uint64_t* keys, const
 
 SideView view(const GroupSide& S) {
   return SideView{S.start.as<int64_t>(), S.nsize.as<int64_t>(), S.foff.as<int64_t>(),
-                  S.gof.as<int32_t>(), S.fw.as<uint64_t>(), S.fi.as<int32_t>()};
+                  S.gof.as<int32_t>(), S.fw.as<uint64_t>(), S.fi.as<int32_t>(),
+                  S.perm.as<int32_t>()};
 }
 
 }  // namespace
@@ -436,34 +459,48 @@ bool prepare_spin_groups(mhip_context* ctx, int W, const uint64_t* d, int64_t n,
                        no.as<int64_t>(), nb.as<int32_t>(), S.foff.as<int64_t>(), S.ng,
                        S.fw.as<uint64_t>(), S.fi.as<int32_t>());
     MHIP_HIP_CHECK(hipGetLastError());
+    S.perm = std::move(sm);
   }
+  g.cnt_b.alloc(4 * n);
   MHIP_HIP_CHECK(hipStreamSynchronize(st));
   g.n = n;
   return true;
 }
 
+namespace {
+template <bool FILL>
+void launch_scans(mhip_context* ctx, int W, const SpinGroups& g, const uint64_t* d, int64_t n,
+                  uint32_t* counts, const int64_t* rowptr, int32_t* col) {
+  const int64_t nblocks = (n + kWaves - 1) / kWaves;
+  // whole rounds of chunks over the XCDs
+  const int64_t per = (int64_t)kXcds * kChunkBlocks;
+  const dim3 grid((unsigned)((nblocks + per - 1) / per * per));
+  uint32_t* cb = g.cnt_b.as<uint32_t>();
+  const SideView A = view(g.A), B = view(g.B);
+  hipStream_t st = ctx->stream;
+  if(W == 1) {
+    hipLaunchKernelGGL((group_scan_kernel<1, FILL, 1>), grid, dim3(256), 0, st, d, n, nblocks, A,
+                       B, cb, counts, rowptr, col);
+    hipLaunchKernelGGL((group_scan_kernel<1, FILL, 0>), grid, dim3(256), 0, st, d, n, nblocks, A,
+                       B, cb, counts, rowptr, col);
+  } else {
+    hipLaunchKernelGGL((group_scan_kernel<2, FILL, 1>), grid, dim3(256), 0, st, d, n, nblocks, A,
+                       B, cb, counts, rowptr, col);
+    hipLaunchKernelGGL((group_scan_kernel<2, FILL, 0>), grid, dim3(256), 0, st, d, n, nblocks, A,
+                       B, cb, counts, rowptr, col);
+  }
+  MHIP_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
 void spin_group_count(mhip_context* ctx, int W, const SpinGroups& g, const uint64_t* d,
                       int64_t n, uint32_t* counts) {
-  const dim3 grid(grid_of(n, kWaves));
-  if(W == 1)
-    hipLaunchKernelGGL((group_scan_kernel<1, false>), grid, dim3(256), 0, ctx->stream, d, n,
-                       view(g.A), view(g.B), counts, nullptr, nullptr);
-  else
-    hipLaunchKernelGGL((group_scan_kernel<2, false>), grid, dim3(256), 0, ctx->stream, d, n,
-                       view(g.A), view(g.B), counts, nullptr, nullptr);
-  MHIP_HIP_CHECK(hipGetLastError());
+  launch_scans<false>(ctx, W, g, d, n, counts, nullptr, nullptr);
 }
 
 void spin_group_fill(mhip_context* ctx, int W, const SpinGroups& g, const uint64_t* d, int64_t n,
                      const int64_t* rowptr, int32_t* col) {
-  const dim3 grid(grid_of(n, kWaves));
-  if(W == 1)
-    hipLaunchKernelGGL((group_scan_kernel<1, true>), grid, dim3(256), 0, ctx->stream, d, n,
-                       view(g.A), view(g.B), nullptr, rowptr, col);
-  else
-    hipLaunchKernelGGL((group_scan_kernel<2, true>), grid, dim3(256), 0, ctx->stream, d, n,
-                       view(g.A), view(g.B), nullptr, rowptr, col);
-  MHIP_HIP_CHECK(hipGetLastError());
+  launch_scans<true>(ctx, W, g, d, n, nullptr, rowptr, col);
 }
 
 void sort_row_columns(mhip_context* ctx, int64_t n, const int64_t* rowptr, int64_t nnz,
