@@ -106,6 +106,29 @@ __device__ __forceinline__ double me_22(const IntsView& I, uint64_t bra_a,
   return sign * I.V[v1 + n * (o1 + n * (v2 + n * o2))];
 }
 
+// h + row[k1] + row[k2] + ... over the set bits k1 < k2 < ... of m, added
+// one at a time in that order; the loads are issued eight at a time (a
+// lane's terms come from one integral row, so a plain loop waits one load
+// latency per term).
+__device__ __forceinline__ double add_row_terms(double h, const double* __restrict__ row,
+                                                uint64_t m) {
+  while(m) {
+    double t[8];
+    uint64_t mm = m;
+#pragma unroll
+    for(int u = 0; u < 8; ++u) {
+      t[u] = row[mm ? ctz64(mm) : 0u];
+      mm &= mm - 1;
+    }
+#pragma unroll
+    for(int u = 0; u < 8; ++u) {
+      if(m) h += t[u];
+      m &= m - 1;
+    }
+  }
+  return h;
+}
+
 // matrix_element_2 (matrix_elements.hpp:86-106): the sums run over the
 // bra's occupied orbitals in ascending order, same spin then other spin.
 __device__ __forceinline__ double me_2(const IntsView& I, uint64_t bra,
@@ -115,10 +138,8 @@ __device__ __forceinline__ double me_2(const IntsView& I, uint64_t bra,
   const uint32_t o1 = ctz64(ket & ex), v1 = ctz64(bra & ex);
   const double sign = excitation_sign(ket, v1, o1);
   double h = I.T[v1 + n * o1];
-  const double* G_ov = I.G_red + n * (v1 + n * o1);
-  for(uint64_t m = bra_same; m; m &= m - 1) h += G_ov[ctz64(m)];
-  const double* V_ov = I.V_red + n * (v1 + n * o1);
-  for(uint64_t m = bra_othr; m; m &= m - 1) h += V_ov[ctz64(m)];
+  h = add_row_terms(h, I.G_red + n * (v1 + n * o1), bra_same);
+  h = add_row_terms(h, I.V_red + n * (v1 + n * o1), bra_othr);
   return sign * h;
 }
 

@@ -493,12 +493,55 @@ __global__ __launch_bounds__(256) void element_kernel(
   bs.b = uniform_u64(bs.b);
   const double d = diag[row];
   int64_t nk = 0;
-  for(int64_t j = st + lane; j < en; j += 64) {
-    const Spins<W> ks = load_spins<W>(ket, colind[j]);
-    const double h = matrix_element(I, bs.a, bs.b, ks.a, ks.b, d);
-    nzval[j] = h;
-    nk += (fabs(h) > thresh) ? 1 : 0;  // double_loop.hpp:76
+  // Single excitations (a sum over the bra's occupied orbitals each) are
+  // queued per spin in LDS and evaluated 64 at a time, so the sums run once
+  // per 64 singles rather than once per 64 columns with most lanes idle;
+  // the other elements are evaluated in place. Same values, any order.
+  __shared__ int32_t pend[kWavesPerBlock][2][64];
+  const int wv = threadIdx.x >> 6;
+  int npend[2] = {0, 0};  // wavefront-uniform
+  auto flush = [&](int sp) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if(lane < (uint32_t)npend[sp]) {
+      const int64_t j = st + pend[wv][sp][lane];
+      const Spins<W> ks = load_spins<W>(ket, colind[j]);
+      const double h = sp == 0 ? me_2(I, bs.a, ks.a, bs.a ^ ks.a, bs.a, bs.b)
+                               : me_2(I, bs.b, ks.b, bs.b ^ ks.b, bs.b, bs.a);
+      nzval[j] = h;
+      nk += (fabs(h) > thresh) ? 1 : 0;  // double_loop.hpp:76
+    }
+    npend[sp] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  };
+  for(int64_t j0 = st; j0 < en; j0 += 64) {
+    const int64_t j = j0 + lane;
+    bool single[2] = {false, false};
+    if(j < en) {
+      const Spins<W> ks = load_spins<W>(ket, colind[j]);
+      const int ca = popc64(bs.a ^ ks.a), cb = popc64(bs.b ^ ks.b);
+      single[0] = ca == 2 && cb == 0;
+      single[1] = cb == 2 && ca == 0;
+      if(!single[0] && !single[1]) {
+        const double h = matrix_element(I, bs.a, bs.b, ks.a, ks.b, d);
+        nzval[j] = h;
+        nk += (fabs(h) > thresh) ? 1 : 0;  // double_loop.hpp:76
+      }
+    }
+#pragma unroll
+    for(int sp = 0; sp < 2; ++sp) {
+      const uint64_t m = ballot(single[sp]);
+      const int c = popc64(m);
+      if(npend[sp] + c > 64) flush(sp);
+      if(single[sp]) pend[wv][sp][npend[sp] + (int)lanes_below(m)] = (int32_t)(j - st);
+      npend[sp] += c;
+    }
   }
+  flush(0);
+  flush(1);
   nk = wave_sum(nk);
   if(lane == 0) {
     kept[row] = nk;
