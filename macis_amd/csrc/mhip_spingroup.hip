@@ -268,29 +268,32 @@ __global__ __launch_bounds__(256) void group_scan_kernel(const uint64_t* __restr
   const int32_t* __restrict__ fi = S.fi;
   const int64_t base = FILL ? rowptr[row] + (SIDE ? 0 : (int64_t)cnt_b[row]) : 0;
   int64_t cnt = 0;
-  // Four steps of 64 entries per iteration, their four loads issued before
-  // any is tested (the scan is bound by load latency with one load in
-  // flight); in the fill pass the index loads of a group's hits are issued
-  // together too.
-  constexpr int U = 4;
-  int64_t q0 = 0;
-  for(; q0 + 64 * U <= len; q0 += 64 * U) {
+  // Groups of eight 64-entry steps, the group's eight loads issued before
+  // any is tested (the scan is bound by load latency); in the fill pass the
+  // index loads are issued together too, by every lane (lanes without a hit
+  // read the list's first entry), so no load sits in a branch and the
+  // stores wait once.
+  constexpr int U = 8;
+  const int64_t ngrp = len / (64 * U);
+  for(int64_t gi = 0; gi < ngrp; ++gi) {
+    const int64_t q0 = gi * 64 * U;
     uint64_t w[U];
 #pragma unroll
     for(int u = 0; u < U; ++u) w[u] = fw[o + q0 + 64 * u + lane];
     bool hit[U];
-    uint64_t m[U];
+    uint64_t m[U], any = 0;
 #pragma unroll
     for(int u = 0; u < U; ++u) {
       const int64_t q = q0 + 64 * u + lane;
       const int e = popc64(w[u] ^ xw);
       hit[u] = q < own ? (e >= emin && e <= 4) : e == 2;
       m[u] = ballot(hit[u]);
+      any |= m[u];
     }
-    if(FILL && (m[0] | m[1] | m[2] | m[3])) {
+    if(FILL && any) {
       int32_t id[U];
 #pragma unroll
-      for(int u = 0; u < U; ++u) id[u] = hit[u] ? fi[o + q0 + 64 * u + lane] : 0;
+      for(int u = 0; u < U; ++u) id[u] = fi[o + (hit[u] ? q0 + 64 * u + lane : 0)];
       int64_t c = cnt;
 #pragma unroll
       for(int u = 0; u < U; ++u) {
@@ -301,6 +304,7 @@ __global__ __launch_bounds__(256) void group_scan_kernel(const uint64_t* __restr
 #pragma unroll
     for(int u = 0; u < U; ++u) cnt += popc64(m[u]);
   }
+  int64_t q0 = ngrp * 64 * U;
   for(; q0 < len; q0 += 64) {
     const int64_t q = q0 + lane;
     bool hit = false;
