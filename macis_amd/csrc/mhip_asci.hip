@@ -32,7 +32,10 @@
 #include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "mhip_device.hpp"
@@ -122,6 +125,9 @@ struct Cand {
   bool keep;
   uint64_t a, b;  // excited alpha / beta strings
   double rv;
+  // entries of the alpha / beta string tables of the core det's strings
+  // (0: the string itself, 1 + single index, 1 + singles + double index)
+  int32_t ga, gb;
 };
 
 // Evaluates flat candidate f of core det c. With FULL = false only the
@@ -133,6 +139,7 @@ __device__ __forceinline__ Cand candidate(const IntsView& I, const CoreArrays& A
   r.keep = false;
   r.a = r.b = 0;
   r.rv = 0.;
+  r.ga = r.gb = 0;
   const int n = I.n;
   const int na = A.counts[c * 4 + 0], nva = A.counts[c * 4 + 1];
   const int nb = A.counts[c * 4 + 2], nvb = A.counts[c * 4 + 3];
@@ -161,6 +168,7 @@ __device__ __forceinline__ Cand candidate(const IntsView& I, const CoreArrays& A
       const uint64_t fl = (1ull << i) | (1ull << a);
       r.a = spin ? sa : sa ^ fl;
       r.b = spin ? sb ^ fl : sb;
+      (spin ? r.gb : r.ga) = (int32_t)(1 + g);
       h *= excitation_sign(same, a, i);
       const double* eps = spin ? eps_b : eps_a;
       const double hd = hd0 + eps[a] - eps[i] - I.G2_red[a + n * i] - I.G2_red[i + n * a];
@@ -200,6 +208,7 @@ __device__ __forceinline__ Cand candidate(const IntsView& I, const CoreArrays& A
       sign *= excitation_sign(ket, v2, o2);
       r.a = spin ? sa : sa ^ fex;
       r.b = spin ? sb ^ fex : sb;
+      (spin ? r.gb : r.ga) = (int32_t)(1 + (spin ? Sb : Sa) + g);
       double h = sign * Gv;
       const double* eps = spin ? eps_b : eps_a;
       const double* G2 = I.G2_red;
@@ -228,6 +237,8 @@ __device__ __forceinline__ Cand candidate(const IntsView& I, const CoreArrays& A
     const double sign = sgn_a * sgn_b;
     r.a = sa ^ ((1ull << a) | (1ull << i));
     r.b = sb ^ ((1ull << j) | (1ull << b));
+    r.ga = (int32_t)(1 + ia);
+    r.gb = (int32_t)(1 + jb);
     double h = sign * Vv;
     const double* G2 = I.G2_red;
     const double* V2 = I.V2_red;
@@ -254,12 +265,144 @@ __device__ __forceinline__ int64_t find_core(const int64_t* __restrict__ start, 
   return lo;
 }
 
-template <int W, bool FILL>
+// ---- dense determinant keys ----------------------------------------------
+// Every candidate string is the alpha (beta) string of a core det with at
+// most two electrons moved. Per distinct core string a table lists, in the
+// order `candidate` enumerates them, that string, its singles and its
+// doubles, each replaced by its rank among all distinct strings of all
+// tables. A candidate's key is rank_beta << bits_a | rank_alpha: equal
+// determinants get equal keys and the key order is bitset_less order (beta
+// string first), as with the full strings, so the sort, accumulation,
+// pruning replay and top-k run unchanged on keys of bits_a + bits_b bits
+// instead of 2 * 32 * W.
+struct KeyMap {
+  const uint32_t* __restrict__ ta;  // alpha table entries -> rank
+  const uint32_t* __restrict__ tb;
+  const int64_t* __restrict__ oa;   // per core det: its alpha string's table block
+  const int64_t* __restrict__ ob;
+  const uint64_t* __restrict__ ua;  // alpha strings by rank
+  const uint64_t* __restrict__ ub;
+  int bits_a;
+};
+
+__device__ __forceinline__ uint64_t rank_key(const KeyMap& M, int64_t c, int32_t ga, int32_t gb) {
+  return ((uint64_t)M.tb[M.ob[c] + gb] << M.bits_a) | (uint64_t)M.ta[M.oa[c] + ga];
+}
+
+// Thread per distinct string: occupied orbitals ascending, then the first
+// norb - nocc clear bits (the lists prep_kernel builds per core det).
+__global__ void str_lists_kernel(const uint64_t* __restrict__ str, int64_t ns, int norb,
+                                 uint8_t* __restrict__ lists, int32_t* __restrict__ cnt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= ns) return;
+  const uint64_t st = str[k];
+  uint8_t* occ = lists + k * 128;
+  uint8_t* vir = occ + 64;
+  int no = 0;
+  for(uint64_t m = st; m; m &= m - 1) occ[no++] = (uint8_t)ctz64(m);
+  const int nv = norb - no;
+  uint64_t t = ~st;
+  for(int v = 0; v < nv; ++v) {
+    vir[v] = (uint8_t)ctz64(t);
+    t &= t - 1;
+  }
+  cnt[2 * k] = no;
+  cnt[2 * k + 1] = nv;
+}
+
+// Thread per table entry e (block k: boff[k] <= e < boff[k + 1]): the
+// string of entry e - boff[k] in `candidate`'s order, and e as payload.
+__global__ void str_table_kernel(const uint64_t* __restrict__ str, const int64_t* __restrict__ boff,
+                                 int64_t ns, const uint8_t* __restrict__ lists,
+                                 const int32_t* __restrict__ cnt, int64_t E,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ slots) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= E) return;
+  int64_t lo = 0, hi = ns;  // last k with boff[k] <= e
+  while(hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if(boff[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int64_t k = lo, g = e - boff[k];
+  const uint8_t* occ = lists + k * 128;
+  const uint8_t* vir = occ + 64;
+  const int no = cnt[2 * k], nv = cnt[2 * k + 1];
+  const int64_t S = (int64_t)no * nv;
+  uint64_t x = str[k];
+  if(g > 0 && g <= S) {
+    const int64_t q = g - 1;
+    x ^= (1ull << occ[q / nv]) | (1ull << vir[q % nv]);
+  } else if(g > S) {
+    const int64_t q = g - 1 - S, pv = pairs(nv);
+    int ii, jj, aa, bb;
+    tri_decode(q / pv, no, ii, jj);
+    tri_decode(q % pv, nv, aa, bb);
+    x ^= (1ull << occ[ii]) | (1ull << occ[jj]) | (1ull << vir[aa]) | (1ull << vir[bb]);
+  }
+  keys[e] = x;
+  slots[e] = (uint32_t)e;
+}
+
+__global__ void str_head_kernel(const uint64_t* __restrict__ sk, int64_t n,
+                                int32_t* __restrict__ head) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(p < n) head[p] = (p == 0 || sk[p] != sk[p - 1]) ? 1 : 0;
+}
+
+// rank1 = inclusive count of distinct strings: table entries get their
+// string's rank, and each rank its string
+__global__ void str_rank_kernel(const uint64_t* __restrict__ sk, const uint32_t* __restrict__ ss,
+                                const int32_t* __restrict__ rank1, int64_t E,
+                                uint32_t* __restrict__ table, uint64_t* __restrict__ uniq) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(p >= E) return;
+  const int32_t r = rank1[p] - 1;
+  table[ss[p]] = (uint32_t)r;
+  if(p == 0 || rank1[p - 1] != rank1[p]) uniq[r] = sk[p];
+}
+
+// keys of the core dets themselves (entry 0 of their strings' tables)
+__global__ void core_rank_keys_kernel(KeyMap M, int64_t nc, uint64_t* __restrict__ keys) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(c < nc) keys[c] = rank_key(M, c, 0, 0);
+}
+
+template <int W>
+__global__ void rank_keys_to_dets_kernel(KeyMap M, const uint64_t* __restrict__ keys, int64_t n,
+                                         uint64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const uint64_t k = keys[i];
+  const uint64_t a = M.ua[k & ((1ull << M.bits_a) - 1ull)], b = M.ub[k >> M.bits_a];
+  if constexpr(W == 1) {
+    out[i] = a | (b << 32);
+  } else {
+    out[2 * i] = a;
+    out[2 * i + 1] = b;
+  }
+}
+
+template <int W, bool RANK>
+using KeyT = typename std::conditional<RANK || W == 1, uint64_t, u128>::type;
+
+template <int W, bool RANK>
+__device__ __forceinline__ KeyT<W, RANK> cand_key(const KeyMap& M, int64_t c, const Cand& cd) {
+  if constexpr(RANK)
+    return rank_key(M, c, cd.ga, cd.gb);
+  else if constexpr(W == 1)
+    return cd.a | (cd.b << 32);
+  else
+    return ((u128)cd.b << 64) | (u128)cd.a;
+}
+
+template <int W, bool FILL, bool RANK>
 __global__ __launch_bounds__(256) void contrib_kernel(
     IntsView I, CoreArrays A, const int64_t* __restrict__ ncand, int64_t nchunks, double E0,
     double tol, int64_t* __restrict__ chunk_counts, const int64_t* __restrict__ chunk_off,
-    typename std::conditional<W == 1, uint64_t, u128>::type* __restrict__ keys,
-    double* __restrict__ vals) {
+    KeyT<W, RANK>* __restrict__ keys, double* __restrict__ vals, KeyMap M) {
   const int64_t chunk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if(chunk >= nchunks) return;
   const uint32_t lane = threadIdx.x & 63;
@@ -277,10 +420,7 @@ __global__ __launch_bounds__(256) void contrib_kernel(
     if constexpr(FILL) {
       if(cd.keep) {
         const int64_t p = out + lanes_below(m);
-        if constexpr(W == 1)
-          keys[p] = cd.a | (cd.b << 32);
-        else
-          keys[p] = ((u128)cd.b << 64) | (u128)cd.a;
+        keys[p] = cand_key<W, RANK>(M, c, cd);
         vals[p] = cd.rv;
       }
       out += popc64(m);
@@ -440,13 +580,12 @@ __global__ void block_key_kernel(const float* __restrict__ pred, float scale, fl
 
 // Candidate count (FILL = false) and fill of the kept blocks: wavefront per
 // kept block (kept ids ascending, so the fill runs in core-det order).
-template <int W, bool FILL>
+template <int W, bool FILL, bool RANK>
 __global__ __launch_bounds__(256) void block_contrib_kernel(
     IntsView I, CoreArrays A, const int64_t* __restrict__ bstart, const int64_t* __restrict__ kept,
     int64_t nkept, double E0, double tol, int64_t* __restrict__ counts,
-    const int64_t* __restrict__ off,
-    typename std::conditional<W == 1, uint64_t, u128>::type* __restrict__ keys,
-    double* __restrict__ vals) {
+    const int64_t* __restrict__ off, KeyT<W, RANK>* __restrict__ keys,
+    double* __restrict__ vals, KeyMap M) {
   const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if(k >= nkept) return;
   const uint32_t lane = threadIdx.x & 63;
@@ -464,10 +603,7 @@ __global__ __launch_bounds__(256) void block_contrib_kernel(
     if constexpr(FILL) {
       if(cd.keep) {
         const int64_t p = out + lanes_below(m);
-        if constexpr(W == 1)
-          keys[p] = cd.a | (cd.b << 32);
-        else
-          keys[p] = ((u128)cd.b << 64) | (u128)cd.a;
+        keys[p] = cand_key<W, RANK>(M, c, cd);
         vals[p] = cd.rv;
       }
       out += popc64(m);
@@ -812,6 +948,129 @@ void prepare_core(mhip_context* ctx, const IntsView& I, const uint64_t* d_cdets,
   A.just_singles = just_singles;
 }
 
+// Dense keys (KeyMap): the string tables of both spins. False (full-string
+// keys) when MHIP_ASCI_KEYS=bits, when the tables would hold more than a
+// quarter as many entries as there are raw candidates (unless
+// MHIP_ASCI_KEYS=rank), or when the two rank widths exceed 64 bits.
+struct RankPrep {
+  DevBuf ta, tb, oa, ob, ua, ub;
+  KeyMap M{};
+  int bits = 0;  // bits_a + bits_b
+  double ms = 0.;
+};
+
+template <int W>
+bool prepare_ranks(mhip_context* ctx, const uint64_t* d_cdets, int64_t nc, int64_t norb,
+                   bool singles_only, RankPrep& R) {
+  const char* env = std::getenv("MHIP_ASCI_KEYS");
+  const bool force = env && std::string(env) == "rank";
+  if(env && std::string(env) == "bits") return false;
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t st = ctx->stream;
+  std::vector<uint64_t> hc(nc * W);
+  d2h(ctx, hc.data(), d_cdets, 8 * W * nc);
+  struct Side {
+    std::vector<uint64_t> u;
+    std::vector<int64_t> boff, off_c;
+    int64_t E = 0;
+  } side[2];
+  int64_t raw = 0;
+  for(int sp = 0; sp < 2; ++sp) {
+    Side& S = side[sp];
+    std::vector<uint64_t> str(nc);
+    for(int64_t c = 0; c < nc; ++c)
+      str[c] = W == 1 ? (sp ? hc[c] >> 32 : hc[c] & 0xffffffffull) : hc[2 * c + sp];
+    S.u = str;
+    std::sort(S.u.begin(), S.u.end());
+    S.u.erase(std::unique(S.u.begin(), S.u.end()), S.u.end());
+    S.boff.assign(S.u.size() + 1, 0);
+    for(size_t k = 0; k < S.u.size(); ++k) {
+      const int64_t no = __builtin_popcountll(S.u[k]), nv = norb - no;
+      int64_t len = 1 + no * nv;
+      if(!singles_only) len += host_pairs(no) * host_pairs(nv);
+      S.boff[k + 1] = S.boff[k] + len;
+    }
+    S.E = S.boff.back();
+    S.off_c.resize(nc);
+    for(int64_t c = 0; c < nc; ++c)
+      S.off_c[c] = S.boff[std::lower_bound(S.u.begin(), S.u.end(), str[c]) - S.u.begin()];
+  }
+  for(int64_t c = 0; c < nc; ++c) {
+    const int64_t na = __builtin_popcountll(W == 1 ? hc[c] & 0xffffffffull : hc[2 * c]);
+    const int64_t nb = __builtin_popcountll(W == 1 ? hc[c] >> 32 : hc[2 * c + 1]);
+    const int64_t Sa = na * (norb - na), Sb = nb * (norb - nb);
+    raw += Sa + Sb;
+    if(!singles_only)
+      raw += host_pairs(na) * host_pairs(norb - na) + host_pairs(nb) * host_pairs(norb - nb) +
+             Sa * Sb;
+  }
+  const int64_t E = side[0].E + side[1].E;
+  if(side[0].E >= (1ll << 31) || side[1].E >= (1ll << 31)) return false;
+  if(!force && 4 * E > raw) return false;
+  int bits[2] = {0, 0};
+  for(int sp = 0; sp < 2; ++sp) {
+    Side& S = side[sp];
+    const int64_t ns = (int64_t)S.u.size(), En = S.E;
+    DevBuf du(8 * ns), dboff(8 * (ns + 1)), lists(128 * ns), cnt(8 * ns);
+    MHIP_HIP_CHECK(hipMemcpyAsync(du.p, S.u.data(), 8 * ns, hipMemcpyHostToDevice, st));
+    MHIP_HIP_CHECK(hipMemcpyAsync(dboff.p, S.boff.data(), 8 * (ns + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(str_lists_kernel, dim3((ns + 255) / 256), dim3(256), 0, st,
+                       du.as<uint64_t>(), ns, (int)norb, lists.as<uint8_t>(), cnt.as<int32_t>());
+    DevBuf keys(8 * En), slots(4 * En), skeys(8 * En), sslots(4 * En), head(4 * En), rank1(4 * En);
+    hipLaunchKernelGGL(str_table_kernel, dim3((unsigned)((En + 255) / 256)), dim3(256), 0, st,
+                       du.as<uint64_t>(), dboff.as<int64_t>(), ns, lists.as<uint8_t>(),
+                       cnt.as<int32_t>(), En, keys.as<uint64_t>(), slots.as<uint32_t>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    size_t tmp = 0;
+    const unsigned end_bit = W == 1 ? 32u : 64u;
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys.as<uint64_t>(),
+                                             skeys.as<uint64_t>(), slots.as<uint32_t>(),
+                                             sslots.as<uint32_t>(), (size_t)En, 0u, end_bit, st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, keys.as<uint64_t>(),
+                                             skeys.as<uint64_t>(), slots.as<uint32_t>(),
+                                             sslots.as<uint32_t>(), (size_t)En, 0u, end_bit, st));
+    hipLaunchKernelGGL(str_head_kernel, dim3((unsigned)((En + 255) / 256)), dim3(256), 0, st,
+                       skeys.as<uint64_t>(), En, head.as<int32_t>());
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::inclusive_scan(nullptr, tmp, head.as<int32_t>(), rank1.as<int32_t>(),
+                                           (size_t)En, rocprim::plus<int32_t>(), st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::inclusive_scan(ctx->scratch_c.p, tmp, head.as<int32_t>(),
+                                           rank1.as<int32_t>(), (size_t)En,
+                                           rocprim::plus<int32_t>(), st));
+    int32_t nu = 0;
+    d2h(ctx, &nu, rank1.as<int32_t>() + En - 1, 4);
+    DevBuf& table = sp ? R.tb : R.ta;
+    DevBuf& uniq = sp ? R.ub : R.ua;
+    DevBuf& offc = sp ? R.ob : R.oa;
+    table.alloc(4 * En);
+    uniq.alloc(8 * (int64_t)nu);
+    hipLaunchKernelGGL(str_rank_kernel, dim3((unsigned)((En + 255) / 256)), dim3(256), 0, st,
+                       skeys.as<uint64_t>(), sslots.as<uint32_t>(), rank1.as<int32_t>(), En,
+                       table.as<uint32_t>(), uniq.as<uint64_t>());
+    MHIP_HIP_CHECK(hipGetLastError());
+    offc.alloc(8 * nc);
+    MHIP_HIP_CHECK(hipMemcpyAsync(offc.p, S.off_c.data(), 8 * nc, hipMemcpyHostToDevice, st));
+    int b = 1;
+    while(b < 63 && (1ll << b) < (int64_t)nu) ++b;
+    bits[sp] = b;
+    // the host vectors and the temporaries above go out of scope: wait
+    MHIP_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  if(bits[0] + bits[1] > 64) return false;
+  R.bits = bits[0] + bits[1];
+  R.M.ta = R.ta.as<uint32_t>();
+  R.M.tb = R.tb.as<uint32_t>();
+  R.M.oa = R.oa.as<int64_t>();
+  R.M.ob = R.ob.as<int64_t>();
+  R.M.ua = R.ua.as<uint64_t>();
+  R.M.ub = R.ub.as<uint64_t>();
+  R.M.bits_a = bits[0];
+  R.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return true;
+}
+
 // First block of every core det (bstart[nc] = number of blocks).
 inline std::vector<int64_t> block_starts(const std::vector<int>& na, const std::vector<int>& nb,
                                          bool singles) {
@@ -820,11 +1079,13 @@ inline std::vector<int64_t> block_starts(const std::vector<int>& na, const std::
   return bs;
 }
 
-template <int W>
+template <int W, bool RANK>
 int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndets_max,
                     const uint64_t* d_cdets, int64_t nc, double E0, const double* d_C,
-                    int64_t norb, uint64_t* d_out, const SearchMode& mode) {
-  using K = typename std::conditional<W == 1, uint64_t, u128>::type;
+                    int64_t norb, uint64_t* d_out, const SearchMode& mode, const RankPrep* R) {
+  using K = KeyT<W, RANK>;
+  const unsigned kbits = RANK ? (unsigned)R->bits : 32u * W * 2u;
+  const KeyMap M = RANK ? R->M : KeyMap{};
   hipStream_t st = ctx->stream;
   mhip_timings& tm = ctx->timings;
   Timer t_all(st), t(st);
@@ -869,9 +1130,9 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     coff.alloc(8 * (nchunks + 1));
     MHIP_HIP_CHECK(hipMemsetAsync(ccount.p, 0, 8 * (nchunks + 1), st));
     MHIP_REQUIRE(grid < (1ll << 31), MHIP_ERR_INVALID_ARG, "too many candidates");
-    hipLaunchKernelGGL((contrib_kernel<W, false>), dim3(grid), dim3(256), 0, st, I, A,
+    hipLaunchKernelGGL((contrib_kernel<W, false, RANK>), dim3(grid), dim3(256), 0, st, I, A,
                        dn.as<int64_t>(), nchunks, E0, s->h_el_tol, ccount.as<int64_t>(), nullptr,
-                       nullptr, nullptr);
+                       nullptr, nullptr, M);
     MHIP_HIP_CHECK(hipGetLastError());
     MHIP_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp, ccount.as<int64_t>(), coff.as<int64_t>(),
                                            (int64_t)0, (size_t)(nchunks + 1),
@@ -883,9 +1144,9 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     d2h(ctx, &P, coff.as<int64_t>() + nchunks, 8);
     keys.alloc(sizeof(K) * std::max<int64_t>(P, 1));
     vals.alloc(8 * std::max<int64_t>(P, 1));
-    hipLaunchKernelGGL((contrib_kernel<W, true>), dim3(grid), dim3(256), 0, st, I, A,
+    hipLaunchKernelGGL((contrib_kernel<W, true, RANK>), dim3(grid), dim3(256), 0, st, I, A,
                        dn.as<int64_t>(), nchunks, E0, s->h_el_tol, nullptr, coff.as<int64_t>(),
-                       keys.as<K>(), vals.as<double>());
+                       keys.as<K>(), vals.as<double>(), M);
     MHIP_HIP_CHECK(hipGetLastError());
   } else {
     // ---- block screen: score every excitation block, keep the best
@@ -949,9 +1210,9 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     coff.alloc(8 * (nkept + 1));
     MHIP_HIP_CHECK(hipMemsetAsync(ccount.p, 0, 8 * (nkept + 1), st));
     if(nblk)
-      hipLaunchKernelGGL((block_contrib_kernel<W, false>), dim3(kgrid), dim3(256), 0, st, I, A,
-                         dbs.as<int64_t>(), kept.as<int64_t>(), nkept, E0, s->h_el_tol,
-                         ccount.as<int64_t>(), nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL((block_contrib_kernel<W, false, RANK>), dim3(kgrid), dim3(256), 0, st,
+                         I, A, dbs.as<int64_t>(), kept.as<int64_t>(), nkept, E0, s->h_el_tol,
+                         ccount.as<int64_t>(), nullptr, nullptr, nullptr, M);
     MHIP_HIP_CHECK(hipGetLastError());
     tmp = 0;
     MHIP_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp, ccount.as<int64_t>(), coff.as<int64_t>(),
@@ -965,15 +1226,15 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     keys.alloc(sizeof(K) * std::max<int64_t>(P, 1));
     vals.alloc(8 * std::max<int64_t>(P, 1));
     if(nblk)
-      hipLaunchKernelGGL((block_contrib_kernel<W, true>), dim3(kgrid), dim3(256), 0, st, I, A,
-                         dbs.as<int64_t>(), kept.as<int64_t>(), nkept, E0, s->h_el_tol, nullptr,
-                         coff.as<int64_t>(), keys.as<K>(), vals.as<double>());
+      hipLaunchKernelGGL((block_contrib_kernel<W, true, RANK>), dim3(kgrid), dim3(256), 0, st,
+                         I, A, dbs.as<int64_t>(), kept.as<int64_t>(), nkept, E0, s->h_el_tol,
+                         nullptr, coff.as<int64_t>(), keys.as<K>(), vals.as<double>(), M);
     MHIP_HIP_CHECK(hipGetLastError());
     tm.asci_nblocks = nblk;
     tm.asci_nblocks_kept = nkept;
   }
   t.stop();
-  tm.asci_contrib_ms = t.ms();
+  tm.asci_contrib_ms = t.ms() + (RANK ? R->ms : 0.);
   tm.asci_npairs = P;
 
   // ---- sort + accumulate
@@ -993,11 +1254,11 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     tmp = 0;
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys.as<K>(), keys2.as<K>(),
                                              rs.as<RvSlot>(), rs2.as<RvSlot>(), (size_t)P, 0,
-                                             32 * W * 2, st));
+                                             kbits, st));
     ctx->scratch_c.reserve(tmp + 16);
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, keys.as<K>(), keys2.as<K>(),
                                              rs.as<RvSlot>(), rs2.as<RvSlot>(), (size_t)P, 0,
-                                             32 * W * 2, st));
+                                             kbits, st));
     rs.release();
     // per core det: first slot, pairs and pairs with |rv| <= rv_prune_tol
     std::vector<int64_t> hcnt(nchunks), htiny(nchunks), dslot(nc + 1, 0), tdet(nc, 0);
@@ -1070,11 +1331,11 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     tmp = 0;
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys.as<K>(), keys2.as<K>(),
                                              vals.as<double>(), vals2.as<double>(), (size_t)P,
-                                             0, 32 * W * 2, st));
+                                             0, kbits, st));
     ctx->scratch_c.reserve(tmp + 16);
     MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, keys.as<K>(), keys2.as<K>(),
                                              vals.as<double>(), vals2.as<double>(), (size_t)P,
-                                             0, 32 * W * 2, st));
+                                             0, kbits, st));
   }
   if(P) {
     tmp = 0;
@@ -1098,14 +1359,18 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   // ---- exclude core dets and zero scores, top-k
   t.start();
   DevBuf core_keys(sizeof(K) * std::max<int64_t>(nc, 1)), core_sorted(sizeof(K) * std::max<int64_t>(nc, 1));
-  hipLaunchKernelGGL((dets_to_keys_kernel<W, K>), dim3((nc + 255) / 256), dim3(256), 0, st,
-                     d_cdets, nc, core_keys.as<K>());
+  if constexpr(RANK)
+    hipLaunchKernelGGL(core_rank_keys_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, M, nc,
+                       core_keys.as<K>());
+  else
+    hipLaunchKernelGGL((dets_to_keys_kernel<W, K>), dim3((nc + 255) / 256), dim3(256), 0, st,
+                       d_cdets, nc, core_keys.as<K>());
   tmp = 0;
   MHIP_HIP_CHECK(rocprim::radix_sort_keys(nullptr, tmp, core_keys.as<K>(), core_sorted.as<K>(),
-                                          (size_t)nc, 0, 32 * W * 2, st));
+                                          (size_t)nc, 0, kbits, st));
   ctx->scratch_c.reserve(tmp + 16);
   MHIP_HIP_CHECK(rocprim::radix_sort_keys(ctx->scratch_c.p, tmp, core_keys.as<K>(),
-                                          core_sorted.as<K>(), (size_t)nc, 0, 32 * W * 2, st));
+                                          core_sorted.as<K>(), (size_t)nc, 0, kbits, st));
   const int64_t top_k = ndets_max - nc;
   if(!mode.exp_dets) {
     // Plain and block-screened search: exclusion and top-k straight on the accumulated arrays.
@@ -1182,8 +1447,12 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
         outk = skeys.as<K>();
         nout = top_k;
       }
-      hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nout + 255) / 256), dim3(256), 0, st,
-                         outk, nout, d_out);
+      if constexpr(RANK)
+        hipLaunchKernelGGL(rank_keys_to_dets_kernel<W>, dim3((nout + 255) / 256), dim3(256), 0,
+                           st, M, outk, nout, d_out);
+      else
+        hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nout + 255) / 256), dim3(256), 0,
+                           st, outk, nout, d_out);
     }
     if(nc)
       MHIP_HIP_CHECK(hipMemcpyAsync(d_out + W * nout, d_cdets, 8 * W * nc,
@@ -1192,7 +1461,7 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     t.stop();
     tm.asci_topk_ms = t.ms();
     t_all.stop();
-    tm.asci_total_ms = t_all.ms();
+    tm.asci_total_ms = t_all.ms() + (RANK ? R->ms : 0.);
     return nout + nc;
   }
   int64_t nkeep = 0;
@@ -1217,8 +1486,12 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   tm.asci_nsurvivors = nkeep;
   MHIP_REQUIRE(nkeep <= mode.exp_cap, MHIP_ERR_INVALID_ARG, "candidate export capacity too small");
   if(nkeep) {
-    hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nkeep + 255) / 256), dim3(256), 0, st,
-                       kkeys.as<K>(), nkeep, mode.exp_dets);
+    if constexpr(RANK)
+      hipLaunchKernelGGL(rank_keys_to_dets_kernel<W>, dim3((nkeep + 255) / 256), dim3(256), 0, st,
+                         M, kkeys.as<K>(), nkeep, mode.exp_dets);
+    else
+      hipLaunchKernelGGL((keys_to_dets_kernel<W, K>), dim3((nkeep + 255) / 256), dim3(256), 0,
+                         st, kkeys.as<K>(), nkeep, mode.exp_dets);
     hipLaunchKernelGGL(bits_to_double_kernel, dim3((nkeep + 255) / 256), dim3(256), 0, st,
                        kabs.as<uint64_t>(), nkeep, mode.exp_abs);
   }
@@ -1228,8 +1501,21 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
   t.stop();
   tm.asci_topk_ms = t.ms();
   t_all.stop();
-  tm.asci_total_ms = t_all.ms();
+  tm.asci_total_ms = t_all.ms() + (RANK ? R->ms : 0.);
   return 0;
+}
+
+// Dense keys when their tables are small against the candidates, else
+// keys of the full strings (same result).
+template <int W>
+int64_t search(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndets_max,
+               const uint64_t* d_cdets, int64_t nc, double E0, const double* d_C, int64_t norb,
+               uint64_t* d_out, const SearchMode& mode) {
+  RankPrep R;
+  if(prepare_ranks<W>(ctx, d_cdets, nc, norb, s->just_singles != 0, R))
+    return search_impl<W, true>(ctx, s, ndets_max, d_cdets, nc, E0, d_C, norb, d_out, mode, &R);
+  return search_impl<W, false>(ctx, s, ndets_max, d_cdets, nc, E0, d_C, norb, d_out, mode,
+                               nullptr);
 }
 
 }  // namespace
@@ -1258,10 +1544,10 @@ extern "C" int mhip_asci_search(mhip_context* ctx, int words_per_det,
     *n_new = 0;
     if(!ncdets) return;
     if(words_per_det == 1)
-      *n_new = search_impl<1>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
+      *n_new = search<1>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
                               d_new_dets, SearchMode{});
     else
-      *n_new = search_impl<2>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
+      *n_new = search<2>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb,
                               d_new_dets, SearchMode{});
   });
 }
@@ -1281,10 +1567,10 @@ int asci_entry(mhip_context* ctx, int W, const mhip_asci_settings* settings, int
                  "bad core determinants");
     *n_new = 0;
     if(W == 1)
-      *n_new = search_impl<1>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out,
+      *n_new = search<1>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out,
                               mode);
     else
-      *n_new = search_impl<2>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out,
+      *n_new = search<2>(ctx, settings, ndets_max, d_cdets, ncdets, E_ASCI, d_C, norb, d_out,
                               mode);
   });
 }

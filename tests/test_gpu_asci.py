@@ -262,3 +262,59 @@ def test_asci_search_pruning_limits(ctx, h2o, h2o_oracle, tol, frac):
     got, ref, ref_rv = run_both(ctx, h2o_oracle, 1, norb, core, C, E2, 3000,
                                 pair_size_max=int(frac * npairs), rv_prune_tol=tol)
     check_topk(got, ref, ref_rv, 100)
+
+
+@pytest.mark.parametrize("case", ["w1_pruned", "w1_singles", "w2"])
+def test_asci_key_forms_agree(ctx, h2o, h2o_oracle, monkeypatch, case):
+    """The search sorts either the full determinant strings or dense keys
+    (rank of the beta string, rank of the alpha string among all strings one
+    or two excitations from a core string); both orders are bitset_less, so
+    the selected determinants, in order, and the exported candidates with
+    their scores are identical. MHIP_ASCI_KEYS forces one form."""
+    announce(f"asci_search dense vs full-string keys: {case}")
+    kw = {}
+    if case == "w2":
+        W, norb = 2, 64
+        T, V = seeded_integrals(norb, 31)
+        orc = O.Ham(T, V)
+        core = selected_ci_dets(norb, 20, 20, 30, seed=4, T=T)
+        C = np.random.default_rng(5).standard_normal(30)
+        C /= np.linalg.norm(C)
+        E = orc.matrix_element(2, core[0], core[0]) - 1.0
+        ndets_max = 20000
+    else:
+        W = 1
+        norb, _, T, V = h2o
+        hf = D.canonical_hf_determinant(1, 5, 5)
+        E0 = h2o_oracle.matrix_element(1, hf, hf)
+        s = O.ASCISettings.defaults()
+        E1, w, X = h2o_oracle.asci_iter(1, s, 100, E0, hf, np.array([1.0]))
+        E, w, X = h2o_oracle.asci_iter(1, s, 800, E1, w, X)
+        order = np.argsort(-np.abs(X), kind="stable")
+        core, C = w[order[:100]], X[order[:100]]
+        ndets_max = 3000
+        if case == "w1_singles":
+            kw["just_singles"] = 1
+        else:
+            ctx.set_integrals(norb, T, V)
+            # pair_size_max at a quarter of the raw pairs: prune events replayed
+            out = ctx.empty((ndets_max, W), np.uint64)
+            ctx.asci_search(W, _capi.ASCISettings.defaults(), ndets_max, ctx.to_device(core),
+                            len(core), E, ctx.to_device(C), norb, out)
+            npairs = ctx.timings()["asci_npairs"]
+            kw["pair_size_max"] = npairs // 4
+    ctx.set_integrals(norb, T, V)
+    s_gpu = _capi.ASCISettings.defaults(**kw)
+    dcore, dC = ctx.to_device(core), ctx.to_device(C)
+    res = {}
+    for keys in ("rank", "bits"):
+        monkeypatch.setenv("MHIP_ASCI_KEYS", keys)
+        out = ctx.empty((ndets_max, W), np.uint64)
+        n = ctx.asci_search(W, s_gpu, ndets_max, dcore, len(core), E, dC, norb, out)
+        sel = out.numpy()[:n].copy()
+        cap = ctx.timings()["asci_npairs"] + 16
+        d_out, a_out, m = ctx.asci_candidates(W, s_gpu, dcore, len(core), E, dC, norb, cap)
+        res[keys] = (sel, d_out.numpy()[:m].copy(), a_out.numpy()[:m].copy())
+    for a, b in zip(res["rank"], res["bits"]):
+        assert np.array_equal(a, b)
+    assert len(res["rank"][0]) == ndets_max
