@@ -705,24 +705,51 @@ __global__ __launch_bounds__(256) void prune_drop_acc_kernel(
 // back to the segment head for its accumulated score and forward over the
 // run. cnt[0] += segments gaining an entry; new entries with |acc| <= tol
 // are marked accf = 2 and listed (cnt[1] of them) for the next prune.
+// hd[j] = j where sorted position j starts a determinant's segment, else 0
+// (an inclusive max-scan then gives every position its segment head)
 template <typename K>
+__global__ void seg_head_kernel(const K* __restrict__ keys, int64_t P, int64_t* __restrict__ hd) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(j < P) hd[j] = (j == 0 || keys[j - 1] != keys[j]) ? j : 0;
+}
+
+// SEQ: threads run over the sorted positions and take those whose slot is
+// in range (a streaming pass, cheaper for an event that covers a large
+// share of the pairs); otherwise over the slots, each finding its sorted
+// position through pos. Same leaders, same result.
+template <typename K, bool SEQ>
 __global__ __launch_bounds__(256) void prune_stage2_kernel(
     const K* __restrict__ keys, const RvSlot* __restrict__ rs, int64_t P,
-    const int64_t* __restrict__ pos, int64_t lim0, int64_t lim1, double tol,
-    double* __restrict__ acc, uint8_t* __restrict__ accf, uint8_t* __restrict__ alive,
-    unsigned long long* __restrict__ cnt, int64_t* __restrict__ list) {
-  const int64_t sl = lim0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t* __restrict__ pos, const int64_t* __restrict__ head, int64_t lim0,
+    int64_t lim1, double tol, double* __restrict__ acc, uint8_t* __restrict__ accf,
+    uint8_t* __restrict__ alive, unsigned long long* __restrict__ cnt,
+    int64_t* __restrict__ list) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   unsigned long long gained = 0;
-  if(sl < lim1) {
-    const int64_t j = pos[sl];
-    int64_t h = j;
-    while(h > 0 && keys[h - 1] == keys[j]) --h;
-    // leader: the first pair of the segment inside [lim0, lim1)
-    if(j == h || rs[j - 1].slot < lim0) {
+  bool in;
+  int64_t j = 0;
+  if constexpr(SEQ) {
+    j = t;
+    in = t < P;
+    if(in) {
+      const int64_t sl = rs[t].slot;
+      in = sl >= lim0 && sl < lim1;
+    }
+  } else {
+    in = lim0 + t < lim1;
+    if(in) j = pos[lim0 + t];
+  }
+  if(in) {
+    const K kj = keys[j];
+    // leader: the first pair of the segment inside [lim0, lim1) (the
+    // segment's pairs are in slot order); it folds for the segment, whose
+    // accumulated score sits at the segment head
+    if(j == 0 || keys[j - 1] != kj || rs[j - 1].slot < lim0) {
+      const int64_t h = head[j];
       const uint8_t had = accf[h];
       bool any = had != 0;
       double a = any ? acc[h] : 0.;
-      for(int64_t k = j; k < P && keys[k] == keys[h] && rs[k].slot < lim1; ++k) {
+      for(int64_t k = j; k < P && keys[k] == kj && rs[k].slot < lim1; ++k) {
         const double v = rs[k].rv;
         if(fabs(v) <= tol) {
           alive[k] = 0;
@@ -1275,6 +1302,18 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     DevBuf pos(8 * P), acc(8 * P), accf(P), alive(P), pcnt(16), slist(8 * P);
     hipLaunchKernelGGL(prune_pos_kernel, dim3(pgrid), dim3(256), 0, st, rs2.as<RvSlot>(), P,
                        pos.as<int64_t>());
+    // segment head of every sorted position (slist serves as the scan input
+    // before the events use it)
+    DevBuf shead(8 * P);
+    hipLaunchKernelGGL((seg_head_kernel<K>), dim3(pgrid), dim3(256), 0, st, keys2.as<K>(), P,
+                       slist.as<int64_t>());
+    tmp = 0;
+    MHIP_HIP_CHECK(rocprim::inclusive_scan(nullptr, tmp, slist.as<int64_t>(), shead.as<int64_t>(),
+                                           (size_t)P, rocprim::maximum<int64_t>(), st));
+    ctx->scratch_c.reserve(tmp + 16);
+    MHIP_HIP_CHECK(rocprim::inclusive_scan(ctx->scratch_c.p, tmp, slist.as<int64_t>(),
+                                           shead.as<int64_t>(), (size_t)P,
+                                           rocprim::maximum<int64_t>(), st));
     MHIP_HIP_CHECK(hipMemsetAsync(accf.p, 0, P, st));
     MHIP_HIP_CHECK(hipMemsetAsync(alive.p, 1, P, st));
     // S: the reference's list length; nseg: accumulated entries (stage 2);
@@ -1294,6 +1333,7 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
       pending_list = 0;
       marks = false;
     };
+    const char* stream_env = std::getenv("MHIP_PRUNE_STREAM");
     for(int64_t c = 0; c < nc; ++c) {
       S += dslot[c + 1] - dslot[c];
       tiny_since += tdet[c];
@@ -1304,12 +1344,25 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
         if(marks) drop_marked();
         const int64_t lim0 = dslot[last_s2 + 1], lim1 = dslot[c + 1];
         MHIP_HIP_CHECK(hipMemsetAsync(pcnt.p, 0, 16, st));
-        if(lim1 > lim0)
-          hipLaunchKernelGGL((prune_stage2_kernel<K>), dim3((unsigned)((lim1 - lim0 + 255) / 256)),
+        // an event over more than 1/16 of the pairs streams the sorted list
+        // (24 bytes a pair) instead of three random lines a slot
+        // (MHIP_PRUNE_STREAM=0 / 1: never / always, for tests)
+        const bool stream = stream_env ? stream_env[0] == '1' : 16 * (lim1 - lim0) > P;
+        if(lim1 > lim0 && stream)
+          hipLaunchKernelGGL((prune_stage2_kernel<K, true>), dim3((unsigned)((P + 255) / 256)),
                              dim3(256), 0, st, keys2.as<K>(), rs2.as<RvSlot>(), P,
-                             pos.as<int64_t>(), lim0, lim1, s->rv_prune_tol, acc.as<double>(),
+                             pos.as<int64_t>(), shead.as<int64_t>(), lim0, lim1,
+                             s->rv_prune_tol, acc.as<double>(),
                              accf.as<uint8_t>(), alive.as<uint8_t>(),
                              pcnt.as<unsigned long long>(), slist.as<int64_t>());
+        else if(lim1 > lim0)
+          hipLaunchKernelGGL((prune_stage2_kernel<K, false>),
+                             dim3((unsigned)((lim1 - lim0 + 255) / 256)), dim3(256), 0, st,
+                             keys2.as<K>(), rs2.as<RvSlot>(), P, pos.as<int64_t>(),
+                             shead.as<int64_t>(), lim0, lim1, s->rv_prune_tol,
+                             acc.as<double>(), accf.as<uint8_t>(),
+                             alive.as<uint8_t>(), pcnt.as<unsigned long long>(),
+                             slist.as<int64_t>());
         MHIP_HIP_CHECK(hipGetLastError());
         int64_t h2[2] = {0, 0};
         d2h(ctx, h2, pcnt.p, 16);

@@ -123,15 +123,21 @@ def _prune_events(counts, tiny, pmax):
 PRUNE_CASES = [(0.6, 0.6), (0.4, 0.6), (0.25, 0.3), (0.15, 0.5), (0.08, 0.3)]
 
 
+@pytest.mark.parametrize("stream", ["auto", "0", "1"])
 @pytest.mark.parametrize("frac,q", PRUNE_CASES)
-def test_asci_search_pair_size_max_pruning(ctx, h2o, h2o_oracle, frac, q):
+def test_asci_search_pair_size_max_pruning(ctx, h2o, h2o_oracle, frac, q, stream, monkeypatch):
     """pair_size_max below the raw candidate count: the GPU replays the
     reference's prunes (stage 1: erase |rv| <= rv_prune_tol; stage 2: sort
     and accumulate; later prunes test accumulated scores) and its top-k
     matches the oracle's asci_search, which runs the reference loop. The
     cases cover prunes with stage 1 only and with stage 2, and the pruning
-    must change the result against the unpruned search."""
-    announce(f"asci_search pruning pair_size_max = {frac} x pairs, tol = q{q}")
+    must change the result against the unpruned search. Stage 2 runs over
+    the slots of its event or, for events over a large share of the pairs,
+    streams the sorted list (MHIP_PRUNE_STREAM forces either); the result is
+    the same."""
+    announce(f"asci_search pruning pair_size_max = {frac} x pairs, tol = q{q}, stream {stream}")
+    if stream != "auto":
+        monkeypatch.setenv("MHIP_PRUNE_STREAM", stream)
     norb, _, T, V = h2o
     ctx.set_integrals(norb, T, V)
     hf = D.canonical_hf_determinant(1, 5, 5)
