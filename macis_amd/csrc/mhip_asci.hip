@@ -47,6 +47,7 @@ IntsView device_view(const Integrals& I);
 namespace {
 
 constexpr int kChunk = 4096;  // candidates per wavefront chunk
+constexpr int64_t kStage2Blocks = 8192;  // grid cap of the pruning stage-2 passes
 using u128 = rocprim::uint128_t;
 
 struct CoreArrays {
@@ -724,46 +725,84 @@ __global__ __launch_bounds__(256) void prune_stage2_kernel(
     int64_t lim1, double tol, double* __restrict__ acc, uint8_t* __restrict__ accf,
     uint8_t* __restrict__ alive, unsigned long long* __restrict__ cnt,
     int64_t* __restrict__ list) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  // grid-stride over the items (sorted positions with SEQ, else slots
+  // lim0...), at most a few thousand blocks: each block adds its count with
+  // one atomic on a single counter, which at one block per 256 items had
+  // serialised the large events
+  const int64_t N = SEQ ? P : lim1 - lim0;
   unsigned long long gained = 0;
-  bool in;
-  int64_t j = 0;
-  if constexpr(SEQ) {
-    j = t;
-    in = t < P;
-    if(in) {
-      const int64_t sl = rs[t].slot;
-      in = sl >= lim0 && sl < lim1;
+  for(int64_t b0 = (int64_t)blockIdx.x * 256; b0 < N; b0 += (int64_t)gridDim.x * 256) {
+    const int64_t t = b0 + threadIdx.x;
+    bool push = false;
+    int64_t hp = 0;
+    bool in;
+    int64_t j = 0;
+    if constexpr(SEQ) {
+      j = t;
+      in = t < P;
+      if(in) {
+        const int64_t sl = rs[t].slot;
+        in = sl >= lim0 && sl < lim1;
+      }
+    } else {
+      in = t < N;
+      if(in) j = pos[lim0 + t];
     }
-  } else {
-    in = lim0 + t < lim1;
-    if(in) j = pos[lim0 + t];
-  }
-  if(in) {
-    const K kj = keys[j];
-    // leader: the first pair of the segment inside [lim0, lim1) (the
-    // segment's pairs are in slot order); it folds for the segment, whose
-    // accumulated score sits at the segment head
-    if(j == 0 || keys[j - 1] != kj || rs[j - 1].slot < lim0) {
-      const int64_t h = head[j];
-      const uint8_t had = accf[h];
-      bool any = had != 0;
-      double a = any ? acc[h] : 0.;
-      for(int64_t k = j; k < P && keys[k] == kj && rs[k].slot < lim1; ++k) {
-        const double v = rs[k].rv;
-        if(fabs(v) <= tol) {
-          alive[k] = 0;
-          continue;
+    if(in) {
+      const K kj = keys[j];
+      // leader: the first pair of the segment inside [lim0, lim1) (the
+      // segment's pairs are in slot order); it folds for the segment, whose
+      // accumulated score sits at the segment head
+      if(j == 0 || keys[j - 1] != kj || rs[j - 1].slot < lim0) {
+        const int64_t h = head[j];
+        const uint8_t had = accf[h];
+        bool any = had != 0;
+        double a = any ? acc[h] : 0.;
+        // the segment's pairs from j while their slot is below lim1, folded
+        // in order; four pairs are loaded per step (a popular determinant
+        // has hundreds)
+        bool go = true;
+        for(int64_t k0 = j; go; k0 += 4) {
+          K kk[4];
+          RvSlot rr[4];
+#pragma unroll
+          for(int u = 0; u < 4; ++u) {
+            const int64_t k = k0 + u < P ? k0 + u : P - 1;
+            kk[u] = keys[k];
+            rr[u] = rs[k];
+          }
+#pragma unroll
+          for(int u = 0; u < 4; ++u) {
+            go = go && k0 + u < P && kk[u] == kj && rr[u].slot < lim1;
+            if(go) {
+              const double v = rr[u].rv;
+              if(fabs(v) <= tol) {
+                alive[k0 + u] = 0;
+              } else {
+                a = any ? a + v : v;
+                any = true;
+              }
+            }
+          }
         }
-        a = any ? a + v : v;
-        any = true;
+        if(any) {
+          acc[h] = a;
+          accf[h] = fabs(a) <= tol ? 2 : 1;
+          gained += had ? 0ull : 1ull;
+          push = fabs(a) <= tol;
+          hp = h;
+        }
       }
-      if(any) {
-        acc[h] = a;
-        accf[h] = fabs(a) <= tol ? 2 : 1;
-        gained = had ? 0ull : 1ull;
-        if(fabs(a) <= tol) list[atomicAdd(cnt + 1, 1ull)] = h;
-      }
+    }
+    // small entries appended with one atomic per wavefront; list order does
+    // not matter
+    const uint64_t m = ballot(push);
+    if(m) {
+      const uint32_t lane = threadIdx.x & 63, first = (uint32_t)ctz64(m);
+      unsigned long long base = 0;
+      if(lane == first) base = atomicAdd(cnt + 1, (unsigned long long)popc64(m));
+      base = __shfl(base, (int)first, 64);
+      if(push) list[base + lanes_below(m)] = hp;
     }
   }
   add_counts(gained, 0ull, cnt);
@@ -1349,7 +1388,8 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
         // (MHIP_PRUNE_STREAM=0 / 1: never / always, for tests)
         const bool stream = stream_env ? stream_env[0] == '1' : 16 * (lim1 - lim0) > P;
         if(lim1 > lim0 && stream)
-          hipLaunchKernelGGL((prune_stage2_kernel<K, true>), dim3((unsigned)((P + 255) / 256)),
+          hipLaunchKernelGGL((prune_stage2_kernel<K, true>),
+                             dim3((unsigned)std::min<int64_t>(kStage2Blocks, (P + 255) / 256)),
                              dim3(256), 0, st, keys2.as<K>(), rs2.as<RvSlot>(), P,
                              pos.as<int64_t>(), shead.as<int64_t>(), lim0, lim1,
                              s->rv_prune_tol, acc.as<double>(),
@@ -1357,7 +1397,9 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
                              pcnt.as<unsigned long long>(), slist.as<int64_t>());
         else if(lim1 > lim0)
           hipLaunchKernelGGL((prune_stage2_kernel<K, false>),
-                             dim3((unsigned)((lim1 - lim0 + 255) / 256)), dim3(256), 0, st,
+                             dim3((unsigned)std::min<int64_t>(kStage2Blocks,
+                                                              (lim1 - lim0 + 255) / 256)),
+                             dim3(256), 0, st,
                              keys2.as<K>(), rs2.as<RvSlot>(), P, pos.as<int64_t>(),
                              shead.as<int64_t>(), lim0, lim1, s->rv_prune_tol,
                              acc.as<double>(), accf.as<uint8_t>(),
