@@ -913,13 +913,39 @@ __global__ __launch_bounds__(256) void score_hist_kernel(const double* __restric
     if(h[i]) atomicAdd(&hist[i], h[i]);
 }
 
-// flags[i] = nonzero score whose |score| top 24 bits are >= thr24.
-__global__ void score_above_kernel(const double* __restrict__ usum, int64_t n, uint64_t thr24,
-                                   uint8_t* __restrict__ flags) {
-  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-      i += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = (uint64_t)__double_as_longlong(fabs(usum[i]));
-    flags[i] = (k != 0 && (k >> 40) >= thr24) ? 1 : 0;
+// Candidates with |score| bits at or above the 24-bit prefix thr24 (and a
+// nonzero score), compacted in one pass: keys and scores written in any order (one atomic per wavefront for the output
+// position; entries past cap are counted, not written). The caller sorts
+// them by key, which restores determinant order.
+template <typename K>
+__global__ __launch_bounds__(256) void select_above_kernel(const K* __restrict__ ukeys,
+                                                           const double* __restrict__ usum,
+                                                           int64_t n, uint64_t thr24, int64_t cap,
+                                                           K* __restrict__ sk,
+                                                           double* __restrict__ sv,
+                                                           unsigned long long* __restrict__ cnt) {
+  const uint32_t lane = threadIdx.x & 63;
+  for(int64_t b0 = (int64_t)blockIdx.x * 256; b0 < n; b0 += (int64_t)gridDim.x * 256) {
+    const int64_t i = b0 + threadIdx.x;
+    double v = 0.;
+    bool take = false;
+    if(i < n) {
+      v = usum[i];
+      const uint64_t k = (uint64_t)__double_as_longlong(fabs(v));
+      take = k != 0 && (k >> 40) >= thr24;
+    }
+    const uint64_t m = ballot(take);
+    if(m) {
+      const uint32_t first = (uint32_t)ctz64(m);
+      unsigned long long base = 0;
+      if(lane == first) base = atomicAdd(cnt, (unsigned long long)popc64(m));
+      base = __shfl(base, (int)first, 64);
+      const int64_t p = (int64_t)(base + lanes_below(m));
+      if(take && p < cap) {
+        sk[p] = ukeys[i];
+        sv[p] = v;
+      }
+    }
   }
 }
 
@@ -1502,27 +1528,34 @@ int64_t search_impl(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndet
     }
     tm.asci_nsurvivors = eligible;
     int64_t nout = 0;
-    DevBuf fl, cnt(16), sk, sv, ab, skeys, sabs;
+    DevBuf cnt(16), sk, sv, ab, skeys, sabs;
     if(top_k > 0 && eligible > 0) {
       const bool all = eligible <= top_k;
       const uint64_t thr = all ? 0 : prefix;
       const int64_t nsel_expected = all ? eligible : above + in_bucket;
-      fl.alloc(nu);
-      hipLaunchKernelGGL(score_above_kernel, dim3(hblocks), dim3(256), 0, st, us, nu, thr,
-                         fl.as<uint8_t>());
-      sk.alloc(sizeof(K) * nsel_expected);
-      sv.alloc(8 * nsel_expected);
-      tmp = 0;
-      MHIP_HIP_CHECK(rocprim::select(nullptr, tmp, ukeys.as<K>(), fl.as<uint8_t>(), sk.as<K>(),
-                                     cnt.as<int64_t>(), (size_t)nu, st));
-      ctx->scratch_c.reserve(tmp + 16);
-      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, ukeys.as<K>(), fl.as<uint8_t>(),
-                                     sk.as<K>(), cnt.as<int64_t>(), (size_t)nu, st));
-      MHIP_HIP_CHECK(rocprim::select(ctx->scratch_c.p, tmp, us, fl.as<uint8_t>(),
-                                     sv.as<double>(), cnt.as<int64_t>(), (size_t)nu, st));
+      // every candidate at or above the prefix, compacted unordered, then
+      // sorted by key (determinant order) with its score
+      DevBuf sku(sizeof(K) * std::max<int64_t>(nsel_expected, 1));
+      DevBuf svu(8 * std::max<int64_t>(nsel_expected, 1));
+      sk.alloc(sizeof(K) * std::max<int64_t>(nsel_expected, 1));
+      sv.alloc(8 * std::max<int64_t>(nsel_expected, 1));
+      MHIP_HIP_CHECK(hipMemsetAsync(cnt.p, 0, 16, st));
+      hipLaunchKernelGGL((select_above_kernel<K>),
+                         dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (nu + 255) / 256))),
+                         dim3(256), 0, st, ukeys.as<K>(), us, nu, thr, nsel_expected, sku.as<K>(),
+                         svu.as<double>(), cnt.as<unsigned long long>());
+      MHIP_HIP_CHECK(hipGetLastError());
       int64_t nsel = 0;
       d2h(ctx, &nsel, cnt.p, 8);
       MHIP_REQUIRE(nsel == nsel_expected, MHIP_ERR_HIP, "top-k selection count mismatch");
+      tmp = 0;
+      MHIP_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, sku.as<K>(), sk.as<K>(),
+                                               svu.as<double>(), sv.as<double>(), (size_t)nsel, 0,
+                                               kbits, st));
+      ctx->scratch_c.reserve(tmp + 16);
+      MHIP_HIP_CHECK(rocprim::radix_sort_pairs(ctx->scratch_c.p, tmp, sku.as<K>(), sk.as<K>(),
+                                               svu.as<double>(), sv.as<double>(), (size_t)nsel, 0,
+                                               kbits, st));
       const K* outk = sk.as<K>();
       nout = nsel;
       if(!all) {
