@@ -1041,9 +1041,9 @@ void prepare_core(mhip_context* ctx, const IntsView& I, const uint64_t* d_cdets,
 }
 
 // Dense keys (KeyMap): the string tables of both spins. False (full-string
-// keys) when MHIP_ASCI_KEYS=bits, when the tables would hold more than a
-// quarter as many entries as there are raw candidates (unless
-// MHIP_ASCI_KEYS=rank), or when the two rank widths exceed 64 bits.
+// keys) when MHIP_ASCI_KEYS=bits, when the tables would hold more entries
+// than there are candidates to generate (unless MHIP_ASCI_KEYS=rank), or
+// when the two rank widths exceed 64 bits.
 struct RankPrep {
   DevBuf ta, tb, oa, ob, ua, ub;
   KeyMap M{};
@@ -1053,7 +1053,7 @@ struct RankPrep {
 
 template <int W>
 bool prepare_ranks(mhip_context* ctx, const uint64_t* d_cdets, int64_t nc, int64_t norb,
-                   bool singles_only, RankPrep& R) {
+                   bool singles_only, double gen_fraction, RankPrep& R) {
   const char* env = std::getenv("MHIP_ASCI_KEYS");
   const bool force = env && std::string(env) == "rank";
   if(env && std::string(env) == "bits") return false;
@@ -1098,7 +1098,11 @@ bool prepare_ranks(mhip_context* ctx, const uint64_t* d_cdets, int64_t nc, int64
   }
   const int64_t E = side[0].E + side[1].E;
   if(side[0].E >= (1ll << 31) || side[1].E >= (1ll << 31)) return false;
-  if(!force && 4 * E > raw) return false;
+  // gen_fraction: the share of the raw candidates that will be generated
+  // (the block screen's keep fraction)
+  // (a table entry costs about as much to build as the dense key saves per
+  // candidate in the sort)
+  if(!force && (double)E > gen_fraction * (double)raw) return false;
   int bits[2] = {0, 0};
   for(int sp = 0; sp < 2; ++sp) {
     Side& S = side[sp];
@@ -1640,7 +1644,8 @@ int64_t search(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndets_max
                const uint64_t* d_cdets, int64_t nc, double E0, const double* d_C, int64_t norb,
                uint64_t* d_out, const SearchMode& mode) {
   RankPrep R;
-  if(prepare_ranks<W>(ctx, d_cdets, nc, norb, s->just_singles != 0, R))
+  if(prepare_ranks<W>(ctx, d_cdets, nc, norb, s->just_singles != 0,
+                     mode.screened() ? std::min(1.0, mode.keep_fraction) : 1.0, R))
     return search_impl<W, true>(ctx, s, ndets_max, d_cdets, nc, E0, d_C, norb, d_out, mode, &R);
   return search_impl<W, false>(ctx, s, ndets_max, d_cdets, nc, E0, d_C, norb, d_out, mode,
                                nullptr);
