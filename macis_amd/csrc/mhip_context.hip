@@ -36,7 +36,11 @@ void init_noalloc_scratch(int device) {
 }  // namespace
 
 namespace {
-constexpr size_t kCacheMinBytes = size_t(64) << 20;
+// Every block is kept for reuse: a hipMalloc + hipFree pair costs ~70 us,
+// which the ~10 small buffers of a Hamiltonian build had added up to
+// ~0.7 ms of a 6 ms build. (A reused block may be up to 1/8 larger than
+// asked; device_cache_trim returns them all when an allocation fails.)
+constexpr size_t kCacheMinBytes = 0;
 struct BlockCache {
   std::mutex mu;
   std::multimap<size_t, void*> idle[kMaxDevices];  // size -> block, per device
