@@ -1644,8 +1644,16 @@ int64_t search(mhip_context* ctx, const mhip_asci_settings* s, int64_t ndets_max
                const uint64_t* d_cdets, int64_t nc, double E0, const double* d_C, int64_t norb,
                uint64_t* d_out, const SearchMode& mode) {
   RankPrep R;
-  if(prepare_ranks<W>(ctx, d_cdets, nc, norb, s->just_singles != 0,
-                     mode.screened() ? std::min(1.0, mode.keep_fraction) : 1.0, R))
+  bool ranks = false;
+  try {
+    ranks = prepare_ranks<W>(ctx, d_cdets, nc, norb, s->just_singles != 0,
+                             mode.screened() ? std::min(1.0, mode.keep_fraction) : 1.0, R);
+  } catch(const Error& e) {  // out of memory for the tables: full-string keys
+    if(e.code != MHIP_ERR_OUT_OF_MEMORY) throw;
+    (void)hipGetLastError();
+    R = RankPrep();
+  }
+  if(ranks)
     return search_impl<W, true>(ctx, s, ndets_max, d_cdets, nc, E0, d_C, norb, d_out, mode, &R);
   return search_impl<W, false>(ctx, s, ndets_max, d_cdets, nc, E0, d_C, norb, d_out, mode,
                                nullptr);

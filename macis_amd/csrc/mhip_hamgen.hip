@@ -945,7 +945,14 @@ mhip_csr* ham_build(mhip_context* ctx, int W, const uint64_t* d_bra,
     // flat lists within a quarter of the free memory (the CSR comes after)
     size_t free_b = 0, total_b = 0;
     device_mem_info(&free_b, &total_b);
-    if(prepare_spin_groups(ctx, W, d_bra, nbra, 0.25 * (double)free_b, sg)) ex.groups = &sg;
+    // out of memory while preparing: the tiled loop needs far less
+    try {
+      if(prepare_spin_groups(ctx, W, d_bra, nbra, 0.25 * (double)free_b, sg)) ex.groups = &sg;
+    } catch(const Error& e) {
+      if(e.code != MHIP_ERR_OUT_OF_MEMORY) throw;
+      (void)hipGetLastError();  // a refused hipMalloc leaves no pending error
+      sg = SpinGroups();
+    }
     ex.detect_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     MHIP_REQUIRE(ex.groups || path != MHIP_BUILD_SPIN_GROUPS, MHIP_ERR_INVALID_ARG,
