@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <array>
 #include <memory>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -254,10 +255,10 @@ __device__ __forceinline__ double code_sign(int32_t code) {
 // values of the tiled path's element pass (the same operations in the same
 // order), so the CSR is bit-identical; |h| > thresh is counted per row for
 // the shared compaction. LDS per wavefront: 18 bytes per outer neighbour
-// (max_seg of the launch) + 132, so that at CAS(12e,12o) (262 outer
+// (max_seg of the launch) + 260, so that at CAS(12e,12o) (262 outer
 // neighbours) the 64-VGPR kernel keeps 8 wavefronts per SIMD.
 __host__ __device__ constexpr size_t fill_lds_per_wave(int max_seg) {
-  return ((size_t)18 * max_seg + 4 + 128 + 7) / 8 * 8;
+  return ((size_t)18 * max_seg + 4 + 128 + 128 + 7) / 8 * 8;
 }
 
 // The alpha-beta double of an outer single x inner single is
@@ -296,6 +297,7 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   int32_t* seg_code = seg_start + max_seg + 1;
   uint16_t* seg_exc = reinterpret_cast<uint16_t*>(seg_code + max_seg);
   int16_t* lv1 = reinterpret_cast<int16_t*>(seg_exc + max_seg);  // level-1 segments (<= 64)
+  uint16_t* tag = reinterpret_cast<uint16_t*>(lv1 + 64);  // window start stamps (64)
   const PSView& P = BATCH ? many[row_inst[row]] : one;
   const IntsView& I = P.I;
   const int64_t L = P.L, row0 = P.row0;
@@ -400,11 +402,21 @@ __global__ __launch_bounds__(256) void product_fill_elements_kernel(
   // 64-bit mask, and a lane's segment is jlo - 1 + the starts at or below it.
   int32_t jlo = 0;
   // lane's segment in the window at p0; advances jl past the window's starts
+  uint32_t stamp = 0;
+  tag[lane] = 0xffffu;
   auto locate = [&](int32_t p0, int32_t& jl) -> int32_t {
     const int32_t js = jl + lane;
     const int32_t sstart = js < nseg ? seg_start[js] : INT32_MAX;
     const bool inwin = sstart < p0 + 64;
-    const uint64_t startmask = wave_or64(inwin ? (1ull << (sstart - p0)) : 0ull);
+    // each segment starting in the window stamps its start offset, each
+    // lane reads its own offset's stamp: one LDS write and one read (an
+    // OR-reduction of the mask through DPP ran 24 VALU operations)
+    stamp = (stamp + 1u) & 0x7fffu;
+    if(inwin) tag[sstart - p0] = (uint16_t)stamp;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint64_t startmask = ballot(tag[lane] == stamp);
     const int32_t lo = jl - 1 + (int32_t)popc64(startmask & lowmask((uint32_t)lane + 1u));
     jl += (int32_t)popc64(ballot(inwin));
     return lo;
@@ -748,7 +760,7 @@ void product_fill_elements(mhip_context* ctx, int W, const ProductSpace& ps, con
   hipLaunchKernelGGL(inner_tables_kernel<false>, dim3((ps.L + kWavesPerBlock - 1) / kWavesPerBlock),
                      dim3(256), 0, ctx->stream, v, nullptr, nullptr, 1, ps.L);
   const int ms = (int)std::max<int64_t>(1, ps.max_seg);
-  hipLaunchKernelGGL(product_fill_elements_kernel<false>,
+  hipLaunchKernelGGL((product_fill_elements_kernel<false>),
                      dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256),
                      kWavesPerBlock * fill_lds_per_wave(ms), ctx->stream, v, nullptr, nullptr, n,
                      ms, diag, rowptr, colind, nzval, thresh, kept, dropped);
@@ -862,7 +874,7 @@ void batch_product_fill_elements(mhip_context* ctx, const ProductBatch& pb,
                      dim3((pb.total_inner + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0,
                      st, PSView{}, v, pb.inner_off.as<int64_t>(), pb.ninst, pb.total_inner);
   const int ms = (int)std::max<int64_t>(1, pb.max_seg);
-  hipLaunchKernelGGL(product_fill_elements_kernel<true>,
+  hipLaunchKernelGGL((product_fill_elements_kernel<true>),
                      dim3((n + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256),
                      kWavesPerBlock * fill_lds_per_wave(ms), st, PSView{}, v, row_inst, n, ms, diag,
                      rowptr, colind, nzval, thresh, kept, dropped);
