@@ -643,6 +643,22 @@ def spmv_stream_floor(ctx, N, row_len, reps=10):
     return ms, spmv_bytes(nnz, N, True) / (ms * 1e-3) / 1e9
 
 
+def write_floor(ctx, nbytes, reps=5):
+    """The device's write-only rate on a buffer as large as the CSR a build
+    writes: mhip_memset (the runtime's fill kernel) over nbytes, average of
+    reps launches after one warm-up. Returns (ms, GB/s)."""
+    buf = ctx.empty(nbytes // 8 + 1, np.float64)
+    buf.zero()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        buf.zero()
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    del buf
+    return ms, nbytes / (ms * 1e-3) / 1e9
+
+
 def c1_section(C, ctx):
     """CAS(10e,10o) (63,504 determinants, seeded integrals n = 10, seed 10) timed
     in full on both sides: the GPU's selected_ci_diag (build + restarted
@@ -852,6 +868,11 @@ def main():
         },
     }
     out["rdms"] = rdm_timing(ctx, T, V, d_dets, N, nnz, d_X, E)
+    # what writing the build's bytes alone takes (the fill's ceiling)
+    wf_ms, wf_gbs = write_floor(ctx, nnz * 12 + (N + 1) * 8)
+    out["hamgen"]["write_floor_ms"] = wf_ms
+    out["hamgen"]["write_floor_GBs"] = wf_gbs
+    out["hamgen"]["fill_GBs"] = (nnz * 12 + (N + 1) * 8) / (last["ham_fill_ms"] * 1e-3) / 1e9
     if sliced and nnz % N == 0:
         # every row of the complete space has the same length
         fl_ms, fl_gbs = spmv_stream_floor(ctx, N, nnz // N)
