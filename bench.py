@@ -221,7 +221,7 @@ def selected_ci_section(C, ctx, cpu=True, keep=None):
     rowptr = np.zeros(nd + 1, dtype=np.int64)
     C.check(C.lib().mhip_csr_copy_to_host_i64(ctx.h, H.h, rowptr.ctypes.data, None, None), "rp")
     x = ctx.to_device(np.random.default_rng(0).standard_normal(nd))
-    y = ctx.empty(nd, np.float64)
+    y = ctx.empty(nd, np.float64).zero()
     ctx.spmv(H, 1, 1.0, x, nd, 0.0, y, nd)
     ctx.synchronize()
     reps = 10
@@ -631,7 +631,7 @@ def spmv_stream_floor(ctx, N, row_len, reps=10):
     del colind, nzval
     assert ctx.prepare_spmv(A, always=True)
     x = ctx.to_device(np.ones(N))
-    y = ctx.empty(N, np.float64)
+    y = ctx.empty(N, np.float64).zero()
     ctx.spmv(A, 1, 1.0, x, N, 0.0, y, N)
     ctx.synchronize()
     t0 = time.perf_counter()

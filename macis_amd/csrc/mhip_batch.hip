@@ -947,6 +947,11 @@ int mhip_batch_davidson(mhip_context* ctx, mhip_batch* B, int64_t max_m, int64_t
       B->E.alloc(8 * B->ninst);
       B->res.alloc(8 * B->ninst);
       B->nleft.alloc(4);
+      // the SpMV forms AV = 1 * H v + 0 * AV (gespmbv, spmbv.hpp:71), which
+      // reads AV: a reused block may hold NaN or Inf patterns, so the
+      // subspace storage starts at zero (outside any graph capture)
+      MHIP_HIP_CHECK(hipMemsetAsync(B->V.p, 0, 8 * N * (max_m + 1), st));
+      MHIP_HIP_CHECK(hipMemsetAsync(B->AV.p, 0, 8 * N * (max_m + 1), st));
     }
     if(!B->ntiles) {
       std::vector<int64_t> tl;

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -63,7 +64,27 @@ int current_device() {
 }
 }  // namespace
 
+namespace {
+// MHIP_ALLOC_POISON=1 (a debugging aid): every block handed out is filled
+// with 0xff bytes first, so a read of memory no kernel has written shows up
+// as NaNs / -1 indices on every run instead of depending on what the block
+// held before.
+void poison(void* p, size_t bytes) {
+  const char* e = std::getenv("MHIP_ALLOC_POISON");
+  if(e && e[0] == '1' && p && bytes) {
+    (void)hipMemset(p, 0xff, bytes);
+    (void)hipDeviceSynchronize();
+  }
+}
+}  // namespace
+
 void* device_alloc(size_t bytes) {
+  void* p = device_alloc_raw(bytes);
+  poison(p, bytes);
+  return p;
+}
+
+void* device_alloc_raw(size_t bytes) {
   if(!bytes) return nullptr;
   void* p = nullptr;
   if(bytes < kCacheMinBytes) {

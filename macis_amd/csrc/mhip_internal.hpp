@@ -64,6 +64,7 @@ int guarded(F&& f) {
 // retried) and when a context of the device is destroyed. device_free waits
 // for the device, as hipFree does, before a block can be handed out again.
 void* device_alloc(size_t bytes);
+void* device_alloc_raw(size_t bytes);
 void device_free(void* p) noexcept;
 void device_cache_trim(int device) noexcept;
 // Free device memory of the current device as hipMemGetInfo reports it plus

@@ -62,7 +62,7 @@ def test_c3_symmetric_and_lowest_root_is_eigenpair(ctx, c3):
     rng = np.random.default_rng(3)
     x, y = rng.standard_normal(N), rng.standard_normal(N)
     dx, dy = ctx.to_device(x), ctx.to_device(y)
-    dHx, dHy = ctx.empty(N, np.float64), ctx.empty(N, np.float64)
+    dHx, dHy = ctx.empty(N, np.float64).zero(), ctx.empty(N, np.float64).zero()
     ctx.spmv(H, 1, 1.0, dx, N, 0.0, dHx, N)
     ctx.spmv(H, 1, 1.0, dy, N, 0.0, dHy, N)
     a, b = float(x @ dHy.numpy()), float(y @ dHx.numpy())

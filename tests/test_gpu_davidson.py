@@ -185,7 +185,8 @@ def test_spmv_sliced_copy_rules(ctx):
     assert not ctx.prepare_spmv(B)
     x = rng.standard_normal(m)
     dx = ctx.to_device(x)
-    y0, y1, y2 = (ctx.empty(m, np.float64) for _ in range(3))
+    # outputs start at zero: beta = 0 still reads y, as gespmbv does
+    y0, y1, y2 = (ctx.empty(m, np.float64).zero() for _ in range(3))
     ctx.spmv(B, 1, 1.0, dx, m, 0.0, y0, m)  # CSR kernel
     assert ctx.prepare_spmv(B, always=True)
     ctx.spmv(B, 1, 1.0, dx, m, 0.0, y1, m)
@@ -253,7 +254,7 @@ def test_spmv_sliced_narrow_and_wide_slices(ctx):
     assert ctx.prepare_spmv(A, always=True) and ctx.prepare_spmv(B, always=True)
     x = rng.standard_normal(n)
     dx = ctx.to_device(x)
-    ya, yb = ctx.empty(m, np.float64), ctx.empty(m, np.float64)
+    ya, yb = ctx.empty(m, np.float64).zero(), ctx.empty(m, np.float64).zero()
     ctx.spmv(A, 1, 1.0, dx, n, 0.0, ya, m)
     ctx.spmv(B, 1, 1.0, dx, n, 0.0, yb, m)
     ref = np.zeros(m)
