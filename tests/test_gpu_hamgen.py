@@ -415,6 +415,31 @@ def test_spin_group_build_matches_tiled(ctx, case):
         assert_same_csr(grp, O.Ham(T, V).csr(W, dets, thresh=EPS))
 
 
+@pytest.mark.parametrize("thresh", [1e-3, 0.05])
+def test_spin_group_threshold_compaction(ctx, thresh):
+    """Elements at or below the threshold are dropped after the group scan
+    by the shared compaction, as after the tiled loop: same CSR."""
+    announce(f"hamgen spin-group scan with threshold {thresh}")
+    n = 32
+    T, V = seeded_integrals(n, 95)
+    ctx.set_integrals(n, T, V)
+    dets = selected_ci_dets(n, 8, 8, 3000, seed=16, energies=0.25 * np.arange(n))
+    dd = ctx.to_device(dets)
+    out = {}
+    for path in (ctx.BUILD_TILED, ctx.BUILD_SPIN_GROUPS):
+        ctx.set_build_path(path)
+        try:
+            out[path] = ctx.ham_build(2, dd, len(dets), thresh=thresh).to_host()
+            assert ctx.timings()["ham_path"] == path
+        finally:
+            ctx.set_build_path(ctx.BUILD_AUTO)
+    for a, b in zip(out[ctx.BUILD_SPIN_GROUPS], out[ctx.BUILD_TILED]):
+        assert np.array_equal(a, b)
+    full = O.Ham(T, V).csr(2, dets, thresh=EPS)
+    assert len(out[ctx.BUILD_TILED][1]) < len(full[1])
+    assert_same_csr(out[ctx.BUILD_SPIN_GROUPS], O.Ham(T, V).csr(2, dets, thresh=thresh))
+
+
 @pytest.mark.parametrize("piece", ["50000", "333333"])
 def test_spin_group_sort_in_pieces(ctx, piece, monkeypatch):
     """The per-row column sort runs in pieces of whole rows when the CSR
