@@ -204,9 +204,27 @@ __global__ __launch_bounds__(256) void flatten_kernel(
   if(g >= ng) return;
   const uint32_t lane = threadIdx.x & 63;
   int64_t out = foff[g];
+  // four 64-entry steps per iteration with their loads issued together: a
+  // large group (the Hartree-Fock string's) is copied into every neighbour's
+  // list, and one load latency per step bounded those copies
   auto copy = [&](int64_t gg) {
     const int64_t p0 = start[gg], p1 = start[gg + 1];
-    for(int64_t p = p0 + lane; p < p1; p += 64) {
+    int64_t p = p0 + lane;
+    for(; p + 192 < p1; p += 256) {
+      uint64_t w[4];
+      int32_t m[4];
+#pragma unroll
+      for(int u = 0; u < 4; ++u) {
+        w[u] = mword[p + 64 * u];
+        m[u] = sm[p + 64 * u];
+      }
+#pragma unroll
+      for(int u = 0; u < 4; ++u) {
+        fw[out + (p + 64 * u - p0)] = w[u];
+        fi[out + (p + 64 * u - p0)] = m[u];
+      }
+    }
+    for(; p < p1; p += 64) {
       fw[out + (p - p0)] = mword[p];
       fi[out + (p - p0)] = sm[p];
     }
