@@ -12,7 +12,7 @@
 // Determinants are grouped by alpha string and by beta string (stable radix
 // sorts, so a group lists its members in ascending index). For each group
 // the groups of the same spin one single excitation away are found (one
-// binary search per hole/particle pair of the group string over the sorted
+// lookup per hole/particle pair of the group string in a hash table of the
 // distinct strings), and a flat list is written per group: its own members,
 // then the members of those neighbour groups, each entry holding the
 // member's other-spin string and its index. A row scans two contiguous
@@ -127,15 +127,53 @@ __global__ void group_kernel(const uint64_t* __restrict__ sk, const int32_t* __r
   mword[p] = other[m];
 }
 
+// Open-addressing table of the distinct strings of one spin (power-of-two
+// size, at most ~40 % full): slot value g + 1 for group g, 0 when empty.
+__device__ __forceinline__ uint64_t str_hash(uint64_t x) {
+  // 64-bit finaliser (xor-shift / multiply rounds)
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// thread per group: claims the first free slot from the string's hash (the
+// strings are distinct, so no slot holds an equal key)
+__global__ void str_insert_kernel(const uint64_t* __restrict__ ustr, int64_t ng, uint64_t mask,
+                                  uint64_t* __restrict__ hkey, int32_t* __restrict__ hval) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(g >= ng) return;
+  const uint64_t x = ustr[g];
+  for(uint64_t h = str_hash(x) & mask;; h = (h + 1) & mask) {
+    if(atomicCAS(hval + h, 0, (int32_t)(g + 1)) == 0) {
+      hkey[h] = x;
+      return;
+    }
+  }
+}
+
+__device__ __forceinline__ int64_t str_find(const uint64_t* __restrict__ hkey,
+                                            const int32_t* __restrict__ hval, uint64_t mask,
+                                            uint64_t x) {
+  for(uint64_t h = str_hash(x) & mask;; h = (h + 1) & mask) {
+    const int32_t v = hval[h];
+    if(v == 0) return -1;
+    if(hkey[h] == x) return v - 1;
+  }
+}
+
 // Single-excitation neighbours of every group string among the group
 // strings (one wavefront per group): each (hole, particle) pair, particles
-// drawn from the union of the strings, is looked up by binary search in the
-// ascending distinct strings. Count pass: neighbours and their total
-// membership; fill pass: the neighbour list.
+// drawn from the union of the strings, is looked up in the strings' hash
+// table. Count pass: neighbours and their total membership; fill pass: the
+// neighbour list.
 template <bool FILL>
 __global__ __launch_bounds__(256) void neighbour_kernel(
     const uint64_t* __restrict__ ustr, const int64_t* __restrict__ start, int64_t ng,
-    uint64_t uni, int64_t* __restrict__ ncount, int64_t* __restrict__ nsize,
+    uint64_t uni, const uint64_t* __restrict__ hkey, const int32_t* __restrict__ hval,
+    uint64_t hmask, int64_t* __restrict__ ncount, int64_t* __restrict__ nsize,
     const int64_t* __restrict__ noff, int32_t* __restrict__ nbr) {
   __shared__ uint8_t pos[kWaves][2][64];
   const int wv = threadIdx.x >> 6;
@@ -157,16 +195,7 @@ __global__ __launch_bounds__(256) void neighbour_kernel(
     int64_t found = -1;
     if(t < tot) {
       const uint32_t h = pos[wv][0][t / np], q = pos[wv][1][t % np];
-      const uint64_t x = s ^ (1ull << h) ^ (1ull << q);
-      int64_t lo = 0, hi = ng;
-      while(lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if(ustr[mid] < x)
-          lo = mid + 1;
-        else
-          hi = mid;
-      }
-      if(lo < ng && ustr[lo] == x) found = lo;
+      found = str_find(hkey, hval, hmask, s ^ (1ull << h) ^ (1ull << q));
     }
     const uint64_t m = ballot(found >= 0);
     if(found >= 0) {
@@ -401,6 +430,15 @@ void build_side(mhip_context* ctx, int W, int64_t n, const uint64_t* keys, const
   sk.release();
   head.release();
   gid.release();
+  // hash table of the distinct strings (size: the power of two >= 2.5 ng)
+  int64_t hsize = 1024;
+  while(hsize < ng * 5 / 2) hsize <<= 1;
+  DevBuf hkey(8 * hsize), hval(4 * hsize);
+  MHIP_HIP_CHECK(hipMemsetAsync(hval.p, 0, 4 * hsize, st));
+  hipLaunchKernelGGL(str_insert_kernel, dim3(grid_of(ng, 256)), dim3(256), 0, st,
+                     ustr.as<uint64_t>(), (int64_t)ng, (uint64_t)(hsize - 1), hkey.as<uint64_t>(),
+                     hval.as<int32_t>());
+  MHIP_HIP_CHECK(hipGetLastError());
   // neighbour groups: counts, offsets, lists
   DevBuf ncount(8 * (ng + 1));
   S.nsize.alloc(8 * (ng + 1));
@@ -408,6 +446,7 @@ void build_side(mhip_context* ctx, int W, int64_t n, const uint64_t* keys, const
   MHIP_HIP_CHECK(hipMemsetAsync(ncount.as<int64_t>() + ng, 0, 8, st));
   hipLaunchKernelGGL(neighbour_kernel<false>, dim3(grid_of(ng, kWaves)), dim3(256), 0, st,
                      ustr.as<uint64_t>(), S.start.as<int64_t>(), (int64_t)ng, uni,
+                     hkey.as<uint64_t>(), hval.as<int32_t>(), (uint64_t)(hsize - 1),
                      ncount.as<int64_t>(), S.nsize.as<int64_t>(), nullptr, nullptr);
   MHIP_HIP_CHECK(hipGetLastError());
   exclusive_scan_i64(ctx, ncount.as<int64_t>(), noff.as<int64_t>(), ng + 1);
@@ -415,7 +454,8 @@ void build_side(mhip_context* ctx, int W, int64_t n, const uint64_t* keys, const
   d2h(ctx, &nn, noff.as<int64_t>() + ng, 8);
   nbr.alloc(4 * std::max<int64_t>(nn, 1));
   hipLaunchKernelGGL(neighbour_kernel<true>, dim3(grid_of(ng, kWaves)), dim3(256), 0, st,
-                     ustr.as<uint64_t>(), S.start.as<int64_t>(), (int64_t)ng, uni, nullptr,
+                     ustr.as<uint64_t>(), S.start.as<int64_t>(), (int64_t)ng, uni,
+                     hkey.as<uint64_t>(), hval.as<int32_t>(), (uint64_t)(hsize - 1), nullptr,
                      nullptr, noff.as<int64_t>(), nbr.as<int32_t>());
   // flat offsets
   DevBuf len(8 * (ng + 1));
