@@ -620,7 +620,7 @@ def c5_section(C, ctx, sel):
 
 def spmv_stream_floor(ctx, N, row_len, reps=10):
     """The roofline kernel's own streaming floor: the sliced SpMV on a matrix
-    of the C2 shape (N rows of row_len entries) whose column indices all point
+    of the CAS(12e,12o) shape (N rows of row_len entries) whose column indices all point
     at x[0], so every x gather hits cache and only the 12 streamed bytes per
     entry remain. Returns (ms per launch, GB/s of its algorithmic bytes)."""
     nnz = N * row_len
